@@ -1,0 +1,396 @@
+// bvh_build.hip -- GPU-built wide BVH (replaces Embree's bvh4.spatialsplit
+// builder, src/geometry/embreeAccelerator.cpp:19-35).
+//
+// Pipeline, all on the device:
+//   1. per-triangle padded bounds and centroid bounds (k_prim_bounds)
+//   2. 63-bit Morton codes of the centroids (k_morton)
+//   3. radix sort of (code, triangle) pairs (hipCUB)
+//   4. binary radix tree over the sorted codes (k_karras; Karras 2012,
+//      "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d
+//      Trees")
+//   5. bottom-up bounds (k_refit), one thread per leaf, agent-scope
+//      acquire/release atomics so the last arriver at a node sees both
+//      children's boxes on every XCD
+//   6. top-down collapse into W-wide nodes (k_collapse), level by level: each
+//      wide node opens the child with the largest surface area until it has W
+//      children; a subtree with <= max_leaf triangles becomes one leaf (a
+//      contiguous range of the sorted triangles)
+//   7. triangle records in sorted order (k_reorder): (v0 | prim), e1, e2 so a
+//      leaf is read with contiguous 16-byte loads
+//
+// Triangle boxes are padded by 1e-5 of the largest coordinate so traversal
+// never culls a triangle the Moller-Trumbore test would accept.
+#include <hipcub/hipcub.hpp>
+
+#include "context.h"
+
+namespace aur {
+
+static constexpr unsigned LEAF_BIT = 0x80000000u;
+
+__device__ inline int f2o(float f) { int i = __float_as_int(f); return i >= 0 ? i : i ^ 0x7fffffff; }
+__device__ inline float o2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__global__ void k_init_bounds(int *cb) {
+    if (threadIdx.x < 3) { cb[threadIdx.x] = f2o(3.0e38f); cb[3 + threadIdx.x] = f2o(-3.0e38f); }
+}
+
+__global__ void k_prim_bounds(const float *__restrict__ verts, const int *__restrict__ tris, int n,
+                              float4 *__restrict__ lo, float4 *__restrict__ hi, int *__restrict__ cb) {
+    __shared__ float s_min[3][256], s_max[3][256];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float cmin[3] = {3.0e38f, 3.0e38f, 3.0e38f}, cmax[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    if (i < n) {
+        float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, h[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+        float mag = 0.f;
+        for (int c = 0; c < 3; ++c) {
+            const float *v = verts + 3 * tris[3 * i + c];
+            for (int k = 0; k < 3; ++k) {
+                l[k] = fminf(l[k], v[k]);
+                h[k] = fmaxf(h[k], v[k]);
+                mag = fmaxf(mag, fabsf(v[k]));
+            }
+        }
+        float pad = mag * 1e-5f + 1e-30f;
+        for (int k = 0; k < 3; ++k) {
+            l[k] -= pad; h[k] += pad;
+            float c = 0.5f * (l[k] + h[k]);
+            cmin[k] = c; cmax[k] = c;
+        }
+        lo[i] = make_float4(l[0], l[1], l[2], 0.f);
+        hi[i] = make_float4(h[0], h[1], h[2], 0.f);
+    }
+    for (int k = 0; k < 3; ++k) { s_min[k][threadIdx.x] = cmin[k]; s_max[k][threadIdx.x] = cmax[k]; }
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 3; ++k) {
+                s_min[k][threadIdx.x] = fminf(s_min[k][threadIdx.x], s_min[k][threadIdx.x + s]);
+                s_max[k][threadIdx.x] = fmaxf(s_max[k][threadIdx.x], s_max[k][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) { atomicMin(&cb[k], f2o(s_min[k][0])); atomicMax(&cb[3 + k], f2o(s_max[k][0])); }
+}
+
+__device__ inline unsigned long long spread21(unsigned x) {
+    unsigned long long v = x & 0x1fffffu;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+__global__ void k_morton(const float4 *__restrict__ lo, const float4 *__restrict__ hi, int n, const int *__restrict__ cb,
+                         unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 l = lo[i], h = hi[i];
+    float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
+    unsigned q[3];
+    for (int k = 0; k < 3; ++k) {
+        float mn = o2f(cb[k]), mx = o2f(cb[3 + k]);
+        float ext = mx - mn;
+        float t = ext > 0.f ? (c[k] - mn) / ext : 0.f;
+        t = fminf(fmaxf(t, 0.f), 1.f);
+        q[k] = (unsigned)(t * 2097151.f);
+    }
+    keys[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+    vals[i] = i;
+}
+
+__device__ inline int delta(const unsigned long long *keys, int n, int i, int j) {
+    if (j < 0 || j > n - 1) return -1;
+    unsigned long long a = keys[i], b = keys[j];
+    if (a == b) return 64 + __clz((unsigned)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+// Karras 2012, algorithm 3: inner node i of n-1 covers [first, last]
+__global__ void k_karras(const unsigned long long *__restrict__ keys, int n, int *__restrict__ left, int *__restrict__ right,
+                         int *__restrict__ first, int *__restrict__ last, int *__restrict__ parent_inner,
+                         int *__restrict__ parent_leaf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    int lc = (lo == gamma) ? (int)(gamma | LEAF_BIT) : gamma;
+    int rc = (hi == gamma + 1) ? (int)((gamma + 1) | LEAF_BIT) : gamma + 1;
+    left[i] = lc;
+    right[i] = rc;
+    first[i] = lo;
+    last[i] = hi;
+    if (lc & LEAF_BIT) parent_leaf[gamma] = i; else parent_inner[gamma] = i;
+    if (rc & LEAF_BIT) parent_leaf[gamma + 1] = i; else parent_inner[gamma + 1] = i;
+    if (i == 0) parent_inner[0] = -1;
+}
+
+__device__ inline void child_box(int code, const float4 *plo, const float4 *phi, const int *vals, const float4 *nlo,
+                                 const float4 *nhi, float4 *l, float4 *h) {
+    if (code & LEAF_BIT) {
+        int p = vals[code & ~LEAF_BIT];
+        *l = plo[p]; *h = phi[p];
+    } else {
+        *l = nlo[code]; *h = nhi[code];
+    }
+}
+
+__global__ void k_refit(int n, const int *__restrict__ vals, const float4 *__restrict__ plo, const float4 *__restrict__ phi,
+                        const int *__restrict__ left, const int *__restrict__ right, const int *__restrict__ parent_inner,
+                        const int *__restrict__ parent_leaf, int *__restrict__ visits, float4 *nlo, float4 *nhi) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int node = parent_leaf[s];
+    while (node >= 0) {
+        int prev = __hip_atomic_fetch_add(&visits[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == 0) return;  // the sibling's thread will finish this node
+        float4 al, ah, bl, bh;
+        child_box(left[node], plo, phi, vals, nlo, nhi, &al, &ah);
+        child_box(right[node], plo, phi, vals, nlo, nhi, &bl, &bh);
+        nlo[node] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
+        nhi[node] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+        node = parent_inner[node];
+    }
+}
+
+struct CollapseIn {
+    const int *left, *right, *first, *last, *vals;
+    const float4 *plo, *phi, *nlo, *nhi;
+    int max_leaf;
+};
+
+__device__ inline float half_area(float4 l, float4 h) {
+    float dx = h.x - l.x, dy = h.y - l.y, dz = h.z - l.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ inline int code_count(const CollapseIn &in, int code) {
+    if (code & LEAF_BIT) return 1;
+    return in.last[code] - in.first[code] + 1;
+}
+__device__ inline int code_first(const CollapseIn &in, int code) {
+    if (code & LEAF_BIT) return code & ~LEAF_BIT;
+    return in.first[code];
+}
+
+template <int W>
+__global__ void k_collapse(CollapseIn in, const int2 *__restrict__ tasks, const int *__restrict__ n_tasks,
+                           int2 *__restrict__ next, int *__restrict__ n_next, int *__restrict__ node_count,
+                           WNode<W> *__restrict__ nodes, int *__restrict__ max_depth, int depth) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_tasks) return;
+    int2 task = tasks[i];  // (binary code, wide index)
+    int ch[W];
+    int nc = 0;
+    int code = task.x;
+    if (!(code & LEAF_BIT) && code_count(in, code) > in.max_leaf) {
+        ch[0] = in.left[code];
+        ch[1] = in.right[code];
+        nc = 2;
+        while (nc < W) {
+            int best = -1;
+            float best_a = -1.f;
+            for (int k = 0; k < nc; ++k) {
+                int c = ch[k];
+                if ((c & LEAF_BIT) || code_count(in, c) <= in.max_leaf) continue;
+                float a = half_area(in.nlo[c], in.nhi[c]);
+                if (a > best_a) { best_a = a; best = k; }
+            }
+            if (best < 0) break;
+            int c = ch[best];
+            ch[best] = in.left[c];
+            ch[nc++] = in.right[c];
+        }
+    } else {
+        ch[0] = code;
+        nc = 1;
+    }
+    WNode<W> nd;
+    for (int k = 0; k < W; ++k) {
+        if (k >= nc) {
+            nd.lox[k] = nd.hix[k] = nd.loy[k] = nd.hiy[k] = nd.loz[k] = nd.hiz[k] = __int_as_float(0x7f800000);
+            nd.child[k] = 0x7fffffff;
+            nd.pad[k] = 0;
+            continue;
+        }
+        int c = ch[k];
+        float4 l, h;
+        child_box(c, in.plo, in.phi, in.vals, in.nlo, in.nhi, &l, &h);
+        nd.lox[k] = l.x; nd.loy[k] = l.y; nd.loz[k] = l.z;
+        nd.hix[k] = h.x; nd.hiy[k] = h.y; nd.hiz[k] = h.z;
+        nd.pad[k] = 0;
+        int cnt = code_count(in, c);
+        if (!(c & LEAF_BIT) && cnt > in.max_leaf) {
+            int idx = atomicAdd(node_count, 1);
+            int q = atomicAdd(n_next, 1);
+            next[q] = make_int2(c, idx);
+            nd.child[k] = idx;
+        } else {
+            nd.child[k] = ~((code_first(in, c) << 4) | (cnt - 1));
+        }
+    }
+    nodes[task.y] = nd;
+    atomicMax(max_depth, depth + 1);
+}
+
+__global__ void k_reorder(const float *__restrict__ verts, const int *__restrict__ tris, const int *__restrict__ vals, int n,
+                          float4 *__restrict__ out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int p = vals[s];
+    const float *a = verts + 3 * tris[3 * p], *b = verts + 3 * tris[3 * p + 1], *c = verts + 3 * tris[3 * p + 2];
+    out[3 * s] = make_float4(a[0], a[1], a[2], __int_as_float(p));
+    out[3 * s + 1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.f);
+    out[3 * s + 2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
+}
+
+__global__ void k_seed_task(int2 *tasks, int *n_tasks, int code) {
+    tasks[0] = make_int2(code, 0);
+    *n_tasks = 1;
+}
+
+__global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+void fill_zero(Ctx *c, void *p, size_t bytes) {
+    size_t words = (bytes + 3) / 4;
+    if (!words) return;
+    unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, c->stream, (unsigned *)p, words, 0u);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <int W>
+static void collapse_levels(Ctx *c, const CollapseIn &in, int root_code, int max_nodes) {
+    Bvh &b = c->bvh;
+    b.nodes.reserve(sizeof(WNode<W>) * (size_t)std::max(max_nodes, 1));
+    DevBuf tq[2], scalars;
+    tq[0].reserve(sizeof(int2) * (size_t)std::max(max_nodes, 1));
+    tq[1].reserve(sizeof(int2) * (size_t)std::max(max_nodes, 1));
+    scalars.reserve(sizeof(int) * 8);
+    int *sc = scalars.as<int>();  // [0] n_tasks A, [1] n_tasks B, [2] node_count, [3] max_depth
+    fill_zero(c, sc, sizeof(int) * 8);
+    hipLaunchKernelGGL(k_seed_task, dim3(1), dim3(1), 0, c->stream, tq[0].as<int2>(), sc + 0, root_code);
+    int one = 1;
+    HIP_CHECK(hipMemcpyAsync(sc + 2, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    int cur = 0, n_tasks = 1, depth = 0;
+    while (n_tasks > 0) {
+        fill_zero(c, sc + (1 - cur), sizeof(int));
+        hipLaunchKernelGGL(k_collapse<W>, dim3((n_tasks + 127) / 128), dim3(128), 0, c->stream, in, tq[cur].as<int2>(),
+                           sc + cur, tq[1 - cur].as<int2>(), sc + (1 - cur), sc + 2, b.nodes.as<WNode<W>>(), sc + 3, depth);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(&n_tasks, sc + (1 - cur), sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        cur = 1 - cur;
+        ++depth;
+    }
+    int hs[4];
+    HIP_CHECK(hipMemcpyAsync(hs, sc, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    b.n_nodes = hs[2];
+    b.depth = hs[3];
+}
+
+void build_bvh(Ctx *c, int width, int max_leaf) {
+    if (width != 4 && width != 8) throw std::runtime_error("BVH width must be 4 or 8");
+    if (max_leaf < 1 || max_leaf > 16) throw std::runtime_error("max_leaf must be in 1..16");
+    const int n = c->n_tris;
+    if (n >= (1 << 27)) throw CapacityError("too many triangles for the leaf encoding (max 2^27)");
+    Bvh &b = c->bvh;
+    b.width = width;
+    b.max_leaf = max_leaf;
+    b.tri_sorted.reserve(sizeof(float4) * 3 * (size_t)std::max(n, 1));
+    if (n == 0) {
+        // a single node with no children: every ray misses
+        size_t nb = width == 4 ? sizeof(WNode<4>) : sizeof(WNode<8>);
+        b.nodes.reserve(nb);
+        std::vector<unsigned> host(nb / 4, 0x7f800000u);
+        HIP_CHECK(hipMemcpyAsync(b.nodes.p, host.data(), nb, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        b.n_nodes = 1;
+        b.depth = 1;
+        return;
+    }
+    const float *verts = c->d_verts.as<float>();
+    const int *tris = c->d_tris.as<int>();
+    DevBuf plo, phi, cb, keys[2], vals[2], temp;
+    plo.reserve(sizeof(float4) * n);
+    phi.reserve(sizeof(float4) * n);
+    cb.reserve(sizeof(int) * 8);
+    keys[0].reserve(sizeof(unsigned long long) * n);
+    keys[1].reserve(sizeof(unsigned long long) * n);
+    vals[0].reserve(sizeof(int) * n);
+    vals[1].reserve(sizeof(int) * n);
+    const int bs = 256, grid = (n + bs - 1) / bs;
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, c->stream, cb.as<int>());
+    hipLaunchKernelGGL(k_prim_bounds, dim3(grid), dim3(bs), 0, c->stream, verts, tris, n, plo.as<float4>(), phi.as<float4>(), cb.as<int>());
+    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(bs), 0, c->stream, plo.as<float4>(), phi.as<float4>(), n, cb.as<int>(),
+                       keys[0].as<unsigned long long>(), vals[0].as<int>());
+    HIP_CHECK(hipGetLastError());
+    size_t temp_bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys[0].as<unsigned long long>(),
+                                                 keys[1].as<unsigned long long>(), vals[0].as<int>(), vals[1].as<int>(), n,
+                                                 0, 63, c->stream));
+    temp.reserve(temp_bytes);
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, keys[0].as<unsigned long long>(),
+                                                 keys[1].as<unsigned long long>(), vals[0].as<int>(), vals[1].as<int>(), n,
+                                                 0, 63, c->stream));
+    const unsigned long long *skeys = keys[1].as<unsigned long long>();
+    const int *svals = vals[1].as<int>();
+
+    int ni = std::max(n - 1, 1);
+    DevBuf left, right, first, last, par_in, par_leaf, visits, nlo, nhi;
+    left.reserve(sizeof(int) * ni);
+    right.reserve(sizeof(int) * ni);
+    first.reserve(sizeof(int) * ni);
+    last.reserve(sizeof(int) * ni);
+    par_in.reserve(sizeof(int) * ni);
+    par_leaf.reserve(sizeof(int) * n);
+    visits.reserve(sizeof(int) * ni);
+    nlo.reserve(sizeof(float4) * ni);
+    nhi.reserve(sizeof(float4) * ni);
+    int root_code;
+    if (n >= 2) {
+        hipLaunchKernelGGL(k_karras, dim3((n - 1 + bs - 1) / bs), dim3(bs), 0, c->stream, skeys, n, left.as<int>(),
+                           right.as<int>(), first.as<int>(), last.as<int>(), par_in.as<int>(), par_leaf.as<int>());
+        fill_zero(c, visits.p, sizeof(int) * ni);
+        hipLaunchKernelGGL(k_refit, dim3(grid), dim3(bs), 0, c->stream, n, svals, plo.as<float4>(), phi.as<float4>(),
+                           left.as<int>(), right.as<int>(), par_in.as<int>(), par_leaf.as<int>(), visits.as<int>(),
+                           nlo.as<float4>(), nhi.as<float4>());
+        HIP_CHECK(hipGetLastError());
+        root_code = 0;
+    } else {
+        root_code = (int)(0u | LEAF_BIT);
+    }
+    CollapseIn in;
+    in.left = left.as<int>(); in.right = right.as<int>(); in.first = first.as<int>(); in.last = last.as<int>();
+    in.vals = svals; in.plo = plo.as<float4>(); in.phi = phi.as<float4>(); in.nlo = nlo.as<float4>(); in.nhi = nhi.as<float4>();
+    in.max_leaf = max_leaf;
+    if (width == 4) collapse_levels<4>(c, in, root_code, ni + 1);
+    else collapse_levels<8>(c, in, root_code, ni + 1);
+    hipLaunchKernelGGL(k_reorder, dim3(grid), dim3(bs), 0, c->stream, verts, tris, svals, n, b.tri_sorted.as<float4>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace aur

@@ -1,0 +1,583 @@
+// capi.cpp -- the extern "C" boundary (include/aurora_hip.h).
+//
+// Every entry point catches C++ exceptions and converts them to error codes
+// with a thread-local message; nothing throws or aborts across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "context.h"
+#include "subdiv.h"
+
+using namespace aur;
+
+struct aur_ctx {
+    Ctx c;
+};
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return AUR_OK;
+    } catch (const HipError &e) {
+        g_last_error = e.what();
+        return AUR_EHIP;
+    } catch (const StateError &e) {
+        g_last_error = e.what();
+        return AUR_ESTATE;
+    } catch (const CapacityError &e) {
+        g_last_error = e.what();
+        return AUR_ECAPACITY;
+    } catch (const std::invalid_argument &e) {
+        g_last_error = e.what();
+        return AUR_EINVAL;
+    } catch (const std::ios_base::failure &e) {
+        g_last_error = e.what();
+        return AUR_EIO;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "out of host memory";
+        return AUR_ECAPACITY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        std::string m = e.what();
+        if (m.rfind("cannot open", 0) == 0) return AUR_EIO;
+        return AUR_EPARSE;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return AUR_EPARSE;
+    }
+}
+
+#define NEED(ptr)                                                      \
+    do {                                                               \
+        if (!(ptr)) throw std::invalid_argument("null argument: " #ptr); \
+    } while (0)
+
+static void set_device(Ctx *c) {
+    if (c->device < 0) throw StateError("host-only context (created with device -1) cannot run GPU work");
+    HIP_CHECK(hipSetDevice(c->device));
+}
+
+static bool needs_gpu_flatten(const HostScene &s) {
+    for (const HostObject &o : s.objects)
+        if (o.subdlevels > 0 || o.displace.on) return true;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// device scene upload
+// ---------------------------------------------------------------------------
+static DLight make_dlight(const aur_light &l) {
+    DLight d;
+    std::memset(&d, 0, sizeof d);
+    d.type = l.type == AUR_LIGHT_AREA ? LIGHT_AREA : LIGHT_ENV;
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 4; ++k) d.m[4 * r + k] = l.m[4 * r + k];
+    d.sx = l.sx;
+    d.sy = l.sy;
+    d.intensity = powf(2, l.exposure);  // light.h:42-45
+    if (d.type == LIGHT_AREA) {
+        for (int k = 0; k < 3; ++k) d.Le[k] = l.color[k] * d.intensity;  // squareLight.h:84-86
+        float pc[4][3], N[3];
+        light_corners(l, pc, N);
+        for (int k = 0; k < 3; ++k) d.N[k] = N[k];
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 3; ++k) d.pcam[i][k] = pc[i][k];
+        // RenderableTriangle(p1, p2, p4) and (p3, p2, p4): e1 = p2 - p, e2 = p4 - p
+        const int base[2] = {0, 2};
+        for (int t = 0; t < 2; ++t) {
+            const float *p = pc[base[t]];
+            for (int k = 0; k < 3; ++k) {
+                d.tri[t][k] = p[k];
+                d.tri[t][4 + k] = pc[1][k] - p[k];
+                d.tri[t][8 + k] = pc[3][k] - p[k];
+            }
+        }
+    } else {
+        double e = std::pow(2.0, (double)l.exposure);  // infiniteAreaLight.cpp:61-78 (double pow)
+        for (int k = 0; k < 3; ++k) d.Le[k] = (float)(l.color[k] * e);
+    }
+    return d;
+}
+
+static void upload(Ctx *c) {
+    HostScene &s = c->scene;
+    c->n_verts = (int)(s.verts.size() / 3);
+    c->n_tris = (int)(s.tris.size() / 3);
+    const int nt = c->n_tris;
+    c->d_verts.reserve(sizeof(float) * std::max<size_t>(s.verts.size(), 1));
+    c->d_tris.reserve(sizeof(int) * std::max<size_t>(s.tris.size(), 1));
+    if (!s.verts.empty())
+        HIP_CHECK(hipMemcpyAsync(c->d_verts.p, s.verts.data(), sizeof(float) * s.verts.size(), hipMemcpyHostToDevice, c->stream));
+    if (!s.tris.empty())
+        HIP_CHECK(hipMemcpyAsync(c->d_tris.p, s.tris.data(), sizeof(int) * s.tris.size(), hipMemcpyHostToDevice, c->stream));
+    for (int i = 0; i < nt; ++i)
+        for (int k = 0; k < 3; ++k) {
+            int v = s.tris[3 * i + k];
+            if (v < 0 || v >= c->n_verts) throw std::invalid_argument("triangle index out of range");
+        }
+    for (int a : s.attr)
+        if (a < 0 || a >= (int)s.attr_mats.size()) throw std::invalid_argument("attribute index out of range");
+    // shading records: (n0, attr), (n1, 0), (n2, 0)
+    std::vector<float> shade((size_t)12 * std::max(nt, 1), 0.f);
+    for (int i = 0; i < nt; ++i) {
+        for (int cidx = 0; cidx < 3; ++cidx)
+            for (int k = 0; k < 3; ++k) shade[12 * (size_t)i + 4 * cidx + k] = s.normals[9 * (size_t)i + 3 * cidx + k];
+        int a = s.attr[i];
+        std::memcpy(&shade[12 * (size_t)i + 3], &a, sizeof(int));
+    }
+    c->d_shade.reserve(sizeof(float) * shade.size());
+    HIP_CHECK(hipMemcpyAsync(c->d_shade.p, shade.data(), sizeof(float) * shade.size(), hipMemcpyHostToDevice, c->stream));
+
+    // materials and Kelemen tables (one per rounded reflectance and distribution,
+    // as the reference loads one table file per reflectance, kelemenMaterial.cpp:47-81)
+    c->dmats.clear();
+    c->table_ids.clear();
+    std::vector<std::pair<float, int>> tables;
+    for (const aur_material &m : s.attr_mats) {
+        DMat d;
+        std::memset(&d, 0, sizeof d);
+        d.type = m.type;
+        d.dist = m.distribution;
+        for (int k = 0; k < 3; ++k) {
+            d.diffuse[k] = m.diffuse[k];
+            d.spec[k] = m.spec[k];
+            d.gain[k] = m.gain[k];
+            d.emission[k] = m.emission[k];
+        }
+        d.rough = m.roughness;
+        d.roughB = m.roughnessB;
+        d.mix = m.mix;
+        d.refl = m.reflectance;
+        d.table = 0;
+        if (m.type == AUR_MAT_KELEMEN) {
+            int q = (int)std::lround(m.reflectance * 20.f);
+            q = std::min(std::max(q, 0), 20);
+            auto key = std::make_pair(q, m.distribution);
+            auto it = c->table_ids.find(key);
+            if (it == c->table_ids.end()) {
+                int id = (int)tables.size();
+                c->table_ids[key] = id;
+                tables.push_back({q / 20.f, m.distribution});
+                d.table = id;
+            } else {
+                d.table = it->second;
+            }
+        }
+        c->dmats.push_back(d);
+    }
+    c->n_mats = (int)c->dmats.size();
+    c->d_mats.reserve(sizeof(DMat) * std::max<size_t>(c->dmats.size(), 1));
+    if (!c->dmats.empty())
+        HIP_CHECK(hipMemcpyAsync(c->d_mats.p, c->dmats.data(), sizeof(DMat) * c->dmats.size(), hipMemcpyHostToDevice, c->stream));
+    const size_t tsz = AUR_KEL_COS * AUR_KEL_ROUGH;
+    c->d_tables.reserve(sizeof(float) * tsz * std::max<size_t>(tables.size(), 1));
+    for (size_t t = 0; t < tables.size(); ++t)
+        kelemen_table(c, tables[t].first, tables[t].second, 65536, 0x4B454Cu, c->d_tables.as<float>() + t * tsz);
+
+    c->dlights.clear();
+    c->n_env = 0;
+    c->env_le[0] = c->env_le[1] = c->env_le[2] = 0.f;
+    for (const aur_light &l : s.lights) {
+        DLight d = make_dlight(l);
+        if (d.type == LIGHT_ENV) {
+            ++c->n_env;
+            for (int k = 0; k < 3; ++k) c->env_le[k] += d.Le[k];
+        }
+        c->dlights.push_back(d);
+    }
+    c->n_lights = (int)c->dlights.size();
+    c->d_lights.reserve(sizeof(DLight) * std::max<size_t>(c->dlights.size(), 1));
+    if (!c->dlights.empty())
+        HIP_CHECK(hipMemcpyAsync(c->d_lights.p, c->dlights.data(), sizeof(DLight) * c->dlights.size(), hipMemcpyHostToDevice, c->stream));
+    if (!c->tri_albedo.empty()) {
+        if (c->tri_albedo.size() != (size_t)3 * nt) throw std::invalid_argument("per-triangle albedo has the wrong size");
+        c->d_albedo.reserve(sizeof(float) * c->tri_albedo.size());
+        HIP_CHECK(hipMemcpyAsync(c->d_albedo.p, c->tri_albedo.data(), sizeof(float) * c->tri_albedo.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// objects (sorted JSON order), then area-light quads (renderer.cpp:118-134)
+static void flatten(Ctx *c) {
+    HostScene &s = c->scene;
+    s.verts.clear(); s.normals.clear(); s.tris.clear(); s.attr.clear();
+    for (const HostObject &o : s.objects) {
+        if (o.subdlevels > 0 || o.displace.on) {
+            SubdivResult r = subdivide_object(c, o);
+            int base = (int)(s.verts.size() / 3);
+            s.verts.insert(s.verts.end(), r.verts.begin(), r.verts.end());
+            for (size_t i = 0; i < r.tris.size(); ++i) s.tris.push_back(base + r.tris[i]);
+            s.normals.insert(s.normals.end(), r.corner_normals.begin(), r.corner_normals.end());
+            for (size_t i = 0; i < r.tris.size() / 3; ++i) s.attr.push_back(o.attr);
+            continue;
+        }
+        // TriangleMesh::makeEmbree, triangleMesh.cpp:57-81
+        int base = (int)(s.verts.size() / 3);
+        size_t ncorner = o.P.size() / 3;
+        for (size_t k = 0; k < ncorner; ++k) {
+            float p[3];
+            transform_point(o.o2c, &o.P[3 * k], p);
+            s.verts.insert(s.verts.end(), p, p + 3);
+            float n[3];
+            transform_vector(o.o2c, &o.N[3 * k], n);
+            s.normals.insert(s.normals.end(), n, n + 3);
+        }
+        for (size_t t = 0; t < ncorner / 3; ++t) {
+            s.tris.push_back(base + (int)(3 * t));
+            s.tris.push_back(base + (int)(3 * t + 1));
+            s.tris.push_back(base + (int)(3 * t + 2));
+            s.attr.push_back(o.attr);
+        }
+    }
+    for (size_t i = 0; i < s.lights.size(); ++i) {
+        const aur_light &l = s.lights[i];
+        if (l.type != AUR_LIGHT_AREA) continue;
+        float v[12], n[3], nt[3];
+        light_geometry(l, v, n);
+        transform_vector(l.m, n, nt);
+        int base = (int)(s.verts.size() / 3);
+        s.verts.insert(s.verts.end(), v, v + 12);
+        const int idx[6] = {0, 1, 3, 2, 1, 3};
+        for (int t = 0; t < 2; ++t) {
+            for (int k = 0; k < 3; ++k) {
+                s.tris.push_back(base + idx[3 * t + k]);
+                s.normals.insert(s.normals.end(), nt, nt + 3);
+            }
+            s.attr.push_back(s.light_attr[i]);
+        }
+    }
+    s.flat = true;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *aur_last_error(void) { return g_last_error.c_str(); }
+
+int aur_create(aur_ctx **out, int hip_device) {
+    return guarded([&] {
+        NEED(out);
+        *out = nullptr;
+        if (hip_device == -1) {
+            // host-only context: scene parsing and flattening, no GPU work
+            std::unique_ptr<aur_ctx> c(new aur_ctx());
+            c->c.device = -1;
+            *out = c.release();
+            return;
+        }
+        int n = 0;
+        HIP_CHECK(hipGetDeviceCount(&n));
+        if (hip_device < 0 || hip_device >= n) throw std::invalid_argument("no such HIP device");
+        std::unique_ptr<aur_ctx> c(new aur_ctx());
+        c->c.device = hip_device;
+        set_device(&c->c);
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, hip_device));
+        c->c.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        c->c.own_stream = true;
+        *out = c.release();
+    });
+}
+
+void aur_destroy(aur_ctx *ctx) {
+    if (!ctx) return;
+    try {
+        if (ctx->c.device < 0) {
+            delete ctx;
+            return;
+        }
+        (void)hipSetDevice(ctx->c.device);
+        (void)hipStreamSynchronize(ctx->c.stream);
+        hipStream_t s = ctx->c.stream;
+        bool own = ctx->c.own_stream;
+        delete ctx;
+        if (own && s) (void)hipStreamDestroy(s);
+    } catch (...) {
+    }
+}
+
+int aur_set_stream(aur_ctx *ctx, void *hip_stream) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        set_device(c);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        if (hip_stream) {
+            c->stream = (hipStream_t)hip_stream;
+            c->own_stream = false;
+        } else {
+            HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+    });
+}
+
+int aur_set_seed(aur_ctx *ctx, uint32_t seed) {
+    return guarded([&] {
+        NEED(ctx);
+        ctx->c.seed = seed;
+    });
+}
+
+int aur_scene_load(aur_ctx *ctx, const char *scene_path, const char *config_json_path) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(scene_path);
+        HostScene s;
+        load_scene(scene_path, config_json_path ? config_json_path : "", &s);
+        ctx->c.scene = std::move(s);
+        ctx->c.have_scene = true;
+        ctx->c.built = false;
+        ctx->c.tri_albedo.clear();
+    });
+}
+
+int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(d);
+        if (d->n_tris < 0 || d->n_verts < 0 || d->n_attrs < 0 || d->n_lights < 0) throw std::invalid_argument("negative count");
+        if (d->n_tris > 0) { NEED(d->verts); NEED(d->tris); NEED(d->normals); NEED(d->attr); }
+        if (d->n_attrs > 0) NEED(d->mats);
+        if (d->n_lights > 0) NEED(d->lights);
+        if (d->width <= 0 || d->height <= 0) throw std::invalid_argument("resolution must be positive");
+        if (d->maxdepth < 0 || d->maxdepth > 200) throw std::invalid_argument("maxdepth out of range");
+        HostScene s;
+        s.opt.width = d->width;
+        s.opt.height = d->height;
+        s.opt.pixelsamples = d->pixelsamples;
+        s.opt.lightsamples = d->lightsamples;
+        s.opt.mindepth = d->mindepth;
+        s.opt.maxdepth = d->maxdepth;
+        s.opt.fov = d->fov;
+        s.verts.assign(d->verts, d->verts + 3 * (size_t)d->n_verts);
+        s.tris.assign(d->tris, d->tris + 3 * (size_t)d->n_tris);
+        s.normals.assign(d->normals, d->normals + 9 * (size_t)d->n_tris);
+        s.attr.assign(d->attr, d->attr + d->n_tris);
+        s.attr_mats.assign(d->mats, d->mats + d->n_attrs);
+        s.lights.assign(d->lights, d->lights + d->n_lights);
+        s.flat = true;
+        ctx->c.scene = std::move(s);
+        ctx->c.have_scene = true;
+        ctx->c.built = false;
+        ctx->c.tri_albedo.clear();
+    });
+}
+
+int aur_set_tri_albedo(aur_ctx *ctx, const float *albedo_rgb) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!albedo_rgb) { c->tri_albedo.clear(); return; }
+        if (!c->scene.flat) throw StateError("per-triangle albedo needs a flattened scene (call aur_build_accel first)");
+        size_t n = c->scene.tris.size();
+        c->tri_albedo.assign(albedo_rgb, albedo_rgb + n);
+        if (c->built) {
+            set_device(c);
+            c->d_albedo.reserve(sizeof(float) * n);
+            HIP_CHECK(hipMemcpyAsync(c->d_albedo.p, c->tri_albedo.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
+int aur_build_accel(aur_ctx *ctx, const aur_accel_opts *opts) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->have_scene) throw StateError("no scene loaded");
+        set_device(c);
+        int width = opts && opts->width ? opts->width : 4;
+        int leaf = opts && opts->max_leaf ? opts->max_leaf : 4;
+        if (width != 4 && width != 8) throw std::invalid_argument("BVH width must be 4 or 8");
+        if (leaf < 1 || leaf > 16) throw std::invalid_argument("max_leaf must be in 1..16");
+        if (!c->scene.flat) flatten(c);
+        upload(c);
+        build_bvh(c, width, leaf);
+        c->built = true;
+    });
+}
+
+int aur_scene_info_get(aur_ctx *ctx, aur_scene_info *out) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(out);
+        Ctx *c = &ctx->c;
+        if (!c->have_scene) throw StateError("no scene loaded");
+        if (!c->scene.flat && !needs_gpu_flatten(c->scene)) flatten(c);
+        const HostScene &s = c->scene;
+        std::memset(out, 0, sizeof *out);
+        out->n_verts = (int)(s.verts.size() / 3);
+        out->n_tris = (int)(s.tris.size() / 3);
+        out->n_attrs = (int)s.attr_mats.size();
+        out->n_lights = (int)s.lights.size();
+        out->n_objects = (int)s.objects.size();
+        out->width = s.opt.width;
+        out->height = s.opt.height;
+        out->pixelsamples = s.opt.pixelsamples;
+        out->lightsamples = s.opt.lightsamples;
+        out->mindepth = s.opt.mindepth;
+        out->maxdepth = s.opt.maxdepth;
+        out->fov = s.opt.fov;
+        out->bvh_width = c->bvh.width;
+        out->bvh_nodes = c->bvh.n_nodes;
+        out->bvh_depth = c->bvh.depth;
+        out->bvh_max_leaf = c->bvh.max_leaf;
+    });
+}
+
+int aur_scene_export(aur_ctx *ctx, float *verts, int *tris, float *normals, int *attr, aur_material *mats, aur_light *lights) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->have_scene) throw StateError("no scene loaded");
+        if (!c->scene.flat) {
+            if (needs_gpu_flatten(c->scene)) throw StateError("scene needs GPU subdivision (call aur_build_accel first)");
+            flatten(c);
+        }
+        const HostScene &s = c->scene;
+        if (verts) std::memcpy(verts, s.verts.data(), sizeof(float) * s.verts.size());
+        if (tris) std::memcpy(tris, s.tris.data(), sizeof(int) * s.tris.size());
+        if (normals) std::memcpy(normals, s.normals.data(), sizeof(float) * s.normals.size());
+        if (attr) std::memcpy(attr, s.attr.data(), sizeof(int) * s.attr.size());
+        if (mats) std::memcpy(mats, s.attr_mats.data(), sizeof(aur_material) * s.attr_mats.size());
+        if (lights) std::memcpy(lights, s.lights.data(), sizeof(aur_light) * s.lights.size());
+    });
+}
+
+int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host, aur_stats *out) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->built) throw StateError("render before aur_build_accel");
+        if (first_pass < 0 || n_passes < 0) throw std::invalid_argument("negative pass range");
+        set_device(c);
+        const int w = c->scene.opt.width, h = c->scene.opt.height;
+        const size_t wh = (size_t)w * h;
+        bool resized = c->fb_w != w || c->fb_h != h;
+        c->accum.reserve(sizeof(float4) * wh);
+        if (first_pass == 0 || resized) fill_zero(c, c->accum.p, sizeof(float4) * wh);
+        c->fb_w = w;
+        c->fb_h = h;
+        aur_stats st;
+        std::memset(&st, 0, sizeof st);
+        if (n_passes > 0) {
+            PathSource src;
+            src.mode = 0;
+            src.first_pass = first_pass;
+            src.pixel = src.sample = nullptr;
+            src.slots = nullptr;
+            run_paths(c, src, wh * (size_t)n_passes, c->accum.as<float4>(), &st);
+        }
+        if (rgba_sum_host) {
+            HIP_CHECK(hipMemcpyAsync(rgba_sum_host, c->accum.p, sizeof(float4) * wh, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        if (out) *out = st;
+    });
+}
+
+int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_paths, const float *slots, float *rgba_out,
+                     aur_stats *out) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->built) throw StateError("render before aur_build_accel");
+        if (n_paths < 0) throw std::invalid_argument("negative path count");
+        if (n_paths == 0) { if (out) std::memset(out, 0, sizeof *out); return; }
+        NEED(pixel); NEED(sample); NEED(rgba_out);
+        set_device(c);
+        const int wh = c->scene.opt.width * c->scene.opt.height;
+        for (int i = 0; i < n_paths; ++i)
+            if (pixel[i] < 0 || pixel[i] >= wh || sample[i] < 0) throw std::invalid_argument("pixel or sample index out of range");
+        const size_t nslots = 2 + AUR_SLOTS_PER_VERTEX * (size_t)c->scene.opt.maxdepth;
+        DevBuf dp, ds, dslots, dout;
+        dp.reserve(sizeof(int) * n_paths);
+        ds.reserve(sizeof(int) * n_paths);
+        dout.reserve(sizeof(float4) * n_paths);
+        HIP_CHECK(hipMemcpyAsync(dp.p, pixel, sizeof(int) * n_paths, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(ds.p, sample, sizeof(int) * n_paths, hipMemcpyHostToDevice, c->stream));
+        if (slots) {
+            dslots.reserve(sizeof(float) * nslots * n_paths);
+            HIP_CHECK(hipMemcpyAsync(dslots.p, slots, sizeof(float) * nslots * n_paths, hipMemcpyHostToDevice, c->stream));
+        }
+        PathSource src;
+        src.mode = 1;
+        src.first_pass = 0;
+        src.pixel = dp.as<int>();
+        src.sample = ds.as<int>();
+        src.slots = slots ? dslots.as<float>() : nullptr;
+        aur_stats st;
+        run_paths(c, src, (size_t)n_paths, dout.as<float4>(), &st);
+        HIP_CHECK(hipMemcpyAsync(rgba_out, dout.p, sizeof(float4) * n_paths, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (out) *out = st;
+    });
+}
+
+int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int any_hit) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (n < 0) throw std::invalid_argument("negative ray count");
+        if (n == 0) return;
+        NEED(rays); NEED(hits);
+        if (!c->built) throw StateError("trace before aur_build_accel");
+        set_device(c);
+        DevBuf dr, dh;
+        dr.reserve(sizeof(aur_ray) * n);
+        dh.reserve(sizeof(aur_hit) * n);
+        HIP_CHECK(hipMemcpyAsync(dr.p, rays, sizeof(aur_ray) * n, hipMemcpyHostToDevice, c->stream));
+        trace_rays(c, dr.as<aur_ray>(), n, dh.as<aur_hit>(), any_hit);
+        HIP_CHECK(hipMemcpyAsync(hits, dh.p, sizeof(aur_hit) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (n < 0) throw std::invalid_argument("negative query count");
+        if (n == 0) return;
+        NEED(q); NEED(r);
+        set_device(c);
+        DevBuf dq, drs;
+        dq.reserve(sizeof(aur_bsdf_query) * n);
+        drs.reserve(sizeof(aur_bsdf_result) * n);
+        HIP_CHECK(hipMemcpyAsync(dq.p, q, sizeof(aur_bsdf_query) * n, hipMemcpyHostToDevice, c->stream));
+        bsdf_queries(c, dq.as<aur_bsdf_query>(), n, drs.as<aur_bsdf_result>());
+        HIP_CHECK(hipMemcpyAsync(r, drs.p, sizeof(aur_bsdf_result) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int aur_kelemen_table(aur_ctx *ctx, float reflectance, int distribution, int n_samples, uint32_t seed, float *out) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(out);
+        if (n_samples <= 0) throw std::invalid_argument("n_samples must be positive");
+        Ctx *c = &ctx->c;
+        set_device(c);
+        DevBuf d;
+        d.reserve(sizeof(float) * AUR_KEL_COS * AUR_KEL_ROUGH);
+        kelemen_table(c, reflectance, distribution, n_samples, seed, d.as<float>());
+        HIP_CHECK(hipMemcpyAsync(out, d.p, sizeof(float) * AUR_KEL_COS * AUR_KEL_ROUGH, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+// context.h -- the library context behind `aur_ctx` and its device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <utility>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/aurora_hip.h"
+#include "dev_common.h"
+#include "host_scene.h"
+
+namespace aur {
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string &m) : std::runtime_error(m) {}
+};
+struct StateError : std::runtime_error {
+    explicit StateError(const std::string &m) : std::runtime_error(m) {}
+};
+struct CapacityError : std::runtime_error {
+    explicit CapacityError(const std::string &m) : std::runtime_error(m) {}
+};
+
+#define HIP_CHECK(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw ::aur::HipError(std::string(#expr) + " failed: " + hipGetErrorString(e_));     \
+    } while (0)
+
+// Owned device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows (never shrinks) to at least n bytes; contents are not preserved.
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        release();
+        size_t want = n < 256 ? 256 : n;
+        HIP_CHECK(hipMalloc(&p, want));
+        bytes = want;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
+};
+
+// Wide-BVH node: W boxes as SoA rows + W child codes (see bvh_build.hip).
+// child >= 0: inner node index; child < 0: leaf, ~child = (first << 4) | (count - 1).
+template <int W> struct alignas(16) WNode {
+    float lox[W], hix[W], loy[W], hiy[W], loz[W], hiz[W];
+    int child[W];
+    int pad[W];
+};
+
+struct Bvh {
+    int width = 4, max_leaf = 4;
+    int n_nodes = 0, depth = 0;
+    DevBuf nodes;       // WNode<width>[n_nodes]
+    DevBuf tri_sorted;  // float4[3 * n_tris]: (v0, prim), (e1, 0), (e2, 0)
+};
+
+// Wavefront working set, sized for `cap` paths per batch.
+struct Wavefront {
+    size_t cap = 0;
+    DevBuf ray[2];      // float4[2 * cap]: (o, tmin), (d, tmax) per queue slot
+    DevBuf qpath[2];    // int[cap]: path id per queue slot
+    DevBuf hit;         // float4[cap]
+    DevBuf T;           // float4[cap] by path id: throughput rgb, continue probability
+    DevBuf La, Lb;      // float4[cap] by path id: radiance partial sums
+    DevBuf flags;       // int[cap] by path id
+    DevBuf sray;        // float4[3 * 2cap]: shadow ray (o, tmin), (d, tmax), (contrib, path|type)
+    DevBuf finq;        // int[cap]: paths to finish after the shadow pass
+    DevBuf out;         // float4[cap]: per-path rgba
+    DevBuf counts;      // int[16]: queue sizes
+    DevBuf stats;       // unsigned long long[8]
+    DevBuf stack_ovf;   // traversal stack overflow
+    size_t stack_ovf_per_thread = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t seed = 0x5EED1234u;
+    int n_cu = 256;
+
+    HostScene scene;
+    bool have_scene = false;
+    bool built = false;
+    std::vector<float> tri_albedo;  // optional per-triangle override
+
+    // device scene
+    DevBuf d_verts, d_tris, d_shade, d_mats, d_lights, d_tables, d_albedo;
+    int n_verts = 0, n_tris = 0, n_mats = 0, n_lights = 0;
+    float env_le[3] = {0, 0, 0};
+    int n_env = 0;
+    Bvh bvh;
+    std::map<std::pair<int, int>, int> table_ids;  // (reflectance/20 index, distribution) -> table
+    std::vector<DMat> dmats;
+    std::vector<DLight> dlights;
+
+    // framebuffer (sum of per-pass rgba)
+    DevBuf accum;
+    int fb_w = 0, fb_h = 0;
+
+    Wavefront wf;
+};
+
+// bvh_build.hip
+void build_bvh(Ctx *c, int width, int max_leaf);
+// wavefront.hip
+struct PathSource {
+    int mode;              // 0: render passes, 1: explicit (pixel, sample) lists
+    int first_pass;
+    const int *pixel;      // device, mode 1
+    const int *sample;     // device, mode 1
+    const float *slots;    // device, optional injected uniforms
+};
+void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats);
+void resolve_passes(Ctx *c, const float4 *out_rgba, int n_passes, float4 *accum);
+void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_hit);
+// parity.hip
+void bsdf_queries(Ctx *c, const aur_bsdf_query *d_q, int n, aur_bsdf_result *d_r);
+void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out);
+// kernels used for device-side clears (graph safe, no hipMemsetAsync)
+void fill_zero(Ctx *c, void *p, size_t bytes);
+
+}  // namespace aur

@@ -1,0 +1,470 @@
+// host_scene.cpp -- scene input for the MI355X tracer (host side).
+//
+// Follows src/translator/jsonParser.cpp (scene keys and composition order),
+// src/core/transform.cpp + matrix4x4.cpp (transform arithmetic),
+// src/geometry/objParser.cpp + objTriangleMesh.cpp (mesh ingest) and
+// src/core/stringUtils.h (path templates).
+#include "host_scene.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <utility>
+
+#include "json_lite.h"
+
+namespace aur {
+
+// ---------------------------------------------------------------------------
+// matrices: matrix4x4.cpp
+// ---------------------------------------------------------------------------
+Mat4 Mat4::identity() {
+    Mat4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = (i == j) ? 1.f : 0.f;
+    return r;
+}
+
+Mat4 Mat4::mul(const Mat4 &b) const {
+    Mat4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            r.m[i][j] = m[i][0] * b.m[0][j] + m[i][1] * b.m[1][j] + m[i][2] * b.m[2][j] + m[i][3] * b.m[3][j];
+    return r;
+}
+
+// Gauss-Jordan elimination with full pivoting, the same pivot order and
+// float arithmetic as matrix4x4.cpp inverse().
+Mat4 Mat4::inverse() const {
+    int indxc[4], indxr[4];
+    int ipiv[4] = {0, 0, 0, 0};
+    float minv[4][4];
+    std::memcpy(minv, m, sizeof minv);
+    for (int i = 0; i < 4; i++) {
+        int irow = -1, icol = -1;
+        float big = 0.f;
+        for (int j = 0; j < 4; j++) {
+            if (ipiv[j] != 1) {
+                for (int k = 0; k < 4; k++) {
+                    if (ipiv[k] == 0) {
+                        if (std::fabs(minv[j][k]) >= big) {
+                            big = std::fabs(minv[j][k]);
+                            irow = j;
+                            icol = k;
+                        }
+                    } else if (ipiv[k] > 1) {
+                        throw std::runtime_error("singular matrix in transform inverse");
+                    }
+                }
+            }
+        }
+        if (icol < 0) throw std::runtime_error("singular matrix in transform inverse");
+        ++ipiv[icol];
+        if (irow != icol)
+            for (int k = 0; k < 4; ++k) std::swap(minv[irow][k], minv[icol][k]);
+        indxr[i] = irow;
+        indxc[i] = icol;
+        if (minv[icol][icol] == 0.f) throw std::runtime_error("singular matrix in transform inverse");
+        float pivinv = 1.f / minv[icol][icol];
+        minv[icol][icol] = 1.f;
+        for (int j = 0; j < 4; j++) minv[icol][j] *= pivinv;
+        for (int j = 0; j < 4; j++) {
+            if (j != icol) {
+                float save = minv[j][icol];
+                minv[j][icol] = 0;
+                for (int k = 0; k < 4; k++) minv[j][k] -= minv[icol][k] * save;
+            }
+        }
+    }
+    for (int j = 3; j >= 0; j--) {
+        if (indxr[j] != indxc[j])
+            for (int k = 0; k < 4; k++) std::swap(minv[k][indxr[j]], minv[k][indxc[j]]);
+    }
+    Mat4 r;
+    std::memcpy(r.m, minv, sizeof minv);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// transform.cpp:44-104: post-multiplication, inverse recomputed each time
+// ---------------------------------------------------------------------------
+static Mat4 make(float a00, float a01, float a02, float a03, float a10, float a11, float a12, float a13,
+                 float a20, float a21, float a22, float a23, float a30, float a31, float a32, float a33) {
+    Mat4 r;
+    float v[16] = {a00, a01, a02, a03, a10, a11, a12, a13, a20, a21, a22, a23, a30, a31, a32, a33};
+    std::memcpy(r.m, v, sizeof v);
+    return r;
+}
+// math_utils.h:212-214: degrees * M_PI / 180.f evaluated in double
+static float radians(float deg) { return (float)(deg * M_PI / 180.f); }
+
+void Xform::translate(float x, float y, float z) {
+    m = m.mul(make(1, 0, 0, x, 0, 1, 0, y, 0, 0, 1, z, 0, 0, 0, 1));
+    inv = m.inverse();
+}
+void Xform::scale(float x, float y, float z) {
+    m = m.mul(make(x, 0, 0, 0, 0, y, 0, 0, 0, 0, z, 0, 0, 0, 0, 1));
+    inv = m.inverse();
+}
+void Xform::rotate_x(float a) {
+    float s = sinf(radians(a)), c = cosf(radians(a));
+    m = m.mul(make(1, 0, 0, 0, 0, c, -s, 0, 0, s, c, 0, 0, 0, 0, 1));
+    inv = m.inverse();
+}
+void Xform::rotate_y(float a) {
+    float s = sinf(radians(a)), c = cosf(radians(a));
+    m = m.mul(make(c, 0, s, 0, 0, 1, 0, 0, -s, 0, c, 0, 0, 0, 0, 1));
+    inv = m.inverse();
+}
+void Xform::rotate_z(float a) {
+    float s = sinf(radians(a)), c = cosf(radians(a));
+    m = m.mul(make(c, -s, 0, 0, s, c, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1));
+    inv = m.inverse();
+}
+
+// transform.cpp:206-214 (points) and 226-232 (vectors)
+static void xpoint(const Mat4 &M, const float *p, float *o) {
+    const float(*m)[4] = M.m;
+    float x = p[0], y = p[1], z = p[2];
+    float xp = m[0][0] * x + m[0][1] * y + m[0][2] * z + m[0][3];
+    float yp = m[1][0] * x + m[1][1] * y + m[1][2] * z + m[1][3];
+    float zp = m[2][0] * x + m[2][1] * y + m[2][2] * z + m[2][3];
+    float wp = m[3][0] * x + m[3][1] * y + m[3][2] * z + m[3][3];
+    if (wp == 1.f) { o[0] = xp; o[1] = yp; o[2] = zp; return; }
+    float inv = 1.f / wp;
+    o[0] = xp * inv; o[1] = yp * inv; o[2] = zp * inv;
+}
+static void xvector(const Mat4 &M, const float *v, float *o) {
+    const float(*m)[4] = M.m;
+    float x = v[0], y = v[1], z = v[2];
+    o[0] = m[0][0] * x + m[0][1] * y + m[0][2] * z;
+    o[1] = m[1][0] * x + m[1][1] * y + m[1][2] * z;
+    o[2] = m[2][0] * x + m[2][1] * y + m[2][2] * z;
+}
+
+void transform_point(const float *m16, const float *p, float *out) {
+    Mat4 M;
+    std::memcpy(M.m, m16, sizeof M.m);
+    xpoint(M, p, out);
+}
+void transform_vector(const float *m16, const float *v, float *out) {
+    Mat4 M;
+    std::memcpy(M.m, m16, sizeof M.m);
+    xvector(M, v, out);
+}
+
+// ---------------------------------------------------------------------------
+// config.json templates: stringUtils.h:37-122
+// ---------------------------------------------------------------------------
+static std::string read_file(const std::string &path) {
+    std::ifstream f(path.c_str(), std::ios::in | std::ios::binary);
+    if (!f.good()) throw std::runtime_error("cannot open file: " + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+std::string string_template(const std::string &input, const std::string &config_path) {
+    if (input.find('$') == std::string::npos && input.find("####") == std::string::npos) return input;
+    std::map<std::string, std::string> rep;
+    if (!config_path.empty()) {
+        JsonValue cfg = JsonReader(read_file(config_path)).parse();
+        auto get = [&](const char *k) { return cfg.has(k) ? cfg.at(k).as_string() : std::string(); };
+        rep["$BASE"] = get("basepath");
+        rep["$CACHE"] = get("cachepath");
+        rep["$RENDER"] = get("renderpath");
+        rep["$TEXTURE"] = get("texturepath");
+        rep["$HDRI"] = get("hdripath");
+    }
+    std::string out = input;
+    while (out.find('$') != std::string::npos) {
+        bool did = false;
+        for (auto &kv : rep) {
+            size_t pos = out.find(kv.first);
+            if (pos != std::string::npos) { out.replace(pos, kv.first.size(), kv.second); did = true; }
+        }
+        if (!did) throw std::runtime_error("cannot replace unknown keywords in string: " + out);
+    }
+    size_t dbl = out.find("//");
+    while (dbl != std::string::npos) { out.replace(dbl, 2, "/"); dbl = out.find("//"); }
+    size_t hash = out.find("####");
+    if (hash != std::string::npos) {
+        for (int frame = 1;; ++frame) {
+            char num[32];
+            std::snprintf(num, sizeof num, "%04d", frame);
+            std::string cand = out;
+            cand.replace(hash, 4, num);
+            std::ifstream f(cand.c_str());
+            if (!f.good()) { out = cand; break; }
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// OBJ: objParser.cpp:58-202, objTriangleMesh.cpp:26-92
+// ---------------------------------------------------------------------------
+static int leading_int(const std::string &s) { return std::atoi(s.c_str()); }
+
+static void load_obj(const std::string &path, bool need_normals, HostObject *obj) {
+    std::ifstream f(path.c_str());
+    if (!f.good()) throw std::runtime_error("cannot open OBJ file: " + path);
+    std::vector<float> pts, nrm;
+    std::vector<int> vi, ni;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty()) continue;
+        std::stringstream ss(line);
+        std::string op;
+        ss >> op;
+        if (line[0] == 'v') {
+            float a = 0, b = 0, c = 0;
+            if (line.size() > 1 && line[1] == 'n') { ss >> a >> b >> c; nrm.push_back(a); nrm.push_back(b); nrm.push_back(c); }
+            else if (line.size() > 1 && line[1] == 't') { /* uvs are not used by the tracer */ }
+            else { ss >> a >> b >> c; pts.push_back(a); pts.push_back(b); pts.push_back(c); }
+        } else if (line[0] == 'f') {
+            std::string e[3];
+            ss >> e[0] >> e[1] >> e[2];
+            for (int k = 0; k < 3; ++k) {
+                vi.push_back(leading_int(e[k].substr(0, e[k].find('/'))));
+                size_t last = e[k].find_last_of('/');
+                ni.push_back(leading_int(last == std::string::npos ? e[k] : e[k].substr(last + 1)));
+            }
+        }
+    }
+    size_t nverts = vi.size();
+    obj->P.resize(nverts * 3);
+    obj->N.resize(nverts * 3);
+    for (size_t i = 0; i < nverts; ++i) {
+        int v = vi[i] - 1, n = ni[i] - 1;
+        if (v < 0 || (size_t)v * 3 + 2 >= pts.size() + 0 || (size_t)(v * 3 + 2) >= pts.size())
+            throw std::runtime_error("OBJ vertex index out of range in " + path);
+        for (int k = 0; k < 3; ++k) obj->P[3 * i + k] = pts[3 * v + k];
+        if (n >= 0 && (size_t)(n * 3 + 2) < nrm.size()) {
+            float x = nrm[3 * n], y = nrm[3 * n + 1], z = nrm[3 * n + 2];
+            if (x == 0.f && y == 0.f && z == 0.f) { x = 0.f; y = 1.f; z = 0.f; }  // objTriangleMesh.cpp:232-235
+            obj->N[3 * i] = x; obj->N[3 * i + 1] = y; obj->N[3 * i + 2] = z;
+        } else if (need_normals) {
+            throw std::runtime_error("OBJ normal index out of range in " + path);
+        } else {
+            obj->N[3 * i] = 0.f; obj->N[3 * i + 1] = 1.f; obj->N[3 * i + 2] = 0.f;
+        }
+    }
+    if (nverts % 3) throw std::runtime_error("OBJ face list is not triangles: " + path);
+}
+
+// ---------------------------------------------------------------------------
+// scene: jsonParser.cpp
+// ---------------------------------------------------------------------------
+static Xform transforms_attr(const JsonValue &root) {
+    // jsonParser.cpp:168-223
+    Xform t;
+    const JsonValue &list = root.at("transforms");
+    if (list.kind != JsonValue::Array && list.kind != JsonValue::Object) throw std::runtime_error("transforms must be a list");
+    for (const JsonValue &item : list.arr) {
+        if (item.kind != JsonValue::Object || item.obj.empty()) continue;
+        const std::string &k = item.obj.begin()->first;
+        const JsonValue &v = item.obj.begin()->second;
+        if (k == "translate") t.translate((float)v.arr.at(0).as_double(), (float)v.arr.at(1).as_double(), (float)v.arr.at(2).as_double());
+        else if (k == "scale") t.scale((float)v.arr.at(0).as_double(), (float)v.arr.at(1).as_double(), (float)v.arr.at(2).as_double());
+        else if (k == "rotateX") t.rotate_x((float)v.as_double());
+        else if (k == "rotateY") t.rotate_y((float)v.as_double());
+        else if (k == "rotateZ") t.rotate_z((float)v.as_double());
+    }
+    return t;
+}
+
+static void color_attr(const JsonValue &root, const char *name, float *out) {
+    const JsonValue &v = root.at(name);
+    if (v.kind == JsonValue::String) throw std::runtime_error(std::string("textured parameter not supported: ") + name);
+    if (v.kind != JsonValue::Array || v.arr.size() != 3) throw std::runtime_error(std::string("expected 3 values for ") + name);
+    for (int i = 0; i < 3; ++i) out[i] = (float)v.arr[i].as_double();
+}
+static float float_attr(const JsonValue &root, const char *name) {
+    const JsonValue &v = root.at(name);
+    if (v.kind == JsonValue::String) throw std::runtime_error(std::string("textured parameter not supported: ") + name);
+    return (float)v.as_double();
+}
+static int distribution_attr(const JsonValue &root) {
+    if (!root.has("distribution")) return AUR_DIST_BECKMANN;
+    const std::string &d = root.at("distribution").as_string();
+    if (d == "ggx" || d == "GGX") return AUR_DIST_GGX;
+    if (d == "beckmann" || d == "Beckmann") return AUR_DIST_BECKMANN;
+    throw std::runtime_error("unknown microfacet distribution: " + d);
+}
+
+// jsonParser.cpp:313-429
+static aur_material parse_material(const JsonValue &root) {
+    aur_material m;
+    std::memset(&m, 0, sizeof m);
+    const std::string &type = root.at("type").as_string();
+    if (type == "kelemenMaterial") {
+        m.type = AUR_MAT_KELEMEN;
+        color_attr(root, "diffusecolor", m.diffuse);
+        color_attr(root, "speccolor", m.spec);
+        m.reflectance = float_attr(root, "reflectance");
+        m.roughness = float_attr(root, "roughness");
+        m.distribution = distribution_attr(root);
+    } else if (type == "matteMaterial") {
+        m.type = AUR_MAT_MATTE;
+        color_attr(root, "color", m.diffuse);
+    } else if (type == "metalMaterial") {
+        m.type = AUR_MAT_METAL;
+        color_attr(root, "reflectance", m.spec);
+        m.roughness = float_attr(root, "roughnessA");
+        m.roughnessB = float_attr(root, "roughnessB");
+        m.mix = float_attr(root, "mix");
+        color_attr(root, "gain", m.gain);
+        m.distribution = distribution_attr(root);
+    } else {
+        throw std::runtime_error("unsupported material type: " + type);
+    }
+    return m;
+}
+
+void light_corners(const aur_light &l, float pc[4][3], float N[3]) {
+    Mat4 M;
+    std::memcpy(M.m, l.m, sizeof M.m);
+    float c[4][3] = {{-l.sx, -l.sy, 0}, {-l.sx, l.sy, 0}, {l.sx, l.sy, 0}, {l.sx, -l.sy, 0}};
+    for (int i = 0; i < 4; ++i) xpoint(M, c[i], pc[i]);
+    // squareLight.cpp:35-38: normalize(cross(p1 - p2, p2 - p3))
+    float v1[3] = {pc[0][0] - pc[1][0], pc[0][1] - pc[1][1], pc[0][2] - pc[1][2]};
+    float v2[3] = {pc[1][0] - pc[2][0], pc[1][1] - pc[2][1], pc[1][2] - pc[2][2]};
+    float n[3] = {(v1[1] * v2[2]) - (v1[2] * v2[1]), (v1[2] * v2[0]) - (v1[0] * v2[2]), (v1[0] * v2[1]) - (v1[1] * v2[0])};
+    float len = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (len == 0.f) len = 0.0001f;
+    float inv = 1.f / len;
+    N[0] = n[0] * inv; N[1] = n[1] * inv; N[2] = n[2] * inv;
+}
+
+// Light quad as two triangles, object vertex order {0,1,3, 2,1,3}
+// (squareLight.cpp shape()), camera-space corners.
+void light_geometry(const aur_light &l, float verts[12], float n[3]) {
+    float pc[4][3];
+    light_corners(l, pc, n);
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 3; ++k) verts[3 * i + k] = pc[i][k];
+}
+
+void load_scene(const std::string &scene_path, const std::string &config_path, HostScene *out) {
+    HostScene s;
+    JsonValue root = JsonReader(read_file(scene_path)).parse();
+    if (root.kind != JsonValue::Object || root.obj.empty()) throw std::runtime_error("no keys in json object: " + scene_path);
+    std::string scene_dir;
+    size_t slash = scene_path.find_last_of('/');
+    if (slash != std::string::npos) scene_dir = scene_path.substr(0, slash + 1);
+
+    // jsonParser.cpp:431-475 (all six keys are required)
+    if (root.has("options")) {
+        const JsonValue &o = root.at("options");
+        s.opt.pixelsamples = o.at("pixelsamples").as_int();
+        s.opt.lightsamples = o.at("lightsamples").as_int();
+        s.opt.mindepth = o.at("mindepth").as_int();
+        s.opt.maxdepth = o.at("maxdepth").as_int();
+        const JsonValue &r = o.at("resolution");
+        if (r.kind != JsonValue::Array || r.arr.size() != 2) throw std::runtime_error("resolution needs two values");
+        s.opt.width = r.arr[0].as_int();
+        s.opt.height = r.arr[1].as_int();
+        s.opt.filename = string_template(o.at("filename").as_string(), config_path);
+    }
+    // jsonParser.cpp:477-505
+    Xform cam;
+    if (root.has("renderCam")) {
+        const JsonValue &c = root.at("renderCam");
+        s.opt.fov = (float)c.at("fov").as_double();
+        cam = transforms_attr(c);
+    }
+    Mat4 inv_cam = cam.inv;  // Transform::inverse swaps m and mInv (transform.cpp:36-38)
+
+    // materials (jsonParser.cpp:507-535), looked up by name at frameBegin
+    std::map<std::string, aur_material> mats;
+    if (root.has("materials"))
+        for (auto &kv : root.at("materials").obj) mats[kv.first] = parse_material(kv.second);
+
+    // objects in sorted key order (jsonParser.cpp:619-661)
+    std::vector<aur_light> lights;
+    if (root.has("objects")) {
+        for (auto &kv : root.at("objects").obj) {
+            const JsonValue &o = kv.second;
+            const std::string &type = o.at("type").as_string();
+            if (type == "geometry") {
+                // jsonParser.cpp:225-241
+                HostObject obj;
+                obj.name = o.at("name").as_string();
+                obj.material = o.at("material").as_string();
+                obj.subdlevels = o.at("subdlevels").as_int();
+                Xform t = transforms_attr(o);
+                Mat4 o2c = inv_cam.mul(t.m);
+                std::memcpy(obj.o2c, o2c.m, sizeof obj.o2c);
+                std::string path = string_template(o.at("path").as_string(), config_path);
+                if (!path.empty() && path[0] != '/') {
+                    std::ifstream probe(path.c_str());
+                    if (!probe.good()) path = scene_dir + path;
+                }
+                if (o.has("displace")) {
+                    const JsonValue &d = o.at("displace");
+                    obj.displace.on = true;
+                    obj.displace.amplitude = (float)d.at("amplitude").as_double();
+                    obj.displace.octaves = d.has("octaves") ? d.at("octaves").as_int() : 3;
+                    obj.displace.frequency = d.has("frequency") ? (float)d.at("frequency").as_double() : 1.f;
+                    obj.displace.seed = d.has("seed") ? (uint32_t)d.at("seed").as_double() : 0u;
+                }
+                load_obj(path, obj.subdlevels == 0, &obj);
+                if (mats.find(obj.material) == mats.end()) throw std::runtime_error("unknown material: " + obj.material);
+                s.objects.push_back(std::move(obj));
+            } else if (type == "light") {
+                // jsonParser.cpp:243-263
+                aur_light l;
+                std::memset(&l, 0, sizeof l);
+                l.type = AUR_LIGHT_AREA;
+                Xform t = transforms_attr(o);
+                Mat4 o2c = inv_cam.mul(t.m);
+                std::memcpy(l.m, o2c.m, sizeof l.m);
+                l.sx = (float)o.at("scaleX").as_double();
+                l.sy = (float)o.at("scaleY").as_double();
+                l.exposure = (float)o.at("exposure").as_double();
+                color_attr(o, "color", l.color);
+                o.at("name").as_string();
+                lights.push_back(l);
+            } else if (type == "envlight") {
+                // jsonParser.cpp:265-282 (constant branch only)
+                aur_light l;
+                std::memset(&l, 0, sizeof l);
+                l.type = AUR_LIGHT_ENV;
+                Xform t = transforms_attr(o);
+                std::memcpy(l.m, t.m.m, sizeof l.m);
+                l.exposure = (float)o.at("exposure").as_double();
+                color_attr(o, "color", l.color);
+                if (!o.at("envmap").as_string().empty()) throw std::runtime_error("environment maps are not supported");
+                lights.push_back(l);
+            }
+            // unknown object types are skipped with a warning in the reference
+        }
+    }
+    // attributes: objects first, then lights (renderer.cpp:118-134)
+    for (size_t i = 0; i < s.objects.size(); ++i) {
+        s.objects[i].attr = (int)i;
+        aur_material m = mats[s.objects[i].material];
+        s.attr_mats.push_back(m);
+    }
+    for (size_t i = 0; i < lights.size(); ++i) {
+        aur_material m;
+        std::memset(&m, 0, sizeof m);
+        m.type = AUR_MAT_LIGHT;
+        if (lights[i].type == AUR_LIGHT_AREA) {
+            // SquareLight::emission(): color * powf(2, exposure)
+            float e = powf(2, lights[i].exposure);
+            for (int k = 0; k < 3; ++k) m.emission[k] = lights[i].color[k] * e;
+        }
+        s.light_attr.push_back((int)(s.objects.size() + i));
+        s.attr_mats.push_back(m);
+    }
+    s.lights = lights;
+    *out = std::move(s);
+}
+
+}  // namespace aur

@@ -1,0 +1,31 @@
+// subdiv.h -- Loop subdivision and displacement on the GPU (preprocessing).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "context.h"
+
+namespace aur {
+
+struct SubdivResult {
+    std::vector<float> verts;           // camera space, 3 per vertex
+    std::vector<int> tris;              // 3 per triangle, local indices
+    std::vector<float> corner_normals;  // camera space, 9 per triangle
+};
+
+// Object-space indexed mesh after subdivision (parity entry point).
+struct LoopMesh {
+    std::vector<float> P;  // 3 per vertex
+    std::vector<int> F;    // 3 per face
+    std::vector<float> N;  // 3 per vertex (limit normals, or area-weighted after displacement)
+};
+
+// Welds the de-indexed corners by exact position, runs `levels` Loop steps,
+// pushes to the limit surface, computes normals, optionally displaces.
+LoopMesh loop_subdivide(Ctx *c, const std::vector<float> &P_indexed, const std::vector<int> &F, int levels,
+                        const Displace &disp);
+SubdivResult subdivide_object(Ctx *c, const HostObject &o);
+
+}  // namespace aur

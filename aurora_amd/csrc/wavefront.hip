@@ -1,0 +1,617 @@
+// wavefront.hip -- the path-tracing hot path as a wavefront of kernels.
+//
+// Replaces the per-pixel recursion of `Integrator::integrateCameraSample`
+// (src/frontend/integrator.cpp:152-300) run inside the TBB pixel loop
+// (src/frontend/renderer.cpp:147-271).  One batch holds many paths (all
+// pixels of several passes); per bounce:
+//
+//   extend  -- closest-hit traversal of the extension-ray queue
+//   shade   -- one vertex of every live path: emission, getBrdf, next-event
+//              estimation (light sample, and for specular lobes a BSDF sample
+//              with the power heuristic), next direction, throughput, roulette;
+//              writes shadow rays, the next extension queue and the finish list
+//   shadow  -- any-hit traversal; unoccluded contributions are added to the
+//              path's radiance (one accumulator per shadow-ray kind, so the
+//              two rays of one path never race)
+//   finish  -- firefly clamp (integrator.cpp:277-281) and per-path output
+//
+// Queues are compacted with one atomic per wave: a 64-lane ballot gives each
+// lane its offset (mbcnt) and the wave's count (popcount).  Every kernel is
+// a grid-stride loop over a count that lives in device memory, so a whole
+// frame is a fixed sequence of launches with no host round trip.
+#include "context.h"
+#include "traverse.h"
+
+namespace aur {
+
+enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2 };
+enum { F_MATTE = 1 << 2, F_NOCLAMP = 1 << 24 };
+enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_FINISH = 3 };
+enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
+
+struct SceneView {
+    const void *nodes;
+    const float4 *tris;
+    const float4 *shade;    // 3 per triangle: (n0, attr), (n1, 0), (n2, 0)
+    const DMat *mats;
+    const DLight *lights;
+    const float *tables;
+    const float *albedo;    // optional per-triangle diffuse override
+    int n_lights;
+    int has_env;
+    float env[3];
+    float fov;
+    int width, height, maxdepth, mindepth;
+};
+
+struct WfView {
+    float4 *ray[2];
+    int *qpath[2];
+    float4 *hit;
+    float4 *T, *La, *Lb;
+    int *flags;
+    float4 *sray;
+    int *finq;
+    float4 *out;
+    int *counts;
+    unsigned long long *stats;
+    int *ovf;
+    int ovf_per_thread;
+    PathSource src;
+    uint32_t seed;
+    int nslots;
+};
+
+// ---------------------------------------------------------------------------
+// wave-level helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline int wave_reserve(int *counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (mask && lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ inline void wave_count(unsigned long long *counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    if (mask && lane == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
+struct PathRng {
+    const float *slots;
+    uint32_t key;
+    __device__ inline float u(int slot) const { return slots ? slots[slot] : slot_uniform(key, (uint32_t)slot); }
+};
+
+__device__ inline void path_pixel_sample(const SceneView &sv, const WfView &wv, int path, int *pixel, int *sample) {
+    if (wv.src.mode == 0) {
+        int wh = sv.width * sv.height;
+        *pixel = path % wh;
+        *sample = wv.src.first_pass + path / wh;
+    } else {
+        *pixel = wv.src.pixel[path];
+        *sample = wv.src.sample[path];
+    }
+}
+
+__device__ inline PathRng path_rng(const SceneView &sv, const WfView &wv, int path) {
+    PathRng r;
+    int pixel, sample;
+    path_pixel_sample(sv, wv, path, &pixel, &sample);
+    r.slots = wv.src.slots ? wv.src.slots + (size_t)path * wv.nslots : nullptr;
+    r.key = path_key(wv.seed, (uint32_t)pixel, (uint32_t)sample);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// generate: camera.cpp:34-62
+// ---------------------------------------------------------------------------
+__global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
+    int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_paths; i += stride) {
+        int pixel, sample;
+        path_pixel_sample(sv, wv, i, &pixel, &sample);
+        PathRng rng = path_rng(sv, wv, i);
+        int x = pixel % sv.width, y = pixel / sv.width;
+        float fovx = sv.fov;
+        float fovy = ((float)sv.height / (float)sv.width) * fovx;
+        float u = x + rng.u(0), v = y + rng.u(1);
+        f3 d = mk3(((2.f * u - sv.width) / (float)sv.width) * fovx, ((-2.f * v + sv.height) / (float)sv.height) * fovy, -1.f);
+        d = norm3(d);
+        wv.ray[0][2 * i] = make_float4(0.f, 0.f, 0.f, 0.1f);
+        wv.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
+        wv.qpath[0][i] = i;
+        wv.T[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+        wv.La[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.Lb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.flags[i] = RT_CAMERA;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        wv.counts[C_Q0] = n_paths;
+        atomicAdd(&wv.stats[S_PATHS], (unsigned long long)n_paths);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// extend: closest hit
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(128) k_extend(SceneView sv, WfView wv, int q) {
+    __shared__ int s_stack[AUR_STACK_LDS * 128];
+    const int n = wv.counts[q];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_EXT], (unsigned long long)n);
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = 128;
+    st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
+    const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
+    for (int i = gtid; i < n; i += stride) {
+        float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
+        TravHit h;
+        traverse<W, false>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
+        wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
+        wave_count(&wv.stats[S_HIT], h.prim >= 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// shadow: any hit; unoccluded contributions go to La (light sample) or Lb
+// (BSDF sample)
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(128) k_shadow(SceneView sv, WfView wv) {
+    __shared__ int s_stack[AUR_STACK_LDS * 128];
+    const int n = wv.counts[C_SHADOW];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_SHADOW], (unsigned long long)n);
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = 128;
+    st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
+    const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
+    for (int i = gtid; i < n; i += stride) {
+        float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
+        TravHit h;
+        traverse<W, true>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
+        if (h.prim < 0) {
+            int tag = __float_as_int(c.w);
+            int path = tag & 0x7fffffff;
+            float4 *acc = (tag < 0) ? wv.Lb : wv.La;
+            float4 v = acc[path];
+            v.x += c.x; v.y += c.y; v.z += c.z;
+            acc[path] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// shade: one path vertex (integrator.cpp:180-274)
+// ---------------------------------------------------------------------------
+struct ShadowOut { bool valid; f3 o, d; float tmin, tmax; f3 c; };
+
+__device__ inline Bsdf get_brdf(const SceneView &sv, const DMat &m, int prim, f3 vn, f3 nn, bool matte, float u_lobe) {
+    Bsdf b;
+    f3 diffuse = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+    if (sv.albedo) diffuse = mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2]);
+    if (m.type == MAT_MATTE) {  // matteMaterial.cpp:30-35
+        b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f;
+    } else if (m.type == MAT_METAL) {  // metalMaterial.cpp:32-62
+        float ct = dot3(vn, nn);
+        f3 sc = mk3(schlick(ct, m.spec[0]), schlick(ct, m.spec[1]), schlick(ct, m.spec[2]));
+        float rough = (u_lobe > m.mix) ? m.rough : m.roughB;
+        if (matte) rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR;
+        b.kind = BSDF_CT; b.dist = m.dist; b.color = mul3v(sc, mk3(m.gain[0], m.gain[1], m.gain[2])); b.rough = rough;
+    } else {  // kelemenMaterial.cpp:225-255
+        float rough = m.rough;
+        if (matte) rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR;
+        b.kind = BSDF_CT; b.dist = m.dist; b.color = mk3(m.spec[0], m.spec[1], m.spec[2]); b.rough = rough;
+        if (m.refl < 1.f) {
+            float c = fminf(fmaxf(dot3(nn, vn), 0.f), 1.f);
+            float albedo = kelemen_albedo(sv.tables + (size_t)m.table * (AUR_KEL_COS * AUR_KEL_ROUGH), c, rough);
+            if (u_lobe > albedo) { b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f; }
+        }
+    }
+    return b;
+}
+
+__global__ void __launch_bounds__(256) k_shade(SceneView sv, WfView wv, int q) {
+    const int n = wv.counts[q];
+    const int stride = gridDim.x * blockDim.x;
+    const int nq = 1 - q;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int path = wv.qpath[q][i];
+        const float4 ra = wv.ray[q][2 * i], rb = wv.ray[q][2 * i + 1];
+        const float4 hv = wv.hit[i];
+        const int prim = __float_as_int(hv.w);
+        int flags = wv.flags[path];
+        const int rtype = flags & 3;
+        const int tb = (flags >> 8) & 0xff;
+        bool cont = false, finish = false;
+        ShadowOut sh[2];
+        sh[0].valid = sh[1].valid = false;
+        f3 P = mk3(0.f, 0.f, 0.f), ndir = mk3(0.f, 0.f, 0.f);
+        float ntmax = 0.f;
+        if (prim < 0) {
+            finish = true;
+            if (rtype == RT_CAMERA) {  // integrator.cpp:287-297: camera miss sees the environment
+                wv.La[path] = make_float4(sv.env[0], sv.env[1], sv.env[2], sv.has_env ? 1.f : 0.f);
+                flags |= F_NOCLAMP;
+            }
+        } else if (tb >= sv.maxdepth) {
+            finish = true;  // the extension ray after the last vertex is traced but not shaded
+        } else {
+            PathRng rng = path_rng(sv, wv, path);
+            const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
+            // hit record: embreeAccelerator.cpp:111-136
+            const f3 o = mk3(ra.x, ra.y, ra.z), d = mk3(rb.x, rb.y, rb.z);
+            const float t = hv.x, u = hv.y, v = hv.z;
+            const float w = 1.f - u - v;
+            P = add3(o, mul3(d, t));
+            const float4 s0 = sv.shade[3 * prim], s1 = sv.shade[3 * prim + 1], s2 = sv.shade[3 * prim + 2];
+            const int attr = __float_as_int(s0.w);
+            f3 nm = mk3(w * s0.x + u * s1.x + v * s2.x, w * s0.y + u * s1.y + v * s2.y, w * s0.z + u * s1.z + v * s2.z);
+            if (dot3(nm, d) > 0.f) nm = mul3(nm, -1.f);
+            f3 nn = norm3(nm);
+            // integrator.cpp:132-150
+            const f3 vn = neg3(norm3(d));
+            if (dot3(vn, nn) <= 0.f) nn = norm3(cross3(vn, cross3(nn, vn)));
+            bool matte = (flags & F_MATTE) != 0;
+            if (rtype == RT_DIFFUSE) matte = true;
+            float4 T4 = wv.T[path];
+            f3 T = mk3(T4.x, T4.y, T4.z);
+            float cp = T4.w;
+            const DMat m = sv.mats[attr];
+            if (rtype == RT_CAMERA) {
+                float4 la = wv.La[path];
+                la.x += T.x * m.emission[0]; la.y += T.y * m.emission[1]; la.z += T.z * m.emission[2];
+                la.w = 1.f;
+                wv.La[path] = la;
+            }
+            if (m.type == MAT_LIGHT) {
+                finish = true;  // ConstantMaterial: getSample has pdf 0 (constant.cpp:23-25)
+            } else {
+                Bsdf b = get_brdf(sv, m, prim, vn, nn, matte, rng.u(vslot + 0));
+                // integrateDirect, integrator.cpp:100-130
+                int nvis = 0;
+                for (int li = 0; li < sv.n_lights; ++li) nvis += light_visible(sv.lights[li], P, nn) ? 1 : 0;
+                if (nvis) {
+                    int pick = min((int)(rng.u(vslot + 1) * nvis), nvis - 1);
+                    int li = 0;
+                    for (int k = 0; li < sv.n_lights; ++li) {
+                        if (light_visible(sv.lights[li], P, nn)) {
+                            if (k == pick) break;
+                            ++k;
+                        }
+                    }
+                    const DLight &l = sv.lights[li];
+                    const float scale = (float)nvis;
+                    LightSample ls = light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
+                    float ct = dot3(nn, ls.dir);
+                    if (b.kind == BSDF_LAMBERT) {
+                        // integrateDirectLight, integrator.cpp:33-54
+                        if (ct >= 0.f && ls.pdf > 0.f) {
+                            f3 f = bsdf_eval(b, ls.dir, vn, nn);
+                            f3 c = div3(mul3(mul3v(ls.color, f), ct), ls.pdf);
+                            sh[0].valid = true; sh[0].o = P; sh[0].d = ls.dir; sh[0].tmin = AUR_RAY_BIAS; sh[0].tmax = ls.tmax;
+                            sh[0].c = mul3v(mul3(c, scale), T);
+                        }
+                    } else {
+                        // integrateDirectMIS, integrator.cpp:56-98
+                        if (ct > 0.f && ls.pdf > 0.f) {
+                            float bpdf = bsdf_pdf(b, ls.dir, vn, nn);
+                            if (bpdf > 0.f) {
+                                float wgt = power_heuristic(ls.pdf, bpdf);
+                                f3 f = bsdf_eval(b, ls.dir, vn, nn);
+                                f3 c = div3(mul3(mul3v(mul3(ls.color, wgt), f), ct), ls.pdf);
+                                sh[0].valid = true; sh[0].o = P; sh[0].d = ls.dir; sh[0].tmin = AUR_RAY_BIAS; sh[0].tmax = ls.tmax;
+                                sh[0].c = mul3v(mul3(c, scale), T);
+                            }
+                        }
+                        BsdfSample bs = bsdf_sample(b, vn, nn, rng.u(vslot + 4), rng.u(vslot + 5));
+                        float ct2 = dot3(nn, bs.dir);
+                        if (ct2 > 0.f) {
+                            float tmax = bs.tmax;
+                            float lpdf = light_pdf(l, P, bs.dir, AUR_RAY_BIAS, &tmax);
+                            if (lpdf > 0.f && bs.pdf > 0.f) {
+                                float wgt = power_heuristic(bs.pdf, lpdf);
+                                f3 le = mk3(l.Le[0], l.Le[1], l.Le[2]);
+                                f3 c = div3(mul3(mul3v(mul3(bs.color, wgt), le), ct2), bs.pdf);
+                                sh[1].valid = true; sh[1].o = P; sh[1].d = bs.dir; sh[1].tmin = AUR_RAY_BIAS; sh[1].tmax = tmax;
+                                sh[1].c = mul3v(mul3(c, scale), T);
+                            }
+                        }
+                    }
+                }
+                // continuation, integrator.cpp:208-257
+                BsdfSample s = bsdf_sample(b, vn, nn, rng.u(vslot + 6), rng.u(vslot + 7));
+                if (s.pdf <= 0.f) {
+                    finish = true;
+                } else {
+                    float ndl = dot3(nn, s.dir);
+                    T = mul3v(T, div3(mul3(s.color, ndl), s.pdf));
+                    const int bounces = (flags >> 16) & 0xff;
+                    if (bounces > sv.mindepth) cp *= lum3(s.color) * ndl / s.pdf;
+                    if (black3(T)) {
+                        finish = true;
+                    } else {
+                        int nrt = (b.kind == BSDF_CT) ? RT_SPECULAR : RT_DIFFUSE;
+                        bool killed = false;
+                        if (bounces > sv.mindepth) {
+                            if (rng.u(vslot + 8) > cp) killed = true;
+                            else T = div3(T, cp);
+                        }
+                        if (killed) {
+                            finish = true;
+                        } else {
+                            cont = true;
+                            ndir = s.dir;
+                            ntmax = s.tmax;
+                            flags = nrt | (matte ? F_MATTE : 0) | ((tb + 1) << 8) | ((bounces + 1) << 16);
+                            wv.T[path] = make_float4(T.x, T.y, T.z, cp);
+                        }
+                    }
+                }
+            }
+        }
+        wv.flags[path] = flags;
+        // compacted writes: next extension queue, shadow queue, finish list
+        int slot = wave_reserve(&wv.counts[nq], cont);
+        if (cont) {
+            wv.ray[nq][2 * slot] = make_float4(P.x, P.y, P.z, AUR_RAY_BIAS);
+            wv.ray[nq][2 * slot + 1] = make_float4(ndir.x, ndir.y, ndir.z, ntmax);
+            wv.qpath[nq][slot] = path;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            int ss = wave_reserve(&wv.counts[C_SHADOW], sh[k].valid);
+            if (sh[k].valid) {
+                wv.sray[3 * ss] = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].tmin);
+                wv.sray[3 * ss + 1] = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].tmax);
+                wv.sray[3 * ss + 2] = make_float4(sh[k].c.x, sh[k].c.y, sh[k].c.z, __int_as_float(path | (k ? (int)0x80000000 : 0)));
+            }
+        }
+        int fs = wave_reserve(&wv.counts[C_FINISH], finish);
+        if (finish) wv.finq[fs] = path;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// finish: firefly clamp (integrator.cpp:277-281), per-path output
+// ---------------------------------------------------------------------------
+__global__ void k_finish(WfView wv) {
+    const int n = wv.counts[C_FINISH];
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        int path = wv.finq[i];
+        float4 a = wv.La[path], b = wv.Lb[path];
+        f3 L = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
+        if (!(wv.flags[path] & F_NOCLAMP))
+            while (L.x > AUR_FIREFLY || L.y > AUR_FIREFLY || L.z > AUR_FIREFLY) L = mul3(L, 0.2f);
+        wv.out[path] = make_float4(L.x, L.y, L.z, a.w);
+    }
+}
+
+__global__ void k_clear_counts(int *counts, int q_done) {
+    // the consumed queue and the per-bounce shadow/finish lists restart at zero
+    if (threadIdx.x == 0) {
+        counts[q_done] = 0;
+        counts[C_SHADOW] = 0;
+        counts[C_FINISH] = 0;
+    }
+}
+
+__global__ void k_reset_all(int *counts, unsigned long long *stats, int reset_stats) {
+    int i = threadIdx.x;
+    if (i < 16) counts[i] = 0;
+    if (reset_stats && i < 8) stats[i] = 0;
+}
+
+// per-pixel sum over the passes of a batch, in pass order (deterministic)
+__global__ void k_resolve(const float4 *__restrict__ out, int wh, int n_passes, float4 *__restrict__ accum) {
+    int stride = gridDim.x * blockDim.x;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < wh; p += stride) {
+        float4 s = accum[p];
+        for (int k = 0; k < n_passes; ++k) {
+            float4 v = out[(size_t)k * wh + p];
+            s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+        }
+        accum[p] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+static SceneView scene_view(Ctx *c) {
+    SceneView sv;
+    sv.nodes = c->bvh.nodes.p;
+    sv.tris = c->bvh.tri_sorted.as<float4>();
+    sv.shade = c->d_shade.as<float4>();
+    sv.mats = c->d_mats.as<DMat>();
+    sv.lights = c->d_lights.as<DLight>();
+    sv.tables = c->d_tables.as<float>();
+    sv.albedo = c->tri_albedo.empty() ? nullptr : c->d_albedo.as<float>();
+    sv.n_lights = c->n_lights;
+    sv.has_env = c->n_env > 0;
+    sv.env[0] = c->env_le[0]; sv.env[1] = c->env_le[1]; sv.env[2] = c->env_le[2];
+    sv.fov = c->scene.opt.fov;
+    sv.width = c->scene.opt.width;
+    sv.height = c->scene.opt.height;
+    sv.maxdepth = c->scene.opt.maxdepth;
+    sv.mindepth = c->scene.opt.mindepth;
+    return sv;
+}
+
+static int persistent_blocks(Ctx *c, int block) {
+    int per_cu = block <= 128 ? 8 : 4;
+    return c->n_cu * per_cu;
+}
+
+static void ensure_wavefront(Ctx *c, size_t cap) {
+    Wavefront &w = c->wf;
+    int need = (c->bvh.width - 1) * (c->bvh.depth + 1) + 2;
+    size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 0;
+    if (cap <= w.cap && ovf <= w.stack_ovf_per_thread && w.counts.p) return;
+    cap = std::max(cap, w.cap);
+    w.cap = cap;
+    for (int k = 0; k < 2; ++k) {
+        w.ray[k].reserve(sizeof(float4) * 2 * cap);
+        w.qpath[k].reserve(sizeof(int) * cap);
+    }
+    w.hit.reserve(sizeof(float4) * cap);
+    w.T.reserve(sizeof(float4) * cap);
+    w.La.reserve(sizeof(float4) * cap);
+    w.Lb.reserve(sizeof(float4) * cap);
+    w.flags.reserve(sizeof(int) * cap);
+    w.sray.reserve(sizeof(float4) * 3 * 2 * cap);
+    w.finq.reserve(sizeof(int) * cap);
+    w.out.reserve(sizeof(float4) * cap);
+    w.counts.reserve(sizeof(int) * 16);
+    w.stats.reserve(sizeof(unsigned long long) * 8);
+    size_t threads = (size_t)persistent_blocks(c, 128) * 128;
+    w.stack_ovf_per_thread = std::max(ovf, w.stack_ovf_per_thread);
+    w.stack_ovf.reserve(sizeof(int) * threads * std::max<size_t>(w.stack_ovf_per_thread, 1));
+}
+
+static WfView wf_view(Ctx *c, const PathSource &src) {
+    Wavefront &w = c->wf;
+    WfView v;
+    for (int k = 0; k < 2; ++k) { v.ray[k] = w.ray[k].as<float4>(); v.qpath[k] = w.qpath[k].as<int>(); }
+    v.hit = w.hit.as<float4>();
+    v.T = w.T.as<float4>(); v.La = w.La.as<float4>(); v.Lb = w.Lb.as<float4>();
+    v.flags = w.flags.as<int>();
+    v.sray = w.sray.as<float4>();
+    v.finq = w.finq.as<int>();
+    v.out = w.out.as<float4>();
+    v.counts = w.counts.as<int>();
+    v.stats = w.stats.as<unsigned long long>();
+    v.ovf = w.stack_ovf.as<int>();
+    v.ovf_per_thread = (int)w.stack_ovf_per_thread;
+    v.src = src;
+    v.seed = c->seed;
+    v.nslots = 2 + AUR_SLOTS_PER_VERTEX * c->scene.opt.maxdepth;
+    return v;
+}
+
+// One batch of n paths: generate, then maxdepth+1 rounds of
+// extend/shade/shadow/finish.  Results land in wf.out (by path id).
+static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
+    const int gen_blocks = std::min((n + 255) / 256, persistent_blocks(c, 256));
+    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, sv, wv, n);
+    const int tb = persistent_blocks(c, 128), sb = persistent_blocks(c, 256);
+    int q = 0;
+    for (int it = 0; it <= sv.maxdepth; ++it) {
+        if (c->bvh.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q);
+        else hipLaunchKernelGGL(k_extend<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q);
+        hipLaunchKernelGGL(k_shade, dim3(sb), dim3(256), 0, c->stream, sv, wv, q);
+        if (c->bvh.width == 8) hipLaunchKernelGGL(k_shadow<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv);
+        else hipLaunchKernelGGL(k_shadow<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv);
+        hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, c->stream, wv);
+        hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, c->stream, wv.counts, q);
+        q = 1 - q;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats) {
+    if (!c->built) throw StateError("render before aur_build_accel");
+    const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
+    size_t cap = src.mode == 0 ? std::max<size_t>(wh, std::min<size_t>(n_paths, (size_t)8 << 20)) : std::min<size_t>(n_paths, (size_t)8 << 20);
+    if (src.mode == 0) cap = std::max<size_t>(wh, (cap / wh) * wh);
+    if (cap > (size_t)0x7fffffff) throw CapacityError("batch too large");
+    ensure_wavefront(c, std::max<size_t>(cap, 1));
+    SceneView sv = scene_view(c);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, c->stream));
+    hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, c->wf.counts.as<int>(),
+                       c->wf.stats.as<unsigned long long>(), 1);
+    size_t done = 0;
+    while (done < n_paths) {
+        size_t n = std::min(cap, n_paths - done);
+        PathSource s = src;
+        if (src.mode == 0) {
+            s.first_pass = src.first_pass + (int)(done / wh);
+        } else {
+            s.pixel = src.pixel + done;
+            s.sample = src.sample + done;
+            s.slots = src.slots ? src.slots + done * (size_t)(2 + AUR_SLOTS_PER_VERTEX * c->scene.opt.maxdepth) : nullptr;
+        }
+        WfView wv = wf_view(c, s);
+        hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, wv.counts, wv.stats, 0);
+        run_batch(c, sv, wv, (int)n);
+        if (src.mode == 0) {
+            int passes = (int)(n / wh);
+            int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));
+            hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh, passes, out_rgba);
+        } else {
+            HIP_CHECK(hipMemcpyAsync(out_rgba + done, wv.out, sizeof(float4) * n, hipMemcpyDeviceToDevice, c->stream));
+        }
+        done += n;
+    }
+    HIP_CHECK(hipEventRecord(e1, c->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (stats) {
+        unsigned long long s[8];
+        HIP_CHECK(hipMemcpy(s, c->wf.stats.p, sizeof s, hipMemcpyDeviceToHost));
+        stats->paths = (int64_t)s[S_PATHS];
+        stats->ext_rays = (int64_t)s[S_EXT];
+        stats->hit_rays = (int64_t)s[S_HIT];
+        stats->shadow_rays = (int64_t)s[S_SHADOW];
+        stats->ms = ms;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// aur_trace: batch closest-hit / any-hit queries for the parity tests
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(128) k_trace(SceneView sv, const aur_ray *__restrict__ rays, int n, aur_hit *__restrict__ hits,
+                                               int any_hit, int *ovf, int ovf_per_thread) {
+    __shared__ int s_stack[AUR_STACK_LDS * 128];
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = 128;
+    st.ovf = ovf + (size_t)gtid * ovf_per_thread;
+    const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
+    for (int i = gtid; i < n; i += gridDim.x * blockDim.x) {
+        aur_ray r = rays[i];
+        TravHit h;
+        if (any_hit) traverse<W, true>(nodes, sv.tris, mk3(r.o[0], r.o[1], r.o[2]), mk3(r.d[0], r.d[1], r.d[2]), r.tmin, r.tmax, st, h);
+        else traverse<W, false>(nodes, sv.tris, mk3(r.o[0], r.o[1], r.o[2]), mk3(r.d[0], r.d[1], r.d[2]), r.tmin, r.tmax, st, h);
+        aur_hit o;
+        o.t = h.t; o.u = h.u; o.v = h.v; o.prim = h.prim;
+        hits[i] = o;
+    }
+}
+
+void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_hit) {
+    if (!c->built) throw StateError("trace before aur_build_accel");
+    ensure_wavefront(c, 1);
+    SceneView sv = scene_view(c);
+    int blocks = std::min((n + 127) / 128, persistent_blocks(c, 128));
+    if (blocks < 1) return;
+    if (c->bvh.width == 8)
+        hipLaunchKernelGGL(k_trace<8>, dim3(blocks), dim3(128), 0, c->stream, sv, d_rays, n, d_hits, any_hit,
+                           c->wf.stack_ovf.as<int>(), (int)c->wf.stack_ovf_per_thread);
+    else
+        hipLaunchKernelGGL(k_trace<4>, dim3(blocks), dim3(128), 0, c->stream, sv, d_rays, n, d_hits, any_hit,
+                           c->wf.stack_ovf.as<int>(), (int)c->wf.stack_ovf_per_thread);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace aur

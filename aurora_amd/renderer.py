@@ -1,0 +1,167 @@
+"""Host-side mirror of Aurora's ``Renderer`` (src/frontend/renderer.cpp) over the C ABI.
+
+``Renderer(scene_file).render()`` follows the reference's sequence
+(renderer.cpp:39-63): parse the scene description, build the render
+environment (acceleration structure), render ``pixelsamples`` passes and
+return the image.  The per-pass running mean of ``OpenexrDisplay::appendValue``
+(openexrDisplay.cpp:61-72) is kept as an FP32 sum divided by the pass count.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Renderer:
+    """One GPU rendering context (``aur_ctx``)."""
+
+    def __init__(self, scene_file: Optional[str] = None, config: Optional[str] = None, device: int = 0):
+        self.lib = L.load_library()
+        self._ctx = C.c_void_p()
+        L.check(self.lib.aur_create(C.byref(self._ctx), device))
+        self.passes_done = 0
+        if scene_file is not None:
+            self.load(scene_file, config)
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self) -> None:
+        if self._ctx:
+            self.lib.aur_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- scene ---------------------------------------------------------------
+    def load(self, scene_file: str, config: Optional[str] = None) -> None:
+        """JsonParser::parseScene + ObjTriangleMesh (jsonParser.cpp:30-59)."""
+        L.check(self.lib.aur_scene_load(self._ctx, scene_file.encode(), config.encode() if config else None))
+        self.passes_done = 0
+
+    def upload(self, scene: dict) -> None:
+        """Uploads an already-flattened camera-space scene (see ``export``)."""
+        self._keep = keep = {}
+        verts = keep["verts"] = np.ascontiguousarray(scene["verts"], dtype=np.float32).reshape(-1)
+        tris = keep["tris"] = np.ascontiguousarray(scene["tris"], dtype=np.int32).reshape(-1)
+        normals = keep["normals"] = np.ascontiguousarray(scene["normals"], dtype=np.float32).reshape(-1)
+        attr = keep["attr"] = np.ascontiguousarray(scene["attr"], dtype=np.int32).reshape(-1)
+        mats = keep["mats"] = (L.Material * max(len(scene["materials"]), 1))(*scene["materials"])
+        lights = keep["lights"] = (L.Light * max(len(scene["lights"]), 1))(*scene["lights"])
+        d = L.SceneDesc()
+        d.n_verts = verts.size // 3
+        d.n_tris = tris.size // 3
+        d.verts, d.tris, d.normals, d.attr = L.fptr(verts), L.iptr(tris), L.fptr(normals), L.iptr(attr)
+        d.n_attrs = len(scene["materials"])
+        d.mats = C.cast(mats, C.POINTER(L.Material))
+        d.n_lights = len(scene["lights"])
+        d.lights = C.cast(lights, C.POINTER(L.Light))
+        d.fov = scene["fov"]
+        d.width, d.height = scene["width"], scene["height"]
+        d.pixelsamples = scene.get("pixelsamples", 1)
+        d.lightsamples = scene.get("lightsamples", 1)
+        d.mindepth, d.maxdepth = scene["mindepth"], scene["maxdepth"]
+        L.check(self.lib.aur_scene_upload(self._ctx, C.byref(d)))
+        self.passes_done = 0
+
+    def build(self, width: int = 4, max_leaf: int = 4) -> None:
+        """Loop subdivision, flattening and the wide BVH, on the GPU."""
+        o = L.AccelOpts(width, max_leaf)
+        L.check(self.lib.aur_build_accel(self._ctx, C.byref(o)))
+
+    def set_seed(self, seed: int) -> None:
+        L.check(self.lib.aur_set_seed(self._ctx, seed & 0xFFFFFFFF))
+
+    def set_tri_albedo(self, albedo: Optional[np.ndarray]) -> None:
+        if albedo is None:
+            L.check(self.lib.aur_set_tri_albedo(self._ctx, None))
+            return
+        self._albedo = np.ascontiguousarray(albedo, dtype=np.float32).reshape(-1)
+        L.check(self.lib.aur_set_tri_albedo(self._ctx, L.fptr(self._albedo)))
+
+    def info(self) -> dict:
+        i = L.SceneInfo()
+        L.check(self.lib.aur_scene_info_get(self._ctx, C.byref(i)))
+        return {n: getattr(i, n) for n, _ in L.SceneInfo._fields_}
+
+    def export(self) -> dict:
+        """The flattened camera-space scene as numpy arrays + structs."""
+        i = self.info()
+        nv, nt = i["n_verts"], i["n_tris"]
+        verts = np.zeros(3 * nv, np.float32)
+        tris = np.zeros(3 * nt, np.int32)
+        normals = np.zeros(9 * nt, np.float32)
+        attr = np.zeros(nt, np.int32)
+        mats = (L.Material * max(i["n_attrs"], 1))()
+        lights = (L.Light * max(i["n_lights"], 1))()
+        L.check(self.lib.aur_scene_export(self._ctx, L.fptr(verts), L.iptr(tris), L.fptr(normals), L.iptr(attr),
+                                          mats, lights))
+        return {"verts": verts.reshape(-1, 3), "tris": tris.reshape(-1, 3), "normals": normals.reshape(-1, 9),
+                "attr": attr, "materials": [mats[k] for k in range(i["n_attrs"])],
+                "lights": [lights[k] for k in range(i["n_lights"])], "fov": i["fov"], "width": i["width"],
+                "height": i["height"], "maxdepth": i["maxdepth"], "mindepth": i["mindepth"],
+                "pixelsamples": i["pixelsamples"], "lightsamples": i["lightsamples"]}
+
+    # -- rendering -----------------------------------------------------------
+    def render_passes(self, first_pass: int, n_passes: int, fetch: bool = True):
+        """Renders passes into the device accumulation (sum of per-pass RGBA)."""
+        i = self.info()
+        out = np.zeros((i["height"], i["width"], 4), np.float32) if fetch else None
+        st = L.Stats()
+        L.check(self.lib.aur_render(self._ctx, first_pass, n_passes, L.fptr(out) if fetch else None, C.byref(st)))
+        self.passes_done = first_pass + n_passes
+        return out, st.as_dict()
+
+    def render(self, n_passes: Optional[int] = None) -> np.ndarray:
+        """Renderer::renderImage: all passes, returns the mean image (H, W, 4)."""
+        n = n_passes if n_passes is not None else self.info()["pixelsamples"]
+        acc, self.last_stats = self.render_passes(0, n)
+        return acc / max(n, 1)
+
+    def render_paths(self, pixel, sample, slots=None):
+        """Per-path RGBA for explicit (pixel, sample) pairs; optional injected uniforms."""
+        pixel = np.ascontiguousarray(pixel, dtype=np.int32)
+        sample = np.ascontiguousarray(sample, dtype=np.int32)
+        n = pixel.size
+        out = np.zeros((n, 4), np.float32)
+        sl = None
+        if slots is not None:
+            sl = np.ascontiguousarray(slots, dtype=np.float32)
+        st = L.Stats()
+        L.check(self.lib.aur_render_paths(self._ctx, L.iptr(pixel), L.iptr(sample), n,
+                                          L.fptr(sl) if sl is not None else None, L.fptr(out), C.byref(st)))
+        return out, st.as_dict()
+
+    # -- parity queries ------------------------------------------------------
+    def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+        """rays: (n, 8) float32 rows (ox, oy, oz, tmin, dx, dy, dz, tmax).
+        Returns a structured array with t, u, v, prim."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros(n, dtype=[("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim", np.int32)])
+        L.check(self.lib.aur_trace(self._ctx, rays.ctypes.data_as(C.POINTER(L.Ray)), n,
+                                   hits.ctypes.data_as(C.POINTER(L.Hit)), int(any_hit)))
+        return hits
+
+    def bsdf(self, queries) -> list:
+        arr = (L.BsdfQuery * len(queries))(*queries)
+        res = (L.BsdfResult * len(queries))()
+        L.check(self.lib.aur_bsdf(self._ctx, arr, len(queries), res))
+        return [res[k] for k in range(len(queries))]
+
+    def kelemen_table(self, reflectance: float, distribution: int, n_samples: int, seed: int) -> np.ndarray:
+        out = np.zeros(64 * 32, np.float32)
+        L.check(self.lib.aur_kelemen_table(self._ctx, reflectance, distribution, n_samples, seed, L.fptr(out)))
+        return out.reshape(64, 32)

@@ -1,0 +1,292 @@
+"""Deterministic scene generators for the benchmark workloads.
+
+Every scene is written in Aurora's own input format: a JSON scene file (the
+PyAurora export, parsed by the reference's ``JsonParser::parseScene``,
+src/translator/jsonParser.cpp:30-59) next to Wavefront OBJ meshes (parsed by
+``ObjParser::run``, src/geometry/objParser.cpp:58-202) and a ``config.json`` of
+path templates (Aurora/config.json:1-7).
+
+Build extensions use keys the reference ignores ("TODO: check for
+invalid/unknown keys", jsonParser.cpp:47):
+  * material ``"distribution": "ggx"`` selects the GGX microfacet distribution
+    for Cook-Torrance lobes (Beckmann otherwise);
+  * geometry ``"displace": {amplitude, octaves, frequency, seed}`` applies
+    radial value-noise displacement after Loop subdivision.
+
+Random numbers come from SplitMix64 with an explicit ``(x >> 40) * 2**-24``
+float conversion, so the scenes are identical on every platform.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+class SplitMix64:
+    """SplitMix64 stream; ``uniform()`` returns floats in [0, 1)."""
+
+    def __init__(self, seed: int):
+        self.state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 40) * (1.0 / 16777216.0))
+
+    def log_uniform(self, lo: float, hi: float) -> float:
+        return math.exp(self.uniform(math.log(lo), math.log(hi)))
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1)) % (hi - lo + 1)
+
+
+# --------------------------------------------------------------------------
+# OBJ writers
+# --------------------------------------------------------------------------
+
+def write_obj(path: str, verts, faces, normals=None) -> None:
+    """Writes an OBJ with ``v``/``vn``/``f v//n`` lines (1-based indices).
+
+    ``normals`` are per vertex (same count as ``verts``); when omitted each
+    face gets its geometric normal, which ``ObjTriangleMesh`` copies per
+    corner (objTriangleMesh.cpp:228-242).
+    """
+    verts = np.asarray(verts, dtype=np.float64)
+    faces = np.asarray(faces, dtype=np.int64)
+    lines = ["# generated by aurora_amd.scenes"]
+    lines += ["v %.9g %.9g %.9g" % tuple(v) for v in verts]
+    if normals is None:
+        fn = []
+        for f in faces:
+            a, b, c = verts[f[0]], verts[f[1]], verts[f[2]]
+            n = np.cross(b - a, c - a)
+            n = n / (np.linalg.norm(n) or 1.0)
+            fn.append(n)
+        lines += ["vn %.9g %.9g %.9g" % tuple(n) for n in fn]
+        lines += ["f %d//%d %d//%d %d//%d" % (f[0] + 1, i + 1, f[1] + 1, i + 1, f[2] + 1, i + 1)
+                  for i, f in enumerate(faces)]
+    else:
+        normals = np.asarray(normals, dtype=np.float64)
+        lines += ["vn %.9g %.9g %.9g" % tuple(n) for n in normals]
+        lines += ["f %d//%d %d//%d %d//%d" % (f[0] + 1, f[0] + 1, f[1] + 1, f[1] + 1, f[2] + 1, f[2] + 1)
+                  for f in faces]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3)."""
+    return [p0, p1, p2, p3], [[0, 1, 2], [0, 2, 3]]
+
+
+def box_mesh(lo, hi):
+    """Axis-aligned box, 12 triangles, outward winding."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    v = [[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+         [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]]
+    f = [[0, 3, 2], [0, 2, 1],   # -z
+         [4, 5, 6], [4, 6, 7],   # +z
+         [0, 4, 7], [0, 7, 3],   # -x
+         [1, 2, 6], [1, 6, 5],   # +x
+         [0, 1, 5], [0, 5, 4],   # -y
+         [3, 7, 6], [3, 6, 2]]   # +y
+    return v, f
+
+
+def icosahedron():
+    """Unit-radius icosahedron: 12 vertices, 20 faces, outward winding."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+         [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+         [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+         [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    return v, np.array(f, dtype=np.int64)
+
+
+# --------------------------------------------------------------------------
+# scene assembly
+# --------------------------------------------------------------------------
+
+@dataclass
+class SceneSpec:
+    """In-memory form of one scene file (the PyAurora ``Scene`` contents)."""
+    options: dict
+    camera: dict
+    materials: dict = field(default_factory=dict)
+    objects: dict = field(default_factory=dict)
+    shaders: dict = field(default_factory=dict)
+    meshes: dict = field(default_factory=dict)  # obj file name -> (verts, faces, normals|None)
+
+    def write(self, out_dir: str, scene_name: str = "scene.asc") -> str:
+        os.makedirs(out_dir, exist_ok=True)
+        mesh_dir = os.path.join(out_dir, "meshes")
+        os.makedirs(mesh_dir, exist_ok=True)
+        for name, (v, f, n) in self.meshes.items():
+            write_obj(os.path.join(mesh_dir, name), v, f, n)
+        cfg = {"basepath": os.path.abspath(out_dir) + "/", "cachepath": "$BASE/cache/",
+               "renderpath": "$BASE/render/", "texturepath": "$BASE/textures",
+               "hdripath": "$BASE/hdri"}
+        with open(os.path.join(out_dir, "config.json"), "w") as fh:
+            json.dump(cfg, fh, indent=4)
+        root = {"options": self.options, "renderCam": self.camera, "materials": self.materials,
+                "objects": self.objects, "shaders": self.shaders}
+        path = os.path.join(out_dir, scene_name)
+        with open(path, "w") as fh:
+            json.dump(root, fh, indent=4)
+        return path
+
+
+def _geo(name, obj, material, transforms=None, subdlevels=0, **extra):
+    d = {"type": "geometry", "name": name, "path": "$BASE/meshes/" + obj, "material": material,
+         "transforms": transforms or [], "subdlevels": int(subdlevels)}
+    d.update(extra)
+    return d
+
+
+def _square_light(name, color, sx, sy, transforms, exposure=0.0):
+    return {"type": "light", "name": name, "scaleX": float(sx), "scaleY": float(sy),
+            "exposure": float(exposure), "color": [float(c) for c in color], "transforms": transforms}
+
+
+def _options(w, h, spp, maxdepth, mindepth, filename="render.exr"):
+    return {"resolution": [int(w), int(h)], "pixelsamples": int(spp), "lightsamples": 1,
+            "mindepth": int(mindepth), "maxdepth": int(maxdepth), "filename": "$RENDER/" + filename}
+
+
+def box_room(res=256, spp=16, maxdepth=5, mindepth=5, kelemen=False, seed=1) -> SceneSpec:
+    """Closed box room (the 256x256, 16 spp, depth-5 CPU-runnable workload).
+
+    Floor, ceiling, back, left and right walls (10 triangles), a matte box and
+    a GGX Cook-Torrance box (24 triangles), one ceiling quad light of radiance
+    (17, 12, 4); pinhole camera with a 40 degree horizontal field of view
+    (``fov`` = tan(20 deg), camera.cpp:36-53).  With ``kelemen`` every
+    material is a Kelemen coat (the inverse-rendering variant).
+    """
+    rng = SplitMix64(seed)
+    s = SceneSpec(options=_options(res, res, spp, maxdepth, mindepth),
+                  camera={"fov": math.tan(math.radians(20.0)),
+                          "transforms": [{"translate": [0.0, 1.0, 3.5]}]})
+    x0, x1, y0, y1, z0, z1 = -1.0, 1.0, 0.0, 2.0, -1.0, 1.0
+    fv, ff = [], []
+    for q in (([x0, y0, z1], [x1, y0, z1], [x1, y0, z0], [x0, y0, z0]),     # floor
+              ([x0, y1, z0], [x1, y1, z0], [x1, y1, z1], [x0, y1, z1]),     # ceiling
+              ([x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0])):    # back
+        v, f = quad(*q)
+        ff += [[a + len(fv) for a in t] for t in f]
+        fv += v
+    s.meshes["walls_white.obj"] = (fv, ff, None)
+    v, f = quad([x0, y0, z1], [x0, y0, z0], [x0, y1, z0], [x0, y1, z1])
+    s.meshes["wall_left.obj"] = (v, f, None)
+    v, f = quad([x1, y0, z0], [x1, y0, z1], [x1, y1, z1], [x1, y1, z0])
+    s.meshes["wall_right.obj"] = (v, f, None)
+    v, f = box_mesh([-0.65, 0.0, -0.6], [-0.05, 1.2, 0.0])
+    s.meshes["box_tall.obj"] = (v, f, None)
+    v, f = box_mesh([0.1, 0.0, -0.1], [0.7, 0.6, 0.5])
+    s.meshes["box_short.obj"] = (v, f, None)
+
+    if kelemen:
+        def kel(name):
+            s.materials[name] = {"type": "kelemenMaterial", "diffusecolor": [0.5, 0.5, 0.5],
+                                 "speccolor": [0.5, 0.5, 0.5], "reflectance": 0.04,
+                                 "roughness": 0.4, "distribution": "ggx"}
+        for n in ("white", "red", "green", "boxmatte", "boxspec"):
+            kel(n)
+    else:
+        s.materials["white"] = {"type": "matteMaterial", "color": [0.73, 0.73, 0.73]}
+        s.materials["red"] = {"type": "matteMaterial", "color": [0.63, 0.12, 0.05]}
+        s.materials["green"] = {"type": "matteMaterial", "color": [0.14, 0.45, 0.09]}
+        s.materials["boxmatte"] = {"type": "matteMaterial", "color": [0.73, 0.73, 0.73]}
+        s.materials["boxspec"] = {"type": "metalMaterial", "reflectance": [0.04, 0.04, 0.04],
+                                  "roughnessA": 0.3, "roughnessB": 0.3, "mix": 0.5,
+                                  "gain": [1.0, 1.0, 1.0], "distribution": "ggx"}
+    s.objects["walls_white"] = _geo("walls_white", "walls_white.obj", "white")
+    s.objects["wall_left"] = _geo("wall_left", "wall_left.obj", "red")
+    s.objects["wall_right"] = _geo("wall_right", "wall_right.obj", "green")
+    s.objects["box_tall"] = _geo("box_tall", "box_tall.obj", "boxmatte")
+    s.objects["box_short"] = _geo("box_short", "box_short.obj", "boxspec")
+    # object-space light normal is -z (squareLight.cpp:27-39): rotateX(-90) faces it down
+    s.objects["light_ceiling"] = _square_light("light_ceiling", [17.0, 12.0, 4.0], 0.25, 0.25,
+                                               [{"translate": [0.0, 1.98, 0.0]}, {"rotateX": -90.0}])
+    del rng
+    return s
+
+
+def displaced_spheres(width=1920, height=1080, spp=64, maxdepth=8, mindepth=8, n_spheres=8,
+                      levels=7, seed=2, ground=20.0) -> SceneSpec:
+    """Large single scene: displaced Loop-subdivided icospheres on a ground quad.
+
+    Eight unit icospheres, Loop-subdivided ``levels`` times (327,680 triangles
+    each at 7 levels), displaced radially by 3-octave value noise (amplitude
+    0.05 * radius), Kelemen-coated with GGX roughness ~U[0.05, 0.8] and
+    albedo ~U[0.1, 0.9]^3, on a 20x20 ground quad, lit by 4 quad lights of
+    grey radiance ~U[5, 20] plus a constant environment of 0.1.
+    """
+    rng = SplitMix64(seed)
+    s = SceneSpec(options=_options(width, height, spp, maxdepth, mindepth),
+                  camera={"fov": 0.6,
+                          "transforms": [{"translate": [0.0, 7.0, 17.0]}, {"rotateX": -22.0}]})
+    iv, iff = icosahedron()
+    s.meshes["icosahedron.obj"] = (iv, iff, iv)
+    h = ground / 2.0
+    v, f = quad([-h, 0.0, h], [h, 0.0, h], [h, 0.0, -h], [-h, 0.0, -h])
+    s.meshes["ground.obj"] = (v, f, None)
+    s.materials["ground"] = {"type": "matteMaterial", "color": [0.5, 0.5, 0.5]}
+    s.objects["ground"] = _geo("ground", "ground.obj", "ground")
+    placed = []
+    for i in range(n_spheres):
+        for _ in range(1000):
+            x, z = rng.uniform(-h + 2, h - 2), rng.uniform(-h + 2, h - 2)
+            if all((x - a) ** 2 + (z - b) ** 2 > (2.3) ** 2 for a, b in placed):
+                break
+        placed.append((x, z))
+        rough = rng.uniform(0.05, 0.8)
+        alb = [rng.uniform(0.1, 0.9) for _ in range(3)]
+        mname = "sphere%02d_mat" % i
+        s.materials[mname] = {"type": "kelemenMaterial", "diffusecolor": alb,
+                              "speccolor": [1.0, 1.0, 1.0], "reflectance": 0.04,
+                              "roughness": rough, "distribution": "ggx"}
+        name = "sphere%02d" % i
+        s.objects[name] = _geo(name, "icosahedron.obj", mname, [{"translate": [x, 1.0, z]}],
+                               subdlevels=levels,
+                               displace={"amplitude": 0.05, "octaves": 3, "frequency": 2.0,
+                                         "seed": int(rng.next_u64() & 0x7FFFFFFF)})
+    for i in range(4):
+        e = rng.uniform(5.0, 20.0)
+        x, z = rng.uniform(-6, 6), rng.uniform(-6, 6)
+        s.objects["light%d" % i] = _square_light("light%d" % i, [e, e, e], 0.75, 0.75,
+                                                 [{"translate": [x, 6.0, z]}, {"rotateX": -90.0}])
+    s.objects["env"] = {"type": "envlight", "name": "env", "color": [0.1, 0.1, 0.1],
+                        "exposure": 0.0, "envmap": "", "transforms": []}
+    return s
+
+
+def write_config(name: str, out_dir: str, **kw) -> str:
+    """Writes a named workload; returns the scene file path."""
+    if name in ("box_room", "c1"):
+        spec = box_room(**kw)
+    elif name in ("box_room_kelemen", "c4"):
+        kw.setdefault("res", 512)
+        kw.setdefault("spp", 32)
+        kw.setdefault("maxdepth", 6)
+        kw.setdefault("mindepth", 6)
+        spec = box_room(kelemen=True, **kw)
+    elif name in ("displaced_spheres", "c2"):
+        spec = displaced_spheres(**kw)
+    else:
+        raise ValueError("unknown workload %r" % name)
+    return spec.write(out_dir)

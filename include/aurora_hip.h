@@ -1,0 +1,190 @@
+/*
+ * aurora_hip.h -- C ABI of the MI355X path tracer (libaurora_hip.so).
+ *
+ * The library replaces Aurora's CPU render loop: the TBB pixel loop
+ * (`Renderer::renderImage` / `IntegrateParallel`, src/frontend/renderer.cpp:147-271)
+ * and everything it calls per pixel -- `Integrator::integrateCameraSample`
+ * (src/frontend/integrator.cpp:152-300), the BRDFs, the lights, and Embree's
+ * `intersect` / `intersectBinary` (src/geometry/embreeAccelerator.cpp:41-100).
+ * It keeps the reference's scene input (the PyAurora JSON scene plus OBJ meshes
+ * and config.json).
+ *
+ * Conventions:
+ *   - every call returns 0 on success or a negative AUR_E* code; the message of
+ *     the last failure on the calling thread is returned by aur_last_error();
+ *     no C++ exception or abort crosses this boundary;
+ *   - host buffers belong to the caller; device memory belongs to the library;
+ *   - a context is used by one host thread at a time (it is not locked).
+ */
+#ifndef AURORA_HIP_H
+#define AURORA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AUR_OK 0
+#define AUR_EINVAL (-1)   /* bad argument */
+#define AUR_EIO (-2)      /* file missing or unreadable */
+#define AUR_EPARSE (-3)   /* malformed scene / OBJ / config */
+#define AUR_EHIP (-4)     /* HIP runtime failure */
+#define AUR_ESTATE (-5)   /* call out of order (e.g. render before build) */
+#define AUR_ECAPACITY (-6) /* a size limit of the library was exceeded */
+
+typedef struct aur_ctx aur_ctx;
+
+/* A ray: origin, [tmin, tmax] interval, direction (32 bytes).  The fields of
+ * `Ray` (src/core/ray.h:17-34). */
+typedef struct {
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} aur_ray;
+
+/* A hit: distance, barycentrics (u weights vertex 1, v weights vertex 2, as
+ * embreeAccelerator.cpp:113-129), global triangle id or -1 on a miss. */
+typedef struct {
+    float t, u, v;
+    int prim;
+} aur_hit;
+
+/* Material kinds (src/shading/materials/*Material.cpp). */
+#define AUR_MAT_MATTE 0
+#define AUR_MAT_METAL 1
+#define AUR_MAT_KELEMEN 2
+#define AUR_MAT_LIGHT 3 /* ConstantMaterial on light triangles (renderer.cpp:131-133) */
+
+#define AUR_DIST_BECKMANN 0
+#define AUR_DIST_GGX 1
+
+#define AUR_LIGHT_AREA 0 /* SquareLight (src/lights/squareLight.*) */
+#define AUR_LIGHT_ENV 1  /* InfiniteAreaLight, constant branch (src/geometry/infiniteAreaLight.*) */
+
+typedef struct {
+    int type;          /* AUR_MAT_* */
+    int distribution;  /* AUR_DIST_* (Cook-Torrance lobes) */
+    float diffuse[3];  /* matte color / kelemen diffusecolor */
+    float spec[3];     /* kelemen speccolor / metal reflectance */
+    float gain[3];     /* metal gain */
+    float roughness;   /* kelemen roughness / metal roughnessA */
+    float roughnessB;  /* metal roughnessB */
+    float mix;         /* metal mix */
+    float reflectance; /* kelemen reflectance (selects the albedo table) */
+    float emission[3]; /* light triangles: color * 2^exposure */
+} aur_material;
+
+typedef struct {
+    int type;        /* AUR_LIGHT_* */
+    float m[16];     /* area light: object-to-camera matrix, row major */
+    float sx, sy;    /* area light: scaleX, scaleY */
+    float color[3];
+    float exposure;
+} aur_light;
+
+/* Flattened, camera-space scene: what `EmbreeMesh::preRender`
+ * (src/geometry/embreeMesh.h:30-58) hands to Embree plus the per-attribute
+ * materials, the lights and the render options. */
+typedef struct {
+    int n_verts, n_tris;
+    const float *verts;   /* 3 * n_verts */
+    const int *tris;      /* 3 * n_tris */
+    const float *normals; /* 9 * n_tris: per-corner normals */
+    const int *attr;      /* n_tris: attribute index (objects, then lights) */
+    int n_attrs;
+    const aur_material *mats; /* n_attrs */
+    int n_lights;
+    const aur_light *lights;
+    float fov;
+    int width, height;
+    int pixelsamples, lightsamples, mindepth, maxdepth;
+} aur_scene_desc;
+
+typedef struct {
+    int n_verts, n_tris, n_attrs, n_lights, n_objects;
+    int width, height, pixelsamples, lightsamples, mindepth, maxdepth;
+    float fov;
+    int bvh_width, bvh_nodes, bvh_depth, bvh_max_leaf;
+} aur_scene_info;
+
+typedef struct {
+    int width;      /* BVH branching factor: 4 or 8 (0 = default 4) */
+    int max_leaf;   /* triangles per leaf, 1..16 (0 = default 4) */
+} aur_accel_opts;
+
+/* Counters, counted as the reference counts rays (integrator.cpp:44,71,88,162,260). */
+typedef struct {
+    int64_t paths;       /* camera paths started */
+    int64_t ext_rays;    /* closest-hit rays traced (camera + extension) */
+    int64_t hit_rays;    /* closest-hit rays that hit */
+    int64_t shadow_rays; /* any-hit rays traced */
+    double ms;           /* wall time of the call on the device stream */
+} aur_stats;
+
+/* One BSDF query: kind (0 Lambert, 1 Cook-Torrance), params, the frame and two
+ * uniforms for getSample. */
+typedef struct {
+    int kind;
+    int distribution;
+    float color[3];
+    float roughness;
+    float V[3], N[3], L[3];
+    float u1, u2;
+} aur_bsdf_query;
+
+typedef struct {
+    float f[3];           /* evalSampleWorld(L, V, N) */
+    float pdf;            /* pdf(L, V, N) */
+    float sample_dir[3];  /* getSample(V, N) direction */
+    float sample_pdf;
+    float sample_color[3];
+} aur_bsdf_result;
+
+/* ---- lifetime / errors ---- */
+int aur_create(aur_ctx **out, int hip_device);
+void aur_destroy(aur_ctx *ctx);
+const char *aur_last_error(void);
+/* hip_stream: a hipStream_t, or NULL for the library's own stream. */
+int aur_set_stream(aur_ctx *ctx, void *hip_stream);
+int aur_set_seed(aur_ctx *ctx, uint32_t seed);
+
+/* ---- scene ---- */
+/* Replaces JsonParser::parseScene (jsonParser.cpp:30-59) + ObjTriangleMesh
+ * (objTriangleMesh.cpp:26-92).  config_json_path may be NULL. */
+int aur_scene_load(aur_ctx *ctx, const char *scene_path, const char *config_json_path);
+/* Replaces the flattened scene wholesale (already camera space). */
+int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *desc);
+/* Per-triangle diffuse albedo override (3 * n_tris floats) or NULL to clear. */
+int aur_set_tri_albedo(aur_ctx *ctx, const float *albedo_rgb);
+/* Loop subdivision, flattening and the wide-BVH build, all on the GPU.
+ * Replaces EmbreeAccelerator's constructor (embreeAccelerator.cpp:19-35). */
+int aur_build_accel(aur_ctx *ctx, const aur_accel_opts *opts);
+int aur_scene_info_get(aur_ctx *ctx, aur_scene_info *out);
+/* Copies the flattened scene to host buffers (any pointer may be NULL). */
+int aur_scene_export(aur_ctx *ctx, float *verts, int *tris, float *normals, int *attr,
+                     aur_material *mats, aur_light *lights);
+
+/* ---- rendering ---- */
+/* Renders passes [first_pass, first_pass + n_passes) into the device
+ * accumulation buffer (sum of per-pass RGBA).  rgba_sum_host (4*W*H floats)
+ * receives a copy of the sum when non-NULL.  first_pass == 0 clears it. */
+int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host, aur_stats *out);
+/* Paths given by (pixel index, sample index); rgba_out receives 4 floats per
+ * path.  slots (n_paths * (2 + 9*maxdepth) floats) injects every random number
+ * of every path; NULL uses the counter-based hash of (seed, pixel, sample). */
+int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_paths,
+                     const float *slots, float *rgba_out, aur_stats *out);
+
+/* ---- parity queries ---- */
+int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int any_hit);
+int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r);
+/* Kelemen albedo table (64 cos x 32 roughness), kelemenMaterial.cpp:113-178. */
+int aur_kelemen_table(aur_ctx *ctx, float reflectance, int distribution, int n_samples,
+                      uint32_t seed, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

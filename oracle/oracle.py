@@ -1,0 +1,236 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, never by the product.  ``build()`` compiles the
+C restatement with the Makefile next to this file.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+KELEMEN_SAMPLES = 65536
+KELEMEN_SEED = 0x4B454C
+
+
+def build() -> str:
+    r = subprocess.run(["make", "-s", "-C", HERE], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    return LIB_PATH
+
+
+class OMaterial(C.Structure):
+    _fields_ = [("type", C.c_int), ("distribution", C.c_int), ("diffuse", C.c_float * 3), ("spec", C.c_float * 3),
+                ("gain", C.c_float * 3), ("roughness", C.c_float), ("roughnessB", C.c_float), ("mix", C.c_float),
+                ("reflectance", C.c_float), ("emission", C.c_float * 3), ("table", C.POINTER(C.c_float))]
+
+
+class OLight(C.Structure):
+    _fields_ = [("type", C.c_int), ("m", C.c_float * 16), ("sx", C.c_float), ("sy", C.c_float),
+                ("color", C.c_float * 3), ("exposure", C.c_float), ("intensity", C.c_float), ("N", C.c_float * 3),
+                ("pcam", (C.c_float * 3) * 4), ("tri", (C.c_float * 9) * 2)]
+
+
+class OScene(C.Structure):
+    _fields_ = [("n_verts", C.c_int), ("n_tris", C.c_int), ("verts", C.POINTER(C.c_float)),
+                ("tris", C.POINTER(C.c_int)), ("normals", C.POINTER(C.c_float)), ("attr", C.POINTER(C.c_int)),
+                ("tri_albedo", C.POINTER(C.c_float)), ("n_attrs", C.c_int), ("mats", C.POINTER(OMaterial)),
+                ("n_lights", C.c_int), ("lights", C.POINTER(OLight)), ("fov", C.c_float), ("width", C.c_int),
+                ("height", C.c_int), ("maxdepth", C.c_int), ("mindepth", C.c_int)]
+
+
+class ORay(C.Structure):
+    _fields_ = [("o", C.c_float * 3), ("d", C.c_float * 3), ("tmin", C.c_float), ("tmax", C.c_float)]
+
+
+class OHit(C.Structure):
+    _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("prim", C.c_int)]
+
+
+class OCounters(C.Structure):
+    _fields_ = [("paths", C.c_int64), ("ext", C.c_int64), ("hit", C.c_int64), ("shadow", C.c_int64)]
+
+    def as_dict(self):
+        return {"paths": self.paths, "ext_rays": self.ext, "hit_rays": self.hit, "shadow_rays": self.shadow}
+
+
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_intersect_tri": (C.c_int, [_FP, _FP, _FP, C.POINTER(ORay), _FP, _FP, _FP]),
+        "orc_trace_brute": (None, [C.POINTER(OScene), C.POINTER(ORay), C.c_int, C.c_int, C.POINTER(OHit)]),
+        "orc_bvh_build": (C.c_void_p, [C.POINTER(OScene), C.c_int]),
+        "orc_bvh_free": (None, [C.c_void_p]),
+        "orc_trace_bvh": (None, [C.POINTER(OScene), C.c_void_p, C.POINTER(ORay), C.c_int, C.c_int, C.POINTER(OHit)]),
+        "orc_light_init": (None, [C.POINTER(OLight)]),
+        "orc_bsdf_eval": (None, [C.c_int, _FP, _FP, _FP, _FP, _FP, _FP]),
+        "orc_bsdf_sample": (None, [C.c_int, _FP, _FP, _FP, C.c_float, C.c_float, _FP, _FP, _FP]),
+        "orc_kelemen_table": (None, [C.c_float, C.c_int, C.c_int, C.c_uint32, _FP]),
+        "orc_kelemen_albedo": (C.c_float, [_FP, C.c_float, C.c_float]),
+        "orc_path_key": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        "orc_slot_uniform": (C.c_float, [C.c_uint32, C.c_uint32]),
+        "orc_path": (None, [C.POINTER(OScene), C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _FP, _FP,
+                            C.POINTER(OCounters)]),
+        "orc_render": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, C.c_int, C.c_int, _FP, C.POINTER(OCounters),
+                              C.c_int]),
+        "orc_render_paths": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, _IP, _IP, C.c_int, _FP, _FP,
+                                    C.POINTER(OCounters), C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+def ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+_table_cache = {}
+
+
+def kelemen_table(reflectance: float, distribution: int, n_samples: int = KELEMEN_SAMPLES,
+                  seed: int = KELEMEN_SEED) -> np.ndarray:
+    key = (float(np.float32(reflectance)), distribution, n_samples, seed)
+    if key not in _table_cache:
+        out = np.zeros(64 * 32, np.float32)
+        lib().orc_kelemen_table(reflectance, distribution, n_samples, seed, fp(out))
+        _table_cache[key] = out
+    return _table_cache[key]
+
+
+class Scene:
+    """An oracle scene built from the renderer's flattened export."""
+
+    def __init__(self, exp: dict, tri_albedo=None, bvh: bool = False, leaf: int = 4):
+        self.exp = exp
+        self.verts = np.ascontiguousarray(exp["verts"], np.float32).reshape(-1)
+        self.tris = np.ascontiguousarray(exp["tris"], np.int32).reshape(-1)
+        self.normals = np.ascontiguousarray(exp["normals"], np.float32).reshape(-1)
+        self.attr = np.ascontiguousarray(exp["attr"], np.int32).reshape(-1)
+        self.albedo = None if tri_albedo is None else np.ascontiguousarray(tri_albedo, np.float32).reshape(-1)
+        mats = exp["materials"]
+        self.mats = (OMaterial * max(len(mats), 1))()
+        self.tables = []
+        for k, m in enumerate(mats):
+            o = self.mats[k]
+            o.type, o.distribution = m.type, m.distribution
+            for j in range(3):
+                o.diffuse[j], o.spec[j], o.gain[j], o.emission[j] = m.diffuse[j], m.spec[j], m.gain[j], m.emission[j]
+            o.roughness, o.roughnessB, o.mix, o.reflectance = m.roughness, m.roughnessB, m.mix, m.reflectance
+            if m.type == 2:
+                q = min(max(int(round(float(np.float32(m.reflectance)) * 20.0)), 0), 20)
+                t = kelemen_table(np.float32(q / 20.0), m.distribution)
+                self.tables.append(t)
+                o.table = fp(t)
+        lights = exp["lights"]
+        self.lights = (OLight * max(len(lights), 1))()
+        for k, l in enumerate(lights):
+            o = self.lights[k]
+            o.type = l.type
+            for j in range(16):
+                o.m[j] = l.m[j]
+            o.sx, o.sy, o.exposure = l.sx, l.sy, l.exposure
+            for j in range(3):
+                o.color[j] = l.color[j]
+            lib().orc_light_init(C.byref(o))
+        s = OScene()
+        s.n_verts = self.verts.size // 3
+        s.n_tris = self.tris.size // 3
+        s.verts, s.tris, s.normals, s.attr = fp(self.verts), ip(self.tris), fp(self.normals), ip(self.attr)
+        s.tri_albedo = fp(self.albedo) if self.albedo is not None else None
+        s.n_attrs = len(mats)
+        s.mats = C.cast(self.mats, C.POINTER(OMaterial))
+        s.n_lights = len(lights)
+        s.lights = C.cast(self.lights, C.POINTER(OLight))
+        s.fov = exp["fov"]
+        s.width, s.height = exp["width"], exp["height"]
+        s.maxdepth, s.mindepth = exp["maxdepth"], exp["mindepth"]
+        self.s = s
+        self.bvh = lib().orc_bvh_build(C.byref(s), leaf) if bvh else None
+
+    def __del__(self):
+        try:
+            if self.bvh:
+                lib().orc_bvh_free(self.bvh)
+        except Exception:
+            pass
+
+    def trace(self, rays: np.ndarray, any_hit: bool = False, use_bvh: bool = False):
+        """rays: (n, 8) rows (ox, oy, oz, tmin, dx, dy, dz, tmax) like aur_ray."""
+        rays = np.asarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        orays = (ORay * max(n, 1))()
+        for i in range(n):
+            r = rays[i]
+            orays[i].o[0], orays[i].o[1], orays[i].o[2] = r[0], r[1], r[2]
+            orays[i].d[0], orays[i].d[1], orays[i].d[2] = r[4], r[5], r[6]
+            orays[i].tmin, orays[i].tmax = r[3], r[7]
+        hits = (OHit * max(n, 1))()
+        if use_bvh:
+            lib().orc_trace_bvh(C.byref(self.s), self.bvh, orays, n, int(any_hit), hits)
+        else:
+            lib().orc_trace_brute(C.byref(self.s), orays, n, int(any_hit), hits)
+        out = np.zeros(n, dtype=[("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim", np.int32)])
+        for i in range(n):
+            out[i] = (hits[i].t, hits[i].u, hits[i].v, hits[i].prim)
+        return out
+
+    def render(self, seed: int, first_pass: int, n_passes: int, threads: int = 0):
+        acc = np.zeros((self.exp["height"], self.exp["width"], 4), np.float32)
+        cnt = OCounters()
+        lib().orc_render(C.byref(self.s), self.bvh, seed, first_pass, n_passes, fp(acc), C.byref(cnt), threads)
+        return acc, cnt.as_dict()
+
+    def render_paths(self, seed: int, pixel, sample, slots=None, threads: int = 0):
+        pixel = np.ascontiguousarray(pixel, np.int32)
+        sample = np.ascontiguousarray(sample, np.int32)
+        n = pixel.size
+        out = np.zeros((n, 4), np.float32)
+        sl = None if slots is None else np.ascontiguousarray(slots, np.float32)
+        cnt = OCounters()
+        lib().orc_render_paths(C.byref(self.s), self.bvh, seed, ip(pixel), ip(sample), n,
+                               fp(sl) if sl is not None else None, fp(out), C.byref(cnt), threads)
+        return out, cnt.as_dict()
+
+
+def bsdf_eval(kind, color, rough, dist, V, N, L):
+    p = np.array(list(color) + [rough, dist], np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().orc_bsdf_eval(kind, fp(p), fp(np.asarray(V, np.float32)), fp(np.asarray(N, np.float32)),
+                        fp(np.asarray(L, np.float32)), fp(f), fp(pdf))
+    return f, float(pdf[0])
+
+
+def bsdf_sample(kind, color, rough, dist, V, N, u1, u2):
+    p = np.array(list(color) + [rough, dist], np.float32)
+    d = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    col = np.zeros(3, np.float32)
+    lib().orc_bsdf_sample(kind, fp(p), fp(np.asarray(V, np.float32)), fp(np.asarray(N, np.float32)), u1, u2, fp(d),
+                          fp(pdf), fp(col))
+    return d, float(pdf[0]), col
