@@ -76,6 +76,10 @@ class Stats(C.Structure):
                 "shadow_rays": self.shadow_rays, "ms": self.ms}
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
+
+
 class BsdfQuery(C.Structure):
     _fields_ = [("kind", C.c_int), ("distribution", C.c_int), ("color", C.c_float * 3), ("roughness", C.c_float),
                 ("V", C.c_float * 3), ("N", C.c_float * 3), ("L", C.c_float * 3), ("u1", C.c_float),
@@ -106,9 +110,13 @@ SIGNATURES = {
     "aur_scene_export": (C.c_int, [_P, _FP, _IP, _FP, _IP, C.POINTER(Material), C.POINTER(Light)]),
     "aur_render": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(Stats)]),
     "aur_render_paths": (C.c_int, [_P, _IP, _IP, C.c_int, _FP, _FP, C.POINTER(Stats)]),
+    "aur_set_profiling": (C.c_int, [_P, C.c_int]),
+    "aur_kernel_stats": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int, _IP]),
     "aur_trace": (C.c_int, [_P, C.POINTER(Ray), C.c_int, C.POINTER(Hit), C.c_int]),
     "aur_bsdf": (C.c_int, [_P, C.POINTER(BsdfQuery), C.c_int, C.POINTER(BsdfResult)]),
     "aur_kelemen_table": (C.c_int, [_P, C.c_float, C.c_int, C.c_int, C.c_uint32, _FP]),
+    "aur_loop_subdivide": (C.c_int, [_P, C.c_int, C.c_int, _FP, _IP, C.c_int, C.c_float, C.c_float, C.c_int,
+                                     C.c_uint32, _FP, _IP, _FP, C.c_int, C.c_int, _IP, _IP]),
 }
 
 _lib = None

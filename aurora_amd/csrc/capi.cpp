@@ -300,6 +300,10 @@ void aur_destroy(aur_ctx *ctx) {
         }
         (void)hipSetDevice(ctx->c.device);
         (void)hipStreamSynchronize(ctx->c.stream);
+        try {
+            timer_release(&ctx->c);
+        } catch (...) {
+        }
         hipStream_t s = ctx->c.stream;
         bool own = ctx->c.own_stream;
         delete ctx;
@@ -337,7 +341,17 @@ int aur_scene_load(aur_ctx *ctx, const char *scene_path, const char *config_json
         NEED(ctx);
         NEED(scene_path);
         HostScene s;
-        load_scene(scene_path, config_json_path ? config_json_path : "", &s);
+        std::string cfg = config_json_path ? config_json_path : "";
+        if (cfg.empty()) {
+            // the reference reads one fixed config.json (constants.h:46); without an
+            // explicit path, use the one next to the scene file when present
+            std::string sp = scene_path;
+            size_t slash = sp.find_last_of('/');
+            std::string cand = (slash == std::string::npos ? std::string() : sp.substr(0, slash + 1)) + "config.json";
+            FILE *f = std::fopen(cand.c_str(), "r");
+            if (f) { std::fclose(f); cfg = cand; }
+        }
+        load_scene(scene_path, cfg, &s);
         ctx->c.scene = std::move(s);
         ctx->c.have_scene = true;
         ctx->c.built = false;
@@ -369,6 +383,14 @@ int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
         s.attr.assign(d->attr, d->attr + d->n_tris);
         s.attr_mats.assign(d->mats, d->mats + d->n_attrs);
         s.lights.assign(d->lights, d->lights + d->n_lights);
+        for (int v : s.tris)
+            if (v < 0 || v >= d->n_verts) throw std::invalid_argument("triangle index out of range");
+        for (int a : s.attr)
+            if (a < 0 || a >= d->n_attrs) throw std::invalid_argument("attribute index out of range");
+        for (const aur_material &m : s.attr_mats)
+            if (m.type < AUR_MAT_MATTE || m.type > AUR_MAT_LIGHT) throw std::invalid_argument("unknown material type");
+        for (const aur_light &l : s.lights)
+            if (l.type != AUR_LIGHT_AREA && l.type != AUR_LIGHT_ENV) throw std::invalid_argument("unknown light type");
         s.flat = true;
         ctx->c.scene = std::move(s);
         ctx->c.have_scene = true;
@@ -528,6 +550,37 @@ int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_pa
     });
 }
 
+int aur_set_profiling(aur_ctx *ctx, int enable) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (enable) {
+            set_device(c);
+            timer_collect(c);
+            for (int k = 0; k < KT_COUNT; ++k) { c->timer.total_ms[k] = 0; c->timer.launches[k] = 0; }
+        }
+        c->timer.on = enable != 0;
+    });
+}
+
+int aur_kernel_stats(aur_ctx *ctx, aur_kernel_stat *out, int max, int *n_out) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(n_out);
+        Ctx *c = &ctx->c;
+        if (c->device >= 0) timer_collect(c);
+        int n = 0;
+        for (int k = 0; k < KT_COUNT && n < max; ++k, ++n) {
+            NEED(out);
+            std::memset(&out[n], 0, sizeof out[n]);
+            std::strncpy(out[n].name, kKernelNames[k], sizeof out[n].name - 1);
+            out[n].launches = c->timer.launches[k];
+            out[n].total_ms = c->timer.total_ms[k];
+        }
+        *n_out = n;
+    });
+}
+
 int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int any_hit) {
     return guarded([&] {
         NEED(ctx);
@@ -578,6 +631,43 @@ int aur_kelemen_table(aur_ctx *ctx, float reflectance, int distribution, int n_s
         HIP_CHECK(hipMemcpyAsync(out, d.p, sizeof(float) * AUR_KEL_COS * AUR_KEL_ROUGH, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
     });
+}
+
+int aur_loop_subdivide(aur_ctx *ctx, int nv, int nf, const float *P, const int *F, int levels, float amplitude,
+                       float frequency, int octaves, uint32_t seed, float *P_out, int *F_out, float *N_out, int nv_cap,
+                       int nf_cap, int *nv_out, int *nf_out) {
+    bool over = false;
+    int rc = guarded([&] {
+        NEED(ctx);
+        NEED(nv_out);
+        NEED(nf_out);
+        if (nv < 0 || nf < 0) throw std::invalid_argument("negative mesh size");
+        if (nv > 0) NEED(P);
+        if (nf > 0) NEED(F);
+        Ctx *c = &ctx->c;
+        set_device(c);
+        std::vector<float> Pin(P, P + 3 * (size_t)nv);
+        std::vector<int> Fin(F, F + 3 * (size_t)nf);
+        Displace d;
+        d.on = amplitude != 0.f;
+        d.amplitude = amplitude;
+        d.frequency = frequency;
+        d.octaves = octaves;
+        d.seed = seed;
+        LoopMesh m = loop_subdivide(c, Pin, Fin, levels, d);
+        int onv = (int)(m.P.size() / 3), onf = (int)(m.F.size() / 3);
+        *nv_out = onv;
+        *nf_out = onf;
+        if (onv > nv_cap || onf > nf_cap) { over = true; return; }
+        if (P_out) std::memcpy(P_out, m.P.data(), sizeof(float) * m.P.size());
+        if (F_out) std::memcpy(F_out, m.F.data(), sizeof(int) * m.F.size());
+        if (N_out) std::memcpy(N_out, m.N.data(), sizeof(float) * m.N.size());
+    });
+    if (rc == AUR_OK && over) {
+        g_last_error = "subdivided mesh exceeds the output capacity";
+        return AUR_ECAPACITY;
+    }
+    return rc;
 }
 
 }  // extern "C"

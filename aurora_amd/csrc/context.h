@@ -94,6 +94,17 @@ struct Wavefront {
     size_t stack_ovf_per_thread = 0;
 };
 
+// Optional per-kernel timing with HIP events recorded on the library stream
+// around each launch.
+enum { KT_GENERATE = 0, KT_EXTEND, KT_SHADE, KT_SHADOW, KT_FINISH, KT_RESOLVE, KT_COUNT };
+struct KernelTimer {
+    bool on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[KT_COUNT] = {0};
+    long long launches[KT_COUNT] = {0};
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -121,7 +132,15 @@ struct Ctx {
     int fb_w = 0, fb_h = 0;
 
     Wavefront wf;
+    KernelTimer timer;
 };
+
+// kernel timing helpers (wavefront.hip)
+void timer_begin(Ctx *c, int kernel, hipEvent_t *start);
+void timer_end(Ctx *c, int kernel, hipEvent_t start);
+void timer_collect(Ctx *c);
+void timer_release(Ctx *c);
+extern const char *const kKernelNames[KT_COUNT];
 
 // bvh_build.hip
 void build_bvh(Ctx *c, int width, int max_leaf);

@@ -428,6 +428,66 @@ __global__ void k_resolve(const float4 *__restrict__ out, int wh, int n_passes, 
 }
 
 // ---------------------------------------------------------------------------
+// per-kernel timing (events on the library stream around each launch)
+// ---------------------------------------------------------------------------
+const char *const kKernelNames[KT_COUNT] = {"k_generate", "k_extend", "k_shade", "k_shadow", "k_finish", "k_resolve"};
+
+static hipEvent_t timer_event(Ctx *c) {
+    KernelTimer &t = c->timer;
+    if (!t.pool.empty()) {
+        hipEvent_t e = t.pool.back();
+        t.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void timer_begin(Ctx *c, int kernel, hipEvent_t *start) {
+    (void)kernel;
+    *start = nullptr;
+    if (!c->timer.on) return;
+    *start = timer_event(c);
+    HIP_CHECK(hipEventRecord(*start, c->stream));
+}
+
+void timer_end(Ctx *c, int kernel, hipEvent_t start) {
+    if (!c->timer.on || !start) return;
+    hipEvent_t e = timer_event(c);
+    HIP_CHECK(hipEventRecord(e, c->stream));
+    c->timer.pending.push_back({kernel, {start, e}});
+}
+
+void timer_collect(Ctx *c) {
+    KernelTimer &t = c->timer;
+    for (auto &p : t.pending) {
+        HIP_CHECK(hipEventSynchronize(p.second.second));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        t.total_ms[p.first] += ms;
+        t.launches[p.first] += 1;
+        t.pool.push_back(p.second.first);
+        t.pool.push_back(p.second.second);
+    }
+    t.pending.clear();
+}
+
+void timer_release(Ctx *c) {
+    timer_collect(c);
+    for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+    c->timer.pool.clear();
+}
+
+#define TIMED(c, kid, launch)            \
+    do {                                 \
+        hipEvent_t ev_;                  \
+        timer_begin(c, kid, &ev_);       \
+        launch;                          \
+        timer_end(c, kid, ev_);          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
 static SceneView scene_view(Ctx *c) {
@@ -505,16 +565,16 @@ static WfView wf_view(Ctx *c, const PathSource &src) {
 // extend/shade/shadow/finish.  Results land in wf.out (by path id).
 static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
     const int gen_blocks = std::min((n + 255) / 256, persistent_blocks(c, 256));
-    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, sv, wv, n);
+    TIMED(c, KT_GENERATE, hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, sv, wv, n));
     const int tb = persistent_blocks(c, 128), sb = persistent_blocks(c, 256);
     int q = 0;
     for (int it = 0; it <= sv.maxdepth; ++it) {
-        if (c->bvh.width == 8) hipLaunchKernelGGL(k_extend<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q);
-        else hipLaunchKernelGGL(k_extend<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q);
-        hipLaunchKernelGGL(k_shade, dim3(sb), dim3(256), 0, c->stream, sv, wv, q);
-        if (c->bvh.width == 8) hipLaunchKernelGGL(k_shadow<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv);
-        else hipLaunchKernelGGL(k_shadow<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv);
-        hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, c->stream, wv);
+        if (c->bvh.width == 8) TIMED(c, KT_EXTEND, hipLaunchKernelGGL(k_extend<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q));
+        else TIMED(c, KT_EXTEND, hipLaunchKernelGGL(k_extend<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q));
+        TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
+        if (c->bvh.width == 8) TIMED(c, KT_SHADOW, hipLaunchKernelGGL(k_shadow<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv));
+        else TIMED(c, KT_SHADOW, hipLaunchKernelGGL(k_shadow<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv));
+        TIMED(c, KT_FINISH, hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, c->stream, wv));
         hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, c->stream, wv.counts, q);
         q = 1 - q;
     }
@@ -552,7 +612,7 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
         if (src.mode == 0) {
             int passes = (int)(n / wh);
             int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));
-            hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh, passes, out_rgba);
+            TIMED(c, KT_RESOLVE, hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh, passes, out_rgba));
         } else {
             HIP_CHECK(hipMemcpyAsync(out_rgba + done, wv.out, sizeof(float4) * n, hipMemcpyDeviceToDevice, c->stream));
         }
@@ -564,6 +624,7 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    timer_collect(c);
     if (stats) {
         unsigned long long s[8];
         HIP_CHECK(hipMemcpy(s, c->wf.stats.p, sizeof s, hipMemcpyDeviceToHost));

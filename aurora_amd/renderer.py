@@ -144,6 +144,17 @@ class Renderer:
                                           L.fptr(sl) if sl is not None else None, L.fptr(out), C.byref(st)))
         return out, st.as_dict()
 
+    # -- instrumentation -----------------------------------------------------
+    def set_profiling(self, on: bool) -> None:
+        L.check(self.lib.aur_set_profiling(self._ctx, int(on)))
+
+    def kernel_stats(self) -> dict:
+        arr = (L.KernelStat * 16)()
+        n = C.c_int()
+        L.check(self.lib.aur_kernel_stats(self._ctx, arr, 16, C.byref(n)))
+        return {arr[k].name.decode(): {"launches": arr[k].launches, "total_ms": arr[k].total_ms}
+                for k in range(n.value)}
+
     # -- parity queries ------------------------------------------------------
     def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         """rays: (n, 8) float32 rows (ox, oy, oz, tmin, dx, dy, dz, tmax).
@@ -160,6 +171,24 @@ class Renderer:
         res = (L.BsdfResult * len(queries))()
         L.check(self.lib.aur_bsdf(self._ctx, arr, len(queries), res))
         return [res[k] for k in range(len(queries))]
+
+    def loop_subdivide(self, P, F, levels: int, amplitude: float = 0.0, frequency: float = 1.0, octaves: int = 0,
+                       seed: int = 0):
+        """GPU Loop subdivision of an indexed mesh; returns (P, F, N) in object space."""
+        P = np.ascontiguousarray(P, np.float32).reshape(-1)
+        F = np.ascontiguousarray(F, np.int32).reshape(-1)
+        nv, nf = P.size // 3, F.size // 3
+        cap_f = nf * 4 ** levels
+        cap_v = nv + 3 * cap_f  # generous: V_L <= V_0 + sum of edges
+        Po = np.zeros(3 * cap_v, np.float32)
+        Fo = np.zeros(3 * cap_f, np.int32)
+        No = np.zeros(3 * cap_v, np.float32)
+        onv, onf = C.c_int(), C.c_int()
+        L.check(self.lib.aur_loop_subdivide(self._ctx, nv, nf, L.fptr(P), L.iptr(F), levels, amplitude, frequency,
+                                            octaves, seed, L.fptr(Po), L.iptr(Fo), L.fptr(No), cap_v, cap_f,
+                                            C.byref(onv), C.byref(onf)))
+        return (Po[:3 * onv.value].reshape(-1, 3), Fo[:3 * onf.value].reshape(-1, 3),
+                No[:3 * onv.value].reshape(-1, 3))
 
     def kelemen_table(self, reflectance: float, distribution: int, n_samples: int, seed: int) -> np.ndarray:
         out = np.zeros(64 * 32, np.float32)

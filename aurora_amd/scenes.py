@@ -143,8 +143,16 @@ class SceneSpec:
                "hdripath": "$BASE/hdri"}
         with open(os.path.join(out_dir, "config.json"), "w") as fh:
             json.dump(cfg, fh, indent=4)
+        # the reference opens OBJ paths verbatim (jsonParser.cpp:227, no template
+        # expansion), so geometry paths are written absolute
+        objects = {}
+        for k, o in self.objects.items():
+            o = dict(o)
+            if o.get("type") == "geometry":
+                o["path"] = os.path.abspath(os.path.join(out_dir, o["path"]))
+            objects[k] = o
         root = {"options": self.options, "renderCam": self.camera, "materials": self.materials,
-                "objects": self.objects, "shaders": self.shaders}
+                "objects": objects, "shaders": self.shaders}
         path = os.path.join(out_dir, scene_name)
         with open(path, "w") as fh:
             json.dump(root, fh, indent=4)
@@ -152,7 +160,7 @@ class SceneSpec:
 
 
 def _geo(name, obj, material, transforms=None, subdlevels=0, **extra):
-    d = {"type": "geometry", "name": name, "path": "$BASE/meshes/" + obj, "material": material,
+    d = {"type": "geometry", "name": name, "path": "meshes/" + obj, "material": material,
          "transforms": transforms or [], "subdlevels": int(subdlevels)}
     d.update(extra)
     return d
@@ -238,8 +246,8 @@ def displaced_spheres(width=1920, height=1080, spp=64, maxdepth=8, mindepth=8, n
     """
     rng = SplitMix64(seed)
     s = SceneSpec(options=_options(width, height, spp, maxdepth, mindepth),
-                  camera={"fov": 0.6,
-                          "transforms": [{"translate": [0.0, 7.0, 17.0]}, {"rotateX": -22.0}]})
+                  camera={"fov": 0.55,
+                          "transforms": [{"translate": [0.0, 7.0, 13.0]}, {"rotateX": -34.0}]})
     iv, iff = icosahedron()
     s.meshes["icosahedron.obj"] = (iv, iff, iv)
     h = ground / 2.0
@@ -269,7 +277,7 @@ def displaced_spheres(width=1920, height=1080, spp=64, maxdepth=8, mindepth=8, n
         e = rng.uniform(5.0, 20.0)
         x, z = rng.uniform(-6, 6), rng.uniform(-6, 6)
         s.objects["light%d" % i] = _square_light("light%d" % i, [e, e, e], 0.75, 0.75,
-                                                 [{"translate": [x, 6.0, z]}, {"rotateX": -90.0}])
+                                                 [{"translate": [x, 8.0, z]}, {"rotateX": -90.0}])
     s.objects["env"] = {"type": "envlight", "name": "env", "color": [0.1, 0.1, 0.1],
                         "exposure": 0.0, "envmap": "", "transforms": []}
     return s

@@ -177,12 +177,36 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
 int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_paths,
                      const float *slots, float *rgba_out, aur_stats *out);
 
+/* ---- instrumentation ---- */
+/* Kernel timing, measured with HIP events recorded on the library stream
+ * around each launch.  The reference only logs a running "K rays/sec"
+ * (renderer.cpp:265-268); this is its device-side counterpart. */
+typedef struct {
+    char name[32];
+    int64_t launches;
+    double total_ms;
+} aur_kernel_stat;
+/* enable != 0 turns timing on and resets the totals. */
+int aur_set_profiling(aur_ctx *ctx, int enable);
+int aur_kernel_stats(aur_ctx *ctx, aur_kernel_stat *out, int max, int *n_out);
+
 /* ---- parity queries ---- */
 int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int any_hit);
 int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r);
 /* Kelemen albedo table (64 cos x 32 roughness), kelemenMaterial.cpp:113-178. */
 int aur_kelemen_table(aur_ctx *ctx, float reflectance, int distribution, int n_samples,
                       uint32_t seed, float *out);
+/* Loop subdivision of an indexed object-space mesh on the GPU
+ * (LoopSubdivMesh::LoopSubdivMesh, loopSubdivMesh.cpp:167-562).  Corners are
+ * welded by exact position.  amplitude > 0 adds the radial value-noise
+ * displacement with area-weighted normals.  Outputs: limit positions
+ * (3 * nv_out), faces (3 * nf_out), vertex normals (3 * nv_out); returns
+ * AUR_ECAPACITY if the result exceeds nv_cap / nf_cap (the counts are still
+ * written). */
+int aur_loop_subdivide(aur_ctx *ctx, int nv, int nf, const float *P, const int *F, int levels,
+                       float amplitude, float frequency, int octaves, uint32_t seed,
+                       float *P_out, int *F_out, float *N_out, int nv_cap, int nf_cap,
+                       int *nv_out, int *nf_out);
 
 #ifdef __cplusplus
 }

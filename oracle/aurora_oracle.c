@@ -893,3 +893,18 @@ void orc_render_paths(const orc_scene *s, const orc_bvh *b, uint32_t seed, const
     }
     if (cnt) { cnt->paths += c_paths; cnt->ext += c_ext; cnt->hit += c_hit; cnt->shadow += c_sh; }
 }
+
+/* ---- single-function entry points for the closed-form tests ---- */
+void orc_camera_ray(const orc_scene *s, int x, int y, float j0, float j1, orc_ray *r) { camera_ray(s, x, y, j0, j1, r); }
+
+void orc_light_sample(const orc_light *l, const float *P, const float *N, float x, float y, orc_ray *ray,
+                      float *pdf, float *color) {
+    Sample3D s = light_sample(l, v3p(P), v3p(N), x, y);
+    *ray = s.ray;
+    *pdf = s.pdf;
+    color[0] = s.color.r; color[1] = s.color.g; color[2] = s.color.b;
+}
+
+float orc_light_pdf(const orc_light *l, orc_ray *ray) { return light_pdf(l, ray); }
+
+int orc_light_visible(const orc_light *l, const float *P, const float *N) { return light_visible(l, v3p(P), v3p(N)); }

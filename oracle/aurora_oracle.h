@@ -140,9 +140,15 @@ int orc_loop_subdivide(int nv, int nf, const float *P, const int *F, int levels,
                        int *nv_out, int *nf_out);
 
 /* ---- value-noise displacement + area-weighted normals ---- */
-void orc_displace_radial(int nv, float *P, const float *center, float radius, float amplitude,
-                         int octaves, uint32_t seed);
+void orc_displace_radial(int nv, float *P, float amplitude, float frequency, int octaves, uint32_t seed);
 void orc_vertex_normals(int nv, int nf, const float *P, const int *F, float *N_out);
+
+/* ---- single routines for closed-form tests ---- */
+void orc_camera_ray(const orc_scene *s, int x, int y, float j0, float j1, orc_ray *r);
+void orc_light_sample(const orc_light *l, const float *P, const float *N, float x, float y, orc_ray *ray,
+                      float *pdf, float *color);
+float orc_light_pdf(const orc_light *l, orc_ray *ray);
+int orc_light_visible(const orc_light *l, const float *P, const float *N);
 
 /* ---- adjoint (path-replay) ---- */
 void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_pass, int n_passes,

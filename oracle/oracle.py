@@ -92,6 +92,14 @@ def lib() -> C.CDLL:
                               C.c_int]),
         "orc_render_paths": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, _IP, _IP, C.c_int, _FP, _FP,
                                     C.POINTER(OCounters), C.c_int]),
+        "orc_loop_subdivide": (C.c_int, [C.c_int, C.c_int, _FP, _IP, C.c_int, _FP, _IP, _FP, C.c_int, C.c_int, _IP,
+                                         _IP]),
+        "orc_displace_radial": (None, [C.c_int, _FP, C.c_float, C.c_float, C.c_int, C.c_uint32]),
+        "orc_vertex_normals": (None, [C.c_int, C.c_int, _FP, _IP, _FP]),
+        "orc_camera_ray": (None, [C.POINTER(OScene), C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(ORay)]),
+        "orc_light_sample": (None, [C.POINTER(OLight), _FP, _FP, C.c_float, C.c_float, C.POINTER(ORay), _FP, _FP]),
+        "orc_light_pdf": (C.c_float, [C.POINTER(OLight), C.POINTER(ORay)]),
+        "orc_light_visible": (C.c_int, [C.POINTER(OLight), _FP, _FP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -234,3 +242,68 @@ def bsdf_sample(kind, color, rough, dist, V, N, u1, u2):
     lib().orc_bsdf_sample(kind, fp(p), fp(np.asarray(V, np.float32)), fp(np.asarray(N, np.float32)), u1, u2, fp(d),
                           fp(pdf), fp(col))
     return d, float(pdf[0]), col
+
+
+def loop_subdivide(P, F, levels: int, amplitude: float = 0.0, frequency: float = 1.0, octaves: int = 0,
+                   seed: int = 0):
+    """Oracle Loop subdivision; with amplitude != 0 the radial displacement and
+    area-weighted normals follow.  Returns (P, F, N)."""
+    P = np.ascontiguousarray(P, np.float32).reshape(-1)
+    F = np.ascontiguousarray(F, np.int32).reshape(-1)
+    nv, nf = P.size // 3, F.size // 3
+    cap_f = nf * 4 ** levels
+    cap_v = nv + 3 * cap_f
+    Po = np.zeros(3 * cap_v, np.float32)
+    Fo = np.zeros(3 * cap_f, np.int32)
+    No = np.zeros(3 * cap_v, np.float32)
+    onv, onf = C.c_int(), C.c_int()
+    rc = lib().orc_loop_subdivide(nv, nf, fp(P), ip(F), levels, fp(Po), ip(Fo), fp(No), cap_v, cap_f, C.byref(onv),
+                                  C.byref(onf))
+    if rc != 0:
+        raise RuntimeError("oracle loop subdivision failed: %d" % rc)
+    Po, Fo, No = Po[:3 * onv.value].copy(), Fo[:3 * onf.value].copy(), No[:3 * onv.value].copy()
+    if amplitude != 0.0:
+        lib().orc_displace_radial(onv.value, fp(Po), amplitude, frequency, octaves, seed)
+        lib().orc_vertex_normals(onv.value, onf.value, fp(Po), ip(Fo), fp(No))
+    return Po.reshape(-1, 3), Fo.reshape(-1, 3), No.reshape(-1, 3)
+
+
+def make_light(m16, sx, sy, color, exposure=0.0, kind=0) -> OLight:
+    o = OLight()
+    o.type = kind
+    for j in range(16):
+        o.m[j] = m16[j]
+    o.sx, o.sy, o.exposure = sx, sy, exposure
+    for j in range(3):
+        o.color[j] = color[j]
+    lib().orc_light_init(C.byref(o))
+    return o
+
+
+def light_sample(light: OLight, P, N, x, y):
+    ray = ORay()
+    pdf = np.zeros(1, np.float32)
+    col = np.zeros(3, np.float32)
+    lib().orc_light_sample(C.byref(light), fp(np.asarray(P, np.float32)), fp(np.asarray(N, np.float32)), x, y,
+                           C.byref(ray), fp(pdf), fp(col))
+    return np.array(ray.d[:], np.float32), float(ray.tmax), float(pdf[0]), col
+
+
+def light_pdf(light: OLight, o, d, tmin=0.0005, tmax=1e7):
+    ray = ORay()
+    for j in range(3):
+        ray.o[j], ray.d[j] = o[j], d[j]
+    ray.tmin, ray.tmax = tmin, tmax
+    p = lib().orc_light_pdf(C.byref(light), C.byref(ray))
+    return float(p), float(ray.tmax)
+
+
+def intersect_tri(p, e1, e2, o, d, tmin, tmax):
+    ray = ORay()
+    for j in range(3):
+        ray.o[j], ray.d[j] = o[j], d[j]
+    ray.tmin, ray.tmax = tmin, tmax
+    t, b1, b2 = np.zeros(1, np.float32), np.zeros(1, np.float32), np.zeros(1, np.float32)
+    hit = lib().orc_intersect_tri(fp(np.asarray(p, np.float32)), fp(np.asarray(e1, np.float32)),
+                                  fp(np.asarray(e2, np.float32)), C.byref(ray), fp(t), fp(b1), fp(b2))
+    return bool(hit), float(t[0]), float(b1[0]), float(b2[0])

@@ -1,0 +1,275 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: bit-exact hit ids (and t, u, v) against brute force; BSDF eval/pdf/sample
+within 1e-5 relative; Kelemen tables within 1e-5; per-path radiance within
+1e-4 relative under injected random numbers; statistical agreement of
+converged images; Loop subdivision within float rounding.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import random_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b, floor):
+    return np.abs(a - b) / (np.abs(b) + floor)
+
+
+# ---------------------------------------------------------------------------
+# traversal
+# ---------------------------------------------------------------------------
+def test_trace_bit_exact_box_room(gpu_box_room, oracle):
+    s = oracle.Scene(gpu_box_room.export())
+    rng = np.random.default_rng(11)
+    inside = random_rays(rng, 30000, [-0.99, -0.99, -4.49], [0.99, 0.99, -2.51])
+    camera = random_rays(rng, 10000, [0, 0, 0], [0, 0, 0], tmin=0.1)
+    camera[:, 4:7] = np.stack([rng.uniform(-0.4, 0.4, 10000), rng.uniform(-0.4, 0.4, 10000),
+                               -np.ones(10000)], 1)
+    camera[:, 4:7] /= np.linalg.norm(camera[:, 4:7], axis=1, keepdims=True)
+    rays = np.concatenate([inside, camera]).astype(np.float32)
+    for width in (4, 8):
+        gpu_box_room.build(width=width, max_leaf=2)
+        g, o = gpu_box_room.trace(rays), s.trace(rays)
+        assert np.array_equal(g["prim"], o["prim"])
+        hit = o["prim"] >= 0
+        assert hit.mean() > 0.8
+        for k in ("t", "u", "v"):
+            assert np.array_equal(g[k][hit], o[k][hit]), k
+        ga, oa = gpu_box_room.trace(rays, any_hit=True), s.trace(rays, any_hit=True)
+        assert np.array_equal(ga["prim"] >= 0, oa["prim"] >= 0)
+    gpu_box_room.build()
+
+
+def test_trace_subdivided_spheres(tmp_path, oracle):
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer
+    spec = scenes.displaced_spheres(width=64, height=36, n_spheres=3, levels=4)
+    path = spec.write(str(tmp_path))
+    r = Renderer(path)
+    r.build(width=8, max_leaf=4)
+    e = r.export()
+    assert e["tris"].shape[0] == 3 * 20 * 4 ** 4 + 2 + 4 * 2
+    s = oracle.Scene(e, bvh=True)
+    rng = np.random.default_rng(3)
+    n = 20000
+    f = e["fov"]
+    d = np.stack([rng.uniform(-f, f, n), rng.uniform(-f, f, n) * e["height"] / e["width"], -np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cam = np.concatenate([np.zeros((n, 3)), np.full((n, 1), 0.1), d, np.full((n, 1), 1e7)], 1).astype(np.float32)
+    h = s.trace(cam, use_bvh=True)
+    hit = h["prim"] >= 0
+    P = cam[hit, 4:7] * h["t"][hit, None]
+    sec = random_rays(rng, int(hit.sum()), [0, 0, 0], [0, 0, 0], tmin=5e-4)
+    sec[:, :3] = P
+    rays = np.concatenate([cam, sec]).astype(np.float32)
+    g, o = r.trace(rays), s.trace(rays)
+    mismatch = np.mean(g["prim"] != o["prim"])
+    assert hit.mean() > 0.5
+    assert mismatch <= 1e-4, mismatch
+    same = g["prim"] == o["prim"]
+    assert np.array_equal(g["t"][same], o["t"][same])
+    ga, oa = r.trace(rays, any_hit=True), s.trace(rays, any_hit=True)
+    assert np.mean((ga["prim"] >= 0) != (oa["prim"] >= 0)) <= 1e-4
+    r.close()
+
+
+# ---------------------------------------------------------------------------
+# BSDF eval / pdf / sample
+# ---------------------------------------------------------------------------
+def test_bsdf_parity(gpu_box_room, oracle):
+    from aurora_amd import _lib as L
+    rng = np.random.default_rng(7)
+    qs, ref = [], []
+    for i in range(3000):
+        kind = 0 if i % 3 == 0 else 1
+        dist = i % 2
+        N = rng.normal(size=3); N /= np.linalg.norm(N)
+        V = rng.normal(size=3); V = V if V @ N > 0 else -V; V /= np.linalg.norm(V)
+        Ld = rng.normal(size=3); Ld = Ld if Ld @ N > 0 else -Ld; Ld /= np.linalg.norm(Ld)
+        col = rng.uniform(0.05, 1.0, 3)
+        rough = float(rng.uniform(0.05, 0.8))
+        u1, u2 = float(rng.uniform()), float(rng.uniform())
+        q = L.BsdfQuery()
+        q.kind, q.distribution, q.roughness, q.u1, q.u2 = kind, dist, rough, u1, u2
+        q.color[:], q.V[:], q.N[:], q.L[:] = list(col), list(V), list(N), list(Ld)
+        qs.append(q)
+        f, pdf = oracle.bsdf_eval(kind, col, rough, dist, V.astype(np.float32), N.astype(np.float32),
+                                  Ld.astype(np.float32))
+        d, spdf, scol = oracle.bsdf_sample(kind, col, rough, dist, V.astype(np.float32), N.astype(np.float32), u1, u2)
+        ref.append((f, pdf, d, spdf, scol))
+    res = gpu_box_room.bsdf(qs)
+    fg = np.array([r.f[:] for r in res]); fo = np.array([x[0] for x in ref])
+    pg = np.array([r.pdf for r in res]); po = np.array([x[1] for x in ref])
+    dg = np.array([r.sample_dir[:] for r in res]); do = np.array([x[2] for x in ref])
+    sg = np.array([r.sample_pdf for r in res]); so = np.array([x[3] for x in ref])
+    cg = np.array([r.sample_color[:] for r in res]); co = np.array([x[4] for x in ref])
+    assert np.max(_rel(fg, fo, 1e-6 * fo.max())) < 1e-5
+    assert np.max(_rel(pg, po, 1e-6 * np.abs(po).max())) < 1e-5
+    assert np.max(np.abs(dg - do)) < 1e-5
+    assert np.max(_rel(sg, so, 1e-6 * so.max())) < 1e-5
+    assert np.max(_rel(cg, co, 1e-6 * co.max())) < 1e-5
+
+
+@pytest.mark.parametrize("dist", [0, 1])
+def test_kelemen_table_parity(gpu_box_room, oracle, dist):
+    g = gpu_box_room.kelemen_table(0.05, dist, 4096, 99).ravel()
+    o = oracle.kelemen_table(0.05, dist, n_samples=4096, seed=99)
+    assert np.max(_rel(g, o, 1e-6)) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# per-path radiance
+# ---------------------------------------------------------------------------
+def test_injected_sample_paths(gpu_box_room, oracle):
+    """Both renderers read the same injected uniform table (2 + 9*maxdepth per path)."""
+    e = gpu_box_room.export()
+    s = oracle.Scene(e)
+    rng = np.random.default_rng(21)
+    n = 20000
+    nslots = 2 + 9 * e["maxdepth"]
+    slots = rng.uniform(0, 1, size=(n, nslots)).astype(np.float32)
+    pix = rng.integers(0, e["width"] * e["height"], n).astype(np.int32)
+    smp = rng.integers(0, 64, n).astype(np.int32)
+    g, gs = gpu_box_room.render_paths(pix, smp, slots=slots)
+    o, os_ = s.render_paths(0, pix, smp, slots=slots)
+    rel = _rel(g, o, 1e-3)
+    assert np.mean(rel > 1e-4) < 2e-3, np.mean(rel > 1e-4)
+    assert np.median(rel) < 1e-6
+    for k in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
+        assert abs(gs[k] - os_[k]) <= max(2, 1e-3 * os_[k]), k
+
+
+def test_full_render_matches_oracle_same_stream(gpu_box_room, oracle):
+    e = gpu_box_room.export()
+    s = oracle.Scene(e)
+    gpu_box_room.set_seed(5)
+    acc, gst = gpu_box_room.render_passes(0, 4)
+    oacc, ost = s.render(5, 0, 4)
+    diff = np.abs(acc - oacc)
+    assert np.mean(diff > 1e-3 * (1 + np.abs(oacc))) < 1e-3
+    np.testing.assert_allclose(acc.mean(axis=(0, 1)), oacc.mean(axis=(0, 1)), rtol=1e-4)
+    for k in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
+        assert abs(gst[k] - ost[k]) <= max(2, 1e-4 * ost[k]), k
+
+
+def test_converged_image_statistics(gpu_box_room, oracle):
+    """Independent streams: per-pixel means agree within 3 sigma up to the
+    multiple-comparison budget, and the RMSE matches the predicted noise."""
+    e = gpu_box_room.export()
+    s = oracle.Scene(e)
+    W, H = e["width"], e["height"]
+    rng = np.random.default_rng(2)
+    pix = rng.choice(W * H, 400, replace=False).astype(np.int32)
+    spp = 256
+    P = np.repeat(pix, spp)
+    S = np.tile(np.arange(spp, dtype=np.int32), pix.size)
+    gpu_box_room.set_seed(101)
+    g, _ = gpu_box_room.render_paths(P, S)
+    o, _ = s.render_paths(202, P, S)
+    g = g[:, :3].reshape(pix.size, spp, 3)
+    o = o[:, :3].reshape(pix.size, spp, 3)
+    mg, mo = g.mean(1), o.mean(1)
+    se = np.sqrt(g.var(1) / spp + o.var(1) / spp) + 1e-6
+    z = np.abs(mg - mo) / se
+    assert np.mean(z > 3.0) < 0.01 + 3 * math.sqrt(0.0027 / z.size)
+    rmse = np.sqrt(np.mean((mg - mo) ** 2))
+    pred = np.sqrt(np.mean(se ** 2))
+    assert rmse < 1.5 * pred
+
+
+def test_render_is_deterministic_and_additive(gpu_box_room):
+    gpu_box_room.set_seed(9)
+    a, _ = gpu_box_room.render_passes(0, 3)
+    b, _ = gpu_box_room.render_passes(0, 3)
+    assert np.array_equal(a, b)
+    gpu_box_room.render_passes(0, 1, fetch=False)
+    c, _ = gpu_box_room.render_passes(1, 2)
+    assert np.array_equal(a, c)
+
+
+# ---------------------------------------------------------------------------
+# Loop subdivision kernels
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("levels", [1, 3, 5])
+def test_loop_subdivision_parity(gpu_box_room, oracle, levels):
+    from aurora_amd.scenes import icosahedron
+    v, f = icosahedron()
+    Pg, Fg, Ng = gpu_box_room.loop_subdivide(v, f, levels)
+    Po, Fo, No = oracle.loop_subdivide(v, f, levels)
+    assert np.array_equal(Fg, Fo)
+    np.testing.assert_allclose(Pg, Po, atol=2e-6)
+    np.testing.assert_allclose(np.abs(np.sum(Ng * No, axis=1)), 1.0, atol=2e-5)
+
+
+def test_loop_subdivision_boundary_and_displacement(gpu_box_room, oracle):
+    # an open strip (boundary edges) and the displaced sphere
+    v = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [0, 1, 0.2], [1, 1, 0], [2, 1, 0.1]], np.float32)
+    f = np.array([[0, 1, 4], [0, 4, 3], [1, 2, 5], [1, 5, 4]], np.int32)
+    Pg, Fg, Ng = gpu_box_room.loop_subdivide(v, f, 3)
+    Po, Fo, No = oracle.loop_subdivide(v, f, 3)
+    assert np.array_equal(Fg, Fo)
+    np.testing.assert_allclose(Pg, Po, atol=2e-6)
+    np.testing.assert_allclose(np.abs(np.sum(Ng * No, axis=1)), 1.0, atol=1e-4)
+    from aurora_amd.scenes import icosahedron
+    v, f = icosahedron()
+    Pg, Fg, Ng = gpu_box_room.loop_subdivide(v, f, 4, amplitude=0.05, frequency=2.0, octaves=3, seed=77)
+    Po, Fo, No = oracle.loop_subdivide(v, f, 4, amplitude=0.05, frequency=2.0, octaves=3, seed=77)
+    np.testing.assert_allclose(Pg, Po, atol=5e-6)
+    np.testing.assert_allclose(np.sum(Ng * No, axis=1), 1.0, atol=1e-4)
+    r = np.linalg.norm(Pg, axis=1)
+    assert r.min() > 0.85 and r.max() < 1.05
+
+
+# ---------------------------------------------------------------------------
+# full-size workload properties (1920x1080, 2.62M triangles)
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def gpu_spheres(tmp_path_factory):
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer
+    d = str(tmp_path_factory.mktemp("spheres"))
+    path = scenes.write_config("displaced_spheres", d)
+    r = Renderer(path)
+    r.build()
+    yield r
+    r.close()
+
+
+def test_full_size_scene_properties(gpu_spheres):
+    info = gpu_spheres.info()
+    assert info["n_tris"] == 8 * 327680 + 2 + 4 * 2
+    assert (info["width"], info["height"], info["maxdepth"]) == (1920, 1080, 8)
+    gpu_spheres.set_seed(1)
+    a, st = gpu_spheres.render_passes(0, 2)
+    assert np.isfinite(a).all() and a[..., :3].min() >= 0
+    assert st["paths"] == 2 * 1920 * 1080
+    assert st["hit_rays"] <= st["ext_rays"] and st["ext_rays"] >= st["paths"]
+    assert st["shadow_rays"] <= 2 * st["hit_rays"]
+    b, _ = gpu_spheres.render_passes(0, 2)
+    assert np.array_equal(a, b)
+
+
+def test_full_size_paths_match_oracle(gpu_spheres, oracle):
+    e = gpu_spheres.export()
+    s = oracle.Scene(e, bvh=True)
+    rng = np.random.default_rng(4)
+    pix = rng.integers(0, e["width"] * e["height"], 4000).astype(np.int32)
+    smp = rng.integers(0, 64, 4000).astype(np.int32)
+    gpu_spheres.set_seed(8)
+    g, _ = gpu_spheres.render_paths(pix, smp)
+    o, _ = s.render_paths(8, pix, smp)
+    rel = _rel(g, o, 1e-3)
+    assert np.mean(rel > 1e-4) < 5e-3, np.mean(rel > 1e-4)
+
+
+def test_api_errors_on_device(gpu_box_room):
+    from aurora_amd._lib import AuroraError
+    with pytest.raises(AuroraError) as ei:
+        gpu_box_room.build(width=6)
+    assert ei.value.code == -1
+    gpu_box_room.build()
