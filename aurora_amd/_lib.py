@@ -76,6 +76,11 @@ class Stats(C.Structure):
                 "shadow_rays": self.shadow_rays, "ms": self.ms}
 
 
+class ParamGrads(C.Structure):
+    _fields_ = [("albedo_tri", C.POINTER(C.c_float)), ("roughness", C.POINTER(C.c_float)),
+                ("spec_weight", C.POINTER(C.c_float))]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
@@ -110,6 +115,8 @@ SIGNATURES = {
     "aur_scene_export": (C.c_int, [_P, _FP, _IP, _FP, _IP, C.POINTER(Material), C.POINTER(Light)]),
     "aur_render": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(Stats)]),
     "aur_render_paths": (C.c_int, [_P, _IP, _IP, C.c_int, _FP, _FP, C.POINTER(Stats)]),
+    "aur_set_material_params": (C.c_int, [_P, _FP, _FP]),
+    "aur_adjoint": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(ParamGrads), C.POINTER(Stats)]),
     "aur_set_profiling": (C.c_int, [_P, C.c_int]),
     "aur_kernel_stats": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int, _IP]),
     "aur_trace": (C.c_int, [_P, C.POINTER(Ray), C.c_int, C.POINTER(Hit), C.c_int]),

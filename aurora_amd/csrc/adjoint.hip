@@ -1,0 +1,344 @@
+// adjoint.hip -- path-replay adjoint of the forward integrator.
+//
+// Gradients of L = sum_pixels dL/dI . I with respect to per-triangle diffuse
+// albedo, per-material (Kelemen) roughness and per-material specular weight,
+// for the estimator `Integrator::integrateCameraSample` computes
+// (integrator.cpp:152-300) with fixed random numbers.
+//
+// Path replay (Vicini, Speierer, Jakob 2021, "Path Replay Backpropagation"):
+// a path is first replayed to get its radiance L; a second replay walks the
+// same vertices (same counter-based random numbers, same rays) carrying the
+// remaining radiance R_j = L - sum_{k<j} D_k, where D_k is the direct-lighting
+// contribution added at vertex k.  At vertex j with BSDF value f_j:
+//   * the continuation weight f_j cos / pdf multiplies every later
+//     contribution:   dL += (d log f_j(w_out) / dtheta) * R_{j+1}
+//   * the next-event terms use f_j at the light-sample and BSDF-sample
+//     directions:     dL += sum_s (d log f_j(w_s) / dtheta) * D_{j,s}
+//   * the Kelemen lobe choice (probability A(cos, roughness), chosen without
+//     reweighting, kelemenMaterial.cpp:236-253) is a discrete choice whose
+//     probability depends on roughness: the score term
+//     (d log P(choice) / dtheta) * R_j covers it.
+// Sampling pdfs are detached (the sampled directions are treated as fixed),
+// which keeps the estimator of the derivative unbiased; MIS weights are
+// detached too, which is unbiased because they sum to one per direction.
+// The firefly clamp factor 0.2^k is applied to the incoming gradient.
+//
+// One thread replays one path (both passes); no per-vertex state is stored.
+// Gradients of small scenes are first summed per workgroup in LDS and then
+// added to HBM with one atomic per parameter per workgroup.
+#include "shade_common.h"
+#include "traverse.h"
+
+namespace aur {
+
+#define AUR_ADJ_LDS_PARAMS 6144
+
+struct AdjParams {
+    const float4 *dldi;   // per pixel dL/dI (rgb, alpha ignored)
+    float inv_passes;
+    int first_pass;
+    uint32_t seed;
+    int n_paths;
+    float *g_albedo;      // 3 * n_tris (or null)
+    float *g_rough;       // n_attrs
+    float *g_spec;        // n_attrs
+    int n_tris, n_attrs;
+    int use_lds;
+    unsigned long long *stats;  // paths, ext rays, shadow rays
+};
+
+struct GradSink {
+    float *albedo, *rough, *spec;
+    __device__ inline void add(float *p, float v) {
+        if (v != 0.f) atomicAdd(p, v);
+    }
+};
+
+template <int W>
+__device__ inline bool occluded(const WNode<W> *nodes, const float4 *tris, f3 o, f3 d, float tmin, float tmax, TravStack &st) {
+    TravHit h;
+    traverse<W, true>(nodes, tris, o, d, tmin, tmax, st, h);
+    return h.prim >= 0;
+}
+
+// One replay of a path.  Without GRAD it returns the unclamped radiance and the
+// number of 0.2 clamp steps; with GRAD it accumulates gradients given delta
+// (incoming dL/dL_path, clamp factor included) and the path's radiance.
+template <int W, bool GRAD>
+__device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st, int pixel, const PathRng &rng,
+                       f3 *L_out, int *clamp_steps, bool *camera_miss, f3 delta, f3 L_total, GradSink &g,
+                       unsigned long long *n_ext, unsigned long long *n_shadow) {
+    const int x = pixel % sv.width, y = pixel / sv.width;
+    const float fovx = sv.fov, fovy = ((float)sv.height / (float)sv.width) * fovx;
+    const float uu = x + rng.u(0), vv = y + rng.u(1);
+    f3 d = norm3(mk3(((2.f * uu - sv.width) / (float)sv.width) * fovx, ((-2.f * vv + sv.height) / (float)sv.height) * fovy, -1.f));
+    f3 o = mk3(0.f, 0.f, 0.f);
+    TravHit h;
+    traverse<W, false>(nodes, sv.tris, o, d, 0.1f, 10000000.f, st, h);
+    (*n_ext)++;
+    *camera_miss = h.prim < 0;
+    f3 L = mk3(0.f, 0.f, 0.f);
+    if (h.prim < 0) {
+        *L_out = mk3(sv.env[0], sv.env[1], sv.env[2]);
+        *clamp_steps = 0;
+        return;
+    }
+    f3 T = mk3(1.f, 1.f, 1.f);
+    float cp = 1.f;
+    bool matte = false;
+    int rtype = 0, tb = 0, bounces = 0;
+    f3 R = L_total;
+    while (tb < sv.maxdepth) {
+        const float t = h.t, u = h.u, v = h.v;
+        const float w = 1.f - u - v;
+        const f3 P = add3(o, mul3(d, t));
+        const int prim = h.prim;
+        const float4 s0 = sv.shade[3 * prim], s1 = sv.shade[3 * prim + 1], s2 = sv.shade[3 * prim + 2];
+        const int attr = __float_as_int(s0.w);
+        f3 nm = mk3(w * s0.x + u * s1.x + v * s2.x, w * s0.y + u * s1.y + v * s2.y, w * s0.z + u * s1.z + v * s2.z);
+        if (dot3(nm, d) > 0.f) nm = mul3(nm, -1.f);
+        f3 nn = norm3(nm);
+        const f3 vn = neg3(norm3(d));
+        if (dot3(vn, nn) <= 0.f) nn = norm3(cross3(vn, cross3(nn, vn)));
+        if (rtype == 1) matte = true;
+        const DMat m = sv.mats[attr];
+        if (rtype == 0) {
+            f3 e = mul3v(T, mk3(m.emission[0], m.emission[1], m.emission[2]));
+            L = add3(L, e);
+            if (GRAD) R = sub3(R, e);
+        }
+        if (m.type == MAT_LIGHT) break;
+        const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
+        BrdfChoice ch = get_brdf_ex(sv, m, prim, vn, nn, matte, rng.u(vslot + 0));
+        const Bsdf &b = ch.b;
+        const bool spec_lobe = b.kind == BSDF_CT;
+        const bool kel = m.type == MAT_KELEMEN;
+        const bool alb = (b.kind == BSDF_LAMBERT) && sv.albedo != nullptr;
+        if (GRAD && kel && ch.kelemen_choice) {
+            float dlogp = spec_lobe ? ch.dA / fmaxf(ch.A, 1e-12f) : -ch.dA / fmaxf(1.f - ch.A, 1e-12f);
+            g.add(&g.rough[attr], dot3(delta, R) * dlogp * ch.drough);
+        }
+        // next-event estimation, integrator.cpp:100-130
+        f3 cl = mk3(0.f, 0.f, 0.f), cb = mk3(0.f, 0.f, 0.f);
+        float dl_l = 0.f, dl_b = 0.f;
+        int nvis = 0;
+        for (int li = 0; li < sv.n_lights; ++li) nvis += light_visible(sv.lights[li], P, nn) ? 1 : 0;
+        if (nvis) {
+            int pick = min((int)(rng.u(vslot + 1) * nvis), nvis - 1);
+            int li = 0;
+            for (int k = 0; li < sv.n_lights; ++li)
+                if (light_visible(sv.lights[li], P, nn)) {
+                    if (k == pick) break;
+                    ++k;
+                }
+            const DLight &l = sv.lights[li];
+            LightSample ls = light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
+            float ct = dot3(nn, ls.dir);
+            if (!spec_lobe) {
+                if (ct >= 0.f && ls.pdf > 0.f) {
+                    (*n_shadow)++;
+                    if (!occluded<W>(nodes, sv.tris, P, ls.dir, AUR_RAY_BIAS, ls.tmax, st))
+                        cl = div3(mul3(mul3v(ls.color, bsdf_eval(b, ls.dir, vn, nn)), ct), ls.pdf);
+                }
+            } else {
+                if (ct > 0.f && ls.pdf > 0.f) {
+                    float bpdf = bsdf_pdf(b, ls.dir, vn, nn);
+                    if (bpdf > 0.f) {
+                        (*n_shadow)++;
+                        if (!occluded<W>(nodes, sv.tris, P, ls.dir, AUR_RAY_BIAS, ls.tmax, st)) {
+                            float wgt = power_heuristic(ls.pdf, bpdf);
+                            cl = div3(mul3(mul3v(mul3(ls.color, wgt), bsdf_eval(b, ls.dir, vn, nn)), ct), ls.pdf);
+                            if (GRAD) dl_l = ct_dlogd_drough(b.dist, b.rough, dot3(nn, norm3(add3(ls.dir, vn))));
+                        }
+                    }
+                }
+                BsdfSample bs = bsdf_sample(b, vn, nn, rng.u(vslot + 4), rng.u(vslot + 5));
+                float ct2 = dot3(nn, bs.dir);
+                if (ct2 > 0.f) {
+                    float tmax = bs.tmax;
+                    float lpdf = light_pdf(l, P, bs.dir, AUR_RAY_BIAS, &tmax);
+                    if (lpdf > 0.f && bs.pdf > 0.f) {
+                        (*n_shadow)++;
+                        if (!occluded<W>(nodes, sv.tris, P, bs.dir, AUR_RAY_BIAS, tmax, st)) {
+                            float wgt = power_heuristic(bs.pdf, lpdf);
+                            cb = div3(mul3(mul3v(mul3(bs.color, wgt), mk3(l.Le[0], l.Le[1], l.Le[2])), ct2), bs.pdf);
+                            if (GRAD) dl_b = ct_dlogd_drough(b.dist, b.rough, dot3(nn, norm3(add3(bs.dir, vn))));
+                        }
+                    }
+                }
+            }
+            const float scale = (float)nvis;
+            cl = mul3v(mul3(cl, scale), T);
+            cb = mul3v(mul3(cb, scale), T);
+            L = add3(L, add3(cl, cb));
+            if (GRAD) {
+                if (alb) {
+                    const float *a = sv.albedo + 3 * prim;
+                    g.add(&g.albedo[3 * prim + 0], delta.x * cl.x / a[0]);
+                    g.add(&g.albedo[3 * prim + 1], delta.y * cl.y / a[1]);
+                    g.add(&g.albedo[3 * prim + 2], delta.z * cl.z / a[2]);
+                }
+                if (kel && spec_lobe) {
+                    g.add(&g.spec[attr], dot3(delta, add3(cl, cb)) / m.spec[0]);
+                    g.add(&g.rough[attr], (dot3(delta, cl) * dl_l + dot3(delta, cb) * dl_b) * ch.drough);
+                }
+                R = sub3(R, add3(cl, cb));
+            }
+        }
+        // continuation, integrator.cpp:208-257
+        BsdfSample s = bsdf_sample(b, vn, nn, rng.u(vslot + 6), rng.u(vslot + 7));
+        if (s.pdf <= 0.f) break;
+        float ndl = dot3(nn, s.dir);
+        T = mul3v(T, div3(mul3(s.color, ndl), s.pdf));
+        if (bounces > sv.mindepth) cp *= lum3(s.color) * ndl / s.pdf;
+        if (black3(T)) break;
+        rtype = spec_lobe ? 2 : 1;
+        if (bounces > sv.mindepth) {
+            if (rng.u(vslot + 8) > cp) break;
+            T = div3(T, cp);
+        }
+        if (GRAD) {
+            if (alb) {
+                const float *a = sv.albedo + 3 * prim;
+                g.add(&g.albedo[3 * prim + 0], delta.x * R.x / a[0]);
+                g.add(&g.albedo[3 * prim + 1], delta.y * R.y / a[1]);
+                g.add(&g.albedo[3 * prim + 2], delta.z * R.z / a[2]);
+            }
+            if (kel && spec_lobe) {
+                float dr = dot3(delta, R);
+                g.add(&g.spec[attr], dr / m.spec[0]);
+                float c = dot3(nn, norm3(add3(s.dir, vn)));
+                g.add(&g.rough[attr], dr * ct_dlogd_drough(b.dist, b.rough, c) * ch.drough);
+            }
+        }
+        o = P;
+        d = s.dir;
+        ++tb;
+        ++bounces;
+        traverse<W, false>(nodes, sv.tris, o, d, AUR_RAY_BIAS, s.tmax, st, h);
+        (*n_ext)++;
+        if (h.prim < 0) break;
+    }
+    int k = 0;
+    f3 Lc = L;
+    while (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY) { Lc = mul3(Lc, 0.2f); ++k; }
+    *L_out = L;
+    *clamp_steps = k;
+}
+
+template <int W>
+__global__ void __launch_bounds__(128) k_adjoint(SceneView sv, AdjParams ap, int *ovf, int ovf_per_thread) {
+    __shared__ int s_stack[AUR_STACK_LDS * 128];
+    __shared__ float s_grad[AUR_ADJ_LDS_PARAMS];
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = 128;
+    st.ovf = ovf + (size_t)gtid * ovf_per_thread;
+    const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
+    const int n_alb = ap.g_albedo ? 3 * ap.n_tris : 0;
+    GradSink g;
+    if (ap.use_lds) {
+        for (int i = threadIdx.x; i < n_alb + 2 * ap.n_attrs; i += blockDim.x) s_grad[i] = 0.f;
+        __syncthreads();
+        g.albedo = s_grad;
+        g.rough = s_grad + n_alb;
+        g.spec = s_grad + n_alb + ap.n_attrs;
+    } else {
+        g.albedo = ap.g_albedo;
+        g.rough = ap.g_rough;
+        g.spec = ap.g_spec;
+    }
+    unsigned long long n_ext = 0, n_shadow = 0, n_paths = 0;
+    const int wh = sv.width * sv.height;
+    for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
+        const int pixel = i % wh, sample = ap.first_pass + i / wh;
+        PathRng rng;
+        rng.slots = nullptr;
+        rng.key = path_key(ap.seed, (uint32_t)pixel, (uint32_t)sample);
+        const float4 dl = ap.dldi[pixel];
+        f3 L;
+        int k;
+        bool miss;
+        f3 zero = mk3(0.f, 0.f, 0.f);
+        replay<W, false>(sv, nodes, st, pixel, rng, &L, &k, &miss, zero, zero, g, &n_ext, &n_shadow);
+        ++n_paths;
+        if (miss) continue;
+        float cf = ap.inv_passes;
+        for (int j = 0; j < k; ++j) cf *= 0.2f;
+        f3 delta = mk3(dl.x * cf, dl.y * cf, dl.z * cf);
+        if (delta.x == 0.f && delta.y == 0.f && delta.z == 0.f) continue;
+        f3 L2;
+        int k2;
+        replay<W, true>(sv, nodes, st, pixel, rng, &L2, &k2, &miss, delta, L, g, &n_ext, &n_shadow);
+    }
+    atomicAdd(&ap.stats[0], n_paths);
+    atomicAdd(&ap.stats[1], n_ext);
+    atomicAdd(&ap.stats[2], n_shadow);
+    if (ap.use_lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < n_alb; i += blockDim.x)
+            if (s_grad[i] != 0.f) atomicAdd(&ap.g_albedo[i], s_grad[i]);
+        for (int i = threadIdx.x; i < ap.n_attrs; i += blockDim.x) {
+            if (s_grad[n_alb + i] != 0.f) atomicAdd(&ap.g_rough[i], s_grad[n_alb + i]);
+            if (s_grad[n_alb + ap.n_attrs + i] != 0.f) atomicAdd(&ap.g_spec[i], s_grad[n_alb + ap.n_attrs + i]);
+        }
+    }
+}
+
+void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *d_galb, float *d_grough, float *d_gspec,
+             aur_stats *stats) {
+    if (!c->built) throw StateError("adjoint before aur_build_accel");
+    SceneView sv = scene_view(c);
+    const int wh = c->scene.opt.width * c->scene.opt.height;
+    const long long n = (long long)wh * n_passes;
+    if (n > 0x7fffffffLL) throw CapacityError("too many paths for one adjoint call");
+    DevBuf st;
+    st.reserve(sizeof(unsigned long long) * 4);
+    fill_zero(c, st.p, sizeof(unsigned long long) * 4);
+    AdjParams ap;
+    ap.dldi = d_dldi;
+    ap.inv_passes = 1.f / (float)n_passes;
+    ap.first_pass = first_pass;
+    ap.seed = c->seed;
+    ap.n_paths = (int)n;
+    ap.g_albedo = d_galb;
+    ap.g_rough = d_grough;
+    ap.g_spec = d_gspec;
+    ap.n_tris = c->n_tris;
+    ap.n_attrs = c->n_mats;
+    ap.use_lds = ((d_galb ? 3 * c->n_tris : 0) + 2 * c->n_mats) <= AUR_ADJ_LDS_PARAMS;
+    ap.stats = st.as<unsigned long long>();
+    // traversal stack spill area, as for the wavefront kernels
+    int need = (c->bvh.width - 1) * (c->bvh.depth + 1) + 2;
+    size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 1;
+    const int blocks = c->n_cu * 4;
+    DevBuf sovf;
+    sovf.reserve(sizeof(int) * (size_t)blocks * 128 * ovf);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, c->stream));
+    if (c->bvh.width == 8)
+        hipLaunchKernelGGL(k_adjoint<8>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, sovf.as<int>(), (int)ovf);
+    else
+        hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, sovf.as<int>(), (int)ovf);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(e1, c->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned long long h[4];
+    HIP_CHECK(hipMemcpy(h, st.p, sizeof h, hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->paths = (int64_t)h[0];
+        stats->ext_rays = (int64_t)h[1];
+        stats->hit_rays = 0;
+        stats->shadow_rays = (int64_t)h[2];
+        stats->ms = ms;
+    }
+}
+
+}  // namespace aur

@@ -550,6 +550,73 @@ int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_pa
     });
 }
 
+int aur_set_material_params(aur_ctx *ctx, const float *roughness, const float *spec_weight) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        HostScene &s = c->scene;
+        if (!c->have_scene) throw StateError("no scene loaded");
+        for (size_t i = 0; i < s.attr_mats.size(); ++i) {
+            aur_material &m = s.attr_mats[i];
+            if (roughness && (m.type == AUR_MAT_KELEMEN || m.type == AUR_MAT_METAL)) {
+                float r = std::max(roughness[i], 1e-6f);
+                m.roughness = r;
+                if (m.type == AUR_MAT_METAL) m.roughnessB = r;
+            }
+            if (spec_weight && m.type == AUR_MAT_KELEMEN) {
+                float w = std::max(spec_weight[i], 1e-6f);
+                m.spec[0] = m.spec[1] = m.spec[2] = w;
+            }
+        }
+        if (c->built) {
+            set_device(c);
+            for (size_t i = 0; i < s.attr_mats.size() && i < c->dmats.size(); ++i) {
+                const aur_material &m = s.attr_mats[i];
+                DMat &d = c->dmats[i];
+                d.rough = m.roughness;
+                d.roughB = m.roughnessB;
+                for (int k = 0; k < 3; ++k) d.spec[k] = m.spec[k];
+            }
+            HIP_CHECK(hipMemcpyAsync(c->d_mats.p, c->dmats.data(), sizeof(DMat) * c->dmats.size(), hipMemcpyHostToDevice,
+                                     c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
+int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_dimage, aur_param_grads *out,
+                aur_stats *stats) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(dloss_dimage);
+        NEED(out);
+        Ctx *c = &ctx->c;
+        if (!c->built) throw StateError("adjoint before aur_build_accel");
+        if (first_pass < 0 || n_passes <= 0) throw std::invalid_argument("bad pass range");
+        if (out->albedo_tri && c->tri_albedo.empty())
+            throw StateError("albedo gradients need a per-triangle albedo (aur_set_tri_albedo)");
+        set_device(c);
+        const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
+        const size_t na = 3 * (size_t)c->n_tris, nm = (size_t)c->n_mats;
+        DevBuf dl, ga, gr, gs;
+        dl.reserve(sizeof(float4) * wh);
+        ga.reserve(sizeof(float) * std::max<size_t>(na, 1));
+        gr.reserve(sizeof(float) * std::max<size_t>(nm, 1));
+        gs.reserve(sizeof(float) * std::max<size_t>(nm, 1));
+        HIP_CHECK(hipMemcpyAsync(dl.p, dloss_dimage, sizeof(float4) * wh, hipMemcpyHostToDevice, c->stream));
+        fill_zero(c, ga.p, sizeof(float) * std::max<size_t>(na, 1));
+        fill_zero(c, gr.p, sizeof(float) * std::max<size_t>(nm, 1));
+        fill_zero(c, gs.p, sizeof(float) * std::max<size_t>(nm, 1));
+        adjoint(c, first_pass, n_passes, dl.as<float4>(), out->albedo_tri ? ga.as<float>() : nullptr, gr.as<float>(),
+                gs.as<float>(), stats);
+        if (out->albedo_tri)
+            HIP_CHECK(hipMemcpyAsync(out->albedo_tri, ga.p, sizeof(float) * na, hipMemcpyDeviceToHost, c->stream));
+        if (out->roughness) HIP_CHECK(hipMemcpyAsync(out->roughness, gr.p, sizeof(float) * nm, hipMemcpyDeviceToHost, c->stream));
+        if (out->spec_weight) HIP_CHECK(hipMemcpyAsync(out->spec_weight, gs.p, sizeof(float) * nm, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
 int aur_set_profiling(aur_ctx *ctx, int enable) {
     return guarded([&] {
         NEED(ctx);

@@ -155,6 +155,9 @@ struct PathSource {
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats);
 void resolve_passes(Ctx *c, const float4 *out_rgba, int n_passes, float4 *accum);
 void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_hit);
+// adjoint.hip
+void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *d_galb, float *d_grough, float *d_gspec,
+             aur_stats *stats);
 // parity.hip
 void bsdf_queries(Ctx *c, const aur_bsdf_query *d_q, int n, aur_bsdf_result *d_r);
 void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out);

@@ -265,6 +265,32 @@ __device__ inline float kelemen_albedo(const float *tab, float costheta, float r
     return (1.f - rmix) * res_a + rmix * res_b;
 }
 
+// d(kelemen_albedo)/d(roughness): the lookup is linear in roughness inside a
+// table cell, so the derivative is (ROUGH-1) * (row rB - row rA) at this cosine.
+__device__ inline float kelemen_albedo_drough(const float *tab, float costheta, float rough) {
+    int i = ref_floor(costheta * (AUR_KEL_COS - 1));
+    int ra = ref_floor(rough * (AUR_KEL_ROUGH - 1));
+    i = min(max(i, 0), AUR_KEL_COS - 1);
+    ra = min(max(ra, 0), AUR_KEL_ROUGH - 1);
+    int rb = AUR_KEL_ROUGH - 1;
+    int i1 = min(i + 1, AUR_KEL_COS - 1);
+    float mix = costheta * (AUR_KEL_COS - 1) - i;
+    float res_a = (1.f - mix) * tab[i * AUR_KEL_ROUGH + ra] + mix * tab[i1 * AUR_KEL_ROUGH + ra];
+    float res_b = (1.f - mix) * tab[i * AUR_KEL_ROUGH + rb] + mix * tab[i1 * AUR_KEL_ROUGH + rb];
+    return (AUR_KEL_ROUGH - 1) * (res_b - res_a);
+}
+
+// d log D / d roughness of the distribution at cos(theta_h) = ndoth (the
+// evalSampleWorld form, cookTorrance.cpp:136, without floors):
+//   Beckmann  D ~ exp(-tan^2/m^2) / (m^2 cos^3):  2 tan^2 / m^3 - 2 / m
+//   GGX       D ~ a^2 / (cos^3 (a^2 + tan^2)^2):  2 / a - 4 a / (a^2 + tan^2)
+__device__ inline float ct_dlogd_drough(int dist, float rough, float ndoth) {
+    float c2 = fmaxf(ndoth * ndoth, 1e-12f);
+    float t2 = fmaxf(1.f - c2, 0.f) / c2;
+    if (dist == DIST_GGX) return 2.f / rough - 4.f * rough / (rough * rough + t2);
+    return 2.f * t2 / (rough * rough * rough) - 2.f / rough;
+}
+
 // ---------------------------------------------------------------------------
 // lights: squareLight.h:57-122, infiniteAreaLight.cpp:35-106
 // ---------------------------------------------------------------------------

@@ -20,6 +20,7 @@
 // a grid-stride loop over a count that lives in device memory, so a whole
 // frame is a fixed sequence of launches with no host round trip.
 #include "context.h"
+#include "shade_common.h"
 #include "traverse.h"
 
 namespace aur {
@@ -28,21 +29,6 @@ enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2 };
 enum { F_MATTE = 1 << 2, F_NOCLAMP = 1 << 24 };
 enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_FINISH = 3 };
 enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
-
-struct SceneView {
-    const void *nodes;
-    const float4 *tris;
-    const float4 *shade;    // 3 per triangle: (n0, attr), (n1, 0), (n2, 0)
-    const DMat *mats;
-    const DLight *lights;
-    const float *tables;
-    const float *albedo;    // optional per-triangle diffuse override
-    int n_lights;
-    int has_env;
-    float env[3];
-    float fov;
-    int width, height, maxdepth, mindepth;
-};
 
 struct WfView {
     float4 *ray[2];
@@ -81,12 +67,6 @@ __device__ inline void wave_count(unsigned long long *counter, bool pred) {
     int leader = __ffsll((long long)mask) - 1;
     if (mask && lane == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
 }
-
-struct PathRng {
-    const float *slots;
-    uint32_t key;
-    __device__ inline float u(int slot) const { return slots ? slots[slot] : slot_uniform(key, (uint32_t)slot); }
-};
 
 __device__ inline void path_pixel_sample(const SceneView &sv, const WfView &wv, int path, int *pixel, int *sample) {
     if (wv.src.mode == 0) {
@@ -196,31 +176,6 @@ __global__ void __launch_bounds__(128) k_shadow(SceneView sv, WfView wv) {
 // shade: one path vertex (integrator.cpp:180-274)
 // ---------------------------------------------------------------------------
 struct ShadowOut { bool valid; f3 o, d; float tmin, tmax; f3 c; };
-
-__device__ inline Bsdf get_brdf(const SceneView &sv, const DMat &m, int prim, f3 vn, f3 nn, bool matte, float u_lobe) {
-    Bsdf b;
-    f3 diffuse = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-    if (sv.albedo) diffuse = mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2]);
-    if (m.type == MAT_MATTE) {  // matteMaterial.cpp:30-35
-        b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f;
-    } else if (m.type == MAT_METAL) {  // metalMaterial.cpp:32-62
-        float ct = dot3(vn, nn);
-        f3 sc = mk3(schlick(ct, m.spec[0]), schlick(ct, m.spec[1]), schlick(ct, m.spec[2]));
-        float rough = (u_lobe > m.mix) ? m.rough : m.roughB;
-        if (matte) rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR;
-        b.kind = BSDF_CT; b.dist = m.dist; b.color = mul3v(sc, mk3(m.gain[0], m.gain[1], m.gain[2])); b.rough = rough;
-    } else {  // kelemenMaterial.cpp:225-255
-        float rough = m.rough;
-        if (matte) rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR;
-        b.kind = BSDF_CT; b.dist = m.dist; b.color = mk3(m.spec[0], m.spec[1], m.spec[2]); b.rough = rough;
-        if (m.refl < 1.f) {
-            float c = fminf(fmaxf(dot3(nn, vn), 0.f), 1.f);
-            float albedo = kelemen_albedo(sv.tables + (size_t)m.table * (AUR_KEL_COS * AUR_KEL_ROUGH), c, rough);
-            if (u_lobe > albedo) { b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f; }
-        }
-    }
-    return b;
-}
 
 __global__ void __launch_bounds__(256) k_shade(SceneView sv, WfView wv, int q) {
     const int n = wv.counts[q];
@@ -490,7 +445,7 @@ void timer_release(Ctx *c) {
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
-static SceneView scene_view(Ctx *c) {
+SceneView scene_view(Ctx *c) {
     SceneView sv;
     sv.nodes = c->bvh.nodes.p;
     sv.tris = c->bvh.tri_sorted.as<float4>();

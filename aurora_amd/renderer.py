@@ -144,6 +144,28 @@ class Renderer:
                                           L.fptr(sl) if sl is not None else None, L.fptr(out), C.byref(st)))
         return out, st.as_dict()
 
+    # -- inverse rendering ---------------------------------------------------
+    def set_material_params(self, roughness=None, spec_weight=None) -> None:
+        """Per-material roughness / Kelemen spec weight (arrays of n_attrs)."""
+        self._rough = None if roughness is None else np.ascontiguousarray(roughness, np.float32)
+        self._specw = None if spec_weight is None else np.ascontiguousarray(spec_weight, np.float32)
+        L.check(self.lib.aur_set_material_params(self._ctx, L.fptr(self._rough) if self._rough is not None else None,
+                                                 L.fptr(self._specw) if self._specw is not None else None))
+
+    def adjoint(self, first_pass: int, n_passes: int, dloss_dimage: np.ndarray, albedo: bool = True):
+        """Path-replay adjoint: returns (albedo grads (n_tris, 3) or None, roughness grads, spec grads, stats)."""
+        i = self.info()
+        dl = np.zeros((i["height"], i["width"], 4), np.float32)
+        g = np.asarray(dloss_dimage, np.float32)
+        dl[..., :g.shape[-1]] = g.reshape(i["height"], i["width"], -1)[..., :4]
+        ga = np.zeros(3 * i["n_tris"], np.float32) if albedo else None
+        gr = np.zeros(i["n_attrs"], np.float32)
+        gs = np.zeros(i["n_attrs"], np.float32)
+        pg = L.ParamGrads(L.fptr(ga) if albedo else None, L.fptr(gr), L.fptr(gs))
+        st = L.Stats()
+        L.check(self.lib.aur_adjoint(self._ctx, first_pass, n_passes, L.fptr(dl), C.byref(pg), C.byref(st)))
+        return (ga.reshape(-1, 3) if albedo else None), gr, gs, st.as_dict()
+
     # -- instrumentation -----------------------------------------------------
     def set_profiling(self, on: bool) -> None:
         L.check(self.lib.aur_set_profiling(self._ctx, int(on)))

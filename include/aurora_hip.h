@@ -177,6 +177,24 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
 int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_paths,
                      const float *slots, float *rgba_out, aur_stats *out);
 
+/* ---- inverse rendering (no counterpart in the reference, which is not
+ * differentiable) ---- */
+/* Per-material parameters (n_attrs floats each; NULL leaves a field as is):
+ * roughness sets Kelemen roughness and metal roughnessA/B; spec_weight sets a
+ * Kelemen material's speccolor to (s, s, s).  Values are clamped to >= 1e-6. */
+int aur_set_material_params(aur_ctx *ctx, const float *roughness, const float *spec_weight);
+typedef struct {
+    float *albedo_tri;   /* 3 * n_tris, or NULL (needs aur_set_tri_albedo) */
+    float *roughness;    /* n_attrs, or NULL */
+    float *spec_weight;  /* n_attrs, or NULL */
+} aur_param_grads;
+/* Path-replay adjoint of passes [first_pass, first_pass + n_passes) rendered
+ * with the current seed: given dL/dI for the mean image I (4 * W * H floats,
+ * host, alpha ignored) writes dL/dtheta into the non-NULL fields of out.
+ * Roughness and spec-weight gradients are produced for Kelemen materials. */
+int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_dimage,
+                aur_param_grads *out, aur_stats *stats);
+
 /* ---- instrumentation ---- */
 /* Kernel timing, measured with HIP events recorded on the library stream
  * around each launch.  The reference only logs a running "K rays/sec"

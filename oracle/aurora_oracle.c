@@ -908,3 +908,228 @@ void orc_light_sample(const orc_light *l, const float *P, const float *N, float 
 float orc_light_pdf(const orc_light *l, orc_ray *ray) { return light_pdf(l, ray); }
 
 int orc_light_visible(const orc_light *l, const float *P, const float *N) { return light_visible(l, v3p(P), v3p(N)); }
+
+/* ------------------------------------------------------------------------ */
+/* path-replay adjoint (CPU reference gradient)                              */
+/*                                                                          */
+/* The reference is not differentiable; this is the CPU counterpart of the  */
+/* GPU adjoint: gradients of sum(dL/dI . I) w.r.t. per-triangle albedo,     */
+/* Kelemen roughness and Kelemen spec weight (speccolor = (s, s, s)), for   */
+/* the estimator path_trace() computes with fixed random numbers.  Sampling */
+/* pdfs and MIS weights are held fixed; the Kelemen lobe choice adds the    */
+/* score term dlogP(choice) * (radiance from this vertex on).               */
+/* ------------------------------------------------------------------------ */
+static double dlogd_drough(int dist, float rough, float ndoth) {
+    double c2 = (double)ndoth * ndoth;
+    if (c2 < 1e-12) c2 = 1e-12;
+    double t2 = (1.0 - c2) / c2;
+    if (t2 < 0) t2 = 0;
+    if (dist == ORC_DIST_GGX) return 2.0 / rough - 4.0 * rough / ((double)rough * rough + t2);
+    return 2.0 * t2 / ((double)rough * rough * rough) - 2.0 / rough;
+}
+
+/* derivative of orc_kelemen_albedo in roughness (linear inside a table cell) */
+static double kelemen_dalbedo(const float *tab, float costheta, float roughness) {
+    int i = ref_floor(costheta * (ORC_KELEMEN_COS - 1));
+    int ra = ref_floor(roughness * (ORC_KELEMEN_ROUGH - 1));
+    if (i < 0) i = 0;
+    if (i > ORC_KELEMEN_COS - 1) i = ORC_KELEMEN_COS - 1;
+    if (ra < 0) ra = 0;
+    if (ra > ORC_KELEMEN_ROUGH - 1) ra = ORC_KELEMEN_ROUGH - 1;
+    int rb = ORC_KELEMEN_ROUGH - 1;
+    int i1 = i + 1 < ORC_KELEMEN_COS - 1 ? i + 1 : ORC_KELEMEN_COS - 1;
+    double mix = costheta * (ORC_KELEMEN_COS - 1) - i;
+    double a = (1. - mix) * tab[i * ORC_KELEMEN_ROUGH + ra] + mix * tab[i1 * ORC_KELEMEN_ROUGH + ra];
+    double b = (1. - mix) * tab[i * ORC_KELEMEN_ROUGH + rb] + mix * tab[i1 * ORC_KELEMEN_ROUGH + rb];
+    return (ORC_KELEMEN_ROUGH - 1) * (b - a);
+}
+
+typedef struct { double *albedo, *rough, *spec; } GradAcc;
+
+/* One replay.  grad == 0: returns radiance in L (unclamped) and the clamp
+ * step count.  grad == 1: accumulates gradients for incoming delta. */
+static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, const Rng *rng, int grad, const double *delta,
+                       const double *Ltot, double *Lout, GradAcc *g) {
+    orc_ray ray;
+    camera_ray(s, x, y, rng_u(rng, 0), rng_u(rng, 1), &ray);
+    orc_hit h;
+    orc_counters cnt = {0, 0, 0, 0};
+    trace(s, bv, &ray, 0, &h);
+    double L[3] = {0, 0, 0}, R[3];
+    if (Ltot) { R[0] = Ltot[0]; R[1] = Ltot[1]; R[2] = Ltot[2]; }
+    if (h.prim < 0) { Lout[0] = Lout[1] = Lout[2] = 0; return -1; }
+    C3 T = c3(1.f, 1.f, 1.f);
+    float cp = 1.f;
+    int matte_path = 0, ray_type = RAY_CAMERA, bounces = 0, tb = 0;
+    while (tb < s->maxdepth) {
+        int attr = s->attr[h.prim], prim = h.prim;
+        V3 P, Nn;
+        hit_geometry(s, &ray, &h, &P, &Nn);
+        V3 vn = vneg(vnorm(v3p(ray.d)));
+        if (dot3(vn, Nn) <= 0.) Nn = vnorm(cross3(vn, cross3(Nn, vn)));
+        if (ray_type == RAY_DIFFUSE) matte_path = 1;
+        const orc_material *m = &s->mats[attr];
+        if (ray_type == RAY_CAMERA) {
+            C3 e = cmulc(T, c3p(m->emission));
+            L[0] += e.r; L[1] += e.g; L[2] += e.b;
+            if (grad) { R[0] -= e.r; R[1] -= e.g; R[2] -= e.b; }
+        }
+        int vslot = 2 + ORC_SLOTS_PER_VERTEX * tb;
+        Bsdf b;
+        float ulobe = rng_u(rng, vslot + 0);
+        if (!get_brdf(s, attr, prim, vn, Nn, matte_path, ulobe, &b)) break;
+        int kel = m->type == ORC_MAT_KELEMEN, spec = b.kind == ORC_BSDF_CT;
+        int alb = b.kind == ORC_BSDF_LAMBERT && s->tri_albedo != NULL;
+        double drough = matte_path && (m->type == ORC_MAT_KELEMEN || m->type == ORC_MAT_METAL) ? (1. - CAUSTIC_BLUR) : 1.;
+        if (grad && kel && m->reflectance < 1.f) {
+            float c = fclampf(dot3(Nn, vn), 0.f, 1.f);
+            double A = orc_kelemen_albedo(m->table, c, b.kind == ORC_BSDF_CT ? b.rough :
+                                          (float)(matte_path ? m->roughness * (1. - CAUSTIC_BLUR) + CAUSTIC_BLUR : m->roughness));
+            float reff = (float)(matte_path ? m->roughness * (1. - CAUSTIC_BLUR) + CAUSTIC_BLUR : m->roughness);
+            double dA = kelemen_dalbedo(m->table, c, reff);
+            double dlogp = spec ? dA / (A > 1e-12 ? A : 1e-12) : -dA / (1 - A > 1e-12 ? 1 - A : 1e-12);
+            g->rough[attr] += (delta[0] * R[0] + delta[1] * R[1] + delta[2] * R[2]) * dlogp * drough;
+        }
+        /* next-event estimation with separate light / BSDF parts */
+        double cl[3] = {0, 0, 0}, cb[3] = {0, 0, 0}, dll = 0, dlb = 0;
+        int idx[64], n = 0;
+        for (int i = 0; i < s->n_lights && n < 64; ++i)
+            if (light_visible(&s->lights[i], P, Nn)) idx[n++] = i;
+        if (n) {
+            int pick = (int)(rng_u(rng, vslot + 1) * n);
+            if (pick > n - 1) pick = n - 1;
+            const orc_light *l = &s->lights[idx[pick]];
+            Sample3D ls = light_sample(l, P, Nn, rng_u(rng, vslot + 2), rng_u(rng, vslot + 3));
+            float ct = dot3(Nn, v3p(ls.ray.d));
+            if (!spec) {
+                if (ct >= 0. && ls.pdf > 0. && !occluded(s, bv, &ls.ray)) {
+                    C3 c = cdivf(cmulf(cmulc(ls.color, bsdf_eval(&b, v3p(ls.ray.d), vn, Nn)), ct), ls.pdf);
+                    cl[0] = c.r; cl[1] = c.g; cl[2] = c.b;
+                }
+            } else {
+                if (ct > 0. && ls.pdf > 0.) {
+                    float bpdf = bsdf_pdf(&b, v3p(ls.ray.d), vn, Nn);
+                    if (bpdf > 0. && !occluded(s, bv, &ls.ray)) {
+                        float w = power_heuristic(1, ls.pdf, 1, bpdf);
+                        C3 c = cdivf(cmulf(cmulc(cmulf(ls.color, w), bsdf_eval(&b, v3p(ls.ray.d), vn, Nn)), ct), ls.pdf);
+                        cl[0] = c.r; cl[1] = c.g; cl[2] = c.b;
+                        dll = dlogd_drough(b.dist, b.rough, dot3(Nn, vnorm(vadd(v3p(ls.ray.d), vn))));
+                    }
+                }
+                Sample3D bs = bsdf_sample(&b, vn, Nn, rng_u(rng, vslot + 4), rng_u(rng, vslot + 5));
+                bs.ray.o[0] = P.x; bs.ray.o[1] = P.y; bs.ray.o[2] = P.z;
+                float ct2 = dot3(Nn, v3p(bs.ray.d));
+                if (ct2 > 0.) {
+                    float lpdf = light_pdf(l, &bs.ray);
+                    if (lpdf > 0. && bs.pdf > 0. && !occluded(s, bv, &bs.ray)) {
+                        float w = power_heuristic(1, bs.pdf, 1, lpdf);
+                        C3 c = cdivf(cmulf(cmulc(cmulf(bs.color, w), light_eval(l)), ct2), bs.pdf);
+                        cb[0] = c.r; cb[1] = c.g; cb[2] = c.b;
+                        dlb = dlogd_drough(b.dist, b.rough, dot3(Nn, vnorm(vadd(v3p(bs.ray.d), vn))));
+                    }
+                }
+            }
+            double Tv[3] = {T.r, T.g, T.b};
+            for (int k = 0; k < 3; ++k) { cl[k] *= n * Tv[k]; cb[k] *= n * Tv[k]; L[k] += cl[k] + cb[k]; }
+            if (grad) {
+                if (alb) {
+                    const float *a = s->tri_albedo + 3 * prim;
+                    for (int k = 0; k < 3; ++k) g->albedo[3 * prim + k] += delta[k] * cl[k] / a[k];
+                }
+                if (kel && spec) {
+                    double dc = 0, dlr = 0;
+                    for (int k = 0; k < 3; ++k) { dc += delta[k] * (cl[k] + cb[k]); dlr += delta[k] * (cl[k] * dll + cb[k] * dlb); }
+                    g->spec[attr] += dc / m->spec[0];
+                    g->rough[attr] += dlr * drough;
+                }
+                for (int k = 0; k < 3; ++k) R[k] -= cl[k] + cb[k];
+            }
+        }
+        Sample3D smp = bsdf_sample(&b, vn, Nn, rng_u(rng, vslot + 6), rng_u(rng, vslot + 7));
+        if (smp.pdf <= 0.f) break;
+        float ndl = dot3(Nn, v3p(smp.ray.d));
+        T = cmulc(T, cdivf(cmulf(smp.color, ndl), smp.pdf));
+        if (bounces > s->mindepth) cp *= clum(smp.color) * ndl / smp.pdf;
+        if (cblack(T)) break;
+        ray_type = spec ? RAY_SPECULAR : RAY_DIFFUSE;
+        if (bounces > s->mindepth) {
+            if (rng_u(rng, vslot + 8) > cp) break;
+            T = cdivf(T, cp);
+        }
+        if (grad) {
+            double dr = delta[0] * R[0] + delta[1] * R[1] + delta[2] * R[2];
+            if (alb) {
+                const float *a = s->tri_albedo + 3 * prim;
+                for (int k = 0; k < 3; ++k) g->albedo[3 * prim + k] += delta[k] * R[k] / a[k];
+            }
+            if (kel && spec) {
+                g->spec[attr] += dr / m->spec[0];
+                g->rough[attr] += dr * dlogd_drough(b.dist, b.rough, dot3(Nn, vnorm(vadd(v3p(smp.ray.d), vn)))) * drough;
+            }
+        }
+        smp.ray.o[0] = P.x; smp.ray.o[1] = P.y; smp.ray.o[2] = P.z;
+        ++tb;
+        ++bounces;
+        ray = smp.ray;
+        trace(s, bv, &ray, 0, &h);
+        if (h.prim < 0) break;
+    }
+    (void)cnt;
+    Lout[0] = L[0]; Lout[1] = L[1]; Lout[2] = L[2];
+    int k = 0;
+    float lc[3] = {(float)L[0], (float)L[1], (float)L[2]};
+    while (lc[0] > FIREFLY || lc[1] > FIREFLY || lc[2] > FIREFLY) { lc[0] *= 0.2f; lc[1] *= 0.2f; lc[2] *= 0.2f; ++k; }
+    return k;
+}
+
+void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_pass, int n_passes, const float *dl_dimage,
+                 const float *albedo_tri, float *g_albedo_tri, float *g_rough_mat, float *g_spec_mat, int n_threads) {
+    (void)albedo_tri;
+    int wh = s->width * s->height;
+    long long n = (long long)wh * n_passes;
+    int nt = n_threads > 0 ? n_threads : 1;
+#ifdef _OPENMP
+    omp_set_num_threads(nt);
+#else
+    nt = 1;
+#endif
+    double *acc = (double *)calloc((size_t)nt * (3 * (size_t)s->n_tris + 2 * (size_t)s->n_attrs), sizeof(double));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *base = acc + (size_t)tid * (3 * (size_t)s->n_tris + 2 * (size_t)s->n_attrs);
+        GradAcc g;
+        g.albedo = base;
+        g.rough = base + 3 * (size_t)s->n_tris;
+        g.spec = g.rough + s->n_attrs;
+#pragma omp for schedule(dynamic, 256)
+        for (long long i = 0; i < n; ++i) {
+            int pixel = (int)(i % wh), sample = first_pass + (int)(i / wh);
+            Rng r;
+            r.slots = NULL;
+            r.key = orc_path_key(seed, (uint32_t)pixel, (uint32_t)sample);
+            double L[3];
+            int k = path_replay(s, b, pixel % s->width, pixel / s->width, &r, 0, NULL, NULL, L, &g);
+            if (k < 0) continue;
+            double cf = 1.0 / n_passes;
+            float f = 1.f;
+            for (int j = 0; j < k; ++j) f *= 0.2f;
+            cf *= f;
+            double delta[3] = {dl_dimage[4 * pixel] * cf, dl_dimage[4 * pixel + 1] * cf, dl_dimage[4 * pixel + 2] * cf};
+            double L2[3];
+            path_replay(s, b, pixel % s->width, pixel / s->width, &r, 1, delta, L, L2, &g);
+        }
+    }
+    size_t np = 3 * (size_t)s->n_tris + 2 * (size_t)s->n_attrs;
+    for (size_t j = 0; j < np; ++j) {
+        double v = 0;
+        for (int t = 0; t < nt; ++t) v += acc[(size_t)t * np + j];
+        if (j < 3 * (size_t)s->n_tris) { if (g_albedo_tri) g_albedo_tri[j] = (float)v; }
+        else if (j < 3 * (size_t)s->n_tris + s->n_attrs) { if (g_rough_mat) g_rough_mat[j - 3 * s->n_tris] = (float)v; }
+        else if (g_spec_mat) g_spec_mat[j - 3 * s->n_tris - s->n_attrs] = (float)v;
+    }
+    free(acc);
+}

@@ -100,6 +100,8 @@ def lib() -> C.CDLL:
         "orc_light_sample": (None, [C.POINTER(OLight), _FP, _FP, C.c_float, C.c_float, C.POINTER(ORay), _FP, _FP]),
         "orc_light_pdf": (C.c_float, [C.POINTER(OLight), C.POINTER(ORay)]),
         "orc_light_visible": (C.c_int, [C.POINTER(OLight), _FP, _FP]),
+        "orc_adjoint": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, C.c_int, C.c_int, _FP, _FP, _FP, _FP, _FP,
+                               C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -150,7 +152,8 @@ class Scene:
                 o.diffuse[j], o.spec[j], o.gain[j], o.emission[j] = m.diffuse[j], m.spec[j], m.gain[j], m.emission[j]
             o.roughness, o.roughnessB, o.mix, o.reflectance = m.roughness, m.roughnessB, m.mix, m.reflectance
             if m.type == 2:
-                q = min(max(int(round(float(np.float32(m.reflectance)) * 20.0)), 0), 20)
+                # roundf(): halves away from zero, like the reference (kelemenMaterial.cpp:50)
+                q = min(max(int(np.floor(float(np.float32(m.reflectance)) * 20.0 + 0.5)), 0), 20)
                 t = kelemen_table(np.float32(q / 20.0), m.distribution)
                 self.tables.append(t)
                 o.table = fp(t)
@@ -212,6 +215,30 @@ class Scene:
         cnt = OCounters()
         lib().orc_render(C.byref(self.s), self.bvh, seed, first_pass, n_passes, fp(acc), C.byref(cnt), threads)
         return acc, cnt.as_dict()
+
+    def set_material_params(self, roughness=None, spec_weight=None):
+        """Mirrors aur_set_material_params (values clamped to >= 1e-6)."""
+        for k in range(len(self.exp["materials"])):
+            o = self.mats[k]
+            if roughness is not None and o.type in (1, 2):
+                o.roughness = max(float(roughness[k]), 1e-6)
+                if o.type == 1:
+                    o.roughnessB = o.roughness
+            if spec_weight is not None and o.type == 2:
+                w = max(float(spec_weight[k]), 1e-6)
+                o.spec[0] = o.spec[1] = o.spec[2] = w
+
+    def adjoint(self, seed: int, first_pass: int, n_passes: int, dldi, threads: int = 0):
+        """CPU reference gradient: (albedo grads (n_tris, 3) or None, roughness grads, spec grads)."""
+        dl = np.ascontiguousarray(dldi, np.float32).reshape(-1)
+        nt = self.tris.size // 3
+        nm = len(self.exp["materials"])
+        ga = np.zeros(3 * nt, np.float32)
+        gr = np.zeros(nm, np.float32)
+        gs = np.zeros(nm, np.float32)
+        lib().orc_adjoint(C.byref(self.s), self.bvh, seed, first_pass, n_passes, fp(dl),
+                          fp(self.albedo) if self.albedo is not None else None, fp(ga), fp(gr), fp(gs), threads)
+        return (ga.reshape(-1, 3) if self.albedo is not None else None), gr, gs
 
     def render_paths(self, seed: int, pixel, sample, slots=None, threads: int = 0):
         pixel = np.ascontiguousarray(pixel, np.int32)

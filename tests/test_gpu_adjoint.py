@@ -1,0 +1,126 @@
+"""GPU path-replay adjoint: against the CPU reference gradient, against central
+finite differences (h = 1e-3) of the GPU forward renderer, and one short
+inverse-rendering run through the torch.autograd.Function."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def kroom(tmp_path_factory):
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer
+    d = str(tmp_path_factory.mktemp("kroom"))
+    scenes.box_room(res=48, spp=8, maxdepth=4, mindepth=4, kelemen=True).write(d)
+    r = Renderer(os.path.join(d, "scene.asc"))
+    r.build()
+    info = r.info()
+    rng = np.random.default_rng(4)
+    albedo = rng.uniform(0.2, 0.8, size=(info["n_tris"], 3)).astype(np.float32)
+    spec = rng.uniform(0.2, 0.8, size=info["n_attrs"]).astype(np.float32)
+    rough = rng.uniform(0.1, 0.7, size=info["n_attrs"]).astype(np.float32)
+    r.set_tri_albedo(albedo)
+    r.set_material_params(rough, spec)
+    w = rng.uniform(-1, 1, size=(info["height"], info["width"], 3)).astype(np.float32)
+    yield r, albedo, rough, spec, w
+    r.close()
+
+
+def _loss(r, albedo, rough, spec, w, passes, seed):
+    r.set_tri_albedo(albedo)
+    r.set_material_params(rough, spec)
+    r.set_seed(seed)
+    acc, _ = r.render_passes(0, passes)
+    return float(np.sum((acc[..., :3] / passes).astype(np.float64) * w))
+
+
+def test_adjoint_matches_cpu_reference(kroom, oracle):
+    r, albedo, rough, spec, w = kroom
+    r.set_seed(13)
+    ga, gr, gs, st = r.adjoint(0, 8, w)
+    s = oracle.Scene(r.export(), tri_albedo=albedo)
+    s.set_material_params(rough, spec)
+    dl = np.zeros(w.shape[:2] + (4,), np.float32)
+    dl[..., :3] = w
+    oa, orr, osp = s.adjoint(13, 0, 8, dl, threads=8)
+    scale = np.abs(oa).max()
+    np.testing.assert_allclose(ga, oa, rtol=2e-3, atol=2e-4 * scale)
+    np.testing.assert_allclose(gs, osp, rtol=2e-3, atol=2e-4 * np.abs(osp).max())
+    np.testing.assert_allclose(gr, orr, rtol=5e-3, atol=1e-3 * np.abs(orr).max())
+    assert st["paths"] == 8 * w.shape[0] * w.shape[1]
+
+
+def test_adjoint_matches_finite_differences(kroom):
+    """Albedo and spec weight: exact for fixed random numbers.  Roughness: the
+    reference's lookup makes the image piecewise in roughness (jumps every 1/31),
+    so h = 1e-3 stays inside one cell; agreement is statistical."""
+    r, albedo, rough, spec, w = kroom
+    passes, seed, h = 16, 21, 1e-3
+    r.set_tri_albedo(albedo)
+    r.set_material_params(rough, spec)
+    r.set_seed(seed)
+    ga, gr, gs, _ = r.adjoint(0, passes, w)
+    worst = 0.0
+    for idx in np.argsort(-np.abs(ga.ravel()))[:24]:
+        t, c = divmod(int(idx), 3)
+        ap, am = albedo.copy(), albedo.copy()
+        ap[t, c] += h
+        am[t, c] -= h
+        fd = (_loss(r, ap, rough, spec, w, passes, seed) - _loss(r, am, rough, spec, w, passes, seed)) / (2 * h)
+        worst = max(worst, abs(ga[t, c] - fd) / max(abs(fd), 1e-3 * np.abs(ga).max()))
+    for m in range(len(spec)):
+        if abs(gs[m]) < 1e-2 * np.abs(gs).max():
+            continue
+        sp, sm = spec.copy(), spec.copy()
+        sp[m] += h
+        sm[m] -= h
+        fd = (_loss(r, albedo, rough, sp, w, passes, seed) - _loss(r, albedo, rough, sm, w, passes, seed)) / (2 * h)
+        worst = max(worst, abs(gs[m] - fd) / abs(fd))
+    assert worst < 1e-2, worst
+    r.set_tri_albedo(albedo)
+    r.set_material_params(rough, spec)
+
+
+def test_roughness_gradient_statistically(kroom):
+    r, albedo, rough, spec, w = kroom
+    passes, h = 256, 1e-3
+    m = int(np.argmax([abs(x) for x in rough]))
+    fds, adj = [], []
+    for seed in range(4):
+        r.set_tri_albedo(albedo)
+        r.set_material_params(rough, spec)
+        r.set_seed(100 + seed)
+        _, gr, _, _ = r.adjoint(0, passes, w)
+        adj.append(gr)
+        fd = []
+        for k in range(len(rough)):
+            rp, rm = rough.copy(), rough.copy()
+            rp[k] += h
+            rm[k] -= h
+            fd.append((_loss(r, albedo, rp, spec, w, passes, 100 + seed) -
+                       _loss(r, albedo, rm, spec, w, passes, 100 + seed)) / (2 * h))
+        fds.append(fd)
+    fds, adj = np.array(fds), np.array(adj)
+    big = np.abs(adj.mean(0)) > 0.2 * np.abs(adj.mean(0)).max()
+    se = np.sqrt(fds.var(0) / len(fds) + adj.var(0) / len(adj)) + 1e-6
+    z = np.abs(fds.mean(0) - adj.mean(0)) / se
+    assert np.all(z[big] < 4.0), (fds.mean(0), adj.mean(0), se)
+    r.set_material_params(rough, spec)
+    del m
+
+
+def test_autograd_function_reduces_loss(tmp_path):
+    torch = pytest.importorskip("torch")
+    from aurora_amd import inverse
+    p = inverse.make_problem(str(tmp_path), res=64, spp=8, depth=4)
+    hist, params = inverse.optimize(p, steps=25, lr=0.05)
+    assert np.isfinite(hist).all()
+    assert np.mean(hist[-5:]) < 0.7 * np.mean(hist[:3])
+    err0 = np.abs(p.init["albedo"] - p.truth["albedo"]).mean()
+    err1 = np.abs(params["albedo"] - p.truth["albedo"]).mean()
+    assert err1 < err0
+    p.renderer.close()
+    del torch
