@@ -10,6 +10,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Every expression in the device code rounds as written, like the reference
+// compiled for x86 without FMA: a contracted a*b + c changes the low bits of
+// dot products, which the microfacet distribution amplifies near its peak.
+// The slab tests of the traversal call fmaf() explicitly where they want it.
+#pragma clang fp contract(off)
+
 namespace aur {
 
 // ---------------------------------------------------------------------------

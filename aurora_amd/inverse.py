@@ -113,7 +113,7 @@ def optimize(p: Problem, steps: int = 200, lr: float = 0.02, log_every: int = 0)
             albedo.clamp_(0.01, 1.0)
             rough.clamp_(0.02, 0.99)
             spec.clamp_(0.001, 1.0)
-        hist.append(float(loss))
+        hist.append(float(loss.detach()))
         if log_every and it % log_every == 0:
             print("step %d loss %.6g" % (it, hist[-1]), flush=True)
     return hist, {"albedo": albedo.detach().numpy(), "rough": rough.detach().numpy(), "spec": spec.detach().numpy()}

@@ -115,12 +115,20 @@ def test_roughness_gradient_statistically(kroom):
 def test_autograd_function_reduces_loss(tmp_path):
     torch = pytest.importorskip("torch")
     from aurora_amd import inverse
-    p = inverse.make_problem(str(tmp_path), res=64, spp=8, depth=4)
-    hist, params = inverse.optimize(p, steps=25, lr=0.05)
+    p = inverse.make_problem(str(tmp_path), res=64, spp=32, depth=4)
+    hist, params = inverse.optimize(p, steps=30, lr=0.05)
     assert np.isfinite(hist).all()
-    assert np.mean(hist[-5:]) < 0.7 * np.mean(hist[:3])
-    err0 = np.abs(p.init["albedo"] - p.truth["albedo"]).mean()
-    err1 = np.abs(params["albedo"] - p.truth["albedo"]).mean()
-    assert err1 < err0
+
+    def loss(prm):  # common random numbers for a noise-free comparison
+        p.renderer.set_seed(77)
+        img = inverse.render_image(p.renderer, prm["albedo"], prm["rough"], prm["spec"], 0, 64)
+        return float(np.mean((img - p.target) ** 2))
+
+    l0, l1 = loss(p.init), loss(params)
+    assert l1 < 0.9 * l0, (l0, l1)
+    seen = np.abs(params["albedo"] - p.init["albedo"]).sum(1) > 0  # triangles the camera sees
+    err0 = np.abs(p.init["albedo"] - p.truth["albedo"])[seen].mean()
+    err1 = np.abs(params["albedo"] - p.truth["albedo"])[seen].mean()
+    assert err1 < err0, (err0, err1)
     p.renderer.close()
     del torch
