@@ -252,7 +252,8 @@ __global__ void __launch_bounds__(128) k_adjoint(SceneView sv, AdjParams ap, int
     unsigned long long n_ext = 0, n_shadow = 0, n_paths = 0;
     const int wh = sv.width * sv.height;
     for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
-        const int pixel = i % wh, sample = ap.first_pass + i / wh;
+        const int pass = i / wh;
+        const int pixel = tiled_pixel(i - pass * wh, sv.width, sv.height), sample = ap.first_pass + pass;
         PathRng rng;
         rng.slots = nullptr;
         rng.key = path_key(ap.seed, (uint32_t)pixel, (uint32_t)sample);

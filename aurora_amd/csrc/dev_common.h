@@ -101,12 +101,15 @@ __device__ inline f3 sample_hemisphere_uniform(float r1, float r2) {
     float z = r1;
     float r = sqrtf(fmaxf(0.f, 1.f - z * z));
     float phi = (2.f * AUR_PI_F) * r2;
-    return mk3(r * cosf(phi), r * sinf(phi), z);
+    float sphi, cphi;
+    sincosf(phi, &sphi, &cphi);
+    return mk3(r * cphi, r * sphi, z);
 }
 // math_utils.h:222-224
 __device__ inline float schlick(float vdoth, float refl) {
     float a = 1.f - vdoth;
-    return refl + powf(a, 5.f) * (1.f - refl);
+    float a2 = a * a;
+    return refl + (a2 * a2 * a) * (1.f - refl);
 }
 // color.h lum() / isBlack()
 __device__ inline float lum3(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
@@ -158,27 +161,37 @@ __device__ inline bool intersect_tri(f3 o, f3 d, f3 p, f3 e1, f3 e2, float tmin,
 // ---------------------------------------------------------------------------
 struct Bsdf { int kind; int dist; f3 color; float rough; };
 
-// cookTorrance.cpp:40-45: Beckmann D(h) * cos(theta_h); GGX analogue
-__device__ inline float ct_d(int dist, float rough, float cosa, float tana) {
+// tan^2(acos(c)) without the two transcendentals: (1 - c)(1 + c) / c^2.
+// 1 - c is exact for c in [0.5, 1] (Sterbenz), so the peak of the
+// distribution (c -> 1) keeps full relative precision.  c > 1 (a rounded dot
+// product) gives a negative value, which the callers treat as the NaN that
+// acos(c > 1) produces in the reference.
+__device__ inline float tan2_of_cos(float c) { return ((1.f - c) * (1.f + c)) / (c * c); }
+
+// cookTorrance.cpp:40-45: Beckmann D(h) * cos(theta_h); GGX analogue.
+// tan2 < 0 stands for the reference's NaN (acos of a cosine above one).
+__device__ inline float ct_d(int dist, float rough, float cosa, float tan2) {
     const float r2 = fmaxf(0.001f, rough * rough);
-    const float c3 = powf(cosa, 3.f);
+    const float c3 = cosa * cosa * cosa;
+    if (tan2 < 0.f) return __int_as_float(0x7fc00000);
     if (dist == DIST_GGX) {
-        float t = r2 + tana * tana;
+        float t = r2 + tan2;
         return r2 / (AUR_PI_F * c3 * t * t);
     }
-    const float e = expf(-(tana * tana) / r2);
+    const float e = expf(-tan2 / r2);
     return e / (r2 * AUR_PI_F * c3);
 }
 // cookTorrance.cpp:136 (no floor)
 __device__ inline float ct_d_eval(int dist, float rough, float ndoth) {
     const float r2 = rough * rough;
-    const float tana = tanf(acosf(ndoth));
-    const float c3 = powf(ndoth, 3.f);
+    const float tan2 = tan2_of_cos(ndoth);
+    if (tan2 < 0.f) return __int_as_float(0x7fc00000);
+    const float c3 = ndoth * ndoth * ndoth;
     if (dist == DIST_GGX) {
-        float t = r2 + tana * tana;
+        float t = r2 + tan2;
         return (1.f / (AUR_PI_F * c3 * t * t)) * r2;
     }
-    return (1.f / (r2 * AUR_PI_F * c3)) * expf(-(tana * tana) / r2);
+    return (1.f / (r2 * AUR_PI_F * c3)) * expf(-tan2 / r2);
 }
 
 struct BsdfSample { f3 dir; float pdf; f3 color; float tmax; };
@@ -194,13 +207,17 @@ __device__ inline BsdfSample bsdf_sample(const Bsdf &b, f3 vn, f3 nn, float r1, 
         s.tmax = 1000000.f;
         return s;
     }
-    // cookTorrance.cpp:47-95
-    float z;
-    if (b.dist == DIST_GGX) z = cosf(atanf(b.rough * sqrtf(r1 / (1.f - r1))));
-    else z = cosf(atanf(b.rough * sqrtf(-logf(1.f - r1))));
+    // cookTorrance.cpp:47-95.  z keeps the reference's cos(atan(x)) rounding:
+    // r = sqrt(1 - z^2) below turns one ulp of z near 1 into ~1e-5 of direction.
+    float ta;
+    if (b.dist == DIST_GGX) ta = b.rough * sqrtf(r1 / (1.f - r1));
+    else ta = b.rough * sqrtf(-logf(1.f - r1));
+    float z = cosf(atanf(ta));
     float r = sqrtf(fmaxf(0.f, 1.f - z * z));
     float phi = (2.f * AUR_PI_F) * r2;
-    f3 dir = norm3(mk3(r * cosf(phi), r * sinf(phi), z));
+    float sphi, cphi;
+    sincosf(phi, &sphi, &cphi);
+    f3 dir = norm3(mk3(r * cphi, r * sphi, z));
     f3 hn = norm3(tangent_to_world(dir, nn));
     float vdoth = dot3(vn, hn);
     f3 ln = norm3(sub3(mul3(mul3(hn, 2.f), vdoth), vn));
@@ -208,8 +225,8 @@ __device__ inline BsdfSample bsdf_sample(const Bsdf &b, f3 vn, f3 nn, float r1, 
     f3 res = mk3(0.f, 0.f, 0.f);
     float pdf = 1.f;
     if (vdoth > 0.0001f) {
-        const float tana = (ndoth > 0.999f) ? 0.f : tanf(acosf(ndoth));
-        float D = ct_d(b.dist, b.rough, ndoth, tana);
+        const float tan2 = (ndoth > 0.999f) ? 0.f : tan2_of_cos(ndoth);
+        float D = ct_d(b.dist, b.rough, ndoth, tan2);
         D = fminf(D, 1000.f);
         pdf = D / (4.f * vdoth);
         res = div3(mul3(b.color, D), 4.f * vdoth * vdoth);
@@ -225,7 +242,7 @@ __device__ inline float bsdf_pdf(const Bsdf &b, f3 ln, f3 vn, f3 nn) {
     f3 hn = norm3(add3(ln, vn));
     float vdoth = dot3(vn, hn);
     float ndoth = dot3(nn, hn);
-    float D = ct_d(b.dist, b.rough, ndoth, tanf(acosf(ndoth)));
+    float D = ct_d(b.dist, b.rough, ndoth, tan2_of_cos(ndoth));
     if (D != D) D = 0.f;
     return D / (4.f * vdoth);
 }

@@ -24,6 +24,21 @@ struct SceneView {
 
 SceneView scene_view(Ctx *c);
 
+// Path order within one pass: bands of 8 rows, each walked in 8-column tiles
+// (the last tile of a band may be narrower, the last band shorter), so the 64
+// lanes of a wave hold an 8x8 block of neighbouring pixels and their camera
+// rays walk the same part of the hierarchy.  A bijection on [0, w*h).
+__device__ __host__ inline int tiled_pixel(int local, int w, int h) {
+    const int band = local / (8 * w);
+    const int y0 = band * 8;
+    const int hb = h - y0 < 8 ? h - y0 : 8;
+    const int l = local - y0 * w;
+    const int cx = l / (hb * 8);
+    const int cw = w - cx * 8 < 8 ? w - cx * 8 : 8;
+    const int idx = l - cx * hb * 8;
+    return (y0 + idx / cw) * w + cx * 8 + idx % cw;
+}
+
 struct PathRng {
     const float *slots;
     uint32_t key;

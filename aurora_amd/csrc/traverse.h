@@ -89,14 +89,16 @@ __device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4
     const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
     st.sp = 0;
     int node = 0;
+    const float inf = __int_as_float(0x7f800000);
     for (;;) {
         NodeRegs<W> r;
         load_node<W>(nodes, node, r);
         // exact-t ties must still be visited, so the far bound is widened slightly
         const float tfar_lim = h.t * 1.0000004f;
+        // entry distance per child (inf: missed or a leaf handled right here);
+        // every index below is a compile-time constant, so the lists stay in VGPRs
         float tn[W];
         int ch[W];
-        int nin = 0;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             float x0 = fmaf(r.lox[k], ix, -ox), x1 = fmaf(r.hix[k], ix, -ox);
@@ -104,25 +106,36 @@ __device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4
             float z0 = fmaf(r.loz[k], iz, -oz), z1 = fmaf(r.hiz[k], iz, -oz);
             float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
             float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
-            if (a <= b) {
-                int c = r.child[k];
-                if (c < 0) {
-                    if (test_leaf<ANY>(tris, c, o, d, tmin, h)) return;
-                } else {
-                    // insertion into a list sorted by entry distance
-                    int j = nin++;
-                    while (j > 0 && tn[j - 1] > a) { tn[j] = tn[j - 1]; ch[j] = ch[j - 1]; --j; }
-                    tn[j] = a; ch[j] = c;
+            int c = r.child[k];
+            bool hit = a <= b;
+            tn[k] = (hit && c >= 0) ? a : inf;
+            ch[k] = c;
+            if (hit && c < 0) {
+                if (test_leaf<ANY>(tris, c, o, d, tmin, h)) return;
+            }
+        }
+        if (!ANY) {
+            // sorting network on (entry distance, child): nearest first
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+#pragma unroll
+                for (int j = (i & 1); j + 1 < W; j += 2) {
+                    bool sw = tn[j] > tn[j + 1];
+                    float ta = sw ? tn[j + 1] : tn[j], tb = sw ? tn[j] : tn[j + 1];
+                    int ca = sw ? ch[j + 1] : ch[j], cb = sw ? ch[j] : ch[j + 1];
+                    tn[j] = ta; tn[j + 1] = tb; ch[j] = ca; ch[j + 1] = cb;
                 }
             }
         }
         // children entered beyond the (possibly shortened) hit distance are skipped
         const float tcut = h.t * 1.0000004f;
         int next = -1;
-        for (int k = nin - 1; k >= 0; --k) {
-            if (tn[k] > tcut) continue;
-            if (next >= 0) st.push(next);
-            next = ch[k];
+#pragma unroll
+        for (int k = W - 1; k >= 0; --k) {
+            if (tn[k] <= tcut) {
+                if (next >= 0) st.push(next);
+                next = ch[k];
+            }
         }
         if (next >= 0) { node = next; continue; }
         if (st.sp == 0) break;

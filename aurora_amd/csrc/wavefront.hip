@@ -71,8 +71,9 @@ __device__ inline void wave_count(unsigned long long *counter, bool pred) {
 __device__ inline void path_pixel_sample(const SceneView &sv, const WfView &wv, int path, int *pixel, int *sample) {
     if (wv.src.mode == 0) {
         int wh = sv.width * sv.height;
-        *pixel = path % wh;
-        *sample = wv.src.first_pass + path / wh;
+        int pass = path / wh;
+        *pixel = tiled_pixel(path - pass * wh, sv.width, sv.height);
+        *sample = wv.src.first_pass + pass;
     } else {
         *pixel = wv.src.pixel[path];
         *sample = wv.src.sample[path];
@@ -341,8 +342,9 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sv, WfView wv, int q) {
 // ---------------------------------------------------------------------------
 // finish: firefly clamp (integrator.cpp:277-281), per-path output
 // ---------------------------------------------------------------------------
-__global__ void k_finish(WfView wv) {
+__global__ void k_finish(SceneView sv, WfView wv) {
     const int n = wv.counts[C_FINISH];
+    const int wh = sv.width * sv.height;
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         int path = wv.finq[i];
@@ -350,7 +352,14 @@ __global__ void k_finish(WfView wv) {
         f3 L = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
         if (!(wv.flags[path] & F_NOCLAMP))
             while (L.x > AUR_FIREFLY || L.y > AUR_FIREFLY || L.z > AUR_FIREFLY) L = mul3(L, 0.2f);
-        wv.out[path] = make_float4(L.x, L.y, L.z, a.w);
+        // whole passes land by (pass, pixel) for the resolve; injected paths by id
+        int slot = path;
+        if (wv.src.mode == 0) {
+            int pixel, sample;
+            path_pixel_sample(sv, wv, path, &pixel, &sample);
+            slot = (sample - wv.src.first_pass) * wh + pixel;
+        }
+        wv.out[slot] = make_float4(L.x, L.y, L.z, a.w);
     }
 }
 
@@ -529,7 +538,7 @@ static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
         TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         if (c->bvh.width == 8) TIMED(c, KT_SHADOW, hipLaunchKernelGGL(k_shadow<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv));
         else TIMED(c, KT_SHADOW, hipLaunchKernelGGL(k_shadow<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv));
-        TIMED(c, KT_FINISH, hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, c->stream, wv));
+        TIMED(c, KT_FINISH, hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, c->stream, sv, wv));
         hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, c->stream, wv.counts, q);
         q = 1 - q;
     }

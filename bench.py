@@ -12,7 +12,10 @@ and BVH resident in HBM.  ``roofline`` describes the kernel with the largest
 share of device time, timed with HIP events on the library stream.
 ``cpu_baseline`` is the CPU restatement of the reference algorithm (the
 oracle, with its own SAH BVH and OpenMP tiles) timed on this host on a bounded
-sample of the same frame.
+sample of the same frame.  ``adjoint`` is the inverse-rendering workload (the
+Kelemen box room at 512x512, 32 spp forward and adjoint, depth 6): forward and
+adjoint times, adjoint Mpaths/s, and the largest relative error of the adjoint
+gradient against central differences on 64 seeded parameters.
 """
 from __future__ import annotations
 
@@ -37,6 +40,10 @@ HBM_PEAK_GBS = 8000.0
 #   k_shadow: shadow record read (48)
 #   k_shade:  ray (32) + hit (16) + path state read/write (2 x 36) per extension ray
 ALGO_BYTES = {"k_extend": 48.0, "k_shadow": 48.0, "k_shade": 120.0}
+
+
+def _progress(msg: str):
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def _hip_sync():
@@ -71,38 +78,140 @@ def _load_traffic(kernel: str):
         return None
 
 
-def cpu_baseline(exp: dict, target_s: float, threads: int):
-    """Times the oracle on every k-th row of one pass, sized to ~target_s."""
+def cpu_baseline(exp: dict, target_s: float, threads: int, spp: int):
+    """Times the oracle on an evenly spaced subset of the frame's rows, taken
+    across the frame's passes (row g of the stacked passes is pass g // H, row
+    g % H), sized to about target_s seconds of CPU work."""
     from oracle import oracle as O
     O.build()
     t0 = time.time()
     scene = O.Scene(exp, bvh=True)
     build_s = time.time() - t0
+    _progress("oracle BVH built in %.1f s" % build_s)
     W, H = exp["width"], exp["height"]
 
-    def sample_rows(step):
-        rows = np.arange(0, H, step)
+    def sample(n_rows):
+        V = H * spp
+        n_rows = max(1, min(V, n_rows))
+        g = (np.arange(n_rows, dtype=np.int64) * V) // n_rows
+        rows, passes = g % H, g // H
         pix = (rows[:, None] * W + np.arange(W)[None, :]).reshape(-1).astype(np.int32)
-        return pix
+        smp = np.repeat(passes, W).astype(np.int32)
+        return pix, smp, n_rows
 
-    # calibrate on a small sample, then size the timed sample
-    pix = sample_rows(64)
-    t0 = time.time()
-    scene.render_paths(7, pix, np.zeros_like(pix), threads=threads)
-    dt = max(time.time() - t0, 1e-4)
-    rate = pix.size / dt
-    want = int(rate * target_s)
-    step = max(1, int(round(W * H / max(want, 1))))
-    pix = sample_rows(step)
-    smp = np.zeros_like(pix)
+    # calibrate on growing samples until one takes a measurable time, then
+    # size the timed sample from that rate
+    n_rows = 16
+    while True:
+        pix, smp, n_rows = sample(n_rows)
+        t0 = time.time()
+        scene.render_paths(7, pix, smp, threads=threads)
+        dt = time.time() - t0
+        if dt >= 0.05 * target_s or n_rows >= H * spp:
+            break
+        n_rows *= 4
+    rate = pix.size / max(dt, 1e-6)
+    pix, smp, n_rows = sample(int(rate * target_s / W))
     t0 = time.time()
     _, cnt = scene.render_paths(7, pix, smp, threads=threads)
     dt = time.time() - t0
     return {"value": round(pix.size / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": "one pass (1 spp) over every %d-th row of the same 1920x1080 frame: %d paths, %.1f s; "
-                      "BVH build %.1f s excluded; host %s, nproc %d" % (step, pix.size, dt, build_s, _cpu_model(),
-                                                                       os.cpu_count() or 0),
+            "sample": "%d evenly spaced rows of the %d stacked passes of the same %dx%d depth-%d frame "
+                      "(%d paths, %.1f s); oracle BVH build %.1f s excluded; host %s, nproc %d"
+                      % (n_rows, spp, W, H, exp["maxdepth"], pix.size, dt, build_s, _cpu_model(),
+                         os.cpu_count() or 0),
             "mrays_per_s": round((cnt["ext_rays"] + cnt["shadow_rays"]) / dt / 1e6, 4)}
+
+
+def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int = 0):
+    """Inverse-rendering workload: forward and path-replay adjoint timing on the
+    Kelemen box room, plus a central-difference check (h = 1e-3) of the
+    gradients on 64 seeded parameters with common random numbers."""
+    from aurora_amd import inverse
+    tmp = tempfile.mkdtemp(prefix="aurora_adj_")
+    p = inverse.make_problem(tmp, res=res, spp=spp, depth=depth)
+    r = p.renderer
+    rng = np.random.default_rng(3)
+    albedo = p.truth["albedo"] * rng.uniform(0.8, 1.2, p.truth["albedo"].shape).astype(np.float32)
+    rough, spec = p.init["rough"], p.init["spec"]
+    r.set_seed(5)
+    img = inverse.render_image(r, albedo, rough, spec, 0, spp)
+    dl = (2.0 / img.size) * (img.astype(np.float64) - p.target)  # d mean((I - T)^2) / dI at this seed
+    r.adjoint(0, spp, dl.astype(np.float32))
+    _hip_sync()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        inverse.render_image(r, albedo, rough, spec, 0, spp)
+    _hip_sync()
+    fwd_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ga, gr, gs, st = r.adjoint(0, spp, dl.astype(np.float32))
+    _hip_sync()
+    adj_s = (time.perf_counter() - t0) / reps
+
+    def loss(a, sw):
+        r.set_seed(5)
+        im = inverse.render_image(r, a, rough, sw, 0, spp)
+        return float(np.mean((im.astype(np.float64) - p.target) ** 2))
+
+    r.set_seed(5)
+    ga, _, gs, _ = r.adjoint(0, spp, dl.astype(np.float32))
+    # 64 seeded parameters among the albedo channels and spec weights whose
+    # gradient is not negligible (the image is linear in both, so central
+    # differences with common random numbers are exact up to rounding)
+    cand = [("a", int(k)) for k in np.flatnonzero(np.abs(ga.ravel()) > 1e-2 * np.abs(ga).max())]
+    cand += [("s", int(k)) for k in np.flatnonzero(np.abs(gs) > 1e-2 * np.abs(gs).max())]
+    pick = np.random.default_rng(64).permutation(len(cand))[:n_fd]
+    h, worst = 1e-3, 0.0
+    for j in pick:
+        kind, k = cand[j]
+        if kind == "a":
+            t, c = divmod(k, 3)
+            ap, am = albedo.copy(), albedo.copy()
+            ap[t, c] += h
+            am[t, c] -= h
+            fd = (loss(ap, spec) - loss(am, spec)) / (2 * h)
+            g, gmax = ga[t, c], np.abs(ga).max()
+        else:
+            sp, sm = spec.copy(), spec.copy()
+            sp[k] += h
+            sm[k] -= h
+            fd = (loss(albedo, sp) - loss(albedo, sm)) / (2 * h)
+            g, gmax = gs[k], np.abs(gs).max()
+        worst = max(worst, abs(g - fd) / max(abs(fd), 1e-3 * gmax))
+    n_paths = res * res * spp
+    adam = None
+    if adam_steps:
+        t0 = time.perf_counter()
+        hist, prm = inverse.optimize(p, steps=adam_steps, lr=0.02)
+        adam_s = time.perf_counter() - t0
+        seen = np.abs(prm["albedo"] - p.init["albedo"]).sum(1) > 0
+        adam = {"steps": adam_steps, "s": round(adam_s, 2), "loss_first": float("%.5g" % hist[0]),
+                "loss_last": float("%.5g" % hist[-1]),
+                "albedo_mae_init": round(float(np.abs(p.init["albedo"] - p.truth["albedo"])[seen].mean()), 4),
+                "albedo_mae_final": round(float(np.abs(prm["albedo"] - p.truth["albedo"])[seen].mean()), 4)}
+    r.close()
+    return {"adam": adam, "workload": "box_room_kelemen_%d_spp%d_depth%d" % (res, spp, depth), "params": int(ga.size + gr.size + gs.size),
+            "forward_ms": round(fwd_s * 1e3, 3), "adjoint_ms": round(adj_s * 1e3, 3),
+            "adjoint_Mpaths_s": round(n_paths / adj_s / 1e6, 3), "adjoint_vs_forward": round(adj_s / fwd_s, 3),
+            "fd_checked": int(len(pick)), "fd_h": h, "fd_max_rel_err": float("%.3g" % worst)}
+
+
+def copy_calibration(mib: int, reps: int):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "reps": reps}), flush=True)
+    return 0
 
 
 def main(argv=None):
@@ -116,7 +225,14 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--levels", type=int, default=7)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adam-steps", type=int, default=200,
+                    help="Adam steps of the inverse-rendering run inside the adjoint leg (0: skip)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
+    if args.calibrate_copy_mib:
+        return copy_calibration(args.calibrate_copy_mib, 3)
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -133,6 +249,7 @@ def main(argv=None):
     t0 = time.time()
     r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
     build_s = time.time() - t0
+    _progress("scene and BVH built in %.2f s" % build_s)
     info = r.info()
     wh = info["width"] * info["height"]
     P = args.passes_per_step
@@ -149,6 +266,7 @@ def main(argv=None):
             tot[key] += st[key]
     _hip_sync()
     elapsed = time.perf_counter() - t0
+    _progress("timed %d steps in %.3f s" % (args.steps, elapsed))
     ks = r.kernel_stats()
     r.set_profiling(False)
 
@@ -181,12 +299,17 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": None,
     }
-    if not args.no_cpu:
+    exp = None if args.no_cpu else r.export()
+    r.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(512, 32, 6, adam_steps=args.adam_steps)
+        _progress("adjoint leg done")
+    if exp is not None:
         threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(r.export(), args.cpu_seconds, threads)
+        out["cpu_baseline"] = cpu_baseline(exp, args.cpu_seconds, threads, max(1, info["pixelsamples"]))
+        _progress("cpu baseline done")
         out["gpu_vs_cpu"] = round(value / max(out["cpu_baseline"]["value"], 1e-9), 2)
     print(json.dumps(out), flush=True)
-    r.close()
     return 0
 
 
