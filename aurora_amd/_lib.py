@@ -96,6 +96,26 @@ class BsdfResult(C.Structure):
                 ("sample_color", C.c_float * 3)]
 
 
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("first_pass", C.c_int), ("n_passes", C.c_int)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("paths", C.c_int64), ("ext_rays", C.c_int64), ("hit_rays", C.c_int64), ("shadow_rays", C.c_int64),
+                ("latency_ms", C.c_double), ("order", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("order", C.c_int), ("use_graph", C.c_int), ("iters_per_poll", C.c_int)]
+
+
+class LaunchOpts(C.Structure):
+    _fields_ = [("trace_block", C.c_int), ("trace_waves_per_cu", C.c_int), ("pool_paths", C.c_int)]
+
+
 _P = C.c_void_p
 _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int)
@@ -115,6 +135,11 @@ SIGNATURES = {
     "aur_scene_export": (C.c_int, [_P, _FP, _IP, _FP, _IP, C.POINTER(Material), C.POINTER(Light)]),
     "aur_render": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(Stats)]),
     "aur_render_paths": (C.c_int, [_P, _IP, _IP, C.c_int, _FP, _FP, C.POINTER(Stats)]),
+    "aur_framebuffer": (C.c_int, [_P, _FP]),
+    "aur_batch_submit": (C.c_int, [_P, C.POINTER(Job), C.c_int, C.POINTER(BatchOpts), C.POINTER(JobResult),
+                                   C.POINTER(Stats)]),
+    "aur_set_launch": (C.c_int, [_P, C.POINTER(LaunchOpts)]),
+    "aur_get_launch": (C.c_int, [_P, C.POINTER(LaunchOpts)]),
     "aur_set_material_params": (C.c_int, [_P, _FP, _FP]),
     "aur_adjoint": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(ParamGrads), C.POINTER(Stats)]),
     "aur_set_profiling": (C.c_int, [_P, C.c_int]),

@@ -512,6 +512,77 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
     });
 }
 
+int aur_framebuffer(aur_ctx *ctx, float *rgba_sum_host) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(rgba_sum_host);
+        Ctx *c = &ctx->c;
+        set_device(c);
+        if (!c->accum.p || c->fb_w <= 0) throw StateError("no framebuffer yet (render first)");
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipMemcpy(rgba_sum_host, c->accum.p, sizeof(float) * 4 * (size_t)c->fb_w * c->fb_h, hipMemcpyDeviceToHost));
+    });
+}
+
+int aur_batch_submit(aur_ctx *exec, const aur_job *jobs, int n_jobs, const aur_batch_opts *opts, aur_job_result *results,
+                     aur_stats *total) {
+    return guarded([&] {
+        NEED(exec);
+        NEED(jobs);
+        NEED(results);
+        if (n_jobs <= 0) throw std::invalid_argument("n_jobs must be positive");
+        Ctx *c = &exec->c;
+        set_device(c);
+        std::vector<Ctx *> cs(n_jobs);
+        std::vector<int> fp(n_jobs), np(n_jobs);
+        for (int k = 0; k < n_jobs; ++k) {
+            if (!jobs[k].scene) throw std::invalid_argument("job " + std::to_string(k) + ": null scene");
+            cs[k] = &jobs[k].scene->c;
+            fp[k] = jobs[k].first_pass;
+            np[k] = jobs[k].n_passes;
+        }
+        aur_batch_opts o;
+        o.order = 0;
+        o.use_graph = 1;
+        o.iters_per_poll = 8;
+        if (opts) o = *opts;
+        if (o.order != 0 && o.order != 1) throw std::invalid_argument("order must be 0 or 1");
+        std::memset(results, 0, sizeof(aur_job_result) * n_jobs);
+        aur_stats st;
+        std::memset(&st, 0, sizeof st);
+        run_jobs(c, cs.data(), fp.data(), np.data(), n_jobs, o.order, o.use_graph, o.iters_per_poll, results, &st);
+        if (total) *total = st;
+    });
+}
+
+int aur_set_launch(aur_ctx *ctx, const aur_launch_opts *opts) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(opts);
+        LaunchCfg &l = ctx->c.launch;
+        if (opts->trace_block > 0) {
+            if (opts->trace_block != 64 && opts->trace_block != 128 && opts->trace_block != 256)
+                throw std::invalid_argument("trace_block must be 64, 128 or 256");
+        }
+        if (opts->trace_waves_per_cu > 64) throw std::invalid_argument("trace_waves_per_cu must be 1..64");
+        if (opts->pool_paths > 0 && opts->pool_paths < 64) throw std::invalid_argument("pool_paths must be >= 64");
+        if (opts->trace_block > 0) l.trace_block = opts->trace_block;
+        if (opts->trace_waves_per_cu > 0) l.trace_waves_per_cu = opts->trace_waves_per_cu;
+        if (opts->pool_paths > 0) l.batch_pool_paths = (size_t)opts->pool_paths;
+    });
+}
+
+int aur_get_launch(aur_ctx *ctx, aur_launch_opts *out) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(out);
+        const LaunchCfg &l = ctx->c.launch;
+        out->trace_block = l.trace_block;
+        out->trace_waves_per_cu = l.trace_waves_per_cu;
+        out->pool_paths = (int)std::min<size_t>(l.batch_pool_paths, 0x7fffffff);
+    });
+}
+
 int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_paths, const float *slots, float *rgba_out,
                      aur_stats *out) {
     return guarded([&] {

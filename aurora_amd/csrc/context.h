@@ -105,6 +105,26 @@ struct KernelTimer {
     long long launches[KT_COUNT] = {0};
 };
 
+// Launch shape of the tracer kernels (set by aur_set_launch / the autotuner).
+struct LaunchCfg {
+    int trace_block = 128;          // traversal workgroup size: 64, 128 or 256
+    int trace_waves_per_cu = 16;    // persistent traversal grid, waves per CU
+    size_t batch_pool_paths = 4u << 20;  // path slots in flight in a batched launch
+};
+
+// Device state of a batched launch (batch.hip), kept between calls.
+struct BatchState {
+    DevBuf jobs, starts, remaining, t_end, jstats, cursor, t0;
+    DevBuf pjob, ppix, psmp;        // per path slot
+    void *host_status = nullptr;    // pinned: cursor + queue counts, polled by the host
+    BatchState() = default;
+    BatchState(const BatchState &) = delete;
+    BatchState &operator=(const BatchState &) = delete;
+    ~BatchState() {
+        if (host_status) (void)hipHostFree(host_status);
+    }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -133,6 +153,8 @@ struct Ctx {
 
     Wavefront wf;
     KernelTimer timer;
+    LaunchCfg launch;
+    BatchState batch;
 };
 
 // kernel timing helpers (wavefront.hip)
@@ -155,6 +177,11 @@ struct PathSource {
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats);
 void resolve_passes(Ctx *c, const float4 *out_rgba, int n_passes, float4 *accum);
 void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_hit);
+// sizes the path pool and the traversal stack overflow for BVHs of the given width and depth
+void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth);
+// batch.hip: jobs[k] renders passes [first_pass[k], first_pass[k] + n_passes[k]) into its framebuffer
+void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_passes, int n_jobs, int order, int use_graph,
+              int iters_per_poll, aur_job_result *results, aur_stats *total_stats);
 // adjoint.hip
 void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *d_galb, float *d_grough, float *d_gspec,
              aur_stats *stats);

@@ -19,54 +19,9 @@
 // lane its offset (mbcnt) and the wave's count (popcount).  Every kernel is
 // a grid-stride loop over a count that lives in device memory, so a whole
 // frame is a fixed sequence of launches with no host round trip.
-#include "context.h"
-#include "shade_common.h"
-#include "traverse.h"
+#include "wavefront_common.h"
 
 namespace aur {
-
-enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2 };
-enum { F_MATTE = 1 << 2, F_NOCLAMP = 1 << 24 };
-enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_FINISH = 3 };
-enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
-
-struct WfView {
-    float4 *ray[2];
-    int *qpath[2];
-    float4 *hit;
-    float4 *T, *La, *Lb;
-    int *flags;
-    float4 *sray;
-    int *finq;
-    float4 *out;
-    int *counts;
-    unsigned long long *stats;
-    int *ovf;
-    int ovf_per_thread;
-    PathSource src;
-    uint32_t seed;
-    int nslots;
-};
-
-// ---------------------------------------------------------------------------
-// wave-level helpers (wave64)
-// ---------------------------------------------------------------------------
-__device__ inline int wave_reserve(int *counter, bool pred) {
-    unsigned long long mask = __ballot(pred);
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (mask && lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader < 0 ? 0 : leader);
-    return base + __popcll(mask & ((1ull << lane) - 1ull));
-}
-
-__device__ inline void wave_count(unsigned long long *counter, bool pred) {
-    unsigned long long mask = __ballot(pred);
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    if (mask && lane == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
-}
 
 __device__ inline void path_pixel_sample(const SceneView &sv, const WfView &wv, int path, int *pixel, int *sample) {
     if (wv.src.mode == 0) {
@@ -98,12 +53,7 @@ __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
         int pixel, sample;
         path_pixel_sample(sv, wv, i, &pixel, &sample);
         PathRng rng = path_rng(sv, wv, i);
-        int x = pixel % sv.width, y = pixel / sv.width;
-        float fovx = sv.fov;
-        float fovy = ((float)sv.height / (float)sv.width) * fovx;
-        float u = x + rng.u(0), v = y + rng.u(1);
-        f3 d = mk3(((2.f * u - sv.width) / (float)sv.width) * fovx, ((-2.f * v + sv.height) / (float)sv.height) * fovy, -1.f);
-        d = norm3(d);
+        f3 d = camera_dir(sv, pixel, rng.u(0), rng.u(1));
         wv.ray[0][2 * i] = make_float4(0.f, 0.f, 0.f, 0.1f);
         wv.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
         wv.qpath[0][i] = i;
@@ -121,16 +71,16 @@ __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
 // ---------------------------------------------------------------------------
 // extend: closest hit
 // ---------------------------------------------------------------------------
-template <int W>
-__global__ void __launch_bounds__(128) k_extend(SceneView sv, WfView wv, int q) {
-    __shared__ int s_stack[AUR_STACK_LDS * 128];
+template <int W, int B>
+__global__ void __launch_bounds__(B) k_extend(SceneView sv, WfView wv, int q) {
+    __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_EXT], (unsigned long long)n);
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
     st.lds = s_stack + threadIdx.x;
-    st.stride = 128;
+    st.stride = B;
     st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     for (int i = gtid; i < n; i += stride) {
@@ -146,16 +96,16 @@ __global__ void __launch_bounds__(128) k_extend(SceneView sv, WfView wv, int q) 
 // shadow: any hit; unoccluded contributions go to La (light sample) or Lb
 // (BSDF sample)
 // ---------------------------------------------------------------------------
-template <int W>
-__global__ void __launch_bounds__(128) k_shadow(SceneView sv, WfView wv) {
-    __shared__ int s_stack[AUR_STACK_LDS * 128];
+template <int W, int B>
+__global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv) {
+    __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_SHADOW], (unsigned long long)n);
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
     st.lds = s_stack + threadIdx.x;
-    st.stride = 128;
+    st.stride = B;
     st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     for (int i = gtid; i < n; i += stride) {
@@ -174,169 +124,13 @@ __global__ void __launch_bounds__(128) k_shadow(SceneView sv, WfView wv) {
 }
 
 // ---------------------------------------------------------------------------
-// shade: one path vertex (integrator.cpp:180-274)
+// shade: one path vertex per queue entry
 // ---------------------------------------------------------------------------
-struct ShadowOut { bool valid; f3 o, d; float tmin, tmax; f3 c; };
-
 __global__ void __launch_bounds__(256) k_shade(SceneView sv, WfView wv, int q) {
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
-    const int nq = 1 - q;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int path = wv.qpath[q][i];
-        const float4 ra = wv.ray[q][2 * i], rb = wv.ray[q][2 * i + 1];
-        const float4 hv = wv.hit[i];
-        const int prim = __float_as_int(hv.w);
-        int flags = wv.flags[path];
-        const int rtype = flags & 3;
-        const int tb = (flags >> 8) & 0xff;
-        bool cont = false, finish = false;
-        ShadowOut sh[2];
-        sh[0].valid = sh[1].valid = false;
-        f3 P = mk3(0.f, 0.f, 0.f), ndir = mk3(0.f, 0.f, 0.f);
-        float ntmax = 0.f;
-        if (prim < 0) {
-            finish = true;
-            if (rtype == RT_CAMERA) {  // integrator.cpp:287-297: camera miss sees the environment
-                wv.La[path] = make_float4(sv.env[0], sv.env[1], sv.env[2], sv.has_env ? 1.f : 0.f);
-                flags |= F_NOCLAMP;
-            }
-        } else if (tb >= sv.maxdepth) {
-            finish = true;  // the extension ray after the last vertex is traced but not shaded
-        } else {
-            PathRng rng = path_rng(sv, wv, path);
-            const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
-            // hit record: embreeAccelerator.cpp:111-136
-            const f3 o = mk3(ra.x, ra.y, ra.z), d = mk3(rb.x, rb.y, rb.z);
-            const float t = hv.x, u = hv.y, v = hv.z;
-            const float w = 1.f - u - v;
-            P = add3(o, mul3(d, t));
-            const float4 s0 = sv.shade[3 * prim], s1 = sv.shade[3 * prim + 1], s2 = sv.shade[3 * prim + 2];
-            const int attr = __float_as_int(s0.w);
-            f3 nm = mk3(w * s0.x + u * s1.x + v * s2.x, w * s0.y + u * s1.y + v * s2.y, w * s0.z + u * s1.z + v * s2.z);
-            if (dot3(nm, d) > 0.f) nm = mul3(nm, -1.f);
-            f3 nn = norm3(nm);
-            // integrator.cpp:132-150
-            const f3 vn = neg3(norm3(d));
-            if (dot3(vn, nn) <= 0.f) nn = norm3(cross3(vn, cross3(nn, vn)));
-            bool matte = (flags & F_MATTE) != 0;
-            if (rtype == RT_DIFFUSE) matte = true;
-            float4 T4 = wv.T[path];
-            f3 T = mk3(T4.x, T4.y, T4.z);
-            float cp = T4.w;
-            const DMat m = sv.mats[attr];
-            if (rtype == RT_CAMERA) {
-                float4 la = wv.La[path];
-                la.x += T.x * m.emission[0]; la.y += T.y * m.emission[1]; la.z += T.z * m.emission[2];
-                la.w = 1.f;
-                wv.La[path] = la;
-            }
-            if (m.type == MAT_LIGHT) {
-                finish = true;  // ConstantMaterial: getSample has pdf 0 (constant.cpp:23-25)
-            } else {
-                Bsdf b = get_brdf(sv, m, prim, vn, nn, matte, rng.u(vslot + 0));
-                // integrateDirect, integrator.cpp:100-130
-                int nvis = 0;
-                for (int li = 0; li < sv.n_lights; ++li) nvis += light_visible(sv.lights[li], P, nn) ? 1 : 0;
-                if (nvis) {
-                    int pick = min((int)(rng.u(vslot + 1) * nvis), nvis - 1);
-                    int li = 0;
-                    for (int k = 0; li < sv.n_lights; ++li) {
-                        if (light_visible(sv.lights[li], P, nn)) {
-                            if (k == pick) break;
-                            ++k;
-                        }
-                    }
-                    const DLight &l = sv.lights[li];
-                    const float scale = (float)nvis;
-                    LightSample ls = light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
-                    float ct = dot3(nn, ls.dir);
-                    if (b.kind == BSDF_LAMBERT) {
-                        // integrateDirectLight, integrator.cpp:33-54
-                        if (ct >= 0.f && ls.pdf > 0.f) {
-                            f3 f = bsdf_eval(b, ls.dir, vn, nn);
-                            f3 c = div3(mul3(mul3v(ls.color, f), ct), ls.pdf);
-                            sh[0].valid = true; sh[0].o = P; sh[0].d = ls.dir; sh[0].tmin = AUR_RAY_BIAS; sh[0].tmax = ls.tmax;
-                            sh[0].c = mul3v(mul3(c, scale), T);
-                        }
-                    } else {
-                        // integrateDirectMIS, integrator.cpp:56-98
-                        if (ct > 0.f && ls.pdf > 0.f) {
-                            float bpdf = bsdf_pdf(b, ls.dir, vn, nn);
-                            if (bpdf > 0.f) {
-                                float wgt = power_heuristic(ls.pdf, bpdf);
-                                f3 f = bsdf_eval(b, ls.dir, vn, nn);
-                                f3 c = div3(mul3(mul3v(mul3(ls.color, wgt), f), ct), ls.pdf);
-                                sh[0].valid = true; sh[0].o = P; sh[0].d = ls.dir; sh[0].tmin = AUR_RAY_BIAS; sh[0].tmax = ls.tmax;
-                                sh[0].c = mul3v(mul3(c, scale), T);
-                            }
-                        }
-                        BsdfSample bs = bsdf_sample(b, vn, nn, rng.u(vslot + 4), rng.u(vslot + 5));
-                        float ct2 = dot3(nn, bs.dir);
-                        if (ct2 > 0.f) {
-                            float tmax = bs.tmax;
-                            float lpdf = light_pdf(l, P, bs.dir, AUR_RAY_BIAS, &tmax);
-                            if (lpdf > 0.f && bs.pdf > 0.f) {
-                                float wgt = power_heuristic(bs.pdf, lpdf);
-                                f3 le = mk3(l.Le[0], l.Le[1], l.Le[2]);
-                                f3 c = div3(mul3(mul3v(mul3(bs.color, wgt), le), ct2), bs.pdf);
-                                sh[1].valid = true; sh[1].o = P; sh[1].d = bs.dir; sh[1].tmin = AUR_RAY_BIAS; sh[1].tmax = tmax;
-                                sh[1].c = mul3v(mul3(c, scale), T);
-                            }
-                        }
-                    }
-                }
-                // continuation, integrator.cpp:208-257
-                BsdfSample s = bsdf_sample(b, vn, nn, rng.u(vslot + 6), rng.u(vslot + 7));
-                if (s.pdf <= 0.f) {
-                    finish = true;
-                } else {
-                    float ndl = dot3(nn, s.dir);
-                    T = mul3v(T, div3(mul3(s.color, ndl), s.pdf));
-                    const int bounces = (flags >> 16) & 0xff;
-                    if (bounces > sv.mindepth) cp *= lum3(s.color) * ndl / s.pdf;
-                    if (black3(T)) {
-                        finish = true;
-                    } else {
-                        int nrt = (b.kind == BSDF_CT) ? RT_SPECULAR : RT_DIFFUSE;
-                        bool killed = false;
-                        if (bounces > sv.mindepth) {
-                            if (rng.u(vslot + 8) > cp) killed = true;
-                            else T = div3(T, cp);
-                        }
-                        if (killed) {
-                            finish = true;
-                        } else {
-                            cont = true;
-                            ndir = s.dir;
-                            ntmax = s.tmax;
-                            flags = nrt | (matte ? F_MATTE : 0) | ((tb + 1) << 8) | ((bounces + 1) << 16);
-                            wv.T[path] = make_float4(T.x, T.y, T.z, cp);
-                        }
-                    }
-                }
-            }
-        }
-        wv.flags[path] = flags;
-        // compacted writes: next extension queue, shadow queue, finish list
-        int slot = wave_reserve(&wv.counts[nq], cont);
-        if (cont) {
-            wv.ray[nq][2 * slot] = make_float4(P.x, P.y, P.z, AUR_RAY_BIAS);
-            wv.ray[nq][2 * slot + 1] = make_float4(ndir.x, ndir.y, ndir.z, ntmax);
-            wv.qpath[nq][slot] = path;
-        }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            int ss = wave_reserve(&wv.counts[C_SHADOW], sh[k].valid);
-            if (sh[k].valid) {
-                wv.sray[3 * ss] = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].tmin);
-                wv.sray[3 * ss + 1] = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].tmax);
-                wv.sray[3 * ss + 2] = make_float4(sh[k].c.x, sh[k].c.y, sh[k].c.z, __int_as_float(path | (k ? (int)0x80000000 : 0)));
-            }
-        }
-        int fs = wave_reserve(&wv.counts[C_FINISH], finish);
-        if (finish) wv.finq[fs] = path;
-    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        shade_one(sv, wv, q, i, [&](int path) { return path_rng(sv, wv, path); });
 }
 
 // ---------------------------------------------------------------------------
@@ -350,8 +144,7 @@ __global__ void k_finish(SceneView sv, WfView wv) {
         int path = wv.finq[i];
         float4 a = wv.La[path], b = wv.Lb[path];
         f3 L = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
-        if (!(wv.flags[path] & F_NOCLAMP))
-            while (L.x > AUR_FIREFLY || L.y > AUR_FIREFLY || L.z > AUR_FIREFLY) L = mul3(L, 0.2f);
+        L = firefly_clamp(L, wv.flags[path]);
         // whole passes land by (pass, pixel) for the resolve; injected paths by id
         int slot = path;
         if (wv.src.mode == 0) {
@@ -479,11 +272,20 @@ static int persistent_blocks(Ctx *c, int block) {
     return c->n_cu * per_cu;
 }
 
-static void ensure_wavefront(Ctx *c, size_t cap) {
+// Persistent grid of the traversal kernels: trace_waves_per_cu waves per CU
+// in workgroups of trace_block lanes.
+int trace_blocks(const Ctx *c) {
+    return c->n_cu * std::max(1, c->launch.trace_waves_per_cu * 64 / c->launch.trace_block);
+}
+size_t trace_threads(const Ctx *c) { return (size_t)trace_blocks(c) * c->launch.trace_block; }
+
+void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     Wavefront &w = c->wf;
-    int need = (c->bvh.width - 1) * (c->bvh.depth + 1) + 2;
+    int need = (width - 1) * (depth + 1) + 2;
     size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 0;
-    if (cap <= w.cap && ovf <= w.stack_ovf_per_thread && w.counts.p) return;
+    if (cap <= w.cap && ovf <= w.stack_ovf_per_thread && w.counts.p &&
+        w.stack_ovf.bytes >= sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1))
+        return;
     cap = std::max(cap, w.cap);
     w.cap = cap;
     for (int k = 0; k < 2; ++k) {
@@ -500,12 +302,13 @@ static void ensure_wavefront(Ctx *c, size_t cap) {
     w.out.reserve(sizeof(float4) * cap);
     w.counts.reserve(sizeof(int) * 16);
     w.stats.reserve(sizeof(unsigned long long) * 8);
-    size_t threads = (size_t)persistent_blocks(c, 128) * 128;
     w.stack_ovf_per_thread = std::max(ovf, w.stack_ovf_per_thread);
-    w.stack_ovf.reserve(sizeof(int) * threads * std::max<size_t>(w.stack_ovf_per_thread, 1));
+    w.stack_ovf.reserve(sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1));
 }
 
-static WfView wf_view(Ctx *c, const PathSource &src) {
+static void ensure_wavefront(Ctx *c, size_t cap) { ensure_wavefront_for(c, cap, c->bvh.width, c->bvh.depth); }
+
+WfView wf_view(Ctx *c, const PathSource &src) {
     Wavefront &w = c->wf;
     WfView v;
     for (int k = 0; k < 2; ++k) { v.ray[k] = w.ray[k].as<float4>(); v.qpath[k] = w.qpath[k].as<int>(); }
@@ -530,14 +333,16 @@ static WfView wf_view(Ctx *c, const PathSource &src) {
 static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
     const int gen_blocks = std::min((n + 255) / 256, persistent_blocks(c, 256));
     TIMED(c, KT_GENERATE, hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, sv, wv, n));
-    const int tb = persistent_blocks(c, 128), sb = persistent_blocks(c, 256);
+    const int tb = trace_blocks(c), sb = persistent_blocks(c, 256);
     int q = 0;
     for (int it = 0; it <= sv.maxdepth; ++it) {
-        if (c->bvh.width == 8) TIMED(c, KT_EXTEND, hipLaunchKernelGGL(k_extend<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q));
-        else TIMED(c, KT_EXTEND, hipLaunchKernelGGL(k_extend<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv, q));
+#define AUR_EXT(w_, b_) hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q)
+#define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv)
+        TIMED(c, KT_EXTEND, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
         TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
-        if (c->bvh.width == 8) TIMED(c, KT_SHADOW, hipLaunchKernelGGL(k_shadow<8>, dim3(tb), dim3(128), 0, c->stream, sv, wv));
-        else TIMED(c, KT_SHADOW, hipLaunchKernelGGL(k_shadow<4>, dim3(tb), dim3(128), 0, c->stream, sv, wv));
+        TIMED(c, KT_SHADOW, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_SHD));
+#undef AUR_EXT
+#undef AUR_SHD
         TIMED(c, KT_FINISH, hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, c->stream, sv, wv));
         hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, c->stream, wv.counts, q);
         q = 1 - q;
@@ -603,14 +408,14 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
 // ---------------------------------------------------------------------------
 // aur_trace: batch closest-hit / any-hit queries for the parity tests
 // ---------------------------------------------------------------------------
-template <int W>
-__global__ void __launch_bounds__(128) k_trace(SceneView sv, const aur_ray *__restrict__ rays, int n, aur_hit *__restrict__ hits,
-                                               int any_hit, int *ovf, int ovf_per_thread) {
-    __shared__ int s_stack[AUR_STACK_LDS * 128];
+template <int W, int B>
+__global__ void __launch_bounds__(B) k_trace(SceneView sv, const aur_ray *__restrict__ rays, int n, aur_hit *__restrict__ hits,
+                                             int any_hit, int *ovf, int ovf_per_thread) {
+    __shared__ int s_stack[AUR_STACK_LDS * B];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TravStack st;
     st.lds = s_stack + threadIdx.x;
-    st.stride = 128;
+    st.stride = B;
     st.ovf = ovf + (size_t)gtid * ovf_per_thread;
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     for (int i = gtid; i < n; i += gridDim.x * blockDim.x) {
@@ -628,14 +433,14 @@ void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_h
     if (!c->built) throw StateError("trace before aur_build_accel");
     ensure_wavefront(c, 1);
     SceneView sv = scene_view(c);
-    int blocks = std::min((n + 127) / 128, persistent_blocks(c, 128));
+    const int B = c->launch.trace_block;
+    int blocks = std::min((n + B - 1) / B, trace_blocks(c));
     if (blocks < 1) return;
-    if (c->bvh.width == 8)
-        hipLaunchKernelGGL(k_trace<8>, dim3(blocks), dim3(128), 0, c->stream, sv, d_rays, n, d_hits, any_hit,
-                           c->wf.stack_ovf.as<int>(), (int)c->wf.stack_ovf_per_thread);
-    else
-        hipLaunchKernelGGL(k_trace<4>, dim3(blocks), dim3(128), 0, c->stream, sv, d_rays, n, d_hits, any_hit,
-                           c->wf.stack_ovf.as<int>(), (int)c->wf.stack_ovf_per_thread);
+#define AUR_TRC(w_, b_)                                                                                      \
+    hipLaunchKernelGGL((k_trace<w_, b_>), dim3(blocks), dim3(b_), 0, c->stream, sv, d_rays, n, d_hits, any_hit, \
+                       c->wf.stack_ovf.as<int>(), (int)c->wf.stack_ovf_per_thread)
+    AUR_TRAV_DISPATCH(c->bvh.width, B, AUR_TRC);
+#undef AUR_TRC
     HIP_CHECK(hipGetLastError());
 }
 

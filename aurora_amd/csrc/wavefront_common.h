@@ -1,0 +1,276 @@
+// wavefront_common.h -- pieces of the wavefront tracer shared by the
+// single-scene launch (wavefront.hip) and the batched persistent launch
+// (batch.hip): queue layout, wave64 compaction helpers, the camera ray and
+// the per-vertex shading step.
+#pragma once
+
+#include "context.h"
+#include "shade_common.h"
+#include "traverse.h"
+
+namespace aur {
+
+// Instantiates a traversal kernel launch for the BVH width and workgroup size
+// chosen at run time; CALL(w, b) expands to the launch.
+#define AUR_TRAV_DISPATCH(width, block, CALL)                               \
+    do {                                                                    \
+        const int w__ = (width), b__ = (block);                             \
+        if (w__ == 8) {                                                     \
+            if (b__ == 64) CALL(8, 64);                                     \
+            else if (b__ == 256) CALL(8, 256);                              \
+            else CALL(8, 128);                                              \
+        } else {                                                            \
+            if (b__ == 64) CALL(4, 64);                                     \
+            else if (b__ == 256) CALL(4, 256);                              \
+            else CALL(4, 128);                                              \
+        }                                                                   \
+    } while (0)
+
+int trace_blocks(const Ctx *c);
+size_t trace_threads(const Ctx *c);
+
+enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2 };
+enum { F_MATTE = 1 << 2, F_NOCLAMP = 1 << 24 };
+enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_FINISH = 3 };
+enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
+
+struct WfView {
+    float4 *ray[2];
+    int *qpath[2];
+    float4 *hit;
+    float4 *T, *La, *Lb;
+    int *flags;
+    float4 *sray;
+    int *finq;
+    float4 *out;
+    int *counts;
+    unsigned long long *stats;
+    int *ovf;
+    int ovf_per_thread;
+    PathSource src;
+    uint32_t seed;
+    int nslots;
+};
+
+WfView wf_view(Ctx *c, const PathSource &src);
+
+// ---------------------------------------------------------------------------
+// wave-level helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline int wave_reserve(int *counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (mask && lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ inline void wave_count(unsigned long long *counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    if (mask && lane == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
+// Counts predicate lanes per job: the wave's first job takes one atomic for
+// all its lanes (the usual case: a wave holds consecutive paths of one job),
+// lanes of other jobs add one each.
+__device__ inline void wave_count_job(unsigned long long *cnt, int stride, int job, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    if (!mask) return;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    int j0 = __shfl(job, leader);
+    unsigned long long same = __ballot(pred && job == j0);
+    if (lane == leader) atomicAdd(&cnt[(size_t)j0 * stride], (unsigned long long)__popcll(same));
+    if (pred && job != j0) atomicAdd(&cnt[(size_t)job * stride], 1ull);
+}
+
+// camera.cpp:34-62: pixel (x, y) jittered by two uniforms, pinhole at the
+// origin looking down -z; the ray spans [0.1, 1e7].
+__device__ inline f3 camera_dir(const SceneView &sv, int pixel, float r0, float r1) {
+    int x = pixel % sv.width, y = pixel / sv.width;
+    float fovx = sv.fov;
+    float fovy = ((float)sv.height / (float)sv.width) * fovx;
+    float u = x + r0, v = y + r1;
+    f3 d = mk3(((2.f * u - sv.width) / (float)sv.width) * fovx, ((-2.f * v + sv.height) / (float)sv.height) * fovy, -1.f);
+    return norm3(d);
+}
+
+// integrator.cpp:277-281: the firefly clamp, skipped for camera rays that saw
+// the environment directly.
+__device__ inline f3 firefly_clamp(f3 L, int flags) {
+    if (!(flags & F_NOCLAMP))
+        while (L.x > AUR_FIREFLY || L.y > AUR_FIREFLY || L.z > AUR_FIREFLY) L = mul3(L, 0.2f);
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// shade: one path vertex (integrator.cpp:180-274) for queue entry i of queue
+// q.  make_rng(path) gives the path's random-number source.  Writes the next
+// extension ray, up to two shadow rays and the finish entry with wave-level
+// compaction, so every active lane of a wave must call it.
+// ---------------------------------------------------------------------------
+struct ShadowOut { bool valid; f3 o, d; float tmin, tmax; f3 c; };
+
+template <class MakeRng>
+__device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng) {
+    const int nq = 1 - q;
+    const int path = wv.qpath[q][i];
+    const float4 ra = wv.ray[q][2 * i], rb = wv.ray[q][2 * i + 1];
+    const float4 hv = wv.hit[i];
+    const int prim = __float_as_int(hv.w);
+    int flags = wv.flags[path];
+    const int rtype = flags & 3;
+    const int tb = (flags >> 8) & 0xff;
+    bool cont = false, finish = false;
+    ShadowOut sh[2];
+    sh[0].valid = sh[1].valid = false;
+    f3 P = mk3(0.f, 0.f, 0.f), ndir = mk3(0.f, 0.f, 0.f);
+    float ntmax = 0.f;
+    if (prim < 0) {
+        finish = true;
+        if (rtype == RT_CAMERA) {  // integrator.cpp:287-297: camera miss sees the environment
+            wv.La[path] = make_float4(sv.env[0], sv.env[1], sv.env[2], sv.has_env ? 1.f : 0.f);
+            flags |= F_NOCLAMP;
+        }
+    } else if (tb >= sv.maxdepth) {
+        finish = true;  // the extension ray after the last vertex is traced but not shaded
+    } else {
+        const PathRng rng = make_rng(path);
+        const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
+        // hit record: embreeAccelerator.cpp:111-136
+        const f3 o = mk3(ra.x, ra.y, ra.z), d = mk3(rb.x, rb.y, rb.z);
+        const float t = hv.x, u = hv.y, v = hv.z;
+        const float w = 1.f - u - v;
+        P = add3(o, mul3(d, t));
+        const float4 s0 = sv.shade[3 * prim], s1 = sv.shade[3 * prim + 1], s2 = sv.shade[3 * prim + 2];
+        const int attr = __float_as_int(s0.w);
+        f3 nm = mk3(w * s0.x + u * s1.x + v * s2.x, w * s0.y + u * s1.y + v * s2.y, w * s0.z + u * s1.z + v * s2.z);
+        if (dot3(nm, d) > 0.f) nm = mul3(nm, -1.f);
+        f3 nn = norm3(nm);
+        // integrator.cpp:132-150
+        const f3 vn = neg3(norm3(d));
+        if (dot3(vn, nn) <= 0.f) nn = norm3(cross3(vn, cross3(nn, vn)));
+        bool matte = (flags & F_MATTE) != 0;
+        if (rtype == RT_DIFFUSE) matte = true;
+        float4 T4 = wv.T[path];
+        f3 T = mk3(T4.x, T4.y, T4.z);
+        float cp = T4.w;
+        const DMat m = sv.mats[attr];
+        if (rtype == RT_CAMERA) {
+            float4 la = wv.La[path];
+            la.x += T.x * m.emission[0]; la.y += T.y * m.emission[1]; la.z += T.z * m.emission[2];
+            la.w = 1.f;
+            wv.La[path] = la;
+        }
+        if (m.type == MAT_LIGHT) {
+            finish = true;  // ConstantMaterial: getSample has pdf 0 (constant.cpp:23-25)
+        } else {
+            Bsdf b = get_brdf(sv, m, prim, vn, nn, matte, rng.u(vslot + 0));
+            // integrateDirect, integrator.cpp:100-130
+            int nvis = 0;
+            for (int li = 0; li < sv.n_lights; ++li) nvis += light_visible(sv.lights[li], P, nn) ? 1 : 0;
+            if (nvis) {
+                int pick = min((int)(rng.u(vslot + 1) * nvis), nvis - 1);
+                int li = 0;
+                for (int k = 0; li < sv.n_lights; ++li) {
+                    if (light_visible(sv.lights[li], P, nn)) {
+                        if (k == pick) break;
+                        ++k;
+                    }
+                }
+                const DLight &l = sv.lights[li];
+                const float scale = (float)nvis;
+                LightSample ls = light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
+                float ct = dot3(nn, ls.dir);
+                if (b.kind == BSDF_LAMBERT) {
+                    // integrateDirectLight, integrator.cpp:33-54
+                    if (ct >= 0.f && ls.pdf > 0.f) {
+                        f3 f = bsdf_eval(b, ls.dir, vn, nn);
+                        f3 c = div3(mul3(mul3v(ls.color, f), ct), ls.pdf);
+                        sh[0].valid = true; sh[0].o = P; sh[0].d = ls.dir; sh[0].tmin = AUR_RAY_BIAS; sh[0].tmax = ls.tmax;
+                        sh[0].c = mul3v(mul3(c, scale), T);
+                    }
+                } else {
+                    // integrateDirectMIS, integrator.cpp:56-98
+                    if (ct > 0.f && ls.pdf > 0.f) {
+                        float bpdf = bsdf_pdf(b, ls.dir, vn, nn);
+                        if (bpdf > 0.f) {
+                            float wgt = power_heuristic(ls.pdf, bpdf);
+                            f3 f = bsdf_eval(b, ls.dir, vn, nn);
+                            f3 c = div3(mul3(mul3v(mul3(ls.color, wgt), f), ct), ls.pdf);
+                            sh[0].valid = true; sh[0].o = P; sh[0].d = ls.dir; sh[0].tmin = AUR_RAY_BIAS; sh[0].tmax = ls.tmax;
+                            sh[0].c = mul3v(mul3(c, scale), T);
+                        }
+                    }
+                    BsdfSample bs = bsdf_sample(b, vn, nn, rng.u(vslot + 4), rng.u(vslot + 5));
+                    float ct2 = dot3(nn, bs.dir);
+                    if (ct2 > 0.f) {
+                        float tmax = bs.tmax;
+                        float lpdf = light_pdf(l, P, bs.dir, AUR_RAY_BIAS, &tmax);
+                        if (lpdf > 0.f && bs.pdf > 0.f) {
+                            float wgt = power_heuristic(bs.pdf, lpdf);
+                            f3 le = mk3(l.Le[0], l.Le[1], l.Le[2]);
+                            f3 c = div3(mul3(mul3v(mul3(bs.color, wgt), le), ct2), bs.pdf);
+                            sh[1].valid = true; sh[1].o = P; sh[1].d = bs.dir; sh[1].tmin = AUR_RAY_BIAS; sh[1].tmax = tmax;
+                            sh[1].c = mul3v(mul3(c, scale), T);
+                        }
+                    }
+                }
+            }
+            // continuation, integrator.cpp:208-257
+            BsdfSample s = bsdf_sample(b, vn, nn, rng.u(vslot + 6), rng.u(vslot + 7));
+            if (s.pdf <= 0.f) {
+                finish = true;
+            } else {
+                float ndl = dot3(nn, s.dir);
+                T = mul3v(T, div3(mul3(s.color, ndl), s.pdf));
+                const int bounces = (flags >> 16) & 0xff;
+                if (bounces > sv.mindepth) cp *= lum3(s.color) * ndl / s.pdf;
+                if (black3(T)) {
+                    finish = true;
+                } else {
+                    int nrt = (b.kind == BSDF_CT) ? RT_SPECULAR : RT_DIFFUSE;
+                    bool killed = false;
+                    if (bounces > sv.mindepth) {
+                        if (rng.u(vslot + 8) > cp) killed = true;
+                        else T = div3(T, cp);
+                    }
+                    if (killed) {
+                        finish = true;
+                    } else {
+                        cont = true;
+                        ndir = s.dir;
+                        ntmax = s.tmax;
+                        flags = nrt | (matte ? F_MATTE : 0) | ((tb + 1) << 8) | ((bounces + 1) << 16);
+                        wv.T[path] = make_float4(T.x, T.y, T.z, cp);
+                    }
+                }
+            }
+        }
+    }
+    wv.flags[path] = flags;
+    // compacted writes: next extension queue, shadow queue, finish list
+    int slot = wave_reserve(&wv.counts[nq], cont);
+    if (cont) {
+        wv.ray[nq][2 * slot] = make_float4(P.x, P.y, P.z, AUR_RAY_BIAS);
+        wv.ray[nq][2 * slot + 1] = make_float4(ndir.x, ndir.y, ndir.z, ntmax);
+        wv.qpath[nq][slot] = path;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        int ss = wave_reserve(&wv.counts[C_SHADOW], sh[k].valid);
+        if (sh[k].valid) {
+            wv.sray[3 * ss] = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].tmin);
+            wv.sray[3 * ss + 1] = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].tmax);
+            wv.sray[3 * ss + 2] = make_float4(sh[k].c.x, sh[k].c.y, sh[k].c.z, __int_as_float(path | (k ? (int)0x80000000 : 0)));
+        }
+    }
+    int fs = wave_reserve(&wv.counts[C_FINISH], finish);
+    if (finish) wv.finq[fs] = path;
+}
+
+}  // namespace aur

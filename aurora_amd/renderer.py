@@ -130,6 +130,23 @@ class Renderer:
         acc, self.last_stats = self.render_passes(0, n)
         return acc / max(n, 1)
 
+    def framebuffer(self) -> np.ndarray:
+        """The accumulation buffer (sum of per-pass RGBA, (H, W, 4)) of the last render or batch."""
+        i = self.info()
+        out = np.zeros((i["height"], i["width"], 4), np.float32)
+        L.check(self.lib.aur_framebuffer(self._ctx, L.fptr(out)))
+        return out
+
+    def set_launch(self, trace_block: int = 0, trace_waves_per_cu: int = 0, pool_paths: int = 0) -> None:
+        """Launch shape of the tracer kernels; 0 keeps a field."""
+        o = L.LaunchOpts(int(trace_block), int(trace_waves_per_cu), int(pool_paths))
+        L.check(self.lib.aur_set_launch(self._ctx, C.byref(o)))
+
+    def get_launch(self) -> dict:
+        o = L.LaunchOpts()
+        L.check(self.lib.aur_get_launch(self._ctx, C.byref(o)))
+        return {"trace_block": o.trace_block, "trace_waves_per_cu": o.trace_waves_per_cu, "pool_paths": o.pool_paths}
+
     def render_paths(self, pixel, sample, slots=None):
         """Per-path RGBA for explicit (pixel, sample) pairs; optional injected uniforms."""
         pixel = np.ascontiguousarray(pixel, dtype=np.int32)
@@ -216,3 +233,22 @@ class Renderer:
         out = np.zeros(64 * 32, np.float32)
         L.check(self.lib.aur_kelemen_table(self._ctx, reflectance, distribution, n_samples, seed, L.fptr(out)))
         return out.reshape(64, 32)
+
+
+def batch_submit(exec_renderer: "Renderer", jobs, order: int = 0, use_graph: bool = True, iters_per_poll: int = 8):
+    """Renders jobs [(renderer, first_pass, n_passes), ...] in one persistent
+    launch on exec_renderer's stream.  Returns (per-job result dicts in job
+    order, total stats dict).  Each job's image lands in its renderer's
+    framebuffer (``Renderer.framebuffer``)."""
+    n = len(jobs)
+    arr = (L.Job * n)()
+    for k, (r, fp, np_) in enumerate(jobs):
+        arr[k].scene = r._ctx.value
+        arr[k].first_pass = int(fp)
+        arr[k].n_passes = int(np_)
+    res = (L.JobResult * n)()
+    st = L.Stats()
+    opts = L.BatchOpts(int(order), int(bool(use_graph)), int(iters_per_poll))
+    lib = exec_renderer.lib
+    L.check(lib.aur_batch_submit(exec_renderer._ctx, arr, n, C.byref(opts), res, C.byref(st)))
+    return [res[k].as_dict() for k in range(n)], st.as_dict()

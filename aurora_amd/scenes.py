@@ -283,6 +283,96 @@ def displaced_spheres(width=1920, height=1080, spp=64, maxdepth=8, mindepth=8, n
     return s
 
 
+def batch_job(seed: int, k: int) -> SceneSpec:
+    """Job k of the batched uneven workload.
+
+    A closed room (the box-room walls, scaled to 4 x 2.5 x 4) holding a random
+    mix of displaced Loop-subdivided icospheres and boxes whose triangle count
+    is log-uniform in [1e3, 5e5]; resolution uniform over {128, 256, 512,
+    1024}^2, spp log-uniform in [4, 256], max depth uniform in [3, 12] (min
+    depth = max depth, as the box room).  Materials are matte (albedo
+    ~U[0.1, 0.9]^3) or GGX metal (F0 0.04, roughness ~U[0.05, 0.8]); one
+    ceiling quad light of radiance (17, 12, 4).
+    """
+    rng = SplitMix64((seed * 0x9E3779B1 + k * 0x85EBCA77 + 1) & 0xFFFFFFFFFFFF)
+    target = rng.log_uniform(1e3, 5e5)
+    res = (128, 256, 512, 1024)[rng.randint(0, 3)]
+    spp = max(4, min(256, int(round(rng.log_uniform(4, 256)))))
+    depth = rng.randint(3, 12)
+    s = SceneSpec(options=_options(res, res, spp, depth, depth),
+                  camera={"fov": math.tan(math.radians(30.0)),
+                          "transforms": [{"translate": [0.0, 1.25, 5.5]}, {"rotateX": -8.0}]})
+    x0, x1, y0, y1, z0, z1 = -2.0, 2.0, 0.0, 2.5, -2.0, 2.0
+    fv, ff = [], []
+    for q in (([x0, y0, z1], [x1, y0, z1], [x1, y0, z0], [x0, y0, z0]),
+              ([x0, y1, z0], [x1, y1, z0], [x1, y1, z1], [x0, y1, z1]),
+              ([x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0]),
+              ([x0, y0, z1], [x0, y0, z0], [x0, y1, z0], [x0, y1, z1]),
+              ([x1, y0, z0], [x1, y0, z1], [x1, y1, z1], [x1, y1, z0])):
+        v, f = quad(*q)
+        ff += [[a + len(fv) for a in t] for t in f]
+        fv += v
+    s.meshes["room.obj"] = (fv, ff, None)
+    s.materials["room"] = {"type": "matteMaterial", "color": [0.73, 0.73, 0.73]}
+    s.objects["room"] = _geo("room", "room.obj", "room")
+    iv, iff = icosahedron()
+    s.meshes["icosahedron.obj"] = (iv, iff, iv)
+
+    def material(name):
+        if rng.uniform() < 0.5:
+            s.materials[name] = {"type": "matteMaterial", "color": [rng.uniform(0.1, 0.9) for _ in range(3)]}
+        else:
+            s.materials[name] = {"type": "metalMaterial", "reflectance": [0.04, 0.04, 0.04],
+                                 "roughnessA": rng.uniform(0.05, 0.8), "roughnessB": 0.0, "mix": 1.0,
+                                 "gain": [1.0, 1.0, 1.0], "distribution": "ggx"}
+        return name
+
+    # spheres: n in [1, 4] at the level that brings n * 20 * 4^L closest to the target
+    level = max(1, min(7, int(math.floor(math.log(target / 20.0, 4.0)))))
+    n_sph = max(1, min(4, int(round(target / (20.0 * 4 ** level)))))
+    n_box = rng.randint(1, 3)
+    placed = []
+
+    def spot(r):
+        for _ in range(200):
+            x, z = rng.uniform(x0 + r + 0.1, x1 - r - 0.1), rng.uniform(z0 + r + 0.1, z1 - r - 0.1)
+            if all((x - a) ** 2 + (z - b) ** 2 > (r + c + 0.05) ** 2 for a, b, c in placed):
+                break
+        placed.append((x, z, r))
+        return x, z
+
+    for i in range(n_sph):
+        r = rng.uniform(0.3, 0.55)
+        x, z = spot(r)
+        name = "sphere%d" % i
+        s.objects[name] = _geo(name, "icosahedron.obj", material(name + "_mat"),
+                               [{"translate": [x, r, z]}, {"scale": [r, r, r]}], subdlevels=level,
+                               displace={"amplitude": 0.05, "octaves": 3, "frequency": 2.0,
+                                         "seed": int(rng.next_u64() & 0x7FFFFFFF)})
+    for i in range(n_box):
+        hx, hy, hz = rng.uniform(0.15, 0.4), rng.uniform(0.2, 0.8), rng.uniform(0.15, 0.4)
+        x, z = spot(max(hx, hz) * 1.42)
+        v, f = box_mesh([x - hx, 0.0, z - hz], [x + hx, 2 * hy, z + hz])
+        s.meshes["box%d.obj" % i] = (v, f, None)
+        name = "box%d" % i
+        s.objects[name] = _geo(name, "box%d.obj" % i, material(name + "_mat"))
+    s.objects["light_ceiling"] = _square_light("light_ceiling", [17.0, 12.0, 4.0], 0.5, 0.5,
+                                               [{"translate": [0.0, 2.48, 0.0]}, {"rotateX": -90.0}])
+    return s
+
+
+def batch_jobs(out_dir: str, n_jobs: int = 64, seed: int = 3, max_spp: int = 0) -> list:
+    """Writes the batched workload's jobs; returns their scene file paths.
+    ``max_spp`` > 0 caps the sample count (for tests)."""
+    paths = []
+    for k in range(n_jobs):
+        spec = batch_job(seed, k)
+        if max_spp > 0:
+            spec.options["pixelsamples"] = min(spec.options["pixelsamples"], max_spp)
+        paths.append(spec.write(os.path.join(out_dir, "job%03d" % k)))
+    return paths
+
+
 def write_config(name: str, out_dir: str, **kw) -> str:
     """Writes a named workload; returns the scene file path."""
     if name in ("box_room", "c1"):

@@ -1,13 +1,19 @@
 """Launch-parameter autotuner for the wavefront tracer.
 
-``autotune`` renders a few short batches of a scene for each candidate
-(BVH branching factor, triangles per leaf, paths per batch) and keeps the
-fastest; results are cached per triangle-count bucket in ``tuning.json`` next
-to this file.  ``choose`` returns the cached choice for a loaded scene (or the
-defaults when no entry exists), so benchmarks and users pick it up without
+``autotune`` renders a few short passes of a scene for each candidate and
+keeps the fastest, in two stages: the BVH structure (branching factor,
+triangles per leaf), then the traversal launch shape (workgroup size,
+persistent waves per CU) on the best structure.  ``autotune_batch`` times the
+batched persistent launch over a list of jobs for each path-pool size (the
+queue depth) and traversal grid.  Results are cached in ``tuning.json`` next
+to this file, per triangle-count bucket for single scenes and under "batch"
+for the batched launch; ``choose`` / ``choose_batch`` return the cached
+choice (or the defaults), so benchmarks and users pick it up without
 re-tuning.
 
-Usage on a GPU box:  python -m aurora_amd.tuning --workload displaced_spheres
+Usage on a GPU box:
+    python -m aurora_amd.tuning --workload displaced_spheres --out tuning.json
+    python -m aurora_amd.tuning --batch --out tuning.json
 """
 from __future__ import annotations
 
@@ -19,32 +25,44 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-DEFAULT = {"bvh_width": 4, "max_leaf": 4}
+DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16}
+BATCH_DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 4 << 20}
 WIDTHS = (4, 8)
 LEAVES = (1, 2, 4, 8)
+BLOCKS = (64, 128, 256)
+WAVES = (8, 16, 24, 32)
+POOLS = (1 << 20, 2 << 20, 4 << 20, 8 << 20)
 
 
 def _bucket(n_tris: int) -> str:
     return "tris_2^%d" % max(0, int(math.log2(max(n_tris, 1))))
 
 
-def _load() -> dict:
-    if not os.path.exists(TABLE):
+def _load(path: str = TABLE) -> dict:
+    if not os.path.exists(path):
         return {}
-    with open(TABLE) as fh:
+    with open(path) as fh:
         return json.load(fh)
 
 
-def choose(renderer, width=None, max_leaf=None) -> dict:
-    """Returns {"bvh_width", "max_leaf"} for the renderer's loaded scene."""
+def _store(key: str, entry: dict, path: str) -> None:
+    table = _load(path)
+    table[key] = entry
+    with open(path, "w") as fh:
+        json.dump(table, fh, indent=2, sort_keys=True)
+
+
+def choose(renderer, width=None, max_leaf=None, table: str = TABLE) -> dict:
+    """Returns {"bvh_width", "max_leaf", "trace_block", "trace_waves_per_cu"}
+    for the renderer's loaded scene."""
     cfg = dict(DEFAULT)
     try:
         n = renderer.info()["n_tris"]
     except Exception:
         n = 0
-    entry = _load().get(_bucket(n)) if n else None
+    entry = _load(table).get(_bucket(n)) if n else None
     if entry:
-        cfg.update({k: entry[k] for k in ("bvh_width", "max_leaf") if k in entry})
+        cfg.update({k: entry[k] for k in DEFAULT if k in entry})
     if width:
         cfg["bvh_width"] = width
     if max_leaf:
@@ -52,44 +70,111 @@ def choose(renderer, width=None, max_leaf=None) -> dict:
     return cfg
 
 
-def autotune(renderer, passes: int = 2, repeats: int = 3) -> dict:
-    """Times every (width, leaf) candidate on the loaded scene; returns the table entry."""
-    results = []
+def choose_batch(table: str = TABLE) -> dict:
+    cfg = dict(BATCH_DEFAULT)
+    entry = _load(table).get("batch")
+    if entry:
+        cfg.update({k: entry[k] for k in BATCH_DEFAULT if k in entry})
+    return cfg
+
+
+def apply_launch(renderer, cfg: dict) -> None:
+    renderer.set_launch(trace_block=cfg.get("trace_block", 0), trace_waves_per_cu=cfg.get("trace_waves_per_cu", 0),
+                        pool_paths=cfg.get("pool_paths", 0))
+
+
+def _time_render(renderer, passes: int, repeats: int) -> float:
+    renderer.render_passes(0, passes, fetch=False)  # warm (allocations, caches)
+    best = float("inf")
+    paths = 0
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        _, st = renderer.render_passes(0, passes, fetch=False)
+        best = min(best, time.perf_counter() - t0)
+        paths = st["paths"]
+    return paths / best / 1e6
+
+
+def autotune(renderer, passes: int = 2, repeats: int = 3, table: str = TABLE) -> dict:
+    """Times the structure candidates, then the launch shapes; returns the table entry."""
+    structure = []
+    apply_launch(renderer, DEFAULT)
     for w in WIDTHS:
         for leaf in LEAVES:
             renderer.build(width=w, max_leaf=leaf)
-            renderer.render_passes(0, passes, fetch=False)  # warm
+            structure.append({"bvh_width": w, "max_leaf": leaf, "mpaths_per_s": _time_render(renderer, passes, repeats)})
+    structure.sort(key=lambda r: -r["mpaths_per_s"])
+    best = structure[0]
+    renderer.build(width=best["bvh_width"], max_leaf=best["max_leaf"])
+    launch = []
+    for b in BLOCKS:
+        for wv in WAVES:
+            renderer.set_launch(trace_block=b, trace_waves_per_cu=wv)
+            launch.append({"trace_block": b, "trace_waves_per_cu": wv,
+                           "mpaths_per_s": _time_render(renderer, passes, repeats)})
+    launch.sort(key=lambda r: -r["mpaths_per_s"])
+    entry = {"bvh_width": best["bvh_width"], "max_leaf": best["max_leaf"], "trace_block": launch[0]["trace_block"],
+             "trace_waves_per_cu": launch[0]["trace_waves_per_cu"], "mpaths_per_s": launch[0]["mpaths_per_s"],
+             "structure_candidates": structure, "launch_candidates": launch}
+    apply_launch(renderer, entry)
+    _store(_bucket(renderer.info()["n_tris"]), entry, table)
+    return entry
+
+
+def autotune_batch(exec_renderer, jobs, repeats: int = 2, table: str = TABLE) -> dict:
+    """jobs: [(renderer, first_pass, n_passes)] with built scenes (one BVH width).
+    Times each (pool size, traversal waves per CU); returns the table entry."""
+    from .renderer import batch_submit
+    cands = []
+    for pool in POOLS:
+        for wv in (16, 24):
+            exec_renderer.set_launch(trace_waves_per_cu=wv, pool_paths=pool)
+            batch_submit(exec_renderer, jobs)  # warm
             best = float("inf")
             for _ in range(repeats):
                 t0 = time.perf_counter()
-                _, st = renderer.render_passes(0, passes, fetch=False)
+                _, tot = batch_submit(exec_renderer, jobs)
                 best = min(best, time.perf_counter() - t0)
-            paths = st["paths"]
-            results.append({"bvh_width": w, "max_leaf": leaf, "mpaths_per_s": paths / best / 1e6})
-    results.sort(key=lambda r: -r["mpaths_per_s"])
-    n = renderer.info()["n_tris"]
-    entry = dict(results[0])
-    entry["candidates"] = results
-    table = _load()
-    table[_bucket(n)] = entry
-    with open(TABLE, "w") as fh:
-        json.dump(table, fh, indent=2, sort_keys=True)
+            cands.append({"pool_paths": pool, "trace_waves_per_cu": wv, "mpaths_per_s": tot["paths"] / best / 1e6})
+    cands.sort(key=lambda r: -r["mpaths_per_s"])
+    launch = exec_renderer.get_launch()
+    i0 = jobs[0][0].info()
+    entry = {"bvh_width": i0["bvh_width"], "max_leaf": i0["bvh_max_leaf"], "trace_block": launch["trace_block"],
+             "trace_waves_per_cu": cands[0]["trace_waves_per_cu"], "pool_paths": cands[0]["pool_paths"],
+             "mpaths_per_s": cands[0]["mpaths_per_s"], "candidates": cands}
+    apply_launch(exec_renderer, entry)
+    _store("batch", entry, table)
     return entry
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="displaced_spheres")
+    ap.add_argument("--batch", action="store_true", help="tune the batched launch on the 64-job workload")
+    ap.add_argument("--out", default=TABLE, help="table to update")
     args = ap.parse_args(argv)
     import tempfile
 
     from . import scenes
     from .renderer import Renderer
     tmp = tempfile.mkdtemp(prefix="aurora_tune_")
+    if args.batch:
+        paths = scenes.batch_jobs(tmp, n_jobs=64, seed=3)
+        rs = []
+        for k, p in enumerate(paths):
+            r = Renderer(p, os.path.join(os.path.dirname(p), "config.json"))
+            r.build()
+            r.set_seed(1000 + k)
+            rs.append(r)
+        ex = Renderer()
+        # one pass of every job: the same mix of scenes at 1/60 of the work
+        print(json.dumps(autotune_batch(ex, [(r, 0, 1) for r in rs], table=args.out)), flush=True)
+        return 0
     path = scenes.write_config(args.workload, tmp)
     r = Renderer(path, os.path.join(tmp, "config.json"))
     r.build()
-    print(json.dumps(autotune(r)))
+    print(json.dumps(autotune(r, table=args.out)), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

@@ -15,7 +15,10 @@ oracle, with its own SAH BVH and OpenMP tiles) timed on this host on a bounded
 sample of the same frame.  ``adjoint`` is the inverse-rendering workload (the
 Kelemen box room at 512x512, 32 spp forward and adjoint, depth 6): forward and
 adjoint times, adjoint Mpaths/s, and the largest relative error of the adjoint
-gradient against central differences on 64 seeded parameters.
+gradient against central differences on 64 seeded parameters.  ``batch`` is
+the batched uneven workload (64 seeded scenes of 1e3..5e5 triangles at
+128^2..1024^2, 4..256 spp, depth 3..12): aggregate Mpaths/s of the batched
+persistent launch and of per-scene launches, and per-job latency p50/p99.
 """
 from __future__ import annotations
 
@@ -199,6 +202,73 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
             "fd_checked": int(len(pick)), "fd_h": h, "fd_max_rel_err": float("%.3g" % worst)}
 
 
+def batch_leg(n_jobs: int = 64, seed: int = 3):
+    """Batched uneven workload: the same jobs rendered once through the batched
+    persistent launch and once as per-scene launches (one aur_render per job,
+    in the same order), with per-job latency from the start of the run."""
+    from aurora_amd import scenes, tuning
+    from aurora_amd.renderer import Renderer, batch_submit
+    tmp = tempfile.mkdtemp(prefix="aurora_batch_")
+    paths = scenes.batch_jobs(tmp, n_jobs=n_jobs, seed=seed)
+    cfg = tuning.choose_batch()
+    t0 = time.time()
+    rs = []
+    for k, p in enumerate(paths):
+        r = Renderer(p, os.path.join(os.path.dirname(p), "config.json"))
+        r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
+        tuning.apply_launch(r, cfg)
+        r.set_seed(1000 + k)
+        rs.append(r)
+    build_s = time.time() - t0
+    info = [r.info() for r in rs]
+    spp = [i["pixelsamples"] for i in info]
+    tris = [i["n_tris"] for i in info]
+    _progress("batch: %d scenes built in %.1f s (%d..%d triangles)" % (n_jobs, build_s, min(tris), max(tris)))
+    ex = Renderer()
+    tuning.apply_launch(ex, cfg)
+    # warm both launch paths: one batched pass, and per job the passes that
+    # size its own working set
+    batch_submit(ex, [(r, 0, 1) for r in rs])
+    for r, i, n in zip(rs, info, spp):
+        r.render_passes(0, min(n, max(1, (8 << 20) // (i["width"] * i["height"]))), fetch=False)
+    _hip_sync()
+    t0 = time.perf_counter()
+    res, tot = batch_submit(ex, [(r, 0, n) for r, n in zip(rs, spp)])
+    batched_s = time.perf_counter() - t0
+    _progress("batch: batched launch %.2f s" % batched_s)
+    order = sorted(range(n_jobs), key=lambda k: res[k]["order"])
+    check = set(order[:2] + order[-2:])
+    fb = {k: rs[k].framebuffer() for k in check}
+    lat_single = [0.0] * n_jobs
+    worst = 0.0
+    t0 = time.perf_counter()
+    for k in order:
+        acc, _ = rs[k].render_passes(0, spp[k], fetch=k in check)
+        lat_single[k] = (time.perf_counter() - t0) * 1e3
+        if k in check:
+            worst = max(worst, float(np.max(np.abs(acc - fb[k]) / (np.abs(acc) + 1e-3))))
+        rs[k].close()
+    single_s = time.perf_counter() - t0
+    _progress("batch: per-scene launches %.2f s" % single_s)
+    ex.close()
+    lat_b = np.array([o["latency_ms"] for o in res])
+    lat_s = np.array(lat_single)
+    total = int(tot["paths"])
+    rays = int(tot["ext_rays"] + tot["shadow_rays"])
+    return {"workload": "%d seeded jobs (seed %d): %d..%d triangles, 128^2..1024^2, spp 4..256, depth 3..12"
+                        % (n_jobs, seed, min(tris), max(tris)),
+            "paths": total, "rays": rays, "scene_build_s": round(build_s, 2),
+            "pool_paths": cfg["pool_paths"], "bvh_width": cfg["bvh_width"], "trace_waves_per_cu": cfg["trace_waves_per_cu"],
+            "batched_Mpaths_s": round(total / batched_s / 1e6, 3), "per_scene_Mpaths_s": round(total / single_s / 1e6, 3),
+            "batched_Mrays_s": round(rays / batched_s / 1e6, 3),
+            "speedup": round(single_s / batched_s, 3),
+            "batched_latency_ms_p50": round(float(np.percentile(lat_b, 50)), 2),
+            "batched_latency_ms_p99": round(float(np.percentile(lat_b, 99)), 2),
+            "per_scene_latency_ms_p50": round(float(np.percentile(lat_s, 50)), 2),
+            "per_scene_latency_ms_p99": round(float(np.percentile(lat_s, 99)), 2),
+            "image_max_rel_diff_4_jobs": float("%.3g" % worst)}
+
+
 def copy_calibration(mib: int, reps: int):
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
     counters to bytes."""
@@ -226,6 +296,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--levels", type=int, default=7)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--adam-steps", type=int, default=200,
                     help="Adam steps of the inverse-rendering run inside the adjoint leg (0: skip)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -248,6 +319,7 @@ def main(argv=None):
     cfg = tuning.choose(r, width=args.bvh_width or None, max_leaf=args.max_leaf or None)
     t0 = time.time()
     r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
+    tuning.apply_launch(r, cfg)
     build_s = time.time() - t0
     _progress("scene and BVH built in %.2f s" % build_s)
     info = r.info()
@@ -291,6 +363,7 @@ def main(argv=None):
         "config": {"workload": "displaced_spheres_1080p_depth8", "width": info["width"], "height": info["height"],
                    "spp_timed": args.steps * P, "passes_per_step": P, "maxdepth": info["maxdepth"],
                    "triangles": info["n_tris"], "bvh_width": info["bvh_width"], "max_leaf": info["bvh_max_leaf"],
+                   "trace_block": cfg["trace_block"], "trace_waves_per_cu": cfg["trace_waves_per_cu"],
                    "bvh_build_s": round(build_s, 3)},
         "mrays_per_s": round((tot["ext_rays"] + tot["shadow_rays"]) / elapsed / 1e6, 3),
         "counters": tot,
@@ -301,6 +374,9 @@ def main(argv=None):
     }
     exp = None if args.no_cpu else r.export()
     r.close()
+    if not args.no_batch:
+        out["batch"] = batch_leg()
+        _progress("batch leg done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(512, 32, 6, adam_steps=args.adam_steps)
         _progress("adjoint leg done")

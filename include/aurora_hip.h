@@ -177,6 +177,47 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
 int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_paths,
                      const float *slots, float *rgba_out, aur_stats *out);
 
+/* Copies the accumulation buffer (sum of per-pass RGBA, 4*W*H floats) of the
+ * last aur_render or aur_batch_submit on this context.  The reference's
+ * framebuffer is OpenexrDisplay's running mean (openexrDisplay.cpp:61-72);
+ * divide by the pass count for it. */
+int aur_framebuffer(aur_ctx *ctx, float *rgba_sum_host);
+
+/* ---- batched persistent launch (no counterpart in the reference, which
+ * renders one scene per process: Renderer::renderImage, renderer.cpp:201-271) */
+typedef struct {
+    aur_ctx *scene;   /* a context with a built scene, on exec's device */
+    int first_pass;   /* passes [first_pass, first_pass + n_passes) are added to */
+    int n_passes;     /* the job's framebuffer; first_pass == 0 clears it first */
+} aur_job;
+typedef struct {
+    int64_t paths, ext_rays, hit_rays, shadow_rays;
+    double latency_ms; /* from the start of the launch to the job's last finished path */
+    int order;         /* the job's position in the launch order */
+} aur_job_result;
+typedef struct {
+    int order;          /* 0: shortest estimated job first, 1: as given */
+    int use_graph;      /* nonzero: replay a captured hipGraph of two bounces */
+    int iters_per_poll; /* bounces between completion checks by the host (0: 8) */
+} aur_batch_opts;
+/* Renders every job through one pool of path slots on exec's stream (exec
+ * may be one of the jobs' contexts).  All jobs need the same BVH width.
+ * opts may be NULL (shortest first, graph replay).  results has n_jobs
+ * entries, in the order of jobs; total (may be NULL) gets the sums and the
+ * device time of the whole launch. */
+int aur_batch_submit(aur_ctx *exec, const aur_job *jobs, int n_jobs, const aur_batch_opts *opts,
+                     aur_job_result *results, aur_stats *total);
+
+/* Launch shape of the tracer kernels on this context (the autotuner's knobs). */
+typedef struct {
+    int trace_block;        /* traversal workgroup size: 64, 128 or 256 */
+    int trace_waves_per_cu; /* persistent traversal grid, in waves per CU (1..64) */
+    int pool_paths;         /* path slots in flight in aur_batch_submit (>= 64) */
+} aur_launch_opts;
+/* Fields <= 0 keep their current value. */
+int aur_set_launch(aur_ctx *ctx, const aur_launch_opts *opts);
+int aur_get_launch(aur_ctx *ctx, aur_launch_opts *out);
+
 /* ---- inverse rendering (no counterpart in the reference, which is not
  * differentiable) ---- */
 /* Per-material parameters (n_attrs floats each; NULL leaves a field as is):

@@ -149,3 +149,57 @@ def test_json_comments_and_sorted_members(tmp_path):
     e = r.export()
     r.close()
     assert e["verts"].shape[0] > 0 and len(e["materials"]) == 6
+
+
+def test_launch_options_validation():
+    r = host_renderer()
+    d = r.get_launch()
+    assert d == {"trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 4 << 20}
+    r.set_launch(trace_block=256, pool_paths=1 << 16)
+    assert r.get_launch() == {"trace_block": 256, "trace_waves_per_cu": 16, "pool_paths": 1 << 16}
+    _expect_error(lambda: r.set_launch(trace_block=100), -1)
+    _expect_error(lambda: r.set_launch(trace_waves_per_cu=65), -1)
+    _expect_error(lambda: r.set_launch(pool_paths=10), -1)
+    assert r.get_launch()["trace_block"] == 256
+    r.close()
+
+
+def test_batch_submit_errors_without_gpu(box_room_dir):
+    import ctypes as C
+    from aurora_amd import _lib as L
+    from aurora_amd.renderer import batch_submit
+    r = host_renderer()
+    r.load(os.path.join(box_room_dir, "scene.asc"), os.path.join(box_room_dir, "config.json"))
+    _expect_error(lambda: batch_submit(r, []), -1)
+    _expect_error(lambda: batch_submit(r, [(r, 0, 1)]), -5)   # host-only context
+    res = (L.JobResult * 1)()
+    assert r.lib.aur_batch_submit(r._ctx, None, 1, None, res, None) == -1
+    _expect_error(lambda: r.framebuffer(), -5)
+    r.close()
+    del C
+
+
+def test_batch_workload_generator_ranges():
+    from aurora_amd import scenes
+    tris, res, spp, depth = [], set(), [], []
+    for k in range(64):
+        s = scenes.batch_job(3, k)
+        n = 0
+        for o in s.objects.values():
+            if o.get("type") != "geometry":
+                continue
+            if "icosahedron" in o["path"]:
+                n += 20 * 4 ** o["subdlevels"]
+            elif "room" in o["path"]:
+                n += 10
+            else:
+                n += 12
+        tris.append(n + 2)  # + the light quad
+        res.add(tuple(s.options["resolution"]))
+        spp.append(s.options["pixelsamples"])
+        depth.append(s.options["maxdepth"])
+        assert s.options["mindepth"] == s.options["maxdepth"]
+    assert res <= {(128, 128), (256, 256), (512, 512), (1024, 1024)} and len(res) == 4
+    assert 4 <= min(spp) and max(spp) <= 256 and 3 <= min(depth) and max(depth) <= 12
+    assert 900 <= min(tris) and max(tris) <= 700000
+    assert scenes.batch_job(3, 7).options == scenes.batch_job(3, 7).options
