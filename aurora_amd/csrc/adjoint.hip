@@ -122,15 +122,8 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
         f3 cl = mk3(0.f, 0.f, 0.f), cb = mk3(0.f, 0.f, 0.f);
         float dl_l = 0.f, dl_b = 0.f;
         int nvis = 0;
-        for (int li = 0; li < sv.n_lights; ++li) nvis += light_visible(sv.lights[li], P, nn) ? 1 : 0;
-        if (nvis) {
-            int pick = min((int)(rng.u(vslot + 1) * nvis), nvis - 1);
-            int li = 0;
-            for (int k = 0; li < sv.n_lights; ++li)
-                if (light_visible(sv.lights[li], P, nn)) {
-                    if (k == pick) break;
-                    ++k;
-                }
+        const int li = pick_visible_light(sv.lights, sv.n_lights, P, nn, rng.u(vslot + 1), &nvis);
+        if (li >= 0) {
             const DLight &l = sv.lights[li];
             LightSample ls = light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
             float ct = dot3(nn, ls.dir);

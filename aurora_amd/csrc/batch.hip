@@ -92,6 +92,7 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
         const long long w = (long long)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
         const bool valid = w < bv.total;
         f3 d = mk3(0.f, 0.f, 1.f);
+        int job = 0;
         if (valid) {
             int a = 0, b = bv.n_jobs - 1;  // last job whose start <= w
             while (a < b) {
@@ -99,6 +100,7 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
                 if (bv.starts[mid] <= w) a = mid;
                 else b = mid - 1;
             }
+            job = a;
             const JobDev &jd = bv.jobs[a];
             const long long local = w - jd.start;
             const int wh = jd.sv.width * jd.sv.height;
@@ -123,6 +125,7 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
             wv.ray[q][2 * qs] = make_float4(0.f, 0.f, 0.f, 0.1f);
             wv.ray[q][2 * qs + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
             wv.qpath[q][qs] = slot;
+            wv.qjob[q][qs] = job;
         }
     }
 }
@@ -139,8 +142,13 @@ __global__ void k_bclear_queue(int *counts, int q) {
     if (threadIdx.x == 0) counts[q] = 0;
 }
 
+// The kernels below run each wave once per distinct job among its lanes
+// (almost always one): the job index is made wave-uniform with
+// readfirstlane, so the scene record is read with scalar loads and the
+// traversal and shading code sees uniform scene pointers, as in the
+// single-scene kernels.
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_bextend(BatchView bv, WfView wv, int q) {
+__global__ void __launch_bounds__(B) k_bextend(BatchView bv, const JobDev *__restrict__ jobs, WfView wv, int q) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_EXT], (unsigned long long)n);
@@ -151,37 +159,47 @@ __global__ void __launch_bounds__(B) k_bextend(BatchView bv, WfView wv, int q) {
     st.stride = B;
     st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
     for (int i = gtid; i < n; i += stride) {
-        const int path = wv.qpath[q][i];
-        const int job = bv.pjob[path];
-        const JobDev &jd = bv.jobs[job];
-        float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
+        const int job = wv.qjob[q][i];
+        const float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
         TravHit h;
-        traverse<W, false>(static_cast<const WNode<W> *>(jd.sv.nodes), jd.sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z),
-                           a.w, b.w, st, h);
+        for (;;) {
+            const int j = __builtin_amdgcn_readfirstlane(job);
+            if (job == j) {
+                traverse<W, false>(static_cast<const WNode<W> *>(jobs[j].sv.nodes), jobs[j].sv.tris, mk3(a.x, a.y, a.z),
+                                   mk3(b.x, b.y, b.z), a.w, b.w, st, h);
+                wave_count(&bv.jstats[(size_t)j * J_STRIDE + J_EXT], true);
+                wave_count(&bv.jstats[(size_t)j * J_STRIDE + J_HIT], h.prim >= 0);
+                break;
+            }
+        }
         wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
         wave_count(&wv.stats[S_HIT], h.prim >= 0);
-        wave_count_job(bv.jstats + J_EXT, J_STRIDE, job, true);
-        wave_count_job(bv.jstats + J_HIT, J_STRIDE, job, h.prim >= 0);
     }
 }
 
-__global__ void __launch_bounds__(256) k_bshade(BatchView bv, WfView wv, int q) {
+__global__ void __launch_bounds__(256) k_bshade(BatchView bv, const JobDev *__restrict__ jobs, WfView wv, int q) {
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int path = wv.qpath[q][i];
-        const JobDev &jd = bv.jobs[bv.pjob[path]];
-        shade_one(jd.sv, wv, q, i, [&](int p) {
-            PathRng r;
-            r.slots = nullptr;
-            r.key = path_key(jd.seed, (uint32_t)bv.ppix[p], (uint32_t)bv.psmp[p]);
-            return r;
-        });
+        const int job = wv.qjob[q][i];
+        for (;;) {
+            const int j = __builtin_amdgcn_readfirstlane(job);
+            if (job == j) {
+                const uint32_t seed = jobs[j].seed;
+                shade_one(jobs[j].sv, wv, q, i, [&](int p) {
+                    PathRng r;
+                    r.slots = nullptr;
+                    r.key = path_key(seed, (uint32_t)bv.ppix[p], (uint32_t)bv.psmp[p]);
+                    return r;
+                }, j);
+                break;
+            }
+        }
     }
 }
 
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_bshadow(BatchView bv, WfView wv) {
+__global__ void __launch_bounds__(B) k_bshadow(BatchView bv, const JobDev *__restrict__ jobs, WfView wv) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_SHADOW], (unsigned long long)n);
@@ -192,21 +210,26 @@ __global__ void __launch_bounds__(B) k_bshadow(BatchView bv, WfView wv) {
     st.stride = B;
     st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
     for (int i = gtid; i < n; i += stride) {
-        float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
+        const float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
         const int tag = __float_as_int(c.w);
         const int path = tag & 0x7fffffff;
         const int job = bv.pjob[path];
-        const JobDev &jd = bv.jobs[job];
         TravHit h;
-        traverse<W, true>(static_cast<const WNode<W> *>(jd.sv.nodes), jd.sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z),
-                          a.w, b.w, st, h);
+        for (;;) {
+            const int j = __builtin_amdgcn_readfirstlane(job);
+            if (job == j) {
+                traverse<W, true>(static_cast<const WNode<W> *>(jobs[j].sv.nodes), jobs[j].sv.tris, mk3(a.x, a.y, a.z),
+                                  mk3(b.x, b.y, b.z), a.w, b.w, st, h);
+                wave_count(&bv.jstats[(size_t)j * J_STRIDE + J_SHADOW], true);
+                break;
+            }
+        }
         if (h.prim < 0) {
             float4 *acc = (tag < 0) ? wv.Lb : wv.La;
             float4 v = acc[path];
             v.x += c.x; v.y += c.y; v.z += c.z;
             acc[path] = v;
         }
-        wave_count_job(bv.jstats + J_SHADOW, J_STRIDE, job, true);
     }
 }
 
@@ -271,10 +294,10 @@ static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q
     const int sb = c->n_cu * 4;
     hipLaunchKernelGGL(k_regen, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
     hipLaunchKernelGGL(k_bclear_lists, dim3(1), dim3(64), 0, c->stream, wv.counts);
-#define AUR_BEXT(w_, b_) hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv, wv, q)
-#define AUR_BSHD(w_, b_) hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv, wv)
+#define AUR_BEXT(w_, b_) hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv, bv.jobs, wv, q)
+#define AUR_BSHD(w_, b_) hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv, bv.jobs, wv)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
-    hipLaunchKernelGGL(k_bshade, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
+    hipLaunchKernelGGL(k_bshade, dim3(sb), dim3(256), 0, c->stream, bv, bv.jobs, wv, q);
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BSHD);
 #undef AUR_BEXT
 #undef AUR_BSHD
@@ -346,6 +369,8 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     b.pjob.reserve(sizeof(int) * pool);
     b.ppix.reserve(sizeof(int) * pool);
     b.psmp.reserve(sizeof(int) * pool);
+    b.qjob[0].reserve(sizeof(int) * pool);
+    b.qjob[1].reserve(sizeof(int) * pool);
     if (!b.host_status) HIP_CHECK(hipHostMalloc(&b.host_status, 256, hipHostMallocDefault));
     HIP_CHECK(hipMemcpyAsync(b.jobs.p, jd.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(b.starts.p, starts.data(), sizeof(long long) * n_jobs, hipMemcpyHostToDevice, c->stream));
@@ -356,6 +381,8 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     src.pixel = src.sample = nullptr;
     src.slots = nullptr;
     WfView wv = wf_view(c, src);
+    wv.qjob[0] = b.qjob[0].as<int>();
+    wv.qjob[1] = b.qjob[1].as<int>();
     BatchView bv = batch_view(c, n_jobs, total);
 
     hipEvent_t e0, e1;

@@ -116,6 +116,7 @@ struct LaunchCfg {
 struct BatchState {
     DevBuf jobs, starts, remaining, t_end, jstats, cursor, t0;
     DevBuf pjob, ppix, psmp;        // per path slot
+    DevBuf qjob[2];                 // job of each extension-queue entry
     void *host_status = nullptr;    // pinned: cursor + queue counts, polled by the host
     BatchState() = default;
     BatchState(const BatchState &) = delete;

@@ -378,6 +378,33 @@ __device__ inline bool light_visible(const DLight &l, f3 orig, f3 nn) {
     return false;
 }
 
+// integrateDirect's light choice (integrator.cpp:100-130): the lights whose
+// visible() test passes from (orig, nn) are counted and the
+// min(floor(u * n), n - 1)-th of them, in light order, is returned (-1 when
+// none is visible).  One pass over the lights keeps a bit mask of the first 64.
+__device__ inline int pick_visible_light(const DLight *lights, int n_lights, f3 orig, f3 nn, float u, int *nvis_out) {
+    unsigned long long mask = 0ull;
+    int nvis = 0;
+    const int nm = n_lights < 64 ? n_lights : 64;
+    for (int li = 0; li < nm; ++li)
+        if (light_visible(lights[li], orig, nn)) { mask |= 1ull << li; ++nvis; }
+    for (int li = 64; li < n_lights; ++li) nvis += light_visible(lights[li], orig, nn) ? 1 : 0;
+    *nvis_out = nvis;
+    if (!nvis) return -1;
+    int pick = min((int)(u * nvis), nvis - 1);
+    const int in_mask = __popcll(mask);
+    if (pick < in_mask) {
+        for (int k = 0; k < pick; ++k) mask &= mask - 1ull;
+        return __ffsll((long long)mask) - 1;
+    }
+    for (int li = 64, k = in_mask; li < n_lights; ++li)
+        if (light_visible(lights[li], orig, nn)) {
+            if (k == pick) return li;
+            ++k;
+        }
+    return -1;
+}
+
 // integrator.cpp:28-31
 __device__ inline float power_heuristic(float fpdf, float gpdf) {
     float f = fpdf, g = gpdf;

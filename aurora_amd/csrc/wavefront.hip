@@ -311,7 +311,11 @@ static void ensure_wavefront(Ctx *c, size_t cap) { ensure_wavefront_for(c, cap, 
 WfView wf_view(Ctx *c, const PathSource &src) {
     Wavefront &w = c->wf;
     WfView v;
-    for (int k = 0; k < 2; ++k) { v.ray[k] = w.ray[k].as<float4>(); v.qpath[k] = w.qpath[k].as<int>(); }
+    for (int k = 0; k < 2; ++k) {
+        v.ray[k] = w.ray[k].as<float4>();
+        v.qpath[k] = w.qpath[k].as<int>();
+        v.qjob[k] = nullptr;
+    }
     v.hit = w.hit.as<float4>();
     v.T = w.T.as<float4>(); v.La = w.La.as<float4>(); v.Lb = w.Lb.as<float4>();
     v.flags = w.flags.as<int>();

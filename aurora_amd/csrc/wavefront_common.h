@@ -37,6 +37,7 @@ enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
 struct WfView {
     float4 *ray[2];
     int *qpath[2];
+    int *qjob[2];       // batched launch: job of each queue entry (null otherwise)
     float4 *hit;
     float4 *T, *La, *Lb;
     int *flags;
@@ -109,14 +110,15 @@ __device__ inline f3 firefly_clamp(f3 L, int flags) {
 
 // ---------------------------------------------------------------------------
 // shade: one path vertex (integrator.cpp:180-274) for queue entry i of queue
-// q.  make_rng(path) gives the path's random-number source.  Writes the next
+// q.  make_rng(path) gives the path's random-number source; job is recorded
+// with the next extension ray when the view carries job queues.  Writes the next
 // extension ray, up to two shadow rays and the finish entry with wave-level
 // compaction, so every active lane of a wave must call it.
 // ---------------------------------------------------------------------------
 struct ShadowOut { bool valid; f3 o, d; float tmin, tmax; f3 c; };
 
 template <class MakeRng>
-__device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng) {
+__device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job = -1) {
     const int nq = 1 - q;
     const int path = wv.qpath[q][i];
     const float4 ra = wv.ray[q][2 * i], rb = wv.ray[q][2 * i + 1];
@@ -172,16 +174,8 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
             Bsdf b = get_brdf(sv, m, prim, vn, nn, matte, rng.u(vslot + 0));
             // integrateDirect, integrator.cpp:100-130
             int nvis = 0;
-            for (int li = 0; li < sv.n_lights; ++li) nvis += light_visible(sv.lights[li], P, nn) ? 1 : 0;
-            if (nvis) {
-                int pick = min((int)(rng.u(vslot + 1) * nvis), nvis - 1);
-                int li = 0;
-                for (int k = 0; li < sv.n_lights; ++li) {
-                    if (light_visible(sv.lights[li], P, nn)) {
-                        if (k == pick) break;
-                        ++k;
-                    }
-                }
+            const int li = pick_visible_light(sv.lights, sv.n_lights, P, nn, rng.u(vslot + 1), &nvis);
+            if (li >= 0) {
                 const DLight &l = sv.lights[li];
                 const float scale = (float)nvis;
                 LightSample ls = light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
@@ -259,6 +253,7 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
         wv.ray[nq][2 * slot] = make_float4(P.x, P.y, P.z, AUR_RAY_BIAS);
         wv.ray[nq][2 * slot + 1] = make_float4(ndir.x, ndir.y, ndir.z, ntmax);
         wv.qpath[nq][slot] = path;
+        if (wv.qjob[nq]) wv.qjob[nq][slot] = job;
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {

@@ -122,26 +122,40 @@ def autotune(renderer, passes: int = 2, repeats: int = 3, table: str = TABLE) ->
 
 
 def autotune_batch(exec_renderer, jobs, repeats: int = 2, table: str = TABLE) -> dict:
-    """jobs: [(renderer, first_pass, n_passes)] with built scenes (one BVH width).
-    Times each (pool size, traversal waves per CU); returns the table entry."""
+    """jobs: [(renderer, first_pass, n_passes)] with loaded scenes.  Times each
+    triangles-per-leaf choice (all jobs rebuilt with BVH width 4), then each
+    (path-pool size, traversal waves per CU, workgroup size); returns the
+    table entry."""
     from .renderer import batch_submit
+
+    def timed():
+        batch_submit(exec_renderer, jobs)  # warm
+        best = float("inf")
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            _, tot = batch_submit(exec_renderer, jobs)
+            best = min(best, time.perf_counter() - t0)
+        return tot["paths"] / best / 1e6
+
+    leaves = []
+    apply_launch(exec_renderer, BATCH_DEFAULT)
+    for leaf in (1, 2, 4):
+        for r, _, _ in jobs:
+            r.build(width=4, max_leaf=leaf)
+        leaves.append({"max_leaf": leaf, "mpaths_per_s": timed()})
+    leaves.sort(key=lambda r: -r["mpaths_per_s"])
+    for r, _, _ in jobs:
+        r.build(width=4, max_leaf=leaves[0]["max_leaf"])
     cands = []
     for pool in POOLS:
-        for wv in (16, 24):
-            exec_renderer.set_launch(trace_waves_per_cu=wv, pool_paths=pool)
-            batch_submit(exec_renderer, jobs)  # warm
-            best = float("inf")
-            for _ in range(repeats):
-                t0 = time.perf_counter()
-                _, tot = batch_submit(exec_renderer, jobs)
-                best = min(best, time.perf_counter() - t0)
-            cands.append({"pool_paths": pool, "trace_waves_per_cu": wv, "mpaths_per_s": tot["paths"] / best / 1e6})
+        for wv in (16, 24, 32):
+            for blk in (64, 128):
+                exec_renderer.set_launch(trace_block=blk, trace_waves_per_cu=wv, pool_paths=pool)
+                cands.append({"pool_paths": pool, "trace_waves_per_cu": wv, "trace_block": blk, "mpaths_per_s": timed()})
     cands.sort(key=lambda r: -r["mpaths_per_s"])
-    launch = exec_renderer.get_launch()
-    i0 = jobs[0][0].info()
-    entry = {"bvh_width": i0["bvh_width"], "max_leaf": i0["bvh_max_leaf"], "trace_block": launch["trace_block"],
+    entry = {"bvh_width": 4, "max_leaf": leaves[0]["max_leaf"], "trace_block": cands[0]["trace_block"],
              "trace_waves_per_cu": cands[0]["trace_waves_per_cu"], "pool_paths": cands[0]["pool_paths"],
-             "mpaths_per_s": cands[0]["mpaths_per_s"], "candidates": cands}
+             "mpaths_per_s": cands[0]["mpaths_per_s"], "leaf_candidates": leaves, "candidates": cands}
     apply_launch(exec_renderer, entry)
     _store("batch", entry, table)
     return entry
@@ -167,8 +181,9 @@ def main(argv=None):
             r.set_seed(1000 + k)
             rs.append(r)
         ex = Renderer()
-        # one pass of every job: the same mix of scenes at 1/60 of the work
-        print(json.dumps(autotune_batch(ex, [(r, 0, 1) for r in rs], table=args.out)), flush=True)
+        # up to four passes of every job: the same mix of scenes at ~1/15 of the work
+        jobs = [(r, 0, min(4, r.info()["pixelsamples"])) for r in rs]
+        print(json.dumps(autotune_batch(ex, jobs, table=args.out)), flush=True)
         return 0
     path = scenes.write_config(args.workload, tmp)
     r = Renderer(path, os.path.join(tmp, "config.json"))

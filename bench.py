@@ -316,11 +316,15 @@ def main(argv=None):
     tmp = tempfile.mkdtemp(prefix="aurora_bench_")
     scene_path = scenes.write_config("displaced_spheres", tmp, levels=args.levels)
     r = Renderer(scene_path, os.path.join(tmp, "config.json"))
-    cfg = tuning.choose(r, width=args.bvh_width or None, max_leaf=args.max_leaf or None)
     t0 = time.time()
-    r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
-    tuning.apply_launch(r, cfg)
+    r.build()  # Loop subdivision happens here: the tuning table is keyed by the final triangle count
     build_s = time.time() - t0
+    cfg = tuning.choose(r, width=args.bvh_width or None, max_leaf=args.max_leaf or None)
+    if (cfg["bvh_width"], cfg["max_leaf"]) != (4, 4):
+        t0 = time.time()
+        r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
+        build_s = time.time() - t0
+    tuning.apply_launch(r, cfg)
     _progress("scene and BVH built in %.2f s" % build_s)
     info = r.info()
     wh = info["width"] * info["height"]
