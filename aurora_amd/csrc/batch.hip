@@ -3,31 +3,38 @@
 // The reference renders one scene per process: `Renderer::renderImage`
 // (src/frontend/renderer.cpp:201-271) runs its passes with one TBB
 // parallel_for each.  Here a list of jobs (each a built scene, a pass range
-// and its own framebuffer) shares one pool of path slots:
+// and its own framebuffer) shares one pool of path slots.  The jobs' paths
+// form one work sequence in launch order (shortest estimated job first by
+// default): work item w of job j is pass (w - start_j) / (W*H) and pixel
+// tiled_pixel((w - start_j) mod (W*H)), the order of a single-scene launch.
+// Per iteration:
 //
-//   regen   -- every free slot takes the next path from a global work cursor
-//              that runs over the jobs in launch order (shortest estimated job
-//              first by default); the job is found by binary search over the
-//              jobs' first work indices, the pixel and pass from the offset in
-//              the job (the same 8x8-tiled order as a single-scene launch)
-//   extend, shade, shadow -- the single-scene kernels' bodies, with the scene
-//              of each path read from its job record
-//   finish  -- firefly clamp, atomic add into the job's framebuffer, and a
-//              per-job countdown; the last path of a job stamps the wall clock
+//   plan    -- one thread: as many new paths as there are free slots (the
+//              previous iteration's finished paths) come off the work cursor
+//   regen   -- the new paths' camera rays are appended to the extension
+//              queue, coalesced, no atomics
+//   extend, shade, shadow -- the single-scene kernels' bodies; each wave
+//              loops once per distinct job among its lanes (almost always
+//              one) with that job's scene record read wave-uniformly
+//   finish  -- firefly clamp, and the path's RGBA goes to a ring buffer at
+//              its work index; per-job ray counts and a per-job countdown
+//              (the last path of a job stamps the wall clock) are summed per
+//              wave
 //
 // so the GPU stays full until the last job drains, whatever the jobs' sizes,
-// resolutions, sample counts and depths.  One iteration is a fixed sequence
-// of launches whose counts live in device memory; two iterations (the two
-// extension queues swap) are captured once as a hipGraph and replayed, and
-// the host reads the cursor and the live count only every few iterations.
-//
-// A path's random numbers depend only on (job seed, pixel, sample), so each
-// job's framebuffer equals a single-scene render of the same passes up to
-// the order of the floating-point sums.
+// resolutions, sample counts and depths.  An iteration is a fixed sequence of
+// launches whose counts live in device memory; two iterations (the extension
+// queues swap) are captured once as a hipGraph and replayed.  Every few
+// iterations the host reads the cursor history: work issued at least
+// maxdepth + 1 iterations ago has finished, and that prefix of the ring is
+// added into the jobs' framebuffers pixel by pixel in pass order -- the same
+// float additions, in the same order, as a single-scene render of the same
+// passes, so each job's framebuffer is bit-identical to it.
 #include "wavefront_common.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 namespace aur {
@@ -42,18 +49,31 @@ struct JobDev {
 };
 
 enum { J_EXT = 0, J_HIT = 1, J_SHADOW = 2, J_STRIDE = 4 };
+// control block: cursor, iterations done, the current iteration's plan, and
+// the cursor after each of the last BATCH_HIST iterations
+enum { CTL_CURSOR = 0, CTL_ITER = 1, CTL_LIVE = 2, CTL_NEW = 3, CTL_BASE = 4, CTL_HIST = 8 };
+#define BATCH_HIST 64
 
 struct BatchView {
     const JobDev *jobs;      // in launch order
     const long long *starts; // jobs[k].start, for the binary search
     int n_jobs;
     long long total;         // paths over all jobs
-    unsigned long long *cursor;
+    long long *ctl;
     int *pjob, *ppix, *psmp; // per slot: job (launch order), pixel, sample
+    long long *pw;           // per slot: work index
+    float4 *ring;            // finished paths' rgba by work index
+    long long ring_mask;
     long long *remaining;    // per job: paths not yet finished
     long long *t_end;        // per job: wall clock when its last path finished
     long long *t0;           // wall clock at the start of the launch
     unsigned long long *jstats;  // per job: J_STRIDE counters
+};
+
+struct ResolveSeg {
+    long long la, lb;   // work range of the job, relative to its start
+    long long tbase;    // first thread of the segment
+    int job, pad;
 };
 
 __global__ void k_binit(BatchView bv, WfView wv, int pool) {
@@ -65,76 +85,76 @@ __global__ void k_binit(BatchView bv, WfView wv, int pool) {
         bv.t_end[j] = 0;
         for (int k = 0; k < J_STRIDE; ++k) bv.jstats[(size_t)j * J_STRIDE + k] = 0;
     }
+    for (int k = gtid; k < CTL_HIST + BATCH_HIST; k += stride) bv.ctl[k] = 0;
     if (gtid == 0) {
         for (int k = 0; k < 16; ++k) wv.counts[k] = 0;
         for (int k = 0; k < 8; ++k) wv.stats[k] = 0;
         wv.counts[C_FINISH] = pool;  // every slot starts free
-        *bv.cursor = 0;
         *bv.t0 = wall_clock64();
     }
 }
 
-// Free slots (the finish list of the previous iteration) take new paths.
-__global__ void k_regen(BatchView bv, WfView wv, int q) {
-    const int n = wv.counts[C_FINISH];
-    const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int slot = wv.finq[i];
-        // one cursor atomic per wave
-        const unsigned long long mask = __ballot(true);
-        const int lane = __lane_id();
-        const int leader = __ffsll((long long)mask) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(bv.cursor, (unsigned long long)__popcll(mask));
-        const unsigned lo = __shfl((unsigned)(base & 0xffffffffu), leader);
-        const unsigned hi = __shfl((unsigned)(base >> 32), leader);
-        base = ((unsigned long long)hi << 32) | lo;
-        const long long w = (long long)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-        const bool valid = w < bv.total;
-        f3 d = mk3(0.f, 0.f, 1.f);
-        int job = 0;
-        if (valid) {
-            int a = 0, b = bv.n_jobs - 1;  // last job whose start <= w
-            while (a < b) {
-                int mid = (a + b + 1) >> 1;
-                if (bv.starts[mid] <= w) a = mid;
-                else b = mid - 1;
-            }
-            job = a;
-            const JobDev &jd = bv.jobs[a];
-            const long long local = w - jd.start;
-            const int wh = jd.sv.width * jd.sv.height;
-            const int pass = (int)(local / wh);
-            const int pixel = tiled_pixel((int)(local - (long long)pass * wh), jd.sv.width, jd.sv.height);
-            const int sample = jd.first_pass + pass;
-            PathRng rng;
-            rng.slots = nullptr;
-            rng.key = path_key(jd.seed, (uint32_t)pixel, (uint32_t)sample);
-            d = camera_dir(jd.sv, pixel, rng.u(0), rng.u(1));
-            bv.pjob[slot] = a;
-            bv.ppix[slot] = pixel;
-            bv.psmp[slot] = sample;
-            wv.T[slot] = make_float4(1.f, 1.f, 1.f, 1.f);
-            wv.La[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-            wv.Lb[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-            wv.flags[slot] = RT_CAMERA;
-        }
-        wave_count(&wv.stats[S_PATHS], valid);
-        const int qs = wave_reserve(&wv.counts[q], valid);
-        if (valid) {
-            wv.ray[q][2 * qs] = make_float4(0.f, 0.f, 0.f, 0.1f);
-            wv.ray[q][2 * qs + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
-            wv.qpath[q][qs] = slot;
-            wv.qjob[q][qs] = job;
-        }
-    }
+// The free slots are the previous iteration's finish list; all of them take
+// new paths while work remains.  Also starts the iteration's shadow and
+// finish lists empty.
+__global__ void k_plan(BatchView bv, WfView wv, int q) {
+    if (threadIdx.x != 0) return;
+    const long long live = wv.counts[q];
+    const long long nfree = wv.counts[C_FINISH];
+    const long long cur = bv.ctl[CTL_CURSOR];
+    long long n_new = bv.total - cur;
+    if (n_new > nfree) n_new = nfree;
+    if (n_new < 0) n_new = 0;
+    bv.ctl[CTL_LIVE] = live;
+    bv.ctl[CTL_NEW] = n_new;
+    bv.ctl[CTL_BASE] = cur;
+    bv.ctl[CTL_CURSOR] = cur + n_new;
+    const long long it = bv.ctl[CTL_ITER];
+    bv.ctl[CTL_HIST + (int)(it % BATCH_HIST)] = cur + n_new;
+    bv.ctl[CTL_ITER] = it + 1;
+    wv.counts[q] = (int)(live + n_new);
+    wv.counts[C_FINISH] = 0;
+    wv.counts[C_SHADOW] = 0;
+    wv.stats[S_PATHS] += (unsigned long long)n_new;
 }
 
-// The finish list has been consumed by regen; shade and shadow start empty.
-__global__ void k_bclear_lists(int *counts) {
-    if (threadIdx.x == 0) {
-        counts[C_FINISH] = 0;
-        counts[C_SHADOW] = 0;
+__global__ void k_regen(BatchView bv, WfView wv, int q) {
+    const int live = (int)bv.ctl[CTL_LIVE];
+    const int n_new = (int)bv.ctl[CTL_NEW];
+    const long long base = bv.ctl[CTL_BASE];
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_new; i += stride) {
+        const int slot = wv.finq[i];
+        const long long w = base + i;
+        int a = 0, b = bv.n_jobs - 1;  // last job whose start <= w
+        while (a < b) {
+            int mid = (a + b + 1) >> 1;
+            if (bv.starts[mid] <= w) a = mid;
+            else b = mid - 1;
+        }
+        const JobDev &jd = bv.jobs[a];
+        const long long local = w - jd.start;
+        const int wh = jd.sv.width * jd.sv.height;
+        const int pass = (int)(local / wh);
+        const int pixel = tiled_pixel((int)(local - (long long)pass * wh), jd.sv.width, jd.sv.height);
+        const int sample = jd.first_pass + pass;
+        PathRng rng;
+        rng.slots = nullptr;
+        rng.key = path_key(jd.seed, (uint32_t)pixel, (uint32_t)sample);
+        const f3 d = camera_dir(jd.sv, pixel, rng.u(0), rng.u(1));
+        bv.pjob[slot] = a;
+        bv.ppix[slot] = pixel;
+        bv.psmp[slot] = sample;
+        bv.pw[slot] = w;
+        wv.T[slot] = make_float4(1.f, 1.f, 1.f, 1.f);
+        wv.La[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.Lb[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.flags[slot] = RT_CAMERA;
+        const int pos = live + i;
+        wv.ray[q][2 * pos] = make_float4(0.f, 0.f, 0.f, 0.1f);
+        wv.ray[q][2 * pos + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
+        wv.qpath[q][pos] = slot;
+        wv.qjob[q][pos] = a;
     }
 }
 
@@ -148,10 +168,9 @@ __global__ void k_bclear_queue(int *counts, int q) {
 // traversal and shading code sees uniform scene pointers, as in the
 // single-scene kernels.
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_bextend(BatchView bv, const JobDev *__restrict__ jobs, WfView wv, int q) {
+__global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, WfView wv, int q) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[q];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_EXT], (unsigned long long)n);
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
@@ -167,17 +186,14 @@ __global__ void __launch_bounds__(B) k_bextend(BatchView bv, const JobDev *__res
             if (job == j) {
                 traverse<W, false>(static_cast<const WNode<W> *>(jobs[j].sv.nodes), jobs[j].sv.tris, mk3(a.x, a.y, a.z),
                                    mk3(b.x, b.y, b.z), a.w, b.w, st, h);
-                wave_count(&bv.jstats[(size_t)j * J_STRIDE + J_EXT], true);
-                wave_count(&bv.jstats[(size_t)j * J_STRIDE + J_HIT], h.prim >= 0);
                 break;
             }
         }
         wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
-        wave_count(&wv.stats[S_HIT], h.prim >= 0);
     }
 }
 
-__global__ void __launch_bounds__(256) k_bshade(BatchView bv, const JobDev *__restrict__ jobs, WfView wv, int q) {
+__global__ void __launch_bounds__(256, 5) k_bshade(BatchView bv, const JobDev *__restrict__ jobs, WfView wv, int q) {
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -202,7 +218,6 @@ template <int W, int B>
 __global__ void __launch_bounds__(B) k_bshadow(BatchView bv, const JobDev *__restrict__ jobs, WfView wv) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_SHADOW], (unsigned long long)n);
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
@@ -220,7 +235,6 @@ __global__ void __launch_bounds__(B) k_bshadow(BatchView bv, const JobDev *__res
             if (job == j) {
                 traverse<W, true>(static_cast<const WNode<W> *>(jobs[j].sv.nodes), jobs[j].sv.tris, mk3(a.x, a.y, a.z),
                                   mk3(b.x, b.y, b.z), a.w, b.w, st, h);
-                wave_count(&bv.jstats[(size_t)j * J_STRIDE + J_SHADOW], true);
                 break;
             }
         }
@@ -233,54 +247,115 @@ __global__ void __launch_bounds__(B) k_bshadow(BatchView bv, const JobDev *__res
     }
 }
 
-__device__ inline void job_done_count(const BatchView &bv, int job) {
-    const unsigned long long mask = __ballot(true);
+__device__ inline long long wave_sum(long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int lo = __shfl_xor((int)(v & 0xffffffffll), off);
+        const int hi = __shfl_xor((int)(v >> 32), off);
+        v += (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+    return v;
+}
+
+// Every lane of the wave runs this (lanes without a path pass job = -1).
+// Lanes of the wave's first job are summed with shuffles and added with one
+// atomic per counter; lanes of other jobs add their own.
+__device__ inline void finish_counts(const BatchView &bv, int job, int ext, int hit, int shadow) {
     const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
+    const unsigned long long act = __ballot(job >= 0);
+    if (!act) return;
+    const int leader = __ffsll((long long)act) - 1;
     const int j0 = __shfl(job, leader);
-    const unsigned long long same = __ballot(job == j0);
+    const bool mine = job == j0;
+    const long long n_e = wave_sum(mine ? ext : 0), n_h = wave_sum(mine ? hit : 0), n_s = wave_sum(mine ? shadow : 0);
+    const long long cnt = (long long)__popcll(__ballot(mine));
     if (lane == leader) {
-        const long long c = (long long)__popcll(same);
-        if (atomicAdd((unsigned long long *)&bv.remaining[j0], (unsigned long long)(-c)) == (unsigned long long)c)
+        unsigned long long *js = bv.jstats + (size_t)j0 * J_STRIDE;
+        atomicAdd(js + J_EXT, (unsigned long long)n_e);
+        atomicAdd(js + J_HIT, (unsigned long long)n_h);
+        atomicAdd(js + J_SHADOW, (unsigned long long)n_s);
+        if (atomicAdd((unsigned long long *)&bv.remaining[j0], (unsigned long long)(-cnt)) == (unsigned long long)cnt)
             bv.t_end[j0] = wall_clock64();
     }
-    if (job != j0) {
+    if (job >= 0 && !mine) {
+        unsigned long long *js = bv.jstats + (size_t)job * J_STRIDE;
+        atomicAdd(js + J_EXT, (unsigned long long)ext);
+        atomicAdd(js + J_HIT, (unsigned long long)hit);
+        atomicAdd(js + J_SHADOW, (unsigned long long)shadow);
         if (atomicAdd((unsigned long long *)&bv.remaining[job], (unsigned long long)(-1ll)) == 1ull)
             bv.t_end[job] = wall_clock64();
     }
 }
 
-__global__ void k_bfinish(BatchView bv, WfView wv) {
+__global__ void __launch_bounds__(256) k_bfinish(BatchView bv, WfView wv) {
     const int n = wv.counts[C_FINISH];
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int slot = wv.finq[i];
-        const float4 a = wv.La[slot], b = wv.Lb[slot];
-        const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), wv.flags[slot]);
-        const int job = bv.pjob[slot];
-        float *acc = reinterpret_cast<float *>(bv.jobs[job].accum + bv.ppix[slot]);
-        atomicAdd(acc + 0, L.x);
-        atomicAdd(acc + 1, L.y);
-        atomicAdd(acc + 2, L.z);
-        atomicAdd(acc + 3, a.w);
-        job_done_count(bv, job);
+    // whole waves iterate together so the per-wave sums see every lane
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        int job = -1, ext = 0, hit = 0, shadow = 0;
+        if (i < n) {
+            const int slot = wv.finq[i];
+            const float4 a = wv.La[slot], b = wv.Lb[slot];
+            const int fl = wv.flags[slot];
+            const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), fl);
+            bv.ring[bv.pw[slot] & bv.ring_mask] = make_float4(L.x, L.y, L.z, a.w);
+            job = bv.pjob[slot];
+            ext = path_ext_rays(fl);
+            hit = path_hit_rays(fl);
+            shadow = path_shadow_rays(fl);
+        }
+        finish_counts(bv, job, ext, hit, shadow);
+    }
+}
+
+// Adds the finished ring entries of the given work ranges to the jobs'
+// framebuffers: one thread per (segment, pixel), passes in increasing order.
+__global__ void k_bresolve(BatchView bv, const ResolveSeg *__restrict__ segs, int n_segs, long long n_threads) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_threads; t += stride) {
+        int a = 0, b = n_segs - 1;
+        while (a < b) {
+            int mid = (a + b + 1) >> 1;
+            if (segs[mid].tbase <= t) a = mid;
+            else b = mid - 1;
+        }
+        const ResolveSeg sg = segs[a];
+        const JobDev &jd = bv.jobs[sg.job];
+        const int wh = jd.sv.width * jd.sv.height;
+        const long long l = t - sg.tbase;
+        const long long p0 = sg.la / wh, p1 = (sg.lb - 1) / wh;
+        float4 *px = jd.accum + tiled_pixel((int)l, jd.sv.width, jd.sv.height);
+        float4 s = *px;
+        bool any = false;
+        for (long long p = p0; p <= p1; ++p) {
+            const long long idx = p * wh + l;
+            if (idx < sg.la || idx >= sg.lb) continue;
+            const float4 v = bv.ring[(jd.start + idx) & bv.ring_mask];
+            s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+            any = true;
+        }
+        if (any) *px = s;
     }
 }
 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static BatchView batch_view(Ctx *c, int n_jobs, long long total) {
+static BatchView batch_view(Ctx *c, int n_jobs, long long total, long long ring_mask) {
     BatchState &b = c->batch;
     BatchView v;
     v.jobs = b.jobs.as<JobDev>();
     v.starts = b.starts.as<long long>();
     v.n_jobs = n_jobs;
     v.total = total;
-    v.cursor = b.cursor.as<unsigned long long>();
+    v.ctl = b.ctl.as<long long>();
     v.pjob = b.pjob.as<int>();
     v.ppix = b.ppix.as<int>();
     v.psmp = b.psmp.as<int>();
+    v.pw = b.pw.as<long long>();
+    v.ring = b.ring.as<float4>();
+    v.ring_mask = ring_mask;
     v.remaining = b.remaining.as<long long>();
     v.t_end = b.t_end.as<long long>();
     v.t0 = b.t0.as<long long>();
@@ -288,13 +363,13 @@ static BatchView batch_view(Ctx *c, int n_jobs, long long total) {
     return v;
 }
 
-// One iteration: regen, extend, shade, shadow, finish over queue q.
+// One iteration over extension queue q.
 static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q, int width) {
     const int tb = trace_blocks(c);
     const int sb = c->n_cu * 4;
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, c->stream, bv, wv, q);
     hipLaunchKernelGGL(k_regen, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
-    hipLaunchKernelGGL(k_bclear_lists, dim3(1), dim3(64), 0, c->stream, wv.counts);
-#define AUR_BEXT(w_, b_) hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv, bv.jobs, wv, q)
+#define AUR_BEXT(w_, b_) hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs, wv, q)
 #define AUR_BSHD(w_, b_) hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv, bv.jobs, wv)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
     hipLaunchKernelGGL(k_bshade, dim3(sb), dim3(256), 0, c->stream, bv, bv.jobs, wv, q);
@@ -308,6 +383,34 @@ static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q
 static double job_cost(const Ctx *j, long long paths) {
     // relative cost estimate: paths x depth x tris^0.3
     return (double)paths * (j->scene.opt.maxdepth + 1) * std::pow((double)std::max(j->n_tris, 1), 0.3);
+}
+
+// Adds ring entries [from, to) (work indices) into the framebuffers.
+static void resolve_range(Ctx *c, const BatchView &bv, const std::vector<long long> &starts,
+                          const std::vector<long long> &npaths, const std::vector<int> &wh, long long from, long long to) {
+    BatchState &b = c->batch;
+    if (to <= from) return;
+    ResolveSeg *seg = static_cast<ResolveSeg *>(b.host_segs);
+    int n = 0;
+    long long threads = 0;
+    int r = (int)(std::upper_bound(starts.begin(), starts.end(), from) - starts.begin()) - 1;
+    for (r = std::max(r, 0); r < (int)starts.size() && starts[r] < to; ++r) {
+        const long long la = std::max(from, starts[r]) - starts[r];
+        const long long lb = std::min(to, starts[r] + npaths[r]) - starts[r];
+        if (la >= lb) continue;
+        seg[n].la = la;
+        seg[n].lb = lb;
+        seg[n].tbase = threads;
+        seg[n].job = r;
+        seg[n].pad = 0;
+        threads += wh[r];  // one thread per pixel of the job
+        ++n;
+    }
+    if (n == 0) return;
+    HIP_CHECK(hipMemcpyAsync(b.segs.p, b.host_segs, sizeof(ResolveSeg) * n, hipMemcpyHostToDevice, c->stream));
+    const long long blocks = std::min<long long>((threads + 255) / 256, (long long)c->n_cu * 8);
+    hipLaunchKernelGGL(k_bresolve, dim3((unsigned)blocks), dim3(256), 0, c->stream, bv, b.segs.as<ResolveSeg>(), n, threads);
+    HIP_CHECK(hipGetLastError());
 }
 
 void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_passes, int n_jobs, int order, int use_graph,
@@ -333,16 +436,17 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return job_cost(jobs[a], paths[a]) < job_cost(jobs[b], paths[b]); });
     // job records and framebuffers
     std::vector<JobDev> jd(n_jobs);
-    std::vector<long long> starts(n_jobs);
+    std::vector<long long> starts(n_jobs), npaths(n_jobs);
+    std::vector<int> wh(n_jobs);
     long long total = 0;
     for (int r = 0; r < n_jobs; ++r) {
         Ctx *j = jobs[ord[r]];
         HIP_CHECK(hipStreamSynchronize(j->stream));
         const int w = j->scene.opt.width, h = j->scene.opt.height;
-        const size_t wh = (size_t)w * h;
+        wh[r] = w * h;
         bool resized = j->fb_w != w || j->fb_h != h;
-        j->accum.reserve(sizeof(float4) * wh);
-        if (first_pass[ord[r]] == 0 || resized) fill_zero(c, j->accum.p, sizeof(float4) * wh);
+        j->accum.reserve(sizeof(float4) * (size_t)wh[r]);
+        if (first_pass[ord[r]] == 0 || resized) fill_zero(c, j->accum.p, sizeof(float4) * (size_t)wh[r]);
         j->fb_w = w;
         j->fb_h = h;
         jd[r].sv = scene_view(j);
@@ -352,11 +456,18 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         jd[r].start = total;
         jd[r].n_paths = paths[ord[r]];
         starts[r] = total;
+        npaths[r] = paths[ord[r]];
         total += paths[ord[r]];
     }
     // buffers
     const size_t pool = std::max<size_t>(64, std::min<size_t>(c->launch.batch_pool_paths, (size_t)std::max<long long>(total, 64)));
     if (pool > (size_t)0x3fffffff) throw CapacityError("path pool too large");
+    // iterations between polls, and the lag after which issued work has finished
+    const int lag = max_path + 1;
+    const int pairs_per_poll = std::max(1, std::min((iters_per_poll > 0 ? iters_per_poll : 8), BATCH_HIST - lag - 4) / 2);
+    // the ring holds every item between the resolved prefix and the cursor
+    size_t ring = 1;
+    while (ring < pool * (size_t)(2 * pairs_per_poll + lag + 2)) ring <<= 1;
     ensure_wavefront_for(c, pool, width, depth);
     BatchState &b = c->batch;
     b.jobs.reserve(sizeof(JobDev) * n_jobs);
@@ -364,14 +475,25 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     b.remaining.reserve(sizeof(long long) * n_jobs);
     b.t_end.reserve(sizeof(long long) * n_jobs);
     b.jstats.reserve(sizeof(unsigned long long) * J_STRIDE * n_jobs);
-    b.cursor.reserve(sizeof(unsigned long long));
+    b.ctl.reserve(sizeof(long long) * (CTL_HIST + BATCH_HIST));
     b.t0.reserve(sizeof(long long));
     b.pjob.reserve(sizeof(int) * pool);
     b.ppix.reserve(sizeof(int) * pool);
     b.psmp.reserve(sizeof(int) * pool);
+    b.pw.reserve(sizeof(long long) * pool);
     b.qjob[0].reserve(sizeof(int) * pool);
     b.qjob[1].reserve(sizeof(int) * pool);
-    if (!b.host_status) HIP_CHECK(hipHostMalloc(&b.host_status, 256, hipHostMallocDefault));
+    b.ring.reserve(sizeof(float4) * ring);
+    b.segs.reserve(sizeof(ResolveSeg) * n_jobs);
+    if (!b.host_status) HIP_CHECK(hipHostMalloc(&b.host_status, sizeof(long long) * (CTL_HIST + BATCH_HIST + 8), hipHostMallocDefault));
+    if (b.host_segs_cap < (size_t)n_jobs) {
+        if (b.host_segs) (void)hipHostFree(b.host_segs);
+        b.host_segs = nullptr;
+        b.host_segs_cap = 0;
+        HIP_CHECK(hipHostMalloc(&b.host_segs, sizeof(ResolveSeg) * (size_t)n_jobs, hipHostMallocDefault));
+        b.host_segs_cap = (size_t)n_jobs;
+    }
+    // the copies read host vectors that outlive the stream synchronisation below
     HIP_CHECK(hipMemcpyAsync(b.jobs.p, jd.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(b.starts.p, starts.data(), sizeof(long long) * n_jobs, hipMemcpyHostToDevice, c->stream));
 
@@ -383,7 +505,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     WfView wv = wf_view(c, src);
     wv.qjob[0] = b.qjob[0].as<int>();
     wv.qjob[1] = b.qjob[1].as<int>();
-    BatchView bv = batch_view(c, n_jobs, total);
+    BatchView bv = batch_view(c, n_jobs, total, (long long)ring - 1);
 
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
@@ -395,65 +517,77 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
 
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    if (use_graph) {
-        HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        batch_iteration(c, bv, wv, 0, width);
-        batch_iteration(c, bv, wv, 1, width);
-        hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-        if (ec != hipSuccess) throw HipError(std::string("hipStreamEndCapture failed: ") + hipGetErrorString(ec));
-        HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-    }
-    const int pairs_per_poll = std::max(1, (iters_per_poll > 0 ? iters_per_poll : 8) / 2);
-    const long long *st = static_cast<const long long *>(b.host_status);
+    auto release = [&]() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        e0 = e1 = nullptr;
+    };
+    long long *hs = static_cast<long long *>(b.host_status);
+    const int *hcounts = reinterpret_cast<const int *>(hs + CTL_HIST + BATCH_HIST);
     // A path holds its slot for at most maxdepth + 1 iterations, so every slot
     // completes a path at least that often and the launch drains within
     // (total / pool + 1) * (maxdepth + 1) iterations; twice that guards the loop.
     const long long max_iters = 2 * (total / (long long)pool + 1) * (long long)max_path + 64;
-    long long iters = 0;
-    for (;;) {
-        for (int k = 0; k < pairs_per_poll; ++k) {
-            if (gexec) HIP_CHECK(hipGraphLaunch(gexec, c->stream));
-            else {
-                batch_iteration(c, bv, wv, 0, width);
-                batch_iteration(c, bv, wv, 1, width);
-            }
-            iters += 2;
-        }
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(b.host_status, b.cursor.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-        HIP_CHECK(hipMemcpyAsync(static_cast<char *>(b.host_status) + 8, wv.counts, sizeof(int) * 4, hipMemcpyDeviceToHost,
-                                 c->stream));
-        HIP_CHECK(hipStreamSynchronize(c->stream));
-        const unsigned long long cur = *reinterpret_cast<const unsigned long long *>(st);
-        const int *cnt = reinterpret_cast<const int *>(static_cast<const char *>(b.host_status) + 8);
-        // after a pair of iterations the live paths sit in queue 0
-        if ((long long)cur >= total && cnt[C_Q0] == 0) break;
-        if (iters > max_iters) {
-            if (gexec) (void)hipGraphExecDestroy(gexec);
-            if (graph) (void)hipGraphDestroy(graph);
-            throw StateError("batched launch did not drain");
-        }
-    }
-    HIP_CHECK(hipEventRecord(e1, c->stream));
-    HIP_CHECK(hipEventSynchronize(e1));
+    long long resolved = 0;
     float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    try {
+        if (use_graph) {
+            HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            batch_iteration(c, bv, wv, 0, width);
+            batch_iteration(c, bv, wv, 1, width);
+            HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
+            HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        for (;;) {
+            for (int k = 0; k < pairs_per_poll; ++k) {
+                if (gexec) HIP_CHECK(hipGraphLaunch(gexec, c->stream));
+                else {
+                    batch_iteration(c, bv, wv, 0, width);
+                    batch_iteration(c, bv, wv, 1, width);
+                }
+            }
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(hs, b.ctl.p, sizeof(long long) * (CTL_HIST + BATCH_HIST), hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipMemcpyAsync(hs + CTL_HIST + BATCH_HIST, wv.counts, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            const long long cur = hs[CTL_CURSOR], iters = hs[CTL_ITER];
+            // after a pair of iterations the live paths sit in queue 0
+            const bool done = cur >= total && hcounts[C_Q0] == 0;
+            long long safe = 0;
+            if (done) safe = total;
+            else if (iters - 1 - lag >= 0) safe = hs[CTL_HIST + (int)((iters - 1 - lag) % BATCH_HIST)];
+            if (safe > resolved) {
+                resolve_range(c, bv, starts, npaths, wh, resolved, safe);
+                resolved = safe;
+            }
+            if (done) break;
+            if (iters > max_iters) throw StateError("batched launch did not drain");
+        }
+        HIP_CHECK(hipEventRecord(e1, c->stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    } catch (...) {
+        (void)hipStreamSynchronize(c->stream);
+        release();
+        throw;
+    }
+    release();
 
     std::vector<long long> t_end(n_jobs), rem(n_jobs);
     std::vector<unsigned long long> js((size_t)n_jobs * J_STRIDE);
     long long t0 = 0;
-    unsigned long long s[8];
     HIP_CHECK(hipMemcpy(t_end.data(), b.t_end.p, sizeof(long long) * n_jobs, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(rem.data(), b.remaining.p, sizeof(long long) * n_jobs, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(js.data(), b.jstats.p, sizeof(unsigned long long) * js.size(), hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(&t0, b.t0.p, sizeof t0, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(s, c->wf.stats.p, sizeof s, hipMemcpyDeviceToHost));
     int rate_khz = 0;
     HIP_CHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device));
+    aur_stats tot;
+    std::memset(&tot, 0, sizeof tot);
     for (int r = 0; r < n_jobs; ++r) {
         if (rem[r] != 0) throw StateError("batched launch left unfinished paths");
         aur_job_result &o = results[ord[r]];
@@ -463,14 +597,13 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         o.shadow_rays = (int64_t)js[(size_t)r * J_STRIDE + J_SHADOW];
         o.latency_ms = (paths[ord[r]] > 0 && rate_khz > 0) ? (double)(t_end[r] - t0) / (double)rate_khz : 0.0;
         o.order = r;
+        tot.paths += o.paths;
+        tot.ext_rays += o.ext_rays;
+        tot.hit_rays += o.hit_rays;
+        tot.shadow_rays += o.shadow_rays;
     }
-    if (total_stats) {
-        total_stats->paths = (int64_t)s[S_PATHS];
-        total_stats->ext_rays = (int64_t)s[S_EXT];
-        total_stats->hit_rays = (int64_t)s[S_HIT];
-        total_stats->shadow_rays = (int64_t)s[S_SHADOW];
-        total_stats->ms = ms;
-    }
+    tot.ms = ms;
+    if (total_stats) *total_stats = tot;
 }
 
 }  // namespace aur

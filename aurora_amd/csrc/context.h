@@ -114,15 +114,19 @@ struct LaunchCfg {
 
 // Device state of a batched launch (batch.hip), kept between calls.
 struct BatchState {
-    DevBuf jobs, starts, remaining, t_end, jstats, cursor, t0;
-    DevBuf pjob, ppix, psmp;        // per path slot
+    DevBuf jobs, starts, remaining, t_end, jstats, ctl, t0, segs;
+    DevBuf pjob, ppix, psmp, pw;    // per path slot
     DevBuf qjob[2];                 // job of each extension-queue entry
-    void *host_status = nullptr;    // pinned: cursor + queue counts, polled by the host
+    DevBuf ring;                    // finished paths' rgba by work index
+    void *host_status = nullptr;    // pinned: control block + queue counts, polled by the host
+    void *host_segs = nullptr;      // pinned: resolve segment table
+    size_t host_segs_cap = 0;
     BatchState() = default;
     BatchState(const BatchState &) = delete;
     BatchState &operator=(const BatchState &) = delete;
     ~BatchState() {
         if (host_status) (void)hipHostFree(host_status);
+        if (host_segs) (void)hipHostFree(host_segs);
     }
 };
 

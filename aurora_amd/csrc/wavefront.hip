@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv) {
 // ---------------------------------------------------------------------------
 // shade: one path vertex per queue entry
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_shade(SceneView sv, WfView wv, int q) {
+__global__ void __launch_bounds__(256, 5) k_shade(SceneView sv, WfView wv, int q) {
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)

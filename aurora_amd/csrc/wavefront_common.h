@@ -30,7 +30,16 @@ int trace_blocks(const Ctx *c);
 size_t trace_threads(const Ctx *c);
 
 enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2 };
-enum { F_MATTE = 1 << 2, F_NOCLAMP = 1 << 24 };
+// Path flags: bits 0-1 ray type, 2 matte path, 3 last ray missed, 8-15
+// vertex index, 16-23 bounces, 24 no firefly clamp, 25-30 shadow rays traced.
+enum { F_MATTE = 1 << 2, F_MISS = 1 << 3, F_NOCLAMP = 1 << 24 };
+#define F_SHADOW_SHIFT 25
+#define F_SHADOW_MASK (63 << F_SHADOW_SHIFT)
+// rays one finished path traced, from its flags: extension rays (camera ray
+// included), those that hit, and shadow rays
+__device__ inline int path_ext_rays(int flags) { return ((flags >> 8) & 0xff) + 1; }
+__device__ inline int path_hit_rays(int flags) { return path_ext_rays(flags) - ((flags & F_MISS) ? 1 : 0); }
+__device__ inline int path_shadow_rays(int flags) { return (flags >> F_SHADOW_SHIFT) & 63; }
 enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_FINISH = 3 };
 enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
 
@@ -125,6 +134,7 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
     const float4 hv = wv.hit[i];
     const int prim = __float_as_int(hv.w);
     int flags = wv.flags[path];
+    const int shadow_before = (flags >> F_SHADOW_SHIFT) & 63;
     const int rtype = flags & 3;
     const int tb = (flags >> 8) & 0xff;
     bool cont = false, finish = false;
@@ -245,6 +255,12 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
                 }
             }
         }
+    }
+    if (prim < 0) flags |= F_MISS;
+    {
+        const int n_sh = (sh[0].valid ? 1 : 0) + (sh[1].valid ? 1 : 0);
+        const int traced = (shadow_before + n_sh) & 63;
+        flags = (flags & ~F_SHADOW_MASK) | (traced << F_SHADOW_SHIFT);
     }
     wv.flags[path] = flags;
     // compacted writes: next extension queue, shadow queue, finish list

@@ -19,6 +19,9 @@ gradient against central differences on 64 seeded parameters.  ``batch`` is
 the batched uneven workload (64 seeded scenes of 1e3..5e5 triangles at
 128^2..1024^2, 4..256 spp, depth 3..12): aggregate Mpaths/s of the batched
 persistent launch and of per-scene launches, and per-job latency p50/p99.
+``box_room`` is the CPU-runnable 256x256, 16 spp, depth-5 box room: GPU and
+host-CPU Mpaths/s on the same frame, with both images' RMSE against a
+4096-spp oracle reference.
 """
 from __future__ import annotations
 
@@ -126,6 +129,70 @@ def cpu_baseline(exp: dict, target_s: float, threads: int, spp: int):
             "mrays_per_s": round((cnt["ext_rays"] + cnt["shadow_rays"]) / dt / 1e6, 4)}
 
 
+def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int = 4):
+    """The CPU-runnable box room (256x256, 16 spp, depth 5): GPU Mpaths/s over
+    repeated full frames, the oracle's Mpaths/s on the host cores (median of
+    three full frames after a warm-up), and both 16-spp images' RMSE against a 4096-spp oracle reference
+    (independent seed, float64 mean) on every 4th pixel of every 4th row."""
+    from aurora_amd import scenes, tuning
+    from aurora_amd.renderer import Renderer
+    from oracle import oracle as O
+    O.build()
+    tmp = tempfile.mkdtemp(prefix="aurora_c1_")
+    path = scenes.write_config("box_room", tmp)
+    r = Renderer(path, os.path.join(tmp, "config.json"))
+    r.build()
+    cfg = tuning.choose(r)
+    if (cfg["bvh_width"], cfg["max_leaf"]) != (4, 4):
+        r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
+    tuning.apply_launch(r, cfg)
+    info = r.info()
+    W, H, spp = info["width"], info["height"], info["pixelsamples"]
+    r.set_seed(11)
+    r.render_passes(0, spp, fetch=False)
+    _hip_sync()
+    t0 = time.perf_counter()
+    for k in range(reps):
+        acc, st = r.render_passes(0, spp, fetch=False)
+    _hip_sync()
+    gpu_s = time.perf_counter() - t0
+    gpu_rate = reps * st["paths"] / gpu_s / 1e6
+    gpu_img = r.framebuffer()[..., :3].astype(np.float64) / spp
+    exp = r.export()
+    r.close()
+    sc = O.Scene(exp, bvh=True)
+    sc.render(12, 0, 1, threads=threads)  # warm-up
+    times = []
+    for _ in range(3):  # median of three full frames
+        t0 = time.time()
+        cacc, _ = sc.render(12, 0, spp, threads=threads)
+        times.append(time.time() - t0)
+    cpu_s = float(np.median(times))
+    cpu_rate = W * H * spp / cpu_s / 1e6
+    cpu_img = cacc[..., :3].astype(np.float64) / spp
+    ys, xs = np.mgrid[0:H:stride, 0:W:stride]
+    pix = (ys * W + xs).ravel().astype(np.int32)
+    ref = np.zeros((pix.size, 3))
+    t0 = time.time()
+    chunk = 256
+    for c0 in range(0, ref_spp, chunk):
+        P = np.repeat(pix, chunk)
+        S = np.tile(np.arange(c0, c0 + chunk, dtype=np.int32), pix.size)
+        o, _ = sc.render_paths(999, P, S, threads=threads)
+        ref += o[:, :3].astype(np.float64).reshape(pix.size, chunk, 3).sum(1)
+    ref /= ref_spp
+    ref_s = time.time() - t0
+    sub = (ys.ravel(), xs.ravel())
+    rmse_g = float(np.sqrt(np.mean((gpu_img[sub] - ref) ** 2)))
+    rmse_c = float(np.sqrt(np.mean((cpu_img[sub] - ref) ** 2)))
+    return {"workload": "box_room_256_spp16_depth5", "paths_per_frame": W * H * spp,
+            "gpu_Mpaths_s": round(gpu_rate, 3), "gpu_frames_timed": reps,
+            "cpu_Mpaths_s": round(cpu_rate, 4), "cpu_cores": threads, "cpu_s": round(cpu_s, 2),
+            "gpu_vs_cpu": round(gpu_rate / cpu_rate, 2),
+            "rmse_gpu_vs_ref": float("%.4g" % rmse_g), "rmse_cpu_vs_ref": float("%.4g" % rmse_c),
+            "ref": "%d-spp oracle on %d pixels (every %dth row and column), %.1f s" % (ref_spp, pix.size, stride, ref_s)}
+
+
 def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int = 0):
     """Inverse-rendering workload: forward and path-replay adjoint timing on the
     Kelemen box room, plus a central-difference check (h = 1e-3) of the
@@ -202,7 +269,7 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
             "fd_checked": int(len(pick)), "fd_h": h, "fd_max_rel_err": float("%.3g" % worst)}
 
 
-def batch_leg(n_jobs: int = 64, seed: int = 3):
+def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
     """Batched uneven workload: the same jobs rendered once through the batched
     persistent launch and once as per-scene launches (one aur_render per job,
     in the same order), with per-job latency from the start of the run."""
@@ -228,12 +295,12 @@ def batch_leg(n_jobs: int = 64, seed: int = 3):
     tuning.apply_launch(ex, cfg)
     # warm both launch paths: one batched pass, and per job the passes that
     # size its own working set
-    batch_submit(ex, [(r, 0, 1) for r in rs])
+    batch_submit(ex, [(r, 0, 1) for r in rs], use_graph=use_graph)
     for r, i, n in zip(rs, info, spp):
         r.render_passes(0, min(n, max(1, (8 << 20) // (i["width"] * i["height"]))), fetch=False)
     _hip_sync()
     t0 = time.perf_counter()
-    res, tot = batch_submit(ex, [(r, 0, n) for r, n in zip(rs, spp)])
+    res, tot = batch_submit(ex, [(r, 0, n) for r, n in zip(rs, spp)], use_graph=use_graph)
     batched_s = time.perf_counter() - t0
     _progress("batch: batched launch %.2f s" % batched_s)
     order = sorted(range(n_jobs), key=lambda k: res[k]["order"])
@@ -258,7 +325,8 @@ def batch_leg(n_jobs: int = 64, seed: int = 3):
     return {"workload": "%d seeded jobs (seed %d): %d..%d triangles, 128^2..1024^2, spp 4..256, depth 3..12"
                         % (n_jobs, seed, min(tris), max(tris)),
             "paths": total, "rays": rays, "scene_build_s": round(build_s, 2),
-            "pool_paths": cfg["pool_paths"], "bvh_width": cfg["bvh_width"], "trace_waves_per_cu": cfg["trace_waves_per_cu"],
+            "pool_paths": cfg["pool_paths"], "bvh_width": cfg["bvh_width"], "max_leaf": cfg["max_leaf"],
+            "trace_block": cfg["trace_block"], "trace_waves_per_cu": cfg["trace_waves_per_cu"], "graph_replay": use_graph,
             "batched_Mpaths_s": round(total / batched_s / 1e6, 3), "per_scene_Mpaths_s": round(total / single_s / 1e6, 3),
             "batched_Mrays_s": round(rays / batched_s / 1e6, 3),
             "speedup": round(single_s / batched_s, 3),
@@ -297,6 +365,8 @@ def main(argv=None):
     ap.add_argument("--levels", type=int, default=7)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-plain-launches", action="store_true",
+                    help="batched launch without graph replay (for kernel tracing)")
     ap.add_argument("--adam-steps", type=int, default=200,
                     help="Adam steps of the inverse-rendering run inside the adjoint leg (0: skip)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -378,8 +448,11 @@ def main(argv=None):
     }
     exp = None if args.no_cpu else r.export()
     r.close()
+    if not args.no_cpu:
+        out["box_room"] = box_room_leg(min(16, os.cpu_count() or 1))
+        _progress("box room leg done")
     if not args.no_batch:
-        out["batch"] = batch_leg()
+        out["batch"] = batch_leg(use_graph=not args.batch_plain_launches)
         _progress("batch leg done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(512, 32, 6, adam_steps=args.adam_steps)
