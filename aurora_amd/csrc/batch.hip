@@ -9,17 +9,16 @@
 // tiled_pixel((w - start_j) mod (W*H)), the order of a single-scene launch.
 // Per iteration:
 //
-//   plan    -- one thread: as many new paths as there are free slots (the
-//              previous iteration's finished paths) come off the work cursor
-//   regen   -- the new paths' camera rays are appended to the extension
-//              queue, coalesced, no atomics
+//   plan    -- one thread: as many new paths as the pool has room for (the
+//              previous iteration's finished paths left it) come off the
+//              work cursor
+//   regen   -- the new paths' camera rays and state are appended to the
+//              extension queue, coalesced, no atomics
 //   extend, shade, shadow -- the single-scene kernels' bodies; each wave
 //              loops once per distinct job among its lanes (almost always
 //              one) with that job's scene record read wave-uniformly
-//   finish  -- firefly clamp, and the path's RGBA goes to a ring buffer at
-//              its work index; per-job ray counts and a per-job countdown
-//              (the last path of a job stamps the wall clock) are summed per
-//              wave
+//   finish  -- firefly clamp; the path's RGBA and its packed ray counts and
+//              finishing iteration go to ring buffers at its work index
 //
 // so the GPU stays full until the last job drains, whatever the jobs' sizes,
 // resolutions, sample counts and depths.  An iteration is a fixed sequence of
@@ -29,7 +28,9 @@
 // maxdepth + 1 iterations ago has finished, and that prefix of the ring is
 // added into the jobs' framebuffers pixel by pixel in pass order -- the same
 // float additions, in the same order, as a single-scene render of the same
-// passes, so each job's framebuffer is bit-identical to it.
+// passes, so each job's framebuffer is bit-identical to it -- and summed into
+// per-job ray counts and the job's completion time (the wall clock stamped at
+// the start of the iteration after its last path finished).
 #include "wavefront_common.h"
 
 #include <algorithm>
@@ -48,11 +49,19 @@ struct JobDev {
     long long n_paths;
 };
 
-enum { J_EXT = 0, J_HIT = 1, J_SHADOW = 2, J_STRIDE = 4 };
-// control block: cursor, iterations done, the current iteration's plan, and
-// the cursor after each of the last BATCH_HIST iterations
-enum { CTL_CURSOR = 0, CTL_ITER = 1, CTL_LIVE = 2, CTL_NEW = 3, CTL_BASE = 4, CTL_HIST = 8 };
+enum { J_EXT = 0, J_HIT = 1, J_SHADOW = 2, J_PATHS = 3, J_STRIDE = 4 };
+// control block: cursor, iterations done, the current iteration's plan, the
+// cursor after each of the last BATCH_HIST iterations and the wall clock at
+// the start of each of them
 #define BATCH_HIST 64
+enum { CTL_CURSOR = 0, CTL_ITER = 1, CTL_LIVE = 2, CTL_NEW = 3, CTL_BASE = 4, CTL_HIST = 8,
+       CTL_TIME = CTL_HIST + BATCH_HIST, CTL_SIZE = CTL_TIME + BATCH_HIST };
+// per finished path, next to its rgba: ray counts and the iteration it
+// finished in (mod BATCH_HIST)
+__device__ inline unsigned pack_counts(int ext, int hit, int shadow, long long iter) {
+    return (unsigned)min(ext, 511) | ((unsigned)min(hit, 511) << 9) | ((unsigned)shadow << 18) |
+           ((unsigned)(iter % BATCH_HIST) << 24);
+}
 
 struct BatchView {
     const JobDev *jobs;      // in launch order
@@ -60,12 +69,11 @@ struct BatchView {
     int n_jobs;
     long long total;         // paths over all jobs
     long long *ctl;
-    int *pjob, *ppix, *psmp; // per slot: job (launch order), pixel, sample
-    long long *pw;           // per slot: work index
+    int pool;                // paths in flight at most
     float4 *ring;            // finished paths' rgba by work index
+    unsigned *ring_cnt;      // finished paths' packed counts by work index
     long long ring_mask;
-    long long *remaining;    // per job: paths not yet finished
-    long long *t_end;        // per job: wall clock when its last path finished
+    unsigned long long *t_end;  // per job: wall clock after its last path finished
     long long *t0;           // wall clock at the start of the launch
     unsigned long long *jstats;  // per job: J_STRIDE counters
 };
@@ -76,34 +84,30 @@ struct ResolveSeg {
     int job, pad;
 };
 
-__global__ void k_binit(BatchView bv, WfView wv, int pool) {
+__global__ void k_binit(BatchView bv, WfView wv) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
-    for (int i = gtid; i < pool; i += stride) wv.finq[i] = i;
     for (int j = gtid; j < bv.n_jobs; j += stride) {
-        bv.remaining[j] = bv.jobs[j].n_paths;
         bv.t_end[j] = 0;
         for (int k = 0; k < J_STRIDE; ++k) bv.jstats[(size_t)j * J_STRIDE + k] = 0;
     }
-    for (int k = gtid; k < CTL_HIST + BATCH_HIST; k += stride) bv.ctl[k] = 0;
+    for (int k = gtid; k < CTL_SIZE; k += stride) bv.ctl[k] = 0;
     if (gtid == 0) {
         for (int k = 0; k < 16; ++k) wv.counts[k] = 0;
         for (int k = 0; k < 8; ++k) wv.stats[k] = 0;
-        wv.counts[C_FINISH] = pool;  // every slot starts free
         *bv.t0 = wall_clock64();
     }
 }
 
-// The free slots are the previous iteration's finish list; all of them take
-// new paths while work remains.  Also starts the iteration's shadow and
-// finish lists empty.
+// The pool's free places (the paths that finished in the previous iteration)
+// take new paths while work remains.  Also starts the iteration's shadow and
+// finish lists empty (the previous finish list has been written out).
 __global__ void k_plan(BatchView bv, WfView wv, int q) {
     if (threadIdx.x != 0) return;
     const long long live = wv.counts[q];
-    const long long nfree = wv.counts[C_FINISH];
     const long long cur = bv.ctl[CTL_CURSOR];
     long long n_new = bv.total - cur;
-    if (n_new > nfree) n_new = nfree;
+    if (n_new > bv.pool - live) n_new = bv.pool - live;
     if (n_new < 0) n_new = 0;
     bv.ctl[CTL_LIVE] = live;
     bv.ctl[CTL_NEW] = n_new;
@@ -111,6 +115,7 @@ __global__ void k_plan(BatchView bv, WfView wv, int q) {
     bv.ctl[CTL_CURSOR] = cur + n_new;
     const long long it = bv.ctl[CTL_ITER];
     bv.ctl[CTL_HIST + (int)(it % BATCH_HIST)] = cur + n_new;
+    bv.ctl[CTL_TIME + (int)(it % BATCH_HIST)] = wall_clock64();
     bv.ctl[CTL_ITER] = it + 1;
     wv.counts[q] = (int)(live + n_new);
     wv.counts[C_FINISH] = 0;
@@ -124,7 +129,6 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
     const long long base = bv.ctl[CTL_BASE];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_new; i += stride) {
-        const int slot = wv.finq[i];
         const long long w = base + i;
         int a = 0, b = bv.n_jobs - 1;  // last job whose start <= w
         while (a < b) {
@@ -136,26 +140,26 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
         const long long local = w - jd.start;
         const int wh = jd.sv.width * jd.sv.height;
         const int pass = (int)(local / wh);
-        const int pixel = tiled_pixel((int)(local - (long long)pass * wh), jd.sv.width, jd.sv.height);
-        const int sample = jd.first_pass + pass;
+        const int l = (int)(local - (long long)pass * wh);
+        const int pixel = tiled_pixel(l, jd.sv.width, jd.sv.height);
         PathRng rng;
         rng.slots = nullptr;
-        rng.key = path_key(jd.seed, (uint32_t)pixel, (uint32_t)sample);
+        rng.key = path_key(jd.seed, (uint32_t)pixel, (uint32_t)(jd.first_pass + pass));
         const f3 d = camera_dir(jd.sv, pixel, rng.u(0), rng.u(1));
-        bv.pjob[slot] = a;
-        bv.ppix[slot] = pixel;
-        bv.psmp[slot] = sample;
-        bv.pw[slot] = w;
-        wv.T[slot] = make_float4(1.f, 1.f, 1.f, 1.f);
-        wv.La[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-        wv.Lb[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-        wv.flags[slot] = RT_CAMERA;
         const int pos = live + i;
         wv.ray[q][2 * pos] = make_float4(0.f, 0.f, 0.f, 0.1f);
         wv.ray[q][2 * pos + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
-        wv.qpath[q][pos] = slot;
-        wv.qjob[q][pos] = a;
+        wv.T[q][pos] = make_float4(1.f, 1.f, 1.f, 1.f);
+        wv.La[q][pos] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.Lb[q][pos] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.meta[q][pos] = make_int4(RT_CAMERA, l, pass, a);
     }
+}
+
+// Stamps the start of the iteration that would follow the last one, so paths
+// that finished in the last iteration get a completion time too.
+__global__ void k_bstamp(BatchView bv) {
+    if (threadIdx.x == 0) bv.ctl[CTL_TIME + (int)(bv.ctl[CTL_ITER] % BATCH_HIST)] = wall_clock64();
 }
 
 __global__ void k_bclear_queue(int *counts, int q) {
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, 
     st.stride = B;
     st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
     for (int i = gtid; i < n; i += stride) {
-        const int job = wv.qjob[q][i];
+        const int job = wv.meta[q][i].w;
         const float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
         TravHit h;
         for (;;) {
@@ -193,19 +197,20 @@ __global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, 
     }
 }
 
-__global__ void __launch_bounds__(256, 5) k_bshade(BatchView bv, const JobDev *__restrict__ jobs, WfView wv, int q) {
+__global__ void __launch_bounds__(256, 5) k_bshade(const JobDev *__restrict__ jobs, WfView wv, int q) {
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int job = wv.qjob[q][i];
+        const int job = wv.meta[q][i].w;
         for (;;) {
             const int j = __builtin_amdgcn_readfirstlane(job);
             if (job == j) {
-                const uint32_t seed = jobs[j].seed;
-                shade_one(jobs[j].sv, wv, q, i, [&](int p) {
+                const JobDev &jd = jobs[j];
+                shade_one(jd.sv, wv, q, i, [&](int4 m) {
                     PathRng r;
                     r.slots = nullptr;
-                    r.key = path_key(seed, (uint32_t)bv.ppix[p], (uint32_t)bv.psmp[p]);
+                    r.key = path_key(jd.seed, (uint32_t)tiled_pixel(m.y, jd.sv.width, jd.sv.height),
+                                     (uint32_t)(jd.first_pass + m.z));
                     return r;
                 }, j);
                 break;
@@ -215,7 +220,7 @@ __global__ void __launch_bounds__(256, 5) k_bshade(BatchView bv, const JobDev *_
 }
 
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_bshadow(BatchView bv, const JobDev *__restrict__ jobs, WfView wv) {
+__global__ void __launch_bounds__(B) k_bshadow(const JobDev *__restrict__ jobs, WfView wv, int nq) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -226,86 +231,79 @@ __global__ void __launch_bounds__(B) k_bshadow(BatchView bv, const JobDev *__res
     st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
     for (int i = gtid; i < n; i += stride) {
         const float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
-        const int tag = __float_as_int(c.w);
-        const int path = tag & 0x7fffffff;
-        const int job = bv.pjob[path];
+        const int job = __float_as_int(a.w);
         TravHit h;
         for (;;) {
             const int j = __builtin_amdgcn_readfirstlane(job);
             if (job == j) {
                 traverse<W, true>(static_cast<const WNode<W> *>(jobs[j].sv.nodes), jobs[j].sv.tris, mk3(a.x, a.y, a.z),
-                                  mk3(b.x, b.y, b.z), a.w, b.w, st, h);
+                                  mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h);
                 break;
             }
         }
-        if (h.prim < 0) {
-            float4 *acc = (tag < 0) ? wv.Lb : wv.La;
-            float4 v = acc[path];
-            v.x += c.x; v.y += c.y; v.z += c.z;
-            acc[path] = v;
-        }
-    }
-}
-
-__device__ inline long long wave_sum(long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const int lo = __shfl_xor((int)(v & 0xffffffffll), off);
-        const int hi = __shfl_xor((int)(v >> 32), off);
-        v += (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-    }
-    return v;
-}
-
-// Every lane of the wave runs this (lanes without a path pass job = -1).
-// Lanes of the wave's first job are summed with shuffles and added with one
-// atomic per counter; lanes of other jobs add their own.
-__device__ inline void finish_counts(const BatchView &bv, int job, int ext, int hit, int shadow) {
-    const int lane = __lane_id();
-    const unsigned long long act = __ballot(job >= 0);
-    if (!act) return;
-    const int leader = __ffsll((long long)act) - 1;
-    const int j0 = __shfl(job, leader);
-    const bool mine = job == j0;
-    const long long n_e = wave_sum(mine ? ext : 0), n_h = wave_sum(mine ? hit : 0), n_s = wave_sum(mine ? shadow : 0);
-    const long long cnt = (long long)__popcll(__ballot(mine));
-    if (lane == leader) {
-        unsigned long long *js = bv.jstats + (size_t)j0 * J_STRIDE;
-        atomicAdd(js + J_EXT, (unsigned long long)n_e);
-        atomicAdd(js + J_HIT, (unsigned long long)n_h);
-        atomicAdd(js + J_SHADOW, (unsigned long long)n_s);
-        if (atomicAdd((unsigned long long *)&bv.remaining[j0], (unsigned long long)(-cnt)) == (unsigned long long)cnt)
-            bv.t_end[j0] = wall_clock64();
-    }
-    if (job >= 0 && !mine) {
-        unsigned long long *js = bv.jstats + (size_t)job * J_STRIDE;
-        atomicAdd(js + J_EXT, (unsigned long long)ext);
-        atomicAdd(js + J_HIT, (unsigned long long)hit);
-        atomicAdd(js + J_SHADOW, (unsigned long long)shadow);
-        if (atomicAdd((unsigned long long *)&bv.remaining[job], (unsigned long long)(-1ll)) == 1ull)
-            bv.t_end[job] = wall_clock64();
+        if (h.prim < 0) shadow_add(wv, nq, __float_as_int(c.w), c);
     }
 }
 
 __global__ void __launch_bounds__(256) k_bfinish(BatchView bv, WfView wv) {
     const int n = wv.counts[C_FINISH];
+    const long long iter = bv.ctl[CTL_ITER] - 1;
     const int stride = gridDim.x * blockDim.x;
-    // whole waves iterate together so the per-wave sums see every lane
-    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const int i = base + threadIdx.x;
-        int job = -1, ext = 0, hit = 0, shadow = 0;
-        if (i < n) {
-            const int slot = wv.finq[i];
-            const float4 a = wv.La[slot], b = wv.Lb[slot];
-            const int fl = wv.flags[slot];
-            const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), fl);
-            bv.ring[bv.pw[slot] & bv.ring_mask] = make_float4(L.x, L.y, L.z, a.w);
-            job = bv.pjob[slot];
-            ext = path_ext_rays(fl);
-            hit = path_hit_rays(fl);
-            shadow = path_shadow_rays(fl);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 a = wv.finLa[i], b = wv.finLb[i];
+        const int4 m = wv.finMeta[i];
+        const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
+        const JobDev &jd = bv.jobs[m.w];
+        const long long w = (jd.start + (long long)m.z * (jd.sv.width * jd.sv.height) + m.y) & bv.ring_mask;
+        bv.ring[w] = make_float4(L.x, L.y, L.z, a.w);
+        bv.ring_cnt[w] = pack_counts(path_ext_rays(m.x), path_hit_rays(m.x), path_shadow_rays(m.x), iter);
+    }
+}
+
+// Per-job sums over chunks of finished work: ray counts, paths, and the latest
+// finish time (the wall clock at the start of the iteration after the path's
+// last one).  One workgroup per chunk, one atomic per counter per chunk.
+struct StatChunk {
+    long long begin, end;  // work indices
+    int job, pad;
+};
+
+__global__ void __launch_bounds__(256) k_bstats(BatchView bv, const StatChunk *__restrict__ chunks) {
+    __shared__ unsigned long long s_sum[4][256];
+    __shared__ unsigned long long s_tmax[256];
+    const StatChunk ch = chunks[blockIdx.x];
+    unsigned long long e = 0, h = 0, sh = 0, np = 0, tmax = 0;
+    for (long long w = ch.begin + threadIdx.x; w < ch.end; w += blockDim.x) {
+        const unsigned c = bv.ring_cnt[w & bv.ring_mask];
+        e += c & 511u;
+        h += (c >> 9) & 511u;
+        sh += (c >> 18) & 63u;
+        np += 1;
+        const int it_next = (int)(((c >> 24) + 1) % BATCH_HIST);
+        const unsigned long long t = (unsigned long long)bv.ctl[CTL_TIME + it_next];
+        tmax = t > tmax ? t : tmax;
+    }
+    s_sum[0][threadIdx.x] = e;
+    s_sum[1][threadIdx.x] = h;
+    s_sum[2][threadIdx.x] = sh;
+    s_sum[3][threadIdx.x] = np;
+    s_tmax[threadIdx.x] = tmax;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            for (int k = 0; k < 4; ++k) s_sum[k][threadIdx.x] += s_sum[k][threadIdx.x + off];
+            const unsigned long long o = s_tmax[threadIdx.x + off];
+            if (o > s_tmax[threadIdx.x]) s_tmax[threadIdx.x] = o;
         }
-        finish_counts(bv, job, ext, hit, shadow);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long *js = bv.jstats + (size_t)ch.job * J_STRIDE;
+        atomicAdd(js + J_EXT, s_sum[0][0]);
+        atomicAdd(js + J_HIT, s_sum[1][0]);
+        atomicAdd(js + J_SHADOW, s_sum[2][0]);
+        atomicAdd(js + J_PATHS, s_sum[3][0]);
+        atomicMax(&bv.t_end[ch.job], s_tmax[0]);
     }
 }
 
@@ -350,14 +348,10 @@ static BatchView batch_view(Ctx *c, int n_jobs, long long total, long long ring_
     v.n_jobs = n_jobs;
     v.total = total;
     v.ctl = b.ctl.as<long long>();
-    v.pjob = b.pjob.as<int>();
-    v.ppix = b.ppix.as<int>();
-    v.psmp = b.psmp.as<int>();
-    v.pw = b.pw.as<long long>();
     v.ring = b.ring.as<float4>();
     v.ring_mask = ring_mask;
-    v.remaining = b.remaining.as<long long>();
-    v.t_end = b.t_end.as<long long>();
+    v.ring_cnt = b.ring_cnt.as<unsigned>();
+    v.t_end = b.t_end.as<unsigned long long>();
     v.t0 = b.t0.as<long long>();
     v.jstats = b.jstats.as<unsigned long long>();
     return v;
@@ -370,9 +364,9 @@ static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, c->stream, bv, wv, q);
     hipLaunchKernelGGL(k_regen, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
 #define AUR_BEXT(w_, b_) hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs, wv, q)
-#define AUR_BSHD(w_, b_) hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv, bv.jobs, wv)
+#define AUR_BSHD(w_, b_) hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs, wv, 1 - q)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
-    hipLaunchKernelGGL(k_bshade, dim3(sb), dim3(256), 0, c->stream, bv, bv.jobs, wv, q);
+    hipLaunchKernelGGL(k_bshade, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BSHD);
 #undef AUR_BEXT
 #undef AUR_BSHD
@@ -385,13 +379,16 @@ static double job_cost(const Ctx *j, long long paths) {
     return (double)paths * (j->scene.opt.maxdepth + 1) * std::pow((double)std::max(j->n_tris, 1), 0.3);
 }
 
-// Adds ring entries [from, to) (work indices) into the framebuffers.
+// Adds ring entries [from, to) (work indices) into the framebuffers and the
+// per-job statistics.
+#define STAT_CHUNK 65536
 static void resolve_range(Ctx *c, const BatchView &bv, const std::vector<long long> &starts,
                           const std::vector<long long> &npaths, const std::vector<int> &wh, long long from, long long to) {
     BatchState &b = c->batch;
     if (to <= from) return;
     ResolveSeg *seg = static_cast<ResolveSeg *>(b.host_segs);
-    int n = 0;
+    StatChunk *chk = static_cast<StatChunk *>(b.host_chunks);
+    int n = 0, nc = 0;
     long long threads = 0;
     int r = (int)(std::upper_bound(starts.begin(), starts.end(), from) - starts.begin()) - 1;
     for (r = std::max(r, 0); r < (int)starts.size() && starts[r] < to; ++r) {
@@ -405,11 +402,21 @@ static void resolve_range(Ctx *c, const BatchView &bv, const std::vector<long lo
         seg[n].pad = 0;
         threads += wh[r];  // one thread per pixel of the job
         ++n;
+        for (long long k = la; k < lb; k += STAT_CHUNK) {
+            if ((size_t)nc >= b.host_chunks_cap) throw CapacityError("statistics chunk table overflow");
+            chk[nc].begin = starts[r] + k;
+            chk[nc].end = starts[r] + std::min(lb, k + STAT_CHUNK);
+            chk[nc].job = r;
+            chk[nc].pad = 0;
+            ++nc;
+        }
     }
     if (n == 0) return;
     HIP_CHECK(hipMemcpyAsync(b.segs.p, b.host_segs, sizeof(ResolveSeg) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(b.chunks.p, b.host_chunks, sizeof(StatChunk) * nc, hipMemcpyHostToDevice, c->stream));
     const long long blocks = std::min<long long>((threads + 255) / 256, (long long)c->n_cu * 8);
     hipLaunchKernelGGL(k_bresolve, dim3((unsigned)blocks), dim3(256), 0, c->stream, bv, b.segs.as<ResolveSeg>(), n, threads);
+    hipLaunchKernelGGL(k_bstats, dim3(nc), dim3(256), 0, c->stream, bv, b.chunks.as<StatChunk>());
     HIP_CHECK(hipGetLastError());
 }
 
@@ -472,20 +479,24 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     BatchState &b = c->batch;
     b.jobs.reserve(sizeof(JobDev) * n_jobs);
     b.starts.reserve(sizeof(long long) * n_jobs);
-    b.remaining.reserve(sizeof(long long) * n_jobs);
-    b.t_end.reserve(sizeof(long long) * n_jobs);
+    b.t_end.reserve(sizeof(unsigned long long) * n_jobs);
     b.jstats.reserve(sizeof(unsigned long long) * J_STRIDE * n_jobs);
-    b.ctl.reserve(sizeof(long long) * (CTL_HIST + BATCH_HIST));
+    b.ctl.reserve(sizeof(long long) * CTL_SIZE);
     b.t0.reserve(sizeof(long long));
-    b.pjob.reserve(sizeof(int) * pool);
-    b.ppix.reserve(sizeof(int) * pool);
-    b.psmp.reserve(sizeof(int) * pool);
-    b.pw.reserve(sizeof(long long) * pool);
-    b.qjob[0].reserve(sizeof(int) * pool);
-    b.qjob[1].reserve(sizeof(int) * pool);
     b.ring.reserve(sizeof(float4) * ring);
+    b.ring_cnt.reserve(sizeof(unsigned) * ring);
     b.segs.reserve(sizeof(ResolveSeg) * n_jobs);
-    if (!b.host_status) HIP_CHECK(hipHostMalloc(&b.host_status, sizeof(long long) * (CTL_HIST + BATCH_HIST + 8), hipHostMallocDefault));
+    // a resolve covers less than the ring: at most ring / STAT_CHUNK chunks plus one per job
+    const size_t max_chunks = ring / STAT_CHUNK + (size_t)n_jobs + 1;
+    b.chunks.reserve(sizeof(StatChunk) * max_chunks);
+    if (b.host_chunks_cap < max_chunks) {
+        if (b.host_chunks) (void)hipHostFree(b.host_chunks);
+        b.host_chunks = nullptr;
+        b.host_chunks_cap = 0;
+        HIP_CHECK(hipHostMalloc(&b.host_chunks, sizeof(StatChunk) * max_chunks, hipHostMallocDefault));
+        b.host_chunks_cap = max_chunks;
+    }
+    if (!b.host_status) HIP_CHECK(hipHostMalloc(&b.host_status, sizeof(long long) * (CTL_SIZE + 8), hipHostMallocDefault));
     if (b.host_segs_cap < (size_t)n_jobs) {
         if (b.host_segs) (void)hipHostFree(b.host_segs);
         b.host_segs = nullptr;
@@ -503,16 +514,14 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     src.pixel = src.sample = nullptr;
     src.slots = nullptr;
     WfView wv = wf_view(c, src);
-    wv.qjob[0] = b.qjob[0].as<int>();
-    wv.qjob[1] = b.qjob[1].as<int>();
     BatchView bv = batch_view(c, n_jobs, total, (long long)ring - 1);
+    bv.pool = (int)pool;
 
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, c->stream));
-    hipLaunchKernelGGL(k_binit, dim3(std::min<size_t>((pool + 255) / 256, (size_t)c->n_cu * 4)), dim3(256), 0, c->stream,
-                       bv, wv, (int)pool);
+    hipLaunchKernelGGL(k_binit, dim3(std::max(1, std::min((n_jobs + 255) / 256, c->n_cu * 4))), dim3(256), 0, c->stream, bv, wv);
     HIP_CHECK(hipGetLastError());
 
     hipGraphExec_t gexec = nullptr;
@@ -527,7 +536,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         e0 = e1 = nullptr;
     };
     long long *hs = static_cast<long long *>(b.host_status);
-    const int *hcounts = reinterpret_cast<const int *>(hs + CTL_HIST + BATCH_HIST);
+    const int *hcounts = reinterpret_cast<const int *>(hs + CTL_SIZE);
     // A path holds its slot for at most maxdepth + 1 iterations, so every slot
     // completes a path at least that often and the launch drains within
     // (total / pool + 1) * (maxdepth + 1) iterations; twice that guards the loop.
@@ -551,8 +560,8 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
                 }
             }
             HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipMemcpyAsync(hs, b.ctl.p, sizeof(long long) * (CTL_HIST + BATCH_HIST), hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(hipMemcpyAsync(hs + CTL_HIST + BATCH_HIST, wv.counts, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipMemcpyAsync(hs, b.ctl.p, sizeof(long long) * CTL_TIME, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipMemcpyAsync(hs + CTL_SIZE, wv.counts, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
             HIP_CHECK(hipStreamSynchronize(c->stream));
             const long long cur = hs[CTL_CURSOR], iters = hs[CTL_ITER];
             // after a pair of iterations the live paths sit in queue 0
@@ -560,6 +569,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
             long long safe = 0;
             if (done) safe = total;
             else if (iters - 1 - lag >= 0) safe = hs[CTL_HIST + (int)((iters - 1 - lag) % BATCH_HIST)];
+            if (done) hipLaunchKernelGGL(k_bstamp, dim3(1), dim3(64), 0, c->stream, bv);
             if (safe > resolved) {
                 resolve_range(c, bv, starts, npaths, wh, resolved, safe);
                 resolved = safe;
@@ -577,11 +587,10 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     }
     release();
 
-    std::vector<long long> t_end(n_jobs), rem(n_jobs);
+    std::vector<unsigned long long> t_end(n_jobs);
     std::vector<unsigned long long> js((size_t)n_jobs * J_STRIDE);
     long long t0 = 0;
-    HIP_CHECK(hipMemcpy(t_end.data(), b.t_end.p, sizeof(long long) * n_jobs, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(rem.data(), b.remaining.p, sizeof(long long) * n_jobs, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(t_end.data(), b.t_end.p, sizeof(unsigned long long) * n_jobs, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(js.data(), b.jstats.p, sizeof(unsigned long long) * js.size(), hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(&t0, b.t0.p, sizeof t0, hipMemcpyDeviceToHost));
     int rate_khz = 0;
@@ -589,13 +598,15 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     aur_stats tot;
     std::memset(&tot, 0, sizeof tot);
     for (int r = 0; r < n_jobs; ++r) {
-        if (rem[r] != 0) throw StateError("batched launch left unfinished paths");
+        if ((long long)js[(size_t)r * J_STRIDE + J_PATHS] != npaths[r])
+            throw StateError("batched launch: job " + std::to_string(ord[r]) + " finished " +
+                             std::to_string(js[(size_t)r * J_STRIDE + J_PATHS]) + " of " + std::to_string(npaths[r]) + " paths");
         aur_job_result &o = results[ord[r]];
         o.paths = paths[ord[r]];
         o.ext_rays = (int64_t)js[(size_t)r * J_STRIDE + J_EXT];
         o.hit_rays = (int64_t)js[(size_t)r * J_STRIDE + J_HIT];
         o.shadow_rays = (int64_t)js[(size_t)r * J_STRIDE + J_SHADOW];
-        o.latency_ms = (paths[ord[r]] > 0 && rate_khz > 0) ? (double)(t_end[r] - t0) / (double)rate_khz : 0.0;
+        o.latency_ms = (paths[ord[r]] > 0 && rate_khz > 0) ? (double)((long long)t_end[r] - t0) / (double)rate_khz : 0.0;
         o.order = r;
         tot.paths += o.paths;
         tot.ext_rays += o.ext_rays;

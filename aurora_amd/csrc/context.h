@@ -76,17 +76,17 @@ struct Bvh {
     DevBuf tri_sorted;  // float4[3 * n_tris]: (v0, prim), (e1, 0), (e2, 0)
 };
 
-// Wavefront working set, sized for `cap` paths per batch.
+// Wavefront working set, sized for `cap` paths in flight.  Path state is
+// stored by queue position and moves with the queue (see wavefront_common.h).
 struct Wavefront {
     size_t cap = 0;
-    DevBuf ray[2];      // float4[2 * cap]: (o, tmin), (d, tmax) per queue slot
-    DevBuf qpath[2];    // int[cap]: path id per queue slot
+    DevBuf ray[2];      // float4[2 * cap]: (o, tmin), (d, tmax) per queue entry
+    DevBuf T[2];        // float4[cap]: throughput rgb, continue probability
+    DevBuf La[2], Lb[2];  // float4[cap]: radiance partial sums (alpha in La.w)
+    DevBuf meta[2];     // int4[cap]: flags and the path's identity
     DevBuf hit;         // float4[cap]
-    DevBuf T;           // float4[cap] by path id: throughput rgb, continue probability
-    DevBuf La, Lb;      // float4[cap] by path id: radiance partial sums
-    DevBuf flags;       // int[cap] by path id
-    DevBuf sray;        // float4[3 * 2cap]: shadow ray (o, tmin), (d, tmax), (contrib, path|type)
-    DevBuf finq;        // int[cap]: paths to finish after the shadow pass
+    DevBuf sray;        // float4[3 * 2cap]: shadow ray (o, job), (d, tmax), (contribution, target)
+    DevBuf finLa, finLb, finMeta;  // finished paths' state, after the shadow pass
     DevBuf out;         // float4[cap]: per-path rgba
     DevBuf counts;      // int[16]: queue sizes
     DevBuf stats;       // unsigned long long[8]
@@ -114,19 +114,20 @@ struct LaunchCfg {
 
 // Device state of a batched launch (batch.hip), kept between calls.
 struct BatchState {
-    DevBuf jobs, starts, remaining, t_end, jstats, ctl, t0, segs;
-    DevBuf pjob, ppix, psmp, pw;    // per path slot
-    DevBuf qjob[2];                 // job of each extension-queue entry
-    DevBuf ring;                    // finished paths' rgba by work index
+    DevBuf jobs, starts, t_end, jstats, ctl, t0, segs, chunks;
+    DevBuf ring, ring_cnt;          // finished paths' rgba and packed counts by work index
     void *host_status = nullptr;    // pinned: control block + queue counts, polled by the host
     void *host_segs = nullptr;      // pinned: resolve segment table
     size_t host_segs_cap = 0;
+    void *host_chunks = nullptr;    // pinned: statistics chunk table
+    size_t host_chunks_cap = 0;
     BatchState() = default;
     BatchState(const BatchState &) = delete;
     BatchState &operator=(const BatchState &) = delete;
     ~BatchState() {
         if (host_status) (void)hipHostFree(host_status);
         if (host_segs) (void)hipHostFree(host_segs);
+        if (host_chunks) (void)hipHostFree(host_chunks);
     }
 };
 

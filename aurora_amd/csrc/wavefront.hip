@@ -56,11 +56,10 @@ __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
         f3 d = camera_dir(sv, pixel, rng.u(0), rng.u(1));
         wv.ray[0][2 * i] = make_float4(0.f, 0.f, 0.f, 0.1f);
         wv.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
-        wv.qpath[0][i] = i;
-        wv.T[i] = make_float4(1.f, 1.f, 1.f, 1.f);
-        wv.La[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        wv.Lb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        wv.flags[i] = RT_CAMERA;
+        wv.T[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
+        wv.La[0][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.Lb[0][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        wv.meta[0][i] = make_int4(RT_CAMERA, i, 0, 0);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         wv.counts[C_Q0] = n_paths;
@@ -97,7 +96,7 @@ __global__ void __launch_bounds__(B) k_extend(SceneView sv, WfView wv, int q) {
 // (BSDF sample)
 // ---------------------------------------------------------------------------
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv) {
+__global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv, int nq) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_SHADOW], (unsigned long long)n);
@@ -111,15 +110,8 @@ __global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv) {
     for (int i = gtid; i < n; i += stride) {
         float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
         TravHit h;
-        traverse<W, true>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
-        if (h.prim < 0) {
-            int tag = __float_as_int(c.w);
-            int path = tag & 0x7fffffff;
-            float4 *acc = (tag < 0) ? wv.Lb : wv.La;
-            float4 v = acc[path];
-            v.x += c.x; v.y += c.y; v.z += c.z;
-            acc[path] = v;
-        }
+        traverse<W, true>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h);
+        if (h.prim < 0) shadow_add(wv, nq, __float_as_int(c.w), c);
     }
 }
 
@@ -130,7 +122,7 @@ __global__ void __launch_bounds__(256, 5) k_shade(SceneView sv, WfView wv, int q
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        shade_one(sv, wv, q, i, [&](int path) { return path_rng(sv, wv, path); });
+        shade_one(sv, wv, q, i, [&](int4 m) { return path_rng(sv, wv, m.y); });
 }
 
 // ---------------------------------------------------------------------------
@@ -141,15 +133,14 @@ __global__ void k_finish(SceneView sv, WfView wv) {
     const int wh = sv.width * sv.height;
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        int path = wv.finq[i];
-        float4 a = wv.La[path], b = wv.Lb[path];
-        f3 L = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
-        L = firefly_clamp(L, wv.flags[path]);
+        const float4 a = wv.finLa[i], b = wv.finLb[i];
+        const int4 m = wv.finMeta[i];
+        const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
         // whole passes land by (pass, pixel) for the resolve; injected paths by id
-        int slot = path;
+        int slot = m.y;
         if (wv.src.mode == 0) {
             int pixel, sample;
-            path_pixel_sample(sv, wv, path, &pixel, &sample);
+            path_pixel_sample(sv, wv, m.y, &pixel, &sample);
             slot = (sample - wv.src.first_pass) * wh + pixel;
         }
         wv.out[slot] = make_float4(L.x, L.y, L.z, a.w);
@@ -290,15 +281,16 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     w.cap = cap;
     for (int k = 0; k < 2; ++k) {
         w.ray[k].reserve(sizeof(float4) * 2 * cap);
-        w.qpath[k].reserve(sizeof(int) * cap);
+        w.T[k].reserve(sizeof(float4) * cap);
+        w.La[k].reserve(sizeof(float4) * cap);
+        w.Lb[k].reserve(sizeof(float4) * cap);
+        w.meta[k].reserve(sizeof(int4) * cap);
     }
     w.hit.reserve(sizeof(float4) * cap);
-    w.T.reserve(sizeof(float4) * cap);
-    w.La.reserve(sizeof(float4) * cap);
-    w.Lb.reserve(sizeof(float4) * cap);
-    w.flags.reserve(sizeof(int) * cap);
     w.sray.reserve(sizeof(float4) * 3 * 2 * cap);
-    w.finq.reserve(sizeof(int) * cap);
+    w.finLa.reserve(sizeof(float4) * cap);
+    w.finLb.reserve(sizeof(float4) * cap);
+    w.finMeta.reserve(sizeof(int4) * cap);
     w.out.reserve(sizeof(float4) * cap);
     w.counts.reserve(sizeof(int) * 16);
     w.stats.reserve(sizeof(unsigned long long) * 8);
@@ -313,14 +305,16 @@ WfView wf_view(Ctx *c, const PathSource &src) {
     WfView v;
     for (int k = 0; k < 2; ++k) {
         v.ray[k] = w.ray[k].as<float4>();
-        v.qpath[k] = w.qpath[k].as<int>();
-        v.qjob[k] = nullptr;
+        v.T[k] = w.T[k].as<float4>();
+        v.La[k] = w.La[k].as<float4>();
+        v.Lb[k] = w.Lb[k].as<float4>();
+        v.meta[k] = w.meta[k].as<int4>();
     }
     v.hit = w.hit.as<float4>();
-    v.T = w.T.as<float4>(); v.La = w.La.as<float4>(); v.Lb = w.Lb.as<float4>();
-    v.flags = w.flags.as<int>();
     v.sray = w.sray.as<float4>();
-    v.finq = w.finq.as<int>();
+    v.finLa = w.finLa.as<float4>();
+    v.finLb = w.finLb.as<float4>();
+    v.finMeta = w.finMeta.as<int4>();
     v.out = w.out.as<float4>();
     v.counts = w.counts.as<int>();
     v.stats = w.stats.as<unsigned long long>();
@@ -341,7 +335,7 @@ static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
     int q = 0;
     for (int it = 0; it <= sv.maxdepth; ++it) {
 #define AUR_EXT(w_, b_) hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q)
-#define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv)
+#define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, 1 - q)
         TIMED(c, KT_EXTEND, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
         TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         TIMED(c, KT_SHADOW, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_SHD));

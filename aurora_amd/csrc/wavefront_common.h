@@ -43,15 +43,24 @@ __device__ inline int path_shadow_rays(int flags) { return (flags >> F_SHADOW_SH
 enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_FINISH = 3 };
 enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
 
+// Path state lives with the queue: entry i of extension queue q holds the
+// ray, the throughput T (rgb, continue probability), the two radiance
+// accumulators La (emission and light-sample NEE, alpha in w) and Lb
+// (BSDF-sample NEE), and a meta record (flags, id0, id1, id2).  Shading copies
+// a continuing path's state to its slot in the next queue and a finished
+// path's to the finish list, so every access is coalesced; shadow rays carry
+// the position their contribution goes to.
+//   single-scene launch: id0 = path index (pixel and sample follow from it)
+//   batched launch:      id0 = local pixel index in the pass, id1 = pass,
+//                        id2 = job
 struct WfView {
     float4 *ray[2];
-    int *qpath[2];
-    int *qjob[2];       // batched launch: job of each queue entry (null otherwise)
+    float4 *T[2], *La[2], *Lb[2];
+    int4 *meta[2];
     float4 *hit;
-    float4 *T, *La, *Lb;
-    int *flags;
-    float4 *sray;
-    int *finq;
+    float4 *sray;       // 3 per shadow ray: (o, job bits), (d, tmax), (contribution, target)
+    float4 *finLa, *finLb;
+    int4 *finMeta;
     float4 *out;
     int *counts;
     unsigned long long *stats;
@@ -61,6 +70,22 @@ struct WfView {
     uint32_t seed;
     int nslots;
 };
+
+// shadow-ray target: position in the next queue, or in the finish list
+// (bit 30), accumulator La or Lb (bit 31)
+#define SH_TO_FINISH 0x40000000
+#define SH_TO_LB ((int)0x80000000)
+#define SH_POS_MASK 0x3fffffff
+
+__device__ inline void shadow_add(const WfView &wv, int nq, int tag, float4 c) {
+    const int pos = tag & SH_POS_MASK;
+    float4 *acc;
+    if (tag & SH_TO_FINISH) acc = (tag & SH_TO_LB) ? wv.finLb : wv.finLa;
+    else acc = (tag & SH_TO_LB) ? wv.Lb[nq] : wv.La[nq];
+    float4 v = acc[pos];
+    v.x += c.x; v.y += c.y; v.z += c.z;
+    acc[pos] = v;
+}
 
 WfView wf_view(Ctx *c, const PathSource &src);
 
@@ -119,21 +144,24 @@ __device__ inline f3 firefly_clamp(f3 L, int flags) {
 
 // ---------------------------------------------------------------------------
 // shade: one path vertex (integrator.cpp:180-274) for queue entry i of queue
-// q.  make_rng(path) gives the path's random-number source; job is recorded
-// with the next extension ray when the view carries job queues.  Writes the next
-// extension ray, up to two shadow rays and the finish entry with wave-level
-// compaction, so every active lane of a wave must call it.
+// q.  make_rng(meta) gives the path's random-number source; shadow rays
+// record `job` for the batched launch.  Writes the next extension ray with
+// the path's state, up to two shadow rays and the finish record with
+// wave-level compaction, so every active lane of a wave must call it.
 // ---------------------------------------------------------------------------
 struct ShadowOut { bool valid; f3 o, d; float tmin, tmax; f3 c; };
 
 template <class MakeRng>
-__device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job = -1) {
+__device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job = 0) {
     const int nq = 1 - q;
-    const int path = wv.qpath[q][i];
+    const int4 meta = wv.meta[q][i];
     const float4 ra = wv.ray[q][2 * i], rb = wv.ray[q][2 * i + 1];
     const float4 hv = wv.hit[i];
     const int prim = __float_as_int(hv.w);
-    int flags = wv.flags[path];
+    float4 La = wv.La[q][i];
+    const float4 Lb = wv.Lb[q][i];
+    float4 T4 = wv.T[q][i];
+    int flags = meta.x;
     const int shadow_before = (flags >> F_SHADOW_SHIFT) & 63;
     const int rtype = flags & 3;
     const int tb = (flags >> 8) & 0xff;
@@ -145,13 +173,13 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
     if (prim < 0) {
         finish = true;
         if (rtype == RT_CAMERA) {  // integrator.cpp:287-297: camera miss sees the environment
-            wv.La[path] = make_float4(sv.env[0], sv.env[1], sv.env[2], sv.has_env ? 1.f : 0.f);
+            La = make_float4(sv.env[0], sv.env[1], sv.env[2], sv.has_env ? 1.f : 0.f);
             flags |= F_NOCLAMP;
         }
     } else if (tb >= sv.maxdepth) {
         finish = true;  // the extension ray after the last vertex is traced but not shaded
     } else {
-        const PathRng rng = make_rng(path);
+        const PathRng rng = make_rng(meta);
         const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
         // hit record: embreeAccelerator.cpp:111-136
         const f3 o = mk3(ra.x, ra.y, ra.z), d = mk3(rb.x, rb.y, rb.z);
@@ -168,15 +196,14 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
         if (dot3(vn, nn) <= 0.f) nn = norm3(cross3(vn, cross3(nn, vn)));
         bool matte = (flags & F_MATTE) != 0;
         if (rtype == RT_DIFFUSE) matte = true;
-        float4 T4 = wv.T[path];
         f3 T = mk3(T4.x, T4.y, T4.z);
         float cp = T4.w;
         const DMat m = sv.mats[attr];
         if (rtype == RT_CAMERA) {
-            float4 la = wv.La[path];
+            float4 &la = La;
             la.x += T.x * m.emission[0]; la.y += T.y * m.emission[1]; la.z += T.z * m.emission[2];
             la.w = 1.f;
-            wv.La[path] = la;
+            
         }
         if (m.type == MAT_LIGHT) {
             finish = true;  // ConstantMaterial: getSample has pdf 0 (constant.cpp:23-25)
@@ -250,7 +277,7 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
                         ndir = s.dir;
                         ntmax = s.tmax;
                         flags = nrt | (matte ? F_MATTE : 0) | ((tb + 1) << 8) | ((bounces + 1) << 16);
-                        wv.T[path] = make_float4(T.x, T.y, T.z, cp);
+                        T4 = make_float4(T.x, T.y, T.z, cp);
                     }
                 }
             }
@@ -262,26 +289,34 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
         const int traced = (shadow_before + n_sh) & 63;
         flags = (flags & ~F_SHADOW_MASK) | (traced << F_SHADOW_SHIFT);
     }
-    wv.flags[path] = flags;
-    // compacted writes: next extension queue, shadow queue, finish list
-    int slot = wave_reserve(&wv.counts[nq], cont);
+    // compacted writes: the path's state to the next queue or the finish
+    // list, then the shadow rays aimed at wherever the state went
+    const int4 nmeta = make_int4(flags, meta.y, meta.z, meta.w);
+    const int slot = wave_reserve(&wv.counts[nq], cont);
     if (cont) {
         wv.ray[nq][2 * slot] = make_float4(P.x, P.y, P.z, AUR_RAY_BIAS);
         wv.ray[nq][2 * slot + 1] = make_float4(ndir.x, ndir.y, ndir.z, ntmax);
-        wv.qpath[nq][slot] = path;
-        if (wv.qjob[nq]) wv.qjob[nq][slot] = job;
+        wv.T[nq][slot] = T4;
+        wv.La[nq][slot] = La;
+        wv.Lb[nq][slot] = Lb;
+        wv.meta[nq][slot] = nmeta;
     }
+    const int fs = wave_reserve(&wv.counts[C_FINISH], finish);
+    if (finish) {
+        wv.finLa[fs] = La;
+        wv.finLb[fs] = Lb;
+        wv.finMeta[fs] = nmeta;
+    }
+    const int target = cont ? slot : (fs | SH_TO_FINISH);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         int ss = wave_reserve(&wv.counts[C_SHADOW], sh[k].valid);
         if (sh[k].valid) {
-            wv.sray[3 * ss] = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, sh[k].tmin);
+            wv.sray[3 * ss] = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, __int_as_float(job));
             wv.sray[3 * ss + 1] = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].tmax);
-            wv.sray[3 * ss + 2] = make_float4(sh[k].c.x, sh[k].c.y, sh[k].c.z, __int_as_float(path | (k ? (int)0x80000000 : 0)));
+            wv.sray[3 * ss + 2] = make_float4(sh[k].c.x, sh[k].c.y, sh[k].c.z, __int_as_float(target | (k ? SH_TO_LB : 0)));
         }
     }
-    int fs = wave_reserve(&wv.counts[C_FINISH], finish);
-    if (finish) wv.finq[fs] = path;
 }
 
 }  // namespace aur

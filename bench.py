@@ -129,11 +129,13 @@ def cpu_baseline(exp: dict, target_s: float, threads: int, spp: int):
             "mrays_per_s": round((cnt["ext_rays"] + cnt["shadow_rays"]) / dt / 1e6, 4)}
 
 
-def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int = 4):
+def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int = 4, n_seeds: int = 8):
     """The CPU-runnable box room (256x256, 16 spp, depth 5): GPU Mpaths/s over
     repeated full frames, the oracle's Mpaths/s on the host cores (median of
-    three full frames after a warm-up), and both 16-spp images' RMSE against a 4096-spp oracle reference
-    (independent seed, float64 mean) on every 4th pixel of every 4th row."""
+    three full frames after a warm-up), and the RMSE of 16-spp images (mean
+    squared error over n_seeds seeds on each side) against a 4096-spp oracle
+    reference (independent seed, float64 mean) on every 4th pixel of every
+    4th row."""
     from aurora_amd import scenes, tuning
     from aurora_amd.renderer import Renderer
     from oracle import oracle as O
@@ -157,7 +159,15 @@ def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int 
     _hip_sync()
     gpu_s = time.perf_counter() - t0
     gpu_rate = reps * st["paths"] / gpu_s / 1e6
-    gpu_img = r.framebuffer()[..., :3].astype(np.float64) / spp
+    ys, xs = np.mgrid[0:H:stride, 0:W:stride]
+    sub = (ys.ravel(), xs.ravel())
+    # 16-spp images of n_seeds independent seeds on both sides: the squared
+    # error is heavy-tailed (caustics), so a single image's RMSE is noisy
+    gpu_imgs = []
+    for k in range(n_seeds):
+        r.set_seed(1000 + k)
+        a, _ = r.render_passes(0, spp)
+        gpu_imgs.append(a[..., :3].astype(np.float64)[sub] / spp)
     exp = r.export()
     r.close()
     sc = O.Scene(exp, bvh=True)
@@ -169,8 +179,10 @@ def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int 
         times.append(time.time() - t0)
     cpu_s = float(np.median(times))
     cpu_rate = W * H * spp / cpu_s / 1e6
-    cpu_img = cacc[..., :3].astype(np.float64) / spp
-    ys, xs = np.mgrid[0:H:stride, 0:W:stride]
+    cpu_imgs = []
+    for k in range(n_seeds):
+        cacc, _ = sc.render(2000 + k, 0, spp, threads=threads)
+        cpu_imgs.append(cacc[..., :3].astype(np.float64)[sub] / spp)
     pix = (ys * W + xs).ravel().astype(np.int32)
     ref = np.zeros((pix.size, 3))
     t0 = time.time()
@@ -182,14 +194,18 @@ def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int 
         ref += o[:, :3].astype(np.float64).reshape(pix.size, chunk, 3).sum(1)
     ref /= ref_spp
     ref_s = time.time() - t0
-    sub = (ys.ravel(), xs.ravel())
-    rmse_g = float(np.sqrt(np.mean((gpu_img[sub] - ref) ** 2)))
-    rmse_c = float(np.sqrt(np.mean((cpu_img[sub] - ref) ** 2)))
+    mse_g = np.array([np.mean((g - ref) ** 2) for g in gpu_imgs])
+    mse_c = np.array([np.mean((c - ref) ** 2) for c in cpu_imgs])
+    rmse_g, rmse_c = float(np.sqrt(mse_g.mean())), float(np.sqrt(mse_c.mean()))
+    # standard error of the difference of the two mean squared errors
+    se = float(np.sqrt(mse_g.var(ddof=1) / n_seeds + mse_c.var(ddof=1) / n_seeds))
     return {"workload": "box_room_256_spp16_depth5", "paths_per_frame": W * H * spp,
             "gpu_Mpaths_s": round(gpu_rate, 3), "gpu_frames_timed": reps,
             "cpu_Mpaths_s": round(cpu_rate, 4), "cpu_cores": threads, "cpu_s": round(cpu_s, 2),
             "gpu_vs_cpu": round(gpu_rate / cpu_rate, 2),
             "rmse_gpu_vs_ref": float("%.4g" % rmse_g), "rmse_cpu_vs_ref": float("%.4g" % rmse_c),
+            "mse_diff_in_se": float("%.3g" % ((mse_g.mean() - mse_c.mean()) / max(se, 1e-30))),
+            "rmse_seeds": n_seeds,
             "ref": "%d-spp oracle on %d pixels (every %dth row and column), %.1f s" % (ref_spp, pix.size, stride, ref_s)}
 
 
@@ -317,6 +333,38 @@ def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
         rs[k].close()
     single_s = time.perf_counter() - t0
     _progress("batch: per-scene launches %.2f s" % single_s)
+    # the reference's launch granularity: one launch per scene and pass
+    # (Renderer::renderImage runs one parallel_for per pass), on the first
+    # third of the jobs in launch order, rebuilt since the others were closed
+    sub = order[: n_jobs // 3]
+    t_pp = 0.0
+    paths_pp = 0
+    rs_sub = []
+    for k in sub:
+        r = Renderer(paths[k], os.path.join(os.path.dirname(paths[k]), "config.json"))
+        r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
+        tuning.apply_launch(r, cfg)
+        r.set_seed(1000 + k)
+        r.render_passes(0, 1, fetch=False)
+        _hip_sync()
+        t0 = time.perf_counter()
+        for p in range(spp[k]):
+            r.render_passes(p, 1, fetch=False)
+        t_pp += time.perf_counter() - t0
+        paths_pp += spp[k] * info[k]["width"] * info[k]["height"]
+        rs_sub.append((r, spp[k]))
+    _progress("batch: per-pass launches of %d jobs %.2f s" % (len(sub), t_pp))
+    # the same jobs through one batched launch
+    ex2 = Renderer()
+    tuning.apply_launch(ex2, cfg)
+    batch_submit(ex2, [(r, 0, 1) for r, _ in rs_sub], use_graph=use_graph)
+    _hip_sync()
+    t0 = time.perf_counter()
+    _, tot_sub = batch_submit(ex2, [(r, 0, n) for r, n in rs_sub], use_graph=use_graph)
+    t_bsub = time.perf_counter() - t0
+    for r, _ in rs_sub:
+        r.close()
+    ex2.close()
     ex.close()
     lat_b = np.array([o["latency_ms"] for o in res])
     lat_s = np.array(lat_single)
@@ -334,7 +382,10 @@ def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
             "batched_latency_ms_p99": round(float(np.percentile(lat_b, 99)), 2),
             "per_scene_latency_ms_p50": round(float(np.percentile(lat_s, 50)), 2),
             "per_scene_latency_ms_p99": round(float(np.percentile(lat_s, 99)), 2),
-            "image_max_rel_diff_4_jobs": float("%.3g" % worst)}
+            "image_max_rel_diff_4_jobs": float("%.3g" % worst),
+            "per_pass_launch_jobs": len(sub), "per_pass_launch_Mpaths_s": round(paths_pp / max(t_pp, 1e-9) / 1e6, 3),
+            "batched_same_jobs_Mpaths_s": round(tot_sub["paths"] / max(t_bsub, 1e-9) / 1e6, 3),
+            "speedup_vs_per_pass": round(t_pp / max(t_bsub, 1e-9), 3), "per_pass_launch_paths": int(paths_pp)}
 
 
 def copy_calibration(mib: int, reps: int):
