@@ -266,9 +266,18 @@ __global__ void __launch_bounds__(128) k_adjoint(SceneView sv, AdjParams ap, int
         int k2;
         replay<W, true>(sv, nodes, st, pixel, rng, &L2, &k2, &miss, delta, L, g, &n_ext, &n_shadow);
     }
-    atomicAdd(&ap.stats[0], n_paths);
-    atomicAdd(&ap.stats[1], n_ext);
-    atomicAdd(&ap.stats[2], n_shadow);
+    // one add per wave: same-address atomics from every thread serialise
+    unsigned cp = (unsigned)n_paths, ce = (unsigned)n_ext, cs = (unsigned)n_shadow;
+    for (int off = 32; off > 0; off >>= 1) {
+        cp += __shfl_xor(cp, off);
+        ce += __shfl_xor(ce, off);
+        cs += __shfl_xor(cs, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&ap.stats[0], (unsigned long long)cp);
+        atomicAdd(&ap.stats[1], (unsigned long long)ce);
+        atomicAdd(&ap.stats[2], (unsigned long long)cs);
+    }
     if (ap.use_lds) {
         __syncthreads();
         for (int i = threadIdx.x; i < n_alb; i += blockDim.x)

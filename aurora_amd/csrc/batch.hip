@@ -198,24 +198,32 @@ __global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, 
 }
 
 __global__ void __launch_bounds__(256, 5) k_bshade(const JobDev *__restrict__ jobs, WfView wv, int q) {
+    __shared__ EmitScratch es;
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int job = wv.meta[q][i].w;
-        for (;;) {
-            const int j = __builtin_amdgcn_readfirstlane(job);
-            if (job == j) {
-                const JobDev &jd = jobs[j];
-                shade_one(jd.sv, wv, q, i, [&](int4 m) {
-                    PathRng r;
-                    r.slots = nullptr;
-                    r.key = path_key(jd.seed, (uint32_t)tiled_pixel(m.y, jd.sv.width, jd.sv.height),
-                                     (uint32_t)(jd.first_pass + m.z));
-                    return r;
-                }, j);
-                break;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        ShadeOut o;
+        o.cont = o.finish = false;
+        o.sh[0].valid = o.sh[1].valid = false;
+        if (i < n) {
+            const int job = wv.meta[q][i].w;
+            for (;;) {
+                const int j = __builtin_amdgcn_readfirstlane(job);
+                if (job == j) {
+                    const JobDev &jd = jobs[j];
+                    shade_compute(jd.sv, wv, q, i, [&](int4 m) {
+                        PathRng r;
+                        r.slots = nullptr;
+                        r.key = path_key(jd.seed, (uint32_t)tiled_pixel(m.y, jd.sv.width, jd.sv.height),
+                                         (uint32_t)(jd.first_pass + m.z));
+                        return r;
+                    }, j, o);
+                    break;
+                }
             }
         }
+        shade_emit(wv, q, i < n, o, es);
     }
 }
 

@@ -90,6 +90,7 @@ struct Wavefront {
     DevBuf out;         // float4[cap]: per-path rgba
     DevBuf counts;      // int[16]: queue sizes
     DevBuf stats;       // unsigned long long[8]
+    DevBuf partials;    // unsigned long long per traversal workgroup
     DevBuf stack_ovf;   // traversal stack overflow
     size_t stack_ovf_per_thread = 0;
 };

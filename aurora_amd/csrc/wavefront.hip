@@ -82,13 +82,17 @@ __global__ void __launch_bounds__(B) k_extend(SceneView sv, WfView wv, int q) {
     st.stride = B;
     st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
+    int hits = 0;
     for (int i = gtid; i < n; i += stride) {
         float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
         TravHit h;
         traverse<W, false>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
         wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
-        wave_count(&wv.stats[S_HIT], h.prim >= 0);
+        hits += h.prim >= 0 ? 1 : 0;
     }
+    // one add per wave into this workgroup's partial (no global same-address atomics)
+    for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
+    if (__lane_id() == 0 && hits) atomicAdd(&wv.partials[blockIdx.x], (unsigned long long)hits);
 }
 
 // ---------------------------------------------------------------------------
@@ -119,10 +123,18 @@ __global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv, int nq) {
 // shade: one path vertex per queue entry
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256, 5) k_shade(SceneView sv, WfView wv, int q) {
+    __shared__ EmitScratch es;
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        shade_one(sv, wv, q, i, [&](int4 m) { return path_rng(sv, wv, m.y); });
+    // whole workgroups iterate together: the emit step reserves per workgroup
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        ShadeOut o;
+        o.cont = o.finish = false;
+        o.sh[0].valid = o.sh[1].valid = false;
+        if (i < n) shade_compute(sv, wv, q, i, [&](int4 m) { return path_rng(sv, wv, m.y); }, 0, o);
+        shade_emit(wv, q, i < n, o, es);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -160,6 +172,23 @@ __global__ void k_reset_all(int *counts, unsigned long long *stats, int reset_st
     int i = threadIdx.x;
     if (i < 16) counts[i] = 0;
     if (reset_stats && i < 8) stats[i] = 0;
+}
+
+// adds the per-workgroup hit counts into the run's statistics and clears them
+__global__ void k_sum_partials(unsigned long long *partials, int n, unsigned long long *stats) {
+    __shared__ unsigned long long s[256];
+    unsigned long long v = 0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        v += partials[k];
+        partials[k] = 0;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) stats[S_HIT] += s[0];
 }
 
 // per-pixel sum over the passes of a batch, in pass order (deterministic)
@@ -275,7 +304,8 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     int need = (width - 1) * (depth + 1) + 2;
     size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 0;
     if (cap <= w.cap && ovf <= w.stack_ovf_per_thread && w.counts.p &&
-        w.stack_ovf.bytes >= sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1))
+        w.stack_ovf.bytes >= sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1) &&
+        w.partials.bytes >= sizeof(unsigned long long) * trace_blocks(c))
         return;
     cap = std::max(cap, w.cap);
     w.cap = cap;
@@ -296,6 +326,7 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     w.stats.reserve(sizeof(unsigned long long) * 8);
     w.stack_ovf_per_thread = std::max(ovf, w.stack_ovf_per_thread);
     w.stack_ovf.reserve(sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1));
+    w.partials.reserve(sizeof(unsigned long long) * trace_blocks(c));
 }
 
 static void ensure_wavefront(Ctx *c, size_t cap) { ensure_wavefront_for(c, cap, c->bvh.width, c->bvh.depth); }
@@ -318,6 +349,7 @@ WfView wf_view(Ctx *c, const PathSource &src) {
     v.out = w.out.as<float4>();
     v.counts = w.counts.as<int>();
     v.stats = w.stats.as<unsigned long long>();
+    v.partials = w.partials.as<unsigned long long>();
     v.ovf = w.stack_ovf.as<int>();
     v.ovf_per_thread = (int)w.stack_ovf_per_thread;
     v.src = src;
@@ -362,6 +394,7 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     HIP_CHECK(hipEventRecord(e0, c->stream));
     hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, c->wf.counts.as<int>(),
                        c->wf.stats.as<unsigned long long>(), 1);
+    fill_zero(c, c->wf.partials.p, sizeof(unsigned long long) * trace_blocks(c));
     size_t done = 0;
     while (done < n_paths) {
         size_t n = std::min(cap, n_paths - done);
@@ -385,6 +418,8 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
         }
         done += n;
     }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->wf.partials.as<unsigned long long>(),
+                       trace_blocks(c), c->wf.stats.as<unsigned long long>());
     HIP_CHECK(hipEventRecord(e1, c->stream));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0.f;

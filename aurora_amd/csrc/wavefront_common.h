@@ -64,6 +64,7 @@ struct WfView {
     float4 *out;
     int *counts;
     unsigned long long *stats;
+    unsigned long long *partials;  // per traversal workgroup: hit rays (summed after the run)
     int *ovf;
     int ovf_per_thread;
     PathSource src;
@@ -144,16 +145,27 @@ __device__ inline f3 firefly_clamp(f3 L, int flags) {
 
 // ---------------------------------------------------------------------------
 // shade: one path vertex (integrator.cpp:180-274) for queue entry i of queue
-// q.  make_rng(meta) gives the path's random-number source; shadow rays
-// record `job` for the batched launch.  Writes the next extension ray with
-// the path's state, up to two shadow rays and the finish record with
-// wave-level compaction, so every active lane of a wave must call it.
+// q.  shade_compute fills a ShadeOut; shade_emit then reserves places in the
+// next queue, the finish list and the shadow list once per workgroup (a
+// wave ballot per list, the waves' totals combined in LDS, one atomic per
+// list) and writes them.  make_rng(meta) gives the path's random-number
+// source; shadow rays record `job` for the batched launch.
 // ---------------------------------------------------------------------------
 struct ShadowOut { bool valid; f3 o, d; float tmin, tmax; f3 c; };
 
+struct ShadeOut {
+    bool cont, finish;
+    ShadowOut sh[2];
+    f3 P, ndir;
+    float ntmax;
+    float4 T4, La, Lb;
+    int4 meta;
+    int job;
+};
+
 template <class MakeRng>
-__device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job = 0) {
-    const int nq = 1 - q;
+__device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job,
+                                     ShadeOut &out) {
     const int4 meta = wv.meta[q][i];
     const float4 ra = wv.ray[q][2 * i], rb = wv.ray[q][2 * i + 1];
     const float4 hv = wv.hit[i];
@@ -289,32 +301,83 @@ __device__ inline void shade_one(const SceneView &sv, const WfView &wv, int q, i
         const int traced = (shadow_before + n_sh) & 63;
         flags = (flags & ~F_SHADOW_MASK) | (traced << F_SHADOW_SHIFT);
     }
-    // compacted writes: the path's state to the next queue or the finish
-    // list, then the shadow rays aimed at wherever the state went
-    const int4 nmeta = make_int4(flags, meta.y, meta.z, meta.w);
-    const int slot = wave_reserve(&wv.counts[nq], cont);
-    if (cont) {
-        wv.ray[nq][2 * slot] = make_float4(P.x, P.y, P.z, AUR_RAY_BIAS);
-        wv.ray[nq][2 * slot + 1] = make_float4(ndir.x, ndir.y, ndir.z, ntmax);
-        wv.T[nq][slot] = T4;
-        wv.La[nq][slot] = La;
-        wv.Lb[nq][slot] = Lb;
-        wv.meta[nq][slot] = nmeta;
+    out.cont = cont;
+    out.finish = finish;
+    out.sh[0] = sh[0];
+    out.sh[1] = sh[1];
+    out.P = P;
+    out.ndir = ndir;
+    out.ntmax = ntmax;
+    out.T4 = T4;
+    out.La = La;
+    out.Lb = Lb;
+    out.meta = make_int4(flags, meta.y, meta.z, meta.w);
+    out.job = job;
+}
+
+// LDS scratch of shade_emit: per-wave counts, then per-wave bases, for the
+// next queue, the finish list and the shadow list.
+#define SHADE_MAX_WAVES 16
+struct EmitScratch {
+    int cnt[SHADE_MAX_WAVES][3];
+    int base[SHADE_MAX_WAVES][3];
+};
+
+// Every thread of the workgroup calls this once per loop round (valid = it
+// shaded an entry this round).
+__device__ inline void shade_emit(const WfView &wv, int q, bool valid, const ShadeOut &o, EmitScratch &es) {
+    const int nq = 1 - q;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const bool pc = valid && o.cont, pf = valid && o.finish;
+    const bool p0 = valid && o.sh[0].valid, p1 = valid && o.sh[1].valid;
+    const unsigned long long mc = __ballot(pc), mf = __ballot(pf), m0 = __ballot(p0), m1 = __ballot(p1);
+    if (lane == 0) {
+        es.cnt[wave][0] = __popcll(mc);
+        es.cnt[wave][1] = __popcll(mf);
+        es.cnt[wave][2] = __popcll(m0) + __popcll(m1);
     }
-    const int fs = wave_reserve(&wv.counts[C_FINISH], finish);
-    if (finish) {
-        wv.finLa[fs] = La;
-        wv.finLb[fs] = Lb;
-        wv.finMeta[fs] = nmeta;
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        const int nw = (blockDim.x + 63) >> 6;
+        int tot = 0;
+        for (int w = 0; w < nw; ++w) tot += es.cnt[w][k];
+        int *ctr = k == 0 ? &wv.counts[nq] : (k == 1 ? &wv.counts[C_FINISH] : &wv.counts[C_SHADOW]);
+        int b = tot ? atomicAdd(ctr, tot) : 0;
+        for (int w = 0; w < nw; ++w) {
+            es.base[w][k] = b;
+            b += es.cnt[w][k];
+        }
     }
-    const int target = cont ? slot : (fs | SH_TO_FINISH);
+    __syncthreads();
+    const int slot = es.base[wave][0] + __popcll(mc & lt);
+    const int fs = es.base[wave][1] + __popcll(mf & lt);
+    const int ss = es.base[wave][2] + __popcll(m0 & lt) + __popcll(m1 & lt);
+    if (pc) {
+        wv.ray[nq][2 * slot] = make_float4(o.P.x, o.P.y, o.P.z, AUR_RAY_BIAS);
+        wv.ray[nq][2 * slot + 1] = make_float4(o.ndir.x, o.ndir.y, o.ndir.z, o.ntmax);
+        wv.T[nq][slot] = o.T4;
+        wv.La[nq][slot] = o.La;
+        wv.Lb[nq][slot] = o.Lb;
+        wv.meta[nq][slot] = o.meta;
+    }
+    if (pf) {
+        wv.finLa[fs] = o.La;
+        wv.finLb[fs] = o.Lb;
+        wv.finMeta[fs] = o.meta;
+    }
+    const int target = o.cont ? slot : (fs | SH_TO_FINISH);
+    int s = ss;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        int ss = wave_reserve(&wv.counts[C_SHADOW], sh[k].valid);
-        if (sh[k].valid) {
-            wv.sray[3 * ss] = make_float4(sh[k].o.x, sh[k].o.y, sh[k].o.z, __int_as_float(job));
-            wv.sray[3 * ss + 1] = make_float4(sh[k].d.x, sh[k].d.y, sh[k].d.z, sh[k].tmax);
-            wv.sray[3 * ss + 2] = make_float4(sh[k].c.x, sh[k].c.y, sh[k].c.z, __int_as_float(target | (k ? SH_TO_LB : 0)));
+        const bool pk = k ? p1 : p0;
+        if (pk) {
+            const ShadowOut &h = o.sh[k];
+            wv.sray[3 * s] = make_float4(h.o.x, h.o.y, h.o.z, __int_as_float(o.job));
+            wv.sray[3 * s + 1] = make_float4(h.d.x, h.d.y, h.d.z, h.tmax);
+            wv.sray[3 * s + 2] = make_float4(h.c.x, h.c.y, h.c.z, __int_as_float(target | (k ? SH_TO_LB : 0)));
+            ++s;
         }
     }
 }
