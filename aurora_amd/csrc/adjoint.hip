@@ -219,10 +219,64 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
     *clamp_steps = k;
 }
 
+// per-wave sums of the ray counters, one atomic each
+__device__ inline void adj_count(unsigned long long *stats, unsigned paths, unsigned ext, unsigned shadow) {
+    for (int off = 32; off > 0; off >>= 1) {
+        paths += __shfl_xor(paths, off);
+        ext += __shfl_xor(ext, off);
+        shadow += __shfl_xor(shadow, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&stats[0], (unsigned long long)paths);
+        atomicAdd(&stats[1], (unsigned long long)ext);
+        atomicAdd(&stats[2], (unsigned long long)shadow);
+    }
+}
+
+__device__ inline void adj_path(const SceneView &sv, const AdjParams &ap, int i, int *pixel, PathRng *rng) {
+    const int wh = sv.width * sv.height;
+    const int pass = i / wh;
+    *pixel = tiled_pixel(i - pass * wh, sv.width, sv.height);
+    rng->slots = nullptr;
+    rng->key = path_key(ap.seed, (uint32_t)*pixel, (uint32_t)(ap.first_pass + pass));
+}
+
+// First replay of every path: its radiance and clamp steps (two kernels keep
+// each replay's registers apart: one kernel holding both spilled to scratch).
 template <int W>
-__global__ void __launch_bounds__(128) k_adjoint(SceneView sv, AdjParams ap, int *ovf, int ovf_per_thread) {
+__global__ void __launch_bounds__(128) k_adj_forward(SceneView sv, AdjParams ap, float4 *rec, int *ovf, int ovf_per_thread) {
     __shared__ int s_stack[AUR_STACK_LDS * 128];
-    __shared__ float s_grad[AUR_ADJ_LDS_PARAMS];
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = 128;
+    st.ovf = ovf + (size_t)gtid * ovf_per_thread;
+    const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
+    GradSink g;
+    g.albedo = g.rough = g.spec = nullptr;
+    unsigned long long n_ext = 0, n_shadow = 0, n_paths = 0;
+    for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
+        int pixel;
+        PathRng rng;
+        adj_path(sv, ap, i, &pixel, &rng);
+        f3 L;
+        int k;
+        bool miss;
+        const f3 zero = mk3(0.f, 0.f, 0.f);
+        replay<W, false>(sv, nodes, st, pixel, rng, &L, &k, &miss, zero, zero, g, &n_ext, &n_shadow);
+        ++n_paths;
+        rec[i] = make_float4(L.x, L.y, L.z, __int_as_float(k | (miss ? 0x10000 : 0)));
+    }
+    adj_count(ap.stats, (unsigned)n_paths, (unsigned)n_ext, (unsigned)n_shadow);
+}
+
+// Second replay, carrying the remaining radiance, for paths with a nonzero
+// incoming gradient.  Gradients of small scenes are summed per workgroup in
+// dynamically sized LDS first.
+template <int W>
+__global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, const float4 *rec, int *ovf, int ovf_per_thread) {
+    __shared__ int s_stack[AUR_STACK_LDS * 128];
+    extern __shared__ float s_grad[];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TravStack st;
     st.lds = s_stack + threadIdx.x;
@@ -242,42 +296,25 @@ __global__ void __launch_bounds__(128) k_adjoint(SceneView sv, AdjParams ap, int
         g.rough = ap.g_rough;
         g.spec = ap.g_spec;
     }
-    unsigned long long n_ext = 0, n_shadow = 0, n_paths = 0;
-    const int wh = sv.width * sv.height;
+    unsigned long long n_ext = 0, n_shadow = 0;
     for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
-        const int pass = i / wh;
-        const int pixel = tiled_pixel(i - pass * wh, sv.width, sv.height), sample = ap.first_pass + pass;
+        const float4 r = rec[i];
+        const int kw = __float_as_int(r.w);
+        if (kw & 0x10000) continue;  // camera miss: no vertex depends on the parameters
+        int pixel;
         PathRng rng;
-        rng.slots = nullptr;
-        rng.key = path_key(ap.seed, (uint32_t)pixel, (uint32_t)sample);
+        adj_path(sv, ap, i, &pixel, &rng);
         const float4 dl = ap.dldi[pixel];
-        f3 L;
-        int k;
-        bool miss;
-        f3 zero = mk3(0.f, 0.f, 0.f);
-        replay<W, false>(sv, nodes, st, pixel, rng, &L, &k, &miss, zero, zero, g, &n_ext, &n_shadow);
-        ++n_paths;
-        if (miss) continue;
         float cf = ap.inv_passes;
-        for (int j = 0; j < k; ++j) cf *= 0.2f;
-        f3 delta = mk3(dl.x * cf, dl.y * cf, dl.z * cf);
+        for (int j = 0; j < (kw & 0xffff); ++j) cf *= 0.2f;
+        const f3 delta = mk3(dl.x * cf, dl.y * cf, dl.z * cf);
         if (delta.x == 0.f && delta.y == 0.f && delta.z == 0.f) continue;
         f3 L2;
         int k2;
-        replay<W, true>(sv, nodes, st, pixel, rng, &L2, &k2, &miss, delta, L, g, &n_ext, &n_shadow);
+        bool miss;
+        replay<W, true>(sv, nodes, st, pixel, rng, &L2, &k2, &miss, delta, mk3(r.x, r.y, r.z), g, &n_ext, &n_shadow);
     }
-    // one add per wave: same-address atomics from every thread serialise
-    unsigned cp = (unsigned)n_paths, ce = (unsigned)n_ext, cs = (unsigned)n_shadow;
-    for (int off = 32; off > 0; off >>= 1) {
-        cp += __shfl_xor(cp, off);
-        ce += __shfl_xor(ce, off);
-        cs += __shfl_xor(cs, off);
-    }
-    if (__lane_id() == 0) {
-        atomicAdd(&ap.stats[0], (unsigned long long)cp);
-        atomicAdd(&ap.stats[1], (unsigned long long)ce);
-        atomicAdd(&ap.stats[2], (unsigned long long)cs);
-    }
+    adj_count(ap.stats, 0u, (unsigned)n_ext, (unsigned)n_shadow);
     if (ap.use_lds) {
         __syncthreads();
         for (int i = threadIdx.x; i < n_alb; i += blockDim.x)
@@ -315,17 +352,23 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     // traversal stack spill area, as for the wavefront kernels
     int need = (c->bvh.width - 1) * (c->bvh.depth + 1) + 2;
     size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 1;
-    const int blocks = c->n_cu * 4;
-    DevBuf sovf;
+    // latency-bound replays: 16 waves per CU
+    const int blocks = std::max(1, std::min((int)((n + 127) / 128), c->n_cu * 8));
+    DevBuf sovf, rec;
     sovf.reserve(sizeof(int) * (size_t)blocks * 128 * ovf);
+    rec.reserve(sizeof(float4) * (size_t)std::max<long long>(n, 1));
+    const size_t lds = ap.use_lds ? sizeof(float) * (size_t)((d_galb ? 3 * c->n_tris : 0) + 2 * c->n_mats) : 0;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, c->stream));
-    if (c->bvh.width == 8)
-        hipLaunchKernelGGL(k_adjoint<8>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, sovf.as<int>(), (int)ovf);
-    else
-        hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, sovf.as<int>(), (int)ovf);
+    if (c->bvh.width == 8) {
+        hipLaunchKernelGGL(k_adj_forward<8>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+        hipLaunchKernelGGL(k_adj_grad<8>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+    } else {
+        hipLaunchKernelGGL(k_adj_forward<4>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+        hipLaunchKernelGGL(k_adj_grad<4>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(e1, c->stream));
     HIP_CHECK(hipEventSynchronize(e1));
