@@ -44,8 +44,9 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per processed item for the wavefront kernels (FP32 SoA):
 #   k_extend: ray record read (32) + hit record written (16)
 #   k_shadow: shadow record read (48)
-#   k_shade:  ray (32) + hit (16) + path state read/write (2 x 36) per extension ray
-ALGO_BYTES = {"k_extend": 48.0, "k_shadow": 48.0, "k_shade": 120.0}
+#   k_shade:  ray (32) + hit (16) + path state (64) read, path state (64) +
+#             next ray (32) written, per extension ray
+ALGO_BYTES = {"k_extend": 48.0, "k_shadow": 48.0, "k_shade": 208.0}
 
 
 def _progress(msg: str):
