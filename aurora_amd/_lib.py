@@ -136,6 +136,10 @@ SIGNATURES = {
     "aur_render": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(Stats)]),
     "aur_render_paths": (C.c_int, [_P, _IP, _IP, C.c_int, _FP, _FP, C.POINTER(Stats)]),
     "aur_framebuffer": (C.c_int, [_P, _FP]),
+    "aur_write_image": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.c_int]),
+    "aur_exr_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _FP, C.c_int, C.c_int, C.POINTER(C.c_char_p),
+                                C.POINTER(C.c_char_p), C.c_int]),
+    "aur_exr_read": (C.c_int, [C.c_char_p, _IP, _IP, _FP, C.c_int64]),
     "aur_batch_submit": (C.c_int, [_P, C.POINTER(Job), C.c_int, C.POINTER(BatchOpts), C.POINTER(JobResult),
                                    C.POINTER(Stats)]),
     "aur_set_launch": (C.c_int, [_P, C.POINTER(LaunchOpts)]),

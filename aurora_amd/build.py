@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaurora_hip.so")
 ARCH = os.environ.get("AURORA_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "host_scene.cpp", "bvh_build.hip", "wavefront.hip", "parity.hip", "subdiv.hip", "adjoint.hip", "batch.hip"]
+SOURCES = ["capi.cpp", "host_scene.cpp", "exr_io.cpp", "bvh_build.hip", "wavefront.hip", "parity.hip", "subdiv.hip", "adjoint.hip", "batch.hip"]
 EXTRA_SOURCES_FILE = os.path.join(CSRC, "sources.txt")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
@@ -66,7 +66,7 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lamdhip64"]
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lamdhip64", "-lz"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stdout))

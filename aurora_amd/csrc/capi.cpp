@@ -11,6 +11,7 @@
 #include <string>
 
 #include "context.h"
+#include "exr_io.h"
 #include "subdiv.h"
 
 using namespace aur;
@@ -36,6 +37,9 @@ static int guarded(F &&f) {
     } catch (const CapacityError &e) {
         g_last_error = e.what();
         return AUR_ECAPACITY;
+    } catch (const IoError &e) {
+        g_last_error = e.what();
+        return AUR_EIO;
     } catch (const std::invalid_argument &e) {
         g_last_error = e.what();
         return AUR_EINVAL;
@@ -521,6 +525,78 @@ int aur_framebuffer(aur_ctx *ctx, float *rgba_sum_host) {
         if (!c->accum.p || c->fb_w <= 0) throw StateError("no framebuffer yet (render first)");
         HIP_CHECK(hipStreamSynchronize(c->stream));
         HIP_CHECK(hipMemcpy(rgba_sum_host, c->accum.p, sizeof(float) * 4 * (size_t)c->fb_w * c->fb_h, hipMemcpyDeviceToHost));
+    });
+}
+
+static std::vector<std::pair<std::string, std::string>> meta_list(const char *const *keys, const char *const *values, int n) {
+    if (n < 0) throw std::invalid_argument("negative metadata count");
+    if (n > 0 && (!keys || !values)) throw std::invalid_argument("null metadata arrays");
+    std::vector<std::pair<std::string, std::string>> m;
+    for (int k = 0; k < n; ++k) {
+        if (!keys[k] || !values[k] || !*keys[k]) throw std::invalid_argument("empty metadata key or value");
+        m.push_back({keys[k], values[k]});
+    }
+    return m;
+}
+
+int aur_write_image(aur_ctx *ctx, const char *path, int n_passes, const char *const *meta_keys,
+                    const char *const *meta_values, int n_meta) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(path);
+        if (n_passes <= 0) throw std::invalid_argument("n_passes must be positive");
+        Ctx *c = &ctx->c;
+        set_device(c);
+        if (!c->accum.p || c->fb_w <= 0) throw StateError("no framebuffer yet (render first)");
+        auto meta = meta_list(meta_keys, meta_values, n_meta);
+        bool has_renderer = false;
+        for (auto &kv : meta) has_renderer |= kv.first == "renderer";
+        if (!has_renderer) meta.insert(meta.begin(), {"renderer", "Aurora (MI355X wavefront tracer)"});
+        const size_t n = (size_t)c->fb_w * c->fb_h;
+        std::vector<float> img(4 * n);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipMemcpy(img.data(), c->accum.p, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+        const float inv = 1.f / (float)n_passes;
+        for (float &v : img) v *= inv;
+        try {
+            exr_write(path, c->fb_w, c->fb_h, img.data(), EXR_ZIP, true, meta);
+        } catch (const std::runtime_error &e) {
+            throw IoError(e.what());
+        }
+    });
+}
+
+int aur_exr_write(const char *path, int width, int height, const float *rgba, int compression, int half,
+                  const char *const *meta_keys, const char *const *meta_values, int n_meta) {
+    return guarded([&] {
+        NEED(path);
+        NEED(rgba);
+        if (width <= 0 || height <= 0) throw std::invalid_argument("image size must be positive");
+        try {
+            exr_write(path, width, height, rgba, compression, half != 0, meta_list(meta_keys, meta_values, n_meta));
+        } catch (const std::runtime_error &e) {
+            throw IoError(e.what());
+        }
+    });
+}
+
+int aur_exr_read(const char *path, int *width, int *height, float *rgba_out, int64_t capacity_floats) {
+    return guarded([&] {
+        NEED(path);
+        NEED(width);
+        NEED(height);
+        ExrImage img;
+        try {
+            img = exr_read(path);
+        } catch (const std::runtime_error &e) {
+            throw IoError(e.what());
+        }
+        *width = img.width;
+        *height = img.height;
+        if (rgba_out) {
+            if ((int64_t)img.rgba.size() > capacity_floats) throw CapacityError("rgba_out holds fewer than 4*width*height floats");
+            std::memcpy(rgba_out, img.rgba.data(), sizeof(float) * img.rgba.size());
+        }
     });
 }
 

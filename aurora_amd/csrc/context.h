@@ -25,6 +25,9 @@ struct StateError : std::runtime_error {
 struct CapacityError : std::runtime_error {
     explicit CapacityError(const std::string &m) : std::runtime_error(m) {}
 };
+struct IoError : std::runtime_error {
+    explicit IoError(const std::string &m) : std::runtime_error(m) {}
+};
 
 #define HIP_CHECK(expr)                                                                          \
     do {                                                                                         \

@@ -137,6 +137,13 @@ class Renderer:
         L.check(self.lib.aur_framebuffer(self._ctx, L.fptr(out)))
         return out
 
+    def write_image(self, path: str, n_passes: int, meta: dict | None = None) -> None:
+        """The mean image (framebuffer / n_passes) as an FP16 RGBA ZIP EXR with
+        string metadata, written to <path>_tmp and renamed (OpenexrDisplay::draw)."""
+        from .exr import _meta
+        keys, vals, n = _meta(meta)
+        L.check(self.lib.aur_write_image(self._ctx, path.encode(), int(n_passes), keys, vals, n))
+
     def set_launch(self, trace_block: int = 0, trace_waves_per_cu: int = 0, pool_paths: int = 0) -> None:
         """Launch shape of the tracer kernels; 0 keeps a field."""
         o = L.LaunchOpts(int(trace_block), int(trace_waves_per_cu), int(pool_paths))

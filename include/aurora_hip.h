@@ -183,6 +183,25 @@ int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_pa
  * divide by the pass count for it. */
 int aur_framebuffer(aur_ctx *ctx, float *rgba_sum_host);
 
+/* ---- image files (OpenEXR, scanline, single part) ---- */
+/* Replaces OpenexrDisplay::draw (openexrDisplay.cpp:132-169): writes the mean
+ * image (framebuffer sum / n_passes) as FP16 RGBA, ZIP compressed, with a
+ * "renderer" string attribute plus n_meta key/value strings (e.g. the
+ * reference's renderTime and RaySpeed, renderer.cpp:292-297), to
+ * <path>_tmp and renames it over path. */
+int aur_write_image(aur_ctx *ctx, const char *path, int n_passes, const char *const *meta_keys,
+                    const char *const *meta_values, int n_meta);
+/* Generic writer: rgba is width*height*4 floats, row 0 first; compression
+ * 0 (NONE), 2 (ZIPS) or 3 (ZIP); half != 0 stores FP16 channels. */
+int aur_exr_write(const char *path, int width, int height, const float *rgba, int compression, int half,
+                  const char *const *meta_keys, const char *const *meta_values, int n_meta);
+/* Reader (the texture loader's Imf::RgbaInputFile, texture2D.cpp:24-46):
+ * channels R, G, B, A as HALF or FLOAT, NONE / ZIPS / ZIP.  Writes width and
+ * height; when rgba_out is non-NULL also width*height*4 floats (missing
+ * channels read 0, alpha 1), failing with AUR_ECAPACITY if capacity_floats is
+ * too small. */
+int aur_exr_read(const char *path, int *width, int *height, float *rgba_out, int64_t capacity_floats);
+
 /* ---- batched persistent launch (no counterpart in the reference, which
  * renders one scene per process: Renderer::renderImage, renderer.cpp:201-271) */
 typedef struct {

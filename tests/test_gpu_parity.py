@@ -230,6 +230,24 @@ def test_render_is_deterministic_and_additive(gpu_box_room):
     assert np.array_equal(a, c)
 
 
+def test_write_image_round_trip(gpu_box_room, tmp_path):
+    """Renderer.write_image stores framebuffer / passes as FP16 RGBA with its metadata."""
+    from aurora_amd import exr
+    gpu_box_room.set_seed(4)
+    gpu_box_room.render_passes(0, 3, fetch=False)
+    fb = gpu_box_room.framebuffer()
+    p = str(tmp_path / "box.exr")
+    gpu_box_room.write_image(p, 3, {"spp": "3"})
+    img = exr.read_exr(p)
+    with np.errstate(over="ignore"):
+        want = (fb * (np.float32(1) / np.float32(3))).astype(np.float16).astype(np.float32)
+    assert img.shape == fb.shape
+    assert np.array_equal(img, want)
+    head = open(p, "rb").read(4096)
+    assert b"spp\x00string\x00" in head and b"renderer\x00string\x00" in head
+    assert not os.path.exists(p + "_tmp")
+
+
 # ---------------------------------------------------------------------------
 # Loop subdivision kernels
 # ---------------------------------------------------------------------------
