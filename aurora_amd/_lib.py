@@ -16,6 +16,10 @@ LIB_PATH = os.path.join(HERE, "libaurora_hip.so")
 AUR_MAT_MATTE, AUR_MAT_METAL, AUR_MAT_KELEMEN, AUR_MAT_LIGHT = 0, 1, 2, 3
 AUR_DIST_BECKMANN, AUR_DIST_GGX = 0, 1
 AUR_LIGHT_AREA, AUR_LIGHT_ENV = 0, 1
+AUR_SLOT_DIFFUSE, AUR_SLOT_SPEC, AUR_SLOT_GAIN, AUR_SLOT_ROUGH, AUR_SLOT_ROUGHB, AUR_SLOT_MIX, AUR_SLOT_NORMAL = range(7)
+AUR_N_SLOTS = 7
+AUR_SHADER_TEXTURE2D, AUR_SHADER_TRIPLANAR, AUR_SHADER_CONSTANT = 0, 1, 2
+AUR_SHADER_COLOR, AUR_SHADER_FLOAT = 0, 1
 
 ERROR_NAMES = {-1: "AUR_EINVAL", -2: "AUR_EIO", -3: "AUR_EPARSE", -4: "AUR_EHIP", -5: "AUR_ESTATE",
                -6: "AUR_ECAPACITY"}
@@ -39,12 +43,35 @@ class Material(C.Structure):
     _fields_ = [("type", C.c_int), ("distribution", C.c_int), ("diffuse", C.c_float * 3),
                 ("spec", C.c_float * 3), ("gain", C.c_float * 3), ("roughness", C.c_float),
                 ("roughnessB", C.c_float), ("mix", C.c_float), ("reflectance", C.c_float),
-                ("emission", C.c_float * 3)]
+                ("emission", C.c_float * 3), ("shader", C.c_int * AUR_N_SLOTS), ("c2o", C.c_float * 16)]
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        if "shader" not in kw and len(args) < 11:
+            for k in range(AUR_N_SLOTS):
+                self.shader[k] = -1
+        if "c2o" not in kw and len(args) < 12:
+            for k in range(4):
+                self.c2o[5 * k] = 1.0
 
 
 class Light(C.Structure):
     _fields_ = [("type", C.c_int), ("m", C.c_float * 16), ("sx", C.c_float), ("sy", C.c_float),
-                ("color", C.c_float * 3), ("exposure", C.c_float)]
+                ("color", C.c_float * 3), ("exposure", C.c_float), ("envmap", C.c_int)]
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        if "envmap" not in kw and len(args) < 7:
+            self.envmap = -1
+
+
+class Shader(C.Structure):
+    _fields_ = [("type", C.c_int), ("kind", C.c_int), ("texture", C.c_int), ("scale", C.c_float),
+                ("value", C.c_float * 3)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("offset", C.c_int64)]
 
 
 class SceneDesc(C.Structure):
@@ -53,14 +80,19 @@ class SceneDesc(C.Structure):
                 ("n_attrs", C.c_int), ("mats", C.POINTER(Material)), ("n_lights", C.c_int),
                 ("lights", C.POINTER(Light)), ("fov", C.c_float), ("width", C.c_int), ("height", C.c_int),
                 ("pixelsamples", C.c_int), ("lightsamples", C.c_int), ("mindepth", C.c_int),
-                ("maxdepth", C.c_int)]
+                ("maxdepth", C.c_int), ("uvs", C.POINTER(C.c_float)), ("n_shaders", C.c_int),
+                ("shaders", C.POINTER(Shader)), ("n_textures", C.c_int), ("textures", C.POINTER(Texture)),
+                ("n_texels", C.c_int64), ("texels", C.POINTER(C.c_float)), ("cam_to_world", C.c_float * 16),
+                ("world_to_cam", C.c_float * 16)]
 
 
 class SceneInfo(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("n_verts", "n_tris", "n_attrs", "n_lights", "n_objects", "width", "height",
                                        "pixelsamples", "lightsamples", "mindepth", "maxdepth")] + \
                [("fov", C.c_float)] + \
-               [(n, C.c_int) for n in ("bvh_width", "bvh_nodes", "bvh_depth", "bvh_max_leaf")]
+               [(n, C.c_int) for n in ("bvh_width", "bvh_nodes", "bvh_depth", "bvh_max_leaf", "n_shaders",
+                                       "n_textures")] + \
+               [("n_texels", C.c_int64)]
 
 
 class AccelOpts(C.Structure):
@@ -133,6 +165,7 @@ SIGNATURES = {
     "aur_build_accel": (C.c_int, [_P, C.POINTER(AccelOpts)]),
     "aur_scene_info_get": (C.c_int, [_P, C.POINTER(SceneInfo)]),
     "aur_scene_export": (C.c_int, [_P, _FP, _IP, _FP, _IP, C.POINTER(Material), C.POINTER(Light)]),
+    "aur_scene_export_shading": (C.c_int, [_P, _FP, C.POINTER(Shader), C.POINTER(Texture), _FP, _FP]),
     "aur_render": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(Stats)]),
     "aur_render_paths": (C.c_int, [_P, _IP, _IP, C.c_int, _FP, _FP, C.POINTER(Stats)]),
     "aur_framebuffer": (C.c_int, [_P, _FP]),

@@ -90,7 +90,7 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
     f3 R = L_total;
     while (tb < sv.maxdepth) {
         const float t = h.t, u = h.u, v = h.v;
-        const float w = 1.f - u - v;
+        const float w = (float)(1.0 - (double)u - (double)v);
         const f3 P = add3(o, mul3(d, t));
         const int prim = h.prim;
         const float4 s0 = sv.shade[3 * prim], s1 = sv.shade[3 * prim + 1], s2 = sv.shade[3 * prim + 2];
@@ -109,7 +109,9 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
         }
         if (m.type == MAT_LIGHT) break;
         const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
-        BrdfChoice ch = get_brdf_ex(sv, m, prim, vn, nn, matte, rng.u(vslot + 0));
+        ShadeGeo sg;  // scenes with shaders are refused by the adjoint entry point
+        sg.P = P; sg.Ng = nn; sg.s = 0.f; sg.t = 0.f; sg.attr = attr;
+        BrdfChoice ch = get_brdf_ex<false>(sv, m, prim, vn, nn, matte, rng.u(vslot + 0), sg);
         const Bsdf &b = ch.b;
         const bool spec_lobe = b.kind == BSDF_CT;
         const bool kel = m.type == MAT_KELEMEN;

@@ -197,7 +197,8 @@ __global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, 
     }
 }
 
-__global__ void __launch_bounds__(256, 5) k_bshade(const JobDev *__restrict__ jobs, WfView wv, int q) {
+template <bool TEX>
+__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__restrict__ jobs, WfView wv, int q) {
     __shared__ EmitScratch es;
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(256, 5) k_bshade(const JobDev *__restrict__ jo
                 const int j = __builtin_amdgcn_readfirstlane(job);
                 if (job == j) {
                     const JobDev &jd = jobs[j];
-                    shade_compute(jd.sv, wv, q, i, [&](int4 m) {
+                    shade_compute<TEX>(jd.sv, wv, q, i, [&](int4 m) {
                         PathRng r;
                         r.slots = nullptr;
                         r.key = path_key(jd.seed, (uint32_t)tiled_pixel(m.y, jd.sv.width, jd.sv.height),
@@ -366,7 +367,7 @@ static BatchView batch_view(Ctx *c, int n_jobs, long long total, long long ring_
 }
 
 // One iteration over extension queue q.
-static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q, int width) {
+static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q, int width, bool tex) {
     const int tb = trace_blocks(c);
     const int sb = c->n_cu * 4;
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, c->stream, bv, wv, q);
@@ -374,7 +375,8 @@ static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q
 #define AUR_BEXT(w_, b_) hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs, wv, q)
 #define AUR_BSHD(w_, b_) hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs, wv, 1 - q)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
-    hipLaunchKernelGGL(k_bshade, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
+    if (tex) hipLaunchKernelGGL(k_bshade<true>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
+    else hipLaunchKernelGGL(k_bshade<false>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BSHD);
 #undef AUR_BEXT
 #undef AUR_BSHD
@@ -432,6 +434,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
               int iters_per_poll, aur_job_result *results, aur_stats *total_stats) {
     if (n_jobs <= 0) throw std::invalid_argument("no jobs");
     int width = 0, depth = 0, max_path = 0;
+    bool tex = false;
     std::vector<long long> paths(n_jobs);
     for (int k = 0; k < n_jobs; ++k) {
         Ctx *j = jobs[k];
@@ -442,6 +445,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         if (j->bvh.width != width) throw std::invalid_argument("all jobs of a batch need the same BVH width");
         depth = std::max(depth, j->bvh.depth);
         max_path = std::max(max_path, j->scene.opt.maxdepth + 1);
+        tex = tex || shade_needs_tex(j);
         paths[k] = (long long)j->scene.opt.width * j->scene.opt.height * n_passes[k];
     }
     // launch order
@@ -554,8 +558,8 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     try {
         if (use_graph) {
             HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            batch_iteration(c, bv, wv, 0, width);
-            batch_iteration(c, bv, wv, 1, width);
+            batch_iteration(c, bv, wv, 0, width, tex);
+            batch_iteration(c, bv, wv, 1, width, tex);
             HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
             HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         }
@@ -563,8 +567,8 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
             for (int k = 0; k < pairs_per_poll; ++k) {
                 if (gexec) HIP_CHECK(hipGraphLaunch(gexec, c->stream));
                 else {
-                    batch_iteration(c, bv, wv, 0, width);
-                    batch_iteration(c, bv, wv, 1, width);
+                    batch_iteration(c, bv, wv, 0, width, tex);
+                    batch_iteration(c, bv, wv, 1, width, tex);
                 }
             }
             HIP_CHECK(hipGetLastError());

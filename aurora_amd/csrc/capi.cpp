@@ -108,8 +108,130 @@ static DLight make_dlight(const aur_light &l) {
     } else {
         double e = std::pow(2.0, (double)l.exposure);  // infiniteAreaLight.cpp:61-78 (double pow)
         for (int k = 0; k < 3; ++k) d.Le[k] = (float)(l.color[k] * e);
+        d.escale = e;
     }
+    d.tex = (d.type == LIGHT_ENV) ? l.envmap : -1;
     return d;
+}
+
+// Texture2D::initPdf (texture2D.cpp:120-155) for env maps: the alpha channel
+// becomes the FP16-rounded mean of r, g, b; the normalisation is the float sum
+// of the unrounded means (row by row) over the texel count.
+static float env_pdf_alpha(HostTexture &t) {
+    float lum = 0.f;
+    const size_t n = (size_t)t.width * t.height;
+    for (size_t i = 0; i < n; ++i) {
+        float *p = &t.rgba[4 * i];
+        const float pdf = (p[0] + p[1] + p[2]) / 3.f;
+        p[3] = half_to_float(float_to_half(pdf));
+        lum += pdf;
+    }
+    return lum / (float)(t.width * t.height);
+}
+
+static void upload_shading(Ctx *c) {
+    HostScene &s = c->scene;
+    const int nt = (int)(s.tris.size() / 3);
+    // texture coordinates, only when some material reads them
+    c->has_uv = false;
+    for (const aur_shader &k : s.shaders) c->has_uv = c->has_uv || k.type == AUR_SHADER_TEXTURE2D;
+    if (c->has_uv && nt > 0) {
+        if (s.uvs.size() != (size_t)6 * nt) throw std::invalid_argument("uv array has the wrong size");
+        c->d_uv.reserve(sizeof(float) * s.uvs.size());
+        HIP_CHECK(hipMemcpyAsync(c->d_uv.p, s.uvs.data(), sizeof(float) * s.uvs.size(), hipMemcpyHostToDevice, c->stream));
+    } else {
+        c->has_uv = false;
+    }
+    // per-attribute slots and transforms
+    std::vector<DAttrX> ax(s.attr_mats.size());
+    c->has_attrx = false;
+    for (size_t a = 0; a < s.attr_mats.size(); ++a) {
+        const aur_material &m = s.attr_mats[a];
+        DAttrX &x = ax[a];
+        std::memset(&x, 0, sizeof x);
+        for (int k = 0; k < 8; ++k) x.slot[k] = -1;
+        int mask = 0;
+        if (m.type != AUR_MAT_LIGHT)
+            for (int k = 0; k < AUR_N_SLOTS; ++k) {
+                const int sh = m.shader[k];
+                if (sh < 0) continue;
+                if (sh >= (int)s.shaders.size()) throw std::invalid_argument("material shader index out of range");
+                x.slot[k] = sh;
+                mask |= 1 << k;
+            }
+        std::memcpy(x.c2o, m.c2o, sizeof x.c2o);
+        c->dmats[a].texmask = mask;
+        c->has_attrx = c->has_attrx || mask != 0;
+    }
+    if (c->has_attrx) {
+        c->d_attrx.reserve(sizeof(DAttrX) * ax.size());
+        HIP_CHECK(hipMemcpyAsync(c->d_attrx.p, ax.data(), sizeof(DAttrX) * ax.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    // shaders
+    std::vector<DShader> ds(s.shaders.size());
+    for (size_t i = 0; i < s.shaders.size(); ++i) {
+        const aur_shader &k = s.shaders[i];
+        DShader &d = ds[i];
+        std::memset(&d, 0, sizeof d);
+        if (k.type < AUR_SHADER_TEXTURE2D || k.type > AUR_SHADER_CONSTANT) throw std::invalid_argument("unknown shader type");
+        if (k.kind != AUR_SHADER_COLOR && k.kind != AUR_SHADER_FLOAT) throw std::invalid_argument("unknown shader kind");
+        if (k.type != AUR_SHADER_CONSTANT && (k.texture < 0 || k.texture >= (int)s.textures.size()))
+            throw std::invalid_argument("shader texture index out of range");
+        if (k.type == AUR_SHADER_TRIPLANAR && k.scale == 0.f) throw std::invalid_argument("triplanar scale must not be zero");
+        d.type = k.type;
+        d.kind = k.kind;
+        d.tex = k.type == AUR_SHADER_CONSTANT ? -1 : k.texture;
+        d.scale = k.scale;
+        for (int j = 0; j < 3; ++j) d.value[j] = k.value[j];
+    }
+    if (!ds.empty()) {
+        c->d_shaders.reserve(sizeof(DShader) * ds.size());
+        HIP_CHECK(hipMemcpyAsync(c->d_shaders.p, ds.data(), sizeof(DShader) * ds.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    // textures: FP16 texel pool; env maps get their pdf in alpha
+    std::vector<char> env_use(s.textures.size(), 0);
+    for (DLight &l : c->dlights) {
+        if (l.type != LIGHT_ENV || l.tex < 0) continue;
+        if (l.tex >= (int)s.textures.size()) throw std::invalid_argument("env map texture index out of range");
+        env_use[l.tex] = 1;
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) {
+                l.c2w[4 * r + k] = s.cam_m[4 * r + k];
+                l.w2c[4 * r + k] = s.cam_inv[4 * r + k];
+            }
+    }
+    int64_t n_texels = 0;
+    std::vector<aur_texture> table = texture_table(s, &n_texels);
+    std::vector<DTex> dt(s.textures.size());
+    std::vector<ushort4> texels((size_t)n_texels);
+    for (size_t i = 0; i < s.textures.size(); ++i) {
+        HostTexture t = s.textures[i];
+        if (t.width <= 0 || t.height <= 0 || t.rgba.size() != (size_t)4 * t.width * t.height)
+            throw std::invalid_argument("bad texture");
+        DTex &d = dt[i];
+        std::memset(&d, 0, sizeof d);
+        d.off = table[i].offset;
+        d.w = t.width;
+        d.h = t.height;
+        d.pdf_norm = env_use[i] ? env_pdf_alpha(t) : 1.f;
+        for (size_t p = 0; p < (size_t)t.width * t.height; ++p) {
+            ushort4 &q = texels[(size_t)d.off + p];
+            q.x = float_to_half(t.rgba[4 * p]);
+            q.y = float_to_half(t.rgba[4 * p + 1]);
+            q.z = float_to_half(t.rgba[4 * p + 2]);
+            q.w = float_to_half(t.rgba[4 * p + 3]);
+        }
+    }
+    if (!dt.empty()) {
+        c->d_texs.reserve(sizeof(DTex) * dt.size());
+        HIP_CHECK(hipMemcpyAsync(c->d_texs.p, dt.data(), sizeof(DTex) * dt.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    if (!texels.empty()) {
+        c->d_texels.reserve(sizeof(ushort4) * texels.size());
+        HIP_CHECK(hipMemcpyAsync(c->d_texels.p, texels.data(), sizeof(ushort4) * texels.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    // pageable sources: the copies above complete before this returns
+    HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
 static void upload(Ctx *c) {
@@ -198,6 +320,9 @@ static void upload(Ctx *c) {
         }
         c->dlights.push_back(d);
     }
+    upload_shading(c);  // sets DMat::texmask and the env maps' transforms
+    if (!c->dmats.empty())
+        HIP_CHECK(hipMemcpyAsync(c->d_mats.p, c->dmats.data(), sizeof(DMat) * c->dmats.size(), hipMemcpyHostToDevice, c->stream));
     c->n_lights = (int)c->dlights.size();
     c->d_lights.reserve(sizeof(DLight) * std::max<size_t>(c->dlights.size(), 1));
     if (!c->dlights.empty())
@@ -213,7 +338,7 @@ static void upload(Ctx *c) {
 // objects (sorted JSON order), then area-light quads (renderer.cpp:118-134)
 static void flatten(Ctx *c) {
     HostScene &s = c->scene;
-    s.verts.clear(); s.normals.clear(); s.tris.clear(); s.attr.clear();
+    s.verts.clear(); s.normals.clear(); s.tris.clear(); s.attr.clear(); s.uvs.clear();
     for (const HostObject &o : s.objects) {
         if (o.subdlevels > 0 || o.displace.on) {
             SubdivResult r = subdivide_object(c, o);
@@ -222,6 +347,7 @@ static void flatten(Ctx *c) {
             for (size_t i = 0; i < r.tris.size(); ++i) s.tris.push_back(base + r.tris[i]);
             s.normals.insert(s.normals.end(), r.corner_normals.begin(), r.corner_normals.end());
             for (size_t i = 0; i < r.tris.size() / 3; ++i) s.attr.push_back(o.attr);
+            s.uvs.resize(s.uvs.size() + 2 * r.tris.size(), 0.f);  // subdivided uvs are not carried (zero)
             continue;
         }
         // TriangleMesh::makeEmbree, triangleMesh.cpp:57-81
@@ -234,6 +360,9 @@ static void flatten(Ctx *c) {
             float n[3];
             transform_vector(o.o2c, &o.N[3 * k], n);
             s.normals.insert(s.normals.end(), n, n + 3);
+            // triangleMesh.cpp:40-46, 73-75: uv() = (0, 0) without `vt`
+            s.uvs.push_back(o.UV.empty() ? 0.f : o.UV[2 * k]);
+            s.uvs.push_back(o.UV.empty() ? 0.f : o.UV[2 * k + 1]);
         }
         for (size_t t = 0; t < ncorner / 3; ++t) {
             s.tris.push_back(base + (int)(3 * t));
@@ -255,6 +384,8 @@ static void flatten(Ctx *c) {
             for (int k = 0; k < 3; ++k) {
                 s.tris.push_back(base + idx[3 * t + k]);
                 s.normals.insert(s.normals.end(), nt, nt + 3);
+                s.uvs.push_back(0.f);
+                s.uvs.push_back(0.f);
             }
             s.attr.push_back(s.light_attr[i]);
         }
@@ -387,6 +518,37 @@ int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
         s.attr.assign(d->attr, d->attr + d->n_tris);
         s.attr_mats.assign(d->mats, d->mats + d->n_attrs);
         s.lights.assign(d->lights, d->lights + d->n_lights);
+        if (d->uvs) s.uvs.assign(d->uvs, d->uvs + 6 * (size_t)d->n_tris);
+        else s.uvs.assign(6 * (size_t)d->n_tris, 0.f);
+        if (d->n_shaders < 0 || d->n_textures < 0 || d->n_texels < 0) throw std::invalid_argument("negative count");
+        if (d->n_shaders > 0) { NEED(d->shaders); s.shaders.assign(d->shaders, d->shaders + d->n_shaders); }
+        if (d->n_textures > 0) {
+            NEED(d->textures);
+            NEED(d->texels);
+            for (int i = 0; i < d->n_textures; ++i) {
+                const aur_texture &t = d->textures[i];
+                if (t.width <= 0 || t.height <= 0 || t.offset < 0 || t.offset + (int64_t)t.width * t.height > d->n_texels)
+                    throw std::invalid_argument("texture outside the texel pool");
+                HostTexture h;
+                h.width = t.width;
+                h.height = t.height;
+                const float *p = d->texels + 4 * t.offset;
+                h.rgba.resize((size_t)4 * t.width * t.height);
+                // the reference holds FP16 texels (Imf::Rgba)
+                for (size_t k = 0; k < h.rgba.size(); ++k) h.rgba[k] = half_to_float(float_to_half(p[k]));
+                s.textures.push_back(std::move(h));
+            }
+        }
+        std::memcpy(s.cam_m, d->cam_to_world, sizeof s.cam_m);
+        std::memcpy(s.cam_inv, d->world_to_cam, sizeof s.cam_inv);
+        for (const aur_shader &k : s.shaders)
+            if (k.type != AUR_SHADER_CONSTANT && (k.texture < 0 || k.texture >= d->n_textures))
+                throw std::invalid_argument("shader texture index out of range");
+        for (const aur_material &m : s.attr_mats)
+            for (int k = 0; k < AUR_N_SLOTS; ++k)
+                if (m.shader[k] >= d->n_shaders) throw std::invalid_argument("material shader index out of range");
+        for (const aur_light &l : s.lights)
+            if (l.type == AUR_LIGHT_ENV && l.envmap >= d->n_textures) throw std::invalid_argument("env map index out of range");
         for (int v : s.tris)
             if (v < 0 || v >= d->n_verts) throw std::invalid_argument("triangle index out of range");
         for (int a : s.attr)
@@ -462,6 +624,11 @@ int aur_scene_info_get(aur_ctx *ctx, aur_scene_info *out) {
         out->bvh_nodes = c->bvh.n_nodes;
         out->bvh_depth = c->bvh.depth;
         out->bvh_max_leaf = c->bvh.max_leaf;
+        out->n_shaders = (int)s.shaders.size();
+        out->n_textures = (int)s.textures.size();
+        int64_t n_texels = 0;
+        texture_table(s, &n_texels);
+        out->n_texels = n_texels;
     });
 }
 
@@ -481,6 +648,36 @@ int aur_scene_export(aur_ctx *ctx, float *verts, int *tris, float *normals, int 
         if (attr) std::memcpy(attr, s.attr.data(), sizeof(int) * s.attr.size());
         if (mats) std::memcpy(mats, s.attr_mats.data(), sizeof(aur_material) * s.attr_mats.size());
         if (lights) std::memcpy(lights, s.lights.data(), sizeof(aur_light) * s.lights.size());
+    });
+}
+
+int aur_scene_export_shading(aur_ctx *ctx, float *uvs, aur_shader *shaders, aur_texture *textures, float *texels,
+                             float *cam_to_world_and_inverse) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->have_scene) throw StateError("no scene loaded");
+        if (!c->scene.flat) {
+            if (needs_gpu_flatten(c->scene)) throw StateError("scene needs GPU subdivision (call aur_build_accel first)");
+            flatten(c);
+        }
+        const HostScene &s = c->scene;
+        if (uvs) {
+            const size_t n = 6 * (s.tris.size() / 3);
+            if (s.uvs.size() == n) std::memcpy(uvs, s.uvs.data(), sizeof(float) * n);
+            else std::memset(uvs, 0, sizeof(float) * n);
+        }
+        if (shaders && !s.shaders.empty()) std::memcpy(shaders, s.shaders.data(), sizeof(aur_shader) * s.shaders.size());
+        int64_t n_texels = 0;
+        std::vector<aur_texture> table = texture_table(s, &n_texels);
+        if (textures && !table.empty()) std::memcpy(textures, table.data(), sizeof(aur_texture) * table.size());
+        if (texels)
+            for (size_t i = 0; i < s.textures.size(); ++i)
+                std::memcpy(texels + 4 * table[i].offset, s.textures[i].rgba.data(), sizeof(float) * s.textures[i].rgba.size());
+        if (cam_to_world_and_inverse) {
+            std::memcpy(cam_to_world_and_inverse, s.cam_m, sizeof(float) * 16);
+            std::memcpy(cam_to_world_and_inverse + 16, s.cam_inv, sizeof(float) * 16);
+        }
     });
 }
 
@@ -742,6 +939,10 @@ int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_d
         if (first_pass < 0 || n_passes <= 0) throw std::invalid_argument("bad pass range");
         if (out->albedo_tri && c->tri_albedo.empty())
             throw StateError("albedo gradients need a per-triangle albedo (aur_set_tri_albedo)");
+        for (const DMat &m : c->dmats)
+            if (m.texmask) throw StateError("the adjoint does not differentiate shader-driven (textured) materials");
+        for (const DLight &l : c->dlights)
+            if (l.tex >= 0) throw StateError("the adjoint does not support environment maps");
         set_device(c);
         const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
         const size_t na = 3 * (size_t)c->n_tris, nm = (size_t)c->n_mats;

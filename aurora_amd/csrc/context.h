@@ -149,6 +149,8 @@ struct Ctx {
 
     // device scene
     DevBuf d_verts, d_tris, d_shade, d_mats, d_lights, d_tables, d_albedo;
+    DevBuf d_uv, d_attrx, d_shaders, d_texs, d_texels;  // shading inputs (empty without shaders)
+    bool has_uv = false, has_attrx = false;
     int n_verts = 0, n_tris = 0, n_mats = 0, n_lights = 0;
     float env_le[3] = {0, 0, 0};
     int n_env = 0;

@@ -38,8 +38,15 @@ enum { BSDF_LAMBERT = 0, BSDF_CT = 1 };
 // ---------------------------------------------------------------------------
 // device records
 // ---------------------------------------------------------------------------
+// Material parameters a shader may drive (AUR_SLOT_* of the C ABI).
+enum { SLOT_DIFFUSE = 0, SLOT_SPEC = 1, SLOT_GAIN = 2, SLOT_ROUGH = 3, SLOT_ROUGHB = 4, SLOT_MIX = 5, SLOT_NORMAL = 6 };
+enum { SH_TEXTURE2D = 0, SH_TRIPLANAR = 1, SH_CONSTANT = 2 };
+enum { SH_COLOR = 0, SH_FLOAT = 1 };
+
 struct alignas(16) DMat {
-    int type, dist, table, pad;
+    int type, dist, table;
+    int texmask;          // bit k set: SLOT k is driven by the shader DAttrX::slot[k]
+
     float diffuse[4];
     float spec[4];
     float gain[4];
@@ -47,14 +54,38 @@ struct alignas(16) DMat {
     float emission[4];
 };
 
+// Per-attribute shading extras, read only when DMat::texmask is non-zero.
+struct alignas(16) DAttrX {
+    int slot[8];          // shader index per SLOT_*, -1 when constant
+    float c2o[16];        // camera-to-object matrix of the attribute's object
+};
+
+struct alignas(16) DShader {
+    int type, kind, tex, pad;
+    float scale, value[3];
+};
+
+// A texture in the texel pool: w x h FP16 RGBA texels from `off`.  For env
+// maps the alpha channel holds the sampling pdf value (Texture2D::initPdf).
+struct alignas(16) DTex {
+    long long off;
+    int w, h;
+    float pdf_norm;       // mean pdf value (m_pdf_normalization)
+    int pad[3];
+};
+
 struct alignas(16) DLight {
-    int type, pad0, pad1, pad2;
+    int type, tex, pad1, pad2;  // tex: env map texture, -1 for a constant env light
     float m[12];          // object-to-camera rows 0..2
     float sx, sy, intensity, pad3;
     float Le[4];          // eval(): area color*2^exp (float), env color*2^exp (double-rounded)
     float N[4];           // camera-space normal
     float pcam[4][4];     // corners p1..p4
     float tri[2][12];     // (p, e1, e2) of the pdf() triangles, padded to 4
+    float c2w[12];        // env: camera-to-world rows (InfiniteAreaLight::cameraToWorld)
+    float w2c[12];        // env: world-to-camera rows
+    double escale;        // env: pow(2, exposure) in double (infiniteAreaLight.cpp:76)
+    double pad4;
 };
 
 // ---------------------------------------------------------------------------

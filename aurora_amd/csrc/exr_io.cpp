@@ -288,6 +288,9 @@ ExrImage exr_read(const std::string &path) {
         if (c.type != 1 && c.type != 2 && c.type != 0) throw std::runtime_error(path + ": unknown channel type");
         pixel_bytes += c.type == 1 ? 2 : 4;
     }
+    // a luminance-only file reads as grey (RgbaInputFile: R = G = B = Y)
+    bool has_rgb = false;
+    for (const Chan &c : chans) has_rgb = has_rgb || c.name == "R" || c.name == "G" || c.name == "B";
     const int lpc = lines_per_chunk(compression);
     const int n_chunks = (h + lpc - 1) / lpc;
     std::vector<uint64_t> offs(n_chunks);
@@ -309,7 +312,8 @@ ExrImage exr_read(const std::string &path) {
         const unsigned char *p = raw.data();
         for (int y = y0; y < y1; ++y)
             for (const Chan &ch : chans) {
-                const int dst = ch.name == "R" ? 0 : ch.name == "G" ? 1 : ch.name == "B" ? 2 : ch.name == "A" ? 3 : -1;
+                const int dst = ch.name == "R" ? 0 : ch.name == "G" ? 1 : ch.name == "B" ? 2 : ch.name == "A" ? 3
+                              : (ch.name == "Y" && !has_rgb) ? 4 : -1;
                 for (int x = 0; x < w; ++x) {
                     float v;
                     if (ch.type == 1) {
@@ -321,7 +325,9 @@ ExrImage exr_read(const std::string &path) {
                         if (ch.type == 2) std::memcpy(&v, &u, 4);
                         else v = (float)u;
                     }
-                    if (dst >= 0) img.rgba[4 * ((size_t)y * w + x) + dst] = v;
+                    float *px = &img.rgba[4 * ((size_t)y * w + x)];
+                    if (dst == 4) px[0] = px[1] = px[2] = v;
+                    else if (dst >= 0) px[dst] = v;
                 }
             }
     }

@@ -4,7 +4,8 @@
 // `Imf::RgbaOutputFile` (src/frontend/openexrDisplay.cpp:132-169: FP16 RGBA,
 // ZIP compression, string metadata, written to `<file>_tmp` and renamed) and
 // the texture loader's `Imf::RgbaInputFile` (src/frontend/texture2D.cpp:24-46).
-// Supported: channels R, G, B, A (any subset) as HALF or FLOAT, compression
+// Supported: channels R, G, B, A (any subset; a file with Y and no R, G, B
+// reads as grey) as HALF, FLOAT or UINT, compression
 // NONE, ZIPS (one scanline per chunk) and ZIP (16 scanlines per chunk), the
 // INCREASING_Y line order.  Layout follows the published OpenEXR file format
 // (magic 20000630, version 2, attribute list, chunk offset table, chunks).

@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <utility>
 
+#include "exr_io.h"
 #include "json_lite.h"
 
 namespace aur {
@@ -210,11 +211,20 @@ std::string string_template(const std::string &input, const std::string &config_
 // ---------------------------------------------------------------------------
 static int leading_int(const std::string &s) { return std::atoi(s.c_str()); }
 
+// The uv index of a face corner: the digits between the first and second '/'
+// (objParser.cpp:140-176), 0 when absent.
+static int uv_index(const std::string &e) {
+    size_t a = e.find('/');
+    if (a == std::string::npos) return 0;
+    size_t b = e.find('/', a + 1);
+    return leading_int(e.substr(a + 1, b == std::string::npos ? std::string::npos : b - a - 1));
+}
+
 static void load_obj(const std::string &path, bool need_normals, HostObject *obj) {
     std::ifstream f(path.c_str());
     if (!f.good()) throw std::runtime_error("cannot open OBJ file: " + path);
-    std::vector<float> pts, nrm;
-    std::vector<int> vi, ni;
+    std::vector<float> pts, nrm, tex;
+    std::vector<int> vi, ni, ti;
     std::string line;
     while (std::getline(f, line)) {
         if (line.empty()) continue;
@@ -224,13 +234,14 @@ static void load_obj(const std::string &path, bool need_normals, HostObject *obj
         if (line[0] == 'v') {
             float a = 0, b = 0, c = 0;
             if (line.size() > 1 && line[1] == 'n') { ss >> a >> b >> c; nrm.push_back(a); nrm.push_back(b); nrm.push_back(c); }
-            else if (line.size() > 1 && line[1] == 't') { /* uvs are not used by the tracer */ }
+            else if (line.size() > 1 && line[1] == 't') { ss >> a >> b; tex.push_back(a); tex.push_back(b); }
             else { ss >> a >> b >> c; pts.push_back(a); pts.push_back(b); pts.push_back(c); }
         } else if (line[0] == 'f') {
             std::string e[3];
             ss >> e[0] >> e[1] >> e[2];
             for (int k = 0; k < 3; ++k) {
                 vi.push_back(leading_int(e[k].substr(0, e[k].find('/'))));
+                ti.push_back(uv_index(e[k]));
                 size_t last = e[k].find_last_of('/');
                 ni.push_back(leading_int(last == std::string::npos ? e[k] : e[k].substr(last + 1)));
             }
@@ -239,6 +250,9 @@ static void load_obj(const std::string &path, bool need_normals, HostObject *obj
     size_t nverts = vi.size();
     obj->P.resize(nverts * 3);
     obj->N.resize(nverts * 3);
+    // objTriangleMesh.cpp:43-72: per-corner uvs only when the file has any
+    if (!tex.empty()) obj->UV.resize(nverts * 2);
+    else obj->UV.clear();
     for (size_t i = 0; i < nverts; ++i) {
         int v = vi[i] - 1, n = ni[i] - 1;
         if (v < 0 || (size_t)v * 3 + 2 >= pts.size() + 0 || (size_t)(v * 3 + 2) >= pts.size())
@@ -252,6 +266,12 @@ static void load_obj(const std::string &path, bool need_normals, HostObject *obj
             throw std::runtime_error("OBJ normal index out of range in " + path);
         } else {
             obj->N[3 * i] = 0.f; obj->N[3 * i + 1] = 1.f; obj->N[3 * i + 2] = 0.f;
+        }
+        if (!tex.empty()) {
+            int t = ti[i] - 1;
+            if (t < 0 || (size_t)(2 * t + 1) >= tex.size()) throw std::runtime_error("OBJ uv index out of range in " + path);
+            obj->UV[2 * i] = tex[2 * t];
+            obj->UV[2 * i + 1] = tex[2 * t + 1];
         }
     }
     if (nverts % 3) throw std::runtime_error("OBJ face list is not triangles: " + path);
@@ -297,33 +317,134 @@ static int distribution_attr(const JsonValue &root) {
     throw std::runtime_error("unknown microfacet distribution: " + d);
 }
 
+// Shaders of the "shaders" section by name (ShadingEngine::m_index_map).
+typedef std::map<std::string, int> ShaderNames;
+
+// getShaderAttr + initShader (jsonParser.cpp:145-165, 284-310): a string names
+// a shader of the "shaders" section, anything else is the constant value.
+// Returns the shader index or -1 for a constant written to out.
+static int color_slot(const JsonValue &root, const char *name, float *out, const ShaderNames &sn,
+                      const std::vector<aur_shader> &shaders) {
+    const JsonValue &v = root.at(name);
+    if (v.kind != JsonValue::String) { color_attr(root, name, out); return -1; }
+    auto it = sn.find(v.as_string());
+    if (it == sn.end()) throw std::runtime_error("unknown shader '" + v.as_string() + "' for " + name);
+    if (shaders[it->second].kind != AUR_SHADER_COLOR)
+        throw std::runtime_error("shader '" + v.as_string() + "' is a float shader, " + name + " needs a color");
+    return it->second;
+}
+static int float_slot(const JsonValue &root, const char *name, float *out, const ShaderNames &sn,
+                      const std::vector<aur_shader> &shaders) {
+    const JsonValue &v = root.at(name);
+    if (v.kind != JsonValue::String) { *out = float_attr(root, name); return -1; }
+    auto it = sn.find(v.as_string());
+    if (it == sn.end()) throw std::runtime_error("unknown shader '" + v.as_string() + "' for " + name);
+    if (shaders[it->second].kind != AUR_SHADER_FLOAT)
+        throw std::runtime_error("shader '" + v.as_string() + "' is a color shader, " + name + " needs a float");
+    return it->second;
+}
+
 // jsonParser.cpp:313-429
-static aur_material parse_material(const JsonValue &root) {
+static aur_material parse_material(const JsonValue &root, const ShaderNames &sn, std::vector<aur_shader> *shaders) {
     aur_material m;
     std::memset(&m, 0, sizeof m);
+    for (int k = 0; k < AUR_N_SLOTS; ++k) m.shader[k] = -1;
+    const std::vector<aur_shader> &sh = *shaders;
     const std::string &type = root.at("type").as_string();
     if (type == "kelemenMaterial") {
         m.type = AUR_MAT_KELEMEN;
-        color_attr(root, "diffusecolor", m.diffuse);
-        color_attr(root, "speccolor", m.spec);
-        m.reflectance = float_attr(root, "reflectance");
-        m.roughness = float_attr(root, "roughness");
+        m.shader[AUR_SLOT_DIFFUSE] = color_slot(root, "diffusecolor", m.diffuse, sn, sh);
+        m.shader[AUR_SLOT_SPEC] = color_slot(root, "speccolor", m.spec, sn, sh);
+        m.reflectance = float_attr(root, "reflectance");  // a constant only (jsonParser.cpp:331-333)
+        m.shader[AUR_SLOT_ROUGH] = float_slot(root, "roughness", &m.roughness, sn, sh);
         m.distribution = distribution_attr(root);
     } else if (type == "matteMaterial") {
         m.type = AUR_MAT_MATTE;
-        color_attr(root, "color", m.diffuse);
+        m.shader[AUR_SLOT_DIFFUSE] = color_slot(root, "color", m.diffuse, sn, sh);
     } else if (type == "metalMaterial") {
         m.type = AUR_MAT_METAL;
-        color_attr(root, "reflectance", m.spec);
-        m.roughness = float_attr(root, "roughnessA");
-        m.roughnessB = float_attr(root, "roughnessB");
-        m.mix = float_attr(root, "mix");
-        color_attr(root, "gain", m.gain);
+        m.shader[AUR_SLOT_SPEC] = color_slot(root, "reflectance", m.spec, sn, sh);
+        m.shader[AUR_SLOT_ROUGH] = float_slot(root, "roughnessA", &m.roughness, sn, sh);
+        m.shader[AUR_SLOT_ROUGHB] = float_slot(root, "roughnessB", &m.roughnessB, sn, sh);
+        m.shader[AUR_SLOT_MIX] = float_slot(root, "mix", &m.mix, sn, sh);
+        m.shader[AUR_SLOT_GAIN] = color_slot(root, "gain", m.gain, sn, sh);
         m.distribution = distribution_attr(root);
     } else {
         throw std::runtime_error("unsupported material type: " + type);
     }
+    // jsonParser.cpp:316-321: an optional normal map, a shader or a constant colour
+    if (root.has("normalmap")) {
+        float c[3];
+        int idx = color_slot(root, "normalmap", c, sn, sh);
+        if (idx < 0) {
+            aur_shader k;
+            std::memset(&k, 0, sizeof k);
+            k.type = AUR_SHADER_CONSTANT;
+            k.kind = AUR_SHADER_COLOR;
+            k.texture = -1;
+            for (int i = 0; i < 3; ++i) k.value[i] = c[i];
+            idx = (int)shaders->size();
+            shaders->push_back(k);
+        }
+        m.shader[AUR_SLOT_NORMAL] = idx;
+    }
     return m;
+}
+
+// Texture2D::Texture2D (texture2D.cpp:23-46): the file through Imf::RgbaInputFile,
+// i.e. every channel rounded to FP16.  One texture per file.
+static int load_texture(const std::string &name, const std::string &scene_dir, std::map<std::string, int> *by_path,
+                        std::vector<HostTexture> *textures) {
+    std::string path = name;
+    if (!path.empty() && path[0] != '/') {
+        std::ifstream probe(path.c_str());
+        if (!probe.good()) path = scene_dir + path;
+    }
+    auto it = by_path->find(path);
+    if (it != by_path->end()) return it->second;
+    ExrImage img;
+    try {
+        img = exr_read(path);
+    } catch (const std::runtime_error &e) {
+        throw std::runtime_error(std::string("texture: ") + e.what());
+    }
+    HostTexture t;
+    t.width = img.width;
+    t.height = img.height;
+    t.rgba.resize(img.rgba.size());
+    for (size_t i = 0; i < img.rgba.size(); ++i) t.rgba[i] = half_to_float(float_to_half(img.rgba[i]));
+    const int idx = (int)textures->size();
+    textures->push_back(std::move(t));
+    (*by_path)[path] = idx;
+    return idx;
+}
+
+// jsonParser.cpp:537-617
+static void parse_shaders(const JsonValue &list, const std::string &scene_dir, ShaderNames *names,
+                          std::vector<aur_shader> *shaders, std::map<std::string, int> *tex_by_path,
+                          std::vector<HostTexture> *textures) {
+    for (auto &kv : list.obj) {
+        const JsonValue &o = kv.second;
+        aur_shader k;
+        std::memset(&k, 0, sizeof k);
+        const std::string &at = o.at("attributetype").as_string();
+        if (at == "color") k.kind = AUR_SHADER_COLOR;
+        else if (at == "float") k.kind = AUR_SHADER_FLOAT;
+        else throw std::runtime_error("shader " + kv.first + ": unknown attribute type " + at);
+        const std::string &type = o.at("type").as_string();
+        if (type == "texture2DShader") {
+            k.type = AUR_SHADER_TEXTURE2D;
+        } else if (type == "triplanarTextureShader") {
+            k.type = AUR_SHADER_TRIPLANAR;
+            k.scale = (float)o.at("scale").as_double();
+            if (k.scale == 0.f) throw std::runtime_error("shader " + kv.first + ": scale must not be zero");
+        } else {
+            throw std::runtime_error("shader " + kv.first + ": unknown shader type " + type);
+        }
+        k.texture = load_texture(o.at("texturename").as_string(), scene_dir, tex_by_path, textures);
+        (*names)[kv.first] = (int)shaders->size();
+        shaders->push_back(k);
+    }
 }
 
 void light_corners(const aur_light &l, float pc[4][3], float N[3]) {
@@ -348,6 +469,25 @@ void light_geometry(const aur_light &l, float verts[12], float n[3]) {
     light_corners(l, pc, n);
     for (int i = 0; i < 4; ++i)
         for (int k = 0; k < 3; ++k) verts[3 * i + k] = pc[i][k];
+}
+
+HostScene::HostScene() {
+    Mat4 I = Mat4::identity();
+    std::memcpy(cam_m, I.m, sizeof cam_m);
+    std::memcpy(cam_inv, I.m, sizeof cam_inv);
+}
+
+std::vector<aur_texture> texture_table(const HostScene &s, int64_t *n_texels) {
+    std::vector<aur_texture> out(s.textures.size());
+    int64_t off = 0;
+    for (size_t i = 0; i < s.textures.size(); ++i) {
+        out[i].width = s.textures[i].width;
+        out[i].height = s.textures[i].height;
+        out[i].offset = off;
+        off += (int64_t)s.textures[i].width * s.textures[i].height;
+    }
+    if (n_texels) *n_texels = off;
+    return out;
 }
 
 void load_scene(const std::string &scene_path, const std::string &config_path, HostScene *out) {
@@ -379,11 +519,16 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
         cam = transforms_attr(c);
     }
     Mat4 inv_cam = cam.inv;  // Transform::inverse swaps m and mInv (transform.cpp:36-38)
+    std::memcpy(s.cam_m, cam.m.m, sizeof s.cam_m);
+    std::memcpy(s.cam_inv, cam.inv.m, sizeof s.cam_inv);
 
-    // materials (jsonParser.cpp:507-535), looked up by name at frameBegin
+    // shaders, then materials (jsonParser.cpp:49-50, 507-535, 580-617)
+    ShaderNames shader_names;
+    std::map<std::string, int> tex_by_path;
+    if (root.has("shaders")) parse_shaders(root.at("shaders"), scene_dir, &shader_names, &s.shaders, &tex_by_path, &s.textures);
     std::map<std::string, aur_material> mats;
     if (root.has("materials"))
-        for (auto &kv : root.at("materials").obj) mats[kv.first] = parse_material(kv.second);
+        for (auto &kv : root.at("materials").obj) mats[kv.first] = parse_material(kv.second, shader_names, &s.shaders);
 
     // objects in sorted key order (jsonParser.cpp:619-661)
     std::vector<aur_light> lights;
@@ -400,6 +545,10 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
                 Xform t = transforms_attr(o);
                 Mat4 o2c = inv_cam.mul(t.m);
                 std::memcpy(obj.o2c, o2c.m, sizeof obj.o2c);
+                // the shape's cameraToObject: (invCam * objTransform)'s inverse
+                // matrix, objTransform.mInv * cam.m (transform.cpp:189-194)
+                Mat4 c2o = t.inv.mul(cam.m);
+                std::memcpy(obj.c2o, c2o.m, sizeof obj.c2o);
                 std::string path = string_template(o.at("path").as_string(), config_path);
                 if (!path.empty() && path[0] != '/') {
                     std::ifstream probe(path.c_str());
@@ -414,7 +563,15 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
                     obj.displace.seed = d.has("seed") ? (uint32_t)d.at("seed").as_double() : 0u;
                 }
                 load_obj(path, obj.subdlevels == 0, &obj);
-                if (mats.find(obj.material) == mats.end()) throw std::runtime_error("unknown material: " + obj.material);
+                auto mi = mats.find(obj.material);
+                if (mi == mats.end()) throw std::runtime_error("unknown material: " + obj.material);
+                if ((obj.subdlevels > 0 || obj.displace.on) && !obj.UV.empty())
+                    for (int k = 0; k < AUR_N_SLOTS; ++k) {
+                        int sh = mi->second.shader[k];
+                        if (sh >= 0 && s.shaders[sh].type == AUR_SHADER_TEXTURE2D)
+                            throw std::runtime_error("object " + obj.name +
+                                                     ": uv-mapped textures on subdivided meshes are not supported");
+                    }
                 s.objects.push_back(std::move(obj));
             } else if (type == "light") {
                 // jsonParser.cpp:243-263
@@ -424,6 +581,7 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
                 Xform t = transforms_attr(o);
                 Mat4 o2c = inv_cam.mul(t.m);
                 std::memcpy(l.m, o2c.m, sizeof l.m);
+                l.envmap = -1;
                 l.sx = (float)o.at("scaleX").as_double();
                 l.sy = (float)o.at("scaleY").as_double();
                 l.exposure = (float)o.at("exposure").as_double();
@@ -431,7 +589,8 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
                 o.at("name").as_string();
                 lights.push_back(l);
             } else if (type == "envlight") {
-                // jsonParser.cpp:265-282 (constant branch only)
+                // jsonParser.cpp:265-282; the map is oriented by the camera
+                // transform alone (c2w / w2c), the light's own transform is unused
                 aur_light l;
                 std::memset(&l, 0, sizeof l);
                 l.type = AUR_LIGHT_ENV;
@@ -439,7 +598,8 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
                 std::memcpy(l.m, t.m.m, sizeof l.m);
                 l.exposure = (float)o.at("exposure").as_double();
                 color_attr(o, "color", l.color);
-                if (!o.at("envmap").as_string().empty()) throw std::runtime_error("environment maps are not supported");
+                const std::string &map = o.at("envmap").as_string();
+                l.envmap = map.empty() ? -1 : load_texture(map, scene_dir, &tex_by_path, &s.textures);
                 lights.push_back(l);
             }
             // unknown object types are skipped with a warning in the reference
@@ -449,12 +609,14 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
     for (size_t i = 0; i < s.objects.size(); ++i) {
         s.objects[i].attr = (int)i;
         aur_material m = mats[s.objects[i].material];
+        std::memcpy(m.c2o, s.objects[i].c2o, sizeof m.c2o);
         s.attr_mats.push_back(m);
     }
     for (size_t i = 0; i < lights.size(); ++i) {
         aur_material m;
         std::memset(&m, 0, sizeof m);
         m.type = AUR_MAT_LIGHT;
+        for (int k = 0; k < AUR_N_SLOTS; ++k) m.shader[k] = -1;
         if (lights[i].type == AUR_LIGHT_AREA) {
             // SquareLight::emission(): color * powf(2, exposure)
             float e = powf(2, lights[i].exposure);

@@ -51,10 +51,17 @@ struct HostObject {
     std::string name, material;
     int attr = 0;
     int subdlevels = 0;
-    float o2c[16];
-    std::vector<float> P;  // 9 per triangle, object space
-    std::vector<float> N;  // 9 per triangle, object space
+    float o2c[16], c2o[16];
+    std::vector<float> P;   // 9 per triangle, object space
+    std::vector<float> N;   // 9 per triangle, object space
+    std::vector<float> UV;  // 6 per triangle, or empty when the OBJ has no `vt`
     Displace displace;
+};
+
+// A texture as the reference holds it: FP16 RGBA (Imf::Rgba) widened to float.
+struct HostTexture {
+    int width = 0, height = 0;
+    std::vector<float> rgba;
 };
 
 struct HostScene {
@@ -63,11 +70,18 @@ struct HostScene {
     std::vector<aur_material> attr_mats;  // objects then lights
     std::vector<aur_light> lights;        // JSON (sorted) order
     std::vector<int> light_attr;          // attribute index of each light
+    std::vector<aur_shader> shaders;
+    std::vector<HostTexture> textures;
+    float cam_m[16], cam_inv[16];         // camera-to-world and its inverse
     // flattened camera-space scene (filled by flatten or by upload)
     bool flat = false;
-    std::vector<float> verts, normals;
+    std::vector<float> verts, normals, uvs;
     std::vector<int> tris, attr;
+    HostScene();
 };
+
+// Texel pool layout of the scene's textures: offsets in texels.
+std::vector<aur_texture> texture_table(const HostScene &s, int64_t *n_texels);
 
 // Parses scene + OBJ files.  Throws std::runtime_error with a message; the
 // C ABI turns that into an error code.

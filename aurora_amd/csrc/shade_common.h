@@ -15,6 +15,11 @@ struct SceneView {
     const DLight *lights;
     const float *tables;
     const float *albedo;    // optional per-triangle diffuse override
+    const float2 *uv;       // 3 per triangle (texture coordinates), or null
+    const DAttrX *attrx;    // per attribute, for materials with shaders
+    const DShader *shaders;
+    const DTex *texs;
+    const ushort4 *texels;  // FP16 RGBA
     int n_lights;
     int has_env;
     float env[3];
@@ -23,6 +28,8 @@ struct SceneView {
 };
 
 SceneView scene_view(Ctx *c);
+// Whether a scene needs the shade kernels built with shader and env-map code.
+bool shade_needs_tex(const Ctx *c);
 
 // Path order within one pass: bands of 8 rows, each walked in 8-column tiles
 // (the last tile of a band may be narrower, the last band shorter), so the 64
@@ -37,6 +44,181 @@ __device__ __host__ inline int tiled_pixel(int local, int w, int h) {
     const int cw = w - cx * 8 < 8 ? w - cx * 8 : 8;
     const int idx = l - cx * hb * 8;
     return (y0 + idx / cw) * w + cx * 8 + idx % cw;
+}
+
+// ---------------------------------------------------------------------------
+// textures and shaders: texture2D.cpp:48-66 (nearest read), texture2DShader.h,
+// triplanarTextureShader.h, material.cpp:19-33 (normal map)
+// ---------------------------------------------------------------------------
+__device__ inline float half_bits_to_float(unsigned short v) { return (float)__builtin_bit_cast(_Float16, v); }
+
+// Texture2D::read with filterSize 0.  The reference clamps an index past the
+// edge to [0, size], which only an argument beyond 1 + 1/size reaches (no
+// caller produces one); the clamp here keeps every read inside the texture.
+__device__ inline float4 tex_read(const SceneView &sv, int t, float s, float tt) {
+    const DTex T = sv.texs[t];
+    int x = (int)floorf(s * (float)T.w);
+    int y = (int)floorf(tt * (float)T.h);
+    if (y == T.h) y -= 1;
+    if (x == T.w) x -= 1;
+    x = min(max(x, 0), T.w - 1);
+    y = min(max(y, 0), T.h - 1);
+    const ushort4 q = sv.texels[T.off + (long long)y * T.w + x];
+    return make_float4(half_bits_to_float(q.x), half_bits_to_float(q.y), half_bits_to_float(q.z),
+                       half_bits_to_float(q.w));
+}
+
+// transform.cpp:206-232 on a row-major 4x4
+__device__ inline f3 xform_vec16(const float *m, f3 v) {
+    return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+               m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+__device__ inline f3 xform_point16(const float *m, f3 p) {
+    f3 r = mk3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+               m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+    const float wp = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    if (wp == 1.f) return r;
+    const float inv = 1.f / wp;
+    return mk3(r.x * inv, r.y * inv, r.z * inv);
+}
+__device__ inline float f3_at(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+// What a shader sees of a hit (ShadingGeometry): the hit point, the
+// face-forwarded interpolated normal Ng, the interpolated uv and the
+// attribute (for its camera-to-object transform).
+struct ShadeGeo {
+    f3 P, Ng;
+    float s, t;
+    int attr;
+};
+
+// Shader<Color>::evaluate / Shader<float>::evaluate (red channel)
+__device__ inline f3 shader_eval(const SceneView &sv, int idx, const ShadeGeo &g) {
+    const DShader sh = sv.shaders[idx];
+    if (sh.type == SH_CONSTANT) return mk3(sh.value[0], sh.value[1], sh.value[2]);
+    if (sh.type == SH_TEXTURE2D) {
+        // texture2DShader.h:28-31: read(u - floor(u), 1 - (v - floor(v)))
+        const float s = g.s - floorf(g.s);
+        const float t = 1.f - (g.t - floorf(g.t));
+        const float4 c = tex_read(sv, sh.tex, s, t);
+        return mk3(c.x, c.y, c.z);
+    }
+    // triplanar (color): triplanarTextureShader.h:28-53 (object-space normal,
+    // threshold 0.25); float: 76-99 (camera-space Ng, threshold 0.2 in double)
+    const float *c2o = sv.attrx[g.attr].c2o;
+    f3 wv;
+    if (sh.kind == SH_COLOR) {
+        const f3 N = norm3(xform_vec16(c2o, g.Ng));
+        wv = mk3(fabsf(N.x) - 0.25f, fabsf(N.y) - 0.25f, fabsf(N.z) - 0.25f);
+    } else {
+        wv = mk3((float)((double)fabsf(g.Ng.x) - 0.2), (float)((double)fabsf(g.Ng.y) - 0.2),
+                 (float)((double)fabsf(g.Ng.z) - 0.2));
+    }
+    // std::max(w, 0.f), then the weights over their float sum
+    wv = mk3(wv.x < 0.f ? 0.f : wv.x, wv.y < 0.f ? 0.f : wv.y, wv.z < 0.f ? 0.f : wv.z);
+    const float wsum = (0.f + wv.x + wv.y) + wv.z;
+    wv = mk3(wv.x / wsum, wv.y / wsum, wv.z / wsum);
+    f3 P = xform_point16(c2o, g.P);
+    const float inv = 1.f / sh.scale;
+    P = mk3(P.x * inv, P.y * inv, P.z * inv);
+    f3 r = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float wi = f3_at(wv, i);
+        if (wi > 0.f) {
+            const float s = fabsf(fmodf(f3_at(P, (i + 1) % 3), 1.f));
+            const float t = fabsf(fmodf(f3_at(P, (i + 2) % 3), 1.f));
+            const float4 c = tex_read(sv, sh.tex, s, t);
+            r = mk3(r.x + c.x * wi, r.y + c.y * wi, r.z + c.z * wi);
+        }
+    }
+    return r;
+}
+
+// A material parameter: the shader's value when the slot is shader-driven.
+// TEX = false compiles the shader code out (scenes without shaders).
+template <bool TEX>
+__device__ inline f3 mat_color(const SceneView &sv, const DMat &m, int slot, const float *k, const ShadeGeo &g) {
+    if constexpr (TEX) {
+        if (m.texmask & (1 << slot)) return shader_eval(sv, sv.attrx[g.attr].slot[slot], g);
+    }
+    return mk3(k[0], k[1], k[2]);
+}
+template <bool TEX>
+__device__ inline float mat_float(const SceneView &sv, const DMat &m, int slot, float k, const ShadeGeo &g) {
+    if constexpr (TEX) {
+        if (m.texmask & (1 << slot)) return shader_eval(sv, sv.attrx[g.attr].slot[slot], g).x;
+    }
+    return k;
+}
+
+// Material::runNormalShader (material.cpp:19-33): the map colour, unscaled,
+// as a tangent-space direction around Ng.
+__device__ inline f3 normal_map(const SceneView &sv, const DMat &m, const ShadeGeo &g) {
+    const f3 c = shader_eval(sv, sv.attrx[g.attr].slot[SLOT_NORMAL], g);
+    const f3 ns = tangent_to_world(c, g.Ng);
+    if (!(ns.x != 0.f || ns.y != 0.f || ns.z != 0.f)) return g.Ng;
+    return norm3(ns);
+}
+
+// ---------------------------------------------------------------------------
+// environment maps: infiniteAreaLight.cpp:35-106
+// ---------------------------------------------------------------------------
+#define AUR_PI_D 3.14159265358979323846
+
+// eval(): the map at the camera-space direction d, times 2^exposure
+__device__ inline f3 env_eval(const SceneView &sv, const DLight &l, f3 d) {
+    if (l.tex < 0) return mk3(l.Le[0], l.Le[1], l.Le[2]);
+    const f3 w = xform_vec16(l.c2w, d);
+    const float v = 0.5f * (1.f - w.y);
+    float u = (float)((double)atan2f(w.z, w.x) / (AUR_PI_D * 2.0));
+    if (u < 0.f) u += 1.f;
+    const float4 c = tex_read(sv, l.tex, u, v);
+    return mk3((float)(c.x * l.escale), (float)(c.y * l.escale), (float)(c.z * l.escale));
+}
+
+// texture pdf value at (s, t): Texture2D::pdf
+__device__ inline float env_tex_pdf(const SceneView &sv, const DLight &l, float s, float t) {
+    return tex_read(sv, l.tex, s, t).w / sv.texs[l.tex].pdf_norm;
+}
+
+// generateSample / eval / pdf with the map; other lights go to dev_common.h
+__device__ inline LightSample light_sample_sv(const SceneView &sv, const DLight &l, f3 orig, f3 nn, float x, float y) {
+    if (l.type != LIGHT_ENV || l.tex < 0) return light_sample(l, orig, nn, x, y);
+    LightSample s;
+    const float vy = 1.f - 2.f * y;
+    const float r = sqrtf(fmaxf(0.f, 1.f - vy * vy));
+    const float phi = (float)(2.0 * AUR_PI_D * (double)x);
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    const f3 dw = xform_vec16(l.w2c, mk3(r * cp, vy, r * sp));
+    const float pdf = env_tex_pdf(sv, l, x, y);
+    s.dir = norm3(dw);
+    s.tmax = 1000000.f;
+    s.pdf = (float)((double)pdf / (AUR_PI_D * 4.0));
+    s.color = env_eval(sv, l, s.dir);
+    return s;
+}
+__device__ inline float light_pdf_sv(const SceneView &sv, const DLight &l, f3 o, f3 d, float tmin, float *tmax) {
+    if (l.type != LIGHT_ENV || l.tex < 0) return light_pdf(l, o, d, tmin, tmax);
+    const f3 w = xform_vec16(l.c2w, d);
+    const float v = 0.5f * (1.f - w.y);
+    float u = (float)((double)atan2f(w.z, w.x) / (AUR_PI_D * 2.0));
+    if (u < 0.f) u += 1.f;
+    return (float)((double)env_tex_pdf(sv, l, u, v) / (AUR_PI_D * 2.0));  // hemisphere domain
+}
+__device__ inline f3 light_le_sv(const SceneView &sv, const DLight &l, f3 d) {
+    if (l.type != LIGHT_ENV || l.tex < 0) return mk3(l.Le[0], l.Le[1], l.Le[2]);
+    return env_eval(sv, l, d);
+}
+// integrator.cpp:287-297: every env light's eval() in the camera direction
+__device__ inline f3 env_miss(const SceneView &sv, f3 d) {
+    f3 L = mk3(0.f, 0.f, 0.f);
+    for (int i = 0; i < sv.n_lights; ++i) {
+        const DLight &l = sv.lights[i];
+        if (l.type == LIGHT_ENV) L = add3(L, env_eval(sv, l, d));
+    }
+    return L;
 }
 
 struct PathRng {
@@ -55,28 +237,32 @@ struct BrdfChoice {
     float drough;         // d(roughness used) / d(material roughness)
 };
 
+template <bool TEX>
 __device__ inline BrdfChoice get_brdf_ex(const SceneView &sv, const DMat &m, int prim, f3 vn, f3 nn, bool matte,
-                                         float u_lobe) {
+                                         float u_lobe, const ShadeGeo &g) {
     BrdfChoice ch;
     ch.kelemen_choice = false;
     ch.A = 1.f;
     ch.dA = 0.f;
     ch.drough = 1.f;
     Bsdf &b = ch.b;
-    f3 diffuse = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-    if (sv.albedo) diffuse = mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2]);
     if (m.type == MAT_MATTE) {
+        f3 diffuse = sv.albedo ? mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2])
+                               : mat_color<TEX>(sv, m, SLOT_DIFFUSE, m.diffuse, g);
         b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f;
     } else if (m.type == MAT_METAL) {
+        const float mix = mat_float<TEX>(sv, m, SLOT_MIX, m.mix, g);
+        const f3 refl = mat_color<TEX>(sv, m, SLOT_SPEC, m.spec, g);
+        const f3 gain = mat_color<TEX>(sv, m, SLOT_GAIN, m.gain, g);
         float ct = dot3(vn, nn);
-        f3 sc = mk3(schlick(ct, m.spec[0]), schlick(ct, m.spec[1]), schlick(ct, m.spec[2]));
-        float rough = (u_lobe > m.mix) ? m.rough : m.roughB;
+        f3 sc = mk3(schlick(ct, refl.x), schlick(ct, refl.y), schlick(ct, refl.z));
+        float rough = (u_lobe > mix) ? mat_float<TEX>(sv, m, SLOT_ROUGH, m.rough, g) : mat_float<TEX>(sv, m, SLOT_ROUGHB, m.roughB, g);
         if (matte) { rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR; ch.drough = 1.f - AUR_CAUSTIC_BLUR; }
-        b.kind = BSDF_CT; b.dist = m.dist; b.color = mul3v(sc, mk3(m.gain[0], m.gain[1], m.gain[2])); b.rough = rough;
+        b.kind = BSDF_CT; b.dist = m.dist; b.color = mul3v(sc, gain); b.rough = rough;
     } else {
-        float rough = m.rough;
+        float rough = mat_float<TEX>(sv, m, SLOT_ROUGH, m.rough, g);
         if (matte) { rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR; ch.drough = 1.f - AUR_CAUSTIC_BLUR; }
-        b.kind = BSDF_CT; b.dist = m.dist; b.color = mk3(m.spec[0], m.spec[1], m.spec[2]); b.rough = rough;
+        b.kind = BSDF_CT; b.dist = m.dist; b.color = mat_color<TEX>(sv, m, SLOT_SPEC, m.spec, g); b.rough = rough;
         if (m.refl < 1.f) {
             float c = fminf(fmaxf(dot3(nn, vn), 0.f), 1.f);
             const float *tab = sv.tables + (size_t)m.table * (AUR_KEL_COS * AUR_KEL_ROUGH);
@@ -84,14 +270,20 @@ __device__ inline BrdfChoice get_brdf_ex(const SceneView &sv, const DMat &m, int
             ch.kelemen_choice = true;
             ch.A = albedo;
             ch.dA = kelemen_albedo_drough(tab, c, rough);
-            if (u_lobe > albedo) { b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f; }
+            if (u_lobe > albedo) {
+                f3 diffuse = sv.albedo ? mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2])
+                                       : mat_color<TEX>(sv, m, SLOT_DIFFUSE, m.diffuse, g);
+                b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f;
+            }
         }
     }
     return ch;
 }
 
-__device__ inline Bsdf get_brdf(const SceneView &sv, const DMat &m, int prim, f3 vn, f3 nn, bool matte, float u_lobe) {
-    return get_brdf_ex(sv, m, prim, vn, nn, matte, u_lobe).b;
+template <bool TEX>
+__device__ inline Bsdf get_brdf(const SceneView &sv, const DMat &m, int prim, f3 vn, f3 nn, bool matte, float u_lobe,
+                                const ShadeGeo &g) {
+    return get_brdf_ex<TEX>(sv, m, prim, vn, nn, matte, u_lobe, g).b;
 }
 
 }  // namespace aur

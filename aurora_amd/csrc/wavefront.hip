@@ -122,7 +122,10 @@ __global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv, int nq) {
 // ---------------------------------------------------------------------------
 // shade: one path vertex per queue entry
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256, 5) k_shade(SceneView sv, WfView wv, int q) {
+// 96 VGPRs (5 waves per SIMD) without the shader code; the shader, normal-map
+// and environment-map code needs more registers and runs at 4.
+template <bool TEX>
+__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView wv, int q) {
     __shared__ EmitScratch es;
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(256, 5) k_shade(SceneView sv, WfView wv, int q
         ShadeOut o;
         o.cont = o.finish = false;
         o.sh[0].valid = o.sh[1].valid = false;
-        if (i < n) shade_compute(sv, wv, q, i, [&](int4 m) { return path_rng(sv, wv, m.y); }, 0, o);
+        if (i < n) shade_compute<TEX>(sv, wv, q, i, [&](int4 m) { return path_rng(sv, wv, m.y); }, 0, o);
         shade_emit(wv, q, i < n, o, es);
     }
 }
@@ -267,6 +270,13 @@ void timer_release(Ctx *c) {
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
+bool shade_needs_tex(const Ctx *c) {
+    if (c->has_attrx) return true;
+    for (const DLight &l : c->dlights)
+        if (l.type == LIGHT_ENV && l.tex >= 0) return true;
+    return false;
+}
+
 SceneView scene_view(Ctx *c) {
     SceneView sv;
     sv.nodes = c->bvh.nodes.p;
@@ -276,6 +286,11 @@ SceneView scene_view(Ctx *c) {
     sv.lights = c->d_lights.as<DLight>();
     sv.tables = c->d_tables.as<float>();
     sv.albedo = c->tri_albedo.empty() ? nullptr : c->d_albedo.as<float>();
+    sv.uv = c->has_uv ? c->d_uv.as<float2>() : nullptr;
+    sv.attrx = c->has_attrx ? c->d_attrx.as<DAttrX>() : nullptr;
+    sv.shaders = c->d_shaders.as<DShader>();
+    sv.texs = c->d_texs.as<DTex>();
+    sv.texels = c->d_texels.as<ushort4>();
     sv.n_lights = c->n_lights;
     sv.has_env = c->n_env > 0;
     sv.env[0] = c->env_le[0]; sv.env[1] = c->env_le[1]; sv.env[2] = c->env_le[2];
@@ -369,7 +384,8 @@ static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
 #define AUR_EXT(w_, b_) hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q)
 #define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, 1 - q)
         TIMED(c, KT_EXTEND, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
-        TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
+        if (shade_needs_tex(c)) TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<true>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
+        else TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<false>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         TIMED(c, KT_SHADOW, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_SHD));
 #undef AUR_EXT
 #undef AUR_SHD

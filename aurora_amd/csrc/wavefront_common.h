@@ -163,7 +163,9 @@ struct ShadeOut {
     int job;
 };
 
-template <class MakeRng>
+// TEX selects the code for shaders, normal maps and environment maps; the
+// host uses it only for scenes that have any.
+template <bool TEX, class MakeRng>
 __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job,
                                      ShadeOut &out) {
     const int4 meta = wv.meta[q][i];
@@ -185,7 +187,9 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
     if (prim < 0) {
         finish = true;
         if (rtype == RT_CAMERA) {  // integrator.cpp:287-297: camera miss sees the environment
-            La = make_float4(sv.env[0], sv.env[1], sv.env[2], sv.has_env ? 1.f : 0.f);
+            f3 e = mk3(sv.env[0], sv.env[1], sv.env[2]);
+            if constexpr (TEX) e = sv.has_env ? env_miss(sv, mk3(rb.x, rb.y, rb.z)) : mk3(0.f, 0.f, 0.f);
+            La = make_float4(e.x, e.y, e.z, sv.has_env ? 1.f : 0.f);
             flags |= F_NOCLAMP;
         }
     } else if (tb >= sv.maxdepth) {
@@ -196,13 +200,24 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
         // hit record: embreeAccelerator.cpp:111-136
         const f3 o = mk3(ra.x, ra.y, ra.z), d = mk3(rb.x, rb.y, rb.z);
         const float t = hv.x, u = hv.y, v = hv.z;
-        const float w = 1.f - u - v;
+        const float w = (float)(1.0 - (double)u - (double)v);  // `float w = 1. - u - v` (double)
         P = add3(o, mul3(d, t));
         const float4 s0 = sv.shade[3 * prim], s1 = sv.shade[3 * prim + 1], s2 = sv.shade[3 * prim + 2];
         const int attr = __float_as_int(s0.w);
         f3 nm = mk3(w * s0.x + u * s1.x + v * s2.x, w * s0.y + u * s1.y + v * s2.y, w * s0.z + u * s1.z + v * s2.z);
         if (dot3(nm, d) > 0.f) nm = mul3(nm, -1.f);
         f3 nn = norm3(nm);
+        const DMat m = sv.mats[attr];
+        ShadeGeo g;
+        g.P = P; g.Ng = nn; g.attr = attr; g.s = 0.f; g.t = 0.f;
+        if (TEX && m.texmask) {
+            if (sv.uv) {
+                const float2 t0 = sv.uv[3 * prim], t1 = sv.uv[3 * prim + 1], t2 = sv.uv[3 * prim + 2];
+                g.s = w * t0.x + u * t1.x + v * t2.x;
+                g.t = w * t0.y + u * t1.y + v * t2.y;
+            }
+            if (m.texmask & (1 << SLOT_NORMAL)) nn = normal_map(sv, m, g);
+        }
         // integrator.cpp:132-150
         const f3 vn = neg3(norm3(d));
         if (dot3(vn, nn) <= 0.f) nn = norm3(cross3(vn, cross3(nn, vn)));
@@ -210,7 +225,6 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
         if (rtype == RT_DIFFUSE) matte = true;
         f3 T = mk3(T4.x, T4.y, T4.z);
         float cp = T4.w;
-        const DMat m = sv.mats[attr];
         if (rtype == RT_CAMERA) {
             float4 &la = La;
             la.x += T.x * m.emission[0]; la.y += T.y * m.emission[1]; la.z += T.z * m.emission[2];
@@ -220,14 +234,15 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
         if (m.type == MAT_LIGHT) {
             finish = true;  // ConstantMaterial: getSample has pdf 0 (constant.cpp:23-25)
         } else {
-            Bsdf b = get_brdf(sv, m, prim, vn, nn, matte, rng.u(vslot + 0));
+            Bsdf b = get_brdf<TEX>(sv, m, prim, vn, nn, matte, rng.u(vslot + 0), g);
             // integrateDirect, integrator.cpp:100-130
             int nvis = 0;
             const int li = pick_visible_light(sv.lights, sv.n_lights, P, nn, rng.u(vslot + 1), &nvis);
             if (li >= 0) {
                 const DLight &l = sv.lights[li];
                 const float scale = (float)nvis;
-                LightSample ls = light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
+                LightSample ls = TEX ? light_sample_sv(sv, l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3))
+                                     : light_sample(l, P, nn, rng.u(vslot + 2), rng.u(vslot + 3));
                 float ct = dot3(nn, ls.dir);
                 if (b.kind == BSDF_LAMBERT) {
                     // integrateDirectLight, integrator.cpp:33-54
@@ -253,10 +268,11 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
                     float ct2 = dot3(nn, bs.dir);
                     if (ct2 > 0.f) {
                         float tmax = bs.tmax;
-                        float lpdf = light_pdf(l, P, bs.dir, AUR_RAY_BIAS, &tmax);
+                        float lpdf = TEX ? light_pdf_sv(sv, l, P, bs.dir, AUR_RAY_BIAS, &tmax)
+                                         : light_pdf(l, P, bs.dir, AUR_RAY_BIAS, &tmax);
                         if (lpdf > 0.f && bs.pdf > 0.f) {
                             float wgt = power_heuristic(bs.pdf, lpdf);
-                            f3 le = mk3(l.Le[0], l.Le[1], l.Le[2]);
+                            const f3 le = TEX ? light_le_sv(sv, l, bs.dir) : mk3(l.Le[0], l.Le[1], l.Le[2]);
                             f3 c = div3(mul3(mul3v(mul3(bs.color, wgt), le), ct2), bs.pdf);
                             sh[1].valid = true; sh[1].o = P; sh[1].d = bs.dir; sh[1].tmin = AUR_RAY_BIAS; sh[1].tmax = tmax;
                             sh[1].c = mul3v(mul3(c, scale), T);

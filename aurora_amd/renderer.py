@@ -73,6 +73,22 @@ class Renderer:
         d.pixelsamples = scene.get("pixelsamples", 1)
         d.lightsamples = scene.get("lightsamples", 1)
         d.mindepth, d.maxdepth = scene["mindepth"], scene["maxdepth"]
+        if scene.get("uvs") is not None:
+            uvs = keep["uvs"] = np.ascontiguousarray(scene["uvs"], dtype=np.float32).reshape(-1)
+            d.uvs = L.fptr(uvs)
+        shaders = scene.get("shaders") or []
+        if shaders:
+            sh = keep["shaders"] = (L.Shader * len(shaders))(*shaders)
+            d.n_shaders, d.shaders = len(shaders), C.cast(sh, C.POINTER(L.Shader))
+        textures = scene.get("textures") or []
+        if textures:
+            tx = keep["textures"] = (L.Texture * len(textures))(*textures)
+            texels = keep["texels"] = np.ascontiguousarray(scene["texels"], dtype=np.float32).reshape(-1)
+            d.n_textures, d.textures = len(textures), C.cast(tx, C.POINTER(L.Texture))
+            d.n_texels, d.texels = texels.size // 4, L.fptr(texels)
+        cam = np.asarray(scene.get("camera", np.concatenate([np.eye(4), np.eye(4)]).reshape(-1)), np.float32)
+        for k in range(16):
+            d.cam_to_world[k], d.world_to_cam[k] = float(cam[k]), float(cam[16 + k])
         L.check(self.lib.aur_scene_upload(self._ctx, C.byref(d)))
         self.passes_done = 0
 
@@ -108,11 +124,21 @@ class Renderer:
         lights = (L.Light * max(i["n_lights"], 1))()
         L.check(self.lib.aur_scene_export(self._ctx, L.fptr(verts), L.iptr(tris), L.fptr(normals), L.iptr(attr),
                                           mats, lights))
+        uvs = np.zeros(6 * nt, np.float32)
+        shaders = (L.Shader * max(i["n_shaders"], 1))()
+        textures = (L.Texture * max(i["n_textures"], 1))()
+        texels = np.zeros(4 * max(i["n_texels"], 1), np.float32)
+        cam = np.zeros(32, np.float32)
+        L.check(self.lib.aur_scene_export_shading(self._ctx, L.fptr(uvs), shaders, textures, L.fptr(texels),
+                                                  L.fptr(cam)))
         return {"verts": verts.reshape(-1, 3), "tris": tris.reshape(-1, 3), "normals": normals.reshape(-1, 9),
                 "attr": attr, "materials": [mats[k] for k in range(i["n_attrs"])],
                 "lights": [lights[k] for k in range(i["n_lights"])], "fov": i["fov"], "width": i["width"],
                 "height": i["height"], "maxdepth": i["maxdepth"], "mindepth": i["mindepth"],
-                "pixelsamples": i["pixelsamples"], "lightsamples": i["lightsamples"]}
+                "pixelsamples": i["pixelsamples"], "lightsamples": i["lightsamples"], "uvs": uvs.reshape(-1, 6),
+                "shaders": [shaders[k] for k in range(i["n_shaders"])],
+                "textures": [textures[k] for k in range(i["n_textures"])], "texels": texels[:4 * i["n_texels"]],
+                "camera": cam}
 
     # -- rendering -----------------------------------------------------------
     def render_passes(self, first_pass: int, n_passes: int, fetch: bool = True):

@@ -13,6 +13,9 @@ invalid/unknown keys", jsonParser.cpp:47):
   * geometry ``"displace": {amplitude, octaves, frequency, seed}`` applies
     radial value-noise displacement after Loop subdivision.
 
+Textures (the "shaders" section, env maps) are OpenEXR files written next to
+the scene.
+
 Random numbers come from SplitMix64 with an explicit ``(x >> 40) * 2**-24``
 float conversion, so the scenes are identical on every platform.
 """
@@ -54,17 +57,35 @@ class SplitMix64:
 # OBJ writers
 # --------------------------------------------------------------------------
 
-def write_obj(path: str, verts, faces, normals=None) -> None:
+def write_obj(path: str, verts, faces, normals=None, uvs=None) -> None:
     """Writes an OBJ with ``v``/``vn``/``f v//n`` lines (1-based indices).
 
     ``normals`` are per vertex (same count as ``verts``); when omitted each
     face gets its geometric normal, which ``ObjTriangleMesh`` copies per
-    corner (objTriangleMesh.cpp:228-242).
+    corner (objTriangleMesh.cpp:228-242).  ``uvs`` (per vertex) adds ``vt``
+    lines and ``f v/t/n`` corners.
     """
     verts = np.asarray(verts, dtype=np.float64)
     faces = np.asarray(faces, dtype=np.int64)
     lines = ["# generated by aurora_amd.scenes"]
     lines += ["v %.9g %.9g %.9g" % tuple(v) for v in verts]
+    if uvs is not None:
+        uvs = np.asarray(uvs, dtype=np.float64)
+        lines += ["vt %.9g %.9g" % tuple(t) for t in uvs]
+        if normals is None:
+            normals = np.zeros_like(verts)
+            for f in faces:
+                a, b, c = verts[f[0]], verts[f[1]], verts[f[2]]
+                n = np.cross(b - a, c - a)
+                normals[f] += n / (np.linalg.norm(n) or 1.0)
+            normals /= np.maximum(np.linalg.norm(normals, axis=1, keepdims=True), 1e-12)
+        lines += ["vn %.9g %.9g %.9g" % tuple(n) for n in np.asarray(normals, dtype=np.float64)]
+        lines += ["f %d/%d/%d %d/%d/%d %d/%d/%d" % tuple(int(x) + 1 for x in (f[0], f[0], f[0], f[1], f[1], f[1],
+                                                                             f[2], f[2], f[2]))
+                  for f in faces]
+        with open(path, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        return
     if normals is None:
         fn = []
         for f in faces:
@@ -130,14 +151,21 @@ class SceneSpec:
     materials: dict = field(default_factory=dict)
     objects: dict = field(default_factory=dict)
     shaders: dict = field(default_factory=dict)
-    meshes: dict = field(default_factory=dict)  # obj file name -> (verts, faces, normals|None)
+    meshes: dict = field(default_factory=dict)  # obj file name -> (verts, faces, normals|None[, uvs])
+    textures: dict = field(default_factory=dict)  # exr file name -> (H, W, 4) float array, FP16 flag
 
     def write(self, out_dir: str, scene_name: str = "scene.asc") -> str:
         os.makedirs(out_dir, exist_ok=True)
         mesh_dir = os.path.join(out_dir, "meshes")
         os.makedirs(mesh_dir, exist_ok=True)
-        for name, (v, f, n) in self.meshes.items():
-            write_obj(os.path.join(mesh_dir, name), v, f, n)
+        for name, mesh in self.meshes.items():
+            write_obj(os.path.join(mesh_dir, name), *mesh)
+        if self.textures:
+            from . import exr
+            tex_dir = os.path.join(out_dir, "textures")
+            os.makedirs(tex_dir, exist_ok=True)
+            for name, (rgba, half) in self.textures.items():
+                exr.write_exr(os.path.join(tex_dir, name), rgba, compression="zip", half=half)
         cfg = {"basepath": os.path.abspath(out_dir) + "/", "cachepath": "$BASE/cache/",
                "renderpath": "$BASE/render/", "texturepath": "$BASE/textures",
                "hdripath": "$BASE/hdri"}
@@ -150,9 +178,16 @@ class SceneSpec:
             o = dict(o)
             if o.get("type") == "geometry":
                 o["path"] = os.path.abspath(os.path.join(out_dir, o["path"]))
+            if o.get("type") == "envlight" and o.get("envmap"):
+                o["envmap"] = os.path.abspath(os.path.join(out_dir, o["envmap"]))
             objects[k] = o
+        shaders = {}
+        for k, sh in self.shaders.items():
+            sh = dict(sh)
+            sh["texturename"] = os.path.abspath(os.path.join(out_dir, sh["texturename"]))
+            shaders[k] = sh
         root = {"options": self.options, "renderCam": self.camera, "materials": self.materials,
-                "objects": objects, "shaders": self.shaders}
+                "objects": objects, "shaders": shaders}
         path = os.path.join(out_dir, scene_name)
         with open(path, "w") as fh:
             json.dump(root, fh, indent=4)
@@ -361,6 +396,134 @@ def batch_job(seed: int, k: int) -> SceneSpec:
     return s
 
 
+def icosphere(level: int):
+    """Unit icosphere by midpoint subdivision of the icosahedron (projected to
+    the sphere), with the vertex normals of a sphere."""
+    v, f = icosahedron()
+    verts = [tuple(x) for x in v]
+    faces = [tuple(x) for x in f]
+    for _ in range(level):
+        cache, nf = {}, []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = (np.array(verts[a]) + np.array(verts[b])) / 2.0
+                verts.append(tuple(m / np.linalg.norm(m)))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = np.array(verts, dtype=np.float64)
+    return v, np.array(faces, dtype=np.int64), v.copy()
+
+
+def _tex_noise(rng: "SplitMix64", h: int, w: int, lo, hi) -> np.ndarray:
+    a = np.array([[rng.uniform() for _ in range(w)] for _ in range(h)])
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    out = np.ones((h, w, 4))
+    out[..., :3] = lo + a[..., None] * (hi - lo)
+    return out.astype(np.float32)
+
+
+def textured_stage(res=96, spp=4, maxdepth=4, mindepth=4, sphere_subd=0, seed=5) -> SceneSpec:
+    """Open stage exercising the shader and environment-map paths.
+
+    * floor: matte, colour from a uv-mapped checker (texture2DShader), uvs
+      tiling 3x and reaching negative values;
+    * back wall: Kelemen coat, uv-mapped diffuse colour, uv-mapped float
+      roughness, a uv-mapped normal map;
+    * sphere: matte with a triplanar colour (``sphere_subd`` > 0 subdivides it
+      on the device instead of using the 320-triangle mesh);
+    * rotated box without uvs: metal with triplanar reflectance, triplanar
+      float roughnessA, a uv-mapped float mix (read at uv = 0) and a constant
+      normal map;
+    * lights: a square light, an env light with a lat-long map (stored as FP32,
+      rounded to FP16 when loaded) and a constant env light;
+    * a rotated camera, so the env map's camera-to-world transform matters.
+    """
+    rng = SplitMix64(seed)
+    s = SceneSpec(options=_options(res, res, spp, maxdepth, mindepth),
+                  camera={"fov": math.tan(math.radians(28.0)),
+                          "transforms": [{"translate": [0.2, 1.3, 4.6]}, {"rotateY": 4.0}, {"rotateX": -9.0}]})
+    # floor with uvs in [-1, 2]
+    fv = [[-3.0, 0.0, 3.0], [3.0, 0.0, 3.0], [3.0, 0.0, -3.0], [-3.0, 0.0, -3.0]]
+    fuv = [[-1.0, -1.0], [2.0, -1.0], [2.0, 2.0], [-1.0, 2.0]]
+    s.meshes["floor.obj"] = (fv, [[0, 1, 2], [0, 2, 3]], None, fuv)
+    wv = [[-3.0, 0.0, -2.5], [3.0, 0.0, -2.5], [3.0, 3.0, -2.5], [-3.0, 3.0, -2.5]]
+    wuv = [[0.0, -0.2], [1.5, -0.2], [1.5, 1.2], [0.0, 1.2]]
+    s.meshes["wall.obj"] = (wv, [[0, 1, 2], [0, 2, 3]], None, wuv)
+    if sphere_subd > 0:
+        v, f = icosahedron()
+        s.meshes["sphere.obj"] = (v, f, v.copy())
+    else:
+        s.meshes["sphere.obj"] = icosphere(2)
+    v, f = box_mesh([-0.5, 0.0, -0.5], [0.5, 1.0, 0.5])
+    s.meshes["box.obj"] = (v, f, None)
+
+    checker = np.ones((16, 16, 4), np.float32)
+    ij = np.add.outer(np.arange(16), np.arange(16)) % 2
+    checker[..., :3] = np.where(ij[..., None] == 1, [0.8, 0.78, 0.7], [0.15, 0.2, 0.3])
+    checker[0, 0, :3] = [0.9, 0.1, 0.1]  # marks the texel at s = 0, t = 0
+    s.textures["checker.exr"] = (checker, True)
+    s.textures["wallcolor.exr"] = (_tex_noise(rng, 32, 32, [0.2, 0.3, 0.5], [0.7, 0.6, 0.4]), True)
+    s.textures["roughmap.exr"] = (_tex_noise(rng, 32, 32, [0.08, 0.0, 0.0], [0.7, 0.0, 0.0]), True)
+    bumps = _tex_noise(rng, 16, 16, [-0.3, -0.3, 1.0], [0.3, 0.3, 1.0])
+    s.textures["bumps.exr"] = (bumps, True)
+    s.textures["tricol.exr"] = (_tex_noise(rng, 24, 24, [0.1, 0.2, 0.1], [0.9, 0.5, 0.8]), True)
+    s.textures["trirefl.exr"] = (_tex_noise(rng, 8, 8, [0.5, 0.4, 0.2], [0.95, 0.8, 0.6]), True)
+    s.textures["trirough.exr"] = (_tex_noise(rng, 16, 16, [0.05, 0.0, 0.0], [0.6, 0.0, 0.0]), True)
+    s.textures["mixmap.exr"] = (_tex_noise(rng, 4, 4, [0.2, 0.0, 0.0], [0.8, 0.0, 0.0]), True)
+    # lat-long sky: vertical gradient, a bright sun, FP32 storage
+    H, W = 32, 64
+    sky = np.ones((H, W, 4), np.float32)
+    t = (np.arange(H) + 0.5) / H
+    sky[..., 0] = (0.3 + 0.5 * t)[:, None]
+    sky[..., 1] = (0.45 + 0.3 * t)[:, None]
+    sky[..., 2] = (0.9 - 0.4 * t)[:, None]
+    sky[8:11, 40:44, :3] = [40.0, 36.0, 30.0]
+    sky[...] += np.float32(1.0 / 3.0) * 1e-3  # values that FP16 rounds
+    s.textures["sky.exr"] = (sky, False)
+
+    s.shaders["checker"] = {"type": "texture2DShader", "attributetype": "color", "texturename": "textures/checker.exr"}
+    s.shaders["wallcolor"] = {"type": "texture2DShader", "attributetype": "color",
+                              "texturename": "textures/wallcolor.exr"}
+    s.shaders["roughmap"] = {"type": "texture2DShader", "attributetype": "float",
+                             "texturename": "textures/roughmap.exr"}
+    s.shaders["bumps"] = {"type": "texture2DShader", "attributetype": "color", "texturename": "textures/bumps.exr"}
+    s.shaders["tricol"] = {"type": "triplanarTextureShader", "attributetype": "color",
+                           "texturename": "textures/tricol.exr", "scale": 0.35}
+    s.shaders["trirefl"] = {"type": "triplanarTextureShader", "attributetype": "color",
+                            "texturename": "textures/trirefl.exr", "scale": 0.5}
+    s.shaders["trirough"] = {"type": "triplanarTextureShader", "attributetype": "float",
+                             "texturename": "textures/trirough.exr", "scale": 0.4}
+    s.shaders["mixmap"] = {"type": "texture2DShader", "attributetype": "float", "texturename": "textures/mixmap.exr"}
+
+    s.materials["floor"] = {"type": "matteMaterial", "color": "checker"}
+    s.materials["wall"] = {"type": "kelemenMaterial", "diffusecolor": "wallcolor", "speccolor": [0.9, 0.9, 0.9],
+                           "reflectance": 0.1, "roughness": "roughmap", "normalmap": "bumps"}
+    s.materials["marble"] = {"type": "matteMaterial", "color": "tricol"}
+    s.materials["brushed"] = {"type": "metalMaterial", "reflectance": "trirefl", "roughnessA": "trirough",
+                              "roughnessB": 0.3, "mix": "mixmap", "gain": [1.0, 0.95, 0.9],
+                              "normalmap": [0.1, 0.05, 1.0], "distribution": "ggx"}
+    s.objects["floor"] = _geo("floor", "floor.obj", "floor")
+    s.objects["wall"] = _geo("wall", "wall.obj", "wall")
+    s.objects["sphere"] = _geo("sphere", "sphere.obj", "marble",
+                               [{"translate": [-0.9, 0.7, -0.3]}, {"scale": [0.7, 0.7, 0.7]}],
+                               subdlevels=sphere_subd)
+    s.objects["box"] = _geo("box", "box.obj", "brushed",
+                            [{"translate": [0.9, 0.0, 0.2]}, {"rotateY": 33.0}, {"scale": [0.8, 1.1, 0.8]}])
+    s.objects["light_key"] = _square_light("light_key", [10.0, 9.0, 8.0], 0.5, 0.5,
+                                           [{"translate": [1.5, 3.2, 1.0]}, {"rotateX": -70.0}])
+    s.objects["sky"] = {"type": "envlight", "name": "sky", "exposure": 0.5, "color": [1.0, 1.0, 1.0],
+                        "envmap": "textures/sky.exr", "transforms": [{"rotateY": 90.0}]}
+    s.objects["sky_fill"] = {"type": "envlight", "name": "sky_fill", "exposure": -2.0, "color": [0.3, 0.3, 0.35],
+                             "envmap": "", "transforms": []}
+    return s
+
+
 def batch_jobs(out_dir: str, n_jobs: int = 64, seed: int = 3, max_spp: int = 0) -> list:
     """Writes the batched workload's jobs; returns their scene file paths.
     ``max_spp`` > 0 caps the sample count (for tests)."""
@@ -385,6 +548,8 @@ def write_config(name: str, out_dir: str, **kw) -> str:
         spec = box_room(kelemen=True, **kw)
     elif name in ("displaced_spheres", "c2"):
         spec = displaced_spheres(**kw)
+    elif name == "textured_stage":
+        spec = textured_stage(**kw)
     else:
         raise ValueError("unknown workload %r" % name)
     return spec.write(out_dir)

@@ -51,7 +51,7 @@ typedef struct {
     int prim;
 } aur_hit;
 
-/* Material kinds (src/shading/materials/*Material.cpp). */
+/* Material kinds (the classes of src/shading/materials/). */
 #define AUR_MAT_MATTE 0
 #define AUR_MAT_METAL 1
 #define AUR_MAT_KELEMEN 2
@@ -61,7 +61,19 @@ typedef struct {
 #define AUR_DIST_GGX 1
 
 #define AUR_LIGHT_AREA 0 /* SquareLight (src/lights/squareLight.*) */
-#define AUR_LIGHT_ENV 1  /* InfiniteAreaLight, constant branch (src/geometry/infiniteAreaLight.*) */
+#define AUR_LIGHT_ENV 1  /* InfiniteAreaLight (src/geometry/infiniteAreaLight.*) */
+
+/* Material parameters that a shader may drive (ShadingEngine indices held by
+ * the materials, jsonParser.cpp:312-365); AUR_SLOT_NORMAL is the normal map
+ * (Material::runNormalShader, material.cpp:19-33). */
+#define AUR_SLOT_DIFFUSE 0 /* matte color / kelemen diffusecolor */
+#define AUR_SLOT_SPEC 1    /* kelemen speccolor / metal reflectance */
+#define AUR_SLOT_GAIN 2    /* metal gain */
+#define AUR_SLOT_ROUGH 3   /* kelemen roughness / metal roughnessA */
+#define AUR_SLOT_ROUGHB 4  /* metal roughnessB */
+#define AUR_SLOT_MIX 5     /* metal mix */
+#define AUR_SLOT_NORMAL 6  /* tangent-space normal map */
+#define AUR_N_SLOTS 7
 
 typedef struct {
     int type;          /* AUR_MAT_* */
@@ -74,6 +86,8 @@ typedef struct {
     float mix;         /* metal mix */
     float reflectance; /* kelemen reflectance (selects the albedo table) */
     float emission[3]; /* light triangles: color * 2^exposure */
+    int shader[AUR_N_SLOTS]; /* aur_shader index driving each AUR_SLOT_*, or -1 for the constant above */
+    float c2o[16];     /* camera-to-object matrix of the attribute's object, row major (triplanar shaders) */
 } aur_material;
 
 typedef struct {
@@ -82,7 +96,34 @@ typedef struct {
     float sx, sy;    /* area light: scaleX, scaleY */
     float color[3];
     float exposure;
+    int envmap;      /* env light: aur_texture index of its map, or -1 for the constant color */
 } aur_light;
+
+/* Shaders (src/shading/shaders/): a texture read by uv (Texture2DShader), a
+ * texture projected along the three object axes (TriplanarTextureShader), or
+ * a constant (ConstantShader, only needed for a constant normal map).  kind
+ * selects the colour or the float (red channel) variant. */
+#define AUR_SHADER_TEXTURE2D 0
+#define AUR_SHADER_TRIPLANAR 1
+#define AUR_SHADER_CONSTANT 2
+#define AUR_SHADER_COLOR 0
+#define AUR_SHADER_FLOAT 1
+
+typedef struct {
+    int type;       /* AUR_SHADER_* */
+    int kind;       /* AUR_SHADER_COLOR or AUR_SHADER_FLOAT */
+    int texture;    /* aur_texture index (texture shaders) */
+    float scale;    /* triplanar: object-space units per texture repeat */
+    float value[3]; /* constant shader */
+} aur_shader;
+
+/* A texture (Texture2D, texture2D.cpp:23-46): width x height RGBA texels,
+ * row 0 first, at `offset` texels into the scene's texel pool.  Values are
+ * the FP16 values the reference's Imf::Rgba buffer holds. */
+typedef struct {
+    int width, height;
+    int64_t offset;
+} aur_texture;
 
 /* Flattened, camera-space scene: what `EmbreeMesh::preRender`
  * (src/geometry/embreeMesh.h:30-58) hands to Embree plus the per-attribute
@@ -100,6 +141,15 @@ typedef struct {
     float fov;
     int width, height;
     int pixelsamples, lightsamples, mindepth, maxdepth;
+    /* shading inputs (all may be empty / NULL) */
+    const float *uvs;     /* 6 * n_tris per-corner texture coordinates, or NULL for all zero */
+    int n_shaders;
+    const aur_shader *shaders;
+    int n_textures;
+    const aur_texture *textures;
+    int64_t n_texels;
+    const float *texels;  /* 4 * n_texels */
+    float cam_to_world[16], world_to_cam[16]; /* camera transform and its inverse (env maps) */
 } aur_scene_desc;
 
 typedef struct {
@@ -107,6 +157,8 @@ typedef struct {
     int width, height, pixelsamples, lightsamples, mindepth, maxdepth;
     float fov;
     int bvh_width, bvh_nodes, bvh_depth, bvh_max_leaf;
+    int n_shaders, n_textures;
+    int64_t n_texels;
 } aur_scene_info;
 
 typedef struct {
@@ -165,6 +217,11 @@ int aur_scene_info_get(aur_ctx *ctx, aur_scene_info *out);
 /* Copies the flattened scene to host buffers (any pointer may be NULL). */
 int aur_scene_export(aur_ctx *ctx, float *verts, int *tris, float *normals, int *attr,
                      aur_material *mats, aur_light *lights);
+/* The shading inputs of the flattened scene (any pointer may be NULL):
+ * 6 * n_tris uvs, the shaders, the textures, 4 * n_texels texel values and
+ * the camera transform with its inverse (32 floats). */
+int aur_scene_export_shading(aur_ctx *ctx, float *uvs, aur_shader *shaders, aur_texture *textures,
+                             float *texels, float *cam_to_world_and_inverse);
 
 /* ---- rendering ---- */
 /* Renders passes [first_pass, first_pass + n_passes) into the device

@@ -406,8 +406,150 @@ void orc_light_init(orc_light *l) {
 
 typedef struct { orc_ray ray; float pdf; C3 color; } Sample3D;
 
+/* ------------------------------------------------------------------------ */
+/* textures and shaders                                                      */
+/* ------------------------------------------------------------------------ */
+/* FP16 round-to-nearest-even, by value: 11 significant bits in the normal
+ * range, steps of 2^-24 below 2^-14, infinity from 65520 on (the rounding of
+ * Imf::Rgba's half). */
+float orc_half_round(float x) {
+    if (isnan(x) || isinf(x)) return x;
+    float a = fabsf(x), r;
+    if (a < 6.103515625e-05f) {
+        r = ldexpf(rintf(ldexpf(a, 24)), -24);
+    } else {
+        int e;
+        float f = frexpf(a, &e);
+        r = ldexpf(rintf(ldexpf(f, 11)), e - 11);
+        if (r > 65504.f) r = INFINITY;
+    }
+    return copysignf(r, x);
+}
+
+static int to_int_x86(float v) {
+    /* cvttss2si: out-of-range and NaN give INT_MIN */
+    if (!(v >= -2147483648.f && v < 2147483648.f)) return INT_MIN;
+    return (int)v;
+}
+
+/* texture2D.cpp:48-66 (filterSize 0).  An index left at the size by the
+ * clamps (arguments beyond 1 + 1/size) would read past the row; no caller
+ * passes one, and the last texel is used instead. */
+static const float *texel(const orc_texture *t, float s, float tt) {
+    int x = to_int_x86(floorf(s * t->width));
+    int y = to_int_x86(floorf(tt * t->height));
+    if (y == t->height) y -= 1;
+    if (x == t->width) x -= 1;
+    if (x > t->width || x < 0) { x = x < t->width ? x : t->width; x = x > 0 ? x : 0; }
+    if (y > t->height || y < 0) { y = y < t->height ? y : t->height; y = y > 0 ? y : 0; }
+    if (x >= t->width) x = t->width - 1;
+    if (y >= t->height) y = t->height - 1;
+    return t->rgba + 4 * ((size_t)y * t->width + x);
+}
+
+void orc_texture_read(const orc_texture *t, float s, float tt, float *rgb_out) {
+    const float *p = texel(t, s, tt);
+    rgb_out[0] = p[0]; rgb_out[1] = p[1]; rgb_out[2] = p[2];
+}
+
+/* texture2D.cpp:120-155: alpha <- (r+g+b)/3 stored as half, normalisation =
+ * float sum of the unrounded values over the texel count */
+void orc_texture_init_pdf(orc_texture *t) {
+    float lum = 0;
+    for (int i = 0; i < t->height; i++)
+        for (int j = 0; j < t->width; j++) {
+            float *p = t->rgba + 4 * ((size_t)i * t->width + j);
+            float pdf = (p[0] + p[1] + p[2]) / 3.f;
+            p[3] = orc_half_round(pdf);
+            lum += pdf;
+        }
+    t->pdf_norm = lum / (t->width * t->height);
+}
+
+/* texture2D.cpp:80-103 */
+static float texture_pdf(const orc_texture *t, float s, float tt) { return texel(t, s, tt)[3] / t->pdf_norm; }
+
+static V3 xform_vec(const float *m, V3 v) {
+    return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+              m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+static float v3_at(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+/* Shader<T>::evaluate: constantShader.h, texture2DShader.h:28-31 and 47-50,
+ * triplanarTextureShader.h:28-53 (color) and 76-99 (float: red channel) */
+static C3 shader_eval(const orc_scene *sc, int idx, V3 P, V3 Ng, float su, float sv, int attr) {
+    const orc_shader *sh = &sc->shaders[idx];
+    if (sh->type == ORC_SHADER_CONSTANT) return c3(sh->value[0], sh->value[1], sh->value[2]);
+    const orc_texture *tex = &sc->textures[sh->texture];
+    if (sh->type == ORC_SHADER_TEXTURE2D) {
+        const float *p = texel(tex, su - floorf(su), 1.f - (sv - floorf(sv)));
+        return c3(p[0], p[1], p[2]);
+    }
+    const float *c2o = sc->mats[attr].c2o;
+    float w[3], wsum = 0.f;
+    if (sh->kind == ORC_SHADER_COLOR) {
+        V3 N = vnorm(xform_vec(c2o, Ng));
+        for (int i = 0; i < 3; ++i) {
+            w[i] = fabsf(v3_at(N, i));
+            w[i] = (float)(w[i] - 0.25);
+            w[i] = (w[i] < 0.f) ? 0.f : w[i];
+            wsum += w[i];
+        }
+    } else {
+        for (int i = 0; i < 3; ++i) {
+            w[i] = fabsf(v3_at(Ng, i));
+            w[i] = (float)(w[i] - 0.2);
+            w[i] = (w[i] < 0.f) ? 0.f : w[i];
+            wsum += w[i];
+        }
+    }
+    for (int i = 0; i < 3; ++i) w[i] /= wsum;
+    V3 Po = xform_point(c2o, P);
+    float inv = 1.f / sh->scale;
+    Po = v3(Po.x * inv, Po.y * inv, Po.z * inv);
+    C3 res = c3(0.f, 0.f, 0.f);
+    for (int i = 0; i < 3; ++i) {
+        if (w[i] > 0) {
+            float s = fabsf(fmodf(v3_at(Po, (i + 1) % 3), 1.f));
+            float t = fabsf(fmodf(v3_at(Po, (i + 2) % 3), 1.f));
+            const float *p = texel(tex, s, t);
+            res = cadd(res, cmulf(c3(p[0], p[1], p[2]), w[i]));
+        }
+    }
+    return res;
+}
+
+void orc_shader_eval(const orc_scene *sc, int shader, const float *P, const float *Ng, float s, float t, int attr,
+                     float *out) {
+    C3 c = shader_eval(sc, shader, v3p(P), v3p(Ng), s, t, attr);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b;
+}
+
+/* infiniteAreaLight.cpp:61-78 with the map: u from atan2 of the world
+ * direction (the division in double), v = (1 - y) / 2 */
+static void env_uv(const orc_light *l, V3 d, float *u, float *v) {
+    V3 dir = xform_vec(l->c2w, d);
+    *v = (float)(0.5 * (1.f - dir.y));
+    *u = (float)(atan2f(dir.z, dir.x) / (M_PI * 2.f));
+    if (*u < 0.f) *u += 1.f;
+}
+
+static C3 light_eval(const orc_scene *sc, const orc_light *l, V3 d) {
+    if (l->type == ORC_LIGHT_AREA) return cmulf(c3p(l->color), l->intensity); /* squareLight.h:84-86 */
+    if (l->envmap < 0) return cmuld(c3p(l->color), pow(2, l->exposure));      /* infiniteAreaLight.cpp:74-76 */
+    float u, v;
+    env_uv(l, d, &u, &v);
+    const float *p = texel(&sc->textures[l->envmap], u, v);
+    return cmuld(c3(p[0], p[1], p[2]), pow(2, l->exposure));
+}
+
+void orc_env_eval(const orc_scene *sc, const orc_light *l, const float *dir, float *rgb_out) {
+    C3 c = light_eval(sc, l, v3p(dir));
+    rgb_out[0] = c.r; rgb_out[1] = c.g; rgb_out[2] = c.b;
+}
+
 /* squareLight.h:57-82 / infiniteAreaLight.cpp:35-59 */
-static Sample3D light_sample(const orc_light *l, V3 orig, V3 nn, float x, float y) {
+static Sample3D light_sample(const orc_scene *sc, const orc_light *l, V3 orig, V3 nn, float x, float y) {
     Sample3D s;
     if (l->type == ORC_LIGHT_AREA) {
         V3 lp = v3((float)((x - 0.5) * l->sx * 2), (float)((y - 0.5) * l->sy * 2), 0.f);
@@ -425,24 +567,30 @@ static Sample3D light_sample(const orc_light *l, V3 orig, V3 nn, float x, float 
         if (d == 0.f) s.pdf = 0.f;
         else s.pdf = (float)((len * len) / (d * l->sx * l->sy * 4.));
     } else {
-        V3 dt = sample_hemisphere_uniform(x, y);
-        V3 dw = vnorm(tangent_to_world(dt, nn));
+        V3 dw;
+        float pdf;
+        if (l->envmap >= 0) {
+            float vy = 1.f - 2.f * y;
+            float r = sqrtf(fmaxf(0.f, 1.f - vy * vy));
+            float phi = (float)(2.f * M_PI * x);
+            dw = xform_vec(l->w2c, v3(r * cosf(phi), vy, r * sinf(phi)));
+            pdf = texture_pdf(&sc->textures[l->envmap], x, y);
+        } else {
+            pdf = 2.f;
+            dw = tangent_to_world(sample_hemisphere_uniform(x, y), nn);
+        }
+        dw = vnorm(dw);
         s.ray.o[0] = orig.x; s.ray.o[1] = orig.y; s.ray.o[2] = orig.z;
         s.ray.d[0] = dw.x; s.ray.d[1] = dw.y; s.ray.d[2] = dw.z;
         s.ray.tmin = (float)RAY_BIAS; s.ray.tmax = 1000000.f;
-        s.pdf = (float)(2.f / (M_PI * 4.));
-        s.color = cmuld(c3p(l->color), pow(2, l->exposure));
+        s.pdf = (float)(pdf / (M_PI * 4.));
+        s.color = light_eval(sc, l, dw);
     }
     return s;
 }
 
-static C3 light_eval(const orc_light *l) {
-    if (l->type == ORC_LIGHT_AREA) return cmulf(c3p(l->color), l->intensity); /* squareLight.h:84-86 */
-    return cmuld(c3p(l->color), pow(2, l->exposure));                      /* infiniteAreaLight.cpp:61-78 */
-}
-
 /* squareLight.h:88-97 (ray maxt shortened on hit) / infiniteAreaLight.cpp:80-106 */
-static float light_pdf(const orc_light *l, orc_ray *ray) {
+static float light_pdf(const orc_scene *sc, const orc_light *l, orc_ray *ray) {
     if (l->type == ORC_LIGHT_AREA) {
         float t, b1, b2;
         int hit = orc_intersect_tri(l->tri[0], l->tri[0] + 3, l->tri[0] + 6, ray, &t, &b1, &b2);
@@ -455,7 +603,10 @@ static float light_pdf(const orc_light *l, orc_ray *ray) {
         ray->tmax = (float)(ray->tmax - RAY_BIAS * 10);
         return pdf;
     }
-    return (float)(1.f / (M_PI * 2.f));
+    if (l->envmap < 0) return (float)(1.f / (M_PI * 2.f));
+    float u, v;
+    env_uv(l, v3p(ray->d), &u, &v);
+    return (float)(texture_pdf(&sc->textures[l->envmap], u, v) / (M_PI * 2.f)); /* Hemisphere domain */
 }
 
 /* squareLight.h:99-110; infiniteAreaLight.h:29 */
@@ -646,34 +797,54 @@ float orc_kelemen_albedo(const float *tab, float costheta, float roughness) {
 /* ------------------------------------------------------------------------ */
 /* materials: getBrdf                                                        */
 /* ------------------------------------------------------------------------ */
-/* Returns 0 for the constant (light) material, 1 otherwise. */
-static int get_brdf(const orc_scene *s, int attr, int prim, V3 vn, V3 nn, int matte_path, float u_lobe, Bsdf *b) {
+/* What the shaders see of a hit (ShadingGeometry): P, the face-forwarded
+ * interpolated normal Ng, the interpolated uv and the attribute. */
+typedef struct { V3 P, Ng; float s, t; int attr; } Geo;
+
+static C3 mat_color(const orc_scene *s, const orc_material *m, int slot, const float *k, const Geo *g) {
+    if (m->shader[slot] >= 0) return shader_eval(s, m->shader[slot], g->P, g->Ng, g->s, g->t, g->attr);
+    return c3p(k);
+}
+static float mat_float(const orc_scene *s, const orc_material *m, int slot, float k, const Geo *g) {
+    if (m->shader[slot] >= 0) return shader_eval(s, m->shader[slot], g->P, g->Ng, g->s, g->t, g->attr).r;
+    return k;
+}
+
+/* Returns 0 for the constant (light) material, 1 otherwise.  Shader-driven
+ * parameters are read through ShadingEngine::getColor / getFloat. */
+static int get_brdf(const orc_scene *s, int attr, int prim, V3 vn, V3 nn, int matte_path, float u_lobe, const Geo *g,
+                    Bsdf *b) {
     const orc_material *m = &s->mats[attr];
-    C3 diffuse = c3p(m->diffuse);
-    if (s->tri_albedo) diffuse = c3p(s->tri_albedo + 3 * prim);
     switch (m->type) {
     case ORC_MAT_MATTE: /* matteMaterial.cpp:30-35 */
-        b->kind = ORC_BSDF_LAMBERT; b->color = diffuse; b->rough = 0.f; b->dist = 0;
+        b->kind = ORC_BSDF_LAMBERT; b->rough = 0.f; b->dist = 0;
+        b->color = s->tri_albedo ? c3p(s->tri_albedo + 3 * prim) : mat_color(s, m, ORC_SLOT_DIFFUSE, m->diffuse, g);
         return 1;
     case ORC_MAT_METAL: { /* metalMaterial.cpp:32-62 */
+        float mix = mat_float(s, m, ORC_SLOT_MIX, m->mix, g);
+        C3 refl = mat_color(s, m, ORC_SLOT_SPEC, m->spec, g);
+        C3 gain = mat_color(s, m, ORC_SLOT_GAIN, m->gain, g);
         float costheta = dot3(vn, nn);
-        C3 sc = c3(schlick(costheta, m->spec[0]), schlick(costheta, m->spec[1]), schlick(costheta, m->spec[2]));
-        float rough = (u_lobe > m->mix) ? m->roughness : m->roughnessB;
+        C3 sc = c3(schlick(costheta, refl.r), schlick(costheta, refl.g), schlick(costheta, refl.b));
+        float rough = (u_lobe > mix) ? mat_float(s, m, ORC_SLOT_ROUGH, m->roughness, g)
+                                     : mat_float(s, m, ORC_SLOT_ROUGHB, m->roughnessB, g);
         if (matte_path) rough = (float)(rough * (1. - CAUSTIC_BLUR) + CAUSTIC_BLUR);
-        b->kind = ORC_BSDF_CT; b->dist = m->distribution; b->color = cmulc(sc, c3p(m->gain)); b->rough = rough;
+        b->kind = ORC_BSDF_CT; b->dist = m->distribution; b->color = cmulc(sc, gain); b->rough = rough;
         return 1;
     }
     case ORC_MAT_KELEMEN: { /* kelemenMaterial.cpp:225-255 */
-        float rough = m->roughness;
+        C3 spec = mat_color(s, m, ORC_SLOT_SPEC, m->spec, g);
+        float rough = mat_float(s, m, ORC_SLOT_ROUGH, m->roughness, g);
         if (matte_path) rough = (float)(rough * (1. - CAUSTIC_BLUR) + CAUSTIC_BLUR);
         if (m->reflectance < 1.f) {
             float albedo = orc_kelemen_albedo(m->table, fclampf(dot3(nn, vn), 0.f, 1.f), rough);
             if (u_lobe > albedo) {
-                b->kind = ORC_BSDF_LAMBERT; b->color = diffuse; b->rough = 0.f; b->dist = 0;
+                b->kind = ORC_BSDF_LAMBERT; b->rough = 0.f; b->dist = 0;
+                b->color = s->tri_albedo ? c3p(s->tri_albedo + 3 * prim) : mat_color(s, m, ORC_SLOT_DIFFUSE, m->diffuse, g);
                 return 1;
             }
         }
-        b->kind = ORC_BSDF_CT; b->dist = m->distribution; b->color = c3p(m->spec); b->rough = rough;
+        b->kind = ORC_BSDF_CT; b->dist = m->distribution; b->color = spec; b->rough = rough;
         return 1;
     }
     default: /* constantMaterial.cpp:19-24: pdf-0 sample terminates the path */
@@ -705,7 +876,7 @@ static C3 integrate_direct(const orc_scene *s, const orc_bvh *bv, V3 P, V3 vn, V
     if (pick > n - 1) pick = n - 1;
     const orc_light *l = &s->lights[idx[pick]];
     if (b->kind == ORC_BSDF_LAMBERT) {
-        Sample3D ls = light_sample(l, P, nn, rng_u(rng, vslot + 2), rng_u(rng, vslot + 3));
+        Sample3D ls = light_sample(s, l, P, nn, rng_u(rng, vslot + 2), rng_u(rng, vslot + 3));
         float costheta = dot3(nn, v3p(ls.ray.d));
         if (costheta >= 0. && ls.pdf > 0.) {
             cnt->shadow++;
@@ -715,7 +886,7 @@ static C3 integrate_direct(const orc_scene *s, const orc_bvh *bv, V3 P, V3 vn, V
             }
         }
     } else {
-        Sample3D ls = light_sample(l, P, nn, rng_u(rng, vslot + 2), rng_u(rng, vslot + 3));
+        Sample3D ls = light_sample(s, l, P, nn, rng_u(rng, vslot + 2), rng_u(rng, vslot + 3));
         float costheta = dot3(nn, v3p(ls.ray.d));
         if (costheta > 0. && ls.pdf > 0.) {
             float bpdf = bsdf_pdf(b, v3p(ls.ray.d), vn, nn);
@@ -732,12 +903,12 @@ static C3 integrate_direct(const orc_scene *s, const orc_bvh *bv, V3 P, V3 vn, V
         bs.ray.o[0] = P.x; bs.ray.o[1] = P.y; bs.ray.o[2] = P.z;
         costheta = dot3(nn, v3p(bs.ray.d));
         if (costheta > 0.) {
-            float lpdf = light_pdf(l, &bs.ray);
+            float lpdf = light_pdf(s, l, &bs.ray);
             if (lpdf > 0. && bs.pdf > 0.) {
                 cnt->shadow++;
                 if (!occluded(s, bv, &bs.ray)) {
                     float w = power_heuristic(1, bs.pdf, 1, lpdf);
-                    res = cadd(res, cdivf(cmulf(cmulc(cmulf(bs.color, w), light_eval(l)), costheta), bs.pdf));
+                    res = cadd(res, cdivf(cmulf(cmulc(cmulf(bs.color, w), light_eval(s, l, v3p(bs.ray.d))), costheta), bs.pdf));
                 }
             }
         }
@@ -745,15 +916,33 @@ static C3 integrate_direct(const orc_scene *s, const orc_bvh *bv, V3 P, V3 vn, V
     return cmulf(res, (float)n);
 }
 
-/* embreeAccelerator.cpp:41-94: hit point and interpolated, face-forwarded normal */
-static void hit_geometry(const orc_scene *s, const orc_ray *r, const orc_hit *h, V3 *P, V3 *N) {
+/* embreeAccelerator.cpp:41-94: hit point, interpolated face-forwarded normal
+ * and interpolated uv */
+static void hit_geometry(const orc_scene *s, const orc_ray *r, const orc_hit *h, Geo *g) {
     float t = h->t, u = h->u, v = h->v;
     float w = (float)(1. - u - v);
-    *P = vadd(v3p(r->o), vmul(v3p(r->d), t));
+    g->P = vadd(v3p(r->o), vmul(v3p(r->d), t));
     const float *n = s->normals + 9 * h->prim;
     V3 nm = v3(w * n[0] + u * n[3] + v * n[6], w * n[1] + u * n[4] + v * n[7], w * n[2] + u * n[5] + v * n[8]);
     if (dot3(nm, v3p(r->d)) > 0.) nm = vmul(nm, -1.f);
-    *N = vnorm(nm);
+    g->Ng = vnorm(nm);
+    g->s = g->t = 0.f;
+    if (s->uvs) {
+        const float *q = s->uvs + 6 * h->prim;
+        g->s = w * q[0] + u * q[2] + v * q[4];
+        g->t = w * q[1] + u * q[3] + v * q[5];
+    }
+    g->attr = s->attr[h->prim];
+}
+
+/* Material::runNormalShader, material.cpp:19-33: the shading normal */
+static V3 shading_normal(const orc_scene *s, const Geo *g) {
+    const orc_material *m = &s->mats[g->attr];
+    if (m->shader[ORC_SLOT_NORMAL] < 0) return g->Ng;
+    C3 c = shader_eval(s, m->shader[ORC_SLOT_NORMAL], g->P, g->Ng, g->s, g->t, g->attr);
+    V3 ns = tangent_to_world(v3(c.r, c.g, c.b), g->Ng);
+    if (!(ns.x || ns.y || ns.z)) return g->Ng;
+    return vnorm(ns);
 }
 
 /* camera.cpp:34-62 */
@@ -782,7 +971,7 @@ static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
     if (h.prim < 0) {
         C3 L = c3(0.f, 0.f, 0.f); float alpha = 0.f;
         for (int i = 0; i < s->n_lights; ++i)
-            if (s->lights[i].type == ORC_LIGHT_ENV) { L = cadd(L, light_eval(&s->lights[i])); alpha = 1.f; }
+            if (s->lights[i].type == ORC_LIGHT_ENV) { L = cadd(L, light_eval(s, &s->lights[i], v3p(ray.d))); alpha = 1.f; }
         out[0] = L.r; out[1] = L.g; out[2] = L.b; out[3] = alpha;
         return;
     }
@@ -792,8 +981,9 @@ static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
     int matte_path = 0, ray_type = RAY_CAMERA, bounces = 0, true_bounces = 0;
     while (true_bounces < s->maxdepth) {
         int attr = s->attr[h.prim];
-        V3 P, Nn;
-        hit_geometry(s, &ray, &h, &P, &Nn);
+        Geo g;
+        hit_geometry(s, &ray, &h, &g);
+        V3 P = g.P, Nn = shading_normal(s, &g);
         /* integrator.cpp:132-150 */
         V3 vn = vneg(vnorm(v3p(ray.d)));
         if (dot3(vn, Nn) <= 0.) Nn = vnorm(cross3(vn, cross3(Nn, vn)));
@@ -801,7 +991,7 @@ static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
         if (ray_type == RAY_CAMERA) Lo = cadd(Lo, cmulc(T, c3p(s->mats[attr].emission)));
         int vslot = 2 + ORC_SLOTS_PER_VERTEX * true_bounces;
         Bsdf b;
-        int has_brdf = get_brdf(s, attr, h.prim, vn, Nn, matte_path, rng_u(rng, vslot + 0), &b);
+        int has_brdf = get_brdf(s, attr, h.prim, vn, Nn, matte_path, rng_u(rng, vslot + 0), &g, &b);
         if (!has_brdf) {
             /* the light-pick draw still happens (integrator.cpp:108-116); the
              * constant BRDF adds nothing and its sample has pdf 0 */
@@ -899,13 +1089,23 @@ void orc_camera_ray(const orc_scene *s, int x, int y, float j0, float j1, orc_ra
 
 void orc_light_sample(const orc_light *l, const float *P, const float *N, float x, float y, orc_ray *ray,
                       float *pdf, float *color) {
-    Sample3D s = light_sample(l, v3p(P), v3p(N), x, y);
+    Sample3D s = light_sample(NULL, l, v3p(P), v3p(N), x, y);
     *ray = s.ray;
     *pdf = s.pdf;
     color[0] = s.color.r; color[1] = s.color.g; color[2] = s.color.b;
 }
 
-float orc_light_pdf(const orc_light *l, orc_ray *ray) { return light_pdf(l, ray); }
+float orc_light_pdf(const orc_light *l, orc_ray *ray) { return light_pdf(NULL, l, ray); }
+void orc_light_sample_in(const orc_scene *sc, const orc_light *l, const float *P, const float *N, float x, float y, orc_ray *ray,
+                      float *pdf, float *color) {
+    Sample3D s = light_sample(sc, l, v3p(P), v3p(N), x, y);
+    *ray = s.ray;
+    *pdf = s.pdf;
+    color[0] = s.color.r; color[1] = s.color.g; color[2] = s.color.b;
+}
+
+float orc_light_pdf_in(const orc_scene *sc, const orc_light *l, orc_ray *ray) { return light_pdf(sc, l, ray); }
+
 
 int orc_light_visible(const orc_light *l, const float *P, const float *N) { return light_visible(l, v3p(P), v3p(N)); }
 
@@ -963,8 +1163,9 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
     int matte_path = 0, ray_type = RAY_CAMERA, bounces = 0, tb = 0;
     while (tb < s->maxdepth) {
         int attr = s->attr[h.prim], prim = h.prim;
-        V3 P, Nn;
-        hit_geometry(s, &ray, &h, &P, &Nn);
+        Geo hg;
+        hit_geometry(s, &ray, &h, &hg);
+        V3 P = hg.P, Nn = hg.Ng;
         V3 vn = vneg(vnorm(v3p(ray.d)));
         if (dot3(vn, Nn) <= 0.) Nn = vnorm(cross3(vn, cross3(Nn, vn)));
         if (ray_type == RAY_DIFFUSE) matte_path = 1;
@@ -977,7 +1178,7 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
         int vslot = 2 + ORC_SLOTS_PER_VERTEX * tb;
         Bsdf b;
         float ulobe = rng_u(rng, vslot + 0);
-        if (!get_brdf(s, attr, prim, vn, Nn, matte_path, ulobe, &b)) break;
+        if (!get_brdf(s, attr, prim, vn, Nn, matte_path, ulobe, &hg, &b)) break;
         int kel = m->type == ORC_MAT_KELEMEN, spec = b.kind == ORC_BSDF_CT;
         int alb = b.kind == ORC_BSDF_LAMBERT && s->tri_albedo != NULL;
         double drough = matte_path && (m->type == ORC_MAT_KELEMEN || m->type == ORC_MAT_METAL) ? (1. - CAUSTIC_BLUR) : 1.;
@@ -999,7 +1200,7 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
             int pick = (int)(rng_u(rng, vslot + 1) * n);
             if (pick > n - 1) pick = n - 1;
             const orc_light *l = &s->lights[idx[pick]];
-            Sample3D ls = light_sample(l, P, Nn, rng_u(rng, vslot + 2), rng_u(rng, vslot + 3));
+            Sample3D ls = light_sample(s, l, P, Nn, rng_u(rng, vslot + 2), rng_u(rng, vslot + 3));
             float ct = dot3(Nn, v3p(ls.ray.d));
             if (!spec) {
                 if (ct >= 0. && ls.pdf > 0. && !occluded(s, bv, &ls.ray)) {
@@ -1020,10 +1221,10 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
                 bs.ray.o[0] = P.x; bs.ray.o[1] = P.y; bs.ray.o[2] = P.z;
                 float ct2 = dot3(Nn, v3p(bs.ray.d));
                 if (ct2 > 0.) {
-                    float lpdf = light_pdf(l, &bs.ray);
+                    float lpdf = light_pdf(s, l, &bs.ray);
                     if (lpdf > 0. && bs.pdf > 0. && !occluded(s, bv, &bs.ray)) {
                         float w = power_heuristic(1, bs.pdf, 1, lpdf);
-                        C3 c = cdivf(cmulf(cmulc(cmulf(bs.color, w), light_eval(l)), ct2), bs.pdf);
+                        C3 c = cdivf(cmulf(cmulc(cmulf(bs.color, w), light_eval(s, l, v3p(bs.ray.d))), ct2), bs.pdf);
                         cb[0] = c.r; cb[1] = c.g; cb[2] = c.b;
                         dlb = dlogd_drough(b.dist, b.rough, dot3(Nn, vnorm(vadd(v3p(bs.ray.d), vn))));
                     }

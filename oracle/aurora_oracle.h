@@ -33,6 +33,11 @@ enum { ORC_DIST_BECKMANN = 0, ORC_DIST_GGX = 1 };
 enum { ORC_LIGHT_AREA = 0, ORC_LIGHT_ENV = 1 };
 enum { ORC_BSDF_LAMBERT = 0, ORC_BSDF_CT = 1 };
 
+enum { ORC_SLOT_DIFFUSE = 0, ORC_SLOT_SPEC = 1, ORC_SLOT_GAIN = 2, ORC_SLOT_ROUGH = 3, ORC_SLOT_ROUGHB = 4,
+       ORC_SLOT_MIX = 5, ORC_SLOT_NORMAL = 6, ORC_N_SLOTS = 7 };
+enum { ORC_SHADER_TEXTURE2D = 0, ORC_SHADER_TRIPLANAR = 1, ORC_SHADER_CONSTANT = 2 };
+enum { ORC_SHADER_COLOR = 0, ORC_SHADER_FLOAT = 1 };
+
 #define ORC_SLOTS_PER_VERTEX 9
 #define ORC_KELEMEN_COS 64
 #define ORC_KELEMEN_ROUGH 32
@@ -49,6 +54,8 @@ typedef struct {
     float reflectance;   /* kelemen scalar reflectance */
     float emission[3];   /* light triangles: color * 2^exposure */
     const float *table;  /* kelemen albedo table [64*32], index cos*32 + rough */
+    int shader[ORC_N_SLOTS]; /* shader driving each ORC_SLOT_*, -1 = the constant above */
+    float c2o[16];       /* camera-to-object matrix of the attribute's object (triplanar) */
 } orc_material;
 
 typedef struct {
@@ -62,7 +69,26 @@ typedef struct {
     float N[3];          /* camera-space normal */
     float pcam[4][3];    /* camera-space corners p1..p4 */
     float tri[2][9];     /* (p, e1, e2) of the two pdf() triangles */
+    int envmap;          /* env: texture index, -1 for the constant colour */
+    float c2w[16], w2c[16]; /* env: camera-to-world and world-to-camera */
 } orc_light;
+
+typedef struct {
+    int type;            /* ORC_SHADER_* */
+    int kind;            /* ORC_SHADER_COLOR / ORC_SHADER_FLOAT */
+    int texture;
+    float scale;         /* triplanar */
+    float value[3];      /* constant */
+} orc_shader;
+
+/* Texture2D: rgba is width*height*4 FP16-exact values (Imf::Rgba).  For an
+ * env map orc_texture_init_pdf rewrites alpha as initPdf does and sets
+ * pdf_norm. */
+typedef struct {
+    int width, height;
+    float *rgba;
+    float pdf_norm;
+} orc_texture;
 
 typedef struct {
     int n_verts, n_tris;
@@ -78,6 +104,11 @@ typedef struct {
     float fov;
     int width, height;
     int maxdepth, mindepth;
+    const float *uvs;      /* 6*n_tris per-corner uvs, or NULL (all zero) */
+    int n_shaders;
+    const orc_shader *shaders;
+    int n_textures;
+    const orc_texture *textures;
 } orc_scene;
 
 typedef struct {
@@ -106,6 +137,14 @@ void orc_trace_bvh(const orc_scene *s, const orc_bvh *b, const orc_ray *rays, in
 
 /* ---- lights ---- */
 void orc_light_init(orc_light *l);
+void orc_texture_init_pdf(orc_texture *t);
+
+/* ---- textures and shaders (single-routine checks) ---- */
+float orc_half_round(float x);
+void orc_texture_read(const orc_texture *t, float s, float tt, float *rgb_out);
+void orc_shader_eval(const orc_scene *sc, int shader, const float *P, const float *Ng, float s, float t, int attr,
+                     float *out);
+void orc_env_eval(const orc_scene *sc, const orc_light *l, const float *dir, float *rgb_out);
 
 /* ---- BSDF (query interface used by the parity tests) ---- */
 /* params: [0..2] color (albedo or spec), [3] roughness, [4] distribution */
@@ -148,6 +187,10 @@ void orc_camera_ray(const orc_scene *s, int x, int y, float j0, float j1, orc_ra
 void orc_light_sample(const orc_light *l, const float *P, const float *N, float x, float y, orc_ray *ray,
                       float *pdf, float *color);
 float orc_light_pdf(const orc_light *l, orc_ray *ray);
+/* the same for lights that read the scene's textures (env maps) */
+void orc_light_sample_in(const orc_scene *sc, const orc_light *l, const float *P, const float *N, float x, float y,
+                         orc_ray *ray, float *pdf, float *color);
+float orc_light_pdf_in(const orc_scene *sc, const orc_light *l, orc_ray *ray);
 int orc_light_visible(const orc_light *l, const float *P, const float *N);
 
 /* ---- adjoint (path-replay) ---- */

@@ -29,13 +29,24 @@ def build() -> str:
 class OMaterial(C.Structure):
     _fields_ = [("type", C.c_int), ("distribution", C.c_int), ("diffuse", C.c_float * 3), ("spec", C.c_float * 3),
                 ("gain", C.c_float * 3), ("roughness", C.c_float), ("roughnessB", C.c_float), ("mix", C.c_float),
-                ("reflectance", C.c_float), ("emission", C.c_float * 3), ("table", C.POINTER(C.c_float))]
+                ("reflectance", C.c_float), ("emission", C.c_float * 3), ("table", C.POINTER(C.c_float)),
+                ("shader", C.c_int * 7), ("c2o", C.c_float * 16)]
 
 
 class OLight(C.Structure):
     _fields_ = [("type", C.c_int), ("m", C.c_float * 16), ("sx", C.c_float), ("sy", C.c_float),
                 ("color", C.c_float * 3), ("exposure", C.c_float), ("intensity", C.c_float), ("N", C.c_float * 3),
-                ("pcam", (C.c_float * 3) * 4), ("tri", (C.c_float * 9) * 2)]
+                ("pcam", (C.c_float * 3) * 4), ("tri", (C.c_float * 9) * 2), ("envmap", C.c_int),
+                ("c2w", C.c_float * 16), ("w2c", C.c_float * 16)]
+
+
+class OShader(C.Structure):
+    _fields_ = [("type", C.c_int), ("kind", C.c_int), ("texture", C.c_int), ("scale", C.c_float),
+                ("value", C.c_float * 3)]
+
+
+class OTexture(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("rgba", C.POINTER(C.c_float)), ("pdf_norm", C.c_float)]
 
 
 class OScene(C.Structure):
@@ -43,7 +54,9 @@ class OScene(C.Structure):
                 ("tris", C.POINTER(C.c_int)), ("normals", C.POINTER(C.c_float)), ("attr", C.POINTER(C.c_int)),
                 ("tri_albedo", C.POINTER(C.c_float)), ("n_attrs", C.c_int), ("mats", C.POINTER(OMaterial)),
                 ("n_lights", C.c_int), ("lights", C.POINTER(OLight)), ("fov", C.c_float), ("width", C.c_int),
-                ("height", C.c_int), ("maxdepth", C.c_int), ("mindepth", C.c_int)]
+                ("height", C.c_int), ("maxdepth", C.c_int), ("mindepth", C.c_int), ("uvs", C.POINTER(C.c_float)),
+                ("n_shaders", C.c_int), ("shaders", C.POINTER(OShader)), ("n_textures", C.c_int),
+                ("textures", C.POINTER(OTexture))]
 
 
 class ORay(C.Structure):
@@ -80,6 +93,11 @@ def lib() -> C.CDLL:
         "orc_bvh_free": (None, [C.c_void_p]),
         "orc_trace_bvh": (None, [C.POINTER(OScene), C.c_void_p, C.POINTER(ORay), C.c_int, C.c_int, C.POINTER(OHit)]),
         "orc_light_init": (None, [C.POINTER(OLight)]),
+        "orc_texture_init_pdf": (None, [C.POINTER(OTexture)]),
+        "orc_half_round": (C.c_float, [C.c_float]),
+        "orc_texture_read": (None, [C.POINTER(OTexture), C.c_float, C.c_float, _FP]),
+        "orc_shader_eval": (None, [C.POINTER(OScene), C.c_int, _FP, _FP, C.c_float, C.c_float, C.c_int, _FP]),
+        "orc_env_eval": (None, [C.POINTER(OScene), C.POINTER(OLight), _FP, _FP]),
         "orc_bsdf_eval": (None, [C.c_int, _FP, _FP, _FP, _FP, _FP, _FP]),
         "orc_bsdf_sample": (None, [C.c_int, _FP, _FP, _FP, C.c_float, C.c_float, _FP, _FP, _FP]),
         "orc_kelemen_table": (None, [C.c_float, C.c_int, C.c_int, C.c_uint32, _FP]),
@@ -99,6 +117,9 @@ def lib() -> C.CDLL:
         "orc_camera_ray": (None, [C.POINTER(OScene), C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(ORay)]),
         "orc_light_sample": (None, [C.POINTER(OLight), _FP, _FP, C.c_float, C.c_float, C.POINTER(ORay), _FP, _FP]),
         "orc_light_pdf": (C.c_float, [C.POINTER(OLight), C.POINTER(ORay)]),
+        "orc_light_sample_in": (None, [C.POINTER(OScene), C.POINTER(OLight), _FP, _FP, C.c_float, C.c_float,
+                                       C.POINTER(ORay), _FP, _FP]),
+        "orc_light_pdf_in": (C.c_float, [C.POINTER(OScene), C.POINTER(OLight), C.POINTER(ORay)]),
         "orc_light_visible": (C.c_int, [C.POINTER(OLight), _FP, _FP]),
         "orc_adjoint": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, C.c_int, C.c_int, _FP, _FP, _FP, _FP, _FP,
                                C.c_int]),
@@ -151,12 +172,39 @@ class Scene:
             for j in range(3):
                 o.diffuse[j], o.spec[j], o.gain[j], o.emission[j] = m.diffuse[j], m.spec[j], m.gain[j], m.emission[j]
             o.roughness, o.roughnessB, o.mix, o.reflectance = m.roughness, m.roughnessB, m.mix, m.reflectance
+            for j in range(7):
+                o.shader[j] = getattr(m, "shader", [-1] * 7)[j]
+            for j in range(16):
+                o.c2o[j] = getattr(m, "c2o", np.eye(4, dtype=np.float32).reshape(-1))[j]
             if m.type == 2:
                 # roundf(): halves away from zero, like the reference (kelemenMaterial.cpp:50)
                 q = min(max(int(np.floor(float(np.float32(m.reflectance)) * 20.0 + 0.5)), 0), 20)
                 t = kelemen_table(np.float32(q / 20.0), m.distribution)
                 self.tables.append(t)
                 o.table = fp(t)
+        cam = np.asarray(exp.get("camera", np.concatenate([np.eye(4), np.eye(4)]).reshape(-1)), np.float32)
+        # textures: private copies (initPdf rewrites alpha); uvs; shaders
+        self.tex_rgba = []
+        texs = exp.get("textures") or []
+        self.textures = (OTexture * max(len(texs), 1))()
+        texels = np.asarray(exp.get("texels", np.zeros(0)), np.float32).reshape(-1)
+        for k, t in enumerate(texs):
+            n = t.width * t.height
+            rgba = np.ascontiguousarray(texels[4 * t.offset:4 * (t.offset + n)].copy())
+            self.tex_rgba.append(rgba)
+            self.textures[k].width, self.textures[k].height = t.width, t.height
+            self.textures[k].rgba = fp(rgba)
+            self.textures[k].pdf_norm = 1.0
+        shs = exp.get("shaders") or []
+        self.shaders = (OShader * max(len(shs), 1))()
+        for k, sh in enumerate(shs):
+            o = self.shaders[k]
+            o.type, o.kind, o.texture, o.scale = sh.type, sh.kind, sh.texture, sh.scale
+            for j in range(3):
+                o.value[j] = sh.value[j]
+        self.uvs = None
+        if exp.get("uvs") is not None:
+            self.uvs = np.ascontiguousarray(exp["uvs"], np.float32).reshape(-1)
         lights = exp["lights"]
         self.lights = (OLight * max(len(lights), 1))()
         for k, l in enumerate(lights):
@@ -167,7 +215,12 @@ class Scene:
             o.sx, o.sy, o.exposure = l.sx, l.sy, l.exposure
             for j in range(3):
                 o.color[j] = l.color[j]
+            o.envmap = getattr(l, "envmap", -1) if l.type == 1 else -1
+            for j in range(16):
+                o.c2w[j], o.w2c[j] = cam[j], cam[16 + j]
             lib().orc_light_init(C.byref(o))
+            if o.envmap >= 0:
+                lib().orc_texture_init_pdf(C.byref(self.textures[o.envmap]))
         s = OScene()
         s.n_verts = self.verts.size // 3
         s.n_tris = self.tris.size // 3
@@ -180,6 +233,9 @@ class Scene:
         s.fov = exp["fov"]
         s.width, s.height = exp["width"], exp["height"]
         s.maxdepth, s.mindepth = exp["maxdepth"], exp["mindepth"]
+        s.uvs = fp(self.uvs) if self.uvs is not None else None
+        s.n_shaders, s.shaders = len(shs), C.cast(self.shaders, C.POINTER(OShader))
+        s.n_textures, s.textures = len(texs), C.cast(self.textures, C.POINTER(OTexture))
         self.s = s
         self.bvh = lib().orc_bvh_build(C.byref(s), leaf) if bvh else None
 
@@ -189,6 +245,33 @@ class Scene:
                 lib().orc_bvh_free(self.bvh)
         except Exception:
             pass
+
+    def camera_rays(self, pixel, j0: float = 0.5, j1: float = 0.5) -> np.ndarray:
+        """Camera rays (camera.cpp:34-62) as (n, 8) rows like aur_ray."""
+        out = np.zeros((len(pixel), 8), np.float32)
+        r = ORay()
+        W = self.exp["width"]
+        for k, p in enumerate(pixel):
+            lib().orc_camera_ray(C.byref(self.s), int(p) % W, int(p) // W, j0, j1, C.byref(r))
+            out[k] = [r.o[0], r.o[1], r.o[2], r.tmin, r.d[0], r.d[1], r.d[2], r.tmax]
+        return out
+
+    # -- textures, shaders, env maps (single routines) --------------------
+    def texture_read(self, tex: int, s: float, t: float) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        lib().orc_texture_read(C.byref(self.textures[tex]), s, t, fp(out))
+        return out
+
+    def shader_eval(self, shader: int, P, Ng, s: float, t: float, attr: int) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        lib().orc_shader_eval(C.byref(self.s), shader, fp(np.asarray(P, np.float32)), fp(np.asarray(Ng, np.float32)),
+                              s, t, attr, fp(out))
+        return out
+
+    def env_eval(self, light: int, d) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        lib().orc_env_eval(C.byref(self.s), C.byref(self.lights[light]), fp(np.asarray(d, np.float32)), fp(out))
+        return out
 
     def trace(self, rays: np.ndarray, any_hit: bool = False, use_bvh: bool = False):
         """rays: (n, 8) rows (ox, oy, oz, tmin, dx, dy, dz, tmax) like aur_ray."""
@@ -307,21 +390,32 @@ def make_light(m16, sx, sy, color, exposure=0.0, kind=0) -> OLight:
     return o
 
 
-def light_sample(light: OLight, P, N, x, y):
+def half_round(x: float) -> float:
+    return float(lib().orc_half_round(float(x)))
+
+
+def light_sample(light: OLight, P, N, x, y, scene=None):
+    """generateSample; pass the oracle Scene for lights that read its textures."""
     ray = ORay()
     pdf = np.zeros(1, np.float32)
     col = np.zeros(3, np.float32)
-    lib().orc_light_sample(C.byref(light), fp(np.asarray(P, np.float32)), fp(np.asarray(N, np.float32)), x, y,
-                           C.byref(ray), fp(pdf), fp(col))
+    P, N = fp(np.asarray(P, np.float32)), fp(np.asarray(N, np.float32))
+    if scene is None:
+        lib().orc_light_sample(C.byref(light), P, N, x, y, C.byref(ray), fp(pdf), fp(col))
+    else:
+        lib().orc_light_sample_in(C.byref(scene.s), C.byref(light), P, N, x, y, C.byref(ray), fp(pdf), fp(col))
     return np.array(ray.d[:], np.float32), float(ray.tmax), float(pdf[0]), col
 
 
-def light_pdf(light: OLight, o, d, tmin=0.0005, tmax=1e7):
+def light_pdf(light: OLight, o, d, tmin=0.0005, tmax=1e7, scene=None):
     ray = ORay()
     for j in range(3):
         ray.o[j], ray.d[j] = o[j], d[j]
     ray.tmin, ray.tmax = tmin, tmax
-    p = lib().orc_light_pdf(C.byref(light), C.byref(ray))
+    if scene is None:
+        p = lib().orc_light_pdf(C.byref(light), C.byref(ray))
+    else:
+        p = lib().orc_light_pdf_in(C.byref(scene.s), C.byref(light), C.byref(ray))
     return float(p), float(ray.tmax)
 
 
