@@ -13,13 +13,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libaurora_hip.so")
 
-AUR_MAT_MATTE, AUR_MAT_METAL, AUR_MAT_KELEMEN, AUR_MAT_LIGHT = 0, 1, 2, 3
+AUR_MAT_MATTE, AUR_MAT_METAL, AUR_MAT_KELEMEN, AUR_MAT_LIGHT, AUR_MAT_GLASS = 0, 1, 2, 3, 4
 AUR_DIST_BECKMANN, AUR_DIST_GGX = 0, 1
 AUR_LIGHT_AREA, AUR_LIGHT_ENV = 0, 1
 AUR_SLOT_DIFFUSE, AUR_SLOT_SPEC, AUR_SLOT_GAIN, AUR_SLOT_ROUGH, AUR_SLOT_ROUGHB, AUR_SLOT_MIX, AUR_SLOT_NORMAL = range(7)
 AUR_N_SLOTS = 7
 AUR_SHADER_TEXTURE2D, AUR_SHADER_TRIPLANAR, AUR_SHADER_CONSTANT = 0, 1, 2
 AUR_SHADER_COLOR, AUR_SHADER_FLOAT = 0, 1
+AUR_SAMPLER_HASH, AUR_SAMPLER_HALTON = 0, 1
 
 ERROR_NAMES = {-1: "AUR_EINVAL", -2: "AUR_EIO", -3: "AUR_EPARSE", -4: "AUR_EHIP", -5: "AUR_ESTATE",
                -6: "AUR_ECAPACITY"}
@@ -43,7 +44,8 @@ class Material(C.Structure):
     _fields_ = [("type", C.c_int), ("distribution", C.c_int), ("diffuse", C.c_float * 3),
                 ("spec", C.c_float * 3), ("gain", C.c_float * 3), ("roughness", C.c_float),
                 ("roughnessB", C.c_float), ("mix", C.c_float), ("reflectance", C.c_float),
-                ("emission", C.c_float * 3), ("shader", C.c_int * AUR_N_SLOTS), ("c2o", C.c_float * 16)]
+                ("emission", C.c_float * 3), ("shader", C.c_int * AUR_N_SLOTS), ("c2o", C.c_float * 16),
+                ("transmit", C.c_float * 3), ("ior", C.c_float)]
 
     def __init__(self, *args, **kw):
         super().__init__(*args, **kw)
@@ -159,6 +161,7 @@ SIGNATURES = {
     "aur_last_error": (C.c_char_p, []),
     "aur_set_stream": (C.c_int, [_P, _P]),
     "aur_set_seed": (C.c_int, [_P, C.c_uint32]),
+    "aur_set_sampler": (C.c_int, [_P, C.c_int]),
     "aur_scene_load": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
     "aur_scene_upload": (C.c_int, [_P, C.POINTER(SceneDesc)]),
     "aur_set_tri_albedo": (C.c_int, [_P, _FP]),

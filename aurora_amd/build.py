@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaurora_hip.so")
 ARCH = os.environ.get("AURORA_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "host_scene.cpp", "exr_io.cpp", "bvh_build.hip", "wavefront.hip", "parity.hip", "subdiv.hip", "adjoint.hip", "batch.hip"]
+SOURCES = ["capi.cpp", "host_scene.cpp", "exr_io.cpp", "halton.cpp", "bvh_build.hip", "wavefront.hip", "parity.hip", "subdiv.hip", "adjoint.hip", "batch.hip"]
 EXTRA_SOURCES_FILE = os.path.join(CSRC, "sources.txt")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

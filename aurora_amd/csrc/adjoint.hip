@@ -65,7 +65,7 @@ __device__ inline bool occluded(const WNode<W> *nodes, const float4 *tris, f3 o,
 // number of 0.2 clamp steps; with GRAD it accumulates gradients given delta
 // (incoming dL/dL_path, clamp factor included) and the path's radiance.
 template <int W, bool GRAD>
-__device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st, int pixel, const PathRng &rng,
+__device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st, int pixel, const PathRngT<false> &rng,
                        f3 *L_out, int *clamp_steps, bool *camera_miss, f3 delta, f3 L_total, GradSink &g,
                        unsigned long long *n_ext, unsigned long long *n_shadow) {
     const int x = pixel % sv.width, y = pixel / sv.width;
@@ -235,12 +235,11 @@ __device__ inline void adj_count(unsigned long long *stats, unsigned paths, unsi
     }
 }
 
-__device__ inline void adj_path(const SceneView &sv, const AdjParams &ap, int i, int *pixel, PathRng *rng) {
+__device__ inline void adj_path(const SceneView &sv, const AdjParams &ap, int i, int *pixel, PathRngT<false> *rng) {
     const int wh = sv.width * sv.height;
     const int pass = i / wh;
     *pixel = tiled_pixel(i - pass * wh, sv.width, sv.height);
-    rng->slots = nullptr;
-    rng->key = path_key(ap.seed, (uint32_t)*pixel, (uint32_t)(ap.first_pass + pass));
+    *rng = make_path_rng<false>(sv, nullptr, ap.seed, *pixel, (uint32_t)(ap.first_pass + pass));
 }
 
 // First replay of every path: its radiance and clamp steps (two kernels keep
@@ -259,7 +258,7 @@ __global__ void __launch_bounds__(128) k_adj_forward(SceneView sv, AdjParams ap,
     unsigned long long n_ext = 0, n_shadow = 0, n_paths = 0;
     for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
         int pixel;
-        PathRng rng;
+        PathRngT<false> rng;
         adj_path(sv, ap, i, &pixel, &rng);
         f3 L;
         int k;
@@ -304,7 +303,7 @@ __global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, co
         const int kw = __float_as_int(r.w);
         if (kw & 0x10000) continue;  // camera miss: no vertex depends on the parameters
         int pixel;
-        PathRng rng;
+        PathRngT<false> rng;
         adj_path(sv, ap, i, &pixel, &rng);
         const float4 dl = ap.dldi[pixel];
         float cf = ap.inv_passes;

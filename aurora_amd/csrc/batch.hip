@@ -142,9 +142,7 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
         const int pass = (int)(local / wh);
         const int l = (int)(local - (long long)pass * wh);
         const int pixel = tiled_pixel(l, jd.sv.width, jd.sv.height);
-        PathRng rng;
-        rng.slots = nullptr;
-        rng.key = path_key(jd.seed, (uint32_t)pixel, (uint32_t)(jd.first_pass + pass));
+        const PathRng rng = make_path_rng(jd.sv, nullptr, jd.seed, pixel, (uint32_t)(jd.first_pass + pass));
         const f3 d = camera_dir(jd.sv, pixel, rng.u(0), rng.u(1));
         const int pos = live + i;
         wv.ray[q][2 * pos] = make_float4(0.f, 0.f, 0.f, 0.1f);
@@ -214,11 +212,9 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__res
                 if (job == j) {
                     const JobDev &jd = jobs[j];
                     shade_compute<TEX>(jd.sv, wv, q, i, [&](int4 m) {
-                        PathRng r;
-                        r.slots = nullptr;
-                        r.key = path_key(jd.seed, (uint32_t)tiled_pixel(m.y, jd.sv.width, jd.sv.height),
-                                         (uint32_t)(jd.first_pass + m.z));
-                        return r;
+                        return make_path_rng<TEX>(jd.sv, nullptr, jd.seed,
+                                                  tiled_pixel(m.y, jd.sv.width, jd.sv.height),
+                                                  (uint32_t)(jd.first_pass + m.z));
                     }, j, o);
                     break;
                 }

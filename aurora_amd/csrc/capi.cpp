@@ -129,6 +129,33 @@ static float env_pdf_alpha(HostTexture &t) {
     return lum / (float)(t.width * t.height);
 }
 
+// The Halton source's tables (once per context) and its enumeration for the
+// scene's resolution (device memory, read through SceneView::qmc).
+static void ensure_halton(Ctx *c) {
+    if (c->sampler != AUR_SAMPLER_HALTON || c->device < 0) return;
+    if (!c->d_halton_dims.p) {
+        std::vector<DHaltonDim> dims;
+        std::vector<uint16_t> tables;
+        halton_tables(&dims, &tables);
+        c->d_halton_dims.reserve(sizeof(DHaltonDim) * dims.size());
+        c->d_halton_tables.reserve(sizeof(uint16_t) * tables.size());
+        HIP_CHECK(hipMemcpy(c->d_halton_dims.p, dims.data(), sizeof(DHaltonDim) * dims.size(), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(c->d_halton_tables.p, tables.data(), sizeof(uint16_t) * tables.size(), hipMemcpyHostToDevice));
+        c->halton_w = c->halton_h = 0;
+    }
+    const int w = c->scene.opt.width, h = c->scene.opt.height;
+    if (w == c->halton_w && h == c->halton_h && c->d_halton_view.p) return;
+    HaltonView v;
+    halton_enum(w, h, &v);
+    v.dims = c->d_halton_dims.as<DHaltonDim>();
+    v.tables = c->d_halton_tables.as<uint16_t>();
+    c->d_halton_view.reserve(sizeof(HaltonView));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipMemcpy(c->d_halton_view.p, &v, sizeof v, hipMemcpyHostToDevice));
+    c->halton_w = w;
+    c->halton_h = h;
+}
+
 static void upload_shading(Ctx *c) {
     HostScene &s = c->scene;
     const int nt = (int)(s.tris.size() / 3);
@@ -283,6 +310,8 @@ static void upload(Ctx *c) {
         d.roughB = m.roughnessB;
         d.mix = m.mix;
         d.refl = m.reflectance;
+        for (int k = 0; k < 3; ++k) d.transmit[k] = m.transmit[k];
+        d.transmit[3] = m.ior;
         d.table = 0;
         if (m.type == AUR_MAT_KELEMEN) {
             int q = (int)std::lround(m.reflectance * 20.f);
@@ -321,6 +350,7 @@ static void upload(Ctx *c) {
         c->dlights.push_back(d);
     }
     upload_shading(c);  // sets DMat::texmask and the env maps' transforms
+    ensure_halton(c);
     if (!c->dmats.empty())
         HIP_CHECK(hipMemcpyAsync(c->d_mats.p, c->dmats.data(), sizeof(DMat) * c->dmats.size(), hipMemcpyHostToDevice, c->stream));
     c->n_lights = (int)c->dlights.size();
@@ -554,7 +584,7 @@ int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
         for (int a : s.attr)
             if (a < 0 || a >= d->n_attrs) throw std::invalid_argument("attribute index out of range");
         for (const aur_material &m : s.attr_mats)
-            if (m.type < AUR_MAT_MATTE || m.type > AUR_MAT_LIGHT) throw std::invalid_argument("unknown material type");
+            if (m.type < AUR_MAT_MATTE || m.type > AUR_MAT_GLASS) throw std::invalid_argument("unknown material type");
         for (const aur_light &l : s.lights)
             if (l.type != AUR_LIGHT_AREA && l.type != AUR_LIGHT_ENV) throw std::invalid_argument("unknown light type");
         s.flat = true;
@@ -710,6 +740,19 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
             HIP_CHECK(hipStreamSynchronize(c->stream));
         }
         if (out) *out = st;
+    });
+}
+
+int aur_set_sampler(aur_ctx *ctx, int sampler) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (sampler != AUR_SAMPLER_HASH && sampler != AUR_SAMPLER_HALTON) throw std::invalid_argument("unknown sampler");
+        c->sampler = sampler;
+        if (c->built) {
+            set_device(c);
+            ensure_halton(c);
+        }
     });
 }
 
@@ -939,8 +982,11 @@ int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_d
         if (first_pass < 0 || n_passes <= 0) throw std::invalid_argument("bad pass range");
         if (out->albedo_tri && c->tri_albedo.empty())
             throw StateError("albedo gradients need a per-triangle albedo (aur_set_tri_albedo)");
-        for (const DMat &m : c->dmats)
+        for (const DMat &m : c->dmats) {
             if (m.texmask) throw StateError("the adjoint does not differentiate shader-driven (textured) materials");
+            if (m.type == MAT_GLASS) throw StateError("the adjoint does not support glass (specular) materials");
+        }
+        if (c->sampler != AUR_SAMPLER_HASH) throw StateError("the adjoint replays hash-sampled paths only");
         for (const DLight &l : c->dlights)
             if (l.tex >= 0) throw StateError("the adjoint does not support environment maps");
         set_device(c);
@@ -1022,6 +1068,8 @@ int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r) {
         if (n < 0) throw std::invalid_argument("negative query count");
         if (n == 0) return;
         NEED(q); NEED(r);
+        for (int i = 0; i < n; ++i)
+            if (q[i].kind < BSDF_LAMBERT || q[i].kind > BSDF_TRANSMIT) throw std::invalid_argument("unknown BSDF kind");
         set_device(c);
         DevBuf dq, drs;
         dq.reserve(sizeof(aur_bsdf_query) * n);

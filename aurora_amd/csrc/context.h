@@ -12,6 +12,7 @@
 
 #include "../../include/aurora_hip.h"
 #include "dev_common.h"
+#include "halton.h"
 #include "host_scene.h"
 
 namespace aur {
@@ -150,6 +151,9 @@ struct Ctx {
     // device scene
     DevBuf d_verts, d_tris, d_shade, d_mats, d_lights, d_tables, d_albedo;
     DevBuf d_uv, d_attrx, d_shaders, d_texs, d_texels;  // shading inputs (empty without shaders)
+    int sampler = 0;                                      // AUR_SAMPLER_*
+    DevBuf d_halton_dims, d_halton_tables, d_halton_view; // Halton source (built on first use)
+    int halton_w = 0, halton_h = 0;
     bool has_uv = false, has_attrx = false;
     int n_verts = 0, n_tris = 0, n_mats = 0, n_lights = 0;
     float env_le[3] = {0, 0, 0};

@@ -30,10 +30,12 @@ namespace aur {
 #define AUR_KEL_COS 64
 #define AUR_KEL_ROUGH 32
 
-enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_KELEMEN = 2, MAT_LIGHT = 3 };
+enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_KELEMEN = 2, MAT_LIGHT = 3, MAT_GLASS = 4 };
 enum { DIST_BECKMANN = 0, DIST_GGX = 1 };
 enum { LIGHT_AREA = 0, LIGHT_ENV = 1 };
-enum { BSDF_LAMBERT = 0, BSDF_CT = 1 };
+// BSDF_MIRROR / BSDF_TRANSMIT are the reference's MirrorBrdf type (SpecMirror,
+// SpecTransmit): a delta lobe, no direct lighting, pdf 1
+enum { BSDF_LAMBERT = 0, BSDF_CT = 1, BSDF_MIRROR = 2, BSDF_TRANSMIT = 3 };
 
 // ---------------------------------------------------------------------------
 // device records
@@ -52,6 +54,7 @@ struct alignas(16) DMat {
     float gain[4];
     float rough, roughB, mix, refl;
     float emission[4];
+    float transmit[4];    // glass: transmitColor, ior in [3]
 };
 
 // Per-attribute shading extras, read only when DMat::texmask is non-zero.
@@ -142,8 +145,8 @@ __device__ inline float schlick(float vdoth, float refl) {
     float a2 = a * a;
     return refl + (a2 * a2 * a) * (1.f - refl);
 }
-// color.h lum() / isBlack()
-__device__ inline float lum3(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+// color.h lum() (double constants, float result) / isBlack()
+__device__ inline float lum3(f3 c) { return (float)(0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z); }
 __device__ inline bool black3(f3 c) { return c.x < AUR_ISBLACK && c.y < AUR_ISBLACK && c.z < AUR_ISBLACK; }
 // color.h operator/(float): multiplies by 1/f
 __device__ inline f3 div3(f3 a, float f) { float d = 1.f / f; return mul3(a, d); }
@@ -264,6 +267,46 @@ __device__ inline BsdfSample bsdf_sample(const Bsdf &b, f3 vn, f3 nn, float r1, 
         if (pdf > 100000.f) { pdf *= 0.01f; res = mul3(res, 0.01f); }
     }
     s.dir = ln; s.pdf = pdf; s.color = res; s.tmax = 10000000.f;
+    return s;
+}
+
+// math_utils.h:91-95: the tangent frame of coordinateSystem, not normalised
+__device__ inline f3 world_to_tangent(f3 dir, f3 n) {
+    f3 t, b;
+    coordinate_system(n, &t, &b);
+    return mk3(dot3(dir, t), dot3(dir, b), dot3(dir, n));
+}
+
+// specMirror.cpp:30-35 and specTransmit.cpp:30-58 (b.rough holds the ior).
+// The reflected / refracted tangent-space vector goes back through
+// tangentToWorld, whose frame is normalised while worldToTangent's is not.
+__device__ inline BsdfSample delta_sample(const Bsdf &b, f3 vn, f3 nn) {
+    BsdfSample s;
+    s.color = b.color;
+    s.pdf = 1.f;
+    const f3 vt = world_to_tangent(vn, nn);
+    if (b.kind == BSDF_MIRROR) {
+        s.dir = norm3(tangent_to_world(mk3(-vt.x, -vt.y, vt.z), nn));
+        s.tmax = 100000000.f;
+        return s;
+    }
+    const float costheta = dot3(nn, vn);
+    float ei = 1.f, et = b.rough;
+    if (costheta < 0.f) { ei = b.rough; et = 1.f; }
+    const float sini2 = fmaxf(0.f, 1.f - costheta * costheta);
+    const float eta = ei / et;
+    const float sint2 = eta * eta * sini2;
+    if (sint2 >= 1.f) {
+        s.pdf = 0.f;
+        s.color = mk3(0.f, 0.f, 0.f);
+        s.dir = mk3(0.f, 0.f, 0.f);
+        s.tmax = 0.f;
+        return s;
+    }
+    float cost = sqrtf(fmaxf(0.f, 1.f - sint2));
+    if (costheta > 0.f) cost = -cost;
+    s.dir = norm3(tangent_to_world(mk3(eta * -vt.x, eta * -vt.y, cost), nn));
+    s.tmax = 10000000.f;
     return s;
 }
 

@@ -369,6 +369,13 @@ static aur_material parse_material(const JsonValue &root, const ShaderNames &sn,
         m.shader[AUR_SLOT_MIX] = float_slot(root, "mix", &m.mix, sn, sh);
         m.shader[AUR_SLOT_GAIN] = color_slot(root, "gain", m.gain, sn, sh);
         m.distribution = distribution_attr(root);
+    } else if (type == "glassMaterial") {
+        // jsonParser.cpp:412-419: constants only (the attributes are read with getColor / getFloat)
+        m.type = AUR_MAT_GLASS;
+        color_attr(root, "specColor", m.spec);
+        m.reflectance = float_attr(root, "reflectance");
+        m.ior = float_attr(root, "ior");
+        color_attr(root, "transmitColor", m.transmit);
     } else {
         throw std::runtime_error("unsupported material type: " + type);
     }

@@ -15,9 +15,11 @@ __global__ void k_bsdf(const aur_bsdf_query *__restrict__ q, int n, aur_bsdf_res
     b.color = mk3(a.color[0], a.color[1], a.color[2]);
     b.rough = a.roughness;
     f3 V = mk3(a.V[0], a.V[1], a.V[2]), N = mk3(a.N[0], a.N[1], a.N[2]), L = mk3(a.L[0], a.L[1], a.L[2]);
-    f3 f = bsdf_eval(b, L, V, N);
-    float pdf = bsdf_pdf(b, L, V, N);
-    BsdfSample s = bsdf_sample(b, V, N, a.u1, a.u2);
+    // SpecMirror / SpecTransmit: eval and pdf are zero, the sample is the delta direction
+    const bool delta = b.kind >= BSDF_MIRROR;
+    f3 f = delta ? mk3(0.f, 0.f, 0.f) : bsdf_eval(b, L, V, N);
+    float pdf = delta ? 0.f : bsdf_pdf(b, L, V, N);
+    BsdfSample s = delta ? delta_sample(b, V, N) : bsdf_sample(b, V, N, a.u1, a.u2);
     aur_bsdf_result o;
     o.f[0] = f.x; o.f[1] = f.y; o.f[2] = f.z;
     o.pdf = pdf;

@@ -4,6 +4,7 @@
 
 #include "context.h"
 #include "dev_common.h"
+#include "halton.h"
 
 namespace aur {
 
@@ -20,6 +21,7 @@ struct SceneView {
     const DShader *shaders;
     const DTex *texs;
     const ushort4 *texels;  // FP16 RGBA
+    const HaltonView *qmc;  // Halton sample source (device memory), or null
     int n_lights;
     int has_env;
     float env[3];
@@ -221,11 +223,74 @@ __device__ inline f3 env_miss(const SceneView &sv, f3 d) {
     return L;
 }
 
-struct PathRng {
+// ---------------------------------------------------------------------------
+// Halton QMC source (halton_sampler.h / halton_enum.h of Aurora/external/halton)
+// ---------------------------------------------------------------------------
+__device__ inline float halton_sample(const HaltonView &h, uint32_t dim, uint32_t index) {
+    if (dim == 0) return __uint_as_float(0x3f800000u | (__brev(index) >> 9)) - 1.f;
+    const DHaltonDim D = h.dims[dim];
+    uint32_t sum = 0, p = D.top, r = index;
+    for (uint32_t k = 0; k < D.n_blocks; ++k) {
+        const uint32_t q = r / D.block;
+        sum += (uint32_t)h.tables[D.off + (r - q * D.block)] * p;
+        r = q;
+        p /= D.block;
+    }
+    return (float)sum * D.scale;
+}
+
+// Halton_enum::get_index: the sample-th point of pixel (x, y)
+__device__ inline uint32_t halton_index(const HaltonView &h, int pixel, uint32_t sample) {
+    const uint32_t x = (uint32_t)(pixel % h.width), y = (uint32_t)(pixel / h.width);
+    const uint32_t hx = h.p2 ? (__brev(x) >> (32 - h.p2)) : 0u;
+    uint32_t hy = 0, r = y;
+    for (uint32_t d = 0; d < h.p3; ++d) {
+        hy = hy * 3 + r % 3;
+        r /= 3;
+    }
+    const uint32_t off = (uint32_t)(((unsigned long long)hx * h.mx + (unsigned long long)hy * h.my) % h.inc);
+    return off + sample * h.inc;
+}
+
+// The random numbers of one path: injected, Halton, or the counter-based hash.
+// QMC = false compiles the Halton branch out (kernels that never see it).
+template <bool QMC>
+struct PathRngT {
     const float *slots;
     uint32_t key;
-    __device__ inline float u(int slot) const { return slots ? slots[slot] : slot_uniform(key, (uint32_t)slot); }
+    int pixel;                // the path's pixel (for the camera direction)
+    const HaltonView *qmc;    // Halton source, or null
+    uint32_t hidx;            // the path's Halton index
+    __device__ inline float u(int slot) const {
+        if (slots) return slots[slot];
+        if constexpr (QMC) {
+            if (qmc && slot < AUR_HALTON_DIMS) {
+                const float v = halton_sample(*qmc, (uint32_t)slot, hidx);
+                // camera.cpp:39-45: the first two dimensions scaled to the
+                // image, here as offsets inside the pixel
+                if (slot == 0) return v * qmc->sx - (float)(pixel % qmc->width);
+                if (slot == 1) return v * qmc->sy - (float)(pixel / qmc->width);
+                return v;
+            }
+        }
+        return slot_uniform(key, (uint32_t)slot);
+    }
 };
+typedef PathRngT<true> PathRng;
+
+// The path (pixel, sample) of a scene: injected slots when given, else the
+// scene's Halton source when set (QMC), else the hash of (seed, pixel, sample).
+template <bool QMC = true>
+__device__ inline PathRngT<QMC> make_path_rng(const SceneView &sv, const float *slots, uint32_t seed, int pixel,
+                                              uint32_t sample) {
+    PathRngT<QMC> r;
+    r.slots = slots;
+    r.key = path_key(seed, (uint32_t)pixel, sample);
+    r.pixel = pixel;
+    r.qmc = QMC ? sv.qmc : nullptr;
+    r.hidx = (QMC && sv.qmc) ? halton_index(*sv.qmc, pixel, sample) : 0u;
+    return r;
+}
 
 // Material::getBrdf: matteMaterial.cpp:30-35, metalMaterial.cpp:32-62,
 // kelemenMaterial.cpp:225-255.  Reports the Kelemen selection probability and
@@ -259,6 +324,18 @@ __device__ inline BrdfChoice get_brdf_ex(const SceneView &sv, const DMat &m, int
         float rough = (u_lobe > mix) ? mat_float<TEX>(sv, m, SLOT_ROUGH, m.rough, g) : mat_float<TEX>(sv, m, SLOT_ROUGHB, m.roughB, g);
         if (matte) { rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR; ch.drough = 1.f - AUR_CAUSTIC_BLUR; }
         b.kind = BSDF_CT; b.dist = m.dist; b.color = mul3v(sc, gain); b.rough = rough;
+    } else if (m.type == MAT_GLASS) {
+        // glassMaterial.cpp:33-58: Fresnel of half the cosine picks reflection
+        // or transmission; b.rough carries the ior
+        b.dist = 0; b.rough = m.transmit[3];
+        bool refl = true;
+        if (m.refl != 1.f) {
+            float ct = dot3(vn, nn) * 0.5f;
+            if (ct < 0.f) ct = -ct;
+            refl = !(u_lobe > schlick(ct, m.refl));
+        }
+        if (refl) { b.kind = BSDF_MIRROR; b.color = mk3(m.spec[0], m.spec[1], m.spec[2]); }
+        else { b.kind = BSDF_TRANSMIT; b.color = mk3(m.transmit[0], m.transmit[1], m.transmit[2]); }
     } else {
         float rough = mat_float<TEX>(sv, m, SLOT_ROUGH, m.rough, g);
         if (matte) { rough = rough * (1.f - AUR_CAUSTIC_BLUR) + AUR_CAUSTIC_BLUR; ch.drough = 1.f - AUR_CAUSTIC_BLUR; }

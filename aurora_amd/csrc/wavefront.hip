@@ -35,13 +35,12 @@ __device__ inline void path_pixel_sample(const SceneView &sv, const WfView &wv, 
     }
 }
 
-__device__ inline PathRng path_rng(const SceneView &sv, const WfView &wv, int path) {
-    PathRng r;
+template <bool QMC = true>
+__device__ inline PathRngT<QMC> path_rng(const SceneView &sv, const WfView &wv, int path) {
     int pixel, sample;
     path_pixel_sample(sv, wv, path, &pixel, &sample);
-    r.slots = wv.src.slots ? wv.src.slots + (size_t)path * wv.nslots : nullptr;
-    r.key = path_key(wv.seed, (uint32_t)pixel, (uint32_t)sample);
-    return r;
+    return make_path_rng<QMC>(sv, wv.src.slots ? wv.src.slots + (size_t)path * wv.nslots : nullptr, wv.seed, pixel,
+                              (uint32_t)sample);
 }
 
 // ---------------------------------------------------------------------------
@@ -135,7 +134,7 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView
         ShadeOut o;
         o.cont = o.finish = false;
         o.sh[0].valid = o.sh[1].valid = false;
-        if (i < n) shade_compute<TEX>(sv, wv, q, i, [&](int4 m) { return path_rng(sv, wv, m.y); }, 0, o);
+        if (i < n) shade_compute<TEX>(sv, wv, q, i, [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); }, 0, o);
         shade_emit(wv, q, i < n, o, es);
     }
 }
@@ -271,7 +270,7 @@ void timer_release(Ctx *c) {
 // host driver
 // ---------------------------------------------------------------------------
 bool shade_needs_tex(const Ctx *c) {
-    if (c->has_attrx) return true;
+    if (c->has_attrx || c->sampler != AUR_SAMPLER_HASH) return true;
     for (const DLight &l : c->dlights)
         if (l.type == LIGHT_ENV && l.tex >= 0) return true;
     return false;
@@ -291,6 +290,7 @@ SceneView scene_view(Ctx *c) {
     sv.shaders = c->d_shaders.as<DShader>();
     sv.texs = c->d_texs.as<DTex>();
     sv.texels = c->d_texels.as<ushort4>();
+    sv.qmc = (c->sampler == AUR_SAMPLER_HALTON && c->d_halton_view.p) ? c->d_halton_view.as<HaltonView>() : nullptr;
     sv.n_lights = c->n_lights;
     sv.has_env = c->n_env > 0;
     sv.env[0] = c->env_le[0]; sv.env[1] = c->env_le[1]; sv.env[2] = c->env_le[2];

@@ -29,7 +29,7 @@ namespace aur {
 int trace_blocks(const Ctx *c);
 size_t trace_threads(const Ctx *c);
 
-enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2 };
+enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2, RT_MIRROR = 3 };
 // Path flags: bits 0-1 ray type, 2 matte path, 3 last ray missed, 8-15
 // vertex index, 16-23 bounces, 24 no firefly clamp, 25-30 shadow rays traced.
 enum { F_MATTE = 1 << 2, F_MISS = 1 << 3, F_NOCLAMP = 1 << 24 };
@@ -163,8 +163,8 @@ struct ShadeOut {
     int job;
 };
 
-// TEX selects the code for shaders, normal maps and environment maps; the
-// host uses it only for scenes that have any.
+// TEX selects the code for shaders, normal maps, environment maps and the
+// Halton sample source; the host uses it only for scenes that need any.
 template <bool TEX, class MakeRng>
 __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job,
                                      ShadeOut &out) {
@@ -191,11 +191,26 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
             if constexpr (TEX) e = sv.has_env ? env_miss(sv, mk3(rb.x, rb.y, rb.z)) : mk3(0.f, 0.f, 0.f);
             La = make_float4(e.x, e.y, e.z, sv.has_env ? 1.f : 0.f);
             flags |= F_NOCLAMP;
+        } else if (rtype == RT_MIRROR && sv.has_env) {
+            // integrator.cpp:262-271: a mirror ray that misses adds every env
+            // light's eval() in the direction of the path's camera ray
+            f3 cd = mk3(0.f, 0.f, 0.f);
+            if constexpr (TEX) {
+                const auto rng = make_rng(meta);
+                cd = camera_dir(sv, rng.pixel, rng.u(0), rng.u(1));
+            }
+            for (int li = 0; li < sv.n_lights; ++li) {
+                const DLight &l = sv.lights[li];
+                if (l.type != LIGHT_ENV) continue;
+                f3 e = mk3(l.Le[0], l.Le[1], l.Le[2]);
+                if constexpr (TEX) e = env_eval(sv, l, cd);
+                La.x += e.x * T4.x; La.y += e.y * T4.y; La.z += e.z * T4.z;
+            }
         }
     } else if (tb >= sv.maxdepth) {
         finish = true;  // the extension ray after the last vertex is traced but not shaded
     } else {
-        const PathRng rng = make_rng(meta);
+        const auto rng = make_rng(meta);
         const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
         // hit record: embreeAccelerator.cpp:111-136
         const f3 o = mk3(ra.x, ra.y, ra.z), d = mk3(rb.x, rb.y, rb.z);
@@ -225,19 +240,19 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
         if (rtype == RT_DIFFUSE) matte = true;
         f3 T = mk3(T4.x, T4.y, T4.z);
         float cp = T4.w;
-        if (rtype == RT_CAMERA) {
+        if (rtype == RT_CAMERA || rtype == RT_MIRROR) {  // integrator.cpp:190-193
             float4 &la = La;
             la.x += T.x * m.emission[0]; la.y += T.y * m.emission[1]; la.z += T.z * m.emission[2];
             la.w = 1.f;
-            
         }
         if (m.type == MAT_LIGHT) {
             finish = true;  // ConstantMaterial: getSample has pdf 0 (constant.cpp:23-25)
         } else {
             Bsdf b = get_brdf<TEX>(sv, m, prim, vn, nn, matte, rng.u(vslot + 0), g);
-            // integrateDirect, integrator.cpp:100-130
+            const bool delta = b.kind >= BSDF_MIRROR;
+            // integrateDirect, integrator.cpp:100-130 (nothing for MirrorBrdf)
             int nvis = 0;
-            const int li = pick_visible_light(sv.lights, sv.n_lights, P, nn, rng.u(vslot + 1), &nvis);
+            const int li = delta ? -1 : pick_visible_light(sv.lights, sv.n_lights, P, nn, rng.u(vslot + 1), &nvis);
             if (li >= 0) {
                 const DLight &l = sv.lights[li];
                 const float scale = (float)nvis;
@@ -281,18 +296,26 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
                 }
             }
             // continuation, integrator.cpp:208-257
-            BsdfSample s = bsdf_sample(b, vn, nn, rng.u(vslot + 6), rng.u(vslot + 7));
+            BsdfSample s = delta ? delta_sample(b, vn, nn) : bsdf_sample(b, vn, nn, rng.u(vslot + 6), rng.u(vslot + 7));
             if (s.pdf <= 0.f) {
                 finish = true;
             } else {
-                float ndl = dot3(nn, s.dir);
-                T = mul3v(T, div3(mul3(s.color, ndl), s.pdf));
                 const int bounces = (flags >> 16) & 0xff;
-                if (bounces > sv.mindepth) cp *= lum3(s.color) * ndl / s.pdf;
-                if (black3(T)) {
+                bool black = false;
+                if (!delta) {
+                    float ndl = dot3(nn, s.dir);
+                    T = mul3v(T, div3(mul3(s.color, ndl), s.pdf));
+                    if (bounces > sv.mindepth) cp *= lum3(s.color) * ndl / s.pdf;
+                    black = black3(T);
+                } else {
+                    // integrator.cpp:231-237: no cosine, no pdf, no black test
+                    T = mul3v(T, s.color);
+                    if (bounces > sv.mindepth) cp *= lum3(s.color);
+                }
+                if (black) {
                     finish = true;
                 } else {
-                    int nrt = (b.kind == BSDF_CT) ? RT_SPECULAR : RT_DIFFUSE;
+                    int nrt = delta ? RT_MIRROR : (b.kind == BSDF_CT) ? RT_SPECULAR : RT_DIFFUSE;
                     bool killed = false;
                     if (bounces > sv.mindepth) {
                         if (rng.u(vslot + 8) > cp) killed = true;
@@ -304,7 +327,8 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
                         cont = true;
                         ndir = s.dir;
                         ntmax = s.tmax;
-                        flags = nrt | (matte ? F_MATTE : 0) | ((tb + 1) << 8) | ((bounces + 1) << 16);
+                        // mirror vertices count as true bounces only
+                        flags = nrt | (matte ? F_MATTE : 0) | ((tb + 1) << 8) | ((bounces + (delta ? 0 : 1)) << 16);
                         T4 = make_float4(T.x, T.y, T.z, cp);
                     }
                 }

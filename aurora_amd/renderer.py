@@ -100,6 +100,10 @@ class Renderer:
     def set_seed(self, seed: int) -> None:
         L.check(self.lib.aur_set_seed(self._ctx, seed & 0xFFFFFFFF))
 
+    def set_sampler(self, sampler: str) -> None:
+        """"hash" (default) or "halton" (Faure-permuted Halton QMC, per-pixel enumeration)."""
+        L.check(self.lib.aur_set_sampler(self._ctx, {"hash": L.AUR_SAMPLER_HASH, "halton": L.AUR_SAMPLER_HALTON}[sampler]))
+
     def set_tri_albedo(self, albedo: Optional[np.ndarray]) -> None:
         if albedo is None:
             L.check(self.lib.aur_set_tri_albedo(self._ctx, None))

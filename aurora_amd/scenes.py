@@ -440,6 +440,9 @@ def textured_stage(res=96, spp=4, maxdepth=4, mindepth=4, sphere_subd=0, seed=5)
     * rotated box without uvs: metal with triplanar reflectance, triplanar
       float roughnessA, a uv-mapped float mix (read at uv = 0) and a constant
       normal map;
+    * glass ball (glassMaterial: Fresnel-weighted SpecMirror / SpecTransmit)
+      and a mirror panel (glass with reflectance 1), whose rays that miss see
+      the environment in the camera ray's direction;
     * lights: a square light, an env light with a lat-long map (stored as FP32,
       rounded to FP16 when loaded) and a constant env light;
     * a rotated camera, so the env map's camera-to-world transform matters.
@@ -462,6 +465,9 @@ def textured_stage(res=96, spp=4, maxdepth=4, mindepth=4, sphere_subd=0, seed=5)
         s.meshes["sphere.obj"] = icosphere(2)
     v, f = box_mesh([-0.5, 0.0, -0.5], [0.5, 1.0, 0.5])
     s.meshes["box.obj"] = (v, f, None)
+    s.meshes["glassball.obj"] = icosphere(2)
+    mv, mf = quad([-0.6, 0.0, 0.0], [0.6, 0.0, 0.0], [0.6, 1.6, 0.0], [-0.6, 1.6, 0.0])
+    s.meshes["mirror.obj"] = (mv, mf, None)
 
     checker = np.ones((16, 16, 4), np.float32)
     ij = np.add.outer(np.arange(16), np.arange(16)) % 2
@@ -515,6 +521,14 @@ def textured_stage(res=96, spp=4, maxdepth=4, mindepth=4, sphere_subd=0, seed=5)
                                subdlevels=sphere_subd)
     s.objects["box"] = _geo("box", "box.obj", "brushed",
                             [{"translate": [0.9, 0.0, 0.2]}, {"rotateY": 33.0}, {"scale": [0.8, 1.1, 0.8]}])
+    s.materials["glass"] = {"type": "glassMaterial", "specColor": [0.95, 0.95, 0.95], "reflectance": 0.08,
+                            "ior": 1.5, "transmitColor": [0.9, 0.97, 0.92]}
+    s.materials["mirror"] = {"type": "glassMaterial", "specColor": [0.9, 0.9, 0.85], "reflectance": 1.0,
+                             "ior": 1.0, "transmitColor": [0.0, 0.0, 0.0]}
+    s.objects["glassball"] = _geo("glassball", "glassball.obj", "glass",
+                                  [{"translate": [0.05, 0.42, 1.1]}, {"scale": [0.42, 0.42, 0.42]}])
+    s.objects["mirror"] = _geo("mirror", "mirror.obj", "mirror",
+                               [{"translate": [2.1, 0.0, -1.2]}, {"rotateY": -35.0}])
     s.objects["light_key"] = _square_light("light_key", [10.0, 9.0, 8.0], 0.5, 0.5,
                                            [{"translate": [1.5, 3.2, 1.0]}, {"rotateX": -70.0}])
     s.objects["sky"] = {"type": "envlight", "name": "sky", "exposure": 0.5, "color": [1.0, 1.0, 1.0],

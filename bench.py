@@ -169,6 +169,24 @@ def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int 
         r.set_seed(1000 + k)
         a, _ = r.render_passes(0, spp)
         gpu_imgs.append(a[..., :3].astype(np.float64)[sub] / spp)
+    # the Halton QMC source (the reference's USE_HALTON sampler): full-frame
+    # rate, and 16-spp estimates of the same pixels from disjoint pass ranges
+    r.set_sampler("halton")
+    r.render_passes(0, spp, fetch=False)
+    _hip_sync()
+    t0 = time.perf_counter()
+    for k in range(reps):
+        r.render_passes(0, spp, fetch=False)
+    _hip_sync()
+    qmc_rate = reps * W * H * spp / (time.perf_counter() - t0) / 1e6
+    pix_sub = (ys * W + xs).ravel().astype(np.int32)
+    qmc_imgs = []
+    for k in range(n_seeds):
+        P = np.repeat(pix_sub, spp)
+        S = np.tile(np.arange(k * spp, (k + 1) * spp, dtype=np.int32), pix_sub.size)
+        o, _ = r.render_paths(P, S)
+        qmc_imgs.append(o[:, :3].astype(np.float64).reshape(pix_sub.size, spp, 3).mean(1))
+    r.set_sampler("hash")
     exp = r.export()
     r.close()
     sc = O.Scene(exp, bvh=True)
@@ -197,7 +215,9 @@ def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int 
     ref_s = time.time() - t0
     mse_g = np.array([np.mean((g - ref) ** 2) for g in gpu_imgs])
     mse_c = np.array([np.mean((c - ref) ** 2) for c in cpu_imgs])
+    mse_q = np.array([np.mean((q - ref) ** 2) for q in qmc_imgs])
     rmse_g, rmse_c = float(np.sqrt(mse_g.mean())), float(np.sqrt(mse_c.mean()))
+    rmse_q = float(np.sqrt(mse_q.mean()))
     # standard error of the difference of the two mean squared errors
     se = float(np.sqrt(mse_g.var(ddof=1) / n_seeds + mse_c.var(ddof=1) / n_seeds))
     return {"workload": "box_room_256_spp16_depth5", "paths_per_frame": W * H * spp,
@@ -207,6 +227,8 @@ def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int 
             "rmse_gpu_vs_ref": float("%.4g" % rmse_g), "rmse_cpu_vs_ref": float("%.4g" % rmse_c),
             "mse_diff_in_se": float("%.3g" % ((mse_g.mean() - mse_c.mean()) / max(se, 1e-30))),
             "rmse_seeds": n_seeds,
+            "gpu_halton_Mpaths_s": round(qmc_rate, 3), "rmse_gpu_halton_vs_ref": float("%.4g" % rmse_q),
+            "halton_rmse_vs_hash": float("%.3g" % (rmse_q / max(rmse_g, 1e-30))),
             "ref": "%d-spp oracle on %d pixels (every %dth row and column), %.1f s" % (ref_spp, pix.size, stride, ref_s)}
 
 

@@ -56,6 +56,7 @@ typedef struct {
 #define AUR_MAT_METAL 1
 #define AUR_MAT_KELEMEN 2
 #define AUR_MAT_LIGHT 3 /* ConstantMaterial on light triangles (renderer.cpp:131-133) */
+#define AUR_MAT_GLASS 4 /* GlassMaterial: SpecMirror / SpecTransmit (glassMaterial.cpp) */
 
 #define AUR_DIST_BECKMANN 0
 #define AUR_DIST_GGX 1
@@ -88,6 +89,8 @@ typedef struct {
     float emission[3]; /* light triangles: color * 2^exposure */
     int shader[AUR_N_SLOTS]; /* aur_shader index driving each AUR_SLOT_*, or -1 for the constant above */
     float c2o[16];     /* camera-to-object matrix of the attribute's object, row major (triplanar shaders) */
+    float transmit[3]; /* glass transmitColor (specColor is spec, reflectance is reflectance) */
+    float ior;         /* glass index of refraction */
 } aur_material;
 
 typedef struct {
@@ -175,8 +178,9 @@ typedef struct {
     double ms;           /* wall time of the call on the device stream */
 } aur_stats;
 
-/* One BSDF query: kind (0 Lambert, 1 Cook-Torrance), params, the frame and two
- * uniforms for getSample. */
+/* One BSDF query: kind (0 Lambert, 1 Cook-Torrance, 2 SpecMirror,
+ * 3 SpecTransmit with roughness = ior), params, the frame and two uniforms for
+ * getSample. */
 typedef struct {
     int kind;
     int distribution;
@@ -201,6 +205,16 @@ const char *aur_last_error(void);
 /* hip_stream: a hipStream_t, or NULL for the library's own stream. */
 int aur_set_stream(aur_ctx *ctx, void *hip_stream);
 int aur_set_seed(aur_ctx *ctx, uint32_t seed);
+/* Sample source of the paths' random numbers.  AUR_SAMPLER_HASH (default):
+ * a counter-based hash of (seed, pixel, pass, slot).  AUR_SAMPLER_HALTON: the
+ * reference's optional QMC source (Aurora/external/halton, enabled there by
+ * USE_HALTON, constants.h:31-41): Faure-permuted Halton points enumerated per
+ * pixel (Halton_enum); the pass-th point of a pixel gives the camera position
+ * (camera.cpp:39-45) and, dimension by dimension, the path's other slots
+ * (2..255; later slots use the hash).  The seed does not affect it. */
+#define AUR_SAMPLER_HASH 0
+#define AUR_SAMPLER_HALTON 1
+int aur_set_sampler(aur_ctx *ctx, int sampler);
 
 /* ---- scene ---- */
 /* Replaces JsonParser::parseScene (jsonParser.cpp:30-59) + ObjTriangleMesh

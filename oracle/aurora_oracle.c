@@ -122,9 +122,28 @@ float orc_slot_uniform(uint32_t key, uint32_t slot) {
     return (float)(pcg_hash(key + slot * 0x85EBCA6Bu) >> 8) * (1.0f / 16777216.0f);
 }
 
-typedef struct { const float *slots; uint32_t key; } Rng;
+/* A path's random numbers: injected slots, the Halton source (scene sampler
+ * 1: the sample-th Halton point of the pixel, slots 0 and 1 the camera offsets
+ * inside the pixel, slot k < 256 dimension k; oracle_halton.c) or the hash. */
+typedef struct {
+    const float *slots;
+    uint32_t key;
+    int qmc;
+    unsigned w, h, x, y, sample;
+} Rng;
 static inline float rng_u(const Rng *r, int slot) {
-    return r->slots ? r->slots[slot] : orc_slot_uniform(r->key, (uint32_t)slot);
+    if (r->slots) return r->slots[slot];
+    if (r->qmc && slot < 256) return orc_halton_slot(r->w, r->h, r->sample, r->x, r->y, (unsigned)slot);
+    return orc_slot_uniform(r->key, (uint32_t)slot);
+}
+static inline Rng make_rng(const orc_scene *s, const float *slots, uint32_t seed, int x, int y, uint32_t sample) {
+    Rng r;
+    r.slots = slots;
+    r.key = orc_path_key(seed, (uint32_t)(y * s->width + x), sample);
+    r.qmc = s->sampler == 1;
+    r.w = (unsigned)s->width; r.h = (unsigned)s->height;
+    r.x = (unsigned)x; r.y = (unsigned)y; r.sample = sample;
+    return r;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -714,9 +733,14 @@ static C3 bsdf_eval(const Bsdf *b, V3 ln, V3 vn, V3 nn) {
     return cdivf(cmulf(b->color, D), (float)(4. * vdoth * vdoth));
 }
 
+static Sample3D delta_sample(const Bsdf *b, V3 vn, V3 nn);
+
+/* kinds ORC_BSDF_MIRROR / ORC_BSDF_TRANSMIT: specMirror.cpp / specTransmit.cpp
+ * (evalSampleWorld and pdf are zero; params[3] is the ior) */
 void orc_bsdf_eval(int kind, const float *params, const float *V, const float *N, const float *L,
                    float *f_out, float *pdf_out) {
     Bsdf b; b.kind = kind; b.color = c3p(params); b.rough = params[3]; b.dist = (int)params[4];
+    if (kind >= ORC_BSDF_MIRROR) { f_out[0] = f_out[1] = f_out[2] = 0.f; *pdf_out = 0.f; return; }
     C3 f = bsdf_eval(&b, v3p(L), v3p(V), v3p(N));
     f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
     *pdf_out = bsdf_pdf(&b, v3p(L), v3p(V), v3p(N));
@@ -725,7 +749,7 @@ void orc_bsdf_eval(int kind, const float *params, const float *V, const float *N
 void orc_bsdf_sample(int kind, const float *params, const float *V, const float *N, float u1, float u2,
                      float *dir_out, float *pdf_out, float *color_out) {
     Bsdf b; b.kind = kind; b.color = c3p(params); b.rough = params[3]; b.dist = (int)params[4];
-    Sample3D s = bsdf_sample(&b, v3p(V), v3p(N), u1, u2);
+    Sample3D s = kind >= ORC_BSDF_MIRROR ? delta_sample(&b, v3p(V), v3p(N)) : bsdf_sample(&b, v3p(V), v3p(N), u1, u2);
     memcpy(dir_out, s.ray.d, sizeof(float) * 3);
     *pdf_out = s.pdf;
     color_out[0] = s.color.r; color_out[1] = s.color.g; color_out[2] = s.color.b;
@@ -847,9 +871,54 @@ static int get_brdf(const orc_scene *s, int attr, int prim, V3 vn, V3 nn, int ma
         b->kind = ORC_BSDF_CT; b->dist = m->distribution; b->color = spec; b->rough = rough;
         return 1;
     }
+    case ORC_MAT_GLASS: { /* glassMaterial.cpp:33-58 */
+        b->rough = m->ior; b->dist = 0;
+        if (m->reflectance == 1.f) { b->kind = ORC_BSDF_MIRROR; b->color = c3p(m->spec); return 1; }
+        float costheta = (float)(dot3(vn, nn) * 0.5);
+        if (costheta < 0.f) costheta = -costheta;
+        float albedo = schlick(costheta, m->reflectance);
+        if (u_lobe > albedo) { b->kind = ORC_BSDF_TRANSMIT; b->color = c3p(m->transmit); }
+        else { b->kind = ORC_BSDF_MIRROR; b->color = c3p(m->spec); }
+        return 1;
+    }
     default: /* constantMaterial.cpp:19-24: pdf-0 sample terminates the path */
         return 0;
     }
+}
+
+/* math_utils.h:91-95 (the coordinateSystem frame, not normalised) */
+static V3 world_to_tangent(V3 dir, V3 n) {
+    V3 t, b;
+    coordinate_system(n, &t, &b);
+    return v3(dot3(dir, t), dot3(dir, b), dot3(dir, n));
+}
+
+/* specMirror.cpp:30-35, specTransmit.cpp:30-58 (b->rough = ior) */
+static Sample3D delta_sample(const Bsdf *b, V3 vn, V3 nn) {
+    Sample3D s;
+    memset(&s, 0, sizeof s);
+    V3 vt = world_to_tangent(vn, nn);
+    if (b->kind == ORC_BSDF_MIRROR) {
+        V3 r = vnorm(tangent_to_world(v3(-vt.x, -vt.y, vt.z), nn));
+        s.ray.d[0] = r.x; s.ray.d[1] = r.y; s.ray.d[2] = r.z;
+        s.ray.tmin = (float)RAY_BIAS; s.ray.tmax = 100000000.f;
+        s.pdf = 1.f; s.color = b->color;
+        return s;
+    }
+    float costheta = dot3(nn, vn);
+    float ei = 1, et = b->rough;
+    if (costheta < 0.) { ei = b->rough; et = 1.; }
+    float sini2 = fmaxf(0.f, 1 - costheta * costheta);
+    float eta = ei / et;
+    float sint2 = eta * eta * sini2;
+    if (sint2 >= 1.) { s.pdf = 0.f; return s; }
+    float cost = sqrtf(fmaxf(0.f, 1 - sint2));
+    if (costheta > 0.) cost = -cost;
+    V3 t = vnorm(tangent_to_world(v3(eta * -vt.x, eta * -vt.y, cost), nn));
+    s.ray.d[0] = t.x; s.ray.d[1] = t.y; s.ray.d[2] = t.z;
+    s.ray.tmin = (float)RAY_BIAS; s.ray.tmax = 10000000.f;
+    s.pdf = 1.f; s.color = b->color;
+    return s;
 }
 
 /* integrator.cpp:28-31 */
@@ -957,13 +1026,14 @@ static void camera_ray(const orc_scene *s, int x, int y, float j0, float j1, orc
     r->tmin = 0.1f; r->tmax = 10000000.f;
 }
 
-enum { RAY_CAMERA, RAY_DIFFUSE, RAY_SPECULAR };
+enum { RAY_CAMERA, RAY_DIFFUSE, RAY_SPECULAR, RAY_MIRROR };
 
 /* integrator.cpp:152-300 with numSamples = 1 (renderer.cpp:233: the first pass
  * always uses one light sample; BASELINE configs use lightsamples = 1). */
 static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, const Rng *rng, float *out, orc_counters *cnt) {
-    orc_ray ray;
+    orc_ray ray, cam;
     camera_ray(s, x, y, rng_u(rng, 0), rng_u(rng, 1), &ray);
+    cam = ray;
     cnt->paths++;
     orc_hit h;
     cnt->ext++;
@@ -988,7 +1058,7 @@ static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
         V3 vn = vneg(vnorm(v3p(ray.d)));
         if (dot3(vn, Nn) <= 0.) Nn = vnorm(cross3(vn, cross3(Nn, vn)));
         if (ray_type == RAY_DIFFUSE) matte_path = 1;
-        if (ray_type == RAY_CAMERA) Lo = cadd(Lo, cmulc(T, c3p(s->mats[attr].emission)));
+        if (ray_type == RAY_CAMERA || ray_type == RAY_MIRROR) Lo = cadd(Lo, cmulc(T, c3p(s->mats[attr].emission)));
         int vslot = 2 + ORC_SLOTS_PER_VERTEX * true_bounces;
         Bsdf b;
         int has_brdf = get_brdf(s, attr, h.prim, vn, Nn, matte_path, rng_u(rng, vslot + 0), &g, &b);
@@ -997,28 +1067,43 @@ static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
              * constant BRDF adds nothing and its sample has pdf 0 */
             break;
         }
-        if (bounces < s->maxdepth) {
+        int delta = b.kind == ORC_BSDF_MIRROR || b.kind == ORC_BSDF_TRANSMIT; /* MirrorBrdf */
+        if (bounces < s->maxdepth && !delta) { /* integrateDirect adds nothing for MirrorBrdf */
             C3 direct = integrate_direct(s, bv, P, vn, Nn, &b, rng, vslot, cnt);
             Lo = cadd(Lo, cmulc(direct, T));
         }
-        Sample3D smp = bsdf_sample(&b, vn, Nn, rng_u(rng, vslot + 6), rng_u(rng, vslot + 7));
+        Sample3D smp = delta ? delta_sample(&b, vn, Nn) : bsdf_sample(&b, vn, Nn, rng_u(rng, vslot + 6), rng_u(rng, vslot + 7));
         if (smp.pdf <= 0.f) break;
-        float ndl = dot3(Nn, v3p(smp.ray.d));
-        T = cmulc(T, cdivf(cmulf(smp.color, ndl), smp.pdf));
-        if (bounces > s->mindepth) cp *= clum(smp.color) * ndl / smp.pdf;
-        if (cblack(T)) break;
-        ray_type = (b.kind == ORC_BSDF_CT) ? RAY_SPECULAR : RAY_DIFFUSE;
+        if (!delta) {
+            float ndl = dot3(Nn, v3p(smp.ray.d));
+            T = cmulc(T, cdivf(cmulf(smp.color, ndl), smp.pdf));
+            if (bounces > s->mindepth) cp *= clum(smp.color) * ndl / smp.pdf;
+            if (cblack(T)) break;
+            ray_type = (b.kind == ORC_BSDF_CT) ? RAY_SPECULAR : RAY_DIFFUSE;
+        } else { /* integrator.cpp:231-237 */
+            T = cmulc(T, smp.color);
+            if (bounces > s->mindepth) cp *= clum(smp.color);
+            ray_type = RAY_MIRROR;
+        }
         if (bounces > s->mindepth) {
             if (rng_u(rng, vslot + 8) > cp) break;
             T = cdivf(T, cp);
         }
         smp.ray.o[0] = P.x; smp.ray.o[1] = P.y; smp.ray.o[2] = P.z;
         ++true_bounces;
-        ++bounces;
+        if (!delta) ++bounces;
         ray = smp.ray;
         cnt->ext++;
         trace(s, bv, &ray, 0, &h);
-        if (h.prim < 0) break;
+        if (h.prim < 0) {
+            /* integrator.cpp:262-271: a mirror ray's miss adds each env light's
+             * eval() in the direction of the camera sample */
+            if (ray_type == RAY_MIRROR)
+                for (int i = 0; i < s->n_lights; ++i)
+                    if (s->lights[i].type == ORC_LIGHT_ENV)
+                        Lo = cadd(Lo, cmulc(light_eval(s, &s->lights[i], v3p(cam.d)), T));
+            break;
+        }
         cnt->hit++;
     }
     /* integrator.cpp:277-281 */
@@ -1028,7 +1113,7 @@ static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
 
 void orc_path(const orc_scene *s, const orc_bvh *b, int x, int y, uint32_t seed, uint32_t sample,
               const float *slots, float *rgba_out, orc_counters *cnt) {
-    Rng r; r.slots = slots; r.key = orc_path_key(seed, (uint32_t)(y * s->width + x), sample);
+    Rng r = make_rng(s, slots, seed, x, y, sample);
     path_trace(s, b, x, y, &r, rgba_out, cnt);
 }
 
@@ -1052,7 +1137,7 @@ void orc_render(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_p
             for (int y = y0; y < y0 + tile && y < s->height; ++y)
                 for (int x = x0; x < x0 + tile && x < s->width; ++x) {
                     float o[4];
-                    Rng r; r.slots = NULL; r.key = orc_path_key(seed, (uint32_t)(y * s->width + x), (uint32_t)p);
+                    Rng r = make_rng(s, NULL, seed, x, y, (uint32_t)p);
                     path_trace(s, b, x, y, &r, o, &lc);
                     float *a = accum + 4 * ((size_t)y * s->width + x);
                     a[0] += o[0]; a[1] += o[1]; a[2] += o[2]; a[3] += o[3];
@@ -1075,9 +1160,8 @@ void orc_render_paths(const orc_scene *s, const orc_bvh *b, uint32_t seed, const
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : c_paths, c_ext, c_hit, c_sh)
     for (int i = 0; i < n; ++i) {
         orc_counters lc = {0, 0, 0, 0};
-        Rng r;
-        r.slots = slots ? slots + (size_t)i * nslots : NULL;
-        r.key = orc_path_key(seed, (uint32_t)pixel[i], (uint32_t)sample[i]);
+        Rng r = make_rng(s, slots ? slots + (size_t)i * nslots : NULL, seed, pixel[i] % s->width, pixel[i] / s->width,
+                         (uint32_t)sample[i]);
         path_trace(s, b, pixel[i] % s->width, pixel[i] / s->width, &r, out + 4 * (size_t)i, &lc);
         c_paths += lc.paths; c_ext += lc.ext; c_hit += lc.hit; c_sh += lc.shadow;
     }
@@ -1309,9 +1393,7 @@ void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_
 #pragma omp for schedule(dynamic, 256)
         for (long long i = 0; i < n; ++i) {
             int pixel = (int)(i % wh), sample = first_pass + (int)(i / wh);
-            Rng r;
-            r.slots = NULL;
-            r.key = orc_path_key(seed, (uint32_t)pixel, (uint32_t)sample);
+            Rng r = make_rng(s, NULL, seed, pixel % s->width, pixel / s->width, (uint32_t)sample);
             double L[3];
             int k = path_replay(s, b, pixel % s->width, pixel / s->width, &r, 0, NULL, NULL, L, &g);
             if (k < 0) continue;

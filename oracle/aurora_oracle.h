@@ -28,10 +28,10 @@
 extern "C" {
 #endif
 
-enum { ORC_MAT_MATTE = 0, ORC_MAT_METAL = 1, ORC_MAT_KELEMEN = 2, ORC_MAT_LIGHT = 3 };
+enum { ORC_MAT_MATTE = 0, ORC_MAT_METAL = 1, ORC_MAT_KELEMEN = 2, ORC_MAT_LIGHT = 3, ORC_MAT_GLASS = 4 };
 enum { ORC_DIST_BECKMANN = 0, ORC_DIST_GGX = 1 };
 enum { ORC_LIGHT_AREA = 0, ORC_LIGHT_ENV = 1 };
-enum { ORC_BSDF_LAMBERT = 0, ORC_BSDF_CT = 1 };
+enum { ORC_BSDF_LAMBERT = 0, ORC_BSDF_CT = 1, ORC_BSDF_MIRROR = 2, ORC_BSDF_TRANSMIT = 3 };
 
 enum { ORC_SLOT_DIFFUSE = 0, ORC_SLOT_SPEC = 1, ORC_SLOT_GAIN = 2, ORC_SLOT_ROUGH = 3, ORC_SLOT_ROUGHB = 4,
        ORC_SLOT_MIX = 5, ORC_SLOT_NORMAL = 6, ORC_N_SLOTS = 7 };
@@ -56,6 +56,8 @@ typedef struct {
     const float *table;  /* kelemen albedo table [64*32], index cos*32 + rough */
     int shader[ORC_N_SLOTS]; /* shader driving each ORC_SLOT_*, -1 = the constant above */
     float c2o[16];       /* camera-to-object matrix of the attribute's object (triplanar) */
+    float transmit[3];   /* glass transmitColor (spec = specColor) */
+    float ior;           /* glass */
 } orc_material;
 
 typedef struct {
@@ -109,6 +111,7 @@ typedef struct {
     const orc_shader *shaders;
     int n_textures;
     const orc_texture *textures;
+    int sampler;           /* 0: hash of (seed, pixel, sample, slot); 1: Halton (oracle_halton.c) */
 } orc_scene;
 
 typedef struct {
@@ -192,6 +195,12 @@ void orc_light_sample_in(const orc_scene *sc, const orc_light *l, const float *P
                          orc_ray *ray, float *pdf, float *color);
 float orc_light_pdf_in(const orc_scene *sc, const orc_light *l, orc_ray *ray);
 int orc_light_visible(const orc_light *l, const float *P, const float *N);
+
+/* ---- Halton QMC (oracle_halton.c) ---- */
+float orc_halton_sample(unsigned dim, unsigned index);
+void orc_halton_enum(unsigned w, unsigned h, unsigned *p2, unsigned *p3, unsigned *mx, unsigned *my, unsigned *inc);
+unsigned orc_halton_index(unsigned w, unsigned h, unsigned i, unsigned x, unsigned y);
+float orc_halton_slot(unsigned w, unsigned h, unsigned i, unsigned x, unsigned y, unsigned slot);
 
 /* ---- adjoint (path-replay) ---- */
 void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_pass, int n_passes,

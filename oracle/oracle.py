@@ -30,7 +30,7 @@ class OMaterial(C.Structure):
     _fields_ = [("type", C.c_int), ("distribution", C.c_int), ("diffuse", C.c_float * 3), ("spec", C.c_float * 3),
                 ("gain", C.c_float * 3), ("roughness", C.c_float), ("roughnessB", C.c_float), ("mix", C.c_float),
                 ("reflectance", C.c_float), ("emission", C.c_float * 3), ("table", C.POINTER(C.c_float)),
-                ("shader", C.c_int * 7), ("c2o", C.c_float * 16)]
+                ("shader", C.c_int * 7), ("c2o", C.c_float * 16), ("transmit", C.c_float * 3), ("ior", C.c_float)]
 
 
 class OLight(C.Structure):
@@ -56,7 +56,7 @@ class OScene(C.Structure):
                 ("n_lights", C.c_int), ("lights", C.POINTER(OLight)), ("fov", C.c_float), ("width", C.c_int),
                 ("height", C.c_int), ("maxdepth", C.c_int), ("mindepth", C.c_int), ("uvs", C.POINTER(C.c_float)),
                 ("n_shaders", C.c_int), ("shaders", C.POINTER(OShader)), ("n_textures", C.c_int),
-                ("textures", C.POINTER(OTexture))]
+                ("textures", C.POINTER(OTexture)), ("sampler", C.c_int)]
 
 
 class ORay(C.Structure):
@@ -94,6 +94,10 @@ def lib() -> C.CDLL:
         "orc_trace_bvh": (None, [C.POINTER(OScene), C.c_void_p, C.POINTER(ORay), C.c_int, C.c_int, C.POINTER(OHit)]),
         "orc_light_init": (None, [C.POINTER(OLight)]),
         "orc_texture_init_pdf": (None, [C.POINTER(OTexture)]),
+        "orc_halton_sample": (C.c_float, [C.c_uint, C.c_uint]),
+        "orc_halton_enum": (None, [C.c_uint, C.c_uint] + [C.POINTER(C.c_uint)] * 5),
+        "orc_halton_index": (C.c_uint, [C.c_uint] * 5),
+        "orc_halton_slot": (C.c_float, [C.c_uint] * 6),
         "orc_half_round": (C.c_float, [C.c_float]),
         "orc_texture_read": (None, [C.POINTER(OTexture), C.c_float, C.c_float, _FP]),
         "orc_shader_eval": (None, [C.POINTER(OScene), C.c_int, _FP, _FP, C.c_float, C.c_float, C.c_int, _FP]),
@@ -176,6 +180,9 @@ class Scene:
                 o.shader[j] = getattr(m, "shader", [-1] * 7)[j]
             for j in range(16):
                 o.c2o[j] = getattr(m, "c2o", np.eye(4, dtype=np.float32).reshape(-1))[j]
+            for j in range(3):
+                o.transmit[j] = getattr(m, "transmit", [0.0] * 3)[j]
+            o.ior = getattr(m, "ior", 1.0)
             if m.type == 2:
                 # roundf(): halves away from zero, like the reference (kelemenMaterial.cpp:50)
                 q = min(max(int(np.floor(float(np.float32(m.reflectance)) * 20.0 + 0.5)), 0), 20)
@@ -236,6 +243,7 @@ class Scene:
         s.uvs = fp(self.uvs) if self.uvs is not None else None
         s.n_shaders, s.shaders = len(shs), C.cast(self.shaders, C.POINTER(OShader))
         s.n_textures, s.textures = len(texs), C.cast(self.textures, C.POINTER(OTexture))
+        s.sampler = 1 if exp.get("sampler") == "halton" else 0
         self.s = s
         self.bvh = lib().orc_bvh_build(C.byref(s), leaf) if bvh else None
 
@@ -388,6 +396,24 @@ def make_light(m16, sx, sy, color, exposure=0.0, kind=0) -> OLight:
         o.color[j] = color[j]
     lib().orc_light_init(C.byref(o))
     return o
+
+
+def halton_sample(dim: int, index: int) -> float:
+    return float(lib().orc_halton_sample(dim, index & 0xFFFFFFFF))
+
+
+def halton_enum(w: int, h: int) -> dict:
+    v = [C.c_uint() for _ in range(5)]
+    lib().orc_halton_enum(w, h, *[C.byref(x) for x in v])
+    return dict(zip(("p2", "p3", "mx", "my", "inc"), [x.value for x in v]))
+
+
+def halton_index(w: int, h: int, i: int, x: int, y: int) -> int:
+    return int(lib().orc_halton_index(w, h, i, x, y))
+
+
+def halton_slot(w: int, h: int, i: int, x: int, y: int, slot: int) -> float:
+    return float(lib().orc_halton_slot(w, h, i, x, y, slot))
 
 
 def half_round(x: float) -> float:
