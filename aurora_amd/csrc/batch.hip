@@ -437,6 +437,8 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         if (!j->built) throw StateError("job " + std::to_string(k) + ": scene not built (aur_build_accel)");
         if (j->device != c->device) throw std::invalid_argument("job " + std::to_string(k) + " is on another device");
         if (first_pass[k] < 0 || n_passes[k] < 0) throw std::invalid_argument("negative pass range");
+        if (j->scene.opt.lightsamples != 1)
+            throw std::invalid_argument("job " + std::to_string(k) + ": batched jobs need lightsamples = 1");
         if (width == 0) width = j->bvh.width;
         if (j->bvh.width != width) throw std::invalid_argument("all jobs of a batch need the same BVH width");
         depth = std::max(depth, j->bvh.depth);

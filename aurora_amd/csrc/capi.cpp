@@ -733,7 +733,9 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
             src.first_pass = first_pass;
             src.pixel = src.sample = nullptr;
             src.slots = nullptr;
-            run_paths(c, src, wh * (size_t)n_passes, c->accum.as<float4>(), &st);
+            src.ls = c->scene.opt.lightsamples;
+            if (src.ls < 1) throw std::invalid_argument("lightsamples must be at least 1");
+            run_paths(c, src, pass_paths(wh, first_pass, n_passes, src.ls), c->accum.as<float4>(), &st);
         }
         if (rgba_sum_host) {
             HIP_CHECK(hipMemcpyAsync(rgba_sum_host, c->accum.p, sizeof(float4) * wh, hipMemcpyDeviceToHost, c->stream));
@@ -987,6 +989,7 @@ int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_d
             if (m.type == MAT_GLASS) throw StateError("the adjoint does not support glass (specular) materials");
         }
         if (c->sampler != AUR_SAMPLER_HASH) throw StateError("the adjoint replays hash-sampled paths only");
+        if (c->scene.opt.lightsamples != 1) throw StateError("the adjoint replays scenes with lightsamples = 1 only");
         for (const DLight &l : c->dlights)
             if (l.tex >= 0) throw StateError("the adjoint does not support environment maps");
         set_device(c);

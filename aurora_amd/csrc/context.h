@@ -186,11 +186,21 @@ void build_bvh(Ctx *c, int width, int max_leaf);
 struct PathSource {
     int mode;              // 0: render passes, 1: explicit (pixel, sample) lists
     int first_pass;
+    int ls = 1;            // mode 0: paths per pixel in passes after the first (lightsamples)
     const int *pixel;      // device, mode 1
     const int *sample;     // device, mode 1
     const float *slots;    // device, optional injected uniforms
 };
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats);
+// Paths of passes [first_pass, first_pass + n_passes) of a w*h image: one per
+// pixel in pass 0, ls per pixel after it (renderer.cpp:233, 257).
+inline size_t pass_paths(size_t wh, int first_pass, int n_passes, int ls) {
+    if (n_passes <= 0) return 0;
+    size_t n = 0;
+    int p = first_pass;
+    if (p == 0) { n += wh; ++p; --n_passes; }
+    return n + wh * (size_t)ls * (size_t)n_passes;
+}
 void resolve_passes(Ctx *c, const float4 *out_rgba, int n_passes, float4 *accum);
 void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_hit);
 // sizes the path pool and the traversal stack overflow for BVHs of the given width and depth

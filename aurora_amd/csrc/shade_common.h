@@ -252,20 +252,29 @@ __device__ inline uint32_t halton_index(const HaltonView &h, int pixel, uint32_t
     return off + sample * h.inc;
 }
 
+// The key of light sample `sub` of a pixel's pass (sub 0: the pass's own key)
+__device__ __host__ inline uint32_t sub_path_key(uint32_t key, int sub) {
+    return sub == 0 ? key : pcg_hash(key ^ ((uint32_t)sub * 0x68E31DA4u));
+}
+
 // The random numbers of one path: injected, Halton, or the counter-based hash.
-// QMC = false compiles the Halton branch out (kernels that never see it).
+// A pass with lightsamples > 1 traces several paths through the pixel's one
+// camera sample (integrator.cpp:162-173): the camera slots 0 and 1 come from
+// the pass's stream, the vertex slots from the path's own.  QMC = false
+// compiles the Halton branch out (kernels that never see it).
 template <bool QMC>
 struct PathRngT {
     const float *slots;
-    uint32_t key;
+    uint32_t key;             // vertex slots
+    uint32_t key_cam;         // camera slots 0 and 1
     int pixel;                // the path's pixel (for the camera direction)
     const HaltonView *qmc;    // Halton source, or null
-    uint32_t hidx;            // the path's Halton index
+    uint32_t hidx, hidx_cam;  // Halton indices of the vertex and camera slots
     __device__ inline float u(int slot) const {
         if (slots) return slots[slot];
         if constexpr (QMC) {
             if (qmc && slot < AUR_HALTON_DIMS) {
-                const float v = halton_sample(*qmc, (uint32_t)slot, hidx);
+                const float v = halton_sample(*qmc, (uint32_t)slot, slot < 2 ? hidx_cam : hidx);
                 // camera.cpp:39-45: the first two dimensions scaled to the
                 // image, here as offsets inside the pixel
                 if (slot == 0) return v * qmc->sx - (float)(pixel % qmc->width);
@@ -273,22 +282,26 @@ struct PathRngT {
                 return v;
             }
         }
-        return slot_uniform(key, (uint32_t)slot);
+        return slot_uniform(slot < 2 ? key_cam : key, (uint32_t)slot);
     }
 };
 typedef PathRngT<true> PathRng;
 
-// The path (pixel, sample) of a scene: injected slots when given, else the
-// scene's Halton source when set (QMC), else the hash of (seed, pixel, sample).
+// Light sample `sub` of pass `sample` of a pixel: injected slots when given,
+// else the scene's Halton source when set (QMC; the camera from point
+// `sample`, the vertices from point sample * ls + sub), else the hash of
+// (seed, pixel, sample) and sub.
 template <bool QMC = true>
 __device__ inline PathRngT<QMC> make_path_rng(const SceneView &sv, const float *slots, uint32_t seed, int pixel,
-                                              uint32_t sample) {
+                                              uint32_t sample, int sub = 0, int ls = 1) {
     PathRngT<QMC> r;
     r.slots = slots;
-    r.key = path_key(seed, (uint32_t)pixel, sample);
+    r.key_cam = path_key(seed, (uint32_t)pixel, sample);
+    r.key = sub_path_key(r.key_cam, sub);
     r.pixel = pixel;
     r.qmc = QMC ? sv.qmc : nullptr;
-    r.hidx = (QMC && sv.qmc) ? halton_index(*sv.qmc, pixel, sample) : 0u;
+    r.hidx_cam = (QMC && sv.qmc) ? halton_index(*sv.qmc, pixel, sample) : 0u;
+    r.hidx = (QMC && sv.qmc && ls > 1) ? halton_index(*sv.qmc, pixel, sample * (uint32_t)ls + (uint32_t)sub) : r.hidx_cam;
     return r;
 }
 

@@ -23,24 +23,41 @@
 
 namespace aur {
 
-__device__ inline void path_pixel_sample(const SceneView &sv, const WfView &wv, int path, int *pixel, int *sample) {
+// Path id -> pixel, pass and light sample.  Mode 0 issues a batch of passes
+// starting at first_pass: pass 0 holds one path per pixel, later passes ls
+// (one block of w*h paths per light sample), each block in 8x8 tile order.
+// `local` is the id's position in its block.
+__device__ inline void path_decode(const SceneView &sv, const WfView &wv, int path, int *pixel, int *sample, int *sub,
+                                   int *local) {
     if (wv.src.mode == 0) {
-        int wh = sv.width * sv.height;
-        int pass = path / wh;
-        *pixel = tiled_pixel(path - pass * wh, sv.width, sv.height);
-        *sample = wv.src.first_pass + pass;
+        const int wh = sv.width * sv.height;
+        int id = path, p = wv.src.first_pass;
+        if (p == 0 && id < wh) {
+            *sample = 0; *sub = 0; *local = id;
+        } else {
+            if (p == 0) { id -= wh; p = 1; }
+            const int per = wv.src.ls * wh;
+            const int k = id / per;
+            const int r = id - k * per;
+            *sample = p + k;
+            *sub = r / wh;
+            *local = r - *sub * wh;
+        }
+        *pixel = tiled_pixel(*local, sv.width, sv.height);
     } else {
         *pixel = wv.src.pixel[path];
         *sample = wv.src.sample[path];
+        *sub = 0;
+        *local = 0;
     }
 }
 
 template <bool QMC = true>
 __device__ inline PathRngT<QMC> path_rng(const SceneView &sv, const WfView &wv, int path) {
-    int pixel, sample;
-    path_pixel_sample(sv, wv, path, &pixel, &sample);
+    int pixel, sample, sub, local;
+    path_decode(sv, wv, path, &pixel, &sample, &sub, &local);
     return make_path_rng<QMC>(sv, wv.src.slots ? wv.src.slots + (size_t)path * wv.nslots : nullptr, wv.seed, pixel,
-                              (uint32_t)sample);
+                              (uint32_t)sample, sub, wv.src.ls);
 }
 
 // ---------------------------------------------------------------------------
@@ -49,10 +66,8 @@ __device__ inline PathRngT<QMC> path_rng(const SceneView &sv, const WfView &wv, 
 __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
     int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_paths; i += stride) {
-        int pixel, sample;
-        path_pixel_sample(sv, wv, i, &pixel, &sample);
-        PathRng rng = path_rng(sv, wv, i);
-        f3 d = camera_dir(sv, pixel, rng.u(0), rng.u(1));
+        const PathRng rng = path_rng(sv, wv, i);
+        f3 d = camera_dir(sv, rng.pixel, rng.u(0), rng.u(1));
         wv.ray[0][2 * i] = make_float4(0.f, 0.f, 0.f, 0.1f);
         wv.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
         wv.T[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
@@ -144,20 +159,38 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView
 // ---------------------------------------------------------------------------
 __global__ void k_finish(SceneView sv, WfView wv) {
     const int n = wv.counts[C_FINISH];
-    const int wh = sv.width * sv.height;
     const int stride = gridDim.x * blockDim.x;
+    unsigned ext_dup = 0, hit_dup = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float4 a = wv.finLa[i], b = wv.finLb[i];
         const int4 m = wv.finMeta[i];
         const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
-        // whole passes land by (pass, pixel) for the resolve; injected paths by id
+        // whole passes land by (pass, light sample, pixel) for the resolve;
+        // a camera miss is marked by alpha + 2 (the resolve takes it once);
+        // injected paths land by id
         int slot = m.y;
+        float w = a.w;
         if (wv.src.mode == 0) {
-            int pixel, sample;
-            path_pixel_sample(sv, wv, m.y, &pixel, &sample);
-            slot = (sample - wv.src.first_pass) * wh + pixel;
+            int pixel, sample, sub, local;
+            path_decode(sv, wv, m.y, &pixel, &sample, &sub, &local);
+            slot = m.y - local + pixel;
+            const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
+            if (cam_miss) w += 2.f;
+            // the reference traces a pass's camera ray once for all its light
+            // samples (integrator.cpp:160-164): later samples do not count it
+            if (sub > 0) { ext_dup += 1; hit_dup += cam_miss ? 0 : 1; }
         }
-        wv.out[slot] = make_float4(L.x, L.y, L.z, a.w);
+        wv.out[slot] = make_float4(L.x, L.y, L.z, w);
+    }
+    if (wv.src.mode == 0 && wv.src.ls > 1) {
+        for (int o = 32; o > 0; o >>= 1) {
+            ext_dup += __shfl_xor(ext_dup, o);
+            hit_dup += __shfl_xor(hit_dup, o);
+        }
+        if ((threadIdx.x & 63) == 0 && ext_dup) {
+            atomicAdd(&wv.stats[S_EXT], (unsigned long long)(0ull - ext_dup));
+            atomicAdd(&wv.stats[S_HIT], (unsigned long long)(0ull - hit_dup));
+        }
     }
 }
 
@@ -193,14 +226,34 @@ __global__ void k_sum_partials(unsigned long long *partials, int n, unsigned lon
     if (threadIdx.x == 0) stats[S_HIT] += s[0];
 }
 
-// per-pixel sum over the passes of a batch, in pass order (deterministic)
-__global__ void k_resolve(const float4 *__restrict__ out, int wh, int n_passes, float4 *__restrict__ accum) {
+// per-pixel sum over the passes of a batch, in pass order (deterministic).  A
+// pass with ls light samples contributes the sum of its paths times 1/ls in
+// sample order, or its first path alone where the camera ray missed
+// (integrator.cpp:176-292: result += Lo / numSamples; a miss returns the env).
+__device__ inline float4 pass_value(const float4 *__restrict__ out, size_t off, int wh, int n_sub, int p) {
+    const float4 v0 = out[off + p];
+    if (v0.w >= 2.f) return make_float4(v0.x, v0.y, v0.z, v0.w - 2.f);
+    if (n_sub == 1) return v0;
+    const float inv = 1.f / (float)n_sub;
+    float4 s = make_float4(0.f, 0.f, 0.f, v0.w);
+    for (int j = 0; j < n_sub; ++j) {
+        const float4 v = out[off + (size_t)j * wh + p];
+        s.x += v.x * inv; s.y += v.y * inv; s.z += v.z * inv;
+    }
+    return s;
+}
+
+__global__ void k_resolve(const float4 *__restrict__ out, int wh, int first_pass, int n_passes, int ls,
+                          float4 *__restrict__ accum) {
     int stride = gridDim.x * blockDim.x;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < wh; p += stride) {
         float4 s = accum[p];
+        size_t off = 0;
         for (int k = 0; k < n_passes; ++k) {
-            float4 v = out[(size_t)k * wh + p];
+            const int n_sub = (first_pass + k == 0) ? 1 : ls;
+            const float4 v = pass_value(out, off, wh, n_sub, p);
             s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+            off += (size_t)n_sub * wh;
         }
         accum[p] = s;
     }
@@ -399,8 +452,10 @@ static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats) {
     if (!c->built) throw StateError("render before aur_build_accel");
     const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
-    size_t cap = src.mode == 0 ? std::max<size_t>(wh, std::min<size_t>(n_paths, (size_t)8 << 20)) : std::min<size_t>(n_paths, (size_t)8 << 20);
-    if (src.mode == 0) cap = std::max<size_t>(wh, (cap / wh) * wh);
+    const int ls = std::max(1, src.ls);
+    // mode 0 runs whole passes per batch (a pass of ls paths per pixel fits)
+    size_t cap = src.mode == 0 ? std::max<size_t>(wh * ls, std::min<size_t>(n_paths, (size_t)8 << 20))
+                               : std::min<size_t>(n_paths, (size_t)8 << 20);
     if (cap > (size_t)0x7fffffff) throw CapacityError("batch too large");
     ensure_wavefront(c, std::max<size_t>(cap, 1));
     SceneView sv = scene_view(c);
@@ -412,11 +467,25 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
                        c->wf.stats.as<unsigned long long>(), 1);
     fill_zero(c, c->wf.partials.p, sizeof(unsigned long long) * trace_blocks(c));
     size_t done = 0;
+    int pass = src.first_pass;
     while (done < n_paths) {
         size_t n = std::min(cap, n_paths - done);
         PathSource s = src;
+        s.ls = ls;
+        int passes = 0;
         if (src.mode == 0) {
-            s.first_pass = src.first_pass + (int)(done / wh);
+            // as many whole passes as fit
+            s.first_pass = pass;
+            size_t m = 0;
+            while (done + m < n_paths) {
+                const size_t np = wh * (size_t)(pass + passes == 0 ? 1 : ls);
+                if (passes > 0 && m + np > cap) break;
+                m += np;
+                ++passes;
+            }
+            if (done + m > n_paths) throw std::invalid_argument("path count is not a whole number of passes");
+            n = m;
+            pass += passes;
         } else {
             s.pixel = src.pixel + done;
             s.sample = src.sample + done;
@@ -426,9 +495,9 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
         hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, wv.counts, wv.stats, 0);
         run_batch(c, sv, wv, (int)n);
         if (src.mode == 0) {
-            int passes = (int)(n / wh);
             int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));
-            TIMED(c, KT_RESOLVE, hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh, passes, out_rgba));
+            TIMED(c, KT_RESOLVE, hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh,
+                                                    s.first_pass, passes, ls, out_rgba));
         } else {
             HIP_CHECK(hipMemcpyAsync(out_rgba + done, wv.out, sizeof(float4) * n, hipMemcpyDeviceToDevice, c->stream));
         }

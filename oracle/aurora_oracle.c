@@ -127,23 +127,34 @@ float orc_slot_uniform(uint32_t key, uint32_t slot) {
  * inside the pixel, slot k < 256 dimension k; oracle_halton.c) or the hash. */
 typedef struct {
     const float *slots;
-    uint32_t key;
+    uint32_t key, key_cam;           /* vertex slots / camera slots 0, 1 */
     int qmc;
-    unsigned w, h, x, y, sample;
+    unsigned w, h, x, y, sample_cam, sample_v;
 } Rng;
 static inline float rng_u(const Rng *r, int slot) {
     if (r->slots) return r->slots[slot];
-    if (r->qmc && slot < 256) return orc_halton_slot(r->w, r->h, r->sample, r->x, r->y, (unsigned)slot);
-    return orc_slot_uniform(r->key, (uint32_t)slot);
+    if (r->qmc && slot < 256)
+        return orc_halton_slot(r->w, r->h, slot < 2 ? r->sample_cam : r->sample_v, r->x, r->y, (unsigned)slot);
+    return orc_slot_uniform(slot < 2 ? r->key_cam : r->key, (uint32_t)slot);
 }
-static inline Rng make_rng(const orc_scene *s, const float *slots, uint32_t seed, int x, int y, uint32_t sample) {
+/* Path `sub` of the camera sample (x, y, sample): camera slots from the
+ * sample's stream, vertex slots from the path's (sub 0: the same stream; the
+ * Halton source takes its vertex point at sample * ls + sub). */
+static inline Rng make_rng_sub(const orc_scene *s, const float *slots, uint32_t seed, int x, int y, uint32_t sample,
+                               int sub, int ls) {
     Rng r;
     r.slots = slots;
-    r.key = orc_path_key(seed, (uint32_t)(y * s->width + x), sample);
+    r.key_cam = orc_path_key(seed, (uint32_t)(y * s->width + x), sample);
+    r.key = sub == 0 ? r.key_cam : pcg_hash(r.key_cam ^ ((uint32_t)sub * 0x68E31DA4u));
     r.qmc = s->sampler == 1;
     r.w = (unsigned)s->width; r.h = (unsigned)s->height;
-    r.x = (unsigned)x; r.y = (unsigned)y; r.sample = sample;
+    r.x = (unsigned)x; r.y = (unsigned)y;
+    r.sample_cam = sample;
+    r.sample_v = ls > 1 ? sample * (unsigned)ls + (unsigned)sub : sample;
     return r;
+}
+static inline Rng make_rng(const orc_scene *s, const float *slots, uint32_t seed, int x, int y, uint32_t sample) {
+    return make_rng_sub(s, slots, seed, x, y, sample, 0, 1);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1028,24 +1039,11 @@ static void camera_ray(const orc_scene *s, int x, int y, float j0, float j1, orc
 
 enum { RAY_CAMERA, RAY_DIFFUSE, RAY_SPECULAR, RAY_MIRROR };
 
-/* integrator.cpp:152-300 with numSamples = 1 (renderer.cpp:233: the first pass
- * always uses one light sample; BASELINE configs use lightsamples = 1). */
-static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, const Rng *rng, float *out, orc_counters *cnt) {
-    orc_ray ray, cam;
-    camera_ray(s, x, y, rng_u(rng, 0), rng_u(rng, 1), &ray);
-    cam = ray;
-    cnt->paths++;
-    orc_hit h;
-    cnt->ext++;
-    trace(s, bv, &ray, 0, &h);
-    if (h.prim < 0) {
-        C3 L = c3(0.f, 0.f, 0.f); float alpha = 0.f;
-        for (int i = 0; i < s->n_lights; ++i)
-            if (s->lights[i].type == ORC_LIGHT_ENV) { L = cadd(L, light_eval(s, &s->lights[i], v3p(ray.d))); alpha = 1.f; }
-        out[0] = L.r; out[1] = L.g; out[2] = L.b; out[3] = alpha;
-        return;
-    }
-    cnt->hit++;
+/* integrator.cpp:176-281: one path from the camera sample's first hit h (ray
+ * `ray`); returns Lo after the firefly clamp */
+static C3 path_from_hit(const orc_scene *s, const orc_bvh *bv, orc_ray ray, orc_hit h, const orc_ray *cam_ray,
+                        const Rng *rng, orc_counters *cnt) {
+    const orc_ray cam = *cam_ray;
     C3 Lo = c3(0.f, 0.f, 0.f), T = c3(1.f, 1.f, 1.f);
     float cp = 1.f;
     int matte_path = 0, ray_type = RAY_CAMERA, bounces = 0, true_bounces = 0;
@@ -1108,7 +1106,43 @@ static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
     }
     /* integrator.cpp:277-281 */
     while (Lo.r > FIREFLY || Lo.g > FIREFLY || Lo.b > FIREFLY) Lo = cmulf(Lo, 0.2f);
-    out[0] = Lo.r; out[1] = Lo.g; out[2] = Lo.b; out[3] = 1.f;
+    return Lo;
+}
+
+/* integrator.cpp:152-300: the camera sample of pixel (x, y) -- camera ray from
+ * rngs[0]'s camera slots, traced once -- and num_samples paths from its first
+ * hit, path j drawing its vertex slots from rngs[j]; the result is the sum of
+ * Lo_j / num_samples in order, a miss returns the env lights' eval.  The
+ * renderer passes num_samples = 1 in pass 0 and LightSamples after it
+ * (renderer.cpp:233, 257). */
+static void camera_sample(const orc_scene *s, const orc_bvh *bv, int x, int y, const Rng *rngs, int num_samples,
+                          float *out, orc_counters *cnt) {
+    orc_ray ray, cam;
+    camera_ray(s, x, y, rng_u(&rngs[0], 0), rng_u(&rngs[0], 1), &ray);
+    cam = ray;
+    cnt->paths += num_samples;
+    orc_hit h;
+    cnt->ext++;
+    trace(s, bv, &ray, 0, &h);
+    if (h.prim < 0) {
+        C3 L = c3(0.f, 0.f, 0.f); float alpha = 0.f;
+        for (int i = 0; i < s->n_lights; ++i)
+            if (s->lights[i].type == ORC_LIGHT_ENV) { L = cadd(L, light_eval(s, &s->lights[i], v3p(ray.d))); alpha = 1.f; }
+        out[0] = L.r; out[1] = L.g; out[2] = L.b; out[3] = alpha;
+        return;
+    }
+    cnt->hit++;
+    C3 res = c3(0.f, 0.f, 0.f);
+    for (int j = 0; j < num_samples; ++j) {
+        C3 Lo = path_from_hit(s, bv, ray, h, &cam, &rngs[j], cnt);
+        if (num_samples == 1) res = cadd(res, Lo);
+        else res = cadd(res, cdivf(Lo, (float)num_samples));
+    }
+    out[0] = res.r; out[1] = res.g; out[2] = res.b; out[3] = 1.f;
+}
+
+static void path_trace(const orc_scene *s, const orc_bvh *bv, int x, int y, const Rng *rng, float *out, orc_counters *cnt) {
+    camera_sample(s, bv, x, y, rng, 1, out, cnt);
 }
 
 void orc_path(const orc_scene *s, const orc_bvh *b, int x, int y, uint32_t seed, uint32_t sample,
@@ -1137,8 +1171,12 @@ void orc_render(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_p
             for (int y = y0; y < y0 + tile && y < s->height; ++y)
                 for (int x = x0; x < x0 + tile && x < s->width; ++x) {
                     float o[4];
-                    Rng r = make_rng(s, NULL, seed, x, y, (uint32_t)p);
-                    path_trace(s, b, x, y, &r, o, &lc);
+                    /* renderer.cpp:233, 257: one path in pass 0, LightSamples after */
+                    const int ls = s->lightsamples > 1 ? s->lightsamples : 1;
+                    const int n_sub = p == 0 ? 1 : ls;
+                    Rng rs[n_sub];
+                    for (int j = 0; j < n_sub; ++j) rs[j] = make_rng_sub(s, NULL, seed, x, y, (uint32_t)p, j, ls);
+                    camera_sample(s, b, x, y, rs, n_sub, o, &lc);
                     float *a = accum + 4 * ((size_t)y * s->width + x);
                     a[0] += o[0]; a[1] += o[1]; a[2] += o[2]; a[3] += o[3];
                 }

@@ -112,6 +112,7 @@ typedef struct {
     int n_textures;
     const orc_texture *textures;
     int sampler;           /* 0: hash of (seed, pixel, sample, slot); 1: Halton (oracle_halton.c) */
+    int lightsamples;      /* paths per camera sample after pass 0 */
 } orc_scene;
 
 typedef struct {

@@ -56,7 +56,7 @@ class OScene(C.Structure):
                 ("n_lights", C.c_int), ("lights", C.POINTER(OLight)), ("fov", C.c_float), ("width", C.c_int),
                 ("height", C.c_int), ("maxdepth", C.c_int), ("mindepth", C.c_int), ("uvs", C.POINTER(C.c_float)),
                 ("n_shaders", C.c_int), ("shaders", C.POINTER(OShader)), ("n_textures", C.c_int),
-                ("textures", C.POINTER(OTexture)), ("sampler", C.c_int)]
+                ("textures", C.POINTER(OTexture)), ("sampler", C.c_int), ("lightsamples", C.c_int)]
 
 
 class ORay(C.Structure):
@@ -244,6 +244,7 @@ class Scene:
         s.n_shaders, s.shaders = len(shs), C.cast(self.shaders, C.POINTER(OShader))
         s.n_textures, s.textures = len(texs), C.cast(self.textures, C.POINTER(OTexture))
         s.sampler = 1 if exp.get("sampler") == "halton" else 0
+        s.lightsamples = int(exp.get("lightsamples", 1))
         self.s = s
         self.bvh = lib().orc_bvh_build(C.byref(s), leaf) if bvh else None
 
