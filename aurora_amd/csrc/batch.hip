@@ -45,11 +45,27 @@ struct JobDev {
     float4 *accum;       // the job's framebuffer: sum of per-pass rgba
     uint32_t seed;
     int first_pass;
+    int ls;              // light samples: paths per pixel in passes after pass 0
     long long start;     // first work index of the job in launch order
     long long n_paths;
 };
 
 enum { J_EXT = 0, J_HIT = 1, J_SHADOW = 2, J_PATHS = 3, J_STRIDE = 4 };
+
+// A job's work is a sequence of blocks of w*h paths: pass 0 (when the job
+// starts there) is one block, every later pass ls blocks (one per light
+// sample).  Block -> pass offset from the job's first pass, and light sample.
+__device__ inline void job_block(const JobDev &jd, int blk, int *pass_rel, int *sub) {
+    int b = blk, p = 0;
+    if (jd.first_pass == 0) {
+        if (b == 0) { *pass_rel = 0; *sub = 0; return; }
+        b -= 1;
+        p = 1;
+    }
+    const int k = b / jd.ls;
+    *pass_rel = p + k;
+    *sub = b - k * jd.ls;
+}
 // control block: cursor, iterations done, the current iteration's plan, the
 // cursor after each of the last BATCH_HIST iterations and the wall clock at
 // the start of each of them
@@ -139,10 +155,12 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
         const JobDev &jd = bv.jobs[a];
         const long long local = w - jd.start;
         const int wh = jd.sv.width * jd.sv.height;
-        const int pass = (int)(local / wh);
-        const int l = (int)(local - (long long)pass * wh);
+        const int blk = (int)(local / wh);
+        const int l = (int)(local - (long long)blk * wh);
         const int pixel = tiled_pixel(l, jd.sv.width, jd.sv.height);
-        const PathRng rng = make_path_rng(jd.sv, nullptr, jd.seed, pixel, (uint32_t)(jd.first_pass + pass));
+        int pass, sub;
+        job_block(jd, blk, &pass, &sub);
+        const PathRng rng = make_path_rng(jd.sv, nullptr, jd.seed, pixel, (uint32_t)(jd.first_pass + pass), sub, jd.ls);
         const f3 d = camera_dir(jd.sv, pixel, rng.u(0), rng.u(1));
         const int pos = live + i;
         wv.ray[q][2 * pos] = make_float4(0.f, 0.f, 0.f, 0.1f);
@@ -150,7 +168,7 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
         wv.T[q][pos] = make_float4(1.f, 1.f, 1.f, 1.f);
         wv.La[q][pos] = make_float4(0.f, 0.f, 0.f, 0.f);
         wv.Lb[q][pos] = make_float4(0.f, 0.f, 0.f, 0.f);
-        wv.meta[q][pos] = make_int4(RT_CAMERA, l, pass, a);
+        wv.meta[q][pos] = make_int4(RT_CAMERA, l, blk, a);
     }
 }
 
@@ -212,9 +230,11 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__res
                 if (job == j) {
                     const JobDev &jd = jobs[j];
                     shade_compute<TEX>(jd.sv, wv, q, i, [&](int4 m) {
+                        int pass, sub;
+                        job_block(jd, m.z, &pass, &sub);
                         return make_path_rng<TEX>(jd.sv, nullptr, jd.seed,
                                                   tiled_pixel(m.y, jd.sv.width, jd.sv.height),
-                                                  (uint32_t)(jd.first_pass + m.z));
+                                                  (uint32_t)(jd.first_pass + pass), sub, jd.ls);
                     }, j, o);
                     break;
                 }
@@ -260,8 +280,15 @@ __global__ void __launch_bounds__(256) k_bfinish(BatchView bv, WfView wv) {
         const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
         const JobDev &jd = bv.jobs[m.w];
         const long long w = (jd.start + (long long)m.z * (jd.sv.width * jd.sv.height) + m.y) & bv.ring_mask;
-        bv.ring[w] = make_float4(L.x, L.y, L.z, a.w);
-        bv.ring_cnt[w] = pack_counts(path_ext_rays(m.x), path_hit_rays(m.x), path_shadow_rays(m.x), iter);
+        // a camera miss is marked by alpha + 2 (the resolve takes it once); a
+        // pass's later light samples do not count the shared camera ray
+        const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
+        int pass, sub;
+        job_block(jd, m.z, &pass, &sub);
+        const int dup = sub > 0 ? 1 : 0;
+        bv.ring[w] = make_float4(L.x, L.y, L.z, a.w + (cam_miss ? 2.f : 0.f));
+        bv.ring_cnt[w] = pack_counts(path_ext_rays(m.x) - dup, path_hit_rays(m.x) - (dup && !cam_miss ? 1 : 0),
+                                     path_shadow_rays(m.x), iter);
     }
 }
 
@@ -314,6 +341,10 @@ __global__ void __launch_bounds__(256) k_bstats(BatchView bv, const StatChunk *_
 
 // Adds the finished ring entries of the given work ranges to the jobs'
 // framebuffers: one thread per (segment, pixel), passes in increasing order.
+// Adds the job-local work range [la, lb) -- whole passes -- into the job's
+// framebuffer, per pixel in pass order: a pass contributes the sum of its
+// light samples times 1/ls in sample order, or its first path alone where the
+// camera ray missed (integrator.cpp:176-292).
 __global__ void k_bresolve(BatchView bv, const ResolveSeg *__restrict__ segs, int n_segs, long long n_threads) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_threads; t += stride) {
@@ -327,18 +358,31 @@ __global__ void k_bresolve(BatchView bv, const ResolveSeg *__restrict__ segs, in
         const JobDev &jd = bv.jobs[sg.job];
         const int wh = jd.sv.width * jd.sv.height;
         const long long l = t - sg.tbase;
-        const long long p0 = sg.la / wh, p1 = (sg.lb - 1) / wh;
         float4 *px = jd.accum + tiled_pixel((int)l, jd.sv.width, jd.sv.height);
         float4 s = *px;
-        bool any = false;
-        for (long long p = p0; p <= p1; ++p) {
-            const long long idx = p * wh + l;
-            if (idx < sg.la || idx >= sg.lb) continue;
-            const float4 v = bv.ring[(jd.start + idx) & bv.ring_mask];
+        const int blk1 = (int)(sg.lb / wh);
+        for (int blk = (int)(sg.la / wh); blk < blk1;) {
+            int pass, sub;
+            job_block(jd, blk, &pass, &sub);
+            const int n_sub = (jd.first_pass + pass == 0) ? 1 : jd.ls;
+            const float4 v0 = bv.ring[(jd.start + (long long)blk * wh + l) & bv.ring_mask];
+            float4 v;
+            if (v0.w >= 2.f) {
+                v = make_float4(v0.x, v0.y, v0.z, v0.w - 2.f);
+            } else if (n_sub == 1) {
+                v = v0;
+            } else {
+                const float inv = 1.f / (float)n_sub;
+                v = make_float4(0.f, 0.f, 0.f, v0.w);
+                for (int j = 0; j < n_sub; ++j) {
+                    const float4 u = bv.ring[(jd.start + (long long)(blk + j) * wh + l) & bv.ring_mask];
+                    v.x += u.x * inv; v.y += u.y * inv; v.z += u.z * inv;
+                }
+            }
             s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
-            any = true;
+            blk += n_sub;
         }
-        if (any) *px = s;
+        *px = s;
     }
 }
 
@@ -388,19 +432,33 @@ static double job_cost(const Ctx *j, long long paths) {
 // Adds ring entries [from, to) (work indices) into the framebuffers and the
 // per-job statistics.
 #define STAT_CHUNK 65536
+// The largest pass boundary of a job at or below job-local work index x.
+static long long pass_floor(long long x, long long wh, int first_pass, int ls) {
+    long long blocks = x / wh;
+    if (first_pass == 0) {
+        if (blocks == 0) return 0;
+        return (1 + ((blocks - 1) / ls) * ls) * wh;
+    }
+    return (blocks / ls) * ls * wh;
+}
+
+// Resolves every job's whole passes finished below global work index `to`
+// (its frontier job_done moves pass by pass) and sums their statistics.
 static void resolve_range(Ctx *c, const BatchView &bv, const std::vector<long long> &starts,
-                          const std::vector<long long> &npaths, const std::vector<int> &wh, long long from, long long to) {
+                          const std::vector<long long> &npaths, const std::vector<int> &wh,
+                          const std::vector<int> &jfirst, const std::vector<int> &jls, std::vector<long long> &job_done,
+                          long long to) {
     BatchState &b = c->batch;
-    if (to <= from) return;
     ResolveSeg *seg = static_cast<ResolveSeg *>(b.host_segs);
     StatChunk *chk = static_cast<StatChunk *>(b.host_chunks);
     int n = 0, nc = 0;
     long long threads = 0;
-    int r = (int)(std::upper_bound(starts.begin(), starts.end(), from) - starts.begin()) - 1;
-    for (r = std::max(r, 0); r < (int)starts.size() && starts[r] < to; ++r) {
-        const long long la = std::max(from, starts[r]) - starts[r];
-        const long long lb = std::min(to, starts[r] + npaths[r]) - starts[r];
+    for (int r = 0; r < (int)starts.size() && starts[r] < to; ++r) {
+        const long long la = job_done[r];
+        const long long lraw = std::min(to, starts[r] + npaths[r]) - starts[r];
+        const long long lb = lraw >= npaths[r] ? npaths[r] : pass_floor(lraw, wh[r], jfirst[r], jls[r]);
         if (la >= lb) continue;
+        job_done[r] = lb;
         seg[n].la = la;
         seg[n].lb = lb;
         seg[n].tbase = threads;
@@ -430,6 +488,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
               int iters_per_poll, aur_job_result *results, aur_stats *total_stats) {
     if (n_jobs <= 0) throw std::invalid_argument("no jobs");
     int width = 0, depth = 0, max_path = 0;
+    long long max_pass = 0;  // work items of the largest pass
     bool tex = false;
     std::vector<long long> paths(n_jobs);
     for (int k = 0; k < n_jobs; ++k) {
@@ -437,14 +496,16 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         if (!j->built) throw StateError("job " + std::to_string(k) + ": scene not built (aur_build_accel)");
         if (j->device != c->device) throw std::invalid_argument("job " + std::to_string(k) + " is on another device");
         if (first_pass[k] < 0 || n_passes[k] < 0) throw std::invalid_argument("negative pass range");
-        if (j->scene.opt.lightsamples != 1)
-            throw std::invalid_argument("job " + std::to_string(k) + ": batched jobs need lightsamples = 1");
+        if (j->scene.opt.lightsamples < 1)
+            throw std::invalid_argument("job " + std::to_string(k) + ": lightsamples must be at least 1");
         if (width == 0) width = j->bvh.width;
         if (j->bvh.width != width) throw std::invalid_argument("all jobs of a batch need the same BVH width");
         depth = std::max(depth, j->bvh.depth);
         max_path = std::max(max_path, j->scene.opt.maxdepth + 1);
         tex = tex || shade_needs_tex(j);
-        paths[k] = (long long)j->scene.opt.width * j->scene.opt.height * n_passes[k];
+        paths[k] = (long long)pass_paths((size_t)j->scene.opt.width * j->scene.opt.height, first_pass[k], n_passes[k],
+                                         j->scene.opt.lightsamples);
+        max_pass = std::max(max_pass, (long long)j->scene.opt.width * j->scene.opt.height * j->scene.opt.lightsamples);
     }
     // launch order
     std::vector<int> ord(n_jobs);
@@ -454,7 +515,8 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     // job records and framebuffers
     std::vector<JobDev> jd(n_jobs);
     std::vector<long long> starts(n_jobs), npaths(n_jobs);
-    std::vector<int> wh(n_jobs);
+    std::vector<int> wh(n_jobs), jfirst(n_jobs), jls(n_jobs);
+    std::vector<long long> job_done(n_jobs, 0);  // resolved work prefix of each job
     long long total = 0;
     for (int r = 0; r < n_jobs; ++r) {
         Ctx *j = jobs[ord[r]];
@@ -470,6 +532,9 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         jd[r].accum = j->accum.as<float4>();
         jd[r].seed = j->seed;
         jd[r].first_pass = first_pass[ord[r]];
+        jd[r].ls = j->scene.opt.lightsamples;
+        jfirst[r] = jd[r].first_pass;
+        jls[r] = jd[r].ls;
         jd[r].start = total;
         jd[r].n_paths = paths[ord[r]];
         starts[r] = total;
@@ -482,9 +547,10 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     // iterations between polls, and the lag after which issued work has finished
     const int lag = max_path + 1;
     const int pairs_per_poll = std::max(1, std::min((iters_per_poll > 0 ? iters_per_poll : 8), BATCH_HIST - lag - 4) / 2);
-    // the ring holds every item between the resolved prefix and the cursor
+    // the ring holds every item between the resolved prefix and the cursor;
+    // a job resolves whole passes, so its prefix may trail by one pass
     size_t ring = 1;
-    while (ring < pool * (size_t)(2 * pairs_per_poll + lag + 2)) ring <<= 1;
+    while (ring < pool * (size_t)(2 * pairs_per_poll + lag + 2) + (size_t)max_pass) ring <<= 1;
     ensure_wavefront_for(c, pool, width, depth);
     BatchState &b = c->batch;
     b.jobs.reserve(sizeof(JobDev) * n_jobs);
@@ -581,7 +647,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
             else if (iters - 1 - lag >= 0) safe = hs[CTL_HIST + (int)((iters - 1 - lag) % BATCH_HIST)];
             if (done) hipLaunchKernelGGL(k_bstamp, dim3(1), dim3(64), 0, c->stream, bv);
             if (safe > resolved) {
-                resolve_range(c, bv, starts, npaths, wh, resolved, safe);
+                resolve_range(c, bv, starts, npaths, wh, jfirst, jls, job_done, safe);
                 resolved = safe;
             }
             if (done) break;

@@ -86,3 +86,36 @@ def test_light_samples_halton(tmp_path, oracle):
     oacc, _ = s.render(0, 0, 3)
     _match(acc, oacc)
     r.close()
+
+
+def test_light_samples_in_batched_launch(tmp_path):
+    """Jobs with 1, 2 and 3 light samples (one starting after pass 0) in one
+    persistent launch with a small pool: framebuffers bit-identical to
+    per-scene renders, ray counts equal."""
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer, batch_submit
+    rs, refs, jobs = [], [], []
+    for k, (L, first, n) in enumerate(((1, 0, 3), (2, 0, 3), (3, 1, 2))):
+        d = str(tmp_path / ("job%d" % k))
+        r = _renderer(d, scenes.box_room(res=48 + 16 * k, spp=4, maxdepth=3, mindepth=3, seed=k), L)
+        r.set_seed(50 + k)
+        if first > 0:
+            r.render_passes(0, first, fetch=False)
+            base = r.framebuffer()
+        acc, st = r.render_passes(first, n)
+        refs.append((acc, st, base if first > 0 else None))
+        rs.append(r)
+        jobs.append((r, first, n))
+    ex = Renderer()
+    ex.set_launch(pool_paths=1 << 14)
+    for r, (acc, st, base) in zip(rs, refs):
+        if base is not None:  # restore the framebuffer of the passes before the batch
+            r.render_passes(0, jobs[rs.index(r)][1], fetch=False)
+    res, tot = batch_submit(ex, jobs)
+    for r, (acc, st, base), o in zip(rs, refs, res):
+        assert np.array_equal(r.framebuffer(), acc)
+        for key in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
+            assert o[key] == st[key], key
+    for r in rs:
+        r.close()
+    ex.close()
