@@ -95,10 +95,12 @@ __device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4
         load_node<W>(nodes, node, r);
         // exact-t ties must still be visited, so the far bound is widened slightly
         const float tfar_lim = h.t * 1.0000004f;
-        // entry distance per child (inf: missed or a leaf handled right here);
-        // every index below is a compile-time constant, so the lists stay in VGPRs
+        // entry distance per inner child (inf: missed, or a leaf: leaves are
+        // collected in a mask and tested after the box tests); every index
+        // below is a compile-time constant, so the lists stay in VGPRs
         float tn[W];
         int ch[W];
+        unsigned leaves = 0;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             float x0 = fmaf(r.lox[k], ix, -ox), x1 = fmaf(r.hix[k], ix, -ox);
@@ -110,9 +112,17 @@ __device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4
             bool hit = a <= b;
             tn[k] = (hit && c >= 0) ? a : inf;
             ch[k] = c;
-            if (hit && c < 0) {
-                if (test_leaf<ANY>(tris, c, o, d, tmin, h)) return;
-            }
+            if (hit && c < 0) leaves |= 1u << k;
+        }
+        // the leaves this node's boxes let through, one per pass of the loop:
+        // lanes with leaves in different slots test them together
+        while (leaves) {
+            const int k = __builtin_ctz(leaves);
+            leaves &= leaves - 1;
+            int c = ch[0];
+#pragma unroll
+            for (int j = 1; j < W; ++j) c = (k == j) ? ch[j] : c;
+            if (test_leaf<ANY>(tris, c, o, d, tmin, h)) return;
         }
         if (!ANY) {
             // sorting network on (entry distance, child): nearest first
