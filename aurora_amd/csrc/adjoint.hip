@@ -249,9 +249,7 @@ __global__ void __launch_bounds__(128) k_adj_forward(SceneView sv, AdjParams ap,
     __shared__ int s_stack[AUR_STACK_LDS * 128];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TravStack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = 128;
-    st.ovf = ovf + (size_t)gtid * ovf_per_thread;
+    st.init(s_stack, 128, ovf, ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     GradSink g;
     g.albedo = g.rough = g.spec = nullptr;
@@ -280,9 +278,7 @@ __global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, co
     extern __shared__ float s_grad[];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TravStack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = 128;
-    st.ovf = ovf + (size_t)gtid * ovf_per_thread;
+    st.init(s_stack, 128, ovf, ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     const int n_alb = ap.g_albedo ? 3 * ap.n_tris : 0;
     GradSink g;

@@ -182,11 +182,11 @@ __global__ void k_bclear_queue(int *counts, int q) {
     if (threadIdx.x == 0) counts[q] = 0;
 }
 
-// The kernels below run each wave once per distinct job among its lanes
-// (almost always one): the job index is made wave-uniform with
-// readfirstlane, so the scene record is read with scalar loads and the
-// traversal and shading code sees uniform scene pointers, as in the
-// single-scene kernels.
+// Batched extension: the single-scene kernel's refilled lanes, each lane with
+// its own ray's scene (the BVH and triangle pointers of the job are per-lane
+// values, read when the ray is fetched).  The four pointer registers take it
+// to 72 VGPRs (7 waves per SIMD); held to 64 it spills inside the loop, which
+// measured slower.
 template <int W, int B>
 __global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, WfView wv, int q) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
@@ -194,24 +194,24 @@ __global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, 
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = B;
-    st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
-    for (int i = gtid; i < n; i += stride) {
-        const int job = wv.meta[q][i].w;
-        const float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
-        TravHit h;
-        for (;;) {
-            const int j = __builtin_amdgcn_readfirstlane(job);
-            if (job == j) {
-                traverse<W, false>(static_cast<const WNode<W> *>(jobs[j].sv.nodes), jobs[j].sv.tris, mk3(a.x, a.y, a.z),
-                                   mk3(b.x, b.y, b.z), a.w, b.w, st, h);
-                break;
-            }
-        }
-        wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
-    }
+    st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
+    traverse_stream<W, false, true>(
+        nullptr, nullptr, st, gtid, n, stride, AUR_REFILL_LANES,
+        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, auto &nodes, auto &tris) {
+            const JobDev &jd = jobs[wv.meta[q][i].w];
+            nodes = static_cast<const WNode<W> *>(jd.sv.nodes);
+            tris = jd.sv.tris;
+            const float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
+            o = mk3(a.x, a.y, a.z); d = mk3(b.x, b.y, b.z); tmin = a.w; tmax = b.w;
+        },
+        [&](int i, const TravHit &h) { wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim)); });
 }
+
+// The kernels below run each wave once per distinct job among its lanes
+// (almost always one): the job index is made wave-uniform with
+// readfirstlane, so the scene record is read with scalar loads and the
+// traversal and shading code sees uniform scene pointers, as in the
+// single-scene kernels.
 
 template <bool TEX>
 __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__restrict__ jobs, WfView wv, int q) {
@@ -244,29 +244,27 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__res
     }
 }
 
+// Batched shadow rays: per-lane scene pointers (no per-job waterfall), and the
+// contribution record is read only for unoccluded rays, which keeps the kernel
+// at 64 VGPRs without spills.
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_bshadow(const JobDev *__restrict__ jobs, WfView wv, int nq) {
+__global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bshadow(const JobDev *__restrict__ jobs, WfView wv, int nq) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = B;
-    st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
+    st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
     for (int i = gtid; i < n; i += stride) {
-        const float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
-        const int job = __float_as_int(a.w);
+        const float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1];
+        const JobDev &jd = jobs[__float_as_int(a.w)];
         TravHit h;
-        for (;;) {
-            const int j = __builtin_amdgcn_readfirstlane(job);
-            if (job == j) {
-                traverse<W, true>(static_cast<const WNode<W> *>(jobs[j].sv.nodes), jobs[j].sv.tris, mk3(a.x, a.y, a.z),
-                                  mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h);
-                break;
-            }
+        traverse<W, true>(static_cast<const WNode<W> *>(jd.sv.nodes), jd.sv.tris, mk3(a.x, a.y, a.z),
+                          mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h);
+        if (h.prim < 0) {
+            const float4 c = wv.sray[3 * i + 2];
+            shadow_add(wv, nq, __float_as_int(c.w), c);
         }
-        if (h.prim < 0) shadow_add(wv, nq, __float_as_int(c.w), c);
     }
 }
 

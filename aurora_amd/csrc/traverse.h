@@ -13,9 +13,20 @@
 
 namespace aur {
 
+// 20 entries x 4 B x 64 lanes = 5 KB of LDS per wave: 32 waves fit in a CU's
+// 160 KB, so LDS does not cap the traversal kernels' occupancy
 #ifndef AUR_STACK_LDS
-#define AUR_STACK_LDS 24
+#define AUR_STACK_LDS 20
 #endif
+
+// Minimum waves per SIMD the traversal kernels are compiled for: 8 (64 VGPRs)
+// for 4-wide nodes; 8-wide nodes need more registers than that without
+// spilling, so they are left to the compiler
+#define AUR_TRAV_WAVES(W) ((W) == 4 ? 8 : 1)
+
+// A lane whose ray is done waits for this many lanes of its wave before they
+// fetch new rays together (traverse_stream)
+#define AUR_REFILL_LANES 8
 
 struct TravHit { float t, u, v; int prim; };
 
@@ -43,20 +54,40 @@ __device__ inline void load_node(const WNode<W> *__restrict__ nodes, int ni, Nod
     }
 }
 
-// Stack: top STACK_LDS entries per lane in LDS, deeper entries in global memory.
+// Stack: top STACK_LDS entries per lane in LDS, deeper entries in global
+// memory.  Only the lane's LDS address and depth are per-lane values; the
+// overflow address is derived from them when needed, so the stack holds two
+// VGPRs across the traversal loop.
 struct TravStack {
-    int *lds;       // base for this lane: lds[k * stride]
-    int stride;
-    int *ovf;       // this thread's overflow area
+    int *lds_base;  // the workgroup's stack array
+    int *lds;       // this lane's entry 0: entry k is lds[k * stride]
+    int stride;     // lanes per workgroup
+    int *ovf;       // the workgroup's overflow area: lane t owns ovf[t * ovf_per_thread ...]
+    int ovf_per_thread;
     int sp;
+    __device__ inline void init(int *s_stack, int lanes, int *ovf_all, int per_thread) {
+        lds_base = s_stack;
+        lds = s_stack + threadIdx.x;
+        stride = lanes;
+        ovf = ovf_all + (size_t)blockIdx.x * lanes * per_thread;
+        ovf_per_thread = per_thread;
+    }
+    __device__ inline int *ovf_entry(int k) const {
+        int lane = (int)(lds - lds_base);
+        // an empty asm makes `lane` opaque here, so the 64-bit overflow
+        // address is not hoisted out of the traversal loop into two more
+        // registers: it is rebuilt on this rarely taken path instead
+        asm volatile("" : "+v"(lane));
+        return ovf + (size_t)lane * ovf_per_thread + (k - AUR_STACK_LDS);
+    }
     __device__ inline void push(int v) {
         if (sp < AUR_STACK_LDS) lds[sp * stride] = v;
-        else ovf[sp - AUR_STACK_LDS] = v;
+        else *ovf_entry(sp) = v;
         ++sp;
     }
     __device__ inline int pop() {
         --sp;
-        return sp < AUR_STACK_LDS ? lds[sp * stride] : ovf[sp - AUR_STACK_LDS];
+        return sp < AUR_STACK_LDS ? lds[sp * stride] : *ovf_entry(sp);
     }
 };
 
@@ -77,79 +108,156 @@ __device__ inline bool test_leaf(const float4 *__restrict__ tris, int code, f3 o
     return false;
 }
 
-template <int W, bool ANY>
-__device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4 *__restrict__ tris, f3 o, f3 d,
-                                float tmin, float tmax, TravStack &st, TravHit &h) {
-    h.t = tmax; h.u = 0.f; h.v = 0.f; h.prim = -1;
-    // reciprocal direction for the slab test (zero components nudged so no 0*inf appears)
+// Per-ray constants of the slab test and the triangle test.
+struct TravRay {
+    f3 o, d;
+    float ix, iy, iz;   // reciprocal direction (zero components nudged so no 0*inf appears)
+    float ox, oy, oz;   // origin times reciprocal direction
+    float tmin;
+};
+
+__device__ inline void trav_setup(TravRay &r, f3 o, f3 d, float tmin, float tmax, TravHit &h) {
+    r.o = o; r.d = d; r.tmin = tmin;
     float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
     float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
     float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
-    const float ix = 1.f / dx, iy = 1.f / dy, iz = 1.f / dz;
-    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    r.ix = 1.f / dx; r.iy = 1.f / dy; r.iz = 1.f / dz;
+    r.ox = o.x * r.ix; r.oy = o.y * r.iy; r.oz = o.z * r.iz;
+    h.t = tmax; h.u = 0.f; h.v = 0.f; h.prim = -1;
+}
+
+__device__ inline void trav_begin(TravRay &r, f3 o, f3 d, float tmin, float tmax, TravStack &st, int &node,
+                                  TravHit &h) {
+    trav_setup(r, o, d, tmin, tmax, h);
     st.sp = 0;
-    int node = 0;
+    node = 0;
+}
+
+// Visits one node: tests its child boxes, tests the leaves they let through,
+// pushes the inner children to visit later and moves `node` to the next one.
+// Returns true when the ray is done (stack empty, or an any-hit query hit).
+template <int W, bool ANY>
+__device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restrict__ tris, const TravRay &r,
+                                  TravStack &st, int &node, TravHit &h) {
     const float inf = __int_as_float(0x7f800000);
+    // exact-t ties must still be visited, so the far bound is widened slightly
+    const float tfar_lim = h.t * 1.0000004f;
+    // entry distance per inner child (inf: missed, or a leaf: leaves are
+    // collected in a mask and tested after the box tests); every index
+    // below is a compile-time constant, so the lists stay in VGPRs
+    float tn[W];
+    int ch[W];
+    unsigned leaves = 0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        float x0 = fmaf(nr.lox[k], r.ix, -r.ox), x1 = fmaf(nr.hix[k], r.ix, -r.ox);
+        float y0 = fmaf(nr.loy[k], r.iy, -r.oy), y1 = fmaf(nr.hiy[k], r.iy, -r.oy);
+        float z0 = fmaf(nr.loz[k], r.iz, -r.oz), z1 = fmaf(nr.hiz[k], r.iz, -r.oz);
+        float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+        float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
+        int c = nr.child[k];
+        bool hit = a <= b;
+        tn[k] = (hit && c >= 0) ? a : inf;
+        ch[k] = c;
+        if (hit && c < 0) leaves |= 1u << k;
+    }
+    // the leaves this node's boxes let through, one per pass of the loop:
+    // lanes with leaves in different slots test them together
+    while (leaves) {
+        const int k = __builtin_ctz(leaves);
+        leaves &= leaves - 1;
+        int c = ch[0];
+#pragma unroll
+        for (int j = 1; j < W; ++j) c = (k == j) ? ch[j] : c;
+        if (test_leaf<ANY>(tris, c, r.o, r.d, r.tmin, h)) return true;
+    }
+    if (!ANY) {
+        // sorting network on (entry distance, child): nearest first
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+#pragma unroll
+            for (int j = (i & 1); j + 1 < W; j += 2) {
+                bool sw = tn[j] > tn[j + 1];
+                float ta = sw ? tn[j + 1] : tn[j], tb = sw ? tn[j] : tn[j + 1];
+                int ca = sw ? ch[j + 1] : ch[j], cb = sw ? ch[j] : ch[j + 1];
+                tn[j] = ta; tn[j + 1] = tb; ch[j] = ca; ch[j + 1] = cb;
+            }
+        }
+    }
+    // children entered beyond the (possibly shortened) hit distance are skipped
+    const float tcut = h.t * 1.0000004f;
+    int next = -1;
+#pragma unroll
+    for (int k = W - 1; k >= 0; --k) {
+        if (tn[k] <= tcut) {
+            if (next >= 0) st.push(next);
+            next = ch[k];
+        }
+    }
+    if (next >= 0) { node = next; return false; }
+    if (st.sp == 0) return true;
+    node = st.pop();
+    return false;
+}
+
+template <int W, bool ANY>
+__device__ inline bool trav_step(const WNode<W> *__restrict__ nodes, const float4 *__restrict__ tris,
+                                 const TravRay &r, TravStack &st, int &node, TravHit &h) {
+    NodeRegs<W> nr;
+    load_node<W>(nodes, node, nr);
+    return trav_visit<W, ANY>(nr, tris, r, st, node, h);
+}
+
+// One whole ray.
+template <int W, bool ANY>
+__device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4 *__restrict__ tris, f3 o, f3 d,
+                                float tmin, float tmax, TravStack &st, TravHit &h) {
+    TravRay r;
+    int node;
+    trav_begin(r, o, d, tmin, tmax, st, node, h);
+    while (!trav_step<W, ANY>(nodes, tris, r, st, node, h)) {
+    }
+}
+
+// Persistent traversal of rays first, first + stride, ... below n with lanes
+// refilled independently: a lane whose ray is done waits (masked) until at
+// least `refill` lanes of its wave are waiting or no lane is busy, then the
+// waiting lanes fetch their next rays together.  The wave keeps stepping
+// while any lane has a ray, so one long ray no longer holds 63 idle lanes,
+// and the fetch cost (a ray load and three divisions) is paid once per group
+// of lanes instead of once per lane.  `load(i, o, d, tmin, tmax)` reads ray i;
+// `done(i, h)` consumes its result.  With PER_LANE, `load(i, o, d, tmin,
+// tmax, nodes, tris)` also points `nodes` and `tris` at the ray's own scene
+// (batched launches).  The per-ray results are those of `traverse`: each
+// ray's visit order does not depend on its neighbours.
+template <int W, bool ANY, bool PER_LANE = false, class Load, class Done>
+__device__ inline void traverse_stream(const WNode<W> *__restrict__ nodes, const float4 *__restrict__ tris,
+                                       TravStack &st, int first, int n, int stride, int refill, Load load,
+                                       Done done) {
+    TravRay r;
+    TravHit h;
+    int node = 0;
+    int i = first;
+    bool busy = false;
     for (;;) {
-        NodeRegs<W> r;
-        load_node<W>(nodes, node, r);
-        // exact-t ties must still be visited, so the far bound is widened slightly
-        const float tfar_lim = h.t * 1.0000004f;
-        // entry distance per inner child (inf: missed, or a leaf: leaves are
-        // collected in a mask and tested after the box tests); every index
-        // below is a compile-time constant, so the lists stay in VGPRs
-        float tn[W];
-        int ch[W];
-        unsigned leaves = 0;
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            float x0 = fmaf(r.lox[k], ix, -ox), x1 = fmaf(r.hix[k], ix, -ox);
-            float y0 = fmaf(r.loy[k], iy, -oy), y1 = fmaf(r.hiy[k], iy, -oy);
-            float z0 = fmaf(r.loz[k], iz, -oz), z1 = fmaf(r.hiz[k], iz, -oz);
-            float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-            float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
-            int c = r.child[k];
-            bool hit = a <= b;
-            tn[k] = (hit && c >= 0) ? a : inf;
-            ch[k] = c;
-            if (hit && c < 0) leaves |= 1u << k;
-        }
-        // the leaves this node's boxes let through, one per pass of the loop:
-        // lanes with leaves in different slots test them together
-        while (leaves) {
-            const int k = __builtin_ctz(leaves);
-            leaves &= leaves - 1;
-            int c = ch[0];
-#pragma unroll
-            for (int j = 1; j < W; ++j) c = (k == j) ? ch[j] : c;
-            if (test_leaf<ANY>(tris, c, o, d, tmin, h)) return;
-        }
-        if (!ANY) {
-            // sorting network on (entry distance, child): nearest first
-#pragma unroll
-            for (int i = 0; i < W; ++i) {
-#pragma unroll
-                for (int j = (i & 1); j + 1 < W; j += 2) {
-                    bool sw = tn[j] > tn[j + 1];
-                    float ta = sw ? tn[j + 1] : tn[j], tb = sw ? tn[j] : tn[j + 1];
-                    int ca = sw ? ch[j + 1] : ch[j], cb = sw ? ch[j] : ch[j + 1];
-                    tn[j] = ta; tn[j + 1] = tb; ch[j] = ca; ch[j + 1] = cb;
-                }
+        const unsigned long long want = __ballot(!busy && i < n);
+        const unsigned long long working = __ballot(busy);
+        if (!want && !working) break;
+        if (want && (!working || __popcll(want) >= (unsigned)refill)) {
+            if (!busy && i < n) {
+                f3 o, d;
+                float tmin, tmax;
+                if constexpr (PER_LANE) load(i, o, d, tmin, tmax, nodes, tris);
+                else load(i, o, d, tmin, tmax);
+                trav_begin(r, o, d, tmin, tmax, st, node, h);
+                busy = true;
             }
         }
-        // children entered beyond the (possibly shortened) hit distance are skipped
-        const float tcut = h.t * 1.0000004f;
-        int next = -1;
-#pragma unroll
-        for (int k = W - 1; k >= 0; --k) {
-            if (tn[k] <= tcut) {
-                if (next >= 0) st.push(next);
-                next = ch[k];
-            }
+        if (busy && trav_step<W, ANY>(nodes, tris, r, st, node, h)) {
+            done(i, h);
+            busy = false;
+            i += stride;
         }
-        if (next >= 0) { node = next; continue; }
-        if (st.sp == 0) break;
-        node = st.pop();
     }
 }
 

@@ -82,28 +82,34 @@ __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
 }
 
 // ---------------------------------------------------------------------------
-// extend: closest hit
+// extend: closest hit.  Lanes are refilled independently (traverse_stream):
+// closest-hit traversals of incoherent rays differ in length several times
+// over, and a wave that waited for its longest ray left most lanes idle.
+// Both traversal kernels are held to 64 VGPRs (8 waves per SIMD) and 5 KB of
+// LDS per wave, so a CU keeps its full 32 waves of dependent node fetches in
+// flight.
 // ---------------------------------------------------------------------------
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_extend(SceneView sv, WfView wv, int q) {
+__global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_extend(SceneView sv, WfView wv, int q) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_EXT], (unsigned long long)n);
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = B;
-    st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
+    st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     int hits = 0;
-    for (int i = gtid; i < n; i += stride) {
-        float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
-        TravHit h;
-        traverse<W, false>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
-        wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
-        hits += h.prim >= 0 ? 1 : 0;
-    }
+    traverse_stream<W, false>(
+        nodes, sv.tris, st, gtid, n, stride, AUR_REFILL_LANES,
+        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
+            const float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
+            o = mk3(a.x, a.y, a.z); d = mk3(b.x, b.y, b.z); tmin = a.w; tmax = b.w;
+        },
+        [&](int i, const TravHit &h) {
+            wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
+            hits += h.prim >= 0 ? 1 : 0;
+        });
     // one add per wave into this workgroup's partial (no global same-address atomics)
     for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
     if (__lane_id() == 0 && hits) atomicAdd(&wv.partials[blockIdx.x], (unsigned long long)hits);
@@ -111,19 +117,18 @@ __global__ void __launch_bounds__(B) k_extend(SceneView sv, WfView wv, int q) {
 
 // ---------------------------------------------------------------------------
 // shadow: any hit; unoccluded contributions go to La (light sample) or Lb
-// (BSDF sample)
+// (BSDF sample).  Any-hit traversals are short and end at the first hit; here
+// the per-lane refill measured slower than whole-wave iteration.
 // ---------------------------------------------------------------------------
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_shadow(SceneView sv, WfView wv, int nq) {
+__global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_shadow(SceneView sv, WfView wv, int nq) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&wv.stats[S_SHADOW], (unsigned long long)n);
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = B;
-    st.ovf = wv.ovf + (size_t)gtid * wv.ovf_per_thread;
+    st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     for (int i = gtid; i < n; i += stride) {
         float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
@@ -532,9 +537,7 @@ __global__ void __launch_bounds__(B) k_trace(SceneView sv, const aur_ray *__rest
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TravStack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = B;
-    st.ovf = ovf + (size_t)gtid * ovf_per_thread;
+    st.init(s_stack, B, ovf, ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     for (int i = gtid; i < n; i += gridDim.x * blockDim.x) {
         aur_ray r = rays[i];
