@@ -5,12 +5,12 @@
 //   1. per-triangle padded bounds and centroid bounds (k_prim_bounds)
 //   2. 63-bit Morton codes of the centroids (k_morton)
 //   3. radix sort of (code, triangle) pairs (hipCUB)
-//   4. binary radix tree over the sorted codes (k_karras; Karras 2012,
-//      "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d
-//      Trees")
-//   5. bottom-up bounds (k_refit), one thread per leaf, agent-scope
-//      acquire/release atomics so the last arriver at a node sees both
-//      children's boxes on every XCD
+//   4. binary tree by parallel locally-ordered clustering over the sorted
+//      triangles (PLOC, k_ploc_*; Meister & Bittner 2018): the Morton order
+//      only seeds the clusters, merges pick the smallest union surface area
+//      among 16 neighbours on either side, inner boxes come from the merges
+//   5. leaves renumbered in the tree's left-to-right order (k_ploc_place,
+//      k_ploc_permute), so every subtree covers a contiguous triangle range
 //   6. top-down collapse into W-wide nodes (k_collapse), level by level: each
 //      wide node opens the child with the largest surface area until it has W
 //      children; a subtree with <= max_leaf triangles becomes one leaf (a
@@ -102,46 +102,6 @@ __global__ void k_morton(const float4 *__restrict__ lo, const float4 *__restrict
     vals[i] = i;
 }
 
-__device__ inline int delta(const unsigned long long *keys, int n, int i, int j) {
-    if (j < 0 || j > n - 1) return -1;
-    unsigned long long a = keys[i], b = keys[j];
-    if (a == b) return 64 + __clz((unsigned)(i ^ j));
-    return __clzll(a ^ b);
-}
-
-// Karras 2012, algorithm 3: inner node i of n-1 covers [first, last]
-__global__ void k_karras(const unsigned long long *__restrict__ keys, int n, int *__restrict__ left, int *__restrict__ right,
-                         int *__restrict__ first, int *__restrict__ last, int *__restrict__ parent_inner,
-                         int *__restrict__ parent_leaf) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    int dmin = delta(keys, n, i, i - d);
-    int lmax = 2;
-    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = delta(keys, n, i, j);
-    int s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    int gamma = i + s * d + min(d, 0);
-    int lo = min(i, j), hi = max(i, j);
-    int lc = (lo == gamma) ? (int)(gamma | LEAF_BIT) : gamma;
-    int rc = (hi == gamma + 1) ? (int)((gamma + 1) | LEAF_BIT) : gamma + 1;
-    left[i] = lc;
-    right[i] = rc;
-    first[i] = lo;
-    last[i] = hi;
-    if (lc & LEAF_BIT) parent_leaf[gamma] = i; else parent_inner[gamma] = i;
-    if (rc & LEAF_BIT) parent_leaf[gamma + 1] = i; else parent_inner[gamma + 1] = i;
-    if (i == 0) parent_inner[0] = -1;
-}
-
 __device__ inline void child_box(int code, const float4 *plo, const float4 *phi, const int *vals, const float4 *nlo,
                                  const float4 *nhi, float4 *l, float4 *h) {
     if (code & LEAF_BIT) {
@@ -149,24 +109,6 @@ __device__ inline void child_box(int code, const float4 *plo, const float4 *phi,
         *l = plo[p]; *h = phi[p];
     } else {
         *l = nlo[code]; *h = nhi[code];
-    }
-}
-
-__global__ void k_refit(int n, const int *__restrict__ vals, const float4 *__restrict__ plo, const float4 *__restrict__ phi,
-                        const int *__restrict__ left, const int *__restrict__ right, const int *__restrict__ parent_inner,
-                        const int *__restrict__ parent_leaf, int *__restrict__ visits, float4 *nlo, float4 *nhi) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    int node = parent_leaf[s];
-    while (node >= 0) {
-        int prev = __hip_atomic_fetch_add(&visits[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;  // the sibling's thread will finish this node
-        float4 al, ah, bl, bh;
-        child_box(left[node], plo, phi, vals, nlo, nhi, &al, &ah);
-        child_box(right[node], plo, phi, vals, nlo, nhi, &bl, &bh);
-        nlo[node] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
-        nhi[node] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
-        node = parent_inner[node];
     }
 }
 
@@ -248,6 +190,139 @@ __global__ void k_collapse(CollapseIn in, const int2 *__restrict__ tasks, const 
     }
     nodes[task.y] = nd;
     atomicMax(max_depth, depth + 1);
+}
+
+// ---------------------------------------------------------------------------
+// PLOC: parallel locally-ordered clustering (Meister & Bittner 2018,
+// "Parallel Locally-Ordered Clustering for Bounding Volume Hierarchy
+// Construction").  Clusters start as the Morton-sorted triangles; every
+// iteration each cluster finds, among the R clusters on either side of it in
+// the current order, the one whose union box has the smallest surface area,
+// mutual nearest neighbours merge into a new inner node (which takes the
+// lower position), and the survivors are compacted in order.  Pair ties are
+// broken on (area, lower index, higher index), so the globally smallest pair
+// is always mutual and every iteration merges.  Node and cluster positions
+// come from one 64-bit exclusive scan of (merges << 32 | survivors): the tree
+// is the same on every run.
+// ---------------------------------------------------------------------------
+static constexpr int PLOC_R = 16;   // neighbours searched on either side
+static constexpr int PLOC_B = 256;
+
+__global__ void k_ploc_init(int n, const int *__restrict__ vals, const float4 *__restrict__ plo,
+                            const float4 *__restrict__ phi, int *__restrict__ cid, float4 *__restrict__ clo,
+                            float4 *__restrict__ chi) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int p = vals[s];
+    cid[s] = (int)(s | LEAF_BIT);
+    clo[s] = plo[p];
+    chi[s] = phi[p];
+}
+
+template <int R>
+__global__ void __launch_bounds__(PLOC_B) k_ploc_nn(int m, const float4 *__restrict__ clo, const float4 *__restrict__ chi,
+                                                    int *__restrict__ nn) {
+    __shared__ float4 slo[PLOC_B + 2 * R], shi[PLOC_B + 2 * R];
+    const int base = blockIdx.x * PLOC_B;
+    for (int t = threadIdx.x; t < PLOC_B + 2 * R; t += PLOC_B) {
+        const int g = base - R + t;
+        if (g >= 0 && g < m) { slo[t] = clo[g]; shi[t] = chi[g]; }
+    }
+    __syncthreads();
+    const int i = base + threadIdx.x;
+    if (i >= m) return;
+    const float4 l = slo[threadIdx.x + R], h = shi[threadIdx.x + R];
+    float best = __int_as_float(0x7f800000);
+    int bj = -1;
+    // candidates in increasing index order: with a strict '<' the first of
+    // equal areas wins, which is the (area, lower, higher) order of pairs
+    for (int k = -R; k <= R; ++k) {
+        const int j = i + k;
+        if (k == 0 || j < 0 || j >= m) continue;
+        const float4 l2 = slo[threadIdx.x + R + k], h2 = shi[threadIdx.x + R + k];
+        const float dx = fmaxf(h.x, h2.x) - fminf(l.x, l2.x);
+        const float dy = fmaxf(h.y, h2.y) - fminf(l.y, l2.y);
+        const float dz = fmaxf(h.z, h2.z) - fminf(l.z, l2.z);
+        const float a = dx * dy + dy * dz + dz * dx;
+        if (a < best) { best = a; bj = j; }
+    }
+    nn[i] = bj;
+}
+
+__global__ void k_ploc_flags(int m, const int *__restrict__ nn, unsigned long long *__restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = j >= 0 && nn[j] == i;
+    const unsigned long long merge = (mutual && i < j) ? 1ull : 0ull;
+    const unsigned long long keep = (mutual && i > j) ? 0ull : 1ull;
+    flags[i] = (merge << 32) | keep;
+}
+
+struct PlocTree {
+    int *left, *right, *count;
+    float4 *nlo, *nhi;
+};
+
+__global__ void k_ploc_merge(int m, int node_base, const int *__restrict__ nn, const unsigned long long *__restrict__ flags,
+                             const unsigned long long *__restrict__ scan, const int *__restrict__ cid,
+                             const float4 *__restrict__ clo, const float4 *__restrict__ chi, int *__restrict__ cid2,
+                             float4 *__restrict__ clo2, float4 *__restrict__ chi2, PlocTree t, int *__restrict__ totals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const unsigned long long sc = scan[i];
+    if (i == m - 1) {
+        const unsigned long long tot = sc + flags[i];
+        totals[0] = (int)(tot & 0xffffffffu);  // survivors
+        totals[1] = (int)(tot >> 32);          // merges
+    }
+    const unsigned long long f = flags[i];
+    if (!(f & 1ull)) return;  // merged into a lower cluster
+    const int out = (int)(sc & 0xffffffffu);
+    if (f >> 32) {
+        const int j = nn[i];
+        const int node = node_base + (int)(sc >> 32);
+        const int a = cid[i], b = cid[j];
+        const float4 l1 = clo[i], h1 = chi[i], l2 = clo[j], h2 = chi[j];
+        const float4 l = make_float4(fminf(l1.x, l2.x), fminf(l1.y, l2.y), fminf(l1.z, l2.z), 0.f);
+        const float4 h = make_float4(fmaxf(h1.x, h2.x), fmaxf(h1.y, h2.y), fmaxf(h1.z, h2.z), 0.f);
+        t.left[node] = a;
+        t.right[node] = b;
+        t.count[node] = ((a & LEAF_BIT) ? 1 : t.count[a]) + ((b & LEAF_BIT) ? 1 : t.count[b]);
+        t.nlo[node] = l;
+        t.nhi[node] = h;
+        cid2[out] = node;
+        clo2[out] = l;
+        chi2[out] = h;
+    } else {
+        cid2[out] = cid[i];
+        clo2[out] = clo[i];
+        chi2[out] = chi[i];
+    }
+}
+
+__global__ void k_ploc_root(int root, int *__restrict__ first) { first[root] = 0; }
+
+// Leaf order = the tree's left-to-right order, so every subtree covers a
+// contiguous range: nodes made in one iteration place their children (made
+// in earlier iterations), iterations from the last to the first.
+__global__ void k_ploc_place(int base, int k, PlocTree t, int *__restrict__ first, int *__restrict__ last,
+                             int *__restrict__ pos) {
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= k) return;
+    const int node = base + u;
+    const int f = first[node];
+    const int l = t.left[node], r = t.right[node];
+    const int cl = (l & LEAF_BIT) ? 1 : t.count[l];
+    if (l & LEAF_BIT) { pos[l & ~LEAF_BIT] = f; t.left[node] = (int)(f | LEAF_BIT); } else first[l] = f;
+    if (r & LEAF_BIT) { pos[r & ~LEAF_BIT] = f + cl; t.right[node] = (int)((f + cl) | LEAF_BIT); } else first[r] = f + cl;
+    last[node] = f + t.count[node] - 1;
+}
+
+__global__ void k_ploc_permute(int n, const int *__restrict__ pos, const int *__restrict__ vals, int *__restrict__ out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[pos[s]] = vals[s];
 }
 
 __global__ void k_reorder(const float *__restrict__ verts, const int *__restrict__ tris, const int *__restrict__ vals, int n,
@@ -355,30 +430,76 @@ void build_bvh(Ctx *c, int width, int max_leaf) {
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, keys[0].as<unsigned long long>(),
                                                  keys[1].as<unsigned long long>(), vals[0].as<int>(), vals[1].as<int>(), n,
                                                  0, 63, c->stream));
-    const unsigned long long *skeys = keys[1].as<unsigned long long>();
     const int *svals = vals[1].as<int>();
 
     int ni = std::max(n - 1, 1);
-    DevBuf left, right, first, last, par_in, par_leaf, visits, nlo, nhi;
+    DevBuf left, right, first, last, nlo, nhi;
     left.reserve(sizeof(int) * ni);
     right.reserve(sizeof(int) * ni);
     first.reserve(sizeof(int) * ni);
     last.reserve(sizeof(int) * ni);
-    par_in.reserve(sizeof(int) * ni);
-    par_leaf.reserve(sizeof(int) * n);
-    visits.reserve(sizeof(int) * ni);
     nlo.reserve(sizeof(float4) * ni);
     nhi.reserve(sizeof(float4) * ni);
     int root_code;
+    DevBuf ploc_vals;
     if (n >= 2) {
-        hipLaunchKernelGGL(k_karras, dim3((n - 1 + bs - 1) / bs), dim3(bs), 0, c->stream, skeys, n, left.as<int>(),
-                           right.as<int>(), first.as<int>(), last.as<int>(), par_in.as<int>(), par_leaf.as<int>());
-        fill_zero(c, visits.p, sizeof(int) * ni);
-        hipLaunchKernelGGL(k_refit, dim3(grid), dim3(bs), 0, c->stream, n, svals, plo.as<float4>(), phi.as<float4>(),
-                           left.as<int>(), right.as<int>(), par_in.as<int>(), par_leaf.as<int>(), visits.as<int>(),
-                           nlo.as<float4>(), nhi.as<float4>());
+        // clusters (double-buffered), nearest neighbours, flags and their scan
+        DevBuf cid[2], clo[2], chi[2], nn, flags, scan, count, pos, totals, stemp;
+        for (int k = 0; k < 2; ++k) {
+            cid[k].reserve(sizeof(int) * n);
+            clo[k].reserve(sizeof(float4) * n);
+            chi[k].reserve(sizeof(float4) * n);
+        }
+        nn.reserve(sizeof(int) * n);
+        flags.reserve(sizeof(unsigned long long) * n);
+        scan.reserve(sizeof(unsigned long long) * n);
+        count.reserve(sizeof(int) * ni);
+        pos.reserve(sizeof(int) * n);
+        totals.reserve(sizeof(int) * 2);
+        ploc_vals.reserve(sizeof(int) * n);
+        size_t st_bytes = 0;
+        HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, st_bytes, flags.as<unsigned long long>(),
+                                                   scan.as<unsigned long long>(), n, c->stream));
+        stemp.reserve(st_bytes);
+        PlocTree t;
+        t.left = left.as<int>(); t.right = right.as<int>(); t.count = count.as<int>();
+        t.nlo = nlo.as<float4>(); t.nhi = nhi.as<float4>();
+        hipLaunchKernelGGL(k_ploc_init, dim3(grid), dim3(bs), 0, c->stream, n, svals, plo.as<float4>(), phi.as<float4>(),
+                           cid[0].as<int>(), clo[0].as<float4>(), chi[0].as<float4>());
+        std::vector<std::pair<int, int>> made;  // (first node, nodes) per iteration
+        int m = n, node_base = 0, cur = 0;
+        while (m > 1) {
+            const int g = (m + PLOC_B - 1) / PLOC_B;
+            hipLaunchKernelGGL(k_ploc_nn<PLOC_R>, dim3(g), dim3(PLOC_B), 0, c->stream, m, clo[cur].as<float4>(),
+                               chi[cur].as<float4>(), nn.as<int>());
+            hipLaunchKernelGGL(k_ploc_flags, dim3(g), dim3(PLOC_B), 0, c->stream, m, nn.as<int>(),
+                               flags.as<unsigned long long>());
+            HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(stemp.p, st_bytes, flags.as<unsigned long long>(),
+                                                       scan.as<unsigned long long>(), m, c->stream));
+            hipLaunchKernelGGL(k_ploc_merge, dim3(g), dim3(PLOC_B), 0, c->stream, m, node_base, nn.as<int>(),
+                               flags.as<unsigned long long>(), scan.as<unsigned long long>(), cid[cur].as<int>(),
+                               clo[cur].as<float4>(), chi[cur].as<float4>(), cid[1 - cur].as<int>(),
+                               clo[1 - cur].as<float4>(), chi[1 - cur].as<float4>(), t, totals.as<int>());
+            HIP_CHECK(hipGetLastError());
+            int tot[2];
+            HIP_CHECK(hipMemcpyAsync(tot, totals.p, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            if (tot[1] <= 0 || tot[0] != m - tot[1]) throw std::runtime_error("PLOC made no progress");
+            made.emplace_back(node_base, tot[1]);
+            node_base += tot[1];
+            m = tot[0];
+            cur = 1 - cur;
+        }
+        if (node_base != n - 1) throw std::runtime_error("PLOC built a wrong node count");
+        root_code = n - 2;
+        hipLaunchKernelGGL(k_ploc_root, dim3(1), dim3(1), 0, c->stream, root_code, first.as<int>());
+        for (auto it = made.rbegin(); it != made.rend(); ++it)
+            hipLaunchKernelGGL(k_ploc_place, dim3((it->second + bs - 1) / bs), dim3(bs), 0, c->stream, it->first,
+                               it->second, t, first.as<int>(), last.as<int>(), pos.as<int>());
+        hipLaunchKernelGGL(k_ploc_permute, dim3(grid), dim3(bs), 0, c->stream, n, pos.as<int>(), svals,
+                           ploc_vals.as<int>());
         HIP_CHECK(hipGetLastError());
-        root_code = 0;
+        svals = ploc_vals.as<int>();
     } else {
         root_code = (int)(0u | LEAF_BIT);
     }
