@@ -886,7 +886,7 @@ int aur_set_launch(aur_ctx *ctx, const aur_launch_opts *opts) {
         if (opts->pool_paths > 0 && opts->pool_paths < 64) throw std::invalid_argument("pool_paths must be >= 64");
         if (opts->trace_block > 0) l.trace_block = opts->trace_block;
         if (opts->trace_waves_per_cu > 0) l.trace_waves_per_cu = opts->trace_waves_per_cu;
-        if (opts->pool_paths > 0) l.batch_pool_paths = (size_t)opts->pool_paths;
+        if (opts->pool_paths > 0) l.pool_paths = (size_t)opts->pool_paths;
     });
 }
 
@@ -897,7 +897,7 @@ int aur_get_launch(aur_ctx *ctx, aur_launch_opts *out) {
         const LaunchCfg &l = ctx->c.launch;
         out->trace_block = l.trace_block;
         out->trace_waves_per_cu = l.trace_waves_per_cu;
-        out->pool_paths = (int)std::min<size_t>(l.batch_pool_paths, 0x7fffffff);
+        out->pool_paths = (int)std::min<size_t>(l.pool_paths, 0x7fffffff);
     });
 }
 

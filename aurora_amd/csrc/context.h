@@ -114,7 +114,9 @@ struct KernelTimer {
 struct LaunchCfg {
     int trace_block = 128;          // traversal workgroup size: 64, 128 or 256
     int trace_waves_per_cu = 16;    // persistent traversal grid, waves per CU
-    size_t batch_pool_paths = 4u << 20;  // path slots in flight in a batched launch
+    // path slots in flight: the batched launch's pool, and the largest chunk
+    // of whole passes a single-scene render traces at once
+    size_t pool_paths = 8u << 20;
 };
 
 // Device state of a batched launch (batch.hip), kept between calls.

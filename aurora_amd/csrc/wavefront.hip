@@ -459,8 +459,9 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
     const int ls = std::max(1, src.ls);
     // mode 0 runs whole passes per batch (a pass of ls paths per pixel fits)
-    size_t cap = src.mode == 0 ? std::max<size_t>(wh * ls, std::min<size_t>(n_paths, (size_t)8 << 20))
-                               : std::min<size_t>(n_paths, (size_t)8 << 20);
+    const size_t pool = c->launch.pool_paths;
+    size_t cap = src.mode == 0 ? std::max<size_t>(wh * ls, std::min<size_t>(n_paths, pool))
+                               : std::min<size_t>(n_paths, pool);
     if (cap > (size_t)0x7fffffff) throw CapacityError("batch too large");
     ensure_wavefront(c, std::max<size_t>(cap, 1));
     SceneView sv = scene_view(c);

@@ -26,12 +26,12 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16}
-BATCH_DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 4 << 20}
+BATCH_DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 8 << 20}
 WIDTHS = (4, 8)
 LEAVES = (1, 2, 4, 8)
 BLOCKS = (64, 128, 256)
 WAVES = (8, 16, 24, 32)
-POOLS = (1 << 20, 2 << 20, 4 << 20, 8 << 20)
+POOLS = (2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20)
 
 
 def _bucket(n_tris: int) -> str:
