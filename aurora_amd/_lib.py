@@ -112,7 +112,7 @@ class Stats(C.Structure):
 
 class ParamGrads(C.Structure):
     _fields_ = [("albedo_tri", C.POINTER(C.c_float)), ("roughness", C.POINTER(C.c_float)),
-                ("spec_weight", C.POINTER(C.c_float))]
+                ("spec_weight", C.POINTER(C.c_float)), ("albedo_texture", C.POINTER(C.c_float))]
 
 
 class KernelStat(C.Structure):
@@ -165,6 +165,7 @@ SIGNATURES = {
     "aur_scene_load": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
     "aur_scene_upload": (C.c_int, [_P, C.POINTER(SceneDesc)]),
     "aur_set_tri_albedo": (C.c_int, [_P, _FP]),
+    "aur_set_albedo_texture": (C.c_int, [_P, C.c_int, C.c_int, _FP, _IP, C.c_int]),
     "aur_build_accel": (C.c_int, [_P, C.POINTER(AccelOpts)]),
     "aur_scene_info_get": (C.c_int, [_P, C.POINTER(SceneInfo)]),
     "aur_scene_export": (C.c_int, [_P, _FP, _IP, _FP, _IP, C.POINTER(Material), C.POINTER(Light)]),

@@ -42,13 +42,15 @@ struct AdjParams {
     float *g_albedo;      // 3 * n_tris (or null)
     float *g_rough;       // n_attrs
     float *g_spec;        // n_attrs
+    float *g_ptex;        // 3 * texels of the albedo texture (or null)
     int n_tris, n_attrs;
     int use_lds;
     unsigned long long *stats;  // paths, ext rays, shadow rays
 };
 
 struct GradSink {
-    float *albedo, *rough, *spec;
+    float *albedo, *rough, *spec;  // null when not requested
+    float *ptex;                   // albedo texture texels (global memory), or null
     __device__ inline void add(float *p, float v) {
         if (v != 0.f) atomicAdd(p, v);
     }
@@ -109,13 +111,37 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
         }
         if (m.type == MAT_LIGHT) break;
         const int vslot = 2 + AUR_SLOTS_PER_VERTEX * tb;
-        ShadeGeo sg;  // scenes with shaders are refused by the adjoint entry point
+        // shaders are refused by the adjoint entry point; the albedo texture
+        // reads the interpolated uv
+        ShadeGeo sg;
         sg.P = P; sg.Ng = nn; sg.s = 0.f; sg.t = 0.f; sg.attr = attr;
-        BrdfChoice ch = get_brdf_ex<false>(sv, m, prim, vn, nn, matte, rng.u(vslot + 0), sg);
+        const bool ptex_mat = (m.texmask & TEXMASK_PTEX) != 0 && sv.ptex != nullptr;
+        if (ptex_mat && sv.uv) {
+            const float2 t0 = sv.uv[3 * prim], t1 = sv.uv[3 * prim + 1], t2 = sv.uv[3 * prim + 2];
+            sg.s = w * t0.x + u * t1.x + v * t2.x;
+            sg.t = w * t0.y + u * t1.y + v * t2.y;
+        }
+        BrdfChoice ch = get_brdf_ex<false, true>(sv, m, prim, vn, nn, matte, rng.u(vslot + 0), sg);
         const Bsdf &b = ch.b;
         const bool spec_lobe = b.kind == BSDF_CT;
         const bool kel = m.type == MAT_KELEMEN;
-        const bool alb = (b.kind == BSDF_LAMBERT) && sv.albedo != nullptr;
+        // the diffuse albedo of this vertex is a parameter when its texel or
+        // its triangle's override is being differentiated
+        const float *a_val = nullptr;
+        float *a_grad = nullptr;
+        if (GRAD && b.kind == BSDF_LAMBERT) {
+            if (ptex_mat) {
+                if (g.ptex) {
+                    const size_t ti = 3 * (size_t)ptex_texel(sv, sg.s, sg.t);
+                    a_val = sv.ptex + ti;
+                    a_grad = g.ptex + ti;
+                }
+            } else if (sv.albedo && g.albedo) {
+                a_val = sv.albedo + 3 * prim;
+                a_grad = g.albedo + 3 * prim;
+            }
+        }
+        const bool alb = a_grad != nullptr;
         if (GRAD && kel && ch.kelemen_choice) {
             float dlogp = spec_lobe ? ch.dA / fmaxf(ch.A, 1e-12f) : -ch.dA / fmaxf(1.f - ch.A, 1e-12f);
             g.add(&g.rough[attr], dot3(delta, R) * dlogp * ch.drough);
@@ -168,10 +194,10 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
             L = add3(L, add3(cl, cb));
             if (GRAD) {
                 if (alb) {
-                    const float *a = sv.albedo + 3 * prim;
-                    g.add(&g.albedo[3 * prim + 0], delta.x * cl.x / a[0]);
-                    g.add(&g.albedo[3 * prim + 1], delta.y * cl.y / a[1]);
-                    g.add(&g.albedo[3 * prim + 2], delta.z * cl.z / a[2]);
+                    // d(f cos / pdf)/d(albedo) = f cos / pdf / albedo per channel
+                    if (a_val[0] > 0.f) g.add(&a_grad[0], delta.x * cl.x / a_val[0]);
+                    if (a_val[1] > 0.f) g.add(&a_grad[1], delta.y * cl.y / a_val[1]);
+                    if (a_val[2] > 0.f) g.add(&a_grad[2], delta.z * cl.z / a_val[2]);
                 }
                 if (kel && spec_lobe) {
                     g.add(&g.spec[attr], dot3(delta, add3(cl, cb)) / m.spec[0]);
@@ -194,10 +220,9 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
         }
         if (GRAD) {
             if (alb) {
-                const float *a = sv.albedo + 3 * prim;
-                g.add(&g.albedo[3 * prim + 0], delta.x * R.x / a[0]);
-                g.add(&g.albedo[3 * prim + 1], delta.y * R.y / a[1]);
-                g.add(&g.albedo[3 * prim + 2], delta.z * R.z / a[2]);
+                if (a_val[0] > 0.f) g.add(&a_grad[0], delta.x * R.x / a_val[0]);
+                if (a_val[1] > 0.f) g.add(&a_grad[1], delta.y * R.y / a_val[1]);
+                if (a_val[2] > 0.f) g.add(&a_grad[2], delta.z * R.z / a_val[2]);
             }
             if (kel && spec_lobe) {
                 float dr = dot3(delta, R);
@@ -252,7 +277,7 @@ __global__ void __launch_bounds__(128) k_adj_forward(SceneView sv, AdjParams ap,
     st.init(s_stack, 128, ovf, ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     GradSink g;
-    g.albedo = g.rough = g.spec = nullptr;
+    g.albedo = g.rough = g.spec = g.ptex = nullptr;
     unsigned long long n_ext = 0, n_shadow = 0, n_paths = 0;
     for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
         int pixel;
@@ -285,7 +310,7 @@ __global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, co
     if (ap.use_lds) {
         for (int i = threadIdx.x; i < n_alb + 2 * ap.n_attrs; i += blockDim.x) s_grad[i] = 0.f;
         __syncthreads();
-        g.albedo = s_grad;
+        g.albedo = n_alb ? s_grad : nullptr;
         g.rough = s_grad + n_alb;
         g.spec = s_grad + n_alb + ap.n_attrs;
     } else {
@@ -293,6 +318,7 @@ __global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, co
         g.rough = ap.g_rough;
         g.spec = ap.g_spec;
     }
+    g.ptex = ap.g_ptex;
     unsigned long long n_ext = 0, n_shadow = 0;
     for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
         const float4 r = rec[i];
@@ -324,7 +350,7 @@ __global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, co
 }
 
 void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *d_galb, float *d_grough, float *d_gspec,
-             aur_stats *stats) {
+             float *d_gptex, aur_stats *stats) {
     if (!c->built) throw StateError("adjoint before aur_build_accel");
     SceneView sv = scene_view(c);
     const int wh = c->scene.opt.width * c->scene.opt.height;
@@ -342,6 +368,7 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     ap.g_albedo = d_galb;
     ap.g_rough = d_grough;
     ap.g_spec = d_gspec;
+    ap.g_ptex = d_gptex;
     ap.n_tris = c->n_tris;
     ap.n_attrs = c->n_mats;
     ap.use_lds = ((d_galb ? 3 * c->n_tris : 0) + 2 * c->n_mats) <= AUR_ADJ_LDS_PARAMS;

@@ -4,6 +4,7 @@
 // with a thread-local message; nothing throws or aborts across the ABI.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -160,7 +161,7 @@ static void upload_shading(Ctx *c) {
     HostScene &s = c->scene;
     const int nt = (int)(s.tris.size() / 3);
     // texture coordinates, only when some material reads them
-    c->has_uv = false;
+    c->has_uv = !c->ptex.empty();
     for (const aur_shader &k : s.shaders) c->has_uv = c->has_uv || k.type == AUR_SHADER_TEXTURE2D;
     if (c->has_uv && nt > 0) {
         if (s.uvs.size() != (size_t)6 * nt) throw std::invalid_argument("uv array has the wrong size");
@@ -168,6 +169,11 @@ static void upload_shading(Ctx *c) {
         HIP_CHECK(hipMemcpyAsync(c->d_uv.p, s.uvs.data(), sizeof(float) * s.uvs.size(), hipMemcpyHostToDevice, c->stream));
     } else {
         c->has_uv = false;
+    }
+    if (!c->ptex.empty()) {
+        c->d_ptex.reserve(sizeof(float) * c->ptex.size());
+        HIP_CHECK(hipMemcpyAsync(c->d_ptex.p, c->ptex.data(), sizeof(float) * c->ptex.size(), hipMemcpyHostToDevice,
+                                 c->stream));
     }
     // per-attribute slots and transforms
     std::vector<DAttrX> ax(s.attr_mats.size());
@@ -187,8 +193,11 @@ static void upload_shading(Ctx *c) {
                 mask |= 1 << k;
             }
         std::memcpy(x.c2o, m.c2o, sizeof x.c2o);
+        // the albedo texture replaces the diffuse colour (and its shader)
+        if (std::find(c->ptex_attrs.begin(), c->ptex_attrs.end(), (int)a) != c->ptex_attrs.end() && !c->ptex.empty())
+            mask = (mask & ~(1 << AUR_SLOT_DIFFUSE)) | TEXMASK_PTEX;
         c->dmats[a].texmask = mask;
-        c->has_attrx = c->has_attrx || mask != 0;
+        c->has_attrx = c->has_attrx || (mask & ~TEXMASK_PTEX) != 0;
     }
     if (c->has_attrx) {
         c->d_attrx.reserve(sizeof(DAttrX) * ax.size());
@@ -521,6 +530,8 @@ int aur_scene_load(aur_ctx *ctx, const char *scene_path, const char *config_json
         ctx->c.have_scene = true;
         ctx->c.built = false;
         ctx->c.tri_albedo.clear();
+        ctx->c.ptex.clear();
+        ctx->c.ptex_attrs.clear();
     });
 }
 
@@ -592,6 +603,53 @@ int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
         ctx->c.have_scene = true;
         ctx->c.built = false;
         ctx->c.tri_albedo.clear();
+        ctx->c.ptex.clear();
+        ctx->c.ptex_attrs.clear();
+    });
+}
+
+int aur_set_albedo_texture(aur_ctx *ctx, int width, int height, const float *rgb, const int *attrs, int n_attrs) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->built) throw StateError("the albedo texture needs a built scene (call aur_build_accel first)");
+        if (!rgb) {
+            c->ptex.clear();
+            c->ptex_attrs.clear();
+        } else {
+            if (width < 1 || height < 1 || (int64_t)width * height > ((int64_t)1 << 28))
+                throw std::invalid_argument("albedo texture size out of range");
+            if (n_attrs < 1 || !attrs) throw std::invalid_argument("the albedo texture needs at least one material");
+            const HostScene &s = c->scene;
+            const int nt = (int)(s.tris.size() / 3);
+            if (s.uvs.size() != (size_t)6 * nt) throw std::invalid_argument("the albedo texture needs per-corner uvs");
+            std::vector<int> as(attrs, attrs + n_attrs);
+            for (int a : as) {
+                if (a < 0 || a >= (int)s.attr_mats.size()) throw std::invalid_argument("material index out of range");
+                const int t = s.attr_mats[a].type;
+                if (t != AUR_MAT_MATTE && t != AUR_MAT_KELEMEN)
+                    throw std::invalid_argument("the albedo texture drives matte and Kelemen diffuse colours only");
+            }
+            const bool same = !c->ptex.empty() && width == c->ptex_w && height == c->ptex_h && as == c->ptex_attrs;
+            c->ptex.assign(rgb, rgb + (size_t)3 * width * height);
+            c->ptex_w = width;
+            c->ptex_h = height;
+            c->ptex_attrs = as;
+            if (same) {
+                // new texel values only (an optimiser's step)
+                set_device(c);
+                HIP_CHECK(hipMemcpyAsync(c->d_ptex.p, c->ptex.data(), sizeof(float) * c->ptex.size(),
+                                         hipMemcpyHostToDevice, c->stream));
+                HIP_CHECK(hipStreamSynchronize(c->stream));
+                return;
+            }
+        }
+        set_device(c);
+        upload_shading(c);
+        if (!c->dmats.empty())
+            HIP_CHECK(hipMemcpyAsync(c->d_mats.p, c->dmats.data(), sizeof(DMat) * c->dmats.size(),
+                                     hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
     });
 }
 
@@ -984,8 +1042,11 @@ int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_d
         if (first_pass < 0 || n_passes <= 0) throw std::invalid_argument("bad pass range");
         if (out->albedo_tri && c->tri_albedo.empty())
             throw StateError("albedo gradients need a per-triangle albedo (aur_set_tri_albedo)");
+        if (out->albedo_texture && c->ptex.empty())
+            throw StateError("albedo texture gradients need an albedo texture (aur_set_albedo_texture)");
         for (const DMat &m : c->dmats) {
-            if (m.texmask) throw StateError("the adjoint does not differentiate shader-driven (textured) materials");
+            if (m.texmask & ~TEXMASK_PTEX)
+                throw StateError("the adjoint does not differentiate shader-driven (textured) materials");
             if (m.type == MAT_GLASS) throw StateError("the adjoint does not support glass (specular) materials");
         }
         if (c->sampler != AUR_SAMPLER_HASH) throw StateError("the adjoint replays hash-sampled paths only");
@@ -995,8 +1056,13 @@ int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_d
         set_device(c);
         const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
         const size_t na = 3 * (size_t)c->n_tris, nm = (size_t)c->n_mats;
-        DevBuf dl, ga, gr, gs;
+        const size_t nx = c->ptex.size();
+        DevBuf dl, ga, gr, gs, gx;
         dl.reserve(sizeof(float4) * wh);
+        if (out->albedo_texture) {
+            gx.reserve(sizeof(float) * nx);
+            fill_zero(c, gx.p, sizeof(float) * nx);
+        }
         ga.reserve(sizeof(float) * std::max<size_t>(na, 1));
         gr.reserve(sizeof(float) * std::max<size_t>(nm, 1));
         gs.reserve(sizeof(float) * std::max<size_t>(nm, 1));
@@ -1005,7 +1071,9 @@ int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_d
         fill_zero(c, gr.p, sizeof(float) * std::max<size_t>(nm, 1));
         fill_zero(c, gs.p, sizeof(float) * std::max<size_t>(nm, 1));
         adjoint(c, first_pass, n_passes, dl.as<float4>(), out->albedo_tri ? ga.as<float>() : nullptr, gr.as<float>(),
-                gs.as<float>(), stats);
+                gs.as<float>(), out->albedo_texture ? gx.as<float>() : nullptr, stats);
+        if (out->albedo_texture)
+            HIP_CHECK(hipMemcpyAsync(out->albedo_texture, gx.p, sizeof(float) * nx, hipMemcpyDeviceToHost, c->stream));
         if (out->albedo_tri)
             HIP_CHECK(hipMemcpyAsync(out->albedo_tri, ga.p, sizeof(float) * na, hipMemcpyDeviceToHost, c->stream));
         if (out->roughness) HIP_CHECK(hipMemcpyAsync(out->roughness, gr.p, sizeof(float) * nm, hipMemcpyDeviceToHost, c->stream));

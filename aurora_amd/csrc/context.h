@@ -149,9 +149,14 @@ struct Ctx {
     bool have_scene = false;
     bool built = false;
     std::vector<float> tri_albedo;  // optional per-triangle override
+    // optional FP32 RGB albedo texture driving the diffuse colour of the
+    // attributes in ptex_attrs (aur_set_albedo_texture)
+    std::vector<float> ptex;
+    int ptex_w = 0, ptex_h = 0;
+    std::vector<int> ptex_attrs;
 
     // device scene
-    DevBuf d_verts, d_tris, d_shade, d_mats, d_lights, d_tables, d_albedo;
+    DevBuf d_verts, d_tris, d_shade, d_mats, d_lights, d_tables, d_albedo, d_ptex;
     DevBuf d_uv, d_attrx, d_shaders, d_texs, d_texels;  // shading inputs (empty without shaders)
     int sampler = 0;                                      // AUR_SAMPLER_*
     DevBuf d_halton_dims, d_halton_tables, d_halton_view; // Halton source (built on first use)
@@ -212,7 +217,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
               int iters_per_poll, aur_job_result *results, aur_stats *total_stats);
 // adjoint.hip
 void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *d_galb, float *d_grough, float *d_gspec,
-             aur_stats *stats);
+             float *d_gptex, aur_stats *stats);
 // parity.hip
 void bsdf_queries(Ctx *c, const aur_bsdf_query *d_q, int n, aur_bsdf_result *d_r);
 void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out);

@@ -42,12 +42,15 @@ enum { BSDF_LAMBERT = 0, BSDF_CT = 1, BSDF_MIRROR = 2, BSDF_TRANSMIT = 3 };
 // ---------------------------------------------------------------------------
 // Material parameters a shader may drive (AUR_SLOT_* of the C ABI).
 enum { SLOT_DIFFUSE = 0, SLOT_SPEC = 1, SLOT_GAIN = 2, SLOT_ROUGH = 3, SLOT_ROUGHB = 4, SLOT_MIX = 5, SLOT_NORMAL = 6 };
+// DMat::texmask bit of a material whose diffuse colour is the FP32 albedo
+// texture (aur_set_albedo_texture) at the hit's uv
+enum { TEXMASK_PTEX = 1 << 7 };
 enum { SH_TEXTURE2D = 0, SH_TRIPLANAR = 1, SH_CONSTANT = 2 };
 enum { SH_COLOR = 0, SH_FLOAT = 1 };
 
 struct alignas(16) DMat {
     int type, dist, table;
-    int texmask;          // bit k set: SLOT k is driven by the shader DAttrX::slot[k]
+    int texmask;          // bit k set: SLOT k is driven by the shader DAttrX::slot[k]; TEXMASK_PTEX
 
     float diffuse[4];
     float spec[4];

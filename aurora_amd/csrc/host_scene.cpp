@@ -6,6 +6,8 @@
 // src/core/stringUtils.h (path templates).
 #include "host_scene.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +15,7 @@
 #include <map>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 #include <utility>
 
 #include "exr_io.h"
@@ -209,43 +212,98 @@ std::string string_template(const std::string &input, const std::string &config_
 // ---------------------------------------------------------------------------
 // OBJ: objParser.cpp:58-202, objTriangleMesh.cpp:26-92
 // ---------------------------------------------------------------------------
-static int leading_int(const std::string &s) { return std::atoi(s.c_str()); }
-
-// The uv index of a face corner: the digits between the first and second '/'
-// (objParser.cpp:140-176), 0 when absent.
-static int uv_index(const std::string &e) {
-    size_t a = e.find('/');
-    if (a == std::string::npos) return 0;
-    size_t b = e.find('/', a + 1);
-    return leading_int(e.substr(a + 1, b == std::string::npos ? std::string::npos : b - a - 1));
+// Reads the whole file and scans it line by line with the semantics of the
+// stream parser it replaces (objParser.cpp: `ss >> op` then values): on a
+// 'v' line the first token is skipped and three floats read ("vn"), two
+// ("vt") or three (positions), a value that does not parse reads 0 and ends
+// the line; an 'f' line contributes its first three corner tokens.  Floats
+// convert with strtof, as the stream extraction does.
+static const char *skip_ws(const char *p, const char *e) {
+    while (p < e && (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
+    return p;
+}
+static const char *skip_token(const char *p, const char *e) {
+    while (p < e && !(*p == ' ' || *p == '\t' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
+    return p;
+}
+static void read_floats(const char *p, const char *e, int n, std::vector<float> *out) {
+    bool ok = true;
+    char buf[64];
+    for (int k = 0; k < n; ++k) {
+        float v = 0.f;
+        if (ok) {
+            p = skip_ws(p, e);
+            const char *q = skip_token(p, e);
+            const size_t len = std::min<size_t>((size_t)(q - p), sizeof buf - 1);
+            std::memcpy(buf, p, len);
+            buf[len] = 0;
+            char *end = nullptr;
+            v = std::strtof(buf, &end);
+            if (end == buf) { v = 0.f; ok = false; }
+            p = q;
+        }
+        out->push_back(v);
+    }
+}
+static int corner_int(const char *p, const char *e) {
+    // atoi of the characters [p, e)
+    char buf[32];
+    const size_t len = std::min<size_t>((size_t)(e - p), sizeof buf - 1);
+    std::memcpy(buf, p, len);
+    buf[len] = 0;
+    return std::atoi(buf);
 }
 
 static void load_obj(const std::string &path, bool need_normals, HostObject *obj) {
-    std::ifstream f(path.c_str());
-    if (!f.good()) throw std::runtime_error("cannot open OBJ file: " + path);
+    std::FILE *fp = std::fopen(path.c_str(), "rb");
+    if (!fp) throw std::runtime_error("cannot open OBJ file: " + path);
+    std::string data;
+    std::fseek(fp, 0, SEEK_END);
+    const long size = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    if (size > 0) {
+        data.resize((size_t)size);
+        const size_t got = std::fread(&data[0], 1, (size_t)size, fp);
+        data.resize(got);
+    }
+    std::fclose(fp);
     std::vector<float> pts, nrm, tex;
     std::vector<int> vi, ni, ti;
-    std::string line;
-    while (std::getline(f, line)) {
-        if (line.empty()) continue;
-        std::stringstream ss(line);
-        std::string op;
-        ss >> op;
-        if (line[0] == 'v') {
-            float a = 0, b = 0, c = 0;
-            if (line.size() > 1 && line[1] == 'n') { ss >> a >> b >> c; nrm.push_back(a); nrm.push_back(b); nrm.push_back(c); }
-            else if (line.size() > 1 && line[1] == 't') { ss >> a >> b; tex.push_back(a); tex.push_back(b); }
-            else { ss >> a >> b >> c; pts.push_back(a); pts.push_back(b); pts.push_back(c); }
-        } else if (line[0] == 'f') {
-            std::string e[3];
-            ss >> e[0] >> e[1] >> e[2];
-            for (int k = 0; k < 3; ++k) {
-                vi.push_back(leading_int(e[k].substr(0, e[k].find('/'))));
-                ti.push_back(uv_index(e[k]));
-                size_t last = e[k].find_last_of('/');
-                ni.push_back(leading_int(last == std::string::npos ? e[k] : e[k].substr(last + 1)));
+    const char *p = data.data(), *end = data.data() + data.size();
+    while (p < end) {
+        const char *eol = static_cast<const char *>(std::memchr(p, '\n', (size_t)(end - p)));
+        if (!eol) eol = end;
+        if (eol > p) {
+            if (p[0] == 'v') {
+                const char *q = skip_token(p, eol);
+                if (eol - p > 1 && p[1] == 'n') read_floats(q, eol, 3, &nrm);
+                else if (eol - p > 1 && p[1] == 't') read_floats(q, eol, 2, &tex);
+                else read_floats(q, eol, 3, &pts);
+            } else if (p[0] == 'f') {
+                const char *q = skip_token(p, eol);
+                for (int k = 0; k < 3; ++k) {
+                    q = skip_ws(q, eol);
+                    const char *t = q, *te = skip_token(q, eol);
+                    q = te;
+                    // vertex: digits before the first '/'; uv: between the first
+                    // and second '/' (objParser.cpp:140-176); normal: after the
+                    // last '/', or the whole token without one
+                    const char *s1 = static_cast<const char *>(std::memchr(t, '/', (size_t)(te - t)));
+                    vi.push_back(corner_int(t, s1 ? s1 : te));
+                    int uv = 0;
+                    const char *last = s1;
+                    if (s1) {
+                        const char *s2 = static_cast<const char *>(std::memchr(s1 + 1, '/', (size_t)(te - s1 - 1)));
+                        uv = corner_int(s1 + 1, s2 ? s2 : te);
+                        for (const char *c = s1; c < te; ++c)
+                            if (*c == '/') last = c;
+                    }
+                    ti.push_back(uv);
+                    ni.push_back(corner_int(last ? last + 1 : t, te));
+                }
             }
         }
+        p = eol + 1;
     }
     size_t nverts = vi.size();
     obj->P.resize(nverts * 3);
@@ -539,6 +597,7 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
 
     // objects in sorted key order (jsonParser.cpp:619-661)
     std::vector<aur_light> lights;
+    std::vector<std::string> obj_paths;  // OBJ file of each geometry object, read below
     if (root.has("objects")) {
         for (auto &kv : root.at("objects").obj) {
             const JsonValue &o = kv.second;
@@ -569,16 +628,8 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
                     obj.displace.frequency = d.has("frequency") ? (float)d.at("frequency").as_double() : 1.f;
                     obj.displace.seed = d.has("seed") ? (uint32_t)d.at("seed").as_double() : 0u;
                 }
-                load_obj(path, obj.subdlevels == 0, &obj);
-                auto mi = mats.find(obj.material);
-                if (mi == mats.end()) throw std::runtime_error("unknown material: " + obj.material);
-                if ((obj.subdlevels > 0 || obj.displace.on) && !obj.UV.empty())
-                    for (int k = 0; k < AUR_N_SLOTS; ++k) {
-                        int sh = mi->second.shader[k];
-                        if (sh >= 0 && s.shaders[sh].type == AUR_SHADER_TEXTURE2D)
-                            throw std::runtime_error("object " + obj.name +
-                                                     ": uv-mapped textures on subdivided meshes are not supported");
-                    }
+                if (mats.find(obj.material) == mats.end()) throw std::runtime_error("unknown material: " + obj.material);
+                obj_paths.push_back(path);
                 s.objects.push_back(std::move(obj));
             } else if (type == "light") {
                 // jsonParser.cpp:243-263
@@ -611,6 +662,39 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
             }
             // unknown object types are skipped with a warning in the reference
         }
+    }
+    // the meshes, read concurrently (one file per task)
+    {
+        const size_t n = s.objects.size();
+        std::vector<std::exception_ptr> err(n);
+        std::atomic<size_t> next(0);
+        auto work = [&] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) {
+                try {
+                    load_obj(obj_paths[i], s.objects[i].subdlevels == 0, &s.objects[i]);
+                } catch (...) {
+                    err[i] = std::current_exception();
+                }
+            }
+        };
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t n_threads = std::min<size_t>(std::min<size_t>(n, hw), 16);
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < n_threads; ++t) pool.emplace_back(work);
+        work();
+        for (std::thread &t : pool) t.join();
+        for (size_t i = 0; i < n; ++i)
+            if (err[i]) std::rethrow_exception(err[i]);
+    }
+    for (const HostObject &obj : s.objects) {
+        const aur_material &mm = mats[obj.material];
+        if ((obj.subdlevels > 0 || obj.displace.on) && !obj.UV.empty())
+            for (int k = 0; k < AUR_N_SLOTS; ++k) {
+                int sh = mm.shader[k];
+                if (sh >= 0 && s.shaders[sh].type == AUR_SHADER_TEXTURE2D)
+                    throw std::runtime_error("object " + obj.name +
+                                             ": uv-mapped textures on subdivided meshes are not supported");
+            }
     }
     // attributes: objects first, then lights (renderer.cpp:118-134)
     for (size_t i = 0; i < s.objects.size(); ++i) {

@@ -22,6 +22,8 @@ struct SceneView {
     const DTex *texs;
     const ushort4 *texels;  // FP16 RGBA
     const HaltonView *qmc;  // Halton sample source (device memory), or null
+    const float *ptex;      // FP32 RGB albedo texture (TEXMASK_PTEX materials), or null
+    int ptex_w, ptex_h;
     int n_lights;
     int has_env;
     float env[3];
@@ -68,6 +70,21 @@ __device__ inline float4 tex_read(const SceneView &sv, int t, float s, float tt)
     const ushort4 q = sv.texels[T.off + (long long)y * T.w + x];
     return make_float4(half_bits_to_float(q.x), half_bits_to_float(q.y), half_bits_to_float(q.z),
                        half_bits_to_float(q.w));
+}
+
+// Texel of the FP32 albedo texture a hit at uv (s, t) reads: texture2DShader's
+// mapping (u - floor(u), 1 - (v - floor(v))) and Texture2D::read's nearest
+// texel with tex_read's clamps.
+__device__ inline int ptex_texel(const SceneView &sv, float s, float t) {
+    const float a = s - floorf(s);
+    const float b = 1.f - (t - floorf(t));
+    int x = (int)floorf(a * (float)sv.ptex_w);
+    int y = (int)floorf(b * (float)sv.ptex_h);
+    if (y == sv.ptex_h) y -= 1;
+    if (x == sv.ptex_w) x -= 1;
+    x = min(max(x, 0), sv.ptex_w - 1);
+    y = min(max(y, 0), sv.ptex_h - 1);
+    return y * sv.ptex_w + x;
 }
 
 // transform.cpp:206-232 on a row-major 4x4
@@ -315,7 +332,22 @@ struct BrdfChoice {
     float drough;         // d(roughness used) / d(material roughness)
 };
 
-template <bool TEX>
+// The diffuse colour of a Lambert lobe: the albedo texture's texel, the
+// per-triangle override, or the material's (shader-driven) colour.  PTEX
+// compiles the albedo-texture lookup in.
+template <bool TEX, bool PTEX>
+__device__ inline f3 diffuse_color(const SceneView &sv, const DMat &m, int prim, const ShadeGeo &g) {
+    if constexpr (PTEX) {
+        if (m.texmask & TEXMASK_PTEX) {
+            const float *p = sv.ptex + 3 * (size_t)ptex_texel(sv, g.s, g.t);
+            return mk3(p[0], p[1], p[2]);
+        }
+    }
+    if (sv.albedo) return mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2]);
+    return mat_color<TEX>(sv, m, SLOT_DIFFUSE, m.diffuse, g);
+}
+
+template <bool TEX, bool PTEX = TEX>
 __device__ inline BrdfChoice get_brdf_ex(const SceneView &sv, const DMat &m, int prim, f3 vn, f3 nn, bool matte,
                                          float u_lobe, const ShadeGeo &g) {
     BrdfChoice ch;
@@ -325,8 +357,7 @@ __device__ inline BrdfChoice get_brdf_ex(const SceneView &sv, const DMat &m, int
     ch.drough = 1.f;
     Bsdf &b = ch.b;
     if (m.type == MAT_MATTE) {
-        f3 diffuse = sv.albedo ? mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2])
-                               : mat_color<TEX>(sv, m, SLOT_DIFFUSE, m.diffuse, g);
+        const f3 diffuse = diffuse_color<TEX, PTEX>(sv, m, prim, g);
         b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f;
     } else if (m.type == MAT_METAL) {
         const float mix = mat_float<TEX>(sv, m, SLOT_MIX, m.mix, g);
@@ -361,8 +392,7 @@ __device__ inline BrdfChoice get_brdf_ex(const SceneView &sv, const DMat &m, int
             ch.A = albedo;
             ch.dA = kelemen_albedo_drough(tab, c, rough);
             if (u_lobe > albedo) {
-                f3 diffuse = sv.albedo ? mk3(sv.albedo[3 * prim], sv.albedo[3 * prim + 1], sv.albedo[3 * prim + 2])
-                                       : mat_color<TEX>(sv, m, SLOT_DIFFUSE, m.diffuse, g);
+                const f3 diffuse = diffuse_color<TEX, PTEX>(sv, m, prim, g);
                 b.kind = BSDF_LAMBERT; b.dist = 0; b.color = diffuse; b.rough = 0.f;
             }
         }

@@ -328,7 +328,7 @@ void timer_release(Ctx *c) {
 // host driver
 // ---------------------------------------------------------------------------
 bool shade_needs_tex(const Ctx *c) {
-    if (c->has_attrx || c->sampler != AUR_SAMPLER_HASH) return true;
+    if (c->has_attrx || c->sampler != AUR_SAMPLER_HASH || !c->ptex.empty()) return true;
     for (const DLight &l : c->dlights)
         if (l.type == LIGHT_ENV && l.tex >= 0) return true;
     return false;
@@ -349,6 +349,9 @@ SceneView scene_view(Ctx *c) {
     sv.texs = c->d_texs.as<DTex>();
     sv.texels = c->d_texels.as<ushort4>();
     sv.qmc = (c->sampler == AUR_SAMPLER_HALTON && c->d_halton_view.p) ? c->d_halton_view.as<HaltonView>() : nullptr;
+    sv.ptex = c->ptex.empty() ? nullptr : c->d_ptex.as<float>();
+    sv.ptex_w = c->ptex_w;
+    sv.ptex_h = c->ptex_h;
     sv.n_lights = c->n_lights;
     sv.has_env = c->n_env > 0;
     sv.env[0] = c->env_le[0]; sv.env[1] = c->env_le[1]; sv.env[2] = c->env_le[2];

@@ -111,6 +111,24 @@ class Renderer:
         self._albedo = np.ascontiguousarray(albedo, dtype=np.float32).reshape(-1)
         L.check(self.lib.aur_set_tri_albedo(self._ctx, L.fptr(self._albedo)))
 
+    def set_albedo_texture(self, rgb: Optional[np.ndarray], materials=None) -> None:
+        """An FP32 RGB albedo texture (height, width, 3), row 0 at the top,
+        replacing the diffuse colour of the attributes `materials` (indices:
+        geometry objects in sorted name order, then lights) at their
+        triangles' uvs; None removes it."""
+        if rgb is None:
+            L.check(self.lib.aur_set_albedo_texture(self._ctx, 0, 0, None, None, 0))
+            self._ptex = None
+            return
+        tex = np.ascontiguousarray(rgb, dtype=np.float32)
+        if tex.ndim != 3 or tex.shape[2] != 3:
+            raise ValueError("albedo texture must be (height, width, 3)")
+        idx = np.array([int(m) for m in materials], np.int32)
+        self._ptex = tex
+        self._ptex_attrs = idx
+        L.check(self.lib.aur_set_albedo_texture(self._ctx, tex.shape[1], tex.shape[0], L.fptr(tex),
+                                                idx.ctypes.data_as(C.POINTER(C.c_int)), len(idx)))
+
     def info(self) -> dict:
         i = L.SceneInfo()
         L.check(self.lib.aur_scene_info_get(self._ctx, C.byref(i)))
@@ -206,8 +224,10 @@ class Renderer:
         L.check(self.lib.aur_set_material_params(self._ctx, L.fptr(self._rough) if self._rough is not None else None,
                                                  L.fptr(self._specw) if self._specw is not None else None))
 
-    def adjoint(self, first_pass: int, n_passes: int, dloss_dimage: np.ndarray, albedo: bool = True):
-        """Path-replay adjoint: returns (albedo grads (n_tris, 3) or None, roughness grads, spec grads, stats)."""
+    def adjoint(self, first_pass: int, n_passes: int, dloss_dimage: np.ndarray, albedo: bool = True,
+                texture: bool = False):
+        """Path-replay adjoint: returns (albedo grads (n_tris, 3) or None, roughness grads, spec grads, stats),
+        and with texture=True the albedo texture's gradient (height, width, 3) as a fifth element."""
         i = self.info()
         dl = np.zeros((i["height"], i["width"], 4), np.float32)
         g = np.asarray(dloss_dimage, np.float32)
@@ -215,10 +235,16 @@ class Renderer:
         ga = np.zeros(3 * i["n_tris"], np.float32) if albedo else None
         gr = np.zeros(i["n_attrs"], np.float32)
         gs = np.zeros(i["n_attrs"], np.float32)
-        pg = L.ParamGrads(L.fptr(ga) if albedo else None, L.fptr(gr), L.fptr(gs))
+        gt = None
+        if texture:
+            if getattr(self, "_ptex", None) is None:
+                raise L.AuroraError(L.AUR_ESTATE, "texture gradients need set_albedo_texture")
+            gt = np.zeros(self._ptex.shape, np.float32)
+        pg = L.ParamGrads(L.fptr(ga) if albedo else None, L.fptr(gr), L.fptr(gs), L.fptr(gt) if texture else None)
         st = L.Stats()
         L.check(self.lib.aur_adjoint(self._ctx, first_pass, n_passes, L.fptr(dl), C.byref(pg), C.byref(st)))
-        return (ga.reshape(-1, 3) if albedo else None), gr, gs, st.as_dict()
+        out = ((ga.reshape(-1, 3) if albedo else None), gr, gs, st.as_dict())
+        return out + (gt,) if texture else out
 
     # -- instrumentation -----------------------------------------------------
     def set_profiling(self, on: bool) -> None:

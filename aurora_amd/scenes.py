@@ -105,6 +105,147 @@ def write_obj(path: str, verts, faces, normals=None, uvs=None) -> None:
         fh.write("\n".join(lines) + "\n")
 
 
+def write_grid_obj(path: str, verts, faces, normals, uvs) -> None:
+    """Writes a large indexed mesh whose vertices carry their own uv and normal
+    (``f a/a/a b/b/b c/c/c``), with vectorised formatting."""
+    faces = np.asarray(faces, dtype=np.int64) + 1
+    with open(path, "w") as fh:
+        fh.write("# generated by aurora_amd.scenes\n")
+        np.savetxt(fh, np.asarray(verts, np.float64), fmt="v %.7g %.7g %.7g")
+        np.savetxt(fh, np.asarray(uvs, np.float64), fmt="vt %.7g %.7g")
+        np.savetxt(fh, np.asarray(normals, np.float64), fmt="vn %.6g %.6g %.6g")
+        np.savetxt(fh, np.repeat(faces, 3, axis=1), fmt="f %d/%d/%d %d/%d/%d %d/%d/%d")
+
+
+def _write_grid(job):
+    path, mesh = job
+    if "terrain" in mesh:
+        mesh = terrain_patch(**mesh["terrain"])
+    write_grid_obj(path, **mesh)
+    return path
+
+
+def _write_grids(jobs):
+    """Generates (terrain descriptions) and writes large meshes, in parallel
+    worker processes when there are several."""
+    if len(jobs) <= 1:
+        for j in jobs:
+            _write_grid(j)
+        return
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("fork")) as ex:
+        list(ex.map(_write_grid, jobs))
+
+
+def _hash2(ix, iz, seed):
+    """Integer lattice hash to [0, 1) (vectorised, uint32 arithmetic that wraps)."""
+    with np.errstate(over="ignore"):
+        h = (np.asarray(ix).astype(np.uint32) * np.uint32(0x8DA6B343)) ^ \
+            (np.asarray(iz).astype(np.uint32) * np.uint32(0xD8163841))
+        h = h ^ np.uint32(seed & 0xFFFFFFFF)
+        h = h ^ (h >> np.uint32(16))
+        h = h * np.uint32(0x7FEB352D)
+        h = h ^ (h >> np.uint32(15))
+        h = h * np.uint32(0x846CA68B)
+        h = h ^ (h >> np.uint32(16))
+    return (h >> np.uint32(8)).astype(np.float64) * (1.0 / (1 << 24))
+
+
+def fbm2(x, z, seed: int, octaves: int, frequency: float) -> np.ndarray:
+    """Value-noise fractal sum in [-1, 1]: octave k has frequency f 2^k and
+    weight 2^-k (smoothstep-interpolated lattice values), normalised by the
+    total weight."""
+    x = np.asarray(x, np.float64)
+    z = np.asarray(z, np.float64)
+    out = np.zeros(np.broadcast(x, z).shape)
+    wsum = 0.0
+    for k in range(octaves):
+        f, w = frequency * (2.0 ** k), 0.5 ** k
+        px, pz = x * f, z * f
+        ix, iz = np.floor(px), np.floor(pz)
+        fx, fz = px - ix, pz - iz
+        sx, sz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+        ix, iz = ix.astype(np.int64), iz.astype(np.int64)
+        sk = seed * 1013 + k * 7919
+        # the lattice values over the cells the points touch, then gathered
+        x0, z0 = int(ix.min()), int(iz.min())
+        gx, gz = np.meshgrid(np.arange(x0, int(ix.max()) + 2), np.arange(z0, int(iz.max()) + 2))
+        lat = _hash2(gx, gz, sk)
+        jx, jz = ix - x0, iz - z0
+        a, b = lat[jz, jx], lat[jz, jx + 1]
+        c, d = lat[jz + 1, jx], lat[jz + 1, jx + 1]
+        v = (a * (1 - sx) + b * sx) * (1 - sz) + (c * (1 - sx) + d * sx) * sz
+        out += w * (2.0 * v - 1.0)
+        wsum += w
+    return out / wsum
+
+
+def terrain_patch(n: int, size: float, seed: int, amplitude: float, origin=(0.0, 0.0), uv0=(0.0, 0.0),
+                  uv1=(1.0, 1.0), octaves: int = 5, frequency: float = 0.15) -> dict:
+    """An n x n-quad heightfield (2 n^2 triangles) over [x0, x0 + size] x
+    [z0, z0 + size] in object space (y up), heights amplitude * fbm2 of the
+    world position; uvs span [uv0, uv1] linearly in x and z; area-weighted
+    vertex normals.  Returns the mesh as write_grid_obj's keyword arguments."""
+    x0, z0 = origin
+    t = np.linspace(0.0, 1.0, n + 1)
+    X, Z = np.meshgrid(x0 + t * size, z0 + t * size)      # rows advance in z
+    Y = amplitude * fbm2(X, Z, seed, octaves, frequency)
+    verts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3)
+    U, V = np.meshgrid(uv0[0] + t * (uv1[0] - uv0[0]), uv0[1] + t * (uv1[1] - uv0[1]))
+    uvs = np.stack([U, V], axis=-1).reshape(-1, 2)
+    j, i = np.meshgrid(np.arange(n), np.arange(n))
+    a = (i * (n + 1) + j).reshape(-1)
+    b, c = a + 1, a + (n + 1)
+    d = c + 1
+    # counter-clockwise seen from +y: (a, c, b) and (b, c, d)
+    faces = np.concatenate([np.stack([a, c, b], 1), np.stack([b, c, d], 1)])
+    p = verts[faces]
+    fn = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    normals = np.zeros_like(verts)
+    for k in range(3):
+        np.add.at(normals, faces[:, k], fn)
+    normals /= np.maximum(np.linalg.norm(normals, axis=1, keepdims=True), 1e-12)
+    return {"verts": verts, "faces": faces, "normals": normals, "uvs": uvs}
+
+
+def albedo_texture(res: int, seed: int, octaves: int = 5, lo: float = 0.1, hi: float = 0.9) -> np.ndarray:
+    """A res x res RGB FP32 albedo texture of octave value noise, one field per
+    channel, rescaled to [lo, hi]."""
+    t = (np.arange(res) + 0.5) / res
+    out = np.empty((res, res, 3), np.float32)
+    for ch in range(3):
+        v = _grid_fbm(t, seed * 3 + ch, octaves, 4.0)
+        v = (v - v.min()) / max(v.max() - v.min(), 1e-12)
+        out[..., ch] = lo + v * (hi - lo)
+    return out
+
+
+def _grid_fbm(t, seed: int, octaves: int, frequency: float) -> np.ndarray:
+    """fbm2 over the grid (x, z) = (t[j], t[i]), interpolating along x once
+    per lattice row and then along z: the same operations per point as fbm2,
+    in row gathers instead of point gathers."""
+    out = np.zeros((t.size, t.size))
+    wsum = 0.0
+    for k in range(octaves):
+        f, w = frequency * (2.0 ** k), 0.5 ** k
+        p = t * f
+        i = np.floor(p)
+        fr = p - i
+        sm = fr * fr * (3 - 2 * fr)
+        i = i.astype(np.int64)
+        i0 = int(i.min())
+        gx, gz = np.meshgrid(np.arange(i0, int(i.max()) + 2), np.arange(i0, int(i.max()) + 2))
+        lat = _hash2(gx, gz, seed * 1013 + k * 7919)
+        j = i - i0
+        rows = lat[:, j] * (1 - sm) + lat[:, j + 1] * sm          # along x, per lattice row
+        v = rows[j] * (1 - sm)[:, None] + rows[j + 1] * sm[:, None]  # along z
+        out += w * (2.0 * v - 1.0)
+        wsum += w
+    return out / wsum
+
+
 def quad(p0, p1, p2, p3):
     """Two triangles (p0,p1,p2), (p0,p2,p3)."""
     return [p0, p1, p2, p3], [[0, 1, 2], [0, 2, 3]]
@@ -154,12 +295,23 @@ class SceneSpec:
     meshes: dict = field(default_factory=dict)  # obj file name -> (verts, faces, normals|None[, uvs])
     textures: dict = field(default_factory=dict)  # exr file name -> (H, W, 4) float array, FP16 flag
 
+    def attr_index(self, name: str) -> int:
+        """The attribute index of geometry object `name`: geometry objects take
+        indices in sorted key order, lights follow (renderer.cpp:118-134)."""
+        geo = sorted(k for k, o in self.objects.items() if o.get("type") == "geometry")
+        return geo.index(name)
+
     def write(self, out_dir: str, scene_name: str = "scene.asc") -> str:
         os.makedirs(out_dir, exist_ok=True)
         mesh_dir = os.path.join(out_dir, "meshes")
         os.makedirs(mesh_dir, exist_ok=True)
+        grids = []
         for name, mesh in self.meshes.items():
-            write_obj(os.path.join(mesh_dir, name), *mesh)
+            if isinstance(mesh, dict):
+                grids.append((os.path.join(mesh_dir, name), mesh))
+            else:
+                write_obj(os.path.join(mesh_dir, name), *mesh)
+        _write_grids(grids)
         if self.textures:
             from . import exr
             tex_dir = os.path.join(out_dir, "textures")
@@ -316,6 +468,78 @@ def displaced_spheres(width=1920, height=1080, spp=64, maxdepth=8, mindepth=8, n
     s.objects["env"] = {"type": "envlight", "name": "env", "color": [0.1, 0.1, 0.1],
                         "exposure": 0.0, "envmap": "", "transforms": []}
     return s
+
+
+def large_scene(width=3840, height=2160, spp=256, maxdepth=16, mindepth=16, n_spheres=32, levels=7,
+                patches=(4, 2), patch_quads=708, patch_size=10.0, tex_res=2048, seed=5):
+    """The large differentiable scene: displaced icospheres on tessellated
+    noise terrain, with an FP32 albedo texture on the terrain as the gradient
+    target.
+
+    * terrain: patches[0] x patches[1] heightfield patches of patch_quads^2
+      quads (2 * 708^2 = 1,002,528 triangles each), patch_size units square,
+      heights 0.6 * 5-octave value noise; one matte material whose diffuse
+      colour is a texture2DShader over uvs spanning the whole terrain;
+    * n_spheres unit icospheres, Loop-subdivided `levels` times (327,680
+      triangles each at 7) and displaced radially by 3-octave value noise
+      (amplitude 0.05 radius), resting on the terrain, Kelemen-coated as in
+      the displaced-spheres scene;
+    * 8 quad lights of grey radiance ~U[5, 20] and a constant environment of 0.1.
+
+    Returns (spec, truth): the scene and the tex_res x tex_res RGB FP32
+    ground-truth albedo texture (5-octave value noise in [0.1, 0.9]), which
+    the scene's texture file also holds (FP16 rounded).  The terrain's
+    attribute is spec.attr_index("terrain00").
+    """
+    rng = SplitMix64(seed)
+    px, pz = patches
+    W, D = px * patch_size, pz * patch_size
+    s = SceneSpec(options=_options(width, height, spp, maxdepth, mindepth),
+                  camera={"fov": 0.62, "transforms": [{"translate": [0.0, 0.55 * D + 3.0, 0.5 * D + 0.45 * W]},
+                                                      {"rotateX": -30.0}]})
+    truth = albedo_texture(tex_res, seed)
+    rgba = np.ones((tex_res, tex_res, 4), np.float32)
+    rgba[..., :3] = truth
+    s.textures["terrain_albedo.exr"] = (rgba, True)
+    s.shaders["terrain_albedo"] = {"type": "texture2DShader", "attributetype": "color",
+                                   "texturename": "textures/terrain_albedo.exr"}
+    s.materials["terrain"] = {"type": "matteMaterial", "color": "terrain_albedo"}
+    amp, tseed = 0.6, int(rng.next_u64() & 0x7FFFFFFF)
+    for i in range(px):
+        for j in range(pz):
+            name = "terrain%d%d" % (i, j)
+            # generated when the scene is written
+            s.meshes[name + ".obj"] = {"terrain": dict(
+                n=patch_quads, size=patch_size, seed=tseed, amplitude=amp,
+                origin=(-W / 2 + i * patch_size, -D / 2 + j * patch_size),
+                uv0=(i / px, j / pz), uv1=((i + 1) / px, (j + 1) / pz))}
+            s.objects[name] = _geo(name, name + ".obj", "terrain")
+    iv, iff = icosahedron()
+    s.meshes["icosahedron.obj"] = (iv, iff, iv)
+    placed = []
+    for k in range(n_spheres):
+        for _ in range(1000):
+            x, z = rng.uniform(-W / 2 + 1.5, W / 2 - 1.5), rng.uniform(-D / 2 + 1.5, D / 2 - 1.5)
+            if all((x - a) ** 2 + (z - b) ** 2 > 2.3 ** 2 for a, b in placed):
+                break
+        placed.append((x, z))
+        y = float(amp * fbm2(np.array(x), np.array(z), tseed, 5, 0.15)) + 1.0
+        mname = "sphere%02d_mat" % k
+        s.materials[mname] = {"type": "kelemenMaterial", "diffusecolor": [rng.uniform(0.1, 0.9) for _ in range(3)],
+                              "speccolor": [1.0, 1.0, 1.0], "reflectance": 0.04,
+                              "roughness": rng.uniform(0.05, 0.8), "distribution": "ggx"}
+        name = "sphere%02d" % k
+        s.objects[name] = _geo(name, "icosahedron.obj", mname, [{"translate": [x, y, z]}], subdlevels=levels,
+                               displace={"amplitude": 0.05, "octaves": 3, "frequency": 2.0,
+                                         "seed": int(rng.next_u64() & 0x7FFFFFFF)})
+    for k in range(8):
+        e = rng.uniform(5.0, 20.0)
+        x, z = rng.uniform(-W / 2 + 2, W / 2 - 2), rng.uniform(-D / 2 + 2, D / 2 - 2)
+        s.objects["light%d" % k] = _square_light("light%d" % k, [e, e, e], 1.0, 1.0,
+                                                 [{"translate": [x, 10.0, z]}, {"rotateX": -90.0}])
+    s.objects["env"] = {"type": "envlight", "name": "env", "color": [0.1, 0.1, 0.1],
+                        "exposure": 0.0, "envmap": "", "transforms": []}
+    return s, truth
 
 
 def batch_job(seed: int, k: int) -> SceneSpec:
@@ -564,6 +788,8 @@ def write_config(name: str, out_dir: str, **kw) -> str:
         spec = displaced_spheres(**kw)
     elif name == "textured_stage":
         spec = textured_stage(**kw)
+    elif name in ("large_scene", "c5"):
+        spec, _ = large_scene(**kw)
     else:
         raise ValueError("unknown workload %r" % name)
     return spec.write(out_dir)

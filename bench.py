@@ -21,7 +21,9 @@ the batched uneven workload (64 seeded scenes of 1e3..5e5 triangles at
 persistent launch and of per-scene launches, and per-job latency p50/p99.
 ``box_room`` is the CPU-runnable 256x256, 16 spp, depth-5 box room: GPU and
 host-CPU Mpaths/s on the same frame, with both images' RMSE against a
-4096-spp oracle reference.
+4096-spp oracle reference.  ``large_scene`` is the 3840x2160, depth-16,
+18.5M-triangle differentiable scene: forward and texture-adjoint Mpaths/s and
+a central-difference check of the 2048x2048 albedo texture's gradient.
 """
 from __future__ import annotations
 
@@ -308,6 +310,89 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
             "fd_checked": int(len(pick)), "fd_h": h, "fd_max_rel_err": float("%.3g" % worst)}
 
 
+def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd: int = 64):
+    """The large differentiable scene (3840x2160, depth 16, 32 displaced level-7
+    icospheres on 8 terrain patches of 1M triangles: 18.5M triangles; a
+    2048x2048 RGB FP32 albedo texture on the terrain): scene build time,
+    forward Mpaths/s over fwd_passes passes, path-replay adjoint Mpaths/s of
+    the texture gradient over adj_passes passes, and central differences
+    (common random numbers, h = 1e-3) on n_fd seeded texels at fd_passes
+    passes."""
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer
+    tmp = tempfile.mkdtemp(prefix="aurora_large_")
+    t0 = time.time()
+    spec, truth = scenes.large_scene()
+    path = spec.write(tmp)
+    gen_s = time.time() - t0
+    _progress("large scene written in %.1f s" % gen_s)
+    t0 = time.time()
+    r = Renderer(path, os.path.join(tmp, "config.json"))
+    load_s = time.time() - t0
+    r.build(width=4, max_leaf=1)
+    _hip_sync()
+    build_s = time.time() - t0 - load_s
+    info = r.info()
+    _progress("large scene: %d triangles loaded in %.1f s, built in %.1f s" % (info["n_tris"], load_s, build_s))
+    attrs = [spec.attr_index(k) for k in sorted(spec.objects) if k.startswith("terrain")]
+    W, H = info["width"], info["height"]
+    # the inverse problem's state: texture at 0.5, loss against the true texture's image
+    r.set_albedo_texture(truth, attrs)
+    r.set_seed(21)
+    target, _ = r.render_passes(0, fd_passes)
+    tex = np.full_like(truth, 0.5)
+    r.set_albedo_texture(tex, attrs)
+    r.set_seed(21)
+    img, _ = r.render_passes(0, fd_passes)
+    dl = ((2.0 / (W * H * 3)) * (img[..., :3].astype(np.float64) - target[..., :3]) / fd_passes).astype(np.float32)
+    # forward rate
+    r.render_passes(0, 1, fetch=False)
+    _hip_sync()
+    t0 = time.perf_counter()
+    _, fst = r.render_passes(0, fwd_passes, fetch=False)
+    _hip_sync()
+    fwd_s = time.perf_counter() - t0
+    # adjoint rate (texture gradient only)
+    r.adjoint(0, 1, dl, albedo=False, texture=True)
+    _hip_sync()
+    t0 = time.perf_counter()
+    _, _, _, ast, gt_rate = r.adjoint(0, adj_passes, dl, albedo=False, texture=True)
+    _hip_sync()
+    adj_s = time.perf_counter() - t0
+    # gradient check at fd_passes passes
+    r.set_seed(21)
+    _, _, _, _, gt = r.adjoint(0, fd_passes, dl * fd_passes, albedo=False, texture=True)
+    flat = np.abs(gt).reshape(-1)
+    cand = np.flatnonzero(flat > 1e-2 * flat.max())
+    pick = np.random.default_rng(64).permutation(cand)[:n_fd]
+    h, worst = 1e-3, 0.0
+
+    def loss(t):
+        r.set_albedo_texture(t, attrs)
+        r.set_seed(21)
+        im, _ = r.render_passes(0, fd_passes)
+        return float(np.sum(dl.astype(np.float64) * im[..., :3].astype(np.float64)))
+
+    base = tex.reshape(-1)
+    for k in pick:
+        tp, tm = base.copy(), base.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp.reshape(tex.shape)) - loss(tm.reshape(tex.shape))) / (2 * h)
+        g = float(gt.reshape(-1)[k])
+        worst = max(worst, abs(g - fd) / max(abs(fd), 1e-3 * float(flat.max())))
+    r.close()
+    return {"workload": "large_scene_3840x2160_depth16", "triangles": int(info["n_tris"]),
+            "texture": [int(truth.shape[1]), int(truth.shape[0])], "scene_write_s": round(gen_s, 1),
+            "load_s": round(load_s, 2), "bvh_build_s": round(build_s, 2),
+            "forward_passes": fwd_passes, "forward_Mpaths_s": round(fst["paths"] / fwd_s / 1e6, 3),
+            "forward_Mrays_s": round((fst["ext_rays"] + fst["shadow_rays"]) / fwd_s / 1e6, 3),
+            "adjoint_passes": adj_passes, "adjoint_Mpaths_s": round(ast["paths"] / adj_s / 1e6, 3),
+            "adjoint_vs_forward": round((adj_s / adj_passes) / (fwd_s / fwd_passes), 3),
+            "texels_with_gradient": int(np.count_nonzero(gt_rate.reshape(-1, 3).any(1))),
+            "fd_checked": int(len(pick)), "fd_h": h, "fd_passes": fd_passes, "fd_max_rel_err": float("%.3g" % worst)}
+
+
 def _warm_jobs(rs, info, spp, pool):
     """A job list of leading passes with at least `pool` paths in total (or all
     of the work, if that is less): a batched launch of it sizes the executor's
@@ -457,6 +542,7 @@ def main(argv=None):
     ap.add_argument("--levels", type=int, default=7)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 18.5M-triangle differentiable scene")
     ap.add_argument("--batch-plain-launches", action="store_true",
                     help="batched launch without graph replay (for kernel tracing)")
     ap.add_argument("--adam-steps", type=int, default=200,
@@ -549,6 +635,9 @@ def main(argv=None):
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(512, 32, 6, adam_steps=args.adam_steps)
         _progress("adjoint leg done")
+    if not args.no_large:
+        out["large_scene"] = large_leg()
+        _progress("large scene leg done")
     if exp is not None:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(exp, args.cpu_seconds, threads, max(1, info["pixelsamples"]))

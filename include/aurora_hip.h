@@ -315,10 +315,20 @@ int aur_get_launch(aur_ctx *ctx, aur_launch_opts *out);
  * roughness sets Kelemen roughness and metal roughnessA/B; spec_weight sets a
  * Kelemen material's speccolor to (s, s, s).  Values are clamped to >= 1e-6. */
 int aur_set_material_params(aur_ctx *ctx, const float *roughness, const float *spec_weight);
+/* An FP32 RGB albedo texture (width * height texels, row 0 at the top, 3
+ * floats each) that replaces the diffuse colour, and any diffuse shader, of
+ * the materials listed in attrs (matte or Kelemen).  A hit reads the texel
+ * texture2DShader.h:28-31 would read at its interpolated uv (nearest texel,
+ * Texture2D::read's clamps), without the FP16 rounding of Imf::Rgba, so the
+ * texels can be optimised.  Needs a built scene with per-corner uvs; rgb =
+ * NULL removes it.  Calls that keep the size and the materials only upload
+ * the new texels. */
+int aur_set_albedo_texture(aur_ctx *ctx, int width, int height, const float *rgb, const int *attrs, int n_attrs);
 typedef struct {
     float *albedo_tri;   /* 3 * n_tris, or NULL (needs aur_set_tri_albedo) */
     float *roughness;    /* n_attrs, or NULL */
     float *spec_weight;  /* n_attrs, or NULL */
+    float *albedo_texture; /* 3 * width * height of the albedo texture, or NULL (needs aur_set_albedo_texture) */
 } aur_param_grads;
 /* Path-replay adjoint of passes [first_pass, first_pass + n_passes) rendered
  * with the current seed: given dL/dI for the mean image I (4 * W * H floats,

@@ -1,0 +1,128 @@
+"""GPU: the FP32 albedo texture (aur_set_albedo_texture) and its path-replay
+gradient, on a small instance of the large differentiable scene (displaced
+subdivided spheres on noise terrain, the texture on the terrain).
+
+* forward: with texel values that FP16 represents exactly, the albedo texture
+  renders the image the scene's own texture2DShader renders, bit for bit (the
+  shader path is pinned against the oracle in test_gpu_textures);
+* gradient: the adjoint's texel gradients of L = sum dL/dI . I against
+  central differences with common random numbers (relative error < 1e-2);
+* the per-triangle albedo override with albedo gradients not requested
+  leaves the other gradients unchanged (no writes through a missing buffer).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _half_exact(x):
+    return np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def small_large_scene(tmp_path_factory):
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer
+    d = str(tmp_path_factory.mktemp("large_small"))
+    spec, truth = scenes.large_scene(width=96, height=54, spp=2, maxdepth=4, mindepth=4, n_spheres=3, levels=3,
+                                     patches=(2, 1), patch_quads=20, patch_size=6.0, tex_res=32, seed=7)
+    path = spec.write(d)
+    r = Renderer(path, os.path.join(d, "config.json"))
+    r.build()
+    yield r, spec, _half_exact(truth)
+    r.close()
+
+
+def test_albedo_texture_matches_texture_shader(small_large_scene):
+    r, spec, truth = small_large_scene
+    r.set_albedo_texture(None)
+    r.set_seed(5)
+    ref, st = r.render_passes(0, 2)
+    r.set_albedo_texture(truth, [spec.attr_index("terrain00"), spec.attr_index("terrain10")])
+    r.set_seed(5)
+    got, st2 = r.render_passes(0, 2)
+    assert np.array_equal(got, ref)
+    assert st["paths"] == st2["paths"] and st["shadow_rays"] == st2["shadow_rays"]
+    # a different texture changes the terrain only where it is seen
+    r.set_albedo_texture(truth * 0.5, [spec.attr_index("terrain00"), spec.attr_index("terrain10")])
+    r.set_seed(5)
+    half, _ = r.render_passes(0, 2)
+    assert np.mean(half[..., :3] < ref[..., :3]) > 0.3
+    r.set_albedo_texture(None)
+
+
+def test_albedo_texture_gradient_matches_central_differences(small_large_scene):
+    r, spec, truth = small_large_scene
+    attrs = [spec.attr_index("terrain00"), spec.attr_index("terrain10")]
+    info = r.info()
+    H, W = info["height"], info["width"]
+    rng = np.random.default_rng(3)
+    dldi = rng.uniform(-1, 1, size=(H, W, 3)).astype(np.float32)
+    tex = truth.copy()
+    r.set_albedo_texture(tex, attrs)
+    r.set_seed(9)
+    ga, gr, gs, st, gt = r.adjoint(0, 2, dldi, albedo=False, texture=True)
+    assert gt.shape == tex.shape and st["paths"] == 2 * H * W
+    assert np.count_nonzero(gt) > 100
+
+    def loss(t):
+        r.set_albedo_texture(t, attrs)
+        r.set_seed(9)
+        img, _ = r.render_passes(0, 2)
+        return float(np.sum(dldi.astype(np.float64) * img[..., :3].astype(np.float64) / 2))
+
+    flat = np.abs(gt).reshape(-1)
+    idx = list(np.argsort(-flat)[:16]) + list(rng.choice(np.flatnonzero(flat > 0.05 * flat.max()), 8, replace=False))
+    h = 2e-3
+    errs = []
+    for k in idx:
+        tp, tm = tex.copy().reshape(-1), tex.copy().reshape(-1)
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp.reshape(tex.shape)) - loss(tm.reshape(tex.shape))) / (2 * h)
+        g = float(gt.reshape(-1)[k])
+        errs.append(abs(fd - g) / max(abs(fd), abs(g), 1e-6))
+    r.set_albedo_texture(None)
+    assert max(errs) < 1e-2, errs
+
+
+def test_albedo_texture_errors(small_large_scene):
+    from aurora_amd._lib import AuroraError
+    r, spec, truth = small_large_scene
+    with pytest.raises(AuroraError):          # a light is not a diffuse material
+        r.set_albedo_texture(truth, [r.info()["n_attrs"] - 1])
+    with pytest.raises(AuroraError):          # no texture set
+        r.adjoint(0, 1, np.zeros((r.info()["height"], r.info()["width"], 3), np.float32), albedo=False, texture=True)
+
+
+def test_tri_albedo_without_albedo_gradients(gpu_box_room_kelemen):
+    """Per-triangle albedo set, albedo gradients not requested: the roughness
+    and spec gradients equal those of the call that requests all three."""
+    r = gpu_box_room_kelemen
+    info = r.info()
+    rng = np.random.default_rng(4)
+    r.set_tri_albedo(rng.uniform(0.2, 0.8, size=(info["n_tris"], 3)).astype(np.float32))
+    dl = rng.uniform(-1, 1, size=(info["height"], info["width"], 3)).astype(np.float32)
+    r.set_seed(2)
+    ga, gr, gs, _ = r.adjoint(0, 1, dl, albedo=True)
+    r.set_seed(2)
+    none, gr2, gs2, _ = r.adjoint(0, 1, dl, albedo=False)
+    assert none is None
+    np.testing.assert_allclose(gr2, gr, rtol=1e-4, atol=1e-7)
+    np.testing.assert_allclose(gs2, gs, rtol=1e-4, atol=1e-7)
+    r.set_tri_albedo(None)
+
+
+@pytest.fixture(scope="module")
+def gpu_box_room_kelemen(tmp_path_factory):
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer
+    d = str(tmp_path_factory.mktemp("kel"))
+    path = scenes.write_config("box_room_kelemen", d, res=48, spp=2, maxdepth=3, mindepth=3)
+    r = Renderer(path, os.path.join(d, "config.json"))
+    r.build()
+    yield r
+    r.close()
