@@ -23,7 +23,9 @@
 // detached too, which is unbiased because they sum to one per direction.
 // The firefly clamp factor 0.2^k is applied to the incoming gradient.
 //
-// One thread replays one path (both passes); no per-vertex state is stored.
+// The first replay is the wavefront renderer itself, recording each path's
+// radiance and clamp steps (run_paths with PathSource::adj_rec); the second
+// runs one thread per path (k_adj_grad); no per-vertex state is stored.
 // Gradients of small scenes are first summed per workgroup in LDS and then
 // added to HBM with one atomic per parameter per workgroup.
 #include "shade_common.h"
@@ -267,33 +269,6 @@ __device__ inline void adj_path(const SceneView &sv, const AdjParams &ap, int i,
     *rng = make_path_rng<false>(sv, nullptr, ap.seed, *pixel, (uint32_t)(ap.first_pass + pass));
 }
 
-// First replay of every path: its radiance and clamp steps (two kernels keep
-// each replay's registers apart: one kernel holding both spilled to scratch).
-template <int W>
-__global__ void __launch_bounds__(128) k_adj_forward(SceneView sv, AdjParams ap, float4 *rec, int *ovf, int ovf_per_thread) {
-    __shared__ int s_stack[AUR_STACK_LDS * 128];
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    TravStack st;
-    st.init(s_stack, 128, ovf, ovf_per_thread);
-    const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
-    GradSink g;
-    g.albedo = g.rough = g.spec = g.ptex = nullptr;
-    unsigned long long n_ext = 0, n_shadow = 0, n_paths = 0;
-    for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
-        int pixel;
-        PathRngT<false> rng;
-        adj_path(sv, ap, i, &pixel, &rng);
-        f3 L;
-        int k;
-        bool miss;
-        const f3 zero = mk3(0.f, 0.f, 0.f);
-        replay<W, false>(sv, nodes, st, pixel, rng, &L, &k, &miss, zero, zero, g, &n_ext, &n_shadow);
-        ++n_paths;
-        rec[i] = make_float4(L.x, L.y, L.z, __int_as_float(k | (miss ? 0x10000 : 0)));
-    }
-    adj_count(ap.stats, (unsigned)n_paths, (unsigned)n_ext, (unsigned)n_shadow);
-}
-
 // Second replay, carrying the remaining radiance, for paths with a nonzero
 // incoming gradient.  Gradients of small scenes are summed per workgroup in
 // dynamically sized LDS first.
@@ -382,17 +357,28 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     sovf.reserve(sizeof(int) * (size_t)blocks * 128 * ovf);
     rec.reserve(sizeof(float4) * (size_t)std::max<long long>(n, 1));
     const size_t lds = ap.use_lds ? sizeof(float) * (size_t)((d_galb ? 3 * c->n_tris : 0) + 2 * c->n_mats) : 0;
+    // first replay: the wavefront renderer traces the same paths (same
+    // counter-based random numbers) and records each path's radiance and
+    // clamp steps instead of adding it to a framebuffer
+    PathSource src;
+    src.mode = 0;
+    src.first_pass = first_pass;
+    src.ls = 1;
+    src.pixel = src.sample = nullptr;
+    src.slots = nullptr;
+    src.adj_rec = rec.as<float4>();
+    src.rec_base = 0;
+    aur_stats fst;
+    run_paths(c, src, (size_t)n, nullptr, &fst);
+    // second replay, carrying the remaining radiance
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, c->stream));
-    if (c->bvh.width == 8) {
-        hipLaunchKernelGGL(k_adj_forward<8>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+    if (c->bvh.width == 8)
         hipLaunchKernelGGL(k_adj_grad<8>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
-    } else {
-        hipLaunchKernelGGL(k_adj_forward<4>, dim3(blocks), dim3(128), 0, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+    else
         hipLaunchKernelGGL(k_adj_grad<4>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
-    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(e1, c->stream));
     HIP_CHECK(hipEventSynchronize(e1));
@@ -403,11 +389,11 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     unsigned long long h[4];
     HIP_CHECK(hipMemcpy(h, st.p, sizeof h, hipMemcpyDeviceToHost));
     if (stats) {
-        stats->paths = (int64_t)h[0];
-        stats->ext_rays = (int64_t)h[1];
+        stats->paths = fst.paths;
+        stats->ext_rays = fst.ext_rays + (int64_t)h[1];
         stats->hit_rays = 0;
-        stats->shadow_rays = (int64_t)h[2];
-        stats->ms = ms;
+        stats->shadow_rays = fst.shadow_rays + (int64_t)h[2];
+        stats->ms = fst.ms + ms;
     }
 }
 

@@ -197,6 +197,11 @@ struct PathSource {
     const int *pixel;      // device, mode 1
     const int *sample;     // device, mode 1
     const float *slots;    // device, optional injected uniforms
+    // mode 0 with ls = 1: per-path records for the adjoint instead of the
+    // framebuffer: rec[rec_base + path] = (unclamped radiance, clamp steps |
+    // 0x10000 for a camera miss), path = pass * w * h + tiled position
+    float4 *adj_rec = nullptr;
+    long long rec_base = 0;
 };
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats);
 // Paths of passes [first_pass, first_pass + n_passes) of a w*h image: one per

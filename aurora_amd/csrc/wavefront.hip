@@ -186,6 +186,17 @@ __global__ void k_finish(SceneView sv, WfView wv) {
             if (sub > 0) { ext_dup += 1; hit_dup += cam_miss ? 0 : 1; }
         }
         wv.out[slot] = make_float4(L.x, L.y, L.z, w);
+        if (wv.src.adj_rec) {
+            // the adjoint's first replay: radiance before the clamp and the
+            // number of 0.2 steps the clamp took (integrator.cpp:276-283)
+            f3 Lc = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
+            int k = 0;
+            if (!(m.x & F_NOCLAMP))
+                while (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY) { Lc = mul3(Lc, 0.2f); ++k; }
+            const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
+            wv.src.adj_rec[wv.src.rec_base + m.y] =
+                make_float4(a.x + b.x, a.y + b.y, a.z + b.z, __int_as_float(k | (cam_miss ? 0x10000 : 0)));
+        }
     }
     if (wv.src.mode == 0 && wv.src.ls > 1) {
         for (int o = 32; o > 0; o >>= 1) {
@@ -500,10 +511,13 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
             s.sample = src.sample + done;
             s.slots = src.slots ? src.slots + done * (size_t)(2 + AUR_SLOTS_PER_VERTEX * c->scene.opt.maxdepth) : nullptr;
         }
+        if (src.adj_rec) s.rec_base = src.rec_base + (long long)done;
         WfView wv = wf_view(c, s);
         hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, wv.counts, wv.stats, 0);
         run_batch(c, sv, wv, (int)n);
-        if (src.mode == 0) {
+        if (src.adj_rec) {
+            // records only
+        } else if (src.mode == 0) {
             int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));
             TIMED(c, KT_RESOLVE, hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh,
                                                     s.first_pass, passes, ls, out_rgba));
