@@ -8,6 +8,9 @@ adjoint (``aur_adjoint``) with the same random numbers.  ``make_problem`` and
 512x512, 32 spp forward and adjoint, depth 6, ground truth albedo
 ~U[0.1, 0.9]^3 per triangle, roughness ~U[0.05, 0.8] and spec weight ~U[0, 1]
 per material, optimised from (0.5, 0.4, 0.5) with Adam on an L2 image loss.
+``TextureRenderFunction`` (``make_texture_function``) does the same for an
+FP32 albedo texture (``aur_set_albedo_texture``), the gradient target of the
+large differentiable scene.
 """
 from __future__ import annotations
 
@@ -61,6 +64,31 @@ def make_function():
                     None, None, None)
 
     return RenderFunction
+
+
+def make_texture_function():
+    """image = render(albedo texture (H, W, 3)) for a renderer, the materials
+    the texture drives, and a pass range: forward through
+    aur_set_albedo_texture + aur_render, backward through aur_adjoint's
+    texture gradient (the same seed and texture)."""
+    torch = _torch()
+
+    class TextureRenderFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, tex, renderer, attrs, first_pass, passes):
+            renderer.set_albedo_texture(tex.detach().cpu().numpy().astype(np.float32), attrs)
+            acc, _ = renderer.render_passes(first_pass, passes)
+            ctx.renderer, ctx.first_pass, ctx.passes = renderer, first_pass, passes
+            ctx.dev = tex.device
+            return torch.from_numpy(acc[..., :3] / passes).to(tex.device)
+
+        @staticmethod
+        def backward(ctx, grad_img):
+            g = grad_img.detach().cpu().numpy().astype(np.float32)
+            *_, gt = ctx.renderer.adjoint(ctx.first_pass, ctx.passes, g, albedo=False, texture=True)
+            return torch.from_numpy(gt).to(ctx.dev), None, None, None, None
+
+    return TextureRenderFunction
 
 
 @dataclass

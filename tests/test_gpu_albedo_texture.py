@@ -89,6 +89,35 @@ def test_albedo_texture_gradient_matches_central_differences(small_large_scene):
     assert max(errs) < 1e-2, errs
 
 
+def test_texture_autograd_descends(small_large_scene):
+    """Adam through the texture autograd function from a flat 0.5 texture:
+    the L2 image loss against the true texture's image falls."""
+    torch = pytest.importorskip("torch")
+    from aurora_amd import inverse
+    r, spec, truth = small_large_scene
+    attrs = [spec.attr_index("terrain00"), spec.attr_index("terrain10")]
+    r.set_albedo_texture(truth, attrs)
+    r.set_seed(77)
+    target, _ = r.render_passes(0, 2)
+    target = torch.from_numpy(target[..., :3] / 2)
+    fn = inverse.make_texture_function()
+    tex = torch.full(truth.shape, 0.5, requires_grad=True)
+    opt = torch.optim.Adam([tex], lr=0.05)
+    hist = []
+    for it in range(25):
+        opt.zero_grad()
+        r.set_seed(77)
+        img = fn.apply(tex, r, attrs, 0, 2)
+        loss = torch.mean((img - target) ** 2)
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            tex.clamp_(0.01, 1.0)
+        hist.append(float(loss))
+    r.set_albedo_texture(None)
+    assert hist[-1] < 0.5 * hist[0], hist
+
+
 def test_albedo_texture_errors(small_large_scene):
     from aurora_amd._lib import AuroraError
     r, spec, truth = small_large_scene
