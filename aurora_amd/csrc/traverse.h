@@ -58,16 +58,21 @@ __device__ inline void load_node(const WNode<W> *__restrict__ nodes, int ni, Nod
 // memory.  Only the lane's LDS address and depth are per-lane values; the
 // overflow address is derived from them when needed, so the stack holds two
 // VGPRs across the traversal loop.
+// LDS-qualified pointers: a pop from LDS stays a ds_read instead of being
+// merged with the overflow path's global load into one flat load (which
+// waits on both counters)
+typedef __attribute__((address_space(3))) int lds_int;
+
 struct TravStack {
-    int *lds_base;  // the workgroup's stack array
-    int *lds;       // this lane's entry 0: entry k is lds[k * stride]
+    lds_int *lds_base;  // the workgroup's stack array
+    lds_int *lds;       // this lane's entry 0: entry k is lds[k * stride]
     int stride;     // lanes per workgroup
     int *ovf;       // the workgroup's overflow area: lane t owns ovf[t * ovf_per_thread ...]
     int ovf_per_thread;
     int sp;
     __device__ inline void init(int *s_stack, int lanes, int *ovf_all, int per_thread) {
-        lds_base = s_stack;
-        lds = s_stack + threadIdx.x;
+        lds_base = (lds_int *)s_stack;
+        lds = lds_base + threadIdx.x;
         stride = lanes;
         ovf = ovf_all + (size_t)blockIdx.x * lanes * per_thread;
         ovf_per_thread = per_thread;
