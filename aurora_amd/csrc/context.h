@@ -115,8 +115,11 @@ struct LaunchCfg {
     int trace_block = 128;          // traversal workgroup size: 64, 128 or 256
     int trace_waves_per_cu = 16;    // persistent traversal grid, waves per CU
     // path slots in flight: the batched launch's pool, and the largest chunk
-    // of whole passes a single-scene render traces at once
-    size_t pool_paths = 8u << 20;
+    // of whole passes a single-scene render traces at once (368 bytes of
+    // wavefront state per slot: 24 GB at the default 64 M).  Every chunk ends
+    // in a latency-bound drain (the last bounces of its longest paths), so
+    // larger chunks amortise it over more paths
+    size_t pool_paths = 64u << 20;
 };
 
 // Device state of a batched launch (batch.hip), kept between calls.

@@ -1,9 +1,10 @@
 """Launch-parameter autotuner for the wavefront tracer.
 
 ``autotune`` renders a few short passes of a scene for each candidate and
-keeps the fastest, in two stages: the BVH structure (branching factor,
+keeps the fastest, in three stages: the BVH structure (branching factor,
 triangles per leaf), then the traversal launch shape (workgroup size,
-persistent waves per CU) on the best structure.  ``autotune_batch`` times the
+persistent waves per CU) on the best structure, then the chunk size (paths
+traced at once, ``pool_paths``).  ``autotune_batch`` times the
 batched persistent launch over a list of jobs for each path-pool size (the
 queue depth) and traversal grid.  Results are cached in ``tuning.json`` next
 to this file, per triangle-count bucket for single scenes and under "batch"
@@ -25,13 +26,14 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16}
+DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 64 << 20}
 BATCH_DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 8 << 20}
 WIDTHS = (4, 8)
 LEAVES = (1, 2, 4, 8)
 BLOCKS = (64, 128, 256)
 WAVES = (8, 16, 24, 32)
 POOLS = (2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20)
+SINGLE_POOLS = (16 << 20, 32 << 20, 64 << 20)
 
 
 def _bucket(n_tris: int) -> str:
@@ -53,8 +55,8 @@ def _store(key: str, entry: dict, path: str) -> None:
 
 
 def choose(renderer, width=None, max_leaf=None, table: str = TABLE) -> dict:
-    """Returns {"bvh_width", "max_leaf", "trace_block", "trace_waves_per_cu"}
-    for the renderer's loaded scene."""
+    """Returns {"bvh_width", "max_leaf", "trace_block", "trace_waves_per_cu",
+    "pool_paths"} for the renderer's loaded scene."""
     cfg = dict(DEFAULT)
     try:
         n = renderer.info()["n_tris"]
@@ -113,9 +115,21 @@ def autotune(renderer, passes: int = 2, repeats: int = 3, table: str = TABLE) ->
             launch.append({"trace_block": b, "trace_waves_per_cu": wv,
                            "mpaths_per_s": _time_render(renderer, passes, repeats)})
     launch.sort(key=lambda r: -r["mpaths_per_s"])
+    renderer.set_launch(trace_block=launch[0]["trace_block"], trace_waves_per_cu=launch[0]["trace_waves_per_cu"])
+    # chunk size: passes traced at once, up to pool_paths paths (each chunk
+    # ends in a latency-bound drain); timed on up to 64 passes
+    info = renderer.info()
+    wh = info["width"] * info["height"]
+    n_pass = max(2, min(64, (SINGLE_POOLS[-1] + wh - 1) // wh))
+    pools = []
+    for pool in SINGLE_POOLS:
+        renderer.set_launch(pool_paths=pool)
+        pools.append({"pool_paths": pool, "mpaths_per_s": _time_render(renderer, n_pass, 2)})
+    pools.sort(key=lambda r: -r["mpaths_per_s"])
     entry = {"bvh_width": best["bvh_width"], "max_leaf": best["max_leaf"], "trace_block": launch[0]["trace_block"],
-             "trace_waves_per_cu": launch[0]["trace_waves_per_cu"], "mpaths_per_s": launch[0]["mpaths_per_s"],
-             "structure_candidates": structure, "launch_candidates": launch}
+             "trace_waves_per_cu": launch[0]["trace_waves_per_cu"], "pool_paths": pools[0]["pool_paths"],
+             "mpaths_per_s": launch[0]["mpaths_per_s"], "structure_candidates": structure, "launch_candidates": launch,
+             "pool_candidates": pools}
     apply_launch(renderer, entry)
     _store(_bucket(renderer.info()["n_tris"]), entry, table)
     return entry
