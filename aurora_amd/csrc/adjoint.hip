@@ -28,6 +28,8 @@
 // runs one thread per path (k_adj_grad); no per-vertex state is stored.
 // Gradients of small scenes are first summed per workgroup in LDS and then
 // added to HBM with one atomic per parameter per workgroup.
+#include <hipcub/hipcub.hpp>
+
 #include "shade_common.h"
 #include "traverse.h"
 
@@ -272,8 +274,22 @@ __device__ inline void adj_path(const SceneView &sv, const AdjParams &ap, int i,
 // Second replay, carrying the remaining radiance, for paths with a nonzero
 // incoming gradient.  Gradients of small scenes are summed per workgroup in
 // dynamically sized LDS first.
+// Sort keys of the gradient replay's work: longest paths first (the
+// wavefront replay recorded each path's ray count), so the lanes of a wave
+// replay paths of equal length and the longest ones start first.
+__global__ void k_adj_keys(const float4 *__restrict__ rec, int n, unsigned *__restrict__ keys, int *__restrict__ idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int w = __float_as_int(rec[i].w);
+    keys[i] = 255u - (unsigned)((w >> 17) & 0xff);
+    idx[i] = i;
+}
+
+// Compiled for 6 waves per SIMD (at most 80 VGPRs): more paths in flight
+// outweigh the spills that bound costs.
 template <int W>
-__global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, const float4 *rec, int *ovf, int ovf_per_thread) {
+__global__ void __launch_bounds__(128, 6) k_adj_grad(SceneView sv, AdjParams ap, const float4 *rec, const int *order,
+                                                  int *ovf, int ovf_per_thread) {
     __shared__ int s_stack[AUR_STACK_LDS * 128];
     extern __shared__ float s_grad[];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,7 +311,8 @@ __global__ void __launch_bounds__(128) k_adj_grad(SceneView sv, AdjParams ap, co
     }
     g.ptex = ap.g_ptex;
     unsigned long long n_ext = 0, n_shadow = 0;
-    for (int i = gtid; i < ap.n_paths; i += gridDim.x * blockDim.x) {
+    for (int j = gtid; j < ap.n_paths; j += gridDim.x * blockDim.x) {
+        const int i = order[j];
         const float4 r = rec[i];
         const int kw = __float_as_int(r.w);
         if (kw & 0x10000) continue;  // camera miss: no vertex depends on the parameters
@@ -351,8 +368,10 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     // traversal stack spill area, as for the wavefront kernels
     int need = (c->bvh.width - 1) * (c->bvh.depth + 1) + 2;
     size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 1;
-    // latency-bound replays: 16 waves per CU
-    const int blocks = std::max(1, std::min((int)((n + 127) / 128), c->n_cu * 8));
+    // latency-bound replays: 24 waves per CU (6 per SIMD, the kernel's
+    // launch bound: at 3 waves without it the replay measured 25 % slower
+    // even though the bound spills a few registers)
+    const int blocks = std::max(1, std::min((int)((n + 127) / 128), c->n_cu * 12));
     DevBuf sovf, rec;
     sovf.reserve(sizeof(int) * (size_t)blocks * 128 * ovf);
     rec.reserve(sizeof(float4) * (size_t)std::max<long long>(n, 1));
@@ -370,15 +389,31 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     src.rec_base = 0;
     aur_stats fst;
     run_paths(c, src, (size_t)n, nullptr, &fst);
+    // replay order: longest paths first
+    DevBuf keys[2], idx[2], stemp;
+    for (int k = 0; k < 2; ++k) {
+        keys[k].reserve(sizeof(unsigned) * (size_t)std::max<long long>(n, 1));
+        idx[k].reserve(sizeof(int) * (size_t)std::max<long long>(n, 1));
+    }
+    hipLaunchKernelGGL(k_adj_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, rec.as<float4>(), (int)n,
+                       keys[0].as<unsigned>(), idx[0].as<int>());
+    size_t st_bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, st_bytes, keys[0].as<unsigned>(), keys[1].as<unsigned>(),
+                                                 idx[0].as<int>(), idx[1].as<int>(), (int)n, 0, 8, c->stream));
+    stemp.reserve(std::max<size_t>(st_bytes, 1));
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stemp.p, st_bytes, keys[0].as<unsigned>(), keys[1].as<unsigned>(),
+                                                 idx[0].as<int>(), idx[1].as<int>(), (int)n, 0, 8, c->stream));
     // second replay, carrying the remaining radiance
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, c->stream));
     if (c->bvh.width == 8)
-        hipLaunchKernelGGL(k_adj_grad<8>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+        hipLaunchKernelGGL(k_adj_grad<8>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(),
+                           idx[1].as<int>(), sovf.as<int>(), (int)ovf);
     else
-        hipLaunchKernelGGL(k_adj_grad<4>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(), sovf.as<int>(), (int)ovf);
+        hipLaunchKernelGGL(k_adj_grad<4>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(),
+                           idx[1].as<int>(), sovf.as<int>(), (int)ovf);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(e1, c->stream));
     HIP_CHECK(hipEventSynchronize(e1));

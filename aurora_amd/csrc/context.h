@@ -202,7 +202,8 @@ struct PathSource {
     const float *slots;    // device, optional injected uniforms
     // mode 0 with ls = 1: per-path records for the adjoint instead of the
     // framebuffer: rec[rec_base + path] = (unclamped radiance, clamp steps |
-    // 0x10000 for a camera miss), path = pass * w * h + tiled position
+    // 0x10000 for a camera miss | rays traced << 17), path = pass * w * h +
+    // tiled position
     float4 *adj_rec = nullptr;
     long long rec_base = 0;
 };

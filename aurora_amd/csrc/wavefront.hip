@@ -194,8 +194,10 @@ __global__ void k_finish(SceneView sv, WfView wv) {
             if (!(m.x & F_NOCLAMP))
                 while (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY) { Lc = mul3(Lc, 0.2f); ++k; }
             const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
-            wv.src.adj_rec[wv.src.rec_base + m.y] =
-                make_float4(a.x + b.x, a.y + b.y, a.z + b.z, __int_as_float(k | (cam_miss ? 0x10000 : 0)));
+            // bits 0-15 clamp steps, bit 16 camera miss, bits 17-24 rays traced
+            wv.src.adj_rec[wv.src.rec_base + m.y] = make_float4(
+                a.x + b.x, a.y + b.y, a.z + b.z,
+                __int_as_float(k | (cam_miss ? 0x10000 : 0) | (min(path_ext_rays(m.x), 255) << 17)));
         }
     }
     if (wv.src.mode == 0 && wv.src.ls > 1) {
