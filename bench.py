@@ -5,7 +5,7 @@ Workload: the 1920x1080, 64 spp, max-depth-8 scene of eight displaced, Loop-
 subdivided (7 levels) icospheres (2,621,450 triangles with the light quads),
 Kelemen/GGX materials, four quad lights and a constant environment.  A step is
 one batch of ``--passes-per-step`` full-frame passes (each pass = one path per
-pixel), traced as one chunk of 33 M paths; the default 4 steps x 16 passes
+pixel), traced as one chunk of 66 M paths; the default 2 steps x 32 passes
 render the full 64-spp frame.
 
 Prints one JSON line.  ``value`` is Mpaths/s of the timed steps with the scene
@@ -533,9 +533,9 @@ def copy_calibration(mib: int, reps: int):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--passes-per-step", type=int, default=16)
+    ap.add_argument("--passes-per-step", type=int, default=32)
     ap.add_argument("--bvh-width", type=int, default=0, help="0: autotuned choice")
     ap.add_argument("--max-leaf", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
