@@ -48,6 +48,7 @@ struct JobDev {
     int ls;              // light samples: paths per pixel in passes after pass 0
     long long start;     // first work index of the job in launch order
     long long n_paths;
+    int root;            // the job's root node in the launch's node arena
 };
 
 enum { J_EXT = 0, J_HIT = 1, J_SHADOW = 2, J_PATHS = 3, J_STRIDE = 4 };
@@ -81,6 +82,8 @@ __device__ inline unsigned pack_counts(int ext, int hit, int shadow, long long i
 
 struct BatchView {
     const JobDev *jobs;      // in launch order
+    const void *nodes;       // every job's BVH in one node arena (WNode<W>)
+    const float4 *tris;      // and their triangle records in one array
     const long long *starts; // jobs[k].start, for the binary search
     int n_jobs;
     long long total;         // paths over all jobs
@@ -182,13 +185,36 @@ __global__ void k_bclear_queue(int *counts, int q) {
     if (threadIdx.x == 0) counts[q] = 0;
 }
 
-// Batched extension: the single-scene kernel's refilled lanes, each lane with
-// its own ray's scene (the BVH and triangle pointers of the job are per-lane
-// values, read when the ray is fetched).  The four pointer registers take it
-// to 72 VGPRs (7 waves per SIMD); held to 64 it spills inside the loop, which
-// measured slower.
+// The node arena of a batched launch: every job's nodes with inner child
+// indices moved by the job's node offset and leaf triangle ranges by its
+// triangle offset, so one pair of pointers (wave-uniform, as in the
+// single-scene kernels) serves every job and a ray only names its root.
+template <int W>
+__global__ void k_arena_nodes(const WNode<W> *__restrict__ src, int n, int node_off, int tri_off,
+                              WNode<W> *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    WNode<W> nd = src[i];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int c = nd.child[k];
+        if (c == 0x7fffffff) continue;  // empty slot
+        if (c >= 0) {
+            nd.child[k] = c + node_off;
+        } else {
+            const int enc = ~c;
+            nd.child[k] = ~((((enc >> 4) + tri_off) << 4) | (enc & 15));
+        }
+    }
+    dst[node_off + i] = nd;
+}
+
+// Batched extension: the single-scene kernel's refilled lanes on the node
+// arena; a ray starts at its job's root.
 template <int W, int B>
-__global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, WfView wv, int q) {
+__global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bextend(const JobDev *__restrict__ jobs,
+                                                                   const WNode<W> *__restrict__ nodes,
+                                                                   const float4 *__restrict__ tris, WfView wv, int q) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[q];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,11 +222,9 @@ __global__ void __launch_bounds__(B) k_bextend(const JobDev *__restrict__ jobs, 
     TravStack st;
     st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
     traverse_stream<W, false, true>(
-        nullptr, nullptr, st, gtid, n, stride, AUR_REFILL_LANES,
-        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, auto &nodes, auto &tris) {
-            const JobDev &jd = jobs[wv.meta[q][i].w];
-            nodes = static_cast<const WNode<W> *>(jd.sv.nodes);
-            tris = jd.sv.tris;
+        nodes, tris, st, gtid, n, stride, AUR_REFILL_LANES,
+        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, int &root) {
+            root = jobs[wv.meta[q][i].w].root;
             const float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
             o = mk3(a.x, a.y, a.z); d = mk3(b.x, b.y, b.z); tmin = a.w; tmax = b.w;
         },
@@ -244,11 +268,12 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__res
     }
 }
 
-// Batched shadow rays: per-lane scene pointers (no per-job waterfall), and the
-// contribution record is read only for unoccluded rays, which keeps the kernel
-// at 64 VGPRs without spills.
+// Batched shadow rays on the node arena, from their job's root; the
+// contribution record is read only for unoccluded rays.
 template <int W, int B>
-__global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bshadow(const JobDev *__restrict__ jobs, WfView wv, int nq) {
+__global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bshadow(const JobDev *__restrict__ jobs,
+                                                                   const WNode<W> *__restrict__ nodes,
+                                                                   const float4 *__restrict__ tris, WfView wv, int nq) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,10 +282,9 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bshadow(const JobDev *
     st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
     for (int i = gtid; i < n; i += stride) {
         const float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1];
-        const JobDev &jd = jobs[__float_as_int(a.w)];
         TravHit h;
-        traverse<W, true>(static_cast<const WNode<W> *>(jd.sv.nodes), jd.sv.tris, mk3(a.x, a.y, a.z),
-                          mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h);
+        traverse<W, true>(nodes, tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h,
+                          jobs[__float_as_int(a.w)].root);
         if (h.prim < 0) {
             const float4 c = wv.sray[3 * i + 2];
             shadow_add(wv, nq, __float_as_int(c.w), c);
@@ -401,6 +425,8 @@ static BatchView batch_view(Ctx *c, int n_jobs, long long total, long long ring_
     v.t_end = b.t_end.as<unsigned long long>();
     v.t0 = b.t0.as<long long>();
     v.jstats = b.jstats.as<unsigned long long>();
+    v.nodes = b.arena_nodes.p;
+    v.tris = b.arena_tris.as<float4>();
     return v;
 }
 
@@ -410,8 +436,12 @@ static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q
     const int sb = c->n_cu * 4;
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, c->stream, bv, wv, q);
     hipLaunchKernelGGL(k_regen, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
-#define AUR_BEXT(w_, b_) hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs, wv, q)
-#define AUR_BSHD(w_, b_) hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs, wv, 1 - q)
+#define AUR_BEXT(w_, b_)                                                                                    \
+    hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs,                       \
+                       static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, q)
+#define AUR_BSHD(w_, b_)                                                                                    \
+    hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs,                       \
+                       static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, 1 - q)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
     if (tex) hipLaunchKernelGGL(k_bshade<true>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
     else hipLaunchKernelGGL(k_bshade<false>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
@@ -577,6 +607,42 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         b.host_segs_cap = 0;
         HIP_CHECK(hipHostMalloc(&b.host_segs, sizeof(ResolveSeg) * (size_t)n_jobs, hipHostMallocDefault));
         b.host_segs_cap = (size_t)n_jobs;
+    }
+    // the node arena: job r's nodes from node_off, its triangles from tri_off
+    {
+        long long node_total = 0, tri_total = 0;
+        std::vector<long long> node_off(n_jobs), tri_off(n_jobs);
+        for (int r = 0; r < n_jobs; ++r) {
+            const Ctx *j = jobs[ord[r]];
+            node_off[r] = node_total;
+            tri_off[r] = tri_total;
+            node_total += j->bvh.n_nodes;
+            tri_total += j->n_tris;
+        }
+        if (node_total > 0x7ffffffeLL || tri_total >= (1LL << 27))
+            throw CapacityError("the batch's BVHs exceed the node arena's index range");
+        const size_t nsz = width == 4 ? sizeof(WNode<4>) : sizeof(WNode<8>);
+        b.arena_nodes.reserve(nsz * (size_t)std::max<long long>(node_total, 1));
+        b.arena_tris.reserve(sizeof(float4) * 3 * (size_t)std::max<long long>(tri_total, 1));
+        for (int r = 0; r < n_jobs; ++r) {
+            const Ctx *j = jobs[ord[r]];
+            const int nn = j->bvh.n_nodes;
+            if (nn > 0) {
+                if (width == 4)
+                    hipLaunchKernelGGL(k_arena_nodes<4>, dim3((nn + 255) / 256), dim3(256), 0, c->stream,
+                                       j->bvh.nodes.as<WNode<4>>(), nn, (int)node_off[r], (int)tri_off[r],
+                                       b.arena_nodes.as<WNode<4>>());
+                else
+                    hipLaunchKernelGGL(k_arena_nodes<8>, dim3((nn + 255) / 256), dim3(256), 0, c->stream,
+                                       j->bvh.nodes.as<WNode<8>>(), nn, (int)node_off[r], (int)tri_off[r],
+                                       b.arena_nodes.as<WNode<8>>());
+                HIP_CHECK(hipGetLastError());
+            }
+            if (j->n_tris > 0)
+                HIP_CHECK(hipMemcpyAsync(b.arena_tris.as<float4>() + 3 * (size_t)tri_off[r], j->bvh.tri_sorted.p,
+                                         sizeof(float4) * 3 * (size_t)j->n_tris, hipMemcpyDeviceToDevice, c->stream));
+            jd[r].root = (int)node_off[r];
+        }
     }
     // the copies read host vectors that outlive the stream synchronisation below
     HIP_CHECK(hipMemcpyAsync(b.jobs.p, jd.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, c->stream));

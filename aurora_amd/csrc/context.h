@@ -126,6 +126,7 @@ struct LaunchCfg {
 struct BatchState {
     DevBuf jobs, starts, t_end, jstats, ctl, t0, segs, chunks;
     DevBuf ring, ring_cnt;          // finished paths' rgba and packed counts by work index
+    DevBuf arena_nodes, arena_tris; // every job's BVH nodes / triangle records, back to back
     void *host_status = nullptr;    // pinned: control block + queue counts, polled by the host
     void *host_segs = nullptr;      // pinned: resolve segment table
     size_t host_segs_cap = 0;

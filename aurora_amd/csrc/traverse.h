@@ -132,10 +132,10 @@ __device__ inline void trav_setup(TravRay &r, f3 o, f3 d, float tmin, float tmax
 }
 
 __device__ inline void trav_begin(TravRay &r, f3 o, f3 d, float tmin, float tmax, TravStack &st, int &node,
-                                  TravHit &h) {
+                                  TravHit &h, int root = 0) {
     trav_setup(r, o, d, tmin, tmax, h);
     st.sp = 0;
-    node = 0;
+    node = root;
 }
 
 // Visits one node: tests its child boxes, tests the leaves they let through,
@@ -213,13 +213,13 @@ __device__ inline bool trav_step(const WNode<W> *__restrict__ nodes, const float
     return trav_visit<W, ANY>(nr, tris, r, st, node, h);
 }
 
-// One whole ray.
+// One whole ray, from node `root` (a batched launch's forest of BVHs).
 template <int W, bool ANY>
 __device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4 *__restrict__ tris, f3 o, f3 d,
-                                float tmin, float tmax, TravStack &st, TravHit &h) {
+                                float tmin, float tmax, TravStack &st, TravHit &h, int root = 0) {
     TravRay r;
     int node;
-    trav_begin(r, o, d, tmin, tmax, st, node, h);
+    trav_begin(r, o, d, tmin, tmax, st, node, h, root);
     while (!trav_step<W, ANY>(nodes, tris, r, st, node, h)) {
     }
 }
@@ -231,11 +231,11 @@ __device__ inline void traverse(const WNode<W> *__restrict__ nodes, const float4
 // while any lane has a ray, so one long ray no longer holds 63 idle lanes,
 // and the fetch cost (a ray load and three divisions) is paid once per group
 // of lanes instead of once per lane.  `load(i, o, d, tmin, tmax)` reads ray i;
-// `done(i, h)` consumes its result.  With PER_LANE, `load(i, o, d, tmin,
-// tmax, nodes, tris)` also points `nodes` and `tris` at the ray's own scene
-// (batched launches).  The per-ray results are those of `traverse`: each
-// ray's visit order does not depend on its neighbours.
-template <int W, bool ANY, bool PER_LANE = false, class Load, class Done>
+// `done(i, h)` consumes its result.  With ROOTED, `load(i, o, d, tmin, tmax,
+// root)` also names the ray's root node (a batched launch's forest of
+// BVHs).  The per-ray results are those of `traverse`: each ray's visit
+// order does not depend on its neighbours.
+template <int W, bool ANY, bool ROOTED = false, class Load, class Done>
 __device__ inline void traverse_stream(const WNode<W> *__restrict__ nodes, const float4 *__restrict__ tris,
                                        TravStack &st, int first, int n, int stride, int refill, Load load,
                                        Done done) {
@@ -252,9 +252,10 @@ __device__ inline void traverse_stream(const WNode<W> *__restrict__ nodes, const
             if (!busy && i < n) {
                 f3 o, d;
                 float tmin, tmax;
-                if constexpr (PER_LANE) load(i, o, d, tmin, tmax, nodes, tris);
+                int root = 0;
+                if constexpr (ROOTED) load(i, o, d, tmin, tmax, root);
                 else load(i, o, d, tmin, tmax);
-                trav_begin(r, o, d, tmin, tmax, st, node, h);
+                trav_begin(r, o, d, tmin, tmax, st, node, h, root);
                 busy = true;
             }
         }
