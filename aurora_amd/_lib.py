@@ -165,6 +165,7 @@ SIGNATURES = {
     "aur_scene_load": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
     "aur_scene_upload": (C.c_int, [_P, C.POINTER(SceneDesc)]),
     "aur_set_tri_albedo": (C.c_int, [_P, _FP]),
+    "aur_display": (C.c_int, [_P, C.POINTER(C.c_uint16)]),
     "aur_set_albedo_texture": (C.c_int, [_P, C.c_int, C.c_int, _FP, _IP, C.c_int]),
     "aur_build_accel": (C.c_int, [_P, C.POINTER(AccelOpts)]),
     "aur_scene_info_get": (C.c_int, [_P, C.POINTER(SceneInfo)]),

@@ -43,6 +43,7 @@ namespace aur {
 struct JobDev {
     SceneView sv;
     float4 *accum;       // the job's framebuffer: sum of per-pass rgba
+    ushort4 *display;    // and its FP16 running mean (display_update)
     uint32_t seed;
     int first_pass;
     int ls;              // light samples: paths per pixel in passes after pass 0
@@ -380,8 +381,10 @@ __global__ void k_bresolve(BatchView bv, const ResolveSeg *__restrict__ segs, in
         const JobDev &jd = bv.jobs[sg.job];
         const int wh = jd.sv.width * jd.sv.height;
         const long long l = t - sg.tbase;
-        float4 *px = jd.accum + tiled_pixel((int)l, jd.sv.width, jd.sv.height);
+        const int pix = tiled_pixel((int)l, jd.sv.width, jd.sv.height);
+        float4 *px = jd.accum + pix;
         float4 s = *px;
+        ushort4 d = jd.display[pix];
         const int blk1 = (int)(sg.lb / wh);
         for (int blk = (int)(sg.la / wh); blk < blk1;) {
             int pass, sub;
@@ -402,9 +405,11 @@ __global__ void k_bresolve(BatchView bv, const ResolveSeg *__restrict__ segs, in
                 }
             }
             s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+            d = display_update(d, v, jd.first_pass + pass + 1);
             blk += n_sub;
         }
         *px = s;
+        jd.display[pix] = d;
     }
 }
 
@@ -553,11 +558,16 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         wh[r] = w * h;
         bool resized = j->fb_w != w || j->fb_h != h;
         j->accum.reserve(sizeof(float4) * (size_t)wh[r]);
-        if (first_pass[ord[r]] == 0 || resized) fill_zero(c, j->accum.p, sizeof(float4) * (size_t)wh[r]);
+        j->display.reserve(sizeof(ushort4) * (size_t)wh[r]);
+        if (first_pass[ord[r]] == 0 || resized) {
+            fill_zero(c, j->accum.p, sizeof(float4) * (size_t)wh[r]);
+            fill_zero(c, j->display.p, sizeof(ushort4) * (size_t)wh[r]);
+        }
         j->fb_w = w;
         j->fb_h = h;
         jd[r].sv = scene_view(j);
         jd[r].accum = j->accum.as<float4>();
+        jd[r].display = j->display.as<ushort4>();
         jd[r].seed = j->seed;
         jd[r].first_pass = first_pass[ord[r]];
         jd[r].ls = j->scene.opt.lightsamples;

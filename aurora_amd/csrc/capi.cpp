@@ -780,7 +780,11 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
         const size_t wh = (size_t)w * h;
         bool resized = c->fb_w != w || c->fb_h != h;
         c->accum.reserve(sizeof(float4) * wh);
-        if (first_pass == 0 || resized) fill_zero(c, c->accum.p, sizeof(float4) * wh);
+        c->display.reserve(sizeof(ushort4) * wh);
+        if (first_pass == 0 || resized) {
+            fill_zero(c, c->accum.p, sizeof(float4) * wh);
+            fill_zero(c, c->display.p, sizeof(ushort4) * wh);
+        }
         c->fb_w = w;
         c->fb_h = h;
         aur_stats st;
@@ -792,6 +796,7 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
             src.pixel = src.sample = nullptr;
             src.slots = nullptr;
             src.ls = c->scene.opt.lightsamples;
+            src.display = c->display.as<ushort4>();
             if (src.ls < 1) throw std::invalid_argument("lightsamples must be at least 1");
             run_paths(c, src, pass_paths(wh, first_pass, n_passes, src.ls), c->accum.as<float4>(), &st);
         }
@@ -828,6 +833,19 @@ int aur_framebuffer(aur_ctx *ctx, float *rgba_sum_host) {
     });
 }
 
+int aur_display(aur_ctx *ctx, uint16_t *rgba_half_host) {
+    return guarded([&] {
+        NEED(ctx);
+        NEED(rgba_half_host);
+        Ctx *c = &ctx->c;
+        set_device(c);
+        if (!c->display.p || c->fb_w <= 0) throw StateError("no framebuffer yet (render first)");
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipMemcpy(rgba_half_host, c->display.p, sizeof(uint16_t) * 4 * (size_t)c->fb_w * c->fb_h,
+                            hipMemcpyDeviceToHost));
+    });
+}
+
 static std::vector<std::pair<std::string, std::string>> meta_list(const char *const *keys, const char *const *values, int n) {
     if (n < 0) throw std::invalid_argument("negative metadata count");
     if (n > 0 && (!keys || !values)) throw std::invalid_argument("null metadata arrays");
@@ -844,7 +862,7 @@ int aur_write_image(aur_ctx *ctx, const char *path, int n_passes, const char *co
     return guarded([&] {
         NEED(ctx);
         NEED(path);
-        if (n_passes <= 0) throw std::invalid_argument("n_passes must be positive");
+        if (n_passes < 0) throw std::invalid_argument("n_passes must not be negative");
         Ctx *c = &ctx->c;
         set_device(c);
         if (!c->accum.p || c->fb_w <= 0) throw StateError("no framebuffer yet (render first)");
@@ -855,9 +873,17 @@ int aur_write_image(aur_ctx *ctx, const char *path, int n_passes, const char *co
         const size_t n = (size_t)c->fb_w * c->fb_h;
         std::vector<float> img(4 * n);
         HIP_CHECK(hipStreamSynchronize(c->stream));
-        HIP_CHECK(hipMemcpy(img.data(), c->accum.p, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
-        const float inv = 1.f / (float)n_passes;
-        for (float &v : img) v *= inv;
+        if (n_passes > 0) {
+            HIP_CHECK(hipMemcpy(img.data(), c->accum.p, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+            const float inv = 1.f / (float)n_passes;
+            for (float &v : img) v *= inv;
+        } else {
+            // the FP16 running mean as the reference's display holds it (the
+            // half -> float -> half round trip of the writer is exact)
+            std::vector<uint16_t> h(4 * n);
+            HIP_CHECK(hipMemcpy(h.data(), c->display.p, sizeof(uint16_t) * 4 * n, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < 4 * n; ++i) img[i] = half_to_float(h[i]);
+        }
         try {
             exr_write(path, c->fb_w, c->fb_h, img.data(), EXR_ZIP, true, meta);
         } catch (const std::runtime_error &e) {

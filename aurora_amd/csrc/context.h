@@ -176,6 +176,7 @@ struct Ctx {
 
     // framebuffer (sum of per-pass rgba)
     DevBuf accum;
+    DevBuf display;  // ushort4 per pixel: OpenexrDisplay's FP16 running mean (display_update)
     int fb_w = 0, fb_h = 0;
 
     Wavefront wf;
@@ -207,6 +208,8 @@ struct PathSource {
     // tiled position
     float4 *adj_rec = nullptr;
     long long rec_base = 0;
+    // mode 0: the FP16 running mean updated with every resolved pass, or null
+    ushort4 *display = nullptr;
 };
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats);
 // Paths of passes [first_pass, first_pass + n_passes) of a w*h image: one per

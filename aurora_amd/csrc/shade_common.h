@@ -55,6 +55,22 @@ __device__ __host__ inline int tiled_pixel(int local, int w, int h) {
 // triplanarTextureShader.h, material.cpp:19-33 (normal map)
 // ---------------------------------------------------------------------------
 __device__ inline float half_bits_to_float(unsigned short v) { return (float)__builtin_bit_cast(_Float16, v); }
+__device__ inline unsigned short float_to_half_bits(float x) { return __builtin_bit_cast(unsigned short, (_Float16)x); }
+
+// OpenexrDisplay::appendValue (openexrDisplay.cpp:61-72) for the pass that is
+// the n-th sample of the pixel: weight = 1/n, pixel = old (1 - weight) +
+// value weight in float, colour and alpha alike, stored as FP16 (Imf::Rgba,
+// rounded to nearest even).
+__device__ inline ushort4 display_update(ushort4 d, float4 v, int n) {
+#pragma clang fp contract(off)
+    const float w = 1.f / (float)n;
+    const float k = 1.f - w;
+    const float r = half_bits_to_float(d.x) * k + v.x * w;
+    const float g = half_bits_to_float(d.y) * k + v.y * w;
+    const float b = half_bits_to_float(d.z) * k + v.z * w;
+    const float a = half_bits_to_float(d.w) * k + v.w * w;
+    return make_ushort4(float_to_half_bits(r), float_to_half_bits(g), float_to_half_bits(b), float_to_half_bits(a));
+}
 
 // Texture2D::read with filterSize 0.  The reference clamps an index past the
 // edge to [0, size], which only an argument beyond 1 + 1/size reaches (no

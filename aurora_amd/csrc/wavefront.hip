@@ -262,18 +262,22 @@ __device__ inline float4 pass_value(const float4 *__restrict__ out, size_t off, 
 }
 
 __global__ void k_resolve(const float4 *__restrict__ out, int wh, int first_pass, int n_passes, int ls,
-                          float4 *__restrict__ accum) {
+                          float4 *__restrict__ accum, ushort4 *__restrict__ display) {
     int stride = gridDim.x * blockDim.x;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < wh; p += stride) {
         float4 s = accum[p];
+        ushort4 d = display ? display[p] : make_ushort4(0, 0, 0, 0);
         size_t off = 0;
         for (int k = 0; k < n_passes; ++k) {
             const int n_sub = (first_pass + k == 0) ? 1 : ls;
             const float4 v = pass_value(out, off, wh, n_sub, p);
             s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+            // the display counts passes from pass 0 (the pixel's n-th sample)
+            if (display) d = display_update(d, v, first_pass + k + 1);
             off += (size_t)n_sub * wh;
         }
         accum[p] = s;
+        if (display) display[p] = d;
     }
 }
 
@@ -522,7 +526,7 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
         } else if (src.mode == 0) {
             int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));
             TIMED(c, KT_RESOLVE, hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh,
-                                                    s.first_pass, passes, ls, out_rgba));
+                                                    s.first_pass, passes, ls, out_rgba, src.display));
         } else {
             HIP_CHECK(hipMemcpyAsync(out_rgba + done, wv.out, sizeof(float4) * n, hipMemcpyDeviceToDevice, c->stream));
         }

@@ -185,9 +185,19 @@ class Renderer:
         L.check(self.lib.aur_framebuffer(self._ctx, L.fptr(out)))
         return out
 
+    def display(self) -> np.ndarray:
+        """The reference display's FP16 running mean of the passes rendered
+        (OpenexrDisplay::appendValue), (H, W, 4) float16."""
+        i = self.info()
+        out = np.zeros((i["height"], i["width"], 4), np.float16)
+        L.check(self.lib.aur_display(self._ctx, out.ctypes.data_as(C.POINTER(C.c_uint16))))
+        return out
+
     def write_image(self, path: str, n_passes: int, meta: dict | None = None) -> None:
-        """The mean image (framebuffer / n_passes) as an FP16 RGBA ZIP EXR with
-        string metadata, written to <path>_tmp and renamed (OpenexrDisplay::draw)."""
+        """The mean image (framebuffer / n_passes; n_passes = 0 writes the
+        reference display's FP16 running mean instead) as an FP16 RGBA ZIP EXR
+        with string metadata, written to <path>_tmp and renamed
+        (OpenexrDisplay::draw)."""
         from .exr import _meta
         keys, vals, n = _meta(meta)
         L.check(self.lib.aur_write_image(self._ctx, path.encode(), int(n_passes), keys, vals, n))

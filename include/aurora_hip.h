@@ -253,10 +253,18 @@ int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_pa
  * framebuffer is OpenexrDisplay's running mean (openexrDisplay.cpp:61-72);
  * divide by the pass count for it. */
 int aur_framebuffer(aur_ctx *ctx, float *rgba_sum_host);
+/* Copies that running mean as the reference keeps it (OpenexrDisplay::
+ * appendValue, openexrDisplay.cpp:61-72): updated with every resolved pass in
+ * pass order, n-th pass weight 1/n, pixel = old (1 - 1/n) + value / n in
+ * float, stored as FP16 RGBA (4*W*H halves, rounded to nearest even).  Passes
+ * are counted from pass 0. */
+int aur_display(aur_ctx *ctx, uint16_t *rgba_half_host);
 
 /* ---- image files (OpenEXR, scanline, single part) ---- */
 /* Replaces OpenexrDisplay::draw (openexrDisplay.cpp:132-169): writes the mean
- * image (framebuffer sum / n_passes) as FP16 RGBA, ZIP compressed, with a
+ * image (framebuffer sum / n_passes; with n_passes = 0 the FP16 running mean
+ * of aur_display, the reference's own pixel values) as FP16 RGBA, ZIP
+ * compressed, with a
  * "renderer" string attribute plus n_meta key/value strings (e.g. the
  * reference's renderTime and RaySpeed, renderer.cpp:292-297), to
  * <path>_tmp and renames it over path. */

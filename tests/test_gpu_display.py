@@ -1,0 +1,67 @@
+"""GPU: the reference display's FP16 running mean (OpenexrDisplay::
+appendValue, openexrDisplay.cpp:61-72), kept beside the FP32 sum.
+
+Each pass's per-pixel value is taken from a fresh context that renders that
+pass alone (its framebuffer starts at zero, so it holds the value exactly);
+the running mean restated in numpy (float32 arithmetic, FP16 storage rounded
+to nearest even) must equal the device's bit for bit, for per-scene renders
+in one call or several, for the batched launch, and in the written EXR."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _running_mean(values):
+    d = np.zeros(values[0].shape, np.float16)
+    for k, v in enumerate(values):
+        w = np.float32(1.0) / np.float32(k + 1)
+        d = (d.astype(np.float32) * (np.float32(1.0) - w) + v.astype(np.float32) * w).astype(np.float16)
+    return d
+
+
+@pytest.fixture(scope="module")
+def room(tmp_path_factory):
+    from aurora_amd import scenes
+    d = str(tmp_path_factory.mktemp("disp"))
+    scenes.write_config("box_room", d, res=48, spp=5, maxdepth=4, mindepth=4)
+    return os.path.join(d, "scene.asc"), os.path.join(d, "config.json")
+
+
+def test_display_is_the_reference_running_mean(room, tmp_path):
+    from aurora_amd.exr import read_exr
+    from aurora_amd.renderer import Renderer, batch_submit
+    P = 5
+    values = []
+    for k in range(P):
+        rk = Renderer(*room)
+        rk.build()
+        rk.set_seed(13)
+        v, _ = rk.render_passes(k, 1)
+        values.append(v)
+        rk.close()
+    want = _running_mean(values)
+    r = Renderer(*room)
+    r.build()
+    r.set_seed(13)
+    r.render_passes(0, P)
+    assert np.array_equal(r.display().view(np.uint16), want.view(np.uint16))
+    # several calls in pass order
+    r.set_seed(13)
+    r.render_passes(0, 2)
+    r.render_passes(2, 3)
+    assert np.array_equal(r.display().view(np.uint16), want.view(np.uint16))
+    # the batched launch keeps the same display
+    ex = Renderer()
+    r.set_seed(13)
+    batch_submit(ex, [(r, 0, P)])
+    assert np.array_equal(r.display().view(np.uint16), want.view(np.uint16))
+    # the written file holds it
+    path = str(tmp_path / "disp.exr")
+    r.write_image(path, 0)
+    img = read_exr(path)
+    assert np.array_equal(np.asarray(img, np.float32).astype(np.float16).view(np.uint16), want.view(np.uint16))
+    ex.close()
+    r.close()
