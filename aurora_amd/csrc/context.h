@@ -112,8 +112,8 @@ struct KernelTimer {
 
 // Launch shape of the tracer kernels (set by aur_set_launch / the autotuner).
 struct LaunchCfg {
-    int trace_block = 128;          // traversal workgroup size: 64, 128 or 256
-    int trace_waves_per_cu = 16;    // persistent traversal grid, waves per CU
+    int trace_block = 64;           // traversal workgroup size: 64, 128 or 256
+    int trace_waves_per_cu = 32;    // persistent traversal grid, waves per CU (the traversal kernels fit 32)
     // path slots in flight: the batched launch's pool, and the largest chunk
     // of whole passes a single-scene render traces at once (368 bytes of
     // wavefront state per slot: 24 GB at the default 64 M).  Every chunk ends

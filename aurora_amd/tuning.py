@@ -26,8 +26,8 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 64 << 20}
-BATCH_DEFAULT = {"bvh_width": 4, "max_leaf": 4, "trace_block": 128, "trace_waves_per_cu": 16, "pool_paths": 8 << 20}
+DEFAULT = {"bvh_width": 4, "max_leaf": 2, "trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 64 << 20}
+BATCH_DEFAULT = {"bvh_width": 4, "max_leaf": 2, "trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 32 << 20}
 WIDTHS = (4, 8)
 LEAVES = (1, 2, 4, 8)
 BLOCKS = (64, 128, 256)
