@@ -581,7 +581,7 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     }
     // buffers
     const size_t pool = std::max<size_t>(64, std::min<size_t>(c->launch.pool_paths, (size_t)std::max<long long>(total, 64)));
-    if (pool > (size_t)0x3fffffff) throw CapacityError("path pool too large");
+    if (pool > kMaxPoolPaths) throw CapacityError("path pool too large");
     // iterations between polls, and the lag after which issued work has finished
     const int lag = max_path + 1;
     const int pairs_per_poll = std::max(1, std::min((iters_per_poll > 0 ? iters_per_poll : 8), BATCH_HIST - lag - 4) / 2);

@@ -968,6 +968,8 @@ int aur_set_launch(aur_ctx *ctx, const aur_launch_opts *opts) {
         }
         if (opts->trace_waves_per_cu > 64) throw std::invalid_argument("trace_waves_per_cu must be 1..64");
         if (opts->pool_paths > 0 && opts->pool_paths < 64) throw std::invalid_argument("pool_paths must be >= 64");
+        if (opts->pool_paths > 0 && (size_t)opts->pool_paths > kMaxPoolPaths)
+            throw std::invalid_argument("pool_paths must be <= " + std::to_string(kMaxPoolPaths));
         if (opts->trace_block > 0) l.trace_block = opts->trace_block;
         if (opts->trace_waves_per_cu > 0) l.trace_waves_per_cu = opts->trace_waves_per_cu;
         if (opts->pool_paths > 0) l.pool_paths = (size_t)opts->pool_paths;

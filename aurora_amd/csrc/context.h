@@ -110,16 +110,21 @@ struct KernelTimer {
     long long launches[KT_COUNT] = {0};
 };
 
+// Largest path pool: queue entries are indexed with 32-bit ints, and the
+// shadow queue (two rays per path) stores three float4 records per ray
+constexpr size_t kMaxPoolPaths = 0x7fffffff / 6;
+
 // Launch shape of the tracer kernels (set by aur_set_launch / the autotuner).
 struct LaunchCfg {
     int trace_block = 64;           // traversal workgroup size: 64, 128 or 256
     int trace_waves_per_cu = 32;    // persistent traversal grid, waves per CU (the traversal kernels fit 32)
     // path slots in flight: the batched launch's pool, and the largest chunk
     // of whole passes a single-scene render traces at once (368 bytes of
-    // wavefront state per slot: 24 GB at the default 64 M).  Every chunk ends
-    // in a latency-bound drain (the last bounces of its longest paths), so
-    // larger chunks amortise it over more paths
-    size_t pool_paths = 64u << 20;
+    // wavefront state per slot, allocated for the chunks a render needs: up
+    // to 47 GB at the default 128 M, which holds a 64-spp 1080p frame).
+    // Every chunk ends in a latency-bound drain (the last bounces of its
+    // longest paths), so larger chunks amortise it over more paths
+    size_t pool_paths = 128u << 20;
 };
 
 // Device state of a batched launch (batch.hip), kept between calls.

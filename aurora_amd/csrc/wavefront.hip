@@ -482,7 +482,7 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     const size_t pool = c->launch.pool_paths;
     size_t cap = src.mode == 0 ? std::max<size_t>(wh * ls, std::min<size_t>(n_paths, pool))
                                : std::min<size_t>(n_paths, pool);
-    if (cap > (size_t)0x7fffffff) throw CapacityError("batch too large");
+    if (cap > kMaxPoolPaths) throw CapacityError("one pass needs more path slots than the queues can index");
     ensure_wavefront(c, std::max<size_t>(cap, 1));
     SceneView sv = scene_view(c);
     hipEvent_t e0, e1;

@@ -26,14 +26,14 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-DEFAULT = {"bvh_width": 4, "max_leaf": 2, "trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 64 << 20}
+DEFAULT = {"bvh_width": 4, "max_leaf": 2, "trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 128 << 20}
 BATCH_DEFAULT = {"bvh_width": 4, "max_leaf": 2, "trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 32 << 20}
 WIDTHS = (4, 8)
 LEAVES = (1, 2, 4, 8)
 BLOCKS = (64, 128, 256)
 WAVES = (8, 16, 24, 32)
 POOLS = (2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20)
-SINGLE_POOLS = (16 << 20, 32 << 20, 64 << 20)
+SINGLE_POOLS = (32 << 20, 64 << 20, 128 << 20)
 
 
 def _bucket(n_tris: int) -> str:

@@ -5,8 +5,10 @@ Workload: the 1920x1080, 64 spp, max-depth-8 scene of eight displaced, Loop-
 subdivided (7 levels) icospheres (2,621,450 triangles with the light quads),
 Kelemen/GGX materials, four quad lights and a constant environment.  A step is
 one batch of ``--passes-per-step`` full-frame passes (each pass = one path per
-pixel), traced as one chunk of 66 M paths; the default 2 steps x 32 passes
-render the full 64-spp frame.
+pixel); by default a step is the whole 64-spp frame (64 passes, 132.7 M
+paths, traced as one chunk), rendered twice after one warm-up frame.  Step k
+renders passes (k * P) mod 64 onward, so with fewer passes per step the steps
+walk through the frame's passes.
 
 Prints one JSON line.  ``value`` is Mpaths/s of the timed steps with the scene
 and BVH resident in HBM.  ``roofline`` describes the kernel with the largest
@@ -535,7 +537,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--passes-per-step", type=int, default=32)
+    ap.add_argument("--passes-per-step", type=int, default=64)
     ap.add_argument("--bvh-width", type=int, default=0, help="0: autotuned choice")
     ap.add_argument("--max-leaf", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -579,14 +581,19 @@ def main(argv=None):
     wh = info["width"] * info["height"]
     P = args.passes_per_step
 
+    spp = max(info["pixelsamples"], P)
+
+    def first(k):
+        return (k * P) % spp
+
     for k in range(args.warmup):
-        r.render_passes(k * P, P, fetch=False)
+        r.render_passes(first(k), P, fetch=False)
     r.set_profiling(True)
     _hip_sync()
     t0 = time.perf_counter()
     tot = {"paths": 0, "ext_rays": 0, "hit_rays": 0, "shadow_rays": 0}
     for k in range(args.steps):
-        _, st = r.render_passes(k * P, P, fetch=False)
+        _, st = r.render_passes(first(k), P, fetch=False)
         for key in tot:
             tot[key] += st[key]
     _hip_sync()

@@ -310,8 +310,8 @@ int aur_batch_submit(aur_ctx *exec, const aur_job *jobs, int n_jobs, const aur_b
 typedef struct {
     int trace_block;        /* traversal workgroup size: 64, 128 or 256 */
     int trace_waves_per_cu; /* persistent traversal grid, in waves per CU (1..64) */
-    int pool_paths;         /* path slots in flight (>= 64): the aur_batch_submit pool, and the largest
-                               chunk of whole passes aur_render traces at once */
+    int pool_paths;         /* path slots in flight (64 .. 357913941): the aur_batch_submit pool, and
+                               the largest chunk of whole passes aur_render traces at once */
 } aur_launch_opts;
 /* Fields <= 0 keep their current value. */
 int aur_set_launch(aur_ctx *ctx, const aur_launch_opts *opts);

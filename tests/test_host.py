@@ -154,12 +154,15 @@ def test_json_comments_and_sorted_members(tmp_path):
 def test_launch_options_validation():
     r = host_renderer()
     d = r.get_launch()
-    assert d == {"trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 64 << 20}
+    assert d == {"trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 128 << 20}
     r.set_launch(trace_block=256, pool_paths=1 << 16)
     assert r.get_launch() == {"trace_block": 256, "trace_waves_per_cu": 32, "pool_paths": 1 << 16}
     _expect_error(lambda: r.set_launch(trace_block=100), -1)
     _expect_error(lambda: r.set_launch(trace_waves_per_cu=65), -1)
     _expect_error(lambda: r.set_launch(pool_paths=10), -1)
+    # shadow records are indexed 3 * (2 * pool) in 32-bit ints
+    _expect_error(lambda: r.set_launch(pool_paths=0x7FFFFFFF // 6 + 1), -1)
+    r.set_launch(pool_paths=0x7FFFFFFF // 6)
     assert r.get_launch()["trace_block"] == 256
     r.close()
 
