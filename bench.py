@@ -321,7 +321,7 @@ def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd
     the texture gradient over adj_passes passes, and central differences
     (common random numbers, h = 1e-3) on n_fd seeded texels at fd_passes
     passes."""
-    from aurora_amd import scenes
+    from aurora_amd import scenes, tuning
     from aurora_amd.renderer import Renderer
     tmp = tempfile.mkdtemp(prefix="aurora_large_")
     t0 = time.time()
@@ -337,6 +337,12 @@ def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd
     build_s = time.time() - t0 - load_s
     info = r.info()
     _progress("large scene: %d triangles loaded in %.1f s, built in %.1f s" % (info["n_tris"], load_s, build_s))
+    # the tuned BVH structure and launch shape for this triangle count (Loop
+    # subdivision happens in build, so the table is read after it)
+    cfg = tuning.choose(r)
+    if (cfg["bvh_width"], cfg["max_leaf"]) != (4, 1):
+        r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
+    tuning.apply_launch(r, cfg)
     attrs = [spec.attr_index(k) for k in sorted(spec.objects) if k.startswith("terrain")]
     W, H = info["width"], info["height"]
     # the inverse problem's state: texture at 0.5, loss against the true texture's image
@@ -388,6 +394,7 @@ def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd
     return {"workload": "large_scene_3840x2160_depth16", "triangles": int(info["n_tris"]),
             "texture": [int(truth.shape[1]), int(truth.shape[0])], "scene_write_s": round(gen_s, 1),
             "load_s": round(load_s, 2), "bvh_build_s": round(build_s, 2),
+            "launch": {k: cfg[k] for k in ("bvh_width", "max_leaf", "trace_block", "trace_waves_per_cu", "pool_paths")},
             "forward_passes": fwd_passes, "forward_Mpaths_s": round(fst["paths"] / fwd_s / 1e6, 3),
             "forward_Mrays_s": round((fst["ext_rays"] + fst["shadow_rays"]) / fwd_s / 1e6, 3),
             "adjoint_passes": adj_passes, "adjoint_Mpaths_s": round(ast["paths"] / adj_s / 1e6, 3),
