@@ -245,7 +245,10 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
     }
     int k = 0;
     f3 Lc = L;
-    while (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY) { Lc = mul3(Lc, 0.2f); ++k; }
+    while (k < AUR_FIREFLY_MAX_STEPS && (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY)) {
+        Lc = mul3(Lc, 0.2f);
+        ++k;
+    }
     *L_out = L;
     *clamp_steps = k;
 }

@@ -77,6 +77,30 @@ static bool needs_gpu_flatten(const HostScene &s) {
     return false;
 }
 
+// Radiance sources must be finite.  The reference takes color * 2^exposure
+// in float for square lights (squareLight.h:84-86) and in double for the
+// environment (infiniteAreaLight.cpp:61-78), and stores textures as FP16
+// (Imf::Rgba); an exposure of 128 or a texel above 65504 turns into +inf
+// there, and an infinite radiance makes the firefly clamp of
+// integrator.cpp:277-281 loop forever.  Such scenes are refused here.
+static void check_radiance(const HostScene &s) {
+    for (const aur_light &l : s.lights) {
+        if (!std::isfinite(l.exposure)) throw std::invalid_argument("light exposure is not finite");
+        for (int k = 0; k < 3; ++k) {
+            const float le = l.type == AUR_LIGHT_AREA ? l.color[k] * powf(2, l.exposure)
+                                                      : (float)(l.color[k] * std::pow(2.0, (double)l.exposure));
+            if (!std::isfinite(le)) throw std::invalid_argument("light radiance color * 2^exposure is not finite");
+        }
+    }
+    for (const aur_material &m : s.attr_mats)
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(m.emission[k])) throw std::invalid_argument("material emission is not finite");
+    for (const HostTexture &t : s.textures)
+        for (float v : t.rgba)
+            if (!std::isfinite(half_to_float(float_to_half(v))))
+                throw std::invalid_argument("texture holds a texel that is not finite as FP16");
+}
+
 // ---------------------------------------------------------------------------
 // device scene upload
 // ---------------------------------------------------------------------------
@@ -526,6 +550,7 @@ int aur_scene_load(aur_ctx *ctx, const char *scene_path, const char *config_json
             if (f) { std::fclose(f); cfg = cand; }
         }
         load_scene(scene_path, cfg, &s);
+        check_radiance(s);
         ctx->c.scene = std::move(s);
         ctx->c.have_scene = true;
         ctx->c.built = false;
@@ -598,6 +623,7 @@ int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
             if (m.type < AUR_MAT_MATTE || m.type > AUR_MAT_GLASS) throw std::invalid_argument("unknown material type");
         for (const aur_light &l : s.lights)
             if (l.type != AUR_LIGHT_AREA && l.type != AUR_LIGHT_ENV) throw std::invalid_argument("unknown light type");
+        check_radiance(s);
         s.flat = true;
         ctx->c.scene = std::move(s);
         ctx->c.have_scene = true;
@@ -677,7 +703,7 @@ int aur_build_accel(aur_ctx *ctx, const aur_accel_opts *opts) {
         if (!c->have_scene) throw StateError("no scene loaded");
         set_device(c);
         int width = opts && opts->width ? opts->width : 4;
-        int leaf = opts && opts->max_leaf ? opts->max_leaf : 4;
+        int leaf = opts && opts->max_leaf ? opts->max_leaf : 2;
         if (width != 4 && width != 8) throw std::invalid_argument("BVH width must be 4 or 8");
         if (leaf < 1 || leaf > 16) throw std::invalid_argument("max_leaf must be in 1..16");
         if (!c->scene.flat) flatten(c);
@@ -784,6 +810,7 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
         if (first_pass == 0 || resized) {
             fill_zero(c, c->accum.p, sizeof(float4) * wh);
             fill_zero(c, c->display.p, sizeof(ushort4) * wh);
+            c->display_passes = 0;
         }
         c->fb_w = w;
         c->fb_h = h;
@@ -797,8 +824,10 @@ int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host,
             src.slots = nullptr;
             src.ls = c->scene.opt.lightsamples;
             src.display = c->display.as<ushort4>();
+            src.display_base = c->display_passes;
             if (src.ls < 1) throw std::invalid_argument("lightsamples must be at least 1");
             run_paths(c, src, pass_paths(wh, first_pass, n_passes, src.ls), c->accum.as<float4>(), &st);
+            c->display_passes += n_passes;
         }
         if (rgba_sum_host) {
             HIP_CHECK(hipMemcpyAsync(rgba_sum_host, c->accum.p, sizeof(float4) * wh, hipMemcpyDeviceToHost, c->stream));
@@ -1141,11 +1170,15 @@ int aur_kernel_stats(aur_ctx *ctx, aur_kernel_stat *out, int max, int *n_out) {
     });
 }
 
-int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int any_hit) {
+int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int mode) {
     return guarded([&] {
         NEED(ctx);
         Ctx *c = &ctx->c;
         if (n < 0) throw std::invalid_argument("negative ray count");
+        if (mode != AUR_TRACE_CLOSEST && mode != AUR_TRACE_ANY && mode != AUR_TRACE_EXTEND)
+            throw std::invalid_argument("unknown trace mode");
+        if (mode == AUR_TRACE_EXTEND && (size_t)n > kMaxPoolPaths)
+            throw CapacityError("more rays than the extension queue can index");
         if (n == 0) return;
         NEED(rays); NEED(hits);
         if (!c->built) throw StateError("trace before aur_build_accel");
@@ -1154,7 +1187,7 @@ int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int any_h
         dr.reserve(sizeof(aur_ray) * n);
         dh.reserve(sizeof(aur_hit) * n);
         HIP_CHECK(hipMemcpyAsync(dr.p, rays, sizeof(aur_ray) * n, hipMemcpyHostToDevice, c->stream));
-        trace_rays(c, dr.as<aur_ray>(), n, dh.as<aur_hit>(), any_hit);
+        trace_rays(c, dr.as<aur_ray>(), n, dh.as<aur_hit>(), mode);
         HIP_CHECK(hipMemcpyAsync(hits, dh.p, sizeof(aur_hit) * n, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
     });

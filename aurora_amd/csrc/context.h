@@ -182,6 +182,9 @@ struct Ctx {
     // framebuffer (sum of per-pass rgba)
     DevBuf accum;
     DevBuf display;  // ushort4 per pixel: OpenexrDisplay's FP16 running mean (display_update)
+    // passes appended to `display` since it was last cleared: the running
+    // mean's sample count (openexrDisplay.cpp:61-72 counts appended values)
+    long long display_passes = 0;
     int fb_w = 0, fb_h = 0;
 
     Wavefront wf;
@@ -213,8 +216,10 @@ struct PathSource {
     // tiled position
     float4 *adj_rec = nullptr;
     long long rec_base = 0;
-    // mode 0: the FP16 running mean updated with every resolved pass, or null
+    // mode 0: the FP16 running mean updated with every resolved pass, or null;
+    // display_base passes were appended to it before this run
     ushort4 *display = nullptr;
+    long long display_base = 0;
 };
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats);
 // Paths of passes [first_pass, first_pass + n_passes) of a w*h image: one per
@@ -227,7 +232,8 @@ inline size_t pass_paths(size_t wh, int first_pass, int n_passes, int ls) {
     return n + wh * (size_t)ls * (size_t)n_passes;
 }
 void resolve_passes(Ctx *c, const float4 *out_rgba, int n_passes, float4 *accum);
-void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_hit);
+// mode: AUR_TRACE_CLOSEST, AUR_TRACE_ANY or AUR_TRACE_EXTEND
+void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode);
 // sizes the path pool and the traversal stack overflow for BVHs of the given width and depth
 void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth);
 // batch.hip: jobs[k] renders passes [first_pass[k], first_pass[k] + n_passes[k]) into its framebuffer

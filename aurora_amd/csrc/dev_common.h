@@ -23,6 +23,11 @@ namespace aur {
 // ---------------------------------------------------------------------------
 #define AUR_CAUSTIC_BLUR 0.2f
 #define AUR_FIREFLY 200.f
+// The clamp loop of integrator.cpp:277-281 takes at most this many 0.2 steps:
+// any finite float drops below 200 within 53 of them, so finite radiance is
+// clamped exactly as the unbounded loop would, while an infinite channel (an
+// overflowing light or texel) ends the loop instead of spinning forever.
+#define AUR_FIREFLY_MAX_STEPS 64
 #define AUR_ISBLACK 0.0001f
 #define AUR_RAY_BIAS 0.0005f
 #define AUR_PI_F 3.14159265358979323846f

@@ -192,7 +192,10 @@ __global__ void k_finish(SceneView sv, WfView wv) {
             f3 Lc = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
             int k = 0;
             if (!(m.x & F_NOCLAMP))
-                while (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY) { Lc = mul3(Lc, 0.2f); ++k; }
+                while (k < AUR_FIREFLY_MAX_STEPS && (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY)) {
+                    Lc = mul3(Lc, 0.2f);
+                    ++k;
+                }
             const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
             // bits 0-15 clamp steps, bit 16 camera miss, bits 17-24 rays traced
             wv.src.adj_rec[wv.src.rec_base + m.y] = make_float4(
@@ -262,7 +265,7 @@ __device__ inline float4 pass_value(const float4 *__restrict__ out, size_t off, 
 }
 
 __global__ void k_resolve(const float4 *__restrict__ out, int wh, int first_pass, int n_passes, int ls,
-                          float4 *__restrict__ accum, ushort4 *__restrict__ display) {
+                          float4 *__restrict__ accum, ushort4 *__restrict__ display, int display_base) {
     int stride = gridDim.x * blockDim.x;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < wh; p += stride) {
         float4 s = accum[p];
@@ -272,8 +275,8 @@ __global__ void k_resolve(const float4 *__restrict__ out, int wh, int first_pass
             const int n_sub = (first_pass + k == 0) ? 1 : ls;
             const float4 v = pass_value(out, off, wh, n_sub, p);
             s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
-            // the display counts passes from pass 0 (the pixel's n-th sample)
-            if (display) d = display_update(d, v, first_pass + k + 1);
+            // the display's n-th appended pass since it was cleared has weight 1/n
+            if (display) d = display_update(d, v, display_base + k + 1);
             off += (size_t)n_sub * wh;
         }
         accum[p] = s;
@@ -526,7 +529,8 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
         } else if (src.mode == 0) {
             int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));
             TIMED(c, KT_RESOLVE, hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh,
-                                                    s.first_pass, passes, ls, out_rgba, src.display));
+                                                    s.first_pass, passes, ls, out_rgba, src.display,
+                                                    (int)(src.display_base + (s.first_pass - src.first_pass))));
         } else {
             HIP_CHECK(hipMemcpyAsync(out_rgba + done, wv.out, sizeof(float4) * n, hipMemcpyDeviceToDevice, c->stream));
         }
@@ -574,8 +578,53 @@ __global__ void __launch_bounds__(B) k_trace(SceneView sv, const aur_ray *__rest
     }
 }
 
-void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int any_hit) {
+// AUR_TRACE_EXTEND: the rays go through the renderer's own closest-hit
+// kernel.  They are written to extension queue 0 in its layout, k_extend runs
+// with the launch shape of a render (persistent grid, per-lane refill), and
+// its hit records are read back.
+__global__ void k_rays_to_queue(const aur_ray *__restrict__ rays, int n, WfView wv) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const aur_ray r = rays[i];
+        wv.ray[0][2 * i] = make_float4(r.o[0], r.o[1], r.o[2], r.tmin);
+        wv.ray[0][2 * i + 1] = make_float4(r.d[0], r.d[1], r.d[2], r.tmax);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) wv.counts[C_Q0] = n;
+}
+
+__global__ void k_queue_hits(WfView wv, int n, aur_hit *__restrict__ hits) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 h = wv.hit[i];
+        aur_hit o;
+        o.t = h.x; o.u = h.y; o.v = h.z; o.prim = __float_as_int(h.w);
+        hits[i] = o;
+    }
+}
+
+void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode) {
     if (!c->built) throw StateError("trace before aur_build_accel");
+    if (mode == AUR_TRACE_EXTEND) {
+        ensure_wavefront(c, std::max(n, 1));
+        SceneView sv = scene_view(c);
+        PathSource src;
+        src.mode = 1;
+        src.first_pass = 0;
+        src.pixel = src.sample = nullptr;
+        src.slots = nullptr;
+        WfView wv = wf_view(c, src);
+        const int blocks = std::max(1, std::min((n + 255) / 256, persistent_blocks(c, 256)));
+        hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, wv.counts, wv.stats, 1);
+        hipLaunchKernelGGL(k_rays_to_queue, dim3(blocks), dim3(256), 0, c->stream, d_rays, n, wv);
+        const int tb = trace_blocks(c);
+#define AUR_EXT(w_, b_) hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, 0)
+        AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT);
+#undef AUR_EXT
+        hipLaunchKernelGGL(k_queue_hits, dim3(blocks), dim3(256), 0, c->stream, wv, n, d_hits);
+        // the extend kernel's per-workgroup hit partials start the next render at zero
+        fill_zero(c, c->wf.partials.p, sizeof(unsigned long long) * trace_blocks(c));
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const int any_hit = mode == AUR_TRACE_ANY ? 1 : 0;
     ensure_wavefront(c, 1);
     SceneView sv = scene_view(c);
     const int B = c->launch.trace_block;

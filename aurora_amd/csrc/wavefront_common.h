@@ -31,15 +31,13 @@ size_t trace_threads(const Ctx *c);
 
 enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2, RT_MIRROR = 3 };
 // Path flags: bits 0-1 ray type, 2 matte path, 3 last ray missed, 8-15
-// vertex index, 16-23 bounces, 24 no firefly clamp, 25-30 shadow rays traced.
+// vertex index, 16-23 bounces, 24 no firefly clamp.  Shadow rays are counted
+// where they are traced (k_shadow, k_bshadow), not per path.
 enum { F_MATTE = 1 << 2, F_MISS = 1 << 3, F_NOCLAMP = 1 << 24 };
-#define F_SHADOW_SHIFT 25
-#define F_SHADOW_MASK (63 << F_SHADOW_SHIFT)
 // rays one finished path traced, from its flags: extension rays (camera ray
-// included), those that hit, and shadow rays
+// included) and those that hit
 __device__ inline int path_ext_rays(int flags) { return ((flags >> 8) & 0xff) + 1; }
 __device__ inline int path_hit_rays(int flags) { return path_ext_rays(flags) - ((flags & F_MISS) ? 1 : 0); }
-__device__ inline int path_shadow_rays(int flags) { return (flags >> F_SHADOW_SHIFT) & 63; }
 enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_FINISH = 3 };
 enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
 
@@ -139,7 +137,8 @@ __device__ inline f3 camera_dir(const SceneView &sv, int pixel, float r0, float 
 // the environment directly.
 __device__ inline f3 firefly_clamp(f3 L, int flags) {
     if (!(flags & F_NOCLAMP))
-        while (L.x > AUR_FIREFLY || L.y > AUR_FIREFLY || L.z > AUR_FIREFLY) L = mul3(L, 0.2f);
+        for (int k = 0; k < AUR_FIREFLY_MAX_STEPS && (L.x > AUR_FIREFLY || L.y > AUR_FIREFLY || L.z > AUR_FIREFLY); ++k)
+            L = mul3(L, 0.2f);
     return L;
 }
 
@@ -176,7 +175,6 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
     const float4 Lb = wv.Lb[q][i];
     float4 T4 = wv.T[q][i];
     int flags = meta.x;
-    const int shadow_before = (flags >> F_SHADOW_SHIFT) & 63;
     const int rtype = flags & 3;
     const int tb = (flags >> 8) & 0xff;
     bool cont = false, finish = false;
@@ -336,11 +334,6 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
         }
     }
     if (prim < 0) flags |= F_MISS;
-    {
-        const int n_sh = (sh[0].valid ? 1 : 0) + (sh[1].valid ? 1 : 0);
-        const int traced = (shadow_before + n_sh) & 63;
-        flags = (flags & ~F_SHADOW_MASK) | (traced << F_SHADOW_SHIFT);
-    }
     out.cont = cont;
     out.finish = finish;
     out.sh[0] = sh[0];

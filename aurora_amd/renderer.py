@@ -92,7 +92,7 @@ class Renderer:
         L.check(self.lib.aur_scene_upload(self._ctx, C.byref(d)))
         self.passes_done = 0
 
-    def build(self, width: int = 4, max_leaf: int = 4) -> None:
+    def build(self, width: int = 4, max_leaf: int = 2) -> None:
         """Loop subdivision, flattening and the wide BVH, on the GPU."""
         o = L.AccelOpts(width, max_leaf)
         L.check(self.lib.aur_build_accel(self._ctx, C.byref(o)))
@@ -268,14 +268,18 @@ class Renderer:
                 for k in range(n.value)}
 
     # -- parity queries ------------------------------------------------------
-    def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+    def trace(self, rays: np.ndarray, any_hit: bool = False, extend_kernel: bool = False) -> np.ndarray:
         """rays: (n, 8) float32 rows (ox, oy, oz, tmin, dx, dy, dz, tmax).
-        Returns a structured array with t, u, v, prim."""
+        Returns a structured array with t, u, v, prim.  extend_kernel traces
+        closest hits through the renderer's extension kernel itself."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         n = rays.shape[0]
         hits = np.zeros(n, dtype=[("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim", np.int32)])
+        if extend_kernel and any_hit:
+            raise ValueError("the extension kernel traces closest hits")
+        mode = 2 if extend_kernel else (1 if any_hit else 0)
         L.check(self.lib.aur_trace(self._ctx, rays.ctypes.data_as(C.POINTER(L.Ray)), n,
-                                   hits.ctypes.data_as(C.POINTER(L.Hit)), int(any_hit)))
+                                   hits.ctypes.data_as(C.POINTER(L.Hit)), mode))
         return hits
 
     def bsdf(self, queries) -> list:

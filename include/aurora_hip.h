@@ -166,7 +166,7 @@ typedef struct {
 
 typedef struct {
     int width;      /* BVH branching factor: 4 or 8 (0 = default 4) */
-    int max_leaf;   /* triangles per leaf, 1..16 (0 = default 4) */
+    int max_leaf;   /* triangles per leaf, 1..16 (0 = default 2) */
 } aur_accel_opts;
 
 /* Counters, counted as the reference counts rays (integrator.cpp:44,71,88,162,260). */
@@ -359,7 +359,17 @@ int aur_set_profiling(aur_ctx *ctx, int enable);
 int aur_kernel_stats(aur_ctx *ctx, aur_kernel_stat *out, int max, int *n_out);
 
 /* ---- parity queries ---- */
-int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int any_hit);
+/* Batch ray queries against the built BVH: AccelerationStructure::intersect
+ * (closest hit) and intersectBinary (any hit), accelerationStructure.h:17-27,
+ * embreeAccelerator.cpp:41-100.  AUR_TRACE_CLOSEST and AUR_TRACE_ANY run one
+ * ray per lane to completion; AUR_TRACE_EXTEND sends the rays through the
+ * renderer's own closest-hit kernel (the extension queue, the persistent
+ * launch shape and the per-lane ray refill of a render), so its hits are
+ * those a render would see. */
+#define AUR_TRACE_CLOSEST 0
+#define AUR_TRACE_ANY 1
+#define AUR_TRACE_EXTEND 2
+int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int mode);
 int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r);
 /* Kelemen albedo table (64 cos x 32 roughness), kelemenMaterial.cpp:113-178. */
 int aur_kelemen_table(aur_ctx *ctx, float reflectance, int distribution, int n_samples,

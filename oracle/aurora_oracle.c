@@ -29,6 +29,9 @@
 /* constants.h:13-23 */
 #define CAUSTIC_BLUR 0.2
 #define FIREFLY 200
+/* at most this many 0.2 steps of the clamp: every finite float is below 200
+ * after 53, an infinite channel ends the loop (the reference would spin) */
+#define FIREFLY_MAX_STEPS 64
 #define ISBLACK 0.0001
 #define RAY_BIAS 0.0005
 
@@ -1105,7 +1108,7 @@ static C3 path_from_hit(const orc_scene *s, const orc_bvh *bv, orc_ray ray, orc_
         cnt->hit++;
     }
     /* integrator.cpp:277-281 */
-    while (Lo.r > FIREFLY || Lo.g > FIREFLY || Lo.b > FIREFLY) Lo = cmulf(Lo, 0.2f);
+    for (int k = 0; k < FIREFLY_MAX_STEPS && (Lo.r > FIREFLY || Lo.g > FIREFLY || Lo.b > FIREFLY); ++k) Lo = cmulf(Lo, 0.2f);
     return Lo;
 }
 
@@ -1401,7 +1404,9 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
     Lout[0] = L[0]; Lout[1] = L[1]; Lout[2] = L[2];
     int k = 0;
     float lc[3] = {(float)L[0], (float)L[1], (float)L[2]};
-    while (lc[0] > FIREFLY || lc[1] > FIREFLY || lc[2] > FIREFLY) { lc[0] *= 0.2f; lc[1] *= 0.2f; lc[2] *= 0.2f; ++k; }
+    while (k < FIREFLY_MAX_STEPS && (lc[0] > FIREFLY || lc[1] > FIREFLY || lc[2] > FIREFLY)) {
+        lc[0] *= 0.2f; lc[1] *= 0.2f; lc[2] *= 0.2f; ++k;
+    }
     return k;
 }
 

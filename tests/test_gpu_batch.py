@@ -119,3 +119,47 @@ def test_batch_rejects_mixed_bvh_widths(tmp_path):
     assert ei.value.code == -1
     a.close()
     b.close()
+
+
+def test_batch_counts_shadow_rays_of_deep_paths(tmp_path, oracle):
+    """A closed room (walls extended behind the camera), depth 100, of matte
+    albedo e/2: each vertex multiplies the throughput by 2 albedo cos with cos
+    uniform, whose log has mean 0, so a path ends mostly on the light and many
+    trace more than 63 shadow rays (counted path by path with the oracle).
+    The batched launch counts shadow rays per job where they are traced, so
+    its counts still equal a single-scene render's."""
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer, batch_submit
+    spec = scenes.box_room(res=32, spp=2, maxdepth=100, mindepth=100)
+    x0, x1, y0, y1, z0, z1 = -1.0, 1.0, 0.0, 2.0, -1.0, 3.6
+    fv, ff = [], []
+    for q in (([x0, y0, z1], [x1, y0, z1], [x1, y0, z0], [x0, y0, z0]),
+              ([x0, y1, z0], [x1, y1, z0], [x1, y1, z1], [x0, y1, z1]),
+              ([x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0]),
+              ([x1, y0, z1], [x0, y0, z1], [x0, y1, z1], [x1, y1, z1])):
+        v, f = scenes.quad(*q)
+        ff += [[a + len(fv) for a in t] for t in f]
+        fv += v
+    spec.meshes["walls_white.obj"] = (fv, ff, None)
+    spec.meshes["wall_left.obj"] = (*scenes.quad([x0, y0, z1], [x0, y0, z0], [x0, y1, z0], [x0, y1, z1]), None)
+    spec.meshes["wall_right.obj"] = (*scenes.quad([x1, y0, z0], [x1, y0, z1], [x1, y1, z1], [x1, y1, z0]), None)
+    for name in list(spec.materials):
+        spec.materials[name] = {"type": "matteMaterial", "color": [1.36, 1.36, 1.36]}
+    path = spec.write(str(tmp_path / "deep"))
+    r = Renderer(path, os.path.join(os.path.dirname(path), "config.json"))
+    r.build()
+    r.set_seed(5)
+    acc, st = r.render_passes(0, 2)
+    o = oracle.Scene(r.export())
+    deep = 0
+    for p in range(0, 32 * 32, 4):
+        _, c = o.render_paths(5, np.array([p], np.int32), np.array([1], np.int32))
+        deep += c["shadow_rays"] > 63
+    assert deep >= 3, deep
+    ex = Renderer()
+    res, tot = batch_submit(ex, [(r, 0, 2)])
+    for key in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
+        assert res[0][key] == st[key], (key, res[0][key], st[key])
+    assert np.array_equal(r.framebuffer(), acc)
+    ex.close()
+    r.close()

@@ -83,3 +83,39 @@ def test_display_is_the_reference_running_mean(room, tmp_path):
     assert np.array_equal(np.asarray(img, np.float32).astype(np.float16).view(np.uint16), want.view(np.uint16))
     ex.close()
     r.close()
+
+
+@pytest.mark.gpu
+def test_display_counts_passes_since_cleared(room):
+    """A fresh framebuffer that starts at pass 3 holds the running mean of
+    passes 3, 4, 5 with weights 1, 1/2, 1/3 (appendValue counts the values
+    appended since the display was cleared), in one call, in two calls, and
+    through the batched launch."""
+    from aurora_amd.renderer import Renderer, batch_submit
+    values = []
+    for k in range(3, 6):
+        rk = Renderer(*room)
+        rk.build()
+        rk.set_seed(17)
+        v, _ = rk.render_passes(k, 1)
+        values.append(v)
+        rk.close()
+    want = _running_mean(values)
+    r = Renderer(*room)
+    r.build()
+    r.set_seed(17)
+    r.render_passes(3, 3)
+    assert np.array_equal(r.display().view(np.uint16), want.view(np.uint16))
+    r2 = Renderer(*room)
+    r2.build()
+    r2.set_seed(17)
+    r2.render_passes(3, 1)
+    r2.render_passes(4, 2)
+    assert np.array_equal(r2.display().view(np.uint16), want.view(np.uint16))
+    r3 = Renderer(*room)
+    r3.build()
+    r3.set_seed(17)
+    batch_submit(r, [(r3, 3, 3)])
+    assert np.array_equal(r3.display().view(np.uint16), want.view(np.uint16))
+    for x in (r, r2, r3):
+        x.close()

@@ -34,12 +34,14 @@ def test_trace_bit_exact_box_room(gpu_box_room, oracle):
     rays = np.concatenate([inside, camera]).astype(np.float32)
     for width in (4, 8):
         gpu_box_room.build(width=width, max_leaf=2)
-        g, o = gpu_box_room.trace(rays), s.trace(rays)
-        assert np.array_equal(g["prim"], o["prim"])
+        o = s.trace(rays)
         hit = o["prim"] >= 0
         assert hit.mean() > 0.8
-        for k in ("t", "u", "v"):
-            assert np.array_equal(g[k][hit], o[k][hit]), k
+        # one ray per lane, and the renderer's own extension kernel
+        for g in (gpu_box_room.trace(rays), gpu_box_room.trace(rays, extend_kernel=True)):
+            assert np.array_equal(g["prim"], o["prim"])
+            for k in ("t", "u", "v"):
+                assert np.array_equal(g[k][hit], o[k][hit]), k
         ga, oa = gpu_box_room.trace(rays, any_hit=True), s.trace(rays, any_hit=True)
         assert np.array_equal(ga["prim"] >= 0, oa["prim"] >= 0)
     gpu_box_room.build()
@@ -180,6 +182,58 @@ def test_injected_sample_paths(gpu_box_room, oracle):
     assert np.median(rel) < 1e-6
     for k in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
         assert abs(gs[k] - os_[k]) <= max(2, 1e-3 * os_[k]), k
+
+
+@pytest.fixture(scope="module")
+def gpu_roulette_room(tmp_path_factory):
+    """The box room with roulette on: mindepth 1 < maxdepth 5 (the reference's
+    default MinDepth is 1, options.h:54)."""
+    from aurora_amd import scenes
+    from aurora_amd.renderer import Renderer
+    d = str(tmp_path_factory.mktemp("rr_room"))
+    scenes.write_config("box_room", d, res=96, spp=8, maxdepth=5, mindepth=1)
+    r = Renderer(os.path.join(d, "scene.asc"), os.path.join(d, "config.json"))
+    r.build()
+    yield r
+    r.close()
+
+
+def test_roulette_injected_paths(gpu_roulette_room, oracle):
+    """Roulette runs (integrator.cpp:219-248): per-path radiance and ray counts
+    equal the oracle's under the same injected uniforms, and paths are killed
+    (fewer extension rays than the same paths without roulette)."""
+    e = gpu_roulette_room.export()
+    assert (e["mindepth"], e["maxdepth"]) == (1, 5)
+    s = oracle.Scene(e)
+    rng = np.random.default_rng(31)
+    n = 20000
+    slots = rng.uniform(0, 1, size=(n, 2 + 9 * 5)).astype(np.float32)
+    pix = rng.integers(0, e["width"] * e["height"], n).astype(np.int32)
+    smp = rng.integers(0, 64, n).astype(np.int32)
+    g, gs = gpu_roulette_room.render_paths(pix, smp, slots=slots)
+    o, os_ = s.render_paths(0, pix, smp, slots=slots)
+    rel = _rel(g, o, 1e-3)
+    assert np.mean(rel > 1e-4) < 2e-3, np.mean(rel > 1e-4)
+    assert np.median(rel) < 1e-6
+    for k in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
+        assert abs(gs[k] - os_[k]) <= max(2, 1e-3 * os_[k]), k
+    e2 = dict(e)
+    e2["mindepth"] = 5
+    _, ns = oracle.Scene(e2).render_paths(0, pix, smp, slots=slots)
+    assert gs["ext_rays"] < 0.95 * ns["ext_rays"]
+
+
+def test_roulette_full_render_matches_oracle(gpu_roulette_room, oracle):
+    e = gpu_roulette_room.export()
+    s = oracle.Scene(e)
+    gpu_roulette_room.set_seed(6)
+    acc, gst = gpu_roulette_room.render_passes(0, 4)
+    oacc, ost = s.render(6, 0, 4)
+    diff = np.abs(acc - oacc)
+    assert np.mean(diff > 1e-3 * (1 + np.abs(oacc))) < 1e-3
+    np.testing.assert_allclose(acc.mean(axis=(0, 1)), oacc.mean(axis=(0, 1)), rtol=1e-4)
+    for k in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
+        assert abs(gst[k] - ost[k]) <= max(2, 1e-4 * ost[k]), k
 
 
 def test_full_render_matches_oracle_same_stream(gpu_box_room, oracle):

@@ -125,6 +125,26 @@ def test_error_codes(tmp_path):
     r.close()
 
 
+def test_non_finite_radiance_is_refused(tmp_path, box_room_export):
+    """color * 2^exposure overflows float at exposure >= 128 (squareLight.h:84-86),
+    a texel above 65504 is +inf as FP16: both would give infinite radiance, so
+    loading or uploading such a scene fails with AUR_EINVAL."""
+    from aurora_amd import scenes
+    r = host_renderer()
+    spec = scenes.box_room()
+    light = next(k for k, o in spec.objects.items() if o["type"] == "light")
+    spec.objects[light]["exposure"] = 200.0
+    _expect_error(lambda: r.load(spec.write(str(tmp_path / "exp"))), -1)
+    spec.objects[light]["exposure"] = 100.0  # 17 * 2^100 is finite
+    r.load(spec.write(str(tmp_path / "exp_ok")))
+    e = dict(box_room_export)
+    lights = [type(l).from_buffer_copy(l) for l in e["lights"]]
+    lights[0].exposure = 130.0
+    e["lights"] = lights
+    _expect_error(lambda: r.upload(e), -1)
+    r.close()
+
+
 def test_upload_rejects_bad_indices(box_room_export):
     e = dict(box_room_export)
     e["tris"] = e["tris"].copy()
