@@ -69,14 +69,21 @@ def test_trace_subdivided_spheres(tmp_path, oracle):
     sec = random_rays(rng, int(hit.sum()), [0, 0, 0], [0, 0, 0], tmin=5e-4)
     sec[:, :3] = P
     rays = np.concatenate([cam, sec]).astype(np.float32)
-    g, o = r.trace(rays), s.trace(rays)
-    mismatch = np.mean(g["prim"] != o["prim"])
+    o = s.trace(rays)
     assert hit.mean() > 0.5
-    assert mismatch <= 1e-4, mismatch
-    same = g["prim"] == o["prim"]
-    assert np.array_equal(g["t"][same], o["t"][same])
+    # bar (i): hit ids bit-exact against brute force; a ray may differ only as
+    # an exact-t tie (the same distance on two triangles, e.g. through a shared
+    # edge), and such rays stay below 1e-5.  Both the one-ray-per-lane
+    # traversal and the renderer's extension kernel are held to it.
+    for g in (r.trace(rays), r.trace(rays, extend_kernel=True)):
+        diff = g["prim"] != o["prim"]
+        assert diff.mean() < 1e-5, diff.mean()
+        assert np.all((g["prim"][diff] >= 0) & (o["prim"][diff] >= 0) & (g["t"][diff] == o["t"][diff]))
+        same = ~diff
+        for k in ("t", "u", "v"):
+            assert np.array_equal(g[k][same & (o["prim"] >= 0)], o[k][same & (o["prim"] >= 0)]), k
     ga, oa = r.trace(rays, any_hit=True), s.trace(rays, any_hit=True)
-    assert np.mean((ga["prim"] >= 0) != (oa["prim"] >= 0)) <= 1e-4
+    assert np.array_equal(ga["prim"] >= 0, oa["prim"] >= 0)
     r.close()
 
 
