@@ -76,9 +76,11 @@ __device__ inline void job_block(const JobDev &jd, int blk, int *pass_rel, int *
 enum { CTL_CURSOR = 0, CTL_ITER = 1, CTL_LIVE = 2, CTL_NEW = 3, CTL_BASE = 4, CTL_HIST = 8,
        CTL_TIME = CTL_HIST + BATCH_HIST, CTL_SIZE = CTL_TIME + BATCH_HIST };
 // per finished path, next to its rgba: extension and hit ray counts (at most
-// maxdepth + 1 <= 201 each) and the iteration it finished in (mod BATCH_HIST)
-__device__ inline unsigned pack_counts(int ext, int hit, long long iter) {
-    return (unsigned)min(ext, 511) | ((unsigned)min(hit, 511) << 9) | ((unsigned)(iter % BATCH_HIST) << 24);
+// maxdepth + 1 <= 201 each: 8 bits), shadow rays (at most 2 maxdepth <= 400:
+// 9 bits) and the iteration it finished in (mod BATCH_HIST: 6 bits)
+__device__ inline unsigned pack_counts(int ext, int hit, int shadow, long long iter) {
+    return (unsigned)min(ext, 255) | ((unsigned)min(hit, 255) << 8) | ((unsigned)min(shadow, 511) << 16) |
+           ((unsigned)(iter % BATCH_HIST) << 25);
 }
 
 struct BatchView {
@@ -270,27 +272,19 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__res
 }
 
 // Batched shadow rays on the node arena, from their job's root; the
-// contribution record is read only for unoccluded rays.  Each ray is counted
-// for its job (per wave: one atomic for the wave's first job).
+// contribution record is read only for unoccluded rays.
 template <int W, int B>
 __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bshadow(const JobDev *__restrict__ jobs,
                                                                    const WNode<W> *__restrict__ nodes,
-                                                                   const float4 *__restrict__ tris, WfView wv, int nq,
-                                                                   unsigned long long *__restrict__ jstats) {
+                                                                   const float4 *__restrict__ tris, WfView wv, int nq) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[C_SHADOW];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
     st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
-    // whole waves iterate together so the per-job count is a wave operation
-    for (int base = gtid - (int)__lane_id(); base < n; base += stride) {
-        const int i = base + (int)__lane_id();
-        const bool valid = i < n;
-        const float4 a = valid ? wv.sray[3 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        wave_count_job(jstats + J_SHADOW, J_STRIDE, __float_as_int(a.w), valid);
-        if (!valid) continue;
-        const float4 b = wv.sray[3 * i + 1];
+    for (int i = gtid; i < n; i += stride) {
+        const float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1];
         TravHit h;
         traverse<W, true>(nodes, tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h,
                           jobs[__float_as_int(a.w)].root);
@@ -318,7 +312,8 @@ __global__ void __launch_bounds__(256) k_bfinish(BatchView bv, WfView wv) {
         job_block(jd, m.z, &pass, &sub);
         const int dup = sub > 0 ? 1 : 0;
         bv.ring[w] = make_float4(L.x, L.y, L.z, a.w + (cam_miss ? 2.f : 0.f));
-        bv.ring_cnt[w] = pack_counts(path_ext_rays(m.x) - dup, path_hit_rays(m.x) - (dup && !cam_miss ? 1 : 0), iter);
+        bv.ring_cnt[w] = pack_counts(path_ext_rays(m.x) - dup, path_hit_rays(m.x) - (dup && !cam_miss ? 1 : 0),
+                                     (int)b.w, iter);
     }
 }
 
@@ -331,27 +326,29 @@ struct StatChunk {
 };
 
 __global__ void __launch_bounds__(256) k_bstats(BatchView bv, const StatChunk *__restrict__ chunks) {
-    __shared__ unsigned long long s_sum[3][256];
+    __shared__ unsigned long long s_sum[4][256];
     __shared__ unsigned long long s_tmax[256];
     const StatChunk ch = chunks[blockIdx.x];
-    unsigned long long e = 0, h = 0, np = 0, tmax = 0;
+    unsigned long long e = 0, h = 0, sh = 0, np = 0, tmax = 0;
     for (long long w = ch.begin + threadIdx.x; w < ch.end; w += blockDim.x) {
         const unsigned c = bv.ring_cnt[w & bv.ring_mask];
-        e += c & 511u;
-        h += (c >> 9) & 511u;
+        e += c & 255u;
+        h += (c >> 8) & 255u;
+        sh += (c >> 16) & 511u;
         np += 1;
-        const int it_next = (int)(((c >> 24) + 1) % BATCH_HIST);
+        const int it_next = (int)(((c >> 25) + 1) % BATCH_HIST);
         const unsigned long long t = (unsigned long long)bv.ctl[CTL_TIME + it_next];
         tmax = t > tmax ? t : tmax;
     }
     s_sum[0][threadIdx.x] = e;
     s_sum[1][threadIdx.x] = h;
-    s_sum[2][threadIdx.x] = np;
+    s_sum[2][threadIdx.x] = sh;
+    s_sum[3][threadIdx.x] = np;
     s_tmax[threadIdx.x] = tmax;
     __syncthreads();
     for (int off = blockDim.x / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) {
-            for (int k = 0; k < 3; ++k) s_sum[k][threadIdx.x] += s_sum[k][threadIdx.x + off];
+            for (int k = 0; k < 4; ++k) s_sum[k][threadIdx.x] += s_sum[k][threadIdx.x + off];
             const unsigned long long o = s_tmax[threadIdx.x + off];
             if (o > s_tmax[threadIdx.x]) s_tmax[threadIdx.x] = o;
         }
@@ -361,7 +358,8 @@ __global__ void __launch_bounds__(256) k_bstats(BatchView bv, const StatChunk *_
         unsigned long long *js = bv.jstats + (size_t)ch.job * J_STRIDE;
         atomicAdd(js + J_EXT, s_sum[0][0]);
         atomicAdd(js + J_HIT, s_sum[1][0]);
-        atomicAdd(js + J_PATHS, s_sum[2][0]);
+        atomicAdd(js + J_SHADOW, s_sum[2][0]);
+        atomicAdd(js + J_PATHS, s_sum[3][0]);
         atomicMax(&bv.t_end[ch.job], s_tmax[0]);
     }
 }
@@ -450,7 +448,7 @@ static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q
                        static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, q)
 #define AUR_BSHD(w_, b_)                                                                                    \
     hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs,                       \
-                       static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, 1 - q, bv.jstats)
+                       static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, 1 - q)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
     if (tex) hipLaunchKernelGGL(k_bshade<true>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
     else hipLaunchKernelGGL(k_bshade<false>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);

@@ -31,8 +31,9 @@ size_t trace_threads(const Ctx *c);
 
 enum { RT_CAMERA = 0, RT_DIFFUSE = 1, RT_SPECULAR = 2, RT_MIRROR = 3 };
 // Path flags: bits 0-1 ray type, 2 matte path, 3 last ray missed, 8-15
-// vertex index, 16-23 bounces, 24 no firefly clamp.  Shadow rays are counted
-// where they are traced (k_shadow, k_bshadow), not per path.
+// vertex index, 16-23 bounces, 24 no firefly clamp.  The shadow rays a path
+// has traced are counted in Lb.w (a float, exact to 2^24), which the batched
+// launch sums per job when the path finishes.
 enum { F_MATTE = 1 << 2, F_MISS = 1 << 3, F_NOCLAMP = 1 << 24 };
 // rays one finished path traced, from its flags: extension rays (camera ray
 // included) and those that hit
@@ -44,7 +45,8 @@ enum { S_PATHS = 0, S_EXT = 1, S_HIT = 2, S_SHADOW = 3 };
 // Path state lives with the queue: entry i of extension queue q holds the
 // ray, the throughput T (rgb, continue probability), the two radiance
 // accumulators La (emission and light-sample NEE, alpha in w) and Lb
-// (BSDF-sample NEE), and a meta record (flags, id0, id1, id2).  Shading copies
+// (BSDF-sample NEE; shadow rays traced in w), and a meta record (flags, id0,
+// id1, id2).  Shading copies
 // a continuing path's state to its slot in the next queue and a finished
 // path's to the finish list, so every access is coalesced; shadow rays carry
 // the position their contribution goes to.
@@ -172,7 +174,7 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
     const float4 hv = wv.hit[i];
     const int prim = __float_as_int(hv.w);
     float4 La = wv.La[q][i];
-    const float4 Lb = wv.Lb[q][i];
+    float4 Lb = wv.Lb[q][i];
     float4 T4 = wv.T[q][i];
     int flags = meta.x;
     const int rtype = flags & 3;
@@ -334,6 +336,7 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
         }
     }
     if (prim < 0) flags |= F_MISS;
+    Lb.w += (float)((sh[0].valid ? 1 : 0) + (sh[1].valid ? 1 : 0));
     out.cont = cont;
     out.finish = finish;
     out.sh[0] = sh[0];
