@@ -187,6 +187,7 @@ SIGNATURES = {
     "aur_set_profiling": (C.c_int, [_P, C.c_int]),
     "aur_kernel_stats": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int, _IP]),
     "aur_trace": (C.c_int, [_P, C.POINTER(Ray), C.c_int, C.POINTER(Hit), C.c_int]),
+    "aur_trace_noalloc": (C.c_int, [_P, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "aur_bsdf": (C.c_int, [_P, C.POINTER(BsdfQuery), C.c_int, C.POINTER(BsdfResult)]),
     "aur_kelemen_table": (C.c_int, [_P, C.c_float, C.c_int, C.c_int, C.c_uint32, _FP]),
     "aur_loop_subdivide": (C.c_int, [_P, C.c_int, C.c_int, _FP, _IP, C.c_int, C.c_float, C.c_float, C.c_int,

@@ -483,6 +483,7 @@ int aur_create(aur_ctx **out, int hip_device) {
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, hip_device));
         c->c.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        noalloc_init(hip_device);  // the device is current (set_device above)
         HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
         c->c.own_stream = true;
         *out = c.release();
@@ -1190,6 +1191,18 @@ int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int mode)
         trace_rays(c, dr.as<aur_ray>(), n, dh.as<aur_hit>(), mode);
         HIP_CHECK(hipMemcpyAsync(hits, dh.p, sizeof(aur_hit) * n, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int aur_trace_noalloc(aur_ctx *ctx, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (n < 0) throw std::invalid_argument("negative ray count");
+        if (n == 0) return;
+        NEED(d_rays); NEED(d_hits);
+        set_device(c);
+        trace_rays_noalloc(c, d_rays, n, d_hits, mode);
     });
 }
 

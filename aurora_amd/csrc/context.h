@@ -234,6 +234,10 @@ inline size_t pass_paths(size_t wh, int first_pass, int n_passes, int ls) {
 void resolve_passes(Ctx *c, const float4 *out_rgba, int n_passes, float4 *accum);
 // mode: AUR_TRACE_CLOSEST, AUR_TRACE_ANY or AUR_TRACE_EXTEND
 void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode);
+// the allocation-free variant (closest / any hit) on the file-scope traversal
+// scratch; noalloc_init looks up that scratch's address on the current device
+void noalloc_init(int device);
+void trace_rays_noalloc(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode);
 // sizes the path pool and the traversal stack overflow for BVHs of the given width and depth
 void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth);
 // batch.hip: jobs[k] renders passes [first_pass[k], first_pass[k] + n_passes[k]) into its framebuffer

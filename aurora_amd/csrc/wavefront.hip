@@ -21,6 +21,8 @@
 // frame is a fixed sequence of launches with no host round trip.
 #include "wavefront_common.h"
 
+#include <mutex>
+
 namespace aur {
 
 // Path id -> pixel, pass and light sample.  Mode 0 issues a batch of passes
@@ -633,6 +635,64 @@ void trace_rays(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode)
 #define AUR_TRC(w_, b_)                                                                                      \
     hipLaunchKernelGGL((k_trace<w_, b_>), dim3(blocks), dim3(b_), 0, c->stream, sv, d_rays, n, d_hits, any_hit, \
                        c->wf.stack_ovf.as<int>(), (int)c->wf.stack_ovf_per_thread)
+    AUR_TRAV_DISPATCH(c->bvh.width, B, AUR_TRC);
+#undef AUR_TRC
+    HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// aur_trace_noalloc: the ray queries of aur_trace for callers that cannot
+// allocate device memory (code under stream capture, serving loops).  The
+// only scratch of these kernels is the traversal stack's overflow area; here
+// it is the file-scope array below: AUR_NOALLOC_THREADS lanes (the fixed
+// grid of these launches) times AUR_NOALLOC_OVF entries, 16 MiB per GPU, held
+// until the process ends.  Every call on one GPU shares it: calls that overlap
+// on that GPU (two streams, two host threads) race on it.  Nothing in it has
+// to start at zero -- a lane writes each overflow entry before it reads it.
+// ---------------------------------------------------------------------------
+#define AUR_NOALLOC_THREADS 32768
+#define AUR_NOALLOC_OVF 128
+static __device__ __align__(256) int scratch[AUR_NOALLOC_THREADS * AUR_NOALLOC_OVF];
+
+// the array's device address, one per GPU (the host-side name is not one)
+static std::mutex g_scratch_mu;
+static std::vector<int *> g_scratch_ptr;
+
+void noalloc_init(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return;
+    // the caller made `device` current
+    void *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned a null address for the traversal scratch");
+    g_scratch_ptr[device] = static_cast<int *>(p);
+}
+
+static int *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= (int)g_scratch_ptr.size() || !g_scratch_ptr[device])
+        throw StateError("the traversal scratch of this device was not set up (aur_create)");
+    return g_scratch_ptr[device];
+}
+
+void trace_rays_noalloc(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode) {
+    if (!c->built) throw StateError("trace before aur_build_accel");
+    if (mode != AUR_TRACE_CLOSEST && mode != AUR_TRACE_ANY)
+        throw std::invalid_argument("aur_trace_noalloc traces AUR_TRACE_CLOSEST or AUR_TRACE_ANY only");
+    // stack entries one ray can need: W - 1 per level below the root, plus two
+    const int need = (c->bvh.width - 1) * (c->bvh.depth + 1) + 2;
+    if (need - AUR_STACK_LDS > AUR_NOALLOC_OVF)
+        throw CapacityError("BVH too deep for the fixed traversal scratch (" + std::to_string(need - AUR_STACK_LDS) +
+                            " overflow entries per ray, at most " + std::to_string(AUR_NOALLOC_OVF) + ")");
+    int *ovf = noalloc_scratch(c->device);
+    SceneView sv = scene_view(c);
+    const int B = c->launch.trace_block;
+    const int blocks = std::min((n + B - 1) / B, AUR_NOALLOC_THREADS / B);
+    if (blocks < 1) return;
+#define AUR_TRC(w_, b_)                                                                                      \
+    hipLaunchKernelGGL((k_trace<w_, b_>), dim3(blocks), dim3(b_), 0, c->stream, sv, d_rays, n, d_hits,        \
+                       mode == AUR_TRACE_ANY ? 1 : 0, ovf, AUR_NOALLOC_OVF)
     AUR_TRAV_DISPATCH(c->bvh.width, B, AUR_TRC);
 #undef AUR_TRC
     HIP_CHECK(hipGetLastError());

@@ -370,6 +370,15 @@ int aur_kernel_stats(aur_ctx *ctx, aur_kernel_stat *out, int max, int *n_out);
 #define AUR_TRACE_ANY 1
 #define AUR_TRACE_EXTEND 2
 int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int mode);
+/* No counterpart in the reference.  aur_trace (AUR_TRACE_CLOSEST or
+ * AUR_TRACE_ANY) for callers that cannot allocate device memory -- code under
+ * stream capture, tight serving loops: d_rays and d_hits are device buffers of
+ * n entries, the call only enqueues work on the context's stream (results are
+ * ready when that stream has reached it) and allocates nothing.  The traversal
+ * stack's overflow lives in a fixed 16 MiB per GPU shared by every call on
+ * that GPU, so calls that overlap on one GPU race; a BVH that needs more than
+ * 128 overflow entries per ray fails with AUR_ECAPACITY. */
+int aur_trace_noalloc(aur_ctx *ctx, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode);
 int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r);
 /* Kelemen albedo table (64 cos x 32 roughness), kelemenMaterial.cpp:113-178. */
 int aur_kelemen_table(aur_ctx *ctx, float reflectance, int distribution, int n_samples,
