@@ -64,6 +64,11 @@ __device__ inline void load_node(const WNode<W> *__restrict__ nodes, int ni, Nod
 typedef __attribute__((address_space(3))) int lds_int;
 
 struct TravStack {
+#ifdef AUR_TRAV_STATS
+    // measurement builds only (-DAUR_TRAV_STATS): nodes visited, leaves and
+    // triangles tested by this lane
+    unsigned n_visits = 0, n_leaves = 0, n_tris = 0;
+#endif
     lds_int *lds_base;  // the workgroup's stack array
     lds_int *lds;       // this lane's entry 0: entry k is lds[k * stride]
     int stride;     // lanes per workgroup
@@ -168,12 +173,19 @@ __device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restric
     }
     // the leaves this node's boxes let through, one per pass of the loop:
     // lanes with leaves in different slots test them together
+#ifdef AUR_TRAV_STATS
+    st.n_visits += 1;
+#endif
     while (leaves) {
         const int k = __builtin_ctz(leaves);
         leaves &= leaves - 1;
         int c = ch[0];
 #pragma unroll
         for (int j = 1; j < W; ++j) c = (k == j) ? ch[j] : c;
+#ifdef AUR_TRAV_STATS
+        st.n_leaves += 1;
+        st.n_tris += (unsigned)((~c) & 15) + 1u;
+#endif
         if (test_leaf<ANY>(tris, c, r.o, r.d, r.tmin, h)) return true;
     }
     if (!ANY) {

@@ -83,6 +83,25 @@ __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
     }
 }
 
+#ifdef AUR_TRAV_STATS
+// measurement builds only: per traversal kernel (0 extend, 1 shadow) nodes
+// visited, leaves and triangles tested, summed over all lanes
+__device__ unsigned long long g_trav_stats[2][4];
+__device__ inline void trav_stats_add(int k, const TravStack &st) {
+    unsigned long long v = st.n_visits, l = st.n_leaves, t = st.n_tris;
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+        l += __shfl_xor(l, off);
+        t += __shfl_xor(t, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&g_trav_stats[k][0], v);
+        atomicAdd(&g_trav_stats[k][1], l);
+        atomicAdd(&g_trav_stats[k][2], t);
+    }
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // extend: closest hit.  Lanes are refilled independently (traverse_stream):
 // closest-hit traversals of incoherent rays differ in length several times
@@ -115,6 +134,9 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_extend(SceneView sv, W
     // one add per wave into this workgroup's partial (no global same-address atomics)
     for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
     if (__lane_id() == 0 && hits) atomicAdd(&wv.partials[blockIdx.x], (unsigned long long)hits);
+#ifdef AUR_TRAV_STATS
+    trav_stats_add(0, st);
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -138,6 +160,9 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_shadow(SceneView sv, W
         traverse<W, true>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h);
         if (h.prim < 0) shadow_add(wv, nq, __float_as_int(c.w), c);
     }
+#ifdef AUR_TRAV_STATS
+    trav_stats_add(1, st);
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -699,3 +724,14 @@ void trace_rays_noalloc(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, i
 }
 
 }  // namespace aur
+
+#ifdef AUR_TRAV_STATS
+// measurement builds only: copies the traversal counters (extend, then
+// shadow: visits, leaves, triangles, unused) and clears them
+extern "C" int aur_debug_trav_stats(unsigned long long *out8) {
+    unsigned long long z[8] = {0};
+    if (hipMemcpyFromSymbol(out8, HIP_SYMBOL(aur::g_trav_stats), sizeof z) != hipSuccess) return -4;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(aur::g_trav_stats), z, sizeof z) != hipSuccess) return -4;
+    return 0;
+}
+#endif
