@@ -1,7 +1,9 @@
 // adjoint.hip -- path-replay adjoint of the forward integrator.
 //
 // Gradients of L = sum_pixels dL/dI . I with respect to per-triangle diffuse
-// albedo, per-material (Kelemen) roughness and per-material specular weight,
+// albedo, per-material roughness (Kelemen, and metal with roughnessA =
+// roughnessB as aur_set_material_params sets them) and per-material Kelemen
+// specular weight,
 // for the estimator `Integrator::integrateCameraSample` computes
 // (integrator.cpp:152-300) with fixed random numbers.
 //
@@ -129,6 +131,10 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
         const Bsdf &b = ch.b;
         const bool spec_lobe = b.kind == BSDF_CT;
         const bool kel = m.type == MAT_KELEMEN;
+        // the roughness of the Cook-Torrance lobe is a parameter: Kelemen, and
+        // metal (metalMaterial.cpp:34-62: the lobe picked by u against mix,
+        // which does not depend on roughness, so there is no score term)
+        const bool rough_lobe = spec_lobe && (kel || m.type == MAT_METAL);
         // the diffuse albedo of this vertex is a parameter when its texel or
         // its triangle's override is being differentiated
         const float *a_val = nullptr;
@@ -203,10 +209,8 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
                     if (a_val[1] > 0.f) g.add(&a_grad[1], delta.y * cl.y / a_val[1]);
                     if (a_val[2] > 0.f) g.add(&a_grad[2], delta.z * cl.z / a_val[2]);
                 }
-                if (kel && spec_lobe) {
-                    g.add(&g.spec[attr], dot3(delta, add3(cl, cb)) / m.spec[0]);
-                    g.add(&g.rough[attr], (dot3(delta, cl) * dl_l + dot3(delta, cb) * dl_b) * ch.drough);
-                }
+                if (kel && spec_lobe) g.add(&g.spec[attr], dot3(delta, add3(cl, cb)) / m.spec[0]);
+                if (rough_lobe) g.add(&g.rough[attr], (dot3(delta, cl) * dl_l + dot3(delta, cb) * dl_b) * ch.drough);
                 R = sub3(R, add3(cl, cb));
             }
         }
@@ -228,9 +232,9 @@ __device__ void replay(const SceneView &sv, const WNode<W> *nodes, TravStack &st
                 if (a_val[1] > 0.f) g.add(&a_grad[1], delta.y * R.y / a_val[1]);
                 if (a_val[2] > 0.f) g.add(&a_grad[2], delta.z * R.z / a_val[2]);
             }
-            if (kel && spec_lobe) {
+            if (rough_lobe) {
                 float dr = dot3(delta, R);
-                g.add(&g.spec[attr], dr / m.spec[0]);
+                if (kel) g.add(&g.spec[attr], dr / m.spec[0]);
                 float c = dot3(nn, norm3(add3(s.dir, vn)));
                 g.add(&g.rough[attr], dr * ct_dlogd_drough(b.dist, b.rough, c) * ch.drough);
             }

@@ -369,7 +369,9 @@ static void upload(Ctx *c) {
     const size_t tsz = AUR_KEL_COS * AUR_KEL_ROUGH;
     c->d_tables.reserve(sizeof(float) * tsz * std::max<size_t>(tables.size(), 1));
     for (size_t t = 0; t < tables.size(); ++t)
-        kelemen_table(c, tables[t].first, tables[t].second, 65536, 0x4B454Cu, c->d_tables.as<float>() + t * tsz);
+        // the reference's table: SPECALBEDO_SAMPLES samples per entry, stored as FP16
+        kelemen_table(c, tables[t].first, tables[t].second, AUR_KEL_SAMPLES, 0x4B454Cu, c->d_tables.as<float>() + t * tsz,
+                      true);
 
     c->dlights.clear();
     c->n_env = 0;

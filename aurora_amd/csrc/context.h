@@ -248,7 +248,9 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
              float *d_gptex, aur_stats *stats);
 // parity.hip
 void bsdf_queries(Ctx *c, const aur_bsdf_query *d_q, int n, aur_bsdf_result *d_r);
-void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out);
+// the Monte-Carlo estimate of the table; half_store rounds each entry to FP16
+void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out,
+                   bool half_store = false);
 // kernels used for device-side clears (graph safe, no hipMemsetAsync)
 void fill_zero(Ctx *c, void *p, size_t bytes);
 

@@ -33,6 +33,8 @@ namespace aur {
 #define AUR_PI_F 3.14159265358979323846f
 #define AUR_SLOTS_PER_VERTEX 9
 #define AUR_KEL_COS 64
+// Monte-Carlo samples per albedo-table entry (SPECALBEDO_SAMPLES, constants.h:26)
+#define AUR_KEL_SAMPLES 250000
 #define AUR_KEL_ROUGH 32
 
 enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_KELEMEN = 2, MAT_LIGHT = 3, MAT_GLASS = 4 };

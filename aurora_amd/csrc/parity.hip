@@ -38,7 +38,11 @@ void bsdf_queries(Ctx *c, const aur_bsdf_query *d_q, int n, aur_bsdf_result *d_r
 // Kelemen albedo table, one workgroup per (cos theta, roughness) entry:
 // the estimator of kelemenMaterial.cpp:113-178 with a counter-based sample
 // stream; partial sums in double, reduced in LDS.
-__global__ void __launch_bounds__(256) k_kelemen(float reflectance, int dist, int n_samples, uint32_t seed, float *__restrict__ out) {
+// half_store: the entry is stored as FP16, as the reference writes its table
+// through the FP16 display and reads it back as Imf::HALF
+// (kelemenMaterial.cpp:53-76, 170-177)
+__global__ void __launch_bounds__(256) k_kelemen(float reflectance, int dist, int n_samples, uint32_t seed, int half_store,
+                                                 float *__restrict__ out) {
     __shared__ double s_sum[256];
     const int e = blockIdx.x;
     const int i = e / AUR_KEL_ROUGH, ri = e % AUR_KEL_ROUGH;
@@ -73,11 +77,15 @@ __global__ void __launch_bounds__(256) k_kelemen(float reflectance, int dist, in
         if ((int)threadIdx.x < s) s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[e] = (float)(s_sum[0] / n_samples);
+    if (threadIdx.x == 0) {
+        const float v = (float)(s_sum[0] / n_samples);
+        out[e] = half_store ? (float)(_Float16)v : v;  // to nearest even
+    }
 }
 
-void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out) {
-    hipLaunchKernelGGL(k_kelemen, dim3(AUR_KEL_COS * AUR_KEL_ROUGH), dim3(256), 0, c->stream, reflectance, dist, n_samples, seed, d_out);
+void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out, bool half_store) {
+    hipLaunchKernelGGL(k_kelemen, dim3(AUR_KEL_COS * AUR_KEL_ROUGH), dim3(256), 0, c->stream, reflectance, dist, n_samples,
+                       seed, half_store ? 1 : 0, d_out);
     HIP_CHECK(hipGetLastError());
 }
 

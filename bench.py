@@ -271,16 +271,21 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
         return float(np.mean((im.astype(np.float64) - p.target) ** 2))
 
     r.set_seed(5)
-    ga, _, gs, _ = r.adjoint(0, spp, dl.astype(np.float32))
-    # 64 seeded parameters among the albedo channels and spec weights whose
-    # gradient is not negligible (the image is linear in both, so central
-    # differences with common random numbers are exact up to rounding)
-    cand = [("a", int(k)) for k in np.flatnonzero(np.abs(ga.ravel()) > 1e-2 * np.abs(ga).max())]
-    cand += [("s", int(k)) for k in np.flatnonzero(np.abs(gs) > 1e-2 * np.abs(gs).max())]
-    pick = np.random.default_rng(64).permutation(len(cand))[:n_fd]
+    ga, gr, gs, _ = r.adjoint(0, spp, dl.astype(np.float32))
+    # 64 seeded parameters: every material roughness, then spec weights and
+    # albedo channels whose gradient is not negligible.  The image is linear
+    # in albedo and spec weight, so central differences with common random
+    # numbers are exact up to rounding; roughness moves sampled directions and
+    # Kelemen lobe choices (and the reference's table lookup jumps at every
+    # 1/31), so its differences (h = 1e-3, inside one table cell) are compared
+    # in the mean over seeds.
+    rough_k = [int(k) for k in np.flatnonzero(np.abs(gr) > 1e-3 * np.abs(gr).max())]
+    cand = [("s", int(k)) for k in np.flatnonzero(np.abs(gs) > 1e-3 * np.abs(gs).max())]
+    cand_a = [("a", int(k)) for k in np.flatnonzero(np.abs(ga.ravel()) > 1e-3 * np.abs(ga).max())]
+    cand += [cand_a[j] for j in np.random.default_rng(64).permutation(len(cand_a))]
+    pick = cand[:max(0, n_fd - len(rough_k))]
     h, worst = 1e-3, 0.0
-    for j in pick:
-        kind, k = cand[j]
+    for kind, k in pick:
         if kind == "a":
             t, c = divmod(k, 3)
             ap, am = albedo.copy(), albedo.copy()
@@ -295,6 +300,28 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
             fd = (loss(albedo, sp) - loss(albedo, sm)) / (2 * h)
             g, gmax = gs[k], np.abs(gs).max()
         worst = max(worst, abs(g - fd) / max(abs(fd), 1e-3 * gmax))
+    # roughness over 8 seeds: adjoint and central differences estimate the
+    # same derivative of the linearised loss <dL/dI, I> (dL/dI fixed at the
+    # seed-5 image, for both sides) with different noise
+    n_seeds = 8
+    fdr, adr = np.zeros((n_seeds, len(rough_k))), np.zeros((n_seeds, len(rough_k)))
+    for si in range(n_seeds):
+        r.set_seed(40 + si)
+        _, gr_s, _, _ = r.adjoint(0, spp, dl.astype(np.float32))
+        for j, k in enumerate(rough_k):
+            adr[si, j] = gr_s[k]
+            rp, rm = rough.copy(), rough.copy()
+            rp[k] += h
+            rm[k] -= h
+            vals = []
+            for rv in (rp, rm):
+                r.set_seed(40 + si)
+                im = inverse.render_image(r, albedo, rv, spec, 0, spp)
+                vals.append(float(np.sum(dl * im.astype(np.float64))))
+            fdr[si, j] = (vals[0] - vals[1]) / (2 * h)
+    se_r = np.sqrt(fdr.var(0, ddof=1) / n_seeds + adr.var(0, ddof=1) / n_seeds) + 1e-30
+    z_r = np.abs(fdr.mean(0) - adr.mean(0)) / se_r
+    rel_r = np.abs(fdr.mean(0) - adr.mean(0)) / np.maximum(np.abs(fdr.mean(0)), 1e-30)
     n_paths = res * res * spp
     adam = None
     if adam_steps:
@@ -310,7 +337,10 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
     return {"adam": adam, "workload": "box_room_kelemen_%d_spp%d_depth%d" % (res, spp, depth), "params": int(ga.size + gr.size + gs.size),
             "forward_ms": round(fwd_s * 1e3, 3), "adjoint_ms": round(adj_s * 1e3, 3),
             "adjoint_Mpaths_s": round(n_paths / adj_s / 1e6, 3), "adjoint_vs_forward": round(adj_s / fwd_s, 3),
-            "fd_checked": int(len(pick)), "fd_h": h, "fd_max_rel_err": float("%.3g" % worst)}
+            "fd_checked": int(len(pick)) + len(rough_k), "fd_h": h, "fd_max_rel_err": float("%.3g" % worst),
+            "fd_albedo_spec_checked": int(len(pick)), "fd_roughness_checked": len(rough_k),
+            "fd_roughness_seeds": n_seeds, "fd_roughness_max_z": float("%.3g" % (z_r.max() if len(z_r) else 0.0)),
+            "fd_roughness_rel_err_of_means": [float("%.3g" % x) for x in rel_r]}
 
 
 def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd: int = 64):

@@ -341,7 +341,9 @@ typedef struct {
 /* Path-replay adjoint of passes [first_pass, first_pass + n_passes) rendered
  * with the current seed: given dL/dI for the mean image I (4 * W * H floats,
  * host, alpha ignored) writes dL/dtheta into the non-NULL fields of out.
- * Roughness and spec-weight gradients are produced for Kelemen materials. */
+ * Roughness gradients are produced for Kelemen materials and for metal
+ * materials (whose roughnessA and roughnessB aur_set_material_params sets to
+ * the one value), spec-weight gradients for Kelemen materials. */
 int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_dimage,
                 aur_param_grads *out, aur_stats *stats);
 

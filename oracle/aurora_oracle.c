@@ -1305,6 +1305,10 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
         float ulobe = rng_u(rng, vslot + 0);
         if (!get_brdf(s, attr, prim, vn, Nn, matte_path, ulobe, &hg, &b)) break;
         int kel = m->type == ORC_MAT_KELEMEN, spec = b.kind == ORC_BSDF_CT;
+        /* the CT lobe's roughness is a parameter for Kelemen and for metal
+         * (roughnessA = roughnessB; metalMaterial.cpp:34-62 picks the lobe by
+         * u against mix, independent of roughness) */
+        int rlobe = spec && (kel || m->type == ORC_MAT_METAL);
         int alb = b.kind == ORC_BSDF_LAMBERT && s->tri_albedo != NULL;
         double drough = matte_path && (m->type == ORC_MAT_KELEMEN || m->type == ORC_MAT_METAL) ? (1. - CAUSTIC_BLUR) : 1.;
         if (grad && kel && m->reflectance < 1.f) {
@@ -1362,10 +1366,10 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
                     const float *a = s->tri_albedo + 3 * prim;
                     for (int k = 0; k < 3; ++k) g->albedo[3 * prim + k] += delta[k] * cl[k] / a[k];
                 }
-                if (kel && spec) {
+                if (rlobe) {
                     double dc = 0, dlr = 0;
                     for (int k = 0; k < 3; ++k) { dc += delta[k] * (cl[k] + cb[k]); dlr += delta[k] * (cl[k] * dll + cb[k] * dlb); }
-                    g->spec[attr] += dc / m->spec[0];
+                    if (kel) g->spec[attr] += dc / m->spec[0];
                     g->rough[attr] += dlr * drough;
                 }
                 for (int k = 0; k < 3; ++k) R[k] -= cl[k] + cb[k];
@@ -1388,8 +1392,8 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
                 const float *a = s->tri_albedo + 3 * prim;
                 for (int k = 0; k < 3; ++k) g->albedo[3 * prim + k] += delta[k] * R[k] / a[k];
             }
-            if (kel && spec) {
-                g->spec[attr] += dr / m->spec[0];
+            if (rlobe) {
+                if (kel) g->spec[attr] += dr / m->spec[0];
                 g->rough[attr] += dr * dlogd_drough(b.dist, b.rough, dot3(Nn, vnorm(vadd(v3p(smp.ray.d), vn)))) * drough;
             }
         }
@@ -1436,6 +1440,8 @@ void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_
 #pragma omp for schedule(dynamic, 256)
         for (long long i = 0; i < n; ++i) {
             int pixel = (int)(i % wh), sample = first_pass + (int)(i / wh);
+            /* a pixel without loss gradient adds exactly nothing */
+            if (dl_dimage[4 * pixel] == 0.f && dl_dimage[4 * pixel + 1] == 0.f && dl_dimage[4 * pixel + 2] == 0.f) continue;
             Rng r = make_rng(s, NULL, seed, pixel % s->width, pixel / s->width, (uint32_t)sample);
             double L[3];
             int k = path_replay(s, b, pixel % s->width, pixel / s->width, &r, 0, NULL, NULL, L, &g);

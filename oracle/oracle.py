@@ -15,7 +15,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
-KELEMEN_SAMPLES = 65536
+# samples per albedo-table entry: SPECALBEDO_SAMPLES (constants.h:26)
+KELEMEN_SAMPLES = 250000
 KELEMEN_SEED = 0x4B454C
 
 
@@ -157,6 +158,17 @@ def kelemen_table(reflectance: float, distribution: int, n_samples: int = KELEME
     return _table_cache[key]
 
 
+def scene_kelemen_table(reflectance: float, distribution: int) -> np.ndarray:
+    """The table a scene's Kelemen material reads: the estimate at
+    SPECALBEDO_SAMPLES per entry, written through the FP16 display and read
+    back as Imf::HALF (kelemenMaterial.cpp:53-76, 170-177): each entry
+    rounded to FP16, to nearest even."""
+    key = ("half", float(np.float32(reflectance)), distribution)
+    if key not in _table_cache:
+        _table_cache[key] = kelemen_table(reflectance, distribution).astype(np.float16).astype(np.float32)
+    return _table_cache[key]
+
+
 class Scene:
     """An oracle scene built from the renderer's flattened export."""
 
@@ -186,7 +198,7 @@ class Scene:
             if m.type == 2:
                 # roundf(): halves away from zero, like the reference (kelemenMaterial.cpp:50)
                 q = min(max(int(np.floor(float(np.float32(m.reflectance)) * 20.0 + 0.5)), 0), 20)
-                t = kelemen_table(np.float32(q / 20.0), m.distribution)
+                t = scene_kelemen_table(np.float32(q / 20.0), m.distribution)
                 self.tables.append(t)
                 o.table = fp(t)
         cam = np.asarray(exp.get("camera", np.concatenate([np.eye(4), np.eye(4)]).reshape(-1)), np.float32)

@@ -167,6 +167,10 @@ def test_kelemen_table_parity(gpu_box_room, oracle, dist):
     g = gpu_box_room.kelemen_table(0.05, dist, 4096, 99).ravel()
     o = oracle.kelemen_table(0.05, dist, n_samples=4096, seed=99)
     assert np.max(_rel(g, o, 1e-6)) < 1e-5
+    # the table scenes read: 250,000 samples per entry (constants.h:26), stored
+    # as FP16 (kelemenMaterial.cpp:53-76): equal bit for bit after the rounding
+    g = gpu_box_room.kelemen_table(0.05, dist, oracle.KELEMEN_SAMPLES, oracle.KELEMEN_SEED).ravel()
+    assert np.array_equal(g.astype(np.float16), oracle.scene_kelemen_table(0.05, dist).astype(np.float16))
 
 
 # ---------------------------------------------------------------------------

@@ -71,3 +71,36 @@ def test_adjoint_matches_finite_differences(oracle, kelemen_room):
         fd = (_loss(oracle, e, albedo, sp, w, seed, passes) - _loss(oracle, e, albedo, sm, w, seed, passes)) / (2 * h)
         assert gs[m] == pytest.approx(fd, rel=1e-2, abs=1e-3 * np.abs(gs).max()), m
     assert np.all(np.isfinite(gr)) and np.abs(gr).max() > 0
+
+
+@pytest.mark.parametrize("dist", [0, 1])
+def test_metal_roughness_gradient_matches_quadrature(oracle, dist):
+    """A single camera ray meets a Beckmann / GGX metal floor under a quad
+    light (depth 1: only next-event estimation contributes).  The expected
+    radiance E(r) is the quadrature of the MIS-weighted f Le cos cos / d^2
+    over the light; its central difference in the roughness r (roughnessA =
+    roughnessB, metalMaterial.cpp:34-62) is the exact gradient.  The oracle's
+    path-replay adjoint, averaged over 16 batches of 8192 passes, matches it
+    within 4 standard errors and 2e-3 relative."""
+    import test_oracle_pins as P
+    F0, gain, r0, h = (0.04, 0.5, 0.9), (1.0, 0.8, 0.6), 0.3, 1e-3
+    e, Rinv = P._metal_floor_scene(45, dist, r0, r0, 0.4, F0, gain, -2.0)
+    s = oracle.Scene(e)
+    hit = s.trace(np.array([[0, 0, 0, 0.1, 0, 0, -1, 1e7]], np.float32))
+    Pt = Rinv.T @ np.array([0, 0, -float(hit["t"][0])], np.float64)
+    V = Rinv.T @ np.array([0, 0, 1.0])
+    # the light sits off the specular peak, where the MIS weights sum to one
+    dE = (P._metal_nee_quadrature(Pt, V, dist, r0 + h, F0, gain, -2.0)[2] -
+          P._metal_nee_quadrature(Pt, V, dist, r0 - h, F0, gain, -2.0)[2]) / (2 * h)
+    dl = np.zeros((1, 1, 4), np.float32)
+    dl[0, 0, :3] = 1.0
+    g = []
+    for b in range(16):
+        sb = oracle.Scene(e)
+        sb.set_material_params(roughness=[r0, r0])
+        _, gr, _ = sb.adjoint(1000 + b, 0, 8192, dl, threads=8)
+        g.append(gr[0])
+    g = np.array(g, np.float64)
+    se = g.std() / np.sqrt(g.size)
+    assert abs(g.mean() - dE.sum()) < 4 * se + 1e-12, (g.mean(), dE.sum(), se)
+    assert abs(g.mean() / dE.sum() - 1) < 2e-3

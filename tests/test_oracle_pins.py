@@ -644,3 +644,20 @@ def test_roulette_is_unbiased_in_full_paths(oracle, box_room_export):
     z = np.abs(d.mean(1)) / (d.std(1) / math.sqrt(spp) + 1e-12)
     assert np.mean(z > 4.0) <= 0.01, np.sort(z)[-5:]
     assert abs(la[:, :3].astype(np.float64).mean() / lb[:, :3].astype(np.float64).mean() - 1) < 0.01
+
+
+@pytest.mark.parametrize("dist", [0, 1])
+def test_scene_kelemen_table_is_the_fp16_250k_estimate(oracle, dist):
+    """A scene's Kelemen table is the 250,000-sample estimate per entry
+    (SPECALBEDO_SAMPLES, constants.h:26) rounded to FP16 as the reference's
+    EXR round trip stores it (kelemenMaterial.cpp:53-76): every entry is an
+    FP16 value within half an FP16 ulp of the float estimate, and the estimate
+    is within its Monte-Carlo error of a 4x smaller independent one."""
+    assert oracle.KELEMEN_SAMPLES == 250000
+    t = oracle.scene_kelemen_table(0.05, dist)
+    est = oracle.kelemen_table(0.05, dist)
+    assert np.array_equal(t, t.astype(np.float16).astype(np.float32))
+    ulp = np.spacing(est.astype(np.float16)).astype(np.float32)
+    assert np.all(np.abs(t - est) <= 0.5 * ulp + 1e-12)
+    small = oracle.kelemen_table(0.05, dist, n_samples=62500, seed=7)
+    assert np.max(np.abs(small - est)) < 0.02 and np.mean(np.abs(small - est)) < 2e-3
