@@ -839,6 +839,27 @@ float orc_kelemen_albedo(const float *tab, float costheta, float roughness) {
  * interpolated normal Ng, the interpolated uv and the attribute. */
 typedef struct { V3 P, Ng; float s, t; int attr; } Geo;
 
+/* The FP32 albedo texture: the texel texture2DShader.h:28-31 reads at the
+ * hit's uv (u - floor(u), 1 - (v - floor(v))) with Texture2D::read's nearest
+ * texel and clamps (texture2D.cpp:48-66), without the FP16 rounding of
+ * Imf::Rgba.  Returns the texel index, or -1 when the attribute has none. */
+static long long ptex_index(const orc_scene *s, int attr, const Geo *g) {
+    if (!s->ptex || !s->ptex_mat || !s->ptex_mat[attr]) return -1;
+    float a = g->s - floorf(g->s), b = 1.f - (g->t - floorf(g->t));
+    int x = to_int_x86(floorf(a * s->ptex_w)), y = to_int_x86(floorf(b * s->ptex_h));
+    if (y == s->ptex_h) y -= 1;
+    if (x == s->ptex_w) x -= 1;
+    if (x < 0) x = 0;
+    if (y < 0) y = 0;
+    if (x > s->ptex_w - 1) x = s->ptex_w - 1;
+    if (y > s->ptex_h - 1) y = s->ptex_h - 1;
+    return (long long)y * s->ptex_w + x;
+}
+
+/* the diffuse colour of a Lambert lobe: albedo texture, per-triangle
+ * override, or the material's (shader-driven) colour */
+static C3 diffuse_color(const orc_scene *s, const orc_material *m, int attr, int prim, const Geo *g);
+
 static C3 mat_color(const orc_scene *s, const orc_material *m, int slot, const float *k, const Geo *g) {
     if (m->shader[slot] >= 0) return shader_eval(s, m->shader[slot], g->P, g->Ng, g->s, g->t, g->attr);
     return c3p(k);
@@ -856,7 +877,7 @@ static int get_brdf(const orc_scene *s, int attr, int prim, V3 vn, V3 nn, int ma
     switch (m->type) {
     case ORC_MAT_MATTE: /* matteMaterial.cpp:30-35 */
         b->kind = ORC_BSDF_LAMBERT; b->rough = 0.f; b->dist = 0;
-        b->color = s->tri_albedo ? c3p(s->tri_albedo + 3 * prim) : mat_color(s, m, ORC_SLOT_DIFFUSE, m->diffuse, g);
+        b->color = diffuse_color(s, m, attr, prim, g);
         return 1;
     case ORC_MAT_METAL: { /* metalMaterial.cpp:32-62 */
         float mix = mat_float(s, m, ORC_SLOT_MIX, m->mix, g);
@@ -878,7 +899,7 @@ static int get_brdf(const orc_scene *s, int attr, int prim, V3 vn, V3 nn, int ma
             float albedo = orc_kelemen_albedo(m->table, fclampf(dot3(nn, vn), 0.f, 1.f), rough);
             if (u_lobe > albedo) {
                 b->kind = ORC_BSDF_LAMBERT; b->rough = 0.f; b->dist = 0;
-                b->color = s->tri_albedo ? c3p(s->tri_albedo + 3 * prim) : mat_color(s, m, ORC_SLOT_DIFFUSE, m->diffuse, g);
+                b->color = diffuse_color(s, m, attr, prim, g);
                 return 1;
             }
         }
@@ -898,6 +919,13 @@ static int get_brdf(const orc_scene *s, int attr, int prim, V3 vn, V3 nn, int ma
     default: /* constantMaterial.cpp:19-24: pdf-0 sample terminates the path */
         return 0;
     }
+}
+
+static C3 diffuse_color(const orc_scene *s, const orc_material *m, int attr, int prim, const Geo *g) {
+    long long ti = ptex_index(s, attr, g);
+    if (ti >= 0) return c3p(s->ptex + 3 * ti);
+    if (s->tri_albedo) return c3p(s->tri_albedo + 3 * prim);
+    return mat_color(s, m, ORC_SLOT_DIFFUSE, m->diffuse, g);
 }
 
 /* math_utils.h:91-95 (the coordinateSystem frame, not normalised) */
@@ -1269,7 +1297,7 @@ static double kelemen_dalbedo(const float *tab, float costheta, float roughness)
     return (ORC_KELEMEN_ROUGH - 1) * (b - a);
 }
 
-typedef struct { double *albedo, *rough, *spec; } GradAcc;
+typedef struct { double *albedo, *rough, *spec; float *ptex; } GradAcc;
 
 /* One replay.  grad == 0: returns radiance in L (unclamped) and the clamp
  * step count.  grad == 1: accumulates gradients for incoming delta. */
@@ -1309,7 +1337,11 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
          * (roughnessA = roughnessB; metalMaterial.cpp:34-62 picks the lobe by
          * u against mix, independent of roughness) */
         int rlobe = spec && (kel || m->type == ORC_MAT_METAL);
-        int alb = b.kind == ORC_BSDF_LAMBERT && s->tri_albedo != NULL;
+        /* the Lambert lobe's colour is a parameter: an albedo-texture texel
+         * (gradients per texel) or the per-triangle override */
+        long long pti = b.kind == ORC_BSDF_LAMBERT ? ptex_index(s, attr, &hg) : -1;
+        int alb = b.kind == ORC_BSDF_LAMBERT && (pti >= 0 ? g->ptex != NULL : s->tri_albedo != NULL);
+        const float *a_val = pti >= 0 ? s->ptex + 3 * pti : (s->tri_albedo ? s->tri_albedo + 3 * prim : NULL);
         double drough = matte_path && (m->type == ORC_MAT_KELEMEN || m->type == ORC_MAT_METAL) ? (1. - CAUSTIC_BLUR) : 1.;
         if (grad && kel && m->reflectance < 1.f) {
             float c = fclampf(dot3(Nn, vn), 0.f, 1.f);
@@ -1363,8 +1395,14 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
             for (int k = 0; k < 3; ++k) { cl[k] *= n * Tv[k]; cb[k] *= n * Tv[k]; L[k] += cl[k] + cb[k]; }
             if (grad) {
                 if (alb) {
-                    const float *a = s->tri_albedo + 3 * prim;
-                    for (int k = 0; k < 3; ++k) g->albedo[3 * prim + k] += delta[k] * cl[k] / a[k];
+                    for (int k = 0; k < 3; ++k) {
+                        if (pti >= 0) {
+#pragma omp atomic
+                            g->ptex[3 * pti + k] += (float)(delta[k] * cl[k] / a_val[k]);
+                        } else {
+                            g->albedo[3 * prim + k] += delta[k] * cl[k] / a_val[k];
+                        }
+                    }
                 }
                 if (rlobe) {
                     double dc = 0, dlr = 0;
@@ -1389,8 +1427,14 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
         if (grad) {
             double dr = delta[0] * R[0] + delta[1] * R[1] + delta[2] * R[2];
             if (alb) {
-                const float *a = s->tri_albedo + 3 * prim;
-                for (int k = 0; k < 3; ++k) g->albedo[3 * prim + k] += delta[k] * R[k] / a[k];
+                for (int k = 0; k < 3; ++k) {
+                    if (pti >= 0) {
+#pragma omp atomic
+                        g->ptex[3 * pti + k] += (float)(delta[k] * R[k] / a_val[k]);
+                    } else {
+                        g->albedo[3 * prim + k] += delta[k] * R[k] / a_val[k];
+                    }
+                }
             }
             if (rlobe) {
                 if (kel) g->spec[attr] += dr / m->spec[0];
@@ -1415,7 +1459,8 @@ static int path_replay(const orc_scene *s, const orc_bvh *bv, int x, int y, cons
 }
 
 void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_pass, int n_passes, const float *dl_dimage,
-                 const float *albedo_tri, float *g_albedo_tri, float *g_rough_mat, float *g_spec_mat, int n_threads) {
+                 const float *albedo_tri, float *g_albedo_tri, float *g_rough_mat, float *g_spec_mat, float *g_ptex,
+                 int n_threads) {
     (void)albedo_tri;
     int wh = s->width * s->height;
     long long n = (long long)wh * n_passes;
@@ -1437,6 +1482,7 @@ void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_
         g.albedo = base;
         g.rough = base + 3 * (size_t)s->n_tris;
         g.spec = g.rough + s->n_attrs;
+        g.ptex = g_ptex;  /* summed with atomics (the texture can be large) */
 #pragma omp for schedule(dynamic, 256)
         for (long long i = 0; i < n; ++i) {
             int pixel = (int)(i % wh), sample = first_pass + (int)(i / wh);

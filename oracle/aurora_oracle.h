@@ -113,6 +113,11 @@ typedef struct {
     const orc_texture *textures;
     int sampler;           /* 0: hash of (seed, pixel, sample, slot); 1: Halton (oracle_halton.c) */
     int lightsamples;      /* paths per camera sample after pass 0 */
+    /* optional FP32 RGB albedo texture (row 0 at the top, 3 floats a texel)
+     * replacing the diffuse colour of the attributes flagged in ptex_mat */
+    const float *ptex;
+    int ptex_w, ptex_h;
+    const int *ptex_mat;   /* n_attrs flags, or NULL */
 } orc_scene;
 
 typedef struct {
@@ -206,7 +211,7 @@ float orc_halton_slot(unsigned w, unsigned h, unsigned i, unsigned x, unsigned y
 /* ---- adjoint (path-replay) ---- */
 void orc_adjoint(const orc_scene *s, const orc_bvh *b, uint32_t seed, int first_pass, int n_passes,
                  const float *dl_dimage, const float *albedo_tri, float *g_albedo_tri,
-                 float *g_rough_mat, float *g_spec_mat, int n_threads);
+                 float *g_rough_mat, float *g_spec_mat, float *g_ptex, int n_threads);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,9 @@ class OScene(C.Structure):
                 ("n_lights", C.c_int), ("lights", C.POINTER(OLight)), ("fov", C.c_float), ("width", C.c_int),
                 ("height", C.c_int), ("maxdepth", C.c_int), ("mindepth", C.c_int), ("uvs", C.POINTER(C.c_float)),
                 ("n_shaders", C.c_int), ("shaders", C.POINTER(OShader)), ("n_textures", C.c_int),
-                ("textures", C.POINTER(OTexture)), ("sampler", C.c_int), ("lightsamples", C.c_int)]
+                ("textures", C.POINTER(OTexture)), ("sampler", C.c_int), ("lightsamples", C.c_int),
+                ("ptex", C.POINTER(C.c_float)), ("ptex_w", C.c_int), ("ptex_h", C.c_int),
+                ("ptex_mat", C.POINTER(C.c_int))]
 
 
 class ORay(C.Structure):
@@ -126,7 +128,7 @@ def lib() -> C.CDLL:
                                        C.POINTER(ORay), _FP, _FP]),
         "orc_light_pdf_in": (C.c_float, [C.POINTER(OScene), C.POINTER(OLight), C.POINTER(ORay)]),
         "orc_light_visible": (C.c_int, [C.POINTER(OLight), _FP, _FP]),
-        "orc_adjoint": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, C.c_int, C.c_int, _FP, _FP, _FP, _FP, _FP,
+        "orc_adjoint": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, C.c_int, C.c_int, _FP, _FP, _FP, _FP, _FP, _FP,
                                C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -172,7 +174,10 @@ def scene_kelemen_table(reflectance: float, distribution: int) -> np.ndarray:
 class Scene:
     """An oracle scene built from the renderer's flattened export."""
 
-    def __init__(self, exp: dict, tri_albedo=None, bvh: bool = False, leaf: int = 4):
+    def __init__(self, exp: dict, tri_albedo=None, bvh: bool = False, leaf: int = 4, ptex=None, ptex_attrs=()):
+        """ptex: an (h, w, 3) FP32 albedo texture (row 0 at the top) replacing
+        the diffuse colour of the attributes in ptex_attrs, read at the hit's
+        uv as aur_set_albedo_texture's texture is."""
         self.exp = exp
         self.verts = np.ascontiguousarray(exp["verts"], np.float32).reshape(-1)
         self.tris = np.ascontiguousarray(exp["tris"], np.int32).reshape(-1)
@@ -257,6 +262,14 @@ class Scene:
         s.n_textures, s.textures = len(texs), C.cast(self.textures, C.POINTER(OTexture))
         s.sampler = 1 if exp.get("sampler") == "halton" else 0
         s.lightsamples = int(exp.get("lightsamples", 1))
+        self.ptex = None
+        if ptex is not None:
+            self.ptex = np.ascontiguousarray(ptex, np.float32)
+            assert self.ptex.ndim == 3 and self.ptex.shape[2] == 3
+            self.ptex_mat = np.zeros(max(len(mats), 1), np.int32)
+            self.ptex_mat[list(ptex_attrs)] = 1
+            s.ptex, s.ptex_h, s.ptex_w = fp(self.ptex), self.ptex.shape[0], self.ptex.shape[1]
+            s.ptex_mat = ip(self.ptex_mat)
         self.s = s
         self.bvh = lib().orc_bvh_build(C.byref(s), leaf) if bvh else None
 
@@ -333,16 +346,20 @@ class Scene:
                 o.spec[0] = o.spec[1] = o.spec[2] = w
 
     def adjoint(self, seed: int, first_pass: int, n_passes: int, dldi, threads: int = 0):
-        """CPU reference gradient: (albedo grads (n_tris, 3) or None, roughness grads, spec grads)."""
+        """CPU reference gradient: (albedo grads (n_tris, 3) or None, roughness grads, spec grads), and the
+        albedo texture's gradient (h, w, 3) as a fourth item when the scene has one."""
         dl = np.ascontiguousarray(dldi, np.float32).reshape(-1)
         nt = self.tris.size // 3
         nm = len(self.exp["materials"])
         ga = np.zeros(3 * nt, np.float32)
         gr = np.zeros(nm, np.float32)
         gs = np.zeros(nm, np.float32)
+        gt = np.zeros(self.ptex.shape, np.float32) if self.ptex is not None else None
         lib().orc_adjoint(C.byref(self.s), self.bvh, seed, first_pass, n_passes, fp(dl),
-                          fp(self.albedo) if self.albedo is not None else None, fp(ga), fp(gr), fp(gs), threads)
-        return (ga.reshape(-1, 3) if self.albedo is not None else None), gr, gs
+                          fp(self.albedo) if self.albedo is not None else None, fp(ga), fp(gr), fp(gs),
+                          fp(gt) if gt is not None else None, threads)
+        out = ((ga.reshape(-1, 3) if self.albedo is not None else None), gr, gs)
+        return out + (gt,) if gt is not None else out
 
     def render_paths(self, seed: int, pixel, sample, slots=None, threads: int = 0):
         pixel = np.ascontiguousarray(pixel, np.int32)

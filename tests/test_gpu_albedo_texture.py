@@ -155,3 +155,37 @@ def gpu_box_room_kelemen(tmp_path_factory):
     r.build()
     yield r
     r.close()
+
+
+def test_albedo_texture_paths_and_gradient_match_oracle(small_large_scene, oracle):
+    """The oracle restates the FP32 albedo-texture read (texture2DShader
+    mapping, nearest texel, no FP16 rounding): per-path radiance of the GPU
+    and the oracle agree under the same random numbers with a texture whose
+    texels are not FP16 values, and so do the texel gradients of
+    L = sum w . I."""
+    r, spec, truth = small_large_scene
+    attrs = [spec.attr_index("terrain00"), spec.attr_index("terrain10")]
+    rng = np.random.default_rng(12)
+    tex = np.clip(truth + rng.uniform(-1e-3, 1e-3, truth.shape), 0.05, 0.95).astype(np.float32)
+    assert not np.array_equal(tex, tex.astype(np.float16).astype(np.float32))
+    r.set_albedo_texture(tex, attrs)
+    e = r.export()
+    s = oracle.Scene(e, bvh=True, ptex=tex, ptex_attrs=attrs)
+    W, H = e["width"], e["height"]
+    pix = rng.integers(0, W * H, 6000).astype(np.int32)
+    smp = rng.integers(0, 16, 6000).astype(np.int32)
+    r.set_seed(9)
+    g, gst = r.render_paths(pix, smp)
+    o, ost = s.render_paths(9, pix, smp)
+    rel = np.abs(g - o) / (np.abs(o) + 1e-3)
+    assert np.mean(rel > 1e-4) < 2e-3, np.mean(rel > 1e-4)
+    assert abs(gst["shadow_rays"] - ost["shadow_rays"]) <= max(2, 1e-3 * ost["shadow_rays"])
+    w = rng.uniform(-1, 1, size=(H, W, 3)).astype(np.float32)
+    r.set_seed(15)
+    _, _, _, _, gt = r.adjoint(0, 2, w, albedo=False, texture=True)
+    dl = np.zeros((H, W, 4), np.float32)
+    dl[..., :3] = w
+    ot = s.adjoint(15, 0, 2, dl, threads=16)[3]
+    assert np.count_nonzero(ot) > 50
+    np.testing.assert_allclose(gt, ot, rtol=2e-3, atol=2e-4 * np.abs(ot).max())
+    r.set_albedo_texture(None)

@@ -163,3 +163,54 @@ def test_batch_counts_shadow_rays_of_deep_paths(tmp_path, oracle):
     assert np.array_equal(r.framebuffer(), acc)
     ex.close()
     r.close()
+
+
+def test_batched_workload_seed3_full(tmp_path, oracle):
+    """The batched workload as benchmarked: the 64 seed-3 jobs (996..327,728
+    triangles, 128^2..1024^2, 4..256 spp, depth 3..12) through one batched
+    launch with the tuned launch shape.  Every job's framebuffer and ray counts
+    equal its own single-scene render bit for bit, and sampled paths of three
+    jobs (the smallest and largest image, the deepest paths) equal the
+    oracle's BVH render."""
+    from aurora_amd import scenes, tuning
+    from aurora_amd.renderer import Renderer, batch_submit
+    paths = scenes.batch_jobs(str(tmp_path), n_jobs=64, seed=3)
+    cfg = tuning.choose_batch()
+    rs, info = [], []
+    for k, p in enumerate(paths):
+        r = Renderer(p, os.path.join(os.path.dirname(p), "config.json"))
+        r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
+        r.set_seed(1000 + k)
+        rs.append(r)
+        info.append(r.info())
+    res_set = {(i["width"], i["height"]) for i in info}
+    depths = {i["maxdepth"] for i in info}
+    assert len(res_set) == 4 and min(depths) <= 4 and max(depths) >= 11
+    ex = Renderer()
+    tuning.apply_launch(ex, cfg)
+    print("batch: 64 jobs", flush=True)
+    res, tot = batch_submit(ex, [(r, 0, i["pixelsamples"]) for r, i in zip(rs, info)])
+    fbs = [r.framebuffer() for r in rs]
+    assert tot["paths"] == sum(i["width"] * i["height"] * i["pixelsamples"] for i in info)
+    print("batch: per-scene renders", flush=True)
+    for k, (r, i) in enumerate(zip(rs, info)):
+        tuning.apply_launch(r, cfg)
+        acc, st = r.render_passes(0, i["pixelsamples"])
+        assert np.array_equal(fbs[k], acc), k
+        for key in ("paths", "ext_rays", "hit_rays", "shadow_rays"):
+            assert res[k][key] == st[key], (k, key)
+    pick = {int(np.argmin([i["width"] for i in info])), int(np.argmax([i["width"] for i in info])),
+            int(np.argmax([i["maxdepth"] for i in info]))}
+    rng = np.random.default_rng(3)
+    for k in sorted(pick):
+        i = info[k]
+        s = oracle.Scene(rs[k].export(), bvh=True)
+        pix = rng.integers(0, i["width"] * i["height"], 2000).astype(np.int32)
+        smp = rng.integers(0, i["pixelsamples"], 2000).astype(np.int32)
+        g, _ = rs[k].render_paths(pix, smp)
+        o, _ = s.render_paths(1000 + k, pix, smp, threads=16)
+        rel = np.abs(g - o) / (np.abs(o) + 1e-3)
+        assert np.mean(rel > 1e-4) < 5e-3, (k, np.mean(rel > 1e-4))
+    for r in rs:
+        r.close()
+    ex.close()

@@ -366,3 +366,64 @@ def test_oracle_renders_textured_stage(stage, oracle):
     assert np.all(np.isfinite(acc))
     assert st["paths"] == e["width"] * e["height"]
     assert np.all(acc[..., 3] == 1.0)  # every camera ray sees geometry or the environment
+
+
+# ---------------------------------------------------------------------------
+# the FP32 albedo texture (aur_set_albedo_texture's restatement in the oracle)
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def terrain(tmp_path_factory):
+    """Two terrain patches of the large differentiable scene with its uv
+    texture shader (no spheres: a host-only context flattens it)."""
+    from aurora_amd import scenes
+    d = str(tmp_path_factory.mktemp("terrain"))
+    spec, truth = scenes.large_scene(width=48, height=27, spp=2, maxdepth=3, mindepth=3, n_spheres=0, levels=0,
+                                     patches=(2, 1), patch_quads=12, patch_size=6.0, tex_res=16, seed=7)
+    path = spec.write(d)
+    r = host_renderer()
+    r.load(path, os.path.join(d, "config.json"))
+    e = r.export()
+    r.close()
+    attrs = [spec.attr_index("terrain00"), spec.attr_index("terrain10")]
+    return e, truth.astype(np.float16).astype(np.float32), attrs
+
+
+def test_albedo_texture_reads_what_the_texture_shader_reads(oracle, terrain):
+    """With texel values FP16 represents exactly, the FP32 albedo texture and
+    the scene's own texture2DShader (texture2DShader.h:28-31 over Texture2D's
+    FP16 texels) give the same image bit for bit; a scaled texture darkens it."""
+    e, truth, attrs = terrain
+    ref, rst = oracle.Scene(e).render(11, 0, 2, threads=8)
+    got, gst = oracle.Scene(e, ptex=truth, ptex_attrs=attrs).render(11, 0, 2, threads=8)
+    assert ref[..., :3].max() > 0 and np.array_equal(got, ref) and rst == gst
+    half, _ = oracle.Scene(e, ptex=truth * 0.5, ptex_attrs=attrs).render(11, 0, 2, threads=8)
+    assert np.mean(half[..., :3] < ref[..., :3]) > 0.3
+
+
+def test_albedo_texture_gradient_matches_central_differences(oracle, terrain):
+    """The image is a polynomial in the texels with fixed random numbers and
+    the largest texel gradients of L = sum w . I match central differences
+    (h = 1e-3) to 1e-2 relative."""
+    e, truth, attrs = terrain
+    rng = np.random.default_rng(2)
+    H, W = e["height"], e["width"]
+    w = rng.uniform(-1, 1, size=(H, W, 3))
+    dl = np.zeros((H, W, 4), np.float32)
+    dl[..., :3] = w
+    s = oracle.Scene(e, ptex=truth, ptex_attrs=attrs)
+    out = s.adjoint(13, 0, 4, dl, threads=8)
+    gt = out[3]
+    assert gt.shape == truth.shape and np.count_nonzero(gt) > 20
+
+    def loss(tex):
+        acc, _ = oracle.Scene(e, ptex=tex, ptex_attrs=attrs).render(13, 0, 4, threads=8)
+        return float(np.sum((acc[..., :3] / 4).astype(np.float64) * w))
+
+    h = 1e-3
+    for idx in np.argsort(-np.abs(gt.ravel()))[:6]:
+        y, x, c = np.unravel_index(int(idx), gt.shape)
+        tp, tm = truth.copy(), truth.copy()
+        tp[y, x, c] += h
+        tm[y, x, c] -= h
+        fd = (loss(tp) - loss(tm)) / (2 * h)
+        assert gt[y, x, c] == pytest.approx(fd, rel=1e-2, abs=1e-3 * np.abs(gt).max()), (y, x, c)
