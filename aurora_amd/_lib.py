@@ -130,6 +130,17 @@ class BsdfResult(C.Structure):
                 ("sample_color", C.c_float * 3)]
 
 
+class LightQuery(C.Structure):
+    _fields_ = [("light", C.c_int), ("P", C.c_float * 3), ("N", C.c_float * 3), ("u1", C.c_float),
+                ("u2", C.c_float), ("D", C.c_float * 3), ("tmax", C.c_float)]
+
+
+class LightResult(C.Structure):
+    _fields_ = [("sample_dir", C.c_float * 3), ("sample_tmax", C.c_float), ("sample_pdf", C.c_float),
+                ("sample_color", C.c_float * 3), ("Le", C.c_float * 3), ("pdf", C.c_float), ("pdf_tmax", C.c_float),
+                ("visible", C.c_int)]
+
+
 class Job(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("first_pass", C.c_int), ("n_passes", C.c_int)]
 
@@ -189,6 +200,7 @@ SIGNATURES = {
     "aur_trace": (C.c_int, [_P, C.POINTER(Ray), C.c_int, C.POINTER(Hit), C.c_int]),
     "aur_trace_noalloc": (C.c_int, [_P, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "aur_bsdf": (C.c_int, [_P, C.POINTER(BsdfQuery), C.c_int, C.POINTER(BsdfResult)]),
+    "aur_light_queries": (C.c_int, [_P, C.POINTER(LightQuery), C.c_int, C.POINTER(LightResult)]),
     "aur_kelemen_table": (C.c_int, [_P, C.c_float, C.c_int, C.c_int, C.c_uint32, _FP]),
     "aur_loop_subdivide": (C.c_int, [_P, C.c_int, C.c_int, _FP, _IP, C.c_int, C.c_float, C.c_float, C.c_int,
                                      C.c_uint32, _FP, _IP, _FP, C.c_int, C.c_int, _IP, _IP]),

@@ -1208,6 +1208,27 @@ int aur_trace_noalloc(aur_ctx *ctx, const aur_ray *d_rays, int n, aur_hit *d_hit
     });
 }
 
+int aur_light_queries(aur_ctx *ctx, const aur_light_query *q, int n, aur_light_result *r) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (n < 0) throw std::invalid_argument("negative query count");
+        if (n == 0) return;
+        NEED(q); NEED(r);
+        if (!c->built) throw StateError("light queries before aur_build_accel");
+        for (int i = 0; i < n; ++i)
+            if (q[i].light < 0 || q[i].light >= c->n_lights) throw std::invalid_argument("light index out of range");
+        set_device(c);
+        DevBuf dq, drs;
+        dq.reserve(sizeof(aur_light_query) * n);
+        drs.reserve(sizeof(aur_light_result) * n);
+        HIP_CHECK(hipMemcpyAsync(dq.p, q, sizeof(aur_light_query) * n, hipMemcpyHostToDevice, c->stream));
+        light_queries(c, dq.as<aur_light_query>(), n, drs.as<aur_light_result>());
+        HIP_CHECK(hipMemcpyAsync(r, drs.p, sizeof(aur_light_result) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
 int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r) {
     return guarded([&] {
         NEED(ctx);

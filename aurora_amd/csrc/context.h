@@ -248,6 +248,7 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
              float *d_gptex, aur_stats *stats);
 // parity.hip
 void bsdf_queries(Ctx *c, const aur_bsdf_query *d_q, int n, aur_bsdf_result *d_r);
+void light_queries(Ctx *c, const aur_light_query *d_q, int n, aur_light_result *d_r);
 // the Monte-Carlo estimate of the table; half_store rounds each entry to FP16
 void kelemen_table(Ctx *c, float reflectance, int dist, int n_samples, uint32_t seed, float *d_out,
                    bool half_store = false);

@@ -1,5 +1,5 @@
-// parity.hip -- batched BSDF queries and the Kelemen albedo-table kernel.
-#include "context.h"
+// parity.hip -- batched BSDF and light queries and the Kelemen albedo-table kernel.
+#include "shade_common.h"
 
 namespace aur {
 
@@ -27,6 +27,37 @@ __global__ void k_bsdf(const aur_bsdf_query *__restrict__ q, int n, aur_bsdf_res
     o.sample_pdf = s.pdf;
     o.sample_color[0] = s.color.x; o.sample_color[1] = s.color.y; o.sample_color[2] = s.color.z;
     r[i] = o;
+}
+
+// One query per lane on the scene's light q.light: generateSample at (P, N,
+// u1, u2), eval and pdf along D (squareLight.h:57-122 and the environment
+// light of infiniteAreaLight.cpp:35-106, maps included), visible(P, N).
+__global__ void k_light(SceneView sv, const aur_light_query *__restrict__ q, int n, aur_light_result *__restrict__ r) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const aur_light_query a = q[i];
+    const DLight &l = sv.lights[a.light];
+    const f3 P = mk3(a.P[0], a.P[1], a.P[2]), N = mk3(a.N[0], a.N[1], a.N[2]), D = mk3(a.D[0], a.D[1], a.D[2]);
+    const LightSample s = light_sample_sv(sv, l, P, N, a.u1, a.u2);
+    const f3 le = light_le_sv(sv, l, D);
+    float tmax = a.tmax;
+    const float pdf = light_pdf_sv(sv, l, P, D, AUR_RAY_BIAS, &tmax);
+    aur_light_result o;
+    o.sample_dir[0] = s.dir.x; o.sample_dir[1] = s.dir.y; o.sample_dir[2] = s.dir.z;
+    o.sample_tmax = s.tmax;
+    o.sample_pdf = s.pdf;
+    o.sample_color[0] = s.color.x; o.sample_color[1] = s.color.y; o.sample_color[2] = s.color.z;
+    o.Le[0] = le.x; o.Le[1] = le.y; o.Le[2] = le.z;
+    o.pdf = pdf;
+    o.pdf_tmax = tmax;
+    o.visible = light_visible(l, P, N) ? 1 : 0;
+    r[i] = o;
+}
+
+void light_queries(Ctx *c, const aur_light_query *d_q, int n, aur_light_result *d_r) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_light, dim3((n + 255) / 256), dim3(256), 0, c->stream, scene_view(c), d_q, n, d_r);
+    HIP_CHECK(hipGetLastError());
 }
 
 void bsdf_queries(Ctx *c, const aur_bsdf_query *d_q, int n, aur_bsdf_result *d_r) {

@@ -282,6 +282,13 @@ class Renderer:
                                    hits.ctypes.data_as(C.POINTER(L.Hit)), mode))
         return hits
 
+    def light(self, queries) -> list:
+        """Light::generateSample / eval / pdf / visible per aur_light_query."""
+        arr = (L.LightQuery * len(queries))(*queries)
+        res = (L.LightResult * len(queries))()
+        L.check(self.lib.aur_light_queries(self._ctx, arr, len(queries), res))
+        return [res[k] for k in range(len(queries))]
+
     def bsdf(self, queries) -> list:
         arr = (L.BsdfQuery * len(queries))(*queries)
         res = (L.BsdfResult * len(queries))()

@@ -190,6 +190,29 @@ typedef struct {
     float u1, u2;
 } aur_bsdf_query;
 
+/* One light query against light `light` of the built scene (the lights in
+ * scene order, as aur_scene_export returns them): the calls the integrator
+ * makes on a Light (light.h:23-81; squareLight.h:57-122,
+ * infiniteAreaLight.cpp:35-106). */
+typedef struct {
+    int light;
+    float P[3], N[3];  /* shading point and normal: generateSample, visible */
+    float u1, u2;      /* generateSample's two uniforms */
+    float D[3];        /* a unit direction from P: eval and pdf */
+    float tmax;        /* pdf: the ray's maxt before the call (a quad hit shortens it) */
+} aur_light_query;
+
+typedef struct {
+    float sample_dir[3];   /* generateSample: unit direction to the sampled point */
+    float sample_tmax;     /* and the shadow ray's maxt */
+    float sample_pdf;      /* solid-angle pdf (0 where a quad faces away) */
+    float sample_color[3]; /* radiance of the sample */
+    float Le[3];           /* eval(D) */
+    float pdf;             /* pdf of the ray (P, D, [RAY_BIAS, tmax]) */
+    float pdf_tmax;        /* the ray's maxt after pdf() */
+    int visible;           /* visible(P, N) */
+} aur_light_result;
+
 typedef struct {
     float f[3];           /* evalSampleWorld(L, V, N) */
     float pdf;            /* pdf(L, V, N) */
@@ -382,6 +405,8 @@ int aur_trace(aur_ctx *ctx, const aur_ray *rays, int n, aur_hit *hits, int mode)
  * 128 overflow entries per ray fails with AUR_ECAPACITY. */
 int aur_trace_noalloc(aur_ctx *ctx, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode);
 int aur_bsdf(aur_ctx *ctx, const aur_bsdf_query *q, int n, aur_bsdf_result *r);
+/* Light::generateSample / eval / pdf / visible per query (light.h:23-81). */
+int aur_light_queries(aur_ctx *ctx, const aur_light_query *q, int n, aur_light_result *r);
 /* Kelemen albedo table (64 cos x 32 roughness), kelemenMaterial.cpp:113-178. */
 int aur_kelemen_table(aur_ctx *ctx, float reflectance, int distribution, int n_samples,
                       uint32_t seed, float *out);
