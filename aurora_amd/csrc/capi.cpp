@@ -412,7 +412,7 @@ static void flatten(Ctx *c) {
             for (size_t i = 0; i < r.tris.size(); ++i) s.tris.push_back(base + r.tris[i]);
             s.normals.insert(s.normals.end(), r.corner_normals.begin(), r.corner_normals.end());
             for (size_t i = 0; i < r.tris.size() / 3; ++i) s.attr.push_back(o.attr);
-            s.uvs.resize(s.uvs.size() + 2 * r.tris.size(), 0.f);  // subdivided uvs are not carried (zero)
+            s.uvs.insert(s.uvs.end(), r.corner_uvs.begin(), r.corner_uvs.end());
             continue;
         }
         // TriangleMesh::makeEmbree, triangleMesh.cpp:57-81

@@ -686,16 +686,6 @@ void load_scene(const std::string &scene_path, const std::string &config_path, H
         for (size_t i = 0; i < n; ++i)
             if (err[i]) std::rethrow_exception(err[i]);
     }
-    for (const HostObject &obj : s.objects) {
-        const aur_material &mm = mats[obj.material];
-        if ((obj.subdlevels > 0 || obj.displace.on) && !obj.UV.empty())
-            for (int k = 0; k < AUR_N_SLOTS; ++k) {
-                int sh = mm.shader[k];
-                if (sh >= 0 && s.shaders[sh].type == AUR_SHADER_TEXTURE2D)
-                    throw std::runtime_error("object " + obj.name +
-                                             ": uv-mapped textures on subdivided meshes are not supported");
-            }
-    }
     // attributes: objects first, then lights (renderer.cpp:118-134)
     for (size_t i = 0; i < s.objects.size(); ++i) {
         s.objects[i].attr = (int)i;

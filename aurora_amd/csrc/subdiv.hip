@@ -143,6 +143,89 @@ __global__ void k_odd(MeshDev m, int nv, int ne, float *__restrict__ Pn, int2 *_
     VSn[nv + e] = make_int2(4 * r.z + 3, l.x);
 }
 
+// ---------------------------------------------------------------------------
+// texture coordinates per face corner (loopSubdivMesh.cpp:174-191, 360-413,
+// 565-613, 639-692, 477-482): a corner's uv belongs to the face, so a seam
+// keeps different uvs on its two sides.  One thread per parent corner (f, j):
+//   even child corner: the one-ring rule with the position's weights, unless
+//     the ring holds a seam (a face's uv at v differs from this corner's, or
+//     the two faces of a ring edge disagree on the neighbour's uv): then the
+//     corner keeps its uv;
+//   odd vertex of edge (v_j, v_j+1): 3/8, 3/8, 1/8, 1/8 of this face's uvs
+//     and the other face's opposite uv when both faces agree on the edge's
+//     two uvs, else (and on a boundary) the midpoint of this face's two;
+// uvs are not pushed to the limit surface.
+// ---------------------------------------------------------------------------
+__device__ inline float2 corner_uv(const MeshDev &m, const float2 *UV, int f, int v) {
+    return UV[3 * f + local_of(m, f, v)];
+}
+__device__ inline bool same_uv(float2 a, float2 b) { return a.x == b.x && a.y == b.y; }
+
+__global__ void k_uv(MeshDev m, int nf, const float2 *__restrict__ UV, float2 *__restrict__ UVn) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nf) return;
+    const int f = t / 3, j = t - 3 * f;
+    const int v = m.F[3 * f + j];
+    const float2 own = UV[3 * f + j];
+    // even child corner: child j, slot j
+    int ring[AUR_MAX_RING], faces[AUR_MAX_RING];
+    bool boundary;
+    const int n = one_ring(m, v, ring, faces, &boundary);
+    float2 ev = own;
+    if (n > 0) {
+        const int nfv = boundary ? n - 1 : n;
+        bool split = false;
+        for (int k = 0; k < nfv; ++k) {
+            if (!same_uv(corner_uv(m, UV, faces[k], v), own)) split = true;
+            // the edge (v, w) shared by faces k and k+1
+            if (boundary) {
+                if (k + 1 < nfv && !same_uv(corner_uv(m, UV, faces[k], ring[k + 1]),
+                                            corner_uv(m, UV, faces[k + 1], ring[k + 1])))
+                    split = true;
+            } else {
+                const int k1 = (k + 1) % n;
+                if (!same_uv(corner_uv(m, UV, faces[k], ring[k]), corner_uv(m, UV, faces[k1], ring[k])))
+                    split = true;
+            }
+        }
+        if (!split) {
+            if (boundary) {
+                const float beta = 1.f / 8.f;
+                const float2 r0 = corner_uv(m, UV, faces[0], ring[0]);
+                const float2 r1 = corner_uv(m, UV, faces[nfv - 1], ring[n - 1]);
+                ev.x = (1.f - 2.f * beta) * own.x + beta * (r0.x + r1.x);
+                ev.y = (1.f - 2.f * beta) * own.y + beta * (r0.y + r1.y);
+            } else {
+                const float beta = (n == 6) ? 1.f / 16.f : loop_beta(n);
+                ev.x = (1.f - n * beta) * own.x;
+                ev.y = (1.f - n * beta) * own.y;
+                for (int k = 0; k < n; ++k) {
+                    const float2 r = corner_uv(m, UV, faces[k], ring[k]);
+                    ev.x += beta * r.x;
+                    ev.y += beta * r.y;
+                }
+            }
+        }
+    }
+    UVn[3 * (4 * f + j) + j] = ev;
+    // odd vertex on the edge (v_j, v_j+1), seen from this face
+    const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    const int b = m.F[3 * f + j1];
+    const float2 ua = own, ub = UV[3 * f + j1];
+    const int g = other_face(m, m.FE[3 * f + j], f);
+    float2 od = make_float2(0.5f * (ua.x + ub.x), 0.5f * (ua.y + ub.y));
+    if (g >= 0 && same_uv(corner_uv(m, UV, g, v), ua) && same_uv(corner_uv(m, UV, g, b), ub)) {
+        const float2 uc = UV[3 * f + j2];
+        const int lg = local_of(m, g, v), lb = local_of(m, g, b);
+        const float2 ud = UV[3 * g + (3 - lg - lb)];
+        od.x = 3.f / 8.f * (ua.x + ub.x) + 1.f / 8.f * (uc.x + ud.x);
+        od.y = 3.f / 8.f * (ua.y + ub.y) + 1.f / 8.f * (uc.y + ud.y);
+    }
+    UVn[3 * (4 * f + j) + j1] = od;
+    UVn[3 * (4 * f + j1) + j] = od;
+    UVn[3 * (4 * f + 3) + j] = od;
+}
+
 __device__ inline int half_id(const MeshDev &m, int e, int at_vertex) {
     return 2 * e + (m.E[e].x == at_vertex ? 0 : 1);
 }
@@ -334,7 +417,8 @@ static void h2d(Ctx *c, DevBuf &d, const std::vector<T> &h) {
 
 static inline unsigned grid_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
-LoopMesh loop_subdivide(Ctx *c, const std::vector<float> &Pin, const std::vector<int> &Fin, int levels, const Displace &disp) {
+LoopMesh loop_subdivide(Ctx *c, const std::vector<float> &Pin, const std::vector<int> &Fin, int levels, const Displace &disp,
+                        const std::vector<float> *UVin) {
     if (levels < 0 || levels > 10) throw std::invalid_argument("subdivision levels must be in 0..10");
     const int nv0 = (int)(Pin.size() / 3);
     // weld by exact position (loopSubdivMesh.cpp:176-193)
@@ -354,14 +438,19 @@ LoopMesh loop_subdivide(Ctx *c, const std::vector<float> &Pin, const std::vector
         }
     }
     int nv = (int)(P.size() / 3);
-    // faces, skipping degenerate ones (loopSubdivMesh.cpp:200-219)
+    // faces, skipping degenerate ones (loopSubdivMesh.cpp:200-219); corner
+    // uvs (6 per input face) follow the faces kept
+    const bool has_uv = UVin != nullptr;
+    if (has_uv && UVin->size() != 2 * Fin.size()) throw std::invalid_argument("corner uvs do not match the faces");
     std::vector<int> F;
+    std::vector<float> FUV;
     for (size_t f = 0; f + 2 < Fin.size(); f += 3) {
         int a = Fin[f], b = Fin[f + 1], d = Fin[f + 2];
         if (a < 0 || b < 0 || d < 0 || a >= nv0 || b >= nv0 || d >= nv0) throw std::invalid_argument("face index out of range");
         a = remap[a]; b = remap[b]; d = remap[d];
         if (a == b || a == d || b == d) continue;
         F.push_back(a); F.push_back(b); F.push_back(d);
+        if (has_uv) FUV.insert(FUV.end(), UVin->begin() + 2 * f, UVin->begin() + 2 * f + 6);
     }
     int nf = (int)(F.size() / 3);
     // edges in face order; neighbour pointers (loopSubdivMesh.cpp:223-253)
@@ -395,6 +484,8 @@ LoopMesh loop_subdivide(Ctx *c, const std::vector<float> &Pin, const std::vector
     for (int v = 0; v < nv; ++v) if (VS[2 * v] < 0) VS[2 * v + 1] = 0;
 
     LevelBufs cur, nxt;
+    DevBuf uv_cur, uv_nxt;
+    if (has_uv) h2d(c, uv_cur, FUV);
     h2d(c, cur.P, P);
     h2d(c, cur.F, F);
     h2d(c, cur.FE, FE);
@@ -418,6 +509,12 @@ LoopMesh loop_subdivide(Ctx *c, const std::vector<float> &Pin, const std::vector
         if (ne) hipLaunchKernelGGL(k_odd, dim3(grid_for(ne)), dim3(256), 0, c->stream, m, nv, ne, nxt.P.as<float>(), nxt.VS.as<int2>());
         if (nf) hipLaunchKernelGGL(k_faces, dim3(grid_for(nf)), dim3(256), 0, c->stream, m, nf, nv, ne, nxt.F.as<int>(), nxt.FE.as<int>());
         if (ne + 3 * nf) hipLaunchKernelGGL(k_edges, dim3(grid_for(ne + 3 * (size_t)nf)), dim3(256), 0, c->stream, m, nf, nv, ne, nxt.E.as<int4>(), nxt.EL.as<int2>());
+        if (has_uv && nf) {
+            uv_nxt.reserve(sizeof(float2) * 3 * nf2);
+            hipLaunchKernelGGL(k_uv, dim3(grid_for(3 * (size_t)nf)), dim3(256), 0, c->stream, m, nf, uv_cur.as<float2>(),
+                               uv_nxt.as<float2>());
+            uv_cur.swap(uv_nxt);
+        }
         HIP_CHECK(hipGetLastError());
         cur.swap(nxt);
         nv = (int)nv2; nf = (int)nf2; ne = (int)ne2;
@@ -451,6 +548,10 @@ LoopMesh loop_subdivide(Ctx *c, const std::vector<float> &Pin, const std::vector
         HIP_CHECK(hipMemcpyAsync(out.N.data(), N.p, sizeof(float) * out.N.size(), hipMemcpyDeviceToHost, c->stream));
     }
     if (nf) HIP_CHECK(hipMemcpyAsync(out.F.data(), cur.F.p, sizeof(int) * out.F.size(), hipMemcpyDeviceToHost, c->stream));
+    if (has_uv) {
+        out.UV.resize(6 * (size_t)nf);
+        if (nf) HIP_CHECK(hipMemcpyAsync(out.UV.data(), uv_cur.p, sizeof(float) * out.UV.size(), hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_CHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     if (herr) throw CapacityError("vertex valence exceeds the one-ring limit of 32");
@@ -461,8 +562,11 @@ SubdivResult subdivide_object(Ctx *c, const HostObject &o) {
     size_t ncorner = o.P.size() / 3;
     std::vector<int> F(ncorner);
     for (size_t i = 0; i < ncorner; ++i) F[i] = (int)i;
-    LoopMesh lm = loop_subdivide(c, o.P, F, o.subdlevels, o.displace);
+    const bool uv = !o.UV.empty();
+    if (uv && o.UV.size() != 2 * ncorner) throw std::invalid_argument("corner uvs do not match the corners");
+    LoopMesh lm = loop_subdivide(c, o.P, F, o.subdlevels, o.displace, uv ? &o.UV : nullptr);
     SubdivResult r;
+    r.corner_uvs = uv ? lm.UV : std::vector<float>(6 * (lm.F.size() / 3), 0.f);
     size_t nv = lm.P.size() / 3, nf = lm.F.size() / 3;
     r.verts.resize(3 * nv);
     r.tris = lm.F;

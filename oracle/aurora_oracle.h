@@ -183,9 +183,13 @@ void orc_render_paths(const orc_scene *s, const orc_bvh *b, uint32_t seed, const
                       int n_threads);
 
 /* ---- Loop subdivision (indexed restatement) ---- */
+/* UV (optional): 6 per input face, the uvs of its corners; UV_out then
+ * receives 6 per output face (not pushed to the limit, loopSubdivMesh.cpp:477-482).
+ * push_limit = 0 (a check of the checker) returns the last level's positions
+ * instead of the limit surface's. */
 int orc_loop_subdivide(int nv, int nf, const float *P, const int *F, int levels,
                        float *P_out, int *F_out, float *N_out, int nv_cap, int nf_cap,
-                       int *nv_out, int *nf_out);
+                       int *nv_out, int *nf_out, const float *UV, float *UV_out, int push_limit);
 
 /* ---- value-noise displacement + area-weighted normals ---- */
 void orc_displace_radial(int nv, float *P, float amplitude, float frequency, int octaves, uint32_t seed);

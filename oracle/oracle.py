@@ -118,7 +118,7 @@ def lib() -> C.CDLL:
         "orc_render_paths": (None, [C.POINTER(OScene), C.c_void_p, C.c_uint32, _IP, _IP, C.c_int, _FP, _FP,
                                     C.POINTER(OCounters), C.c_int]),
         "orc_loop_subdivide": (C.c_int, [C.c_int, C.c_int, _FP, _IP, C.c_int, _FP, _IP, _FP, C.c_int, C.c_int, _IP,
-                                         _IP]),
+                                         _IP, _FP, _FP, C.c_int]),
         "orc_displace_radial": (None, [C.c_int, _FP, C.c_float, C.c_float, C.c_int, C.c_uint32]),
         "orc_vertex_normals": (None, [C.c_int, C.c_int, _FP, _IP, _FP]),
         "orc_camera_ray": (None, [C.POINTER(OScene), C.c_int, C.c_int, C.c_float, C.c_float, C.POINTER(ORay)]),
@@ -393,9 +393,11 @@ def bsdf_sample(kind, color, rough, dist, V, N, u1, u2):
 
 
 def loop_subdivide(P, F, levels: int, amplitude: float = 0.0, frequency: float = 1.0, octaves: int = 0,
-                   seed: int = 0):
+                   seed: int = 0, uv=None, limit: bool = True):
     """Oracle Loop subdivision; with amplitude != 0 the radial displacement and
-    area-weighted normals follow.  Returns (P, F, N)."""
+    area-weighted normals follow.  Returns (P, F, N), and the output faces'
+    corner uvs (nf_out, 3, 2) as a fourth item when uv (nf, 3, 2 corner uvs
+    of the input faces) is given."""
     P = np.ascontiguousarray(P, np.float32).reshape(-1)
     F = np.ascontiguousarray(F, np.int32).reshape(-1)
     nv, nf = P.size // 3, F.size // 3
@@ -405,15 +407,21 @@ def loop_subdivide(P, F, levels: int, amplitude: float = 0.0, frequency: float =
     Fo = np.zeros(3 * cap_f, np.int32)
     No = np.zeros(3 * cap_v, np.float32)
     onv, onf = C.c_int(), C.c_int()
+    uvi = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1)
+    if uvi is not None and uvi.size != 6 * nf:
+        raise ValueError("uv needs 6 values per input face")
+    uvo = np.zeros(6 * cap_f, np.float32) if uvi is not None else None
     rc = lib().orc_loop_subdivide(nv, nf, fp(P), ip(F), levels, fp(Po), ip(Fo), fp(No), cap_v, cap_f, C.byref(onv),
-                                  C.byref(onf))
+                                  C.byref(onf), fp(uvi) if uvi is not None else None,
+                                  fp(uvo) if uvo is not None else None, int(limit))
     if rc != 0:
         raise RuntimeError("oracle loop subdivision failed: %d" % rc)
     Po, Fo, No = Po[:3 * onv.value].copy(), Fo[:3 * onf.value].copy(), No[:3 * onv.value].copy()
     if amplitude != 0.0:
         lib().orc_displace_radial(onv.value, fp(Po), amplitude, frequency, octaves, seed)
         lib().orc_vertex_normals(onv.value, onf.value, fp(Po), ip(Fo), fp(No))
-    return Po.reshape(-1, 3), Fo.reshape(-1, 3), No.reshape(-1, 3)
+    out = (Po.reshape(-1, 3), Fo.reshape(-1, 3), No.reshape(-1, 3))
+    return out + (uvo[:6 * onf.value].reshape(-1, 3, 2),) if uvo is not None else out
 
 
 def make_light(m16, sx, sy, color, exposure=0.0, kind=0) -> OLight:

@@ -17,6 +17,15 @@
  * Then limit positions (gamma = 1/(n + 3/(8 beta)); boundary 1/5, 439-458) and
  * limit normals normalize(cross(T, S)) over the ordered one-ring (496-541).
  *
+ * Texture coordinates ride on face corners (a seam gives the two faces of an
+ * edge different uvs at its ends), per level (174-191, 360-413, 565-613,
+ * 639-692): an even corner takes the one-ring rule of its vertex with the
+ * position's weights, unless uvs disagree around the vertex (two faces'
+ * uvs at it, or two faces' uvs at one ring neighbour): then it keeps its uv;
+ * an odd vertex takes 3/8, 3/8, 1/8, 1/8 when both faces agree on the edge's
+ * end uvs, else each face the midpoint of its own two (boundary: midpoint).
+ * uvs are not pushed to the limit (477-482).
+ *
  * Neighbour sums here run in edge-creation order; the GPU walks rings face by
  * face.  Positions therefore agree to float rounding, not bit for bit.
  */
@@ -187,8 +196,77 @@ static void faces_csr(int nv, int nf, const int *F, int **off_out, int **inc_out
     *inc_out = inc;
 }
 
+/* the uv of vertex v in face f */
+static const float *fuv(const float *UV, const int *F, int f, int v) {
+    int j = F[3 * f] == v ? 0 : (F[3 * f + 1] == v ? 1 : 2);
+    return UV + 6 * f + 2 * j;
+}
+
+/* One level of corner uvs (UV: 6 per face of F) into UV2 (6 per child face). */
+static void uv_level(int nv, int nf, const int *F, int ne, const Edge *E, const int *FE, const int *val, const int *bnd,
+                     const float *UV, float *UV2) {
+    float *first = (float *)malloc(sizeof(float) * 2 * (size_t)(nv > 0 ? nv : 1));
+    float *sum = (float *)calloc(2 * (size_t)(nv > 0 ? nv : 1), sizeof(float));
+    float *bsum = (float *)calloc(2 * (size_t)(nv > 0 ? nv : 1), sizeof(float));
+    int *seen = (int *)calloc((size_t)(nv > 0 ? nv : 1), sizeof(int));
+    int *split = (int *)calloc((size_t)(nv > 0 ? nv : 1), sizeof(int));
+    for (int f = 0; f < nf; ++f)
+        for (int j = 0; j < 3; ++j) {
+            int v = F[3 * f + j];
+            const float *u = UV + 6 * f + 2 * j;
+            if (!seen[v]) { first[2 * v] = u[0]; first[2 * v + 1] = u[1]; seen[v] = 1; }
+            else if (u[0] != first[2 * v] || u[1] != first[2 * v + 1]) split[v] = 1;
+        }
+    for (int e = 0; e < ne; ++e) {
+        int a = E[e].a, b = E[e].b;
+        const float *ua = fuv(UV, F, E[e].f0, a), *ub = fuv(UV, F, E[e].f0, b);
+        if (E[e].f1 >= 0) {
+            const float *ua1 = fuv(UV, F, E[e].f1, a), *ub1 = fuv(UV, F, E[e].f1, b);
+            if (ub1[0] != ub[0] || ub1[1] != ub[1]) split[a] = 1;
+            if (ua1[0] != ua[0] || ua1[1] != ua[1]) split[b] = 1;
+        } else {
+            bsum[2 * a] += ub[0]; bsum[2 * a + 1] += ub[1];
+            bsum[2 * b] += ua[0]; bsum[2 * b + 1] += ua[1];
+        }
+        sum[2 * a] += ub[0]; sum[2 * a + 1] += ub[1];
+        sum[2 * b] += ua[0]; sum[2 * b + 1] += ua[1];
+    }
+    for (int f = 0; f < nf; ++f)
+        for (int j = 0; j < 3; ++j) {
+            int v = F[3 * f + j], j1 = (j + 1) % 3, j2 = (j + 2) % 3, b = F[3 * f + j1];
+            const float *own = UV + 6 * f + 2 * j, *ub = UV + 6 * f + 2 * j1;
+            float ev[2] = {own[0], own[1]};
+            if (val[v] > 0 && !split[v]) {
+                int n = val[v];
+                float beta = bnd[v] ? 1.f / 8.f : (n == 6 ? 1.f / 16.f : loop_beta(n));
+                for (int k = 0; k < 2; ++k)
+                    ev[k] = bnd[v] ? (1.f - 2.f * beta) * own[k] + beta * bsum[2 * v + k]
+                                   : (1.f - n * beta) * own[k] + beta * sum[2 * v + k];
+            }
+            float *ce = UV2 + 6 * (4 * f + j) + 2 * j;
+            ce[0] = ev[0]; ce[1] = ev[1];
+            const Edge *ed = &E[FE[3 * f + j]];
+            int g = ed->f0 == f ? ed->f1 : ed->f0;
+            float od[2] = {0.5f * (own[0] + ub[0]), 0.5f * (own[1] + ub[1])};
+            if (g >= 0) {
+                const float *ga = fuv(UV, F, g, v), *gb = fuv(UV, F, g, b);
+                if (ga[0] == own[0] && ga[1] == own[1] && gb[0] == ub[0] && gb[1] == ub[1]) {
+                    const float *uc = UV + 6 * f + 2 * j2;
+                    int d = F[3 * g] != v && F[3 * g] != b ? F[3 * g] : (F[3 * g + 1] != v && F[3 * g + 1] != b ? F[3 * g + 1] : F[3 * g + 2]);
+                    const float *ud = fuv(UV, F, g, d);
+                    for (int k = 0; k < 2; ++k) od[k] = 3.f / 8.f * (own[k] + ub[k]) + 1.f / 8.f * (uc[k] + ud[k]);
+                }
+            }
+            float *o1 = UV2 + 6 * (4 * f + j) + 2 * j1, *o2 = UV2 + 6 * (4 * f + j1) + 2 * j, *o3 = UV2 + 6 * (4 * f + 3) + 2 * j;
+            o1[0] = o2[0] = o3[0] = od[0];
+            o1[1] = o2[1] = o3[1] = od[1];
+        }
+    free(first); free(sum); free(bsum); free(seen); free(split);
+}
+
 int orc_loop_subdivide(int nv_in, int nf_in, const float *Pin, const int *Fin, int levels, float *P_out, int *F_out,
-                       float *N_out, int nv_cap, int nf_cap, int *nv_out, int *nf_out) {
+                       float *N_out, int nv_cap, int nf_cap, int *nv_out, int *nf_out, const float *UVin, float *UV_out,
+                       int push_limit) {
     /* weld by exact position (loopSubdivMesh.cpp:176-193) */
     int *remap = (int *)malloc(sizeof(int) * (size_t)(nv_in > 0 ? nv_in : 1));
     float *P = (float *)malloc(sizeof(float) * 3 * (size_t)(nv_in > 0 ? nv_in : 1));
@@ -201,11 +279,14 @@ int orc_loop_subdivide(int nv_in, int nf_in, const float *Pin, const int *Fin, i
         remap[i] = found;
     }
     int *F = (int *)malloc(sizeof(int) * 3 * (size_t)(nf_in > 0 ? nf_in : 1));
+    float *UV = UVin ? (float *)malloc(sizeof(float) * 6 * (size_t)(nf_in > 0 ? nf_in : 1)) : NULL;
     int nf = 0;
     for (int f = 0; f < nf_in; ++f) {
         int a = remap[Fin[3 * f]], b = remap[Fin[3 * f + 1]], c = remap[Fin[3 * f + 2]];
         if (a == b || a == c || b == c) continue; /* degenerate: skipped */
-        F[3 * nf] = a; F[3 * nf + 1] = b; F[3 * nf + 2] = c; ++nf;
+        F[3 * nf] = a; F[3 * nf + 1] = b; F[3 * nf + 2] = c;
+        if (UV) memcpy(UV + 6 * nf, UVin + 6 * f, sizeof(float) * 6);
+        ++nf;
     }
     free(remap);
     int rc = 0;
@@ -248,6 +329,12 @@ int orc_loop_subdivide(int nv_in, int nf_in, const float *Pin, const int *Fin, i
             }
             for (int k = 0; k < 3; ++k) F2[3 * (4 * f + 3) + k] = odd[k];
         }
+        if (UV) {
+            float *UV2 = (float *)malloc(sizeof(float) * 6 * (size_t)nf2);
+            uv_level(nv, nf, F, ne, E, FE, val, bnd, UV, UV2);
+            free(UV);
+            UV = UV2;
+        }
         free(E); free(FE); free(val); free(bnd); free(bn); free(sum);
         free(P); free(F);
         P = P2; F = F2; nv = nv2; nf = nf2;
@@ -256,7 +343,7 @@ int orc_loop_subdivide(int nv_in, int nf_in, const float *Pin, const int *Fin, i
     if (rc == 0) {
         /* limit positions and normals */
         float *PL = (float *)malloc(sizeof(float) * 3 * (size_t)(nv > 0 ? nv : 1));
-        if (levels > 0) {
+        if (levels > 0 && push_limit) {
             Edge *E = (Edge *)malloc(sizeof(Edge) * (size_t)3 * (nf > 0 ? nf : 1));
             int *FE = (int *)malloc(sizeof(int) * (size_t)3 * (nf > 0 ? nf : 1));
             int ne = build_edges(nf, F, E, FE);
@@ -309,11 +396,12 @@ int orc_loop_subdivide(int nv_in, int nf_in, const float *Pin, const int *Fin, i
         free(off); free(inc);
         memcpy(P_out, PL, sizeof(float) * 3 * (size_t)nv);
         memcpy(F_out, F, sizeof(int) * 3 * (size_t)nf);
+        if (UV && UV_out) memcpy(UV_out, UV, sizeof(float) * 6 * (size_t)nf);
         free(PL);
     }
     *nv_out = nv;
     *nf_out = nf;
-    free(P); free(F);
+    free(P); free(F); free(UV);
     return rc;
 }
 

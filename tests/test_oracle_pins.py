@@ -661,3 +661,56 @@ def test_scene_kelemen_table_is_the_fp16_250k_estimate(oracle, dist):
     assert np.all(np.abs(t - est) <= 0.5 * ulp + 1e-12)
     small = oracle.kelemen_table(0.05, dist, n_samples=62500, seed=7)
     assert np.max(np.abs(small - est)) < 0.02 and np.mean(np.abs(small - est)) < 2e-3
+
+
+# ---------------------------------------------------------------------------
+# texture coordinates through Loop subdivision (loopSubdivMesh.cpp:174-191,
+# 360-413, 565-613, 639-692, 477-482)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("levels", [1, 2, 3])
+def test_loop_uv_uses_the_position_stencils(oracle, levels):
+    """Seamless uvs equal to each vertex's (x, y) on the icosahedron (valence
+    5 everywhere, every edge interior) come out as the (x, y) of the vertices
+    before the limit step: the even rule with Warren's beta and the odd rule
+    3/8, 3/8, 1/8, 1/8 act on uvs exactly as on positions, and the uvs are not
+    pushed to the limit (with the limit step the positions move away)."""
+    from aurora_amd.scenes import icosahedron
+    v, f = icosahedron()
+    P, F = v.astype(np.float32), f.astype(np.int32)
+    uv = P[F][..., :2].copy()
+    Pp, Fp, _, UVo = oracle.loop_subdivide(P, F, levels, uv=uv, limit=False)
+    np.testing.assert_allclose(UVo, Pp[Fp][..., :2], atol=2e-6)
+    Pl, Fl, _, UVl = oracle.loop_subdivide(P, F, levels, uv=uv)
+    np.testing.assert_array_equal(UVl, UVo)
+    assert np.array_equal(Fl, Fp) and np.abs(Pl - Pp).max() > 1e-3
+
+
+def test_loop_uv_seam_one_level(oracle):
+    """Two triangles (a quad) sharing the edge v0-v2.  Without a seam the
+    shared edge's odd uv is 3/8 (u0 + u2) + 1/8 (u1 + u3) and the corners take
+    the boundary rule 3/4 u + 1/8 (neighbours); with the right triangle's uvs
+    shifted by (10, 0) the shared edge is a seam: each face gets the midpoint
+    of its own end uvs there, and the seam's end vertices keep their uvs."""
+    P = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    F = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    u = {0: np.array([0.0, 0.0]), 1: np.array([1.0, 0.0]), 2: np.array([1.0, 1.0]), 3: np.array([0.0, 1.0])}
+    # face 0 = (v0, v1, v2), face 1 = (v0, v2, v3) in position
+    uv = np.array([[u[0], u[1], u[2]], [u[0], u[2], u[3]]], np.float32)
+    _, Fo, _, UVo = oracle.loop_subdivide(P, F, 1, uv=uv)
+    # child 0 of face 0: (even v0, odd e01, odd e20); child 3: (e01, e12, e20)
+    even0 = 0.75 * u[0] + 0.125 * (u[1] + u[3])
+    odd02 = 3 / 8 * (u[0] + u[2]) + 1 / 8 * (u[1] + u[3])
+    np.testing.assert_allclose(UVo[0, 0], even0, atol=1e-6)
+    np.testing.assert_allclose(UVo[0, 2], odd02, atol=1e-6)
+    np.testing.assert_allclose(UVo[3, 0], 0.5 * (u[0] + u[1]), atol=1e-6)
+    # the seam: face 1's uvs shifted
+    uv2 = uv.copy()
+    uv2[1] += np.array([10.0, 0.0], np.float32)
+    _, _, _, UVs = oracle.loop_subdivide(P, F, 1, uv=uv2)
+    np.testing.assert_allclose(UVs[0, 0], u[0], atol=1e-6)                    # v0 keeps its uv (face 0)
+    np.testing.assert_allclose(UVs[4, 0], u[0] + [10, 0], atol=1e-6)          # and in face 1
+    np.testing.assert_allclose(UVs[0, 2], 0.5 * (u[0] + u[2]), atol=1e-6)      # face 0's seam midpoint
+    np.testing.assert_allclose(UVs[4, 1], 0.5 * (u[0] + u[2]) + [10, 0], atol=1e-6)  # face 1's
+    np.testing.assert_allclose(UVs[0, 1], 0.5 * (u[0] + u[1]), atol=1e-6)      # a boundary edge
+    # v1, off the seam: boundary rule on its own face's uvs
+    np.testing.assert_allclose(UVs[1, 1], 0.75 * u[1] + 0.125 * (u[0] + u[2]), atol=1e-6)

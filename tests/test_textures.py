@@ -202,17 +202,16 @@ def test_loader_rejects_bad_shaders(tmp_path, edit, msg):
     assert msg in str(ei.value)
 
 
-def test_loader_rejects_uv_textures_on_subdivided_meshes(tmp_path):
+def test_loader_accepts_uv_textures_on_subdivided_meshes(tmp_path):
+    """Subdivided meshes carry their texture coordinates (Loop subdivision on
+    the GPU, loopSubdivMesh.cpp:174-191), so a uv-mapped texture on one loads."""
     from aurora_amd import scenes
-    from aurora_amd._lib import AuroraError
     spec = scenes.textured_stage(res=16, spp=1, maxdepth=2, mindepth=2)
     spec.objects["floor"]["subdlevels"] = 1
     path = spec.write(str(tmp_path))
     r = host_renderer()
-    with pytest.raises(AuroraError) as ei:
-        r.load(path, os.path.join(str(tmp_path), "config.json"))
+    r.load(path, os.path.join(str(tmp_path), "config.json"))
     r.close()
-    assert "subdivided" in str(ei.value)
 
 
 # ---------------------------------------------------------------------------
