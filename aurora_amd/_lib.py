@@ -186,6 +186,7 @@ SIGNATURES = {
     "aur_render_paths": (C.c_int, [_P, _IP, _IP, C.c_int, _FP, _FP, C.POINTER(Stats)]),
     "aur_framebuffer": (C.c_int, [_P, _FP]),
     "aur_write_image": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.c_int]),
+    "aur_render_image": (C.c_int, [_P, C.c_char_p, C.c_double, C.POINTER(Stats), C.POINTER(C.c_int)]),
     "aur_exr_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _FP, C.c_int, C.c_int, C.POINTER(C.c_char_p),
                                 C.POINTER(C.c_char_p), C.c_int]),
     "aur_exr_read": (C.c_int, [C.c_char_p, _IP, _IP, _FP, C.c_int64]),

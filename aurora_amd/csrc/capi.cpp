@@ -5,11 +5,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "context.h"
 #include "exr_io.h"
@@ -798,45 +802,153 @@ int aur_scene_export_shading(aur_ctx *ctx, float *uvs, aur_shader *shaders, aur_
     });
 }
 
+// Passes [first_pass, first_pass + n_passes) into the accumulation buffer and
+// the FP16 display; first_pass == 0 (or a new resolution) clears both.
+static void render_range(Ctx *c, int first_pass, int n_passes, aur_stats *st) {
+    if (!c->built) throw StateError("render before aur_build_accel");
+    if (first_pass < 0 || n_passes < 0) throw std::invalid_argument("negative pass range");
+    set_device(c);
+    const int w = c->scene.opt.width, h = c->scene.opt.height;
+    const size_t wh = (size_t)w * h;
+    bool resized = c->fb_w != w || c->fb_h != h;
+    c->accum.reserve(sizeof(float4) * wh);
+    c->display.reserve(sizeof(ushort4) * wh);
+    if (first_pass == 0 || resized) {
+        fill_zero(c, c->accum.p, sizeof(float4) * wh);
+        fill_zero(c, c->display.p, sizeof(ushort4) * wh);
+        c->display_passes = 0;
+    }
+    c->fb_w = w;
+    c->fb_h = h;
+    std::memset(st, 0, sizeof *st);
+    if (n_passes > 0) {
+        PathSource src;
+        src.mode = 0;
+        src.first_pass = first_pass;
+        src.pixel = src.sample = nullptr;
+        src.slots = nullptr;
+        src.ls = c->scene.opt.lightsamples;
+        src.display = c->display.as<ushort4>();
+        src.display_base = c->display_passes;
+        if (src.ls < 1) throw std::invalid_argument("lightsamples must be at least 1");
+        run_paths(c, src, pass_paths(wh, first_pass, n_passes, src.ls), c->accum.as<float4>(), st);
+        c->display_passes += n_passes;
+    }
+}
+
 int aur_render(aur_ctx *ctx, int first_pass, int n_passes, float *rgba_sum_host, aur_stats *out) {
     return guarded([&] {
         NEED(ctx);
         Ctx *c = &ctx->c;
-        if (!c->built) throw StateError("render before aur_build_accel");
-        if (first_pass < 0 || n_passes < 0) throw std::invalid_argument("negative pass range");
-        set_device(c);
-        const int w = c->scene.opt.width, h = c->scene.opt.height;
-        const size_t wh = (size_t)w * h;
-        bool resized = c->fb_w != w || c->fb_h != h;
-        c->accum.reserve(sizeof(float4) * wh);
-        c->display.reserve(sizeof(ushort4) * wh);
-        if (first_pass == 0 || resized) {
-            fill_zero(c, c->accum.p, sizeof(float4) * wh);
-            fill_zero(c, c->display.p, sizeof(ushort4) * wh);
-            c->display_passes = 0;
-        }
-        c->fb_w = w;
-        c->fb_h = h;
         aur_stats st;
-        std::memset(&st, 0, sizeof st);
-        if (n_passes > 0) {
-            PathSource src;
-            src.mode = 0;
-            src.first_pass = first_pass;
-            src.pixel = src.sample = nullptr;
-            src.slots = nullptr;
-            src.ls = c->scene.opt.lightsamples;
-            src.display = c->display.as<ushort4>();
-            src.display_base = c->display_passes;
-            if (src.ls < 1) throw std::invalid_argument("lightsamples must be at least 1");
-            run_paths(c, src, pass_paths(wh, first_pass, n_passes, src.ls), c->accum.as<float4>(), &st);
-            c->display_passes += n_passes;
-        }
+        render_range(c, first_pass, n_passes, &st);
         if (rgba_sum_host) {
+            const size_t wh = (size_t)c->fb_w * c->fb_h;
             HIP_CHECK(hipMemcpyAsync(rgba_sum_host, c->accum.p, sizeof(float4) * wh, hipMemcpyDeviceToHost, c->stream));
             HIP_CHECK(hipStreamSynchronize(c->stream));
         }
         if (out) *out = st;
+    });
+}
+
+// The display written by a host thread while the device traces on: one write
+// in flight at a time (the reference waits for its draw task before the next
+// pass, renderer.cpp:238), every exception carried back to the caller.
+struct DisplayWriter {
+    std::thread t;
+    std::exception_ptr err;
+    void wait() {
+        if (t.joinable()) t.join();
+        if (err) {
+            std::exception_ptr e = err;
+            err = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+    ~DisplayWriter() {
+        if (t.joinable()) t.join();
+    }
+};
+
+// Renderer::renderImage, outputStats and postRenderCleanup (renderer.cpp:200-304).
+int aur_render_image(aur_ctx *ctx, const char *path, double draw_delay_s, aur_stats *out, int *n_draws) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->built) throw StateError("render before aur_build_accel");
+        const std::string fn = path ? std::string(path) : c->scene.opt.filename;
+        if (fn.empty()) throw std::invalid_argument("render output filename is blank");  // renderer.cpp:217-219
+        if (!(draw_delay_s <= 1e9)) throw std::invalid_argument("draw delay must be a number of seconds");
+        const double delay = draw_delay_s > 0 ? draw_delay_s : 1.0;  // DRAW_DELAY (constants.h:20)
+        const int total = c->scene.opt.pixelsamples;
+        const int w = c->scene.opt.width, h = c->scene.opt.height;
+        const size_t n = (size_t)w * h;
+        using clock = std::chrono::steady_clock;
+        const clock::time_point t0 = clock::now();
+        clock::time_point last_draw = t0;
+        std::vector<uint16_t> shown(4 * n);
+        DisplayWriter writer;
+        int draws = 0;
+        auto draw = [&](std::vector<std::pair<std::string, std::string>> meta) {
+            writer.wait();
+            HIP_CHECK(hipMemcpyAsync(shown.data(), c->display.p, sizeof(uint16_t) * 4 * n, hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            meta.insert(meta.begin(), {"renderer", "Aurora (MI355X wavefront tracer)"});
+            writer.t = std::thread([&writer, &shown, fn, w, h, n, meta] {
+                try {
+                    std::vector<float> img(4 * n);
+                    for (size_t i = 0; i < 4 * n; ++i) img[i] = half_to_float(shown[i]);
+                    exr_write(fn, w, h, img.data(), EXR_ZIP, true, meta);
+                } catch (const std::exception &e) {
+                    writer.err = std::make_exception_ptr(IoError(e.what()));
+                }
+            });
+            ++draws;
+        };
+        aur_stats sum, st;
+        std::memset(&sum, 0, sizeof sum);
+        // Passes are traced in chunks sized to a quarter of the draw delay
+        // (pass 0 alone: it is always drawn); the reference's per-pass check
+        // then falls on the end of the first chunk that ends past the delay.
+        int done = 0;
+        double per_pass_s = 0.0;
+        render_range(c, 0, 0, &st);
+        while (done < total) {
+            int chunk = 1;
+            if (done > 0 && per_pass_s > 0.0)
+                chunk = (int)std::min<double>(total - done, std::max(1.0, std::floor(0.25 * delay / per_pass_s)));
+            const clock::time_point ts = clock::now();
+            render_range(c, done, chunk, &st);
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            const clock::time_point te = clock::now();
+            per_pass_s = std::chrono::duration<double>(te - ts).count() / chunk;
+            sum.paths += st.paths;
+            sum.ext_rays += st.ext_rays;
+            sum.hit_rays += st.hit_rays;
+            sum.shadow_rays += st.shadow_rays;
+            sum.ms += st.ms;
+            const bool first = done == 0;
+            done += chunk;
+            if (first || std::chrono::duration<double>(te - last_draw).count() > delay) {
+                draw({});
+                last_draw = clock::now();
+            }
+        }
+        // outputStats (renderer.cpp:276-298): whole seconds since the render
+        // began, and the traced rays (camera, extension and shadow rays, as
+        // integrator.cpp counts them) per second of tracing, in thousands
+        const long long secs = (long long)std::chrono::duration<double>(clock::now() - t0).count();
+        const double trace_s = sum.ms * 1e-3;
+        const long long krays =
+            trace_s > 0.0 ? (long long)((double)(sum.ext_rays + sum.shadow_rays) * 1e-3 / trace_s) : 0;
+        draw({{"renderTime", std::to_string(secs / 3600) + " h " + std::to_string((secs / 60) % 60) + " min " +
+                                 std::to_string(secs % 60) + " sec."},
+              {"RaySpeed", std::to_string(krays) + "K rays/sec."}});
+        writer.wait();
+        sum.ms = std::chrono::duration<double>(clock::now() - t0).count() * 1e3;
+        if (out) *out = sum;
+        if (n_draws) *n_draws = draws;
     });
 }
 

@@ -202,6 +202,20 @@ class Renderer:
         keys, vals, n = _meta(meta)
         L.check(self.lib.aur_write_image(self._ctx, path.encode(), int(n_passes), keys, vals, n))
 
+    def render_image(self, path: Optional[str] = None, draw_delay: float = 1.0) -> dict:
+        """Renderer::renderImage + outputStats + postRenderCleanup
+        (renderer.cpp:200-304): all passes, the EXR rewritten after pass 0 and
+        every ``draw_delay`` seconds, then a final write carrying renderTime
+        and RaySpeed.  ``path`` None writes the scene's options filename.
+        Returns the counters plus ``draws`` (files written)."""
+        st = L.Stats()
+        draws = C.c_int(0)
+        L.check(self.lib.aur_render_image(self._ctx, path.encode() if path else None, float(draw_delay),
+                                          C.byref(st), C.byref(draws)))
+        self.passes_done = self.info()["pixelsamples"]
+        self.last_stats = st.as_dict()
+        return dict(self.last_stats, draws=draws.value)
+
     def set_launch(self, trace_block: int = 0, trace_waves_per_cu: int = 0, pool_paths: int = 0) -> None:
         """Launch shape of the tracer kernels; 0 keeps a field."""
         o = L.LaunchOpts(int(trace_block), int(trace_waves_per_cu), int(pool_paths))

@@ -293,6 +293,21 @@ int aur_display(aur_ctx *ctx, uint16_t *rgba_half_host);
  * <path>_tmp and renames it over path. */
 int aur_write_image(aur_ctx *ctx, const char *path, int n_passes, const char *const *meta_keys,
                     const char *const *meta_values, int n_meta);
+/* Replaces Renderer::renderImage, outputStats and postRenderCleanup
+ * (renderer.cpp:200-304): renders the scene's pixelsamples passes from pass 0
+ * and writes the display (aur_display's FP16 running mean, as
+ * aur_write_image with n_passes = 0) to path -- NULL: the scene's options
+ * "filename" -- after pass 0, again whenever more than draw_delay_s seconds
+ * (<= 0: the reference's DRAW_DELAY, 1 s) have passed since the last write,
+ * and a final time with the "renderTime" ("H h M min S sec.") and "RaySpeed"
+ * ("NK rays/sec.") attributes.  Passes are traced in chunks of about a quarter
+ * of the delay, so a timed write lands at the end of the first chunk that ends
+ * past the delay; a file is encoded on a host thread while the next chunk
+ * traces.  RaySpeed counts camera, extension and shadow rays per second of
+ * device time (the reference averages per-pass rates over whole seconds, at
+ * least one).  out receives the summed counters with ms = the call's wall
+ * time; n_draws (may be NULL) the number of files written. */
+int aur_render_image(aur_ctx *ctx, const char *path, double draw_delay_s, aur_stats *out, int *n_draws);
 /* Generic writer: rgba is width*height*4 floats, row 0 first; compression
  * 0 (NONE), 2 (ZIPS) or 3 (ZIP); half != 0 stores FP16 channels. */
 int aur_exr_write(const char *path, int width, int height, const float *rgba, int compression, int half,
