@@ -196,6 +196,7 @@ SIGNATURES = {
     "aur_get_launch": (C.c_int, [_P, C.POINTER(LaunchOpts)]),
     "aur_set_material_params": (C.c_int, [_P, _FP, _FP]),
     "aur_adjoint": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(ParamGrads), C.POINTER(Stats)]),
+    "aur_keep_path_records": (C.c_int, [_P, C.c_int, C.c_int]),
     "aur_set_profiling": (C.c_int, [_P, C.c_int]),
     "aur_kernel_stats": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int, _IP]),
     "aur_trace": (C.c_int, [_P, C.POINTER(Ray), C.c_int, C.POINTER(Hit), C.c_int]),

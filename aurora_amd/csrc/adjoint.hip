@@ -27,10 +27,14 @@
 //
 // The first replay is the wavefront renderer itself, recording each path's
 // radiance and clamp steps (run_paths with PathSource::adj_rec); the second
-// runs one thread per path (k_adj_grad); no per-vertex state is stored.
-// Gradients of small scenes are first summed per workgroup in LDS and then
-// added to HBM with one atomic per parameter per workgroup.
+// runs one thread per path (k_adj_grad); no per-vertex state is stored.  When
+// a forward render of the same passes with the same inputs kept those records
+// (aur_keep_path_records: 16 bytes per path in HBM), the first replay is
+// skipped.  Gradients of small scenes are first summed per workgroup in LDS
+// and then added to HBM with one atomic per parameter per workgroup.
 #include <hipcub/hipcub.hpp>
+
+#include <cstring>
 
 #include "shade_common.h"
 #include "traverse.h"
@@ -348,6 +352,18 @@ __global__ void __launch_bounds__(128, 6) k_adj_grad(SceneView sv, AdjParams ap,
     }
 }
 
+// The records a forward render kept for passes [first_pass, first_pass +
+// n_passes), or null when any of them is missing or was recorded with other
+// inputs or another image size.
+static float4 *kept_records(Ctx *c, int first_pass, int n_passes) {
+    const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
+    if (c->rec_n <= 0 || !c->path_rec.p || c->rec_wh != wh || c->scene.opt.lightsamples != 1) return nullptr;
+    if (first_pass < c->rec_first || first_pass + n_passes > c->rec_first + c->rec_n) return nullptr;
+    for (int p = first_pass; p < first_pass + n_passes; ++p)
+        if (c->rec_version[p - c->rec_first] != c->version) return nullptr;
+    return c->path_rec.as<float4>() + (size_t)(first_pass - c->rec_first) * wh;
+}
+
 void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *d_galb, float *d_grough, float *d_gspec,
              float *d_gptex, aur_stats *stats) {
     if (!c->built) throw StateError("adjoint before aur_build_accel");
@@ -379,30 +395,41 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     // launch bound: at 3 waves without it the replay measured 25 % slower
     // even though the bound spills a few registers)
     const int blocks = std::max(1, std::min((int)((n + 127) / 128), c->n_cu * 12));
-    DevBuf sovf, rec;
+    DevBuf sovf, own_rec;
     sovf.reserve(sizeof(int) * (size_t)blocks * 128 * ovf);
-    rec.reserve(sizeof(float4) * (size_t)std::max<long long>(n, 1));
     const size_t lds = ap.use_lds ? sizeof(float) * (size_t)((d_galb ? 3 * c->n_tris : 0) + 2 * c->n_mats) : 0;
-    // first replay: the wavefront renderer traces the same paths (same
-    // counter-based random numbers) and records each path's radiance and
-    // clamp steps instead of adding it to a framebuffer
-    PathSource src;
-    src.mode = 0;
-    src.first_pass = first_pass;
-    src.ls = 1;
-    src.pixel = src.sample = nullptr;
-    src.slots = nullptr;
-    src.adj_rec = rec.as<float4>();
-    src.rec_base = 0;
     aur_stats fst;
-    run_paths(c, src, (size_t)n, nullptr, &fst);
+    std::memset(&fst, 0, sizeof fst);
+    float4 *rec = kept_records(c, first_pass, n_passes);
+    if (rec) {
+        // a forward render of these passes with the current inputs kept the
+        // records: the first replay is not needed
+        fst.paths = n;
+    } else {
+        // first replay: the wavefront renderer traces the same paths (same
+        // counter-based random numbers) and records each path's radiance and
+        // clamp steps instead of adding it to a framebuffer
+        own_rec.reserve(sizeof(float4) * (size_t)std::max<long long>(n, 1));
+        rec = own_rec.as<float4>();
+        PathSource src;
+        src.mode = 0;
+        src.first_pass = first_pass;
+        src.ls = 1;
+        src.pixel = src.sample = nullptr;
+        src.slots = nullptr;
+        src.adj_rec = rec;
+        src.rec_base = 0;
+        src.rec_count = n;
+        src.rec_only = true;
+        run_paths(c, src, (size_t)n, nullptr, &fst);
+    }
     // replay order: longest paths first
     DevBuf keys[2], idx[2], stemp;
     for (int k = 0; k < 2; ++k) {
         keys[k].reserve(sizeof(unsigned) * (size_t)std::max<long long>(n, 1));
         idx[k].reserve(sizeof(int) * (size_t)std::max<long long>(n, 1));
     }
-    hipLaunchKernelGGL(k_adj_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, rec.as<float4>(), (int)n,
+    hipLaunchKernelGGL(k_adj_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, rec, (int)n,
                        keys[0].as<unsigned>(), idx[0].as<int>());
     size_t st_bytes = 0;
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, st_bytes, keys[0].as<unsigned>(), keys[1].as<unsigned>(),
@@ -416,10 +443,10 @@ void adjoint(Ctx *c, int first_pass, int n_passes, const float4 *d_dldi, float *
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, c->stream));
     if (c->bvh.width == 8)
-        hipLaunchKernelGGL(k_adj_grad<8>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(),
+        hipLaunchKernelGGL(k_adj_grad<8>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec,
                            idx[1].as<int>(), sovf.as<int>(), (int)ovf);
     else
-        hipLaunchKernelGGL(k_adj_grad<4>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec.as<float4>(),
+        hipLaunchKernelGGL(k_adj_grad<4>, dim3(blocks), dim3(128), lds, c->stream, sv, ap, rec,
                            idx[1].as<int>(), sovf.as<int>(), (int)ovf);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(e1, c->stream));

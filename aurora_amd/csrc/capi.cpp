@@ -537,6 +537,7 @@ int aur_set_stream(aur_ctx *ctx, void *hip_stream) {
 int aur_set_seed(aur_ctx *ctx, uint32_t seed) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         ctx->c.seed = seed;
     });
 }
@@ -544,6 +545,7 @@ int aur_set_seed(aur_ctx *ctx, uint32_t seed) {
 int aur_scene_load(aur_ctx *ctx, const char *scene_path, const char *config_json_path) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         NEED(scene_path);
         HostScene s;
         std::string cfg = config_json_path ? config_json_path : "";
@@ -570,6 +572,7 @@ int aur_scene_load(aur_ctx *ctx, const char *scene_path, const char *config_json
 int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         NEED(d);
         if (d->n_tris < 0 || d->n_verts < 0 || d->n_attrs < 0 || d->n_lights < 0) throw std::invalid_argument("negative count");
         if (d->n_tris > 0) { NEED(d->verts); NEED(d->tris); NEED(d->normals); NEED(d->attr); }
@@ -644,6 +647,7 @@ int aur_scene_upload(aur_ctx *ctx, const aur_scene_desc *d) {
 int aur_set_albedo_texture(aur_ctx *ctx, int width, int height, const float *rgb, const int *attrs, int n_attrs) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         Ctx *c = &ctx->c;
         if (!c->built) throw StateError("the albedo texture needs a built scene (call aur_build_accel first)");
         if (!rgb) {
@@ -689,6 +693,7 @@ int aur_set_albedo_texture(aur_ctx *ctx, int width, int height, const float *rgb
 int aur_set_tri_albedo(aur_ctx *ctx, const float *albedo_rgb) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         Ctx *c = &ctx->c;
         if (!albedo_rgb) { c->tri_albedo.clear(); return; }
         if (!c->scene.flat) throw StateError("per-triangle albedo needs a flattened scene (call aur_build_accel first)");
@@ -706,6 +711,7 @@ int aur_set_tri_albedo(aur_ctx *ctx, const float *albedo_rgb) {
 int aur_build_accel(aur_ctx *ctx, const aur_accel_opts *opts) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         Ctx *c = &ctx->c;
         if (!c->have_scene) throw StateError("no scene loaded");
         set_device(c);
@@ -831,8 +837,20 @@ static void render_range(Ctx *c, int first_pass, int n_passes, aur_stats *st) {
         src.display = c->display.as<ushort4>();
         src.display_base = c->display_passes;
         if (src.ls < 1) throw std::invalid_argument("lightsamples must be at least 1");
+        // kept records: the passes of this call inside the kept range also
+        // write their per-path records (one path per pixel and pass)
+        const int r0 = std::max(first_pass, c->rec_first), r1 = std::min(first_pass + n_passes, c->rec_first + c->rec_n);
+        const bool keep = c->rec_n > 0 && c->path_rec.p && c->rec_wh == wh && src.ls == 1 && r0 < r1;
+        if (keep) {
+            src.adj_rec = c->path_rec.as<float4>();
+            src.rec_base = (long long)(first_pass - c->rec_first) * (long long)wh;
+            src.rec_count = (long long)c->rec_n * (long long)wh;
+            src.rec_only = false;
+        }
         run_paths(c, src, pass_paths(wh, first_pass, n_passes, src.ls), c->accum.as<float4>(), st);
         c->display_passes += n_passes;
+        if (keep)
+            for (int p = r0; p < r1; ++p) c->rec_version[p - c->rec_first] = c->version;
     }
 }
 
@@ -955,6 +973,7 @@ int aur_render_image(aur_ctx *ctx, const char *path, double draw_delay_s, aur_st
 int aur_set_sampler(aur_ctx *ctx, int sampler) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         Ctx *c = &ctx->c;
         if (sampler != AUR_SAMPLER_HASH && sampler != AUR_SAMPLER_HALTON) throw std::invalid_argument("unknown sampler");
         c->sampler = sampler;
@@ -1172,6 +1191,7 @@ int aur_render_paths(aur_ctx *ctx, const int *pixel, const int *sample, int n_pa
 int aur_set_material_params(aur_ctx *ctx, const float *roughness, const float *spec_weight) {
     return guarded([&] {
         NEED(ctx);
+        ++ctx->c.version;  // kept path records no longer match
         Ctx *c = &ctx->c;
         HostScene &s = c->scene;
         if (!c->have_scene) throw StateError("no scene loaded");
@@ -1200,6 +1220,30 @@ int aur_set_material_params(aur_ctx *ctx, const float *roughness, const float *s
                                      c->stream));
             HIP_CHECK(hipStreamSynchronize(c->stream));
         }
+    });
+}
+
+int aur_keep_path_records(aur_ctx *ctx, int first_pass, int n_passes) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (first_pass < 0 || n_passes < 0) throw std::invalid_argument("negative pass range");
+        if (n_passes == 0) {
+            c->path_rec.release();
+            c->rec_first = c->rec_n = 0;
+            c->rec_wh = 0;
+            c->rec_version.clear();
+            return;
+        }
+        if (!c->have_scene) throw StateError("no scene loaded");
+        set_device(c);
+        const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
+        if ((long long)wh * n_passes > 0x7fffffffLL) throw CapacityError("too many paths to keep records of");
+        c->path_rec.reserve(sizeof(float4) * wh * (size_t)n_passes);
+        c->rec_first = first_pass;
+        c->rec_n = n_passes;
+        c->rec_wh = wh;
+        c->rec_version.assign((size_t)n_passes, 0);
     });
 }
 

@@ -187,6 +187,18 @@ struct Ctx {
     long long display_passes = 0;
     int fb_w = 0, fb_h = 0;
 
+    // Inputs generation: bumped by every call that can change a path's
+    // radiance (scene, seed, sampler, material parameters, albedo overrides).
+    uint64_t version = 1;
+    // Per-path records of passes [rec_first, rec_first + rec_n) kept from
+    // forward renders (aur_keep_path_records), in the adjoint's record layout;
+    // rec_version[p] is the inputs generation pass rec_first + p was recorded
+    // at (0: not recorded), rec_wh the image size they were recorded for.
+    DevBuf path_rec;
+    int rec_first = 0, rec_n = 0;
+    size_t rec_wh = 0;
+    std::vector<uint64_t> rec_version;
+
     Wavefront wf;
     KernelTimer timer;
     LaunchCfg launch;
@@ -210,12 +222,16 @@ struct PathSource {
     const int *pixel;      // device, mode 1
     const int *sample;     // device, mode 1
     const float *slots;    // device, optional injected uniforms
-    // mode 0 with ls = 1: per-path records for the adjoint instead of the
-    // framebuffer: rec[rec_base + path] = (unclamped radiance, clamp steps |
-    // 0x10000 for a camera miss | rays traced << 17), path = pass * w * h +
-    // tiled position
+    // mode 0 with ls = 1: per-path records for the adjoint:
+    // rec[rec_base + path] = (unclamped radiance, clamp steps | 0x10000 for a
+    // camera miss | rays traced << 17), path = pass * w * h + tiled position;
+    // only indices in [0, rec_count) are written.  rec_only: records instead
+    // of the framebuffer (the adjoint's first replay); otherwise both (a
+    // forward render keeping its records).
     float4 *adj_rec = nullptr;
     long long rec_base = 0;
+    long long rec_count = 0;
+    bool rec_only = false;
     // mode 0: the FP16 running mean updated with every resolved pass, or null;
     // display_base passes were appended to it before this run
     ushort4 *display = nullptr;

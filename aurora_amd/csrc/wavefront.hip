@@ -213,7 +213,8 @@ __global__ void k_finish(SceneView sv, WfView wv) {
             if (sub > 0) { ext_dup += 1; hit_dup += cam_miss ? 0 : 1; }
         }
         wv.out[slot] = make_float4(L.x, L.y, L.z, w);
-        if (wv.src.adj_rec) {
+        const long long ri = wv.src.rec_base + m.y;
+        if (wv.src.adj_rec && ri >= 0 && ri < wv.src.rec_count) {
             // the adjoint's first replay: radiance before the clamp and the
             // number of 0.2 steps the clamp took (integrator.cpp:276-283)
             f3 Lc = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
@@ -225,7 +226,7 @@ __global__ void k_finish(SceneView sv, WfView wv) {
                 }
             const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
             // bits 0-15 clamp steps, bit 16 camera miss, bits 17-24 rays traced
-            wv.src.adj_rec[wv.src.rec_base + m.y] = make_float4(
+            wv.src.adj_rec[ri] = make_float4(
                 a.x + b.x, a.y + b.y, a.z + b.z,
                 __int_as_float(k | (cam_miss ? 0x10000 : 0) | (min(path_ext_rays(m.x), 255) << 17)));
         }
@@ -551,7 +552,7 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
         WfView wv = wf_view(c, s);
         hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, wv.counts, wv.stats, 0);
         run_batch(c, sv, wv, (int)n);
-        if (src.adj_rec) {
+        if (src.adj_rec && src.rec_only) {
             // records only
         } else if (src.mode == 0) {
             int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));

@@ -3,7 +3,9 @@
 ``RenderFunction`` is a ``torch.autograd.Function``: forward renders the mean
 image for per-triangle diffuse albedo, per-material roughness and per-material
 Kelemen specular weight (``aur_render``); backward runs the path-replay
-adjoint (``aur_adjoint``) with the same random numbers.  ``make_problem`` and
+adjoint (``aur_adjoint``) with the same random numbers, reusing the radiance
+records the forward render kept (``aur_keep_path_records``) instead of
+replaying each path a first time.  ``make_problem`` and
 ``optimize`` set up and run the inverse-rendering workload on the box room:
 512x512, 32 spp forward and adjoint, depth 6, ground truth albedo
 ~U[0.1, 0.9]^3 per triangle, roughness ~U[0.05, 0.8] and spec weight ~U[0, 1]
@@ -49,6 +51,8 @@ def make_function():
 
         @staticmethod
         def forward(ctx, albedo, rough, spec, renderer, first_pass, passes):
+            # the render keeps its per-path records for the backward replay
+            renderer.keep_path_records(first_pass, passes)
             img = render_image(renderer, albedo.detach().cpu().numpy(), rough.detach().cpu().numpy(),
                                spec.detach().cpu().numpy(), first_pass, passes)
             ctx.renderer, ctx.first_pass, ctx.passes = renderer, first_pass, passes
@@ -76,6 +80,7 @@ def make_texture_function():
     class TextureRenderFunction(torch.autograd.Function):
         @staticmethod
         def forward(ctx, tex, renderer, attrs, first_pass, passes):
+            renderer.keep_path_records(first_pass, passes)
             renderer.set_albedo_texture(tex.detach().cpu().numpy().astype(np.float32), attrs)
             acc, _ = renderer.render_passes(first_pass, passes)
             ctx.renderer, ctx.first_pass, ctx.passes = renderer, first_pass, passes

@@ -248,6 +248,13 @@ class Renderer:
         L.check(self.lib.aur_set_material_params(self._ctx, L.fptr(self._rough) if self._rough is not None else None,
                                                  L.fptr(self._specw) if self._specw is not None else None))
 
+    def keep_path_records(self, first_pass: int, n_passes: int) -> None:
+        """Later renders of passes [first_pass, first_pass + n_passes) keep each
+        path's radiance record (16 bytes per path), so that an adjoint of those
+        passes with unchanged inputs skips its first replay; n_passes = 0
+        frees them."""
+        L.check(self.lib.aur_keep_path_records(self._ctx, int(first_pass), int(n_passes)))
+
     def adjoint(self, first_pass: int, n_passes: int, dloss_dimage: np.ndarray, albedo: bool = True,
                 texture: bool = False):
         """Path-replay adjoint: returns (albedo grads (n_tris, 3) or None, roughness grads, spec grads, stats),

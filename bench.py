@@ -66,6 +66,20 @@ def _hip_sync():
         raise RuntimeError("hipDeviceSynchronize failed: %d" % rc)
 
 
+def _host_threads() -> int:
+    """Every host core this process may run on (its CPU affinity), limited
+    by OMP_NUM_THREADS where the host sets it to the job's CPU share."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    return max(1, min(n, omp) if omp > 0 else n)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -78,16 +92,22 @@ def _cpu_model():
 
 
 def _load_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter profile
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE of a profiled run of
+    this benchmark), and where that figure comes from.  The counters cannot be
+    read inside this run (a counter pass is a separate profiled process), so
+    the figure belongs to the profiled build and launch shape named there."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     try:
         with open(path) as fh:
             t = json.load(fh)
         v = t.get(kernel, {}).get("hbm_bytes_per_launch")
-        return float(v) if v is not None else None
-    except (OSError, ValueError):
-        return None
+        src = "profiles/traffic.json: " + t.get("source", "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch")
+        return (float(v), src) if v is not None else (None, None)
+    except (OSError, ValueError, AttributeError):
+        return None, None
 
 
 def cpu_baseline(exp: dict, target_s: float, threads: int, spp: int):
@@ -151,7 +171,7 @@ def box_room_leg(threads: int, reps: int = 32, ref_spp: int = 4096, stride: int 
     r = Renderer(path, os.path.join(tmp, "config.json"))
     r.build()
     cfg = tuning.choose(r)
-    if (cfg["bvh_width"], cfg["max_leaf"]) != (4, 4):
+    if (cfg["bvh_width"], cfg["max_leaf"]) != (4, 2):
         r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
     tuning.apply_launch(r, cfg)
     info = r.info()
@@ -264,6 +284,21 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
         ga, gr, gs, st = r.adjoint(0, spp, dl.astype(np.float32))
     _hip_sync()
     adj_s = (time.perf_counter() - t0) / reps
+    # the optimisation loop's form: each forward render keeps its per-path
+    # records and the adjoint of the same passes skips its first replay
+    r.keep_path_records(0, spp)
+    fwd_k, adj_k = 0.0, 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        inverse.render_image(r, albedo, rough, spec, 0, spp)
+        _hip_sync()
+        t1 = time.perf_counter()
+        r.adjoint(0, spp, dl.astype(np.float32))
+        _hip_sync()
+        fwd_k += t1 - t0
+        adj_k += time.perf_counter() - t1
+    fwd_k, adj_k = fwd_k / reps, adj_k / reps
+    r.keep_path_records(0, 0)
 
     def loss(a, sw):
         r.set_seed(5)
@@ -337,13 +372,16 @@ def adjoint_leg(res: int, spp: int, depth: int, n_fd: int = 64, adam_steps: int 
     return {"adam": adam, "workload": "box_room_kelemen_%d_spp%d_depth%d" % (res, spp, depth), "params": int(ga.size + gr.size + gs.size),
             "forward_ms": round(fwd_s * 1e3, 3), "adjoint_ms": round(adj_s * 1e3, 3),
             "adjoint_Mpaths_s": round(n_paths / adj_s / 1e6, 3), "adjoint_vs_forward": round(adj_s / fwd_s, 3),
+            "kept_records": {"forward_ms": round(fwd_k * 1e3, 3), "adjoint_ms": round(adj_k * 1e3, 3),
+                             "adjoint_Mpaths_s": round(n_paths / adj_k / 1e6, 3),
+                             "adjoint_vs_forward": round(adj_k / fwd_k, 3)},
             "fd_checked": int(len(pick)) + len(rough_k), "fd_h": h, "fd_max_rel_err": float("%.3g" % worst),
             "fd_albedo_spec_checked": int(len(pick)), "fd_roughness_checked": len(rough_k),
             "fd_roughness_seeds": n_seeds, "fd_roughness_max_z": float("%.3g" % (z_r.max() if len(z_r) else 0.0)),
             "fd_roughness_rel_err_of_means": [float("%.3g" % x) for x in rel_r]}
 
 
-def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd: int = 64):
+def large_leg(fwd_passes: int = 256, adj_passes: int = 64, fd_passes: int = 1, n_fd: int = 64):
     """The large differentiable scene (3840x2160, depth 16, 32 displaced level-7
     icospheres on 8 terrain patches of 1M triangles: 18.5M triangles; a
     2048x2048 RGB FP32 albedo texture on the terrain): scene build time,
@@ -397,13 +435,33 @@ def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd
     _, fst = r.render_passes(0, fwd_passes, fetch=False)
     _hip_sync()
     fwd_s = time.perf_counter() - t0
-    # adjoint rate (texture gradient only)
+    _progress("large scene: %d forward passes in %.2f s" % (fwd_passes, fwd_s))
+    # adjoint rate (texture gradient only); the wavefront kernels' time inside
+    # the call is its first replay (radiance per path), the rest its gradient replay
     r.adjoint(0, 1, dl, albedo=False, texture=True)
     _hip_sync()
+    r.set_profiling(True)
     t0 = time.perf_counter()
     _, _, _, ast, gt_rate = r.adjoint(0, adj_passes, dl, albedo=False, texture=True)
     _hip_sync()
     adj_s = time.perf_counter() - t0
+    first_replay_ms = sum(v["total_ms"] for v in r.kernel_stats().values())
+    r.set_profiling(False)
+    _progress("large scene: %d adjoint passes in %.2f s" % (adj_passes, adj_s))
+    # the same with the forward render keeping its per-path records for the
+    # adjoint's passes (16 B per path), which then skips its first replay
+    r.keep_path_records(0, adj_passes)
+    t0 = time.perf_counter()
+    r.render_passes(0, fwd_passes, fetch=False)
+    _hip_sync()
+    fwd_k = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, _, kst, gt_k = r.adjoint(0, adj_passes, dl, albedo=False, texture=True)
+    _hip_sync()
+    adj_k = time.perf_counter() - t0
+    r.keep_path_records(0, 0)
+    kept_diff = float(np.abs(gt_k - gt_rate).max() / max(np.abs(gt_rate).max(), 1e-30))
+    _progress("large scene: kept records, forward %.2f s, adjoint %.2f s" % (fwd_k, adj_k))
     # gradient check at fd_passes passes
     r.set_seed(21)
     _, _, _, _, gt = r.adjoint(0, fd_passes, dl * fd_passes, albedo=False, texture=True)
@@ -435,6 +493,13 @@ def large_leg(fwd_passes: int = 4, adj_passes: int = 4, fd_passes: int = 1, n_fd
             "forward_Mrays_s": round((fst["ext_rays"] + fst["shadow_rays"]) / fwd_s / 1e6, 3),
             "adjoint_passes": adj_passes, "adjoint_Mpaths_s": round(ast["paths"] / adj_s / 1e6, 3),
             "adjoint_vs_forward": round((adj_s / adj_passes) / (fwd_s / fwd_passes), 3),
+            "adjoint_first_replay_ms": round(first_replay_ms, 1),
+            "adjoint_gradient_replay_ms": round(ast["ms"] - first_replay_ms, 1),
+            "kept_records": {"forward_Mpaths_s": round(fst["paths"] / fwd_k / 1e6, 3),
+                             "adjoint_Mpaths_s": round(kst["paths"] / adj_k / 1e6, 3),
+                             "adjoint_vs_forward": round((adj_k / adj_passes) / (fwd_k / fwd_passes), 3),
+                             "record_bytes": int(16 * info["width"] * info["height"] * adj_passes),
+                             "gradient_max_rel_diff": float("%.3g" % kept_diff)},
             "texels_with_gradient": int(np.count_nonzero(gt_rate.reshape(-1, 3).any(1))),
             "fd_checked": int(len(pick)), "fd_h": h, "fd_passes": fd_passes, "fd_max_rel_err": float("%.3g" % worst)}
 
@@ -614,7 +679,7 @@ def main(argv=None):
     r.build()  # Loop subdivision happens here: the tuning table is keyed by the final triangle count
     build_s = time.time() - t0
     cfg = tuning.choose(r, width=args.bvh_width or None, max_leaf=args.max_leaf or None)
-    if (cfg["bvh_width"], cfg["max_leaf"]) != (4, 4):
+    if (cfg["bvh_width"], cfg["max_leaf"]) != (4, 2):
         t0 = time.time()
         r.build(width=cfg["bvh_width"], max_leaf=cfg["max_leaf"])
         build_s = time.time() - t0
@@ -653,9 +718,10 @@ def main(argv=None):
     items = {"k_extend": tot["ext_rays"], "k_shadow": tot["shadow_rays"], "k_shade": tot["ext_rays"]}.get(dom, 0)
     per_launch_bytes = ALGO_BYTES.get(dom, 0.0) * items / max(launches, 1)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic, traffic_src = _load_traffic(dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": _load_traffic(dom), "kernel": dom,
-                "avg_launch_ms": round(avg_ms, 4), "launches": launches,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": dom, "avg_launch_ms": round(avg_ms, 4), "launches": launches,
                 "algorithmic_bytes_per_launch": round(per_launch_bytes, 1)}
     algo_total = 24 * paths + 168 * tot["ext_rays"] + 112 * tot["hit_rays"] + 96 * tot["shadow_rays"]
 
@@ -678,7 +744,7 @@ def main(argv=None):
     exp = None if args.no_cpu else r.export()
     r.close()
     if not args.no_cpu:
-        out["box_room"] = box_room_leg(min(16, os.cpu_count() or 1))
+        out["box_room"] = box_room_leg(_host_threads())
         _progress("box room leg done")
     if not args.no_batch:
         out["batch"] = batch_leg(use_graph=not args.batch_plain_launches)
@@ -690,7 +756,7 @@ def main(argv=None):
         out["large_scene"] = large_leg()
         _progress("large scene leg done")
     if exp is not None:
-        threads = min(16, os.cpu_count() or 1)
+        threads = _host_threads()
         out["cpu_baseline"] = cpu_baseline(exp, args.cpu_seconds, threads, max(1, info["pixelsamples"]))
         _progress("cpu baseline done")
         out["gpu_vs_cpu"] = round(value / max(out["cpu_baseline"]["value"], 1e-9), 2)

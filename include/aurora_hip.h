@@ -381,9 +381,24 @@ typedef struct {
  * host, alpha ignored) writes dL/dtheta into the non-NULL fields of out.
  * Roughness gradients are produced for Kelemen materials and for metal
  * materials (whose roughnessA and roughnessB aur_set_material_params sets to
- * the one value), spec-weight gradients for Kelemen materials. */
+ * the one value), spec-weight gradients for Kelemen materials.  The adjoint
+ * replays every path twice (its radiance, then its gradient); when
+ * aur_render kept the records of all these passes with the current inputs
+ * (aur_keep_path_records), the first replay is skipped.  stats counts the
+ * rays the call traced. */
 int aur_adjoint(aur_ctx *ctx, int first_pass, int n_passes, const float *dloss_dimage,
                 aur_param_grads *out, aur_stats *stats);
+/* No reference counterpart (the reference is not differentiable).  From now
+ * on, aur_render keeps the per-path record the adjoint's first replay would
+ * compute -- the path's radiance before the firefly clamp, its clamp steps
+ * and ray count, 16 bytes per path in device memory -- for every path of
+ * passes [first_pass, first_pass + n_passes) it traces (scenes with
+ * lightsamples = 1).  A record stays valid until the seed, sampler, scene,
+ * acceleration structure, material parameters or albedo overrides change;
+ * aur_adjoint uses records only when every pass of its range has a valid
+ * one.  n_passes = 0 frees them.  Calling it again re-arms (and invalidates)
+ * the records. */
+int aur_keep_path_records(aur_ctx *ctx, int first_pass, int n_passes);
 
 /* ---- instrumentation ---- */
 /* Kernel timing, measured with HIP events recorded on the library stream

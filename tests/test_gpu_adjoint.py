@@ -232,3 +232,45 @@ def test_adjoint_at_inverse_rendering_size(tmp_path, oracle):
     np.testing.assert_allclose(gs, osp, rtol=2e-3, atol=2e-4 * np.abs(osp).max())
     np.testing.assert_allclose(gr, orr, rtol=5e-3, atol=1e-3 * np.abs(orr).max())
     r.close()
+
+
+def test_kept_records_replace_the_first_replay(kroom):
+    """A forward render that kept its per-path records lets the adjoint of the
+    same passes skip its first replay: the gradients equal those of the full
+    two-replay adjoint (up to the order of float atomic adds), the call traces
+    only the gradient replay's rays, and any change of inputs, or a pass range
+    the records do not cover, falls back to the full adjoint."""
+    r, albedo, rough, spec, w = kroom
+    r.set_tri_albedo(albedo)
+    r.set_material_params(rough, spec)
+    r.set_seed(29)
+    r.keep_path_records(0, 0)
+    full = r.adjoint(0, 8, w)
+    r.keep_path_records(0, 8)
+    r.render_passes(0, 8, fetch=False)
+    kept = r.adjoint(0, 8, w)
+    assert kept[3]["paths"] == full[3]["paths"]
+    assert kept[3]["ext_rays"] < full[3]["ext_rays"] and kept[3]["shadow_rays"] < full[3]["shadow_rays"]
+    for a, b in zip(kept[:3], full[:3]):
+        np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * np.abs(b).max())
+    # a sub-range of the kept passes is served from the records too
+    sub = r.adjoint(2, 4, w)
+    r.keep_path_records(0, 0)
+    sub_full = r.adjoint(2, 4, w)
+    np.testing.assert_allclose(sub[1], sub_full[1], rtol=1e-4, atol=1e-6 * np.abs(sub_full[1]).max())
+    assert sub[3]["ext_rays"] < sub_full[3]["ext_rays"]
+    # a new seed invalidates the records: the full adjoint runs
+    r.keep_path_records(0, 8)
+    r.render_passes(0, 8, fetch=False)
+    r.set_seed(31)
+    stale = r.adjoint(0, 8, w)
+    r.keep_path_records(0, 0)
+    fresh = r.adjoint(0, 8, w)
+    assert stale[3]["ext_rays"] == fresh[3]["ext_rays"]
+    np.testing.assert_allclose(stale[1], fresh[1], rtol=1e-4, atol=1e-6 * np.abs(fresh[1]).max())
+    # records of only part of the range: the full adjoint runs
+    r.keep_path_records(0, 8)
+    r.render_passes(0, 4, fetch=False)
+    part = r.adjoint(0, 8, w)
+    assert part[3]["ext_rays"] == fresh[3]["ext_rays"]
+    r.keep_path_records(0, 0)
