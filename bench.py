@@ -566,9 +566,31 @@ def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
         lat_single[k] = (time.perf_counter() - t0) * 1e3
         if k in check:
             worst = max(worst, float(np.max(np.abs(acc - fb[k]) / (np.abs(acc) + 1e-3))))
-        rs[k].close()
     single_s = time.perf_counter() - t0
     _progress("batch: per-scene launches %.2f s" % single_s)
+    # What batching can remove: a job's per-scene time is its paths' marginal
+    # cost plus a fixed part (launch gaps, the drain of its last bounces while
+    # the queues no longer fill the GPU).  Rendering each job at spp and at
+    # 2 spp passes separates the two: the marginal time of the second spp
+    # passes is what the job costs with the fixed part amortised, so
+    # sum(per-scene) / sum(marginal) bounds the speedup of any launch scheme
+    # that does not make the paths themselves cheaper.
+    t_once, t_twice = 0.0, 0.0
+    for k in order:
+        rs[k].render_passes(0, 2 * spp[k], fetch=False)  # grows the context's buffers untimed
+        _hip_sync()
+        t1 = time.perf_counter()
+        rs[k].render_passes(0, spp[k], fetch=False)
+        _hip_sync()
+        t2 = time.perf_counter()
+        rs[k].render_passes(0, 2 * spp[k], fetch=False)
+        _hip_sync()
+        t3 = time.perf_counter()
+        t_once += t2 - t1
+        t_twice += t3 - t2
+        rs[k].close()
+    marginal_s = max(t_twice - t_once, 1e-9)
+    _progress("batch: per-scene %.2f s, marginal %.2f s" % (t_once, marginal_s))
     # the reference's launch granularity: one launch per scene and pass
     # (Renderer::renderImage runs one parallel_for per pass), on the first
     # third of the jobs in launch order, rebuilt since the others were closed
@@ -622,7 +644,10 @@ def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
             "image_max_rel_diff_4_jobs": float("%.3g" % worst),
             "per_pass_launch_jobs": len(sub), "per_pass_launch_Mpaths_s": round(paths_pp / max(t_pp, 1e-9) / 1e6, 3),
             "batched_same_jobs_Mpaths_s": round(tot_sub["paths"] / max(t_bsub, 1e-9) / 1e6, 3),
-            "speedup_vs_per_pass": round(t_pp / max(t_bsub, 1e-9), 3), "per_pass_launch_paths": int(paths_pp)}
+            "speedup_vs_per_pass": round(t_pp / max(t_bsub, 1e-9), 3), "per_pass_launch_paths": int(paths_pp),
+            "bound": {"per_scene_s": round(t_once, 3), "marginal_s": round(marginal_s, 3),
+                      "max_speedup": round(t_once / marginal_s, 3),
+                      "batched_vs_marginal": round(batched_s / marginal_s, 3)}}
 
 
 def copy_calibration(mib: int, reps: int):
