@@ -117,25 +117,13 @@ def test_bsdf_parity(gpu_box_room, oracle):
     dg = np.array([r.sample_dir[:] for r in res]); do = np.array([x[2] for x in ref])
     sg = np.array([r.sample_pdf for r in res]); so = np.array([x[3] for x in ref])
     cg = np.array([r.sample_color[:] for r in res]); co = np.array([x[4] for x in ref])
-    # Both sides evaluate tan(acos(.)), exp(.), pow(.) with correctly rounded inputs
-    # but different libm implementations (each within an ulp).  Where the
-    # distribution's relative condition number kappa = |dlogD/dlog tan| =
-    # 2 tan^2 / m^2 is large, one input ulp moves D by kappa ulps; the tolerance
-    # is 1e-5 relative or that float envelope, whichever is larger.
-    def cond(q, d):
-        if q.kind == 0:
-            return 0.0
-        V, N = np.array(q.V[:]), np.array(q.N[:])
-        h = (V + d) / max(np.linalg.norm(V + d), 1e-12)
-        c = np.clip(abs(h @ N), 1e-6, 1.0)
-        t2 = (1 - c * c) / (c * c)
-        return 2 * t2 / q.roughness ** 2 + 3 * t2
-
-    kappa = [cond(q, np.array(q.L[:])) for q in qs]
-    tol = np.maximum(1e-5, 16 * 2.0 ** -24 * (1 + np.array(kappa)))
-    assert np.all(_rel(fg, fo, 1e-6 * fo.max()).max(1) < tol)
-    assert np.all(_rel(pg, po, 1e-6 * np.abs(po).max()) < tol)
-    assert np.mean(_rel(fg, fo, 1e-6 * fo.max()).max(1) < 1e-5) > 0.99
+    # bar (ii): eval and pdf within 1e-5 relative for every query.  The oracle
+    # evaluates the reference's tanf(acosf(c)); the device computes tan^2 as
+    # (1 - c)(1 + c) / c^2, which is closer to the correctly rounded value than
+    # the device's own tanf(acosf(c)) (on 20,000 such queries: largest error
+    # 9.5e-6 against 4.3e-5 with the device libm form).
+    assert np.all(_rel(fg, fo, 1e-6 * fo.max()).max(1) < 1e-5)
+    assert np.all(_rel(pg, po, 1e-6 * np.abs(po).max()) < 1e-5)
     assert np.max(np.abs(dg - do)) < 1e-5
     # getSample's pdf and colour depend on the sampled direction, whose last bits
     # come from cos/sin/atan/log (the directions agree to ~1e-6): the tolerance
