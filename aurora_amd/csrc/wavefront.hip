@@ -68,10 +68,7 @@ __device__ inline PathRngT<QMC> path_rng(const SceneView &sv, const WfView &wv, 
 __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
     int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_paths; i += stride) {
-        const PathRng rng = path_rng(sv, wv, i);
-        f3 d = camera_dir(sv, rng.pixel, rng.u(0), rng.u(1));
-        wv.ray[0][2 * i] = make_float4(0.f, 0.f, 0.f, 0.1f);
-        wv.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 10000000.f);
+        camera_ray_of(sv, path_rng(sv, wv, i), &wv.ray[0][2 * i], &wv.ray[0][2 * i + 1]);
         wv.T[0][i] = make_float4(1.f, 1.f, 1.f, 1.f);
         wv.La[0][i] = make_float4(0.f, 0.f, 0.f, 0.f);
         wv.Lb[0][i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -81,6 +78,15 @@ __global__ void k_generate(SceneView sv, WfView wv, int n_paths) {
         wv.counts[C_Q0] = n_paths;
         atomicAdd(&wv.stats[S_PATHS], (unsigned long long)n_paths);
     }
+}
+
+// Hash-sampled renders store no camera rays: the first extension kernel and
+// the first shade kernel recompute each path's camera ray from its id (the
+// 96 bytes per path k_generate would write and they would read back).  This
+// only sets the queue's size.
+__global__ void k_start(WfView wv, int n_paths) {
+    wv.counts[C_Q0] = n_paths;
+    atomicAdd(&wv.stats[S_PATHS], (unsigned long long)n_paths);
 }
 
 #ifdef AUR_TRAV_STATS
@@ -110,7 +116,7 @@ __device__ inline void trav_stats_add(int k, const TravStack &st) {
 // LDS per wave, so a CU keeps its full 32 waves of dependent node fetches in
 // flight.
 // ---------------------------------------------------------------------------
-template <int W, int B>
+template <int W, int B, bool CAM = false>
 __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_extend(SceneView sv, WfView wv, int q) {
     __shared__ int s_stack[AUR_STACK_LDS * B];
     const int n = wv.counts[q];
@@ -124,7 +130,9 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_extend(SceneView sv, W
     traverse_stream<W, false>(
         nodes, sv.tris, st, gtid, n, stride, AUR_REFILL_LANES,
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
-            const float4 a = wv.ray[q][2 * i], b = wv.ray[q][2 * i + 1];
+            float4 a, b;
+            if constexpr (CAM) camera_ray_of(sv, path_rng<false>(sv, wv, i), &a, &b);
+            else { a = wv.ray[q][2 * i]; b = wv.ray[q][2 * i + 1]; }
             o = mk3(a.x, a.y, a.z); d = mk3(b.x, b.y, b.z); tmin = a.w; tmax = b.w;
         },
         [&](int i, const TravHit &h) {
@@ -170,7 +178,7 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_shadow(SceneView sv, W
 // ---------------------------------------------------------------------------
 // 96 VGPRs (5 waves per SIMD) without the shader code; the shader, normal-map
 // and environment-map code needs more registers and runs at 4.
-template <bool TEX>
+template <bool TEX, bool CAM = false>
 __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView wv, int q) {
     __shared__ EmitScratch es;
     const int n = wv.counts[q];
@@ -181,7 +189,7 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView
         ShadeOut o;
         o.cont = o.finish = false;
         o.sh[0].valid = o.sh[1].valid = false;
-        if (i < n) shade_compute<TEX>(sv, wv, q, i, [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); }, 0, o);
+        if (i < n) shade_compute<TEX, CAM>(sv, wv, q, i, [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); }, 0, o);
         shade_emit(wv, q, i < n, o, es);
     }
 }
@@ -485,15 +493,29 @@ WfView wf_view(Ctx *c, const PathSource &src) {
 // One batch of n paths: generate, then maxdepth+1 rounds of
 // extend/shade/shadow/finish.  Results land in wf.out (by path id).
 static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
-    const int gen_blocks = std::min((n + 255) / 256, persistent_blocks(c, 256));
-    TIMED(c, KT_GENERATE, hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, sv, wv, n));
+    // hash-sampled paths recompute their camera rays where they are used
+    const bool cam_inline = sv.qmc == nullptr;
+    if (cam_inline) {
+        TIMED(c, KT_GENERATE, hipLaunchKernelGGL(k_start, dim3(1), dim3(1), 0, c->stream, wv, n));
+    } else {
+        const int gen_blocks = std::min((n + 255) / 256, persistent_blocks(c, 256));
+        TIMED(c, KT_GENERATE, hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, sv, wv, n));
+    }
     const int tb = trace_blocks(c), sb = persistent_blocks(c, 256);
+    const bool tex = shade_needs_tex(c);
     int q = 0;
     for (int it = 0; it <= sv.maxdepth; ++it) {
-#define AUR_EXT(w_, b_) hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q)
+        const bool cam = cam_inline && it == 0;
+#define AUR_EXT(w_, b_)                                                                                    \
+    do {                                                                                                   \
+        if (cam) hipLaunchKernelGGL((k_extend<w_, b_, true>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q); \
+        else hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q);          \
+    } while (0)
 #define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, 1 - q)
         TIMED(c, KT_EXTEND, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
-        if (shade_needs_tex(c)) TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<true>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
+        if (tex && cam) TIMED(c, KT_SHADE, hipLaunchKernelGGL((k_shade<true, true>), dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
+        else if (tex) TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<true>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
+        else if (cam) TIMED(c, KT_SHADE, hipLaunchKernelGGL((k_shade<false, true>), dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         else TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<false>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         TIMED(c, KT_SHADOW, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_SHD));
 #undef AUR_EXT

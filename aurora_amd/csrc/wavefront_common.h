@@ -135,6 +135,15 @@ __device__ inline f3 camera_dir(const SceneView &sv, int pixel, float r0, float 
     return norm3(d);
 }
 
+// The camera ray of a path as k_generate stores it: origin 0, tmin 0.1 and
+// tmax 1e7 (camera.cpp:58-59), direction through the path's pixel.
+template <class Rng>
+__device__ inline void camera_ray_of(const SceneView &sv, const Rng &rng, float4 *a, float4 *b) {
+    const f3 d = camera_dir(sv, rng.pixel, rng.u(0), rng.u(1));
+    *a = make_float4(0.f, 0.f, 0.f, 0.1f);
+    *b = make_float4(d.x, d.y, d.z, 10000000.f);
+}
+
 // integrator.cpp:277-281: the firefly clamp, skipped for camera rays that saw
 // the environment directly.
 __device__ inline f3 firefly_clamp(f3 L, int flags) {
@@ -166,16 +175,29 @@ struct ShadeOut {
 
 // TEX selects the code for shaders, normal maps, environment maps and the
 // Halton sample source; the host uses it only for scenes that need any.
-template <bool TEX, class MakeRng>
+// CAM: the camera vertex of a hash-sampled render, whose ray and state are
+// not stored (path i, its camera ray recomputed as camera_ray_of builds it
+// for k_extend, unit throughput, no radiance yet).
+template <bool TEX, bool CAM = false, class MakeRng>
 __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int q, int i, MakeRng make_rng, int job,
                                      ShadeOut &out) {
-    const int4 meta = wv.meta[q][i];
-    const float4 ra = wv.ray[q][2 * i], rb = wv.ray[q][2 * i + 1];
+    int4 meta;
+    float4 ra, rb, La, Lb, T4;
+    if constexpr (CAM) {
+        meta = make_int4(RT_CAMERA, i, 0, 0);
+        camera_ray_of(sv, make_rng(meta), &ra, &rb);
+        La = Lb = make_float4(0.f, 0.f, 0.f, 0.f);
+        T4 = make_float4(1.f, 1.f, 1.f, 1.f);
+    } else {
+        meta = wv.meta[q][i];
+        ra = wv.ray[q][2 * i];
+        rb = wv.ray[q][2 * i + 1];
+        La = wv.La[q][i];
+        Lb = wv.Lb[q][i];
+        T4 = wv.T[q][i];
+    }
     const float4 hv = wv.hit[i];
     const int prim = __float_as_int(hv.w);
-    float4 La = wv.La[q][i];
-    float4 Lb = wv.Lb[q][i];
-    float4 T4 = wv.T[q][i];
     int flags = meta.x;
     const int rtype = flags & 3;
     const int tb = (flags >> 8) & 0xff;
