@@ -180,6 +180,7 @@ SIGNATURES = {
     "aur_set_albedo_texture": (C.c_int, [_P, C.c_int, C.c_int, _FP, _IP, C.c_int]),
     "aur_build_accel": (C.c_int, [_P, C.POINTER(AccelOpts)]),
     "aur_scene_info_get": (C.c_int, [_P, C.POINTER(SceneInfo)]),
+    "aur_bvh_export": (C.c_int, [_P, C.c_void_p, _FP]),
     "aur_scene_export": (C.c_int, [_P, _FP, _IP, _FP, _IP, C.POINTER(Material), C.POINTER(Light)]),
     "aur_scene_export_shading": (C.c_int, [_P, _FP, C.POINTER(Shader), C.POINTER(Texture), _FP, _FP]),
     "aur_render": (C.c_int, [_P, C.c_int, C.c_int, _FP, C.POINTER(Stats)]),

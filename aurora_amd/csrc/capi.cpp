@@ -759,6 +759,21 @@ int aur_scene_info_get(aur_ctx *ctx, aur_scene_info *out) {
     });
 }
 
+int aur_bvh_export(aur_ctx *ctx, void *nodes, float *tri_records) {
+    return guarded([&] {
+        NEED(ctx);
+        Ctx *c = &ctx->c;
+        if (!c->built) throw StateError("no acceleration structure (call aur_build_accel first)");
+        set_device(c);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (nodes)
+            HIP_CHECK(hipMemcpy(nodes, c->bvh.nodes.p, (size_t)c->bvh.n_nodes * 32 * c->bvh.width, hipMemcpyDeviceToHost));
+        if (tri_records)
+            HIP_CHECK(hipMemcpy(tri_records, c->bvh.tri_sorted.p, sizeof(float) * 12 * (size_t)c->n_tris,
+                                hipMemcpyDeviceToHost));
+    });
+}
+
 int aur_scene_export(aur_ctx *ctx, float *verts, int *tris, float *normals, int *attr, aur_material *mats, aur_light *lights) {
     return guarded([&] {
         NEED(ctx);

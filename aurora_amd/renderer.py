@@ -129,6 +129,18 @@ class Renderer:
         L.check(self.lib.aur_set_albedo_texture(self._ctx, tex.shape[1], tex.shape[0], L.fptr(tex),
                                                 idx.ctypes.data_as(C.POINTER(C.c_int)), len(idx)))
 
+    def bvh(self) -> dict:
+        """The built wide BVH: per node the child boxes (n, 6, W) as lo x, hi x,
+        lo y, hi y, lo z, hi z rows, the child codes (n, W) and the triangle
+        records in BVH order (n_tris, 3, 4): v0 + global id bits, e1, e2."""
+        i = self.info()
+        n, w = i["bvh_nodes"], i["bvh_width"]
+        raw = np.zeros(n * 8 * w, np.int32)
+        tri = np.zeros((i["n_tris"], 3, 4), np.float32)
+        L.check(self.lib.aur_bvh_export(self._ctx, raw.ctypes.data_as(C.c_void_p), L.fptr(tri)))
+        raw = raw.reshape(n, 8, w)
+        return {"width": w, "boxes": raw[:, :6].view(np.float32), "child": raw[:, 6].copy(), "tris": tri}
+
     def info(self) -> dict:
         i = L.SceneInfo()
         L.check(self.lib.aur_scene_info_get(self._ctx, C.byref(i)))

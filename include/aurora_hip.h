@@ -251,6 +251,16 @@ int aur_set_tri_albedo(aur_ctx *ctx, const float *albedo_rgb);
  * Replaces EmbreeAccelerator's constructor (embreeAccelerator.cpp:19-35). */
 int aur_build_accel(aur_ctx *ctx, const aur_accel_opts *opts);
 int aur_scene_info_get(aur_ctx *ctx, aur_scene_info *out);
+/* The built acceleration structure, for inspection (Embree keeps its
+ * bvh4.spatialsplit internal, embreeAccelerator.cpp:19-35; no counterpart).
+ * nodes (bvh_nodes * 32 * bvh_width bytes, may be NULL): each node is six
+ * rows of bvh_width floats (lo x, hi x, lo y, hi y, lo z, hi z of every child
+ * box), bvh_width int child codes (>= 0 inner node index, 0x7fffffff empty,
+ * < 0 leaf: ~code = first << 4 | (count - 1), a range of triangle records)
+ * and bvh_width pad ints; node 0 is the root.  tri_records (12 * n_tris
+ * floats, may be NULL): per triangle in BVH order v0 (3 floats) and the
+ * global triangle id (int bits), e1 and 0, e2 and 0. */
+int aur_bvh_export(aur_ctx *ctx, void *nodes, float *tri_records);
 /* Copies the flattened scene to host buffers (any pointer may be NULL). */
 int aur_scene_export(aur_ctx *ctx, float *verts, int *tris, float *normals, int *attr,
                      aur_material *mats, aur_light *lights);

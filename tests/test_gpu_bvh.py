@@ -1,0 +1,96 @@
+"""The GPU-built wide BVH (the replacement of Embree's bvh4.spatialsplit
+build, embreeAccelerator.cpp:19-35) checked structurally: every node is
+reached once from the root, the leaves partition the triangle records, every
+triangle appears once with its own vertices, every child box contains what is
+below it, leaves respect the leaf size, and the surface-area cost of the tree
+is reported (traversal itself is checked against brute force in
+test_gpu_parity.py)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EMPTY = 0x7FFFFFFF
+
+
+def _check(r, max_leaf):
+    e = r.export()
+    b = r.bvh()
+    W, boxes, child, recs = b["width"], b["boxes"], b["child"], b["tris"]
+    n_nodes, n_tris = child.shape[0], recs.shape[0]
+    assert n_tris == e["tris"].shape[0]
+    # triangle records: the scene's triangles, each exactly once
+    ids = recs[:, 0, 3].view(np.int32)
+    assert np.array_equal(np.sort(ids), np.arange(n_tris))
+    V = e["verts"][e["tris"][ids]]  # (n, 3, 3) in record order
+    assert np.array_equal(recs[:, 0, :3], V[:, 0])
+    assert np.array_equal(recs[:, 1, :3], (V[:, 1] - V[:, 0]).astype(np.float32))
+    assert np.array_equal(recs[:, 2, :3], (V[:, 2] - V[:, 0]).astype(np.float32))
+    # walk the tree from the root
+    seen = np.zeros(n_nodes, np.int32)
+    covered = np.zeros(n_tris, np.int32)
+    area = lambda lo, hi: 2.0 * ((hi - lo)[..., 0] * (hi - lo)[..., 1] + (hi - lo)[..., 1] * (hi - lo)[..., 2]
+                                 + (hi - lo)[..., 2] * (hi - lo)[..., 0])
+    lo_all = np.stack([boxes[:, 0], boxes[:, 2], boxes[:, 4]], -1).astype(np.float64)  # (n, W, 3)
+    hi_all = np.stack([boxes[:, 1], boxes[:, 3], boxes[:, 5]], -1).astype(np.float64)
+    used = child != EMPTY
+    root_lo = lo_all[0][used[0]].min(0)
+    root_hi = hi_all[0][used[0]].max(0)
+    a_root = area(root_lo, root_hi)
+    sah_nodes, sah_tris, leaves, leaf_sizes = 0.0, 0.0, 0, []
+    stack = [0]
+    seen[0] = 1
+    while stack:
+        v = stack.pop()
+        for k in range(W):
+            c = int(child[v, k])
+            if c == EMPTY:
+                continue
+            lo, hi = lo_all[v, k], hi_all[v, k]
+            assert np.all(lo <= hi)
+            if c >= 0:
+                assert 0 < c < n_nodes
+                seen[c] += 1
+                stack.append(c)
+                cu = used[c]
+                # the child node's boxes lie inside this slot's box
+                assert np.all(lo_all[c][cu] >= lo) and np.all(hi_all[c][cu] <= hi), (v, k)
+                sah_nodes += area(lo, hi) / a_root
+            else:
+                code = ~c
+                first, cnt = code >> 4, (code & 15) + 1
+                assert cnt <= max_leaf and first + cnt <= n_tris
+                covered[first:first + cnt] += 1
+                leaves += 1
+                leaf_sizes.append(cnt)
+                pts = V[first:first + cnt].reshape(-1, 3)
+                assert np.all(pts >= lo) and np.all(pts <= hi), (v, k)
+                sah_tris += cnt * area(lo, hi) / a_root
+    assert np.all(seen == 1), "every node is reached exactly once"
+    assert np.all(covered == 1), "the leaves partition the triangle records"
+    # surface-area cost with unit node and triangle costs, root visit included
+    return {"nodes": n_nodes, "leaves": leaves, "mean_leaf": float(np.mean(leaf_sizes)),
+            "sah": 1.0 + sah_nodes + sah_tris, "sah_nodes": 1.0 + sah_nodes, "sah_tris": sah_tris}
+
+
+@pytest.fixture(scope="module")
+def spheres(tmp_path_factory):
+    from aurora_amd import scenes
+    d = str(tmp_path_factory.mktemp("bvh_spheres"))
+    return scenes.displaced_spheres(width=64, height=36, n_spheres=4, levels=4).write(d)
+
+
+@pytest.mark.parametrize("width,max_leaf", [(4, 1), (4, 2), (4, 4), (8, 2), (8, 8)])
+def test_bvh_structure(spheres, gpu_box_room, width, max_leaf):
+    from aurora_amd.renderer import Renderer
+    r = Renderer(spheres)
+    r.build(width=width, max_leaf=max_leaf)
+    s = _check(r, max_leaf)
+    print("BVH%d leaf<=%d spheres: %s" % (width, max_leaf, s), flush=True)
+    r.close()
+    gpu_box_room.build(width=width, max_leaf=max_leaf)
+    s = _check(gpu_box_room, max_leaf)
+    print("BVH%d leaf<=%d box room: %s" % (width, max_leaf, s), flush=True)
+    gpu_box_room.build()
