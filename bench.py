@@ -650,6 +650,49 @@ def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
                       "batched_vs_marginal": round(batched_s / marginal_s, 3)}}
 
 
+def copy_peak_gbps(mib: int = 2048, reps: int = 5) -> float:
+    """Achievable HBM bandwidth on this GPU: a device-to-device copy of `mib`
+    MiB (read + write bytes per second), best of `reps`, timed with HIP
+    events on the null stream."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    for f, args in (("hipMalloc", [ctypes.POINTER(vp), ctypes.c_size_t]), ("hipFree", [vp]),
+                    ("hipMemcpy", [vp, vp, ctypes.c_size_t, ctypes.c_int]),
+                    ("hipEventCreate", [ctypes.POINTER(vp)]), ("hipEventRecord", [vp, vp]),
+                    ("hipEventSynchronize", [vp]), ("hipEventDestroy", [vp]),
+                    ("hipEventElapsedTime", [ctypes.POINTER(ctypes.c_float), vp, vp])):
+        getattr(hip, f).restype = ctypes.c_int
+        getattr(hip, f).argtypes = args
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError("HIP error %d in the copy measurement" % rc)
+
+    n = mib << 20
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    ok(hip.hipMalloc(ctypes.byref(a), n))
+    ok(hip.hipMalloc(ctypes.byref(b), n))
+    ok(hip.hipEventCreate(ctypes.byref(e0)))
+    ok(hip.hipEventCreate(ctypes.byref(e1)))
+    best = float("inf")
+    try:
+        ok(hip.hipMemcpy(b, a, n, 3))  # hipMemcpyDeviceToDevice, warm
+        for _ in range(reps):
+            ok(hip.hipEventRecord(e0, None))
+            ok(hip.hipMemcpy(b, a, n, 3))
+            ok(hip.hipEventRecord(e1, None))
+            ok(hip.hipEventSynchronize(e1))
+            ms = ctypes.c_float()
+            ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1))
+            best = min(best, ms.value)
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * n / (best * 1e-3) / 1e9
+
+
 def copy_calibration(mib: int, reps: int):
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
     counters to bytes."""
@@ -744,10 +787,13 @@ def main(argv=None):
     per_launch_bytes = ALGO_BYTES.get(dom, 0.0) * items / max(launches, 1)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic, traffic_src = _load_traffic(dom)
+    copy_peak = copy_peak_gbps()
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom, "avg_launch_ms": round(avg_ms, 4), "launches": launches,
-                "algorithmic_bytes_per_launch": round(per_launch_bytes, 1)}
+                "algorithmic_bytes_per_launch": round(per_launch_bytes, 1),
+                "copy_peak_measured": round(copy_peak, 1),
+                "frac_of_copy_peak": round(achieved / copy_peak, 5)}
     algo_total = 24 * paths + 168 * tot["ext_rays"] + 112 * tot["hit_rays"] + 96 * tot["shadow_rays"]
 
     out = {
