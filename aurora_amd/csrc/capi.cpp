@@ -511,6 +511,13 @@ void aur_destroy(aur_ctx *ctx) {
         }
         hipStream_t s = ctx->c.stream;
         bool own = ctx->c.own_stream;
+        if (ctx->c.side) {
+            (void)hipStreamSynchronize(ctx->c.side);
+            (void)hipStreamDestroy(ctx->c.side);
+            (void)hipEventDestroy(ctx->c.ev_fork);
+            (void)hipEventDestroy(ctx->c.ev_shaded);
+            (void)hipEventDestroy(ctx->c.ev_done);
+        }
         delete ctx;
         if (own && s) (void)hipStreamDestroy(s);
     } catch (...) {

@@ -199,6 +199,12 @@ struct Ctx {
     size_t rec_wh = 0;
     std::vector<uint64_t> rec_version;
 
+    // Second stream of the wavefront: a bounce's shadow rays and finished
+    // paths are traced on it while the next bounce's extension rays are
+    // traced on `stream` (created on first use, default priority).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_shaded = nullptr, ev_done = nullptr;
+
     Wavefront wf;
     KernelTimer timer;
     LaunchCfg launch;
@@ -206,8 +212,8 @@ struct Ctx {
 };
 
 // kernel timing helpers (wavefront.hip)
-void timer_begin(Ctx *c, int kernel, hipEvent_t *start);
-void timer_end(Ctx *c, int kernel, hipEvent_t start);
+void timer_begin(Ctx *c, int kernel, hipEvent_t *start, hipStream_t stream);
+void timer_end(Ctx *c, int kernel, hipEvent_t start, hipStream_t stream);
 void timer_collect(Ctx *c);
 void timer_release(Ctx *c);
 extern const char *const kKernelNames[KT_COUNT];

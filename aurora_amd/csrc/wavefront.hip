@@ -18,7 +18,10 @@
 // Queues are compacted with one atomic per wave: a 64-lane ballot gives each
 // lane its offset (mbcnt) and the wave's count (popcount).  Every kernel is
 // a grid-stride loop over a count that lives in device memory, so a whole
-// frame is a fixed sequence of launches with no host round trip.
+// frame is a fixed sequence of launches with no host round trip.  Shadow and
+// finish of bounce b run on a second stream beside extend of bounce b + 1
+// (events order them before shade of bounce b + 1), so the tail of one
+// traversal kernel overlaps the start of the next.
 #include "wavefront_common.h"
 
 #include <mutex>
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_shadow(SceneView sv, W
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     TravStack st;
-    st.init(s_stack, B, wv.ovf, wv.ovf_per_thread);
+    st.init(s_stack, B, wv.sovf, wv.ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     for (int i = gtid; i < n; i += stride) {
         float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
@@ -337,18 +340,18 @@ static hipEvent_t timer_event(Ctx *c) {
     return e;
 }
 
-void timer_begin(Ctx *c, int kernel, hipEvent_t *start) {
+void timer_begin(Ctx *c, int kernel, hipEvent_t *start, hipStream_t stream) {
     (void)kernel;
     *start = nullptr;
     if (!c->timer.on) return;
     *start = timer_event(c);
-    HIP_CHECK(hipEventRecord(*start, c->stream));
+    HIP_CHECK(hipEventRecord(*start, stream));
 }
 
-void timer_end(Ctx *c, int kernel, hipEvent_t start) {
+void timer_end(Ctx *c, int kernel, hipEvent_t start, hipStream_t stream) {
     if (!c->timer.on || !start) return;
     hipEvent_t e = timer_event(c);
-    HIP_CHECK(hipEventRecord(e, c->stream));
+    HIP_CHECK(hipEventRecord(e, stream));
     c->timer.pending.push_back({kernel, {start, e}});
 }
 
@@ -372,13 +375,14 @@ void timer_release(Ctx *c) {
     c->timer.pool.clear();
 }
 
-#define TIMED(c, kid, launch)            \
-    do {                                 \
-        hipEvent_t ev_;                  \
-        timer_begin(c, kid, &ev_);       \
-        launch;                          \
-        timer_end(c, kid, ev_);          \
+#define TIMED_ON(c, kid, strm, ...)       \
+    do {                                  \
+        hipEvent_t ev_;                   \
+        timer_begin(c, kid, &ev_, strm);  \
+        __VA_ARGS__;                      \
+        timer_end(c, kid, ev_, strm);     \
     } while (0)
+#define TIMED(c, kid, ...) TIMED_ON(c, kid, (c)->stream, __VA_ARGS__)
 
 // ---------------------------------------------------------------------------
 // host driver
@@ -436,7 +440,7 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     int need = (width - 1) * (depth + 1) + 2;
     size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 0;
     if (cap <= w.cap && ovf <= w.stack_ovf_per_thread && w.counts.p &&
-        w.stack_ovf.bytes >= sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1) &&
+        w.stack_ovf.bytes >= 2 * sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1) &&
         w.partials.bytes >= sizeof(unsigned long long) * trace_blocks(c))
         return;
     cap = std::max(cap, w.cap);
@@ -457,7 +461,8 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     w.counts.reserve(sizeof(int) * 16);
     w.stats.reserve(sizeof(unsigned long long) * 8);
     w.stack_ovf_per_thread = std::max(ovf, w.stack_ovf_per_thread);
-    w.stack_ovf.reserve(sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1));
+    // two regions: extension and shadow traversals may run at the same time
+    w.stack_ovf.reserve(2 * sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1));
     w.partials.reserve(sizeof(unsigned long long) * trace_blocks(c));
 }
 
@@ -484,6 +489,7 @@ WfView wf_view(Ctx *c, const PathSource &src) {
     v.partials = w.partials.as<unsigned long long>();
     v.ovf = w.stack_ovf.as<int>();
     v.ovf_per_thread = (int)w.stack_ovf_per_thread;
+    v.sovf = v.ovf + trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1);
     v.src = src;
     v.seed = c->seed;
     v.nslots = 2 + AUR_SLOTS_PER_VERTEX * c->scene.opt.maxdepth;
@@ -503,6 +509,24 @@ static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
     }
     const int tb = trace_blocks(c), sb = persistent_blocks(c, 256);
     const bool tex = shade_needs_tex(c);
+    // A bounce's shadow rays and finished paths go to the side stream, so
+    // they run beside the next bounce's extension rays, which read and write
+    // none of their buffers (the shadow traversal has its own stack overflow
+    // region); the next shade waits for them, and the caller's stream for
+    // the last of them.  With kernel timing on, everything stays on the
+    // context's stream so that each kernel's time is its own.
+    const bool overlap = !c->timer.on;
+    if (overlap && !c->side) {
+        HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_shaded, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
+    }
+    hipStream_t s2 = overlap ? c->side : c->stream;
+    if (overlap) {
+        HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
+        HIP_CHECK(hipStreamWaitEvent(s2, c->ev_fork, 0));
+    }
     int q = 0;
     for (int it = 0; it <= sv.maxdepth; ++it) {
         const bool cam = cam_inline && it == 0;
@@ -511,19 +535,26 @@ static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
         if (cam) hipLaunchKernelGGL((k_extend<w_, b_, true>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q); \
         else hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q);          \
     } while (0)
-#define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, 1 - q)
+#define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, s2, sv, wv, 1 - q)
         TIMED(c, KT_EXTEND, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
+        if (overlap && it > 0) HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_done, 0));
         if (tex && cam) TIMED(c, KT_SHADE, hipLaunchKernelGGL((k_shade<true, true>), dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         else if (tex) TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<true>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         else if (cam) TIMED(c, KT_SHADE, hipLaunchKernelGGL((k_shade<false, true>), dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
         else TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<false>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
-        TIMED(c, KT_SHADOW, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_SHD));
+        if (overlap) {
+            HIP_CHECK(hipEventRecord(c->ev_shaded, c->stream));
+            HIP_CHECK(hipStreamWaitEvent(s2, c->ev_shaded, 0));
+        }
+        TIMED_ON(c, KT_SHADOW, s2, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_SHD));
 #undef AUR_EXT
 #undef AUR_SHD
-        TIMED(c, KT_FINISH, hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, c->stream, sv, wv));
-        hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, c->stream, wv.counts, q);
+        TIMED_ON(c, KT_FINISH, s2, hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, s2, sv, wv));
+        hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, s2, wv.counts, q);
+        if (overlap) HIP_CHECK(hipEventRecord(c->ev_done, s2));
         q = 1 - q;
     }
+    if (overlap) HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_done, 0));
     HIP_CHECK(hipGetLastError());
 }
 

@@ -65,7 +65,8 @@ struct WfView {
     int *counts;
     unsigned long long *stats;
     unsigned long long *partials;  // per traversal workgroup: hit rays (summed after the run)
-    int *ovf;
+    int *ovf;                      // traversal stack overflow: extension rays (and the batched kernels)
+    int *sovf;                     // and shadow rays, which may run beside them
     int ovf_per_thread;
     PathSource src;
     uint32_t seed;

@@ -11,8 +11,11 @@ renders passes (k * P) mod 64 onward, so with fewer passes per step the steps
 walk through the frame's passes.
 
 Prints one JSON line.  ``value`` is Mpaths/s of the timed steps with the scene
-and BVH resident in HBM.  ``roofline`` describes the kernel with the largest
-share of device time, timed with HIP events on the library stream.
+and BVH resident in HBM (a bounce's shadow and finish kernels run on a second
+stream beside the next bounce's extension kernel).  ``roofline`` describes the
+kernel with the largest share of device time, timed with HIP events on its
+stream in one more step whose kernels run one after another, so that each
+kernel's time is its own.
 ``cpu_baseline`` is the CPU restatement of the reference algorithm (the
 oracle, with its own SAH BVH and OpenMP tiles) timed on this host on a bounded
 sample of the same frame.  ``adjoint`` is the inverse-rendering workload (the
@@ -440,11 +443,14 @@ def large_leg(fwd_passes: int = 256, adj_passes: int = 64, fd_passes: int = 1, n
     # the call is its first replay (radiance per path), the rest its gradient replay
     r.adjoint(0, 1, dl, albedo=False, texture=True)
     _hip_sync()
-    r.set_profiling(True)
     t0 = time.perf_counter()
     _, _, _, ast, gt_rate = r.adjoint(0, adj_passes, dl, albedo=False, texture=True)
     _hip_sync()
     adj_s = time.perf_counter() - t0
+    # the split, from the same adjoint with kernel timing on (kernels one
+    # after another)
+    r.set_profiling(True)
+    _, _, _, ast_p, _ = r.adjoint(0, adj_passes, dl, albedo=False, texture=True)
     first_replay_ms = sum(v["total_ms"] for v in r.kernel_stats().values())
     r.set_profiling(False)
     _progress("large scene: %d adjoint passes in %.2f s" % (adj_passes, adj_s))
@@ -494,7 +500,7 @@ def large_leg(fwd_passes: int = 256, adj_passes: int = 64, fd_passes: int = 1, n
             "adjoint_passes": adj_passes, "adjoint_Mpaths_s": round(ast["paths"] / adj_s / 1e6, 3),
             "adjoint_vs_forward": round((adj_s / adj_passes) / (fwd_s / fwd_passes), 3),
             "adjoint_first_replay_ms": round(first_replay_ms, 1),
-            "adjoint_gradient_replay_ms": round(ast["ms"] - first_replay_ms, 1),
+            "adjoint_gradient_replay_ms": round(ast_p["ms"] - first_replay_ms, 1),
             "kept_records": {"forward_Mpaths_s": round(fst["paths"] / fwd_k / 1e6, 3),
                              "adjoint_Mpaths_s": round(kst["paths"] / adj_k / 1e6, 3),
                              "adjoint_vs_forward": round((adj_k / adj_passes) / (fwd_k / fwd_passes), 3),
@@ -726,6 +732,8 @@ def main(argv=None):
                     help="batched launch without graph replay (for kernel tracing)")
     ap.add_argument("--adam-steps", type=int, default=200,
                     help="Adam steps of the inverse-rendering run inside the adjoint leg (0: skip)")
+    ap.add_argument("--serial-timed", action="store_true",
+                    help="time the steps with kernel timing on (kernels one after another)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -762,9 +770,17 @@ def main(argv=None):
     def first(k):
         return (k * P) % spp
 
+    if args.serial_timed:
+        r.set_profiling(True)  # the warm-up too, so that a trace holds one kind of step
     for k in range(args.warmup):
         r.render_passes(first(k), P, fetch=False)
-    r.set_profiling(True)
+    # the timed steps run as a render does (a bounce's shadow and finish
+    # kernels beside the next bounce's extension kernel); with kernel timing
+    # on the kernels run one after another, so the per-kernel times of the
+    # roofline come from one more step timed that way (--serial-timed: the
+    # timed steps themselves, e.g. under a profiler's kernel trace)
+    if args.serial_timed:
+        r.set_profiling(True)  # resets the totals after the warm-up
     _hip_sync()
     t0 = time.perf_counter()
     tot = {"paths": 0, "ext_rays": 0, "hit_rays": 0, "shadow_rays": 0}
@@ -775,6 +791,11 @@ def main(argv=None):
     _hip_sync()
     elapsed = time.perf_counter() - t0
     _progress("timed %d steps in %.3f s" % (args.steps, elapsed))
+    kt = dict(tot)
+    if not args.serial_timed:
+        r.set_profiling(True)
+        _, kt = r.render_passes(first(args.steps), P, fetch=False)
+        _hip_sync()
     ks = r.kernel_stats()
     r.set_profiling(False)
 
@@ -783,7 +804,7 @@ def main(argv=None):
     dom = max((k for k in ks if ks[k]["launches"] > 0), key=lambda k: ks[k]["total_ms"])
     launches = ks[dom]["launches"]
     avg_ms = ks[dom]["total_ms"] / max(launches, 1)
-    items = {"k_extend": tot["ext_rays"], "k_shadow": tot["shadow_rays"], "k_shade": tot["ext_rays"]}.get(dom, 0)
+    items = {"k_extend": kt["ext_rays"], "k_shadow": kt["shadow_rays"], "k_shade": kt["ext_rays"]}.get(dom, 0)
     per_launch_bytes = ALGO_BYTES.get(dom, 0.0) * items / max(launches, 1)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic, traffic_src = _load_traffic(dom)
@@ -809,6 +830,8 @@ def main(argv=None):
         "counters": tot,
         "path_algorithmic_GBps": round(algo_total / elapsed / 1e9, 2),
         "kernel_ms": {k: round(v["total_ms"], 3) for k, v in ks.items()},
+        "kernel_timing": ("the timed steps, kernels one after another" if args.serial_timed else
+                          "one more %d-pass step after the timed ones, kernels one after another" % P),
         "roofline": roofline,
         "cpu_baseline": None,
     }

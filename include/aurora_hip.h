@@ -419,7 +419,10 @@ typedef struct {
     int64_t launches;
     double total_ms;
 } aur_kernel_stat;
-/* enable != 0 turns timing on and resets the totals. */
+/* enable != 0 turns timing on and resets the totals.  While timing is on,
+ * the tracer's kernels run one after another on the context's stream
+ * (normally a bounce's shadow and finish kernels overlap the next bounce's
+ * extension kernel on a second stream), so each kernel's time is its own. */
 int aur_set_profiling(aur_ctx *ctx, int enable);
 int aur_kernel_stats(aur_ctx *ctx, aur_kernel_stat *out, int max, int *n_out);
 
