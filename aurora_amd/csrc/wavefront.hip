@@ -444,7 +444,9 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
         w.partials.bytes >= sizeof(unsigned long long) * trace_blocks(c))
         return;
     cap = std::max(cap, w.cap);
-    w.cap = cap;
+    // a failed allocation leaves some buffers released: forget the capacity
+    // so that the next call allocates them all again
+    w.cap = 0;
     for (int k = 0; k < 2; ++k) {
         w.ray[k].reserve(sizeof(float4) * 2 * cap);
         w.T[k].reserve(sizeof(float4) * cap);
@@ -464,6 +466,7 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     // two regions: extension and shadow traversals may run at the same time
     w.stack_ovf.reserve(2 * sizeof(int) * trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1));
     w.partials.reserve(sizeof(unsigned long long) * trace_blocks(c));
+    w.cap = cap;
 }
 
 static void ensure_wavefront(Ctx *c, size_t cap) { ensure_wavefront_for(c, cap, c->bvh.width, c->bvh.depth); }

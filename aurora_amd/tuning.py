@@ -24,6 +24,8 @@ import math
 import os
 import time
 
+from . import _lib as L
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 DEFAULT = {"bvh_width": 4, "max_leaf": 2, "trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 128 << 20}
@@ -33,7 +35,7 @@ LEAVES = (1, 2, 4, 8)
 BLOCKS = (64, 128, 256)
 WAVES = (8, 16, 24, 32)
 POOLS = (2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20)
-SINGLE_POOLS = (32 << 20, 64 << 20, 128 << 20)
+SINGLE_POOLS = (32 << 20, 64 << 20, 128 << 20, 256 << 20, 320 << 20)
 
 
 def _bucket(n_tris: int) -> str:
@@ -123,8 +125,13 @@ def autotune(renderer, passes: int = 2, repeats: int = 3, table: str = TABLE) ->
     n_pass = max(2, min(64, (SINGLE_POOLS[-1] + wh - 1) // wh))
     pools = []
     for pool in SINGLE_POOLS:
+        if pool > SINGLE_POOLS[2] and n_pass * wh <= SINGLE_POOLS[2]:
+            continue  # the timed passes already fit one 128 M chunk
         renderer.set_launch(pool_paths=pool)
-        pools.append({"pool_paths": pool, "mpaths_per_s": _time_render(renderer, n_pass, 2)})
+        try:
+            pools.append({"pool_paths": pool, "mpaths_per_s": _time_render(renderer, n_pass, 2)})
+        except L.AuroraError:
+            break  # 368 bytes of path state per slot: a pool the device memory cannot hold ends the sweep
     pools.sort(key=lambda r: -r["mpaths_per_s"])
     entry = {"bvh_width": best["bvh_width"], "max_leaf": best["max_leaf"], "trace_block": launch[0]["trace_block"],
              "trace_waves_per_cu": launch[0]["trace_waves_per_cu"], "pool_paths": pools[0]["pool_paths"],
