@@ -56,7 +56,7 @@ class Material(_Entry):
         an extension of the loader, without a PyAurora counterpart."""
         if distribution.lower() not in ("beckmann", "ggx"):
             raise ValueError("unknown microfacet distribution: " + distribution)
-        self._s["distribution"] = distribution
+        self._s["distribution"] = distribution.lower()
         if "distribution" not in self._serialize:
             self._serialize.append("distribution")
 
