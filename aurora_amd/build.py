@@ -16,6 +16,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaurora_hip.so")
+HOST = os.path.join(HERE, "host")
+HOST_BIN = os.path.join(HOST, "aurora")
+# a C++ program of the test suite that drives the host classes
+HOST_CHECK_SRC = os.path.join(HERE, "..", "tests", "cpp", "host_api_check.cpp")
+HOST_CHECK_BIN = os.path.join(HERE, "..", "tests", "cpp", "host_api_check")
 ARCH = os.environ.get("AURORA_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.cpp", "host_scene.cpp", "exr_io.cpp", "halton.cpp", "bvh_build.hip", "wavefront.hip", "parity.hip", "subdiv.hip", "adjoint.hip", "batch.hip"]
@@ -71,9 +76,33 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stdout))
     os.replace(tmp, LIB)
+    build_host()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", HOST_BIN)
     return LIB
+
+
+def _link_host(src: str, out: str) -> str:
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    tmp = out + ".tmp"
+    rpath = os.path.relpath(HERE, os.path.dirname(out))
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include"),
+           src, "-o", tmp, "-L", HERE, "-laurora_hip", "-Wl,-rpath,$ORIGIN/" + rpath]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0 or r.stdout.strip():
+        raise RuntimeError("host build %s: %s\n%s" % ("failed" if r.returncode else "warned", " ".join(cmd), r.stdout))
+    os.replace(tmp, out)
+    return out
+
+
+def build_host() -> str:
+    """Links the C++ command line (host/aurora_main.cpp) and the test suite's
+    C++ program against the library with the host compiler; the binaries find
+    the library through their rpath."""
+    _link_host(os.path.join(HOST, "aurora_main.cpp"), HOST_BIN)
+    if os.path.exists(HOST_CHECK_SRC):
+        _link_host(HOST_CHECK_SRC, HOST_CHECK_BIN)
+    return HOST_BIN
 
 
 if __name__ == "__main__":
