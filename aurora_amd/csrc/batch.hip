@@ -209,6 +209,7 @@ __global__ void k_arena_nodes(const WNode<W> *__restrict__ src, int n, int node_
             nd.child[k] = ~((((enc >> 4) + tri_off) << 4) | (enc & 15));
         }
     }
+    nd.pad[0] += node_off;  // first inner child
     dst[node_off + i] = nd;
 }
 

@@ -14,7 +14,8 @@
 //   6. top-down collapse into W-wide nodes (k_collapse), level by level: each
 //      wide node opens the child with the largest surface area until it has W
 //      children; a subtree with <= max_leaf triangles becomes one leaf (a
-//      contiguous range of the sorted triangles)
+//      contiguous range of the sorted triangles); a node's inner children
+//      take its first slots and consecutive node indices from pad[0]
 //   7. triangle records in sorted order (k_reorder): (v0 | prim), e1, e2 so a
 //      leaf is read with contiguous 16-byte loads
 //
@@ -164,6 +165,21 @@ __global__ void k_collapse(CollapseIn in, const int2 *__restrict__ tasks, const 
         ch[0] = code;
         nc = 1;
     }
+    // inner children first, in one block of consecutive node indices
+    // (pad[0] = the block's first index), so traversal names an inner child
+    // by its slot: child = pad[0] + slot
+    int n_inner = 0;
+    {
+        int ord[W];
+        for (int k = 0; k < nc; ++k)
+            if (!(ch[k] & LEAF_BIT) && code_count(in, ch[k]) > in.max_leaf) ord[n_inner++] = ch[k];
+        int j = n_inner;
+        for (int k = 0; k < nc; ++k)
+            if ((ch[k] & LEAF_BIT) || code_count(in, ch[k]) <= in.max_leaf) ord[j++] = ch[k];
+        for (int k = 0; k < nc; ++k) ch[k] = ord[k];
+    }
+    const int base = n_inner ? atomicAdd(node_count, n_inner) : 0;
+    const int q0 = n_inner ? atomicAdd(n_next, n_inner) : 0;
     WNode<W> nd;
     for (int k = 0; k < W; ++k) {
         if (k >= nc) {
@@ -179,15 +195,14 @@ __global__ void k_collapse(CollapseIn in, const int2 *__restrict__ tasks, const 
         nd.hix[k] = h.x; nd.hiy[k] = h.y; nd.hiz[k] = h.z;
         nd.pad[k] = 0;
         int cnt = code_count(in, c);
-        if (!(c & LEAF_BIT) && cnt > in.max_leaf) {
-            int idx = atomicAdd(node_count, 1);
-            int q = atomicAdd(n_next, 1);
-            next[q] = make_int2(c, idx);
-            nd.child[k] = idx;
+        if (k < n_inner) {
+            next[q0 + k] = make_int2(c, base + k);
+            nd.child[k] = base + k;
         } else {
             nd.child[k] = ~((code_first(in, c) << 4) | (cnt - 1));
         }
     }
+    nd.pad[0] = base;
     nodes[task.y] = nd;
     atomicMax(max_depth, depth + 1);
 }

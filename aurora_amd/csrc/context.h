@@ -67,6 +67,8 @@ struct DevBuf {
 
 // Wide-BVH node: W boxes as SoA rows + W child codes (see bvh_build.hip).
 // child >= 0: inner node index; child < 0: leaf, ~child = (first << 4) | (count - 1).
+// Inner children occupy the first slots and consecutive indices:
+// child[k] = pad[0] + k for each inner slot k; pad[1..] are zero.
 template <int W> struct alignas(16) WNode {
     float lox[W], hix[W], loy[W], hiy[W], loz[W], hiz[W];
     int child[W];

@@ -34,6 +34,7 @@ template <int W>
 struct NodeRegs {
     float lox[W], hix[W], loy[W], hiy[W], loz[W], hiz[W];
     int child[W];
+    int base;  // index of the first inner child: inner slot k is node base + k
 };
 
 template <int W>
@@ -52,6 +53,7 @@ __device__ inline void load_node(const WNode<W> *__restrict__ nodes, int ni, Nod
         r.hiz[4 * q + 0] = f.x; r.hiz[4 * q + 1] = f.y; r.hiz[4 * q + 2] = f.z; r.hiz[4 * q + 3] = f.w;
         r.child[4 * q + 0] = g.x; r.child[4 * q + 1] = g.y; r.child[4 * q + 2] = g.z; r.child[4 * q + 3] = g.w;
     }
+    r.base = nodes[ni].pad[0];
 }
 
 // Stack: top STACK_LDS entries per lane in LDS, deeper entries in global
@@ -143,6 +145,27 @@ __device__ inline void trav_begin(TravRay &r, f3 o, f3 d, float tmin, float tmax
     node = root;
 }
 
+// Ascending sort of W keys by a min/max network (Batcher's odd-even merge
+// sort: 5 comparators for 4 keys, 19 for 8).
+template <int W>
+__device__ inline void sort_keys(unsigned (&k)[W]) {
+    auto ce = [&](int i, int j) {
+        const unsigned lo = min(k[i], k[j]), hi = max(k[i], k[j]);
+        k[i] = lo;
+        k[j] = hi;
+    };
+    if constexpr (W == 4) {
+        ce(0, 1); ce(2, 3); ce(0, 2); ce(1, 3); ce(1, 2);
+    } else {
+        ce(0, 1); ce(2, 3); ce(4, 5); ce(6, 7);
+        ce(0, 2); ce(1, 3); ce(4, 6); ce(5, 7);
+        ce(1, 2); ce(5, 6);
+        ce(0, 4); ce(1, 5); ce(2, 6); ce(3, 7);
+        ce(2, 4); ce(3, 5);
+        ce(1, 2); ce(3, 4); ce(5, 6);
+    }
+}
+
 // Visits one node: tests its child boxes, tests the leaves they let through,
 // pushes the inner children to visit later and moves `node` to the next one.
 // Returns true when the ray is done (stack empty, or an any-hit query hit).
@@ -155,7 +178,13 @@ __device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restric
     // entry distance per inner child (inf: missed, or a leaf: leaves are
     // collected in a mask and tested after the box tests); every index
     // below is a compile-time constant, so the lists stay in VGPRs
+    // Closest hit orders the inner children by 32-bit keys: the entry
+    // distance's bits (a non-negative float orders as an unsigned integer)
+    // with the slot in the low bits, so the sort moves one register per
+    // child and the child is base + slot.  Missed children get ~0u.
+    constexpr unsigned SLOT = W == 4 ? 3u : 7u;
     float tn[W];
+    unsigned key[W];
     int ch[W];
     unsigned leaves = 0;
 #pragma unroll
@@ -167,7 +196,9 @@ __device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restric
         float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
         int c = nr.child[k];
         bool hit = a <= b;
-        tn[k] = (hit && c >= 0) ? a : inf;
+        if (ANY) tn[k] = (hit && c >= 0) ? a : inf;
+        // a >= tmin >= 0: clearing the sign bit only turns -0 into +0
+        else key[k] = (hit && c >= 0) ? ((__float_as_uint(a) & 0x7fffffffu & ~SLOT) | (unsigned)k) : ~0u;
         ch[k] = c;
         if (hit && c < 0) leaves |= 1u << k;
     }
@@ -188,30 +219,32 @@ __device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restric
 #endif
         if (test_leaf<ANY>(tris, c, r.o, r.d, r.tmin, h)) return true;
     }
-    if (!ANY) {
-        // sorting network on (entry distance, child): nearest first
+    if (ANY) {
+        // any hit: children in slot order
+        const float tcut = h.t * 1.0000004f;
+        int next = -1;
 #pragma unroll
-        for (int i = 0; i < W; ++i) {
-#pragma unroll
-            for (int j = (i & 1); j + 1 < W; j += 2) {
-                bool sw = tn[j] > tn[j + 1];
-                float ta = sw ? tn[j + 1] : tn[j], tb = sw ? tn[j] : tn[j + 1];
-                int ca = sw ? ch[j + 1] : ch[j], cb = sw ? ch[j] : ch[j + 1];
-                tn[j] = ta; tn[j + 1] = tb; ch[j] = ca; ch[j + 1] = cb;
+        for (int k = W - 1; k >= 0; --k) {
+            if (tn[k] <= tcut) {
+                if (next >= 0) st.push(next);
+                next = ch[k];
             }
         }
-    }
-    // children entered beyond the (possibly shortened) hit distance are skipped
-    const float tcut = h.t * 1.0000004f;
-    int next = -1;
+        if (next >= 0) { node = next; return false; }
+    } else {
+        sort_keys<W>(key);
+        // children entered beyond the (possibly shortened) hit distance are
+        // skipped; the slot bits and the cleared low distance bits only ever
+        // admit more children (ties and a few ulps), never fewer
+        const unsigned kcut = __float_as_uint(fmaxf(h.t * 1.0000004f, 0.f)) | SLOT;
+        if (key[0] <= kcut) {
 #pragma unroll
-    for (int k = W - 1; k >= 0; --k) {
-        if (tn[k] <= tcut) {
-            if (next >= 0) st.push(next);
-            next = ch[k];
+            for (int k = W - 1; k >= 1; --k)
+                if (key[k] <= kcut) st.push(nr.base + (int)(key[k] & SLOT));
+            node = nr.base + (int)(key[0] & SLOT);
+            return false;
         }
     }
-    if (next >= 0) { node = next; return false; }
     if (st.sp == 0) return true;
     node = st.pop();
     return false;
@@ -253,6 +286,8 @@ __device__ inline void traverse_stream(const WNode<W> *__restrict__ nodes, const
                                        Done done) {
     TravRay r;
     TravHit h;
+    trav_setup(r, mk3(0.f, 0.f, 0.f), mk3(1.f, 1.f, 1.f), 0.f, 0.f, h);
+    st.sp = 0;
     int node = 0;
     int i = first;
     bool busy = false;
@@ -261,15 +296,29 @@ __device__ inline void traverse_stream(const WNode<W> *__restrict__ nodes, const
         const unsigned long long working = __ballot(busy);
         if (!want && !working) break;
         if (want && (!working || __popcll(want) >= (unsigned)refill)) {
-            if (!busy && i < n) {
-                f3 o, d;
-                float tmin, tmax;
-                int root = 0;
+            // wave-uniform branch; the new rays enter the lane state through
+            // selects, so the state never merges at a divergent join
+            const bool take = !busy && i < n;
+            f3 o = mk3(0.f, 0.f, 0.f), d = mk3(1.f, 1.f, 1.f);
+            float tmin = 0.f, tmax = 0.f;
+            int root = 0;
+            if (take) {
                 if constexpr (ROOTED) load(i, o, d, tmin, tmax, root);
                 else load(i, o, d, tmin, tmax);
-                trav_begin(r, o, d, tmin, tmax, st, node, h, root);
-                busy = true;
             }
+            TravRay nr;
+            TravHit nh;
+            trav_setup(nr, o, d, tmin, tmax, nh);
+            r.o.x = take ? nr.o.x : r.o.x; r.o.y = take ? nr.o.y : r.o.y; r.o.z = take ? nr.o.z : r.o.z;
+            r.d.x = take ? nr.d.x : r.d.x; r.d.y = take ? nr.d.y : r.d.y; r.d.z = take ? nr.d.z : r.d.z;
+            r.ix = take ? nr.ix : r.ix; r.iy = take ? nr.iy : r.iy; r.iz = take ? nr.iz : r.iz;
+            r.ox = take ? nr.ox : r.ox; r.oy = take ? nr.oy : r.oy; r.oz = take ? nr.oz : r.oz;
+            r.tmin = take ? nr.tmin : r.tmin;
+            h.t = take ? nh.t : h.t; h.u = take ? nh.u : h.u; h.v = take ? nh.v : h.v;
+            h.prim = take ? nh.prim : h.prim;
+            node = take ? root : node;
+            st.sp = take ? 0 : st.sp;
+            busy = busy || take;
         }
         if (busy && trav_step<W, ANY>(nodes, tris, r, st, node, h)) {
             done(i, h);

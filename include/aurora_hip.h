@@ -257,7 +257,9 @@ int aur_scene_info_get(aur_ctx *ctx, aur_scene_info *out);
  * rows of bvh_width floats (lo x, hi x, lo y, hi y, lo z, hi z of every child
  * box), bvh_width int child codes (>= 0 inner node index, 0x7fffffff empty,
  * < 0 leaf: ~code = first << 4 | (count - 1), a range of triangle records)
- * and bvh_width pad ints; node 0 is the root.  tri_records (12 * n_tris
+ * and bvh_width ints of which the first is the index of the node's first
+ * inner child (inner children take the first slots and consecutive indices)
+ * and the rest are 0; node 0 is the root.  tri_records (12 * n_tris
  * floats, may be NULL): per triangle in BVH order v0 (3 floats) and the
  * global triangle id (int bits), e1 and 0, e2 and 0. */
 int aur_bvh_export(aur_ctx *ctx, void *nodes, float *tri_records);
