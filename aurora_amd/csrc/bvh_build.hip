@@ -11,11 +11,12 @@
 //      among 16 neighbours on either side, inner boxes come from the merges
 //   5. leaves renumbered in the tree's left-to-right order (k_ploc_place,
 //      k_ploc_permute), so every subtree covers a contiguous triangle range
-//   6. top-down collapse into W-wide nodes (k_collapse), level by level: each
-//      wide node opens the child with the largest surface area until it has W
-//      children; a subtree with <= max_leaf triangles becomes one leaf (a
-//      contiguous range of the sorted triangles); a node's inner children
-//      take its first slots and consecutive node indices from pad[0]
+//   6. SAH-optimal collapse into W-wide nodes: a bottom-up dynamic program
+//      over the binary tree (k_collapse_cost) decides which subtrees become
+//      leaves of up to max_leaf triangles (contiguous ranges of the sorted
+//      triangles) and how each wide node's W slots are filled; the nodes are
+//      then written top-down, level by level (k_collapse); a node's inner
+//      children take its first slots and consecutive node indices from pad[0]
 //   7. triangle records in sorted order (k_reorder): (v0 | prim), e1, e2 so a
 //      leaf is read with contiguous 16-byte loads
 //
@@ -113,11 +114,25 @@ __device__ inline void child_box(int code, const float4 *plo, const float4 *phi,
     }
 }
 
+// Collapse decisions of the binary tree (k_collapse_cost): per inner node,
+// 4-bit fields of dec[x]: field 0 = 1 when the subtree is best kept as one
+// leaf; field i - 1 (i = 2..W) = how many of i slots the left child takes
+// when x's children are spread over i slots of a wide node (0: i - 1 slots
+// are as good); field W = that split for x's own wide node (all W slots).
 struct CollapseIn {
     const int *left, *right, *first, *last, *vals;
     const float4 *plo, *phi, *nlo, *nhi;
+    const float *S;                  // per inner node, W costs: subtree in at most 1..W slots
+    const unsigned long long *dec;   // per inner node, the decisions above
     int max_leaf;
 };
+
+// SAH weights of the collapse: a wide node visited, a leaf visited (its
+// triangle records fetched) and a triangle tested, per unit of surface area
+// relative to the parent.
+static constexpr float SAH_NODE = 1.0f;
+static constexpr float SAH_LEAF = 1.0f;
+static constexpr float SAH_TRI = 0.25f;
 
 __device__ inline float half_area(float4 l, float4 h) {
     float dx = h.x - l.x, dy = h.y - l.y, dz = h.z - l.z;
@@ -133,6 +148,77 @@ __device__ inline int code_first(const CollapseIn &in, int code) {
     return in.first[code];
 }
 
+// A child code kept whole in one slot: a leaf when it is one triangle or the
+// collapse chose a leaf for it, else a wide node of its own.
+__device__ inline bool dp_leaf(const CollapseIn &in, int code) {
+    return (code & LEAF_BIT) || (in.dec[code] & 1ull);
+}
+
+template <int W>
+__device__ inline void slot_costs(const CollapseIn &in, int c, float (&s)[W]) {
+    if (c & LEAF_BIT) {
+        float4 l, h;
+        child_box(c, in.plo, in.phi, in.vals, in.nlo, in.nhi, &l, &h);
+        const float v = half_area(l, h) * (SAH_LEAF + SAH_TRI);
+#pragma unroll
+        for (int i = 0; i < W; ++i) s[i] = v;
+    } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) s[i] = in.S[(size_t)c * W + i];
+    }
+}
+
+// SAH-optimal collapse of the binary tree into W-wide nodes (the dynamic
+// program of Ylitie, Karras and Laine 2017, "Efficient Incoherent Ray
+// Traversal on GPUs Through Compressed Wide BVHs", with leaves of up to
+// max_leaf triangles), bottom-up: the nodes one PLOC iteration made
+// [base, base + k) only have children made by earlier iterations.
+//   S(x, 1) = min(leaf cost, A(x) * SAH_NODE + D(x, W))
+//   S(x, i) = min(S(x, i - 1), D(x, i)),  D(x, j) = min_q S(l, q) + S(r, j - q)
+template <int W>
+__global__ void k_collapse_cost(int base, int k, CollapseIn in, float *__restrict__ S,
+                                unsigned long long *__restrict__ dec) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= k) return;
+    const int x = base + u;
+    float sa[W], sb[W];
+    slot_costs<W>(in, in.left[x], sa);
+    slot_costs<W>(in, in.right[x], sb);
+    const float inf = __int_as_float(0x7f800000);
+    float D[W + 1];
+    int kd[W + 1];
+#pragma unroll
+    for (int j = 2; j <= W; ++j) {
+        float best = inf;
+        int bk = 1;
+#pragma unroll
+        for (int q = 1; q < j; ++q) {
+            const float v = sa[q - 1] + sb[j - q - 1];
+            if (v < best) { best = v; bk = q; }
+        }
+        D[j] = best;
+        kd[j] = bk;
+    }
+    const float A = half_area(in.nlo[x], in.nhi[x]);
+    const int cnt = in.last[x] - in.first[x] + 1;
+    const float c_node = A * SAH_NODE + D[W];
+    const float c_leaf = cnt <= in.max_leaf ? A * (SAH_LEAF + SAH_TRI * (float)cnt) : inf;
+    unsigned long long d = 0;
+    float s = fminf(c_leaf, c_node);
+    if (c_leaf <= c_node) d |= 1ull;
+    S[(size_t)x * W] = s;
+#pragma unroll
+    for (int i = 2; i <= W; ++i) {
+        if (D[i] < s) {
+            s = D[i];
+            d |= (unsigned long long)kd[i] << (4 * (i - 1));
+        }
+        S[(size_t)x * W + i - 1] = s;
+    }
+    d |= (unsigned long long)kd[W] << (4 * W);
+    dec[x] = d;
+}
+
 template <int W>
 __global__ void k_collapse(CollapseIn in, const int2 *__restrict__ tasks, const int *__restrict__ n_tasks,
                            int2 *__restrict__ next, int *__restrict__ n_next, int *__restrict__ node_count,
@@ -143,25 +229,27 @@ __global__ void k_collapse(CollapseIn in, const int2 *__restrict__ tasks, const 
     int ch[W];
     int nc = 0;
     int code = task.x;
-    if (!(code & LEAF_BIT) && code_count(in, code) > in.max_leaf) {
-        ch[0] = in.left[code];
-        ch[1] = in.right[code];
-        nc = 2;
-        while (nc < W) {
-            int best = -1;
-            float best_a = -1.f;
-            for (int k = 0; k < nc; ++k) {
-                int c = ch[k];
-                if ((c & LEAF_BIT) || code_count(in, c) <= in.max_leaf) continue;
-                float a = half_area(in.nlo[c], in.nhi[c]);
-                if (a > best_a) { best_a = a; best = k; }
+    if (!dp_leaf(in, code)) {
+        // the W slots of x's wide node, as the collapse decided: a (subtree,
+        // slots) stack, left subtrees first
+        int sy[W], si[W], sp = 0;
+        const int kx = (int)((in.dec[code] >> (4 * W)) & 15ull);
+        sy[sp] = in.right[code]; si[sp++] = W - kx;
+        sy[sp] = in.left[code]; si[sp++] = kx;
+        while (sp > 0) {
+            --sp;
+            const int y = sy[sp], i = si[sp];
+            if ((y & LEAF_BIT) || i == 1) { ch[nc++] = y; continue; }
+            const int q = (int)((in.dec[y] >> (4 * (i - 1))) & 15ull);
+            if (q == 0) {
+                sy[sp] = y; si[sp++] = i - 1;
+            } else {
+                sy[sp] = in.right[y]; si[sp++] = i - q;
+                sy[sp] = in.left[y]; si[sp++] = q;
             }
-            if (best < 0) break;
-            int c = ch[best];
-            ch[best] = in.left[c];
-            ch[nc++] = in.right[c];
         }
     } else {
+        // only the root: a tree of one leaf
         ch[0] = code;
         nc = 1;
     }
@@ -172,10 +260,10 @@ __global__ void k_collapse(CollapseIn in, const int2 *__restrict__ tasks, const 
     {
         int ord[W];
         for (int k = 0; k < nc; ++k)
-            if (!(ch[k] & LEAF_BIT) && code_count(in, ch[k]) > in.max_leaf) ord[n_inner++] = ch[k];
+            if (!dp_leaf(in, ch[k])) ord[n_inner++] = ch[k];
         int j = n_inner;
         for (int k = 0; k < nc; ++k)
-            if ((ch[k] & LEAF_BIT) || code_count(in, ch[k]) <= in.max_leaf) ord[j++] = ch[k];
+            if (dp_leaf(in, ch[k])) ord[j++] = ch[k];
         for (int k = 0; k < nc; ++k) ch[k] = ord[k];
     }
     const int base = n_inner ? atomicAdd(node_count, n_inner) : 0;
@@ -457,6 +545,7 @@ void build_bvh(Ctx *c, int width, int max_leaf) {
     nhi.reserve(sizeof(float4) * ni);
     int root_code;
     DevBuf ploc_vals;
+    std::vector<std::pair<int, int>> made;  // (first node, nodes) per PLOC iteration
     if (n >= 2) {
         // clusters (double-buffered), nearest neighbours, flags and their scan
         DevBuf cid[2], clo[2], chi[2], nn, flags, scan, count, pos, totals, stemp;
@@ -481,7 +570,6 @@ void build_bvh(Ctx *c, int width, int max_leaf) {
         t.nlo = nlo.as<float4>(); t.nhi = nhi.as<float4>();
         hipLaunchKernelGGL(k_ploc_init, dim3(grid), dim3(bs), 0, c->stream, n, svals, plo.as<float4>(), phi.as<float4>(),
                            cid[0].as<int>(), clo[0].as<float4>(), chi[0].as<float4>());
-        std::vector<std::pair<int, int>> made;  // (first node, nodes) per iteration
         int m = n, node_base = 0, cur = 0;
         while (m > 1) {
             const int g = (m + PLOC_B - 1) / PLOC_B;
@@ -522,6 +610,25 @@ void build_bvh(Ctx *c, int width, int max_leaf) {
     in.left = left.as<int>(); in.right = right.as<int>(); in.first = first.as<int>(); in.last = last.as<int>();
     in.vals = svals; in.plo = plo.as<float4>(); in.phi = phi.as<float4>(); in.nlo = nlo.as<float4>(); in.nhi = nhi.as<float4>();
     in.max_leaf = max_leaf;
+    in.S = nullptr;
+    in.dec = nullptr;
+    DevBuf cost, dec;
+    if (n >= 2) {
+        cost.reserve(sizeof(float) * (size_t)width * ni);
+        dec.reserve(sizeof(unsigned long long) * (size_t)ni);
+        in.S = cost.as<float>();
+        in.dec = dec.as<unsigned long long>();
+        for (const auto &it : made) {
+            const dim3 g((it.second + bs - 1) / bs);
+            if (width == 4)
+                hipLaunchKernelGGL(k_collapse_cost<4>, g, dim3(bs), 0, c->stream, it.first, it.second, in,
+                                   cost.as<float>(), dec.as<unsigned long long>());
+            else
+                hipLaunchKernelGGL(k_collapse_cost<8>, g, dim3(bs), 0, c->stream, it.first, it.second, in,
+                                   cost.as<float>(), dec.as<unsigned long long>());
+        }
+        HIP_CHECK(hipGetLastError());
+    }
     if (width == 4) collapse_levels<4>(c, in, root_code, ni + 1);
     else collapse_levels<8>(c, in, root_code, ni + 1);
     hipLaunchKernelGGL(k_reorder, dim3(grid), dim3(bs), 0, c->stream, verts, tris, svals, n, b.tri_sorted.as<float4>());

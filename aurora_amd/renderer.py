@@ -131,15 +131,17 @@ class Renderer:
 
     def bvh(self) -> dict:
         """The built wide BVH: per node the child boxes (n, 6, W) as lo x, hi x,
-        lo y, hi y, lo z, hi z rows, the child codes (n, W) and the triangle
-        records in BVH order (n_tris, 3, 4): v0 + global id bits, e1, e2."""
+        lo y, hi y, lo z, hi z rows, the child codes (n, W), the index of
+        each node's first inner child (n,) and the triangle records in BVH
+        order (n_tris, 3, 4): v0 + global id bits, e1, e2."""
         i = self.info()
         n, w = i["bvh_nodes"], i["bvh_width"]
         raw = np.zeros(n * 8 * w, np.int32)
         tri = np.zeros((i["n_tris"], 3, 4), np.float32)
         L.check(self.lib.aur_bvh_export(self._ctx, raw.ctypes.data_as(C.c_void_p), L.fptr(tri)))
         raw = raw.reshape(n, 8, w)
-        return {"width": w, "boxes": raw[:, :6].view(np.float32), "child": raw[:, 6].copy(), "tris": tri}
+        return {"width": w, "boxes": raw[:, :6].view(np.float32), "child": raw[:, 6].copy(),
+                "first_inner": raw[:, 7, 0].copy(), "tris": tri}
 
     def info(self) -> dict:
         i = L.SceneInfo()

@@ -1,9 +1,9 @@
 """The GPU-built wide BVH (the replacement of Embree's bvh4.spatialsplit
 build, embreeAccelerator.cpp:19-35) checked structurally: every node is
-reached once from the root, the leaves partition the triangle records, every
-triangle appears once with its own vertices, every child box contains what is
-below it, leaves respect the leaf size, and the surface-area cost of the tree
-is reported (traversal itself is checked against brute force in
+reached once from the root, inner children come first with consecutive
+indices, the leaves partition the triangle records, every triangle appears
+once with its own vertices, every child box contains what is below it, leaves
+respect the leaf size, and the surface-area cost of the tree is reported (traversal itself is checked against brute force in
 test_gpu_parity.py)."""
 import os
 
@@ -20,6 +20,13 @@ def _check(r, max_leaf):
     b = r.bvh()
     W, boxes, child, recs = b["width"], b["boxes"], b["child"], b["tris"]
     n_nodes, n_tris = child.shape[0], recs.shape[0]
+    # inner children take a node's first slots and consecutive indices from
+    # its first_inner (the traversal names them base + slot)
+    inner = (child >= 0) & (child != EMPTY)
+    n_inner = inner.sum(1)
+    assert np.array_equal(inner, np.arange(W)[None, :] < n_inner[:, None])
+    k_idx = np.broadcast_to(np.arange(W), child.shape)
+    assert np.array_equal(child[inner], (b["first_inner"][:, None] + k_idx)[inner])
     assert n_tris == e["tris"].shape[0]
     # triangle records: the scene's triangles, each exactly once
     ids = recs[:, 0, 3].view(np.int32)
