@@ -1,9 +1,11 @@
 // traverse.h -- wide-BVH ray traversal (replaces Embree's intersect/occluded,
 // src/geometry/embreeAccelerator.cpp:41-100).
 //
-// One ray per lane.  The traversal stack lives in LDS (STACK_LDS entries per
-// lane, lane-minor so a wave's pushes hit distinct banks) and spills to a
-// per-thread global area beyond that depth.  Closest hit keeps the smallest t
+// One ray per lane.  A lane's current item is an inner node or a leaf, and
+// one step of the loop fetches and processes one item (trav_step).  The
+// traversal stack lives in LDS (STACK_LDS entries per lane, lane-minor so a
+// wave's pushes hit distinct banks) and spills to a per-thread global area
+// beyond that depth.  Closest hit keeps the smallest t
 // and, on an exact tie, the smallest triangle id -- the same rule as the CPU
 // brute-force oracle, so hit ids compare bit-exactly.
 #pragma once
@@ -33,28 +35,8 @@ struct TravHit { float t, u, v; int prim; };
 template <int W>
 struct NodeRegs {
     float lox[W], hix[W], loy[W], hiy[W], loz[W], hiz[W];
-    int child[W];
-    int base;  // index of the first inner child: inner slot k is node base + k
+    int child[W];  // inner child: node index; leaf: ~(first << 4 | count - 1)
 };
-
-template <int W>
-__device__ inline void load_node(const WNode<W> *__restrict__ nodes, int ni, NodeRegs<W> &r) {
-    const float4 *p = reinterpret_cast<const float4 *>(nodes + ni);
-    constexpr int Q = W / 4;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        float4 a = p[0 * Q + q], b = p[1 * Q + q], c = p[2 * Q + q], d = p[3 * Q + q], e = p[4 * Q + q], f = p[5 * Q + q];
-        int4 g = reinterpret_cast<const int4 *>(p)[6 * Q + q];
-        r.lox[4 * q + 0] = a.x; r.lox[4 * q + 1] = a.y; r.lox[4 * q + 2] = a.z; r.lox[4 * q + 3] = a.w;
-        r.hix[4 * q + 0] = b.x; r.hix[4 * q + 1] = b.y; r.hix[4 * q + 2] = b.z; r.hix[4 * q + 3] = b.w;
-        r.loy[4 * q + 0] = c.x; r.loy[4 * q + 1] = c.y; r.loy[4 * q + 2] = c.z; r.loy[4 * q + 3] = c.w;
-        r.hiy[4 * q + 0] = d.x; r.hiy[4 * q + 1] = d.y; r.hiy[4 * q + 2] = d.z; r.hiy[4 * q + 3] = d.w;
-        r.loz[4 * q + 0] = e.x; r.loz[4 * q + 1] = e.y; r.loz[4 * q + 2] = e.z; r.loz[4 * q + 3] = e.w;
-        r.hiz[4 * q + 0] = f.x; r.hiz[4 * q + 1] = f.y; r.hiz[4 * q + 2] = f.z; r.hiz[4 * q + 3] = f.w;
-        r.child[4 * q + 0] = g.x; r.child[4 * q + 1] = g.y; r.child[4 * q + 2] = g.z; r.child[4 * q + 3] = g.w;
-    }
-    r.base = nodes[ni].pad[0];
-}
 
 // Stack: top STACK_LDS entries per lane in LDS, deeper entries in global
 // memory.  Only the lane's LDS address and depth are per-lane values; the
@@ -103,23 +85,6 @@ struct TravStack {
     }
 };
 
-// Tests the triangles of one leaf; returns true when an any-hit query is done.
-template <bool ANY>
-__device__ inline bool test_leaf(const float4 *__restrict__ tris, int code, f3 o, f3 d, float tmin, TravHit &h) {
-    int enc = ~code;
-    int first = enc >> 4, cnt = (enc & 15) + 1;
-    for (int i = first; i < first + cnt; ++i) {
-        float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
-        float t, b1, b2;
-        if (intersect_tri(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), tmin, h.t, &t, &b1, &b2)) {
-            int prim = __float_as_int(a.w);
-            if (ANY) { h.t = t; h.u = b1; h.v = b2; h.prim = prim; return true; }
-            if (h.prim < 0 || t < h.t || prim < h.prim) { h.t = t; h.u = b1; h.v = b2; h.prim = prim; }
-        }
-    }
-    return false;
-}
-
 // Per-ray constants of the slab test and the triangle test.
 struct TravRay {
     f3 o, d;
@@ -166,27 +131,37 @@ __device__ inline void sort_keys(unsigned (&k)[W]) {
     }
 }
 
-// Visits one node: tests its child boxes, tests the leaves they let through,
-// pushes the inner children to visit later and moves `node` to the next one.
-// Returns true when the ray is done (stack empty, or an any-hit query hit).
+// The child code in slot s of a node (a select tree over the W codes, so
+// nothing is dynamically indexed).
+template <int W>
+__device__ inline int child_of(const int (&ch)[W], unsigned s) {
+    if constexpr (W == 4) {
+        const int lo = (s & 1u) ? ch[1] : ch[0], hi = (s & 1u) ? ch[3] : ch[2];
+        return (s & 2u) ? hi : lo;
+    } else {
+        const int a = (s & 1u) ? ch[1] : ch[0], b = (s & 1u) ? ch[3] : ch[2];
+        const int c = (s & 1u) ? ch[5] : ch[4], d = (s & 1u) ? ch[7] : ch[6];
+        const int ab = (s & 2u) ? b : a, cd = (s & 2u) ? d : c;
+        return (s & 4u) ? cd : ab;
+    }
+}
+
+// Visits an inner node: tests its child boxes and moves `node` to the next
+// item, pushing the other children that the ray enters.  Leaves are items
+// like inner nodes: they are tested in a later step of their own.  Returns
+// true when the ray is done (nothing entered and the stack empty).
 template <int W, bool ANY>
-__device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restrict__ tris, const TravRay &r,
-                                  TravStack &st, int &node, TravHit &h) {
-    const float inf = __int_as_float(0x7f800000);
+__device__ inline bool visit_inner(const NodeRegs<W> &nr, const TravRay &r, TravStack &st, int &node,
+                                   const TravHit &h) {
     // exact-t ties must still be visited, so the far bound is widened slightly
     const float tfar_lim = h.t * 1.0000004f;
-    // entry distance per inner child (inf: missed, or a leaf: leaves are
-    // collected in a mask and tested after the box tests); every index
-    // below is a compile-time constant, so the lists stay in VGPRs
-    // Closest hit orders the inner children by 32-bit keys: the entry
+    // Closest hit orders the entered children by 32-bit keys: the entry
     // distance's bits (a non-negative float orders as an unsigned integer)
     // with the slot in the low bits, so the sort moves one register per
-    // child and the child is base + slot.  Missed children get ~0u.
+    // child.  Missed children get ~0u.
     constexpr unsigned SLOT = W == 4 ? 3u : 7u;
-    float tn[W];
     unsigned key[W];
-    int ch[W];
-    unsigned leaves = 0;
+    unsigned inner_m = 0, leaf_m = 0;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         float x0 = fmaf(nr.lox[k], r.ix, -r.ox), x1 = fmaf(nr.hix[k], r.ix, -r.ox);
@@ -194,54 +169,47 @@ __device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restric
         float z0 = fmaf(nr.loz[k], r.iz, -r.oz), z1 = fmaf(nr.hiz[k], r.iz, -r.oz);
         float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
         float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
-        int c = nr.child[k];
-        bool hit = a <= b;
-        if (ANY) tn[k] = (hit && c >= 0) ? a : inf;
-        // a >= tmin >= 0: clearing the sign bit only turns -0 into +0
-        else key[k] = (hit && c >= 0) ? ((__float_as_uint(a) & 0x7fffffffu & ~SLOT) | (unsigned)k) : ~0u;
-        ch[k] = c;
-        if (hit && c < 0) leaves |= 1u << k;
+        const bool hit = a <= b;
+        if (ANY) {
+            if (hit) {
+                if (nr.child[k] < 0) leaf_m |= 1u << k;
+                else inner_m |= 1u << k;
+            }
+        } else {
+            // a >= tmin >= 0: clearing the sign bit only turns -0 into +0
+            key[k] = hit ? ((__float_as_uint(a) & 0x7fffffffu & ~SLOT) | (unsigned)k) : ~0u;
+        }
     }
-    // the leaves this node's boxes let through, one per pass of the loop:
-    // lanes with leaves in different slots test them together
 #ifdef AUR_TRAV_STATS
     st.n_visits += 1;
 #endif
-    while (leaves) {
-        const int k = __builtin_ctz(leaves);
-        leaves &= leaves - 1;
-        int c = ch[0];
-#pragma unroll
-        for (int j = 1; j < W; ++j) c = (k == j) ? ch[j] : c;
-#ifdef AUR_TRAV_STATS
-        st.n_leaves += 1;
-        st.n_tris += (unsigned)((~c) & 15) + 1u;
-#endif
-        if (test_leaf<ANY>(tris, c, r.o, r.d, r.tmin, h)) return true;
-    }
     if (ANY) {
-        // any hit: children in slot order
-        const float tcut = h.t * 1.0000004f;
-        int next = -1;
+        // any hit: the leaves first (the query ends at its first accepted
+        // triangle), then the inner children, each in slot order
+        int next = 0;
+        bool have = false;
 #pragma unroll
-        for (int k = W - 1; k >= 0; --k) {
-            if (tn[k] <= tcut) {
-                if (next >= 0) st.push(next);
-                next = ch[k];
+        for (int k = W - 1; k >= 0; --k)
+            if ((inner_m >> k) & 1u) {
+                if (have) st.push(next);
+                next = nr.child[k];
+                have = true;
             }
-        }
-        if (next >= 0) { node = next; return false; }
+#pragma unroll
+        for (int k = W - 1; k >= 0; --k)
+            if ((leaf_m >> k) & 1u) {
+                if (have) st.push(next);
+                next = nr.child[k];
+                have = true;
+            }
+        if (have) { node = next; return false; }
     } else {
         sort_keys<W>(key);
-        // children entered beyond the (possibly shortened) hit distance are
-        // skipped; the slot bits and the cleared low distance bits only ever
-        // admit more children (ties and a few ulps), never fewer
-        const unsigned kcut = __float_as_uint(fmaxf(h.t * 1.0000004f, 0.f)) | SLOT;
-        if (key[0] <= kcut) {
+        if (key[0] != ~0u) {
 #pragma unroll
             for (int k = W - 1; k >= 1; --k)
-                if (key[k] <= kcut) st.push(nr.base + (int)(key[k] & SLOT));
-            node = nr.base + (int)(key[0] & SLOT);
+                if (key[k] != ~0u) st.push(child_of<W>(nr.child, key[k] & SLOT));
+            node = child_of<W>(nr.child, key[0] & SLOT);
             return false;
         }
     }
@@ -250,12 +218,89 @@ __device__ inline bool trav_visit(const NodeRegs<W> &nr, const float4 *__restric
     return false;
 }
 
+// Takes one triangle test's result into the hit record; returns true when an
+// any-hit query is done.  Closest hit keeps the smallest t and, on an exact
+// tie, the smallest triangle id, so the result does not depend on the order
+// in which triangles are tested.
+template <bool ANY>
+__device__ inline bool take_tri(const float4 &a, const float4 &b, const float4 &c, const TravRay &r, TravHit &h) {
+    float t, b1, b2;
+    if (!intersect_tri(r.o, r.d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r.tmin, h.t, &t, &b1, &b2))
+        return false;
+    const int prim = __float_as_int(a.w);
+    if (ANY) { h.t = t; h.u = b1; h.v = b2; h.prim = prim; return true; }
+    if (h.prim < 0 || t < h.t || prim < h.prim) { h.t = t; h.u = b1; h.v = b2; h.prim = prim; }
+    return false;
+}
+
+// One step of a ray's traversal: the lane's current item is an inner node
+// (node >= 0) or a leaf (~(first << 4 | count - 1)).  Either way the step
+// fetches six float4 rows into the same registers -- a node's first six
+// box rows, or the records of the leaf's next two triangles (48 B each) --
+// so every step costs one round trip to memory whichever kind of item each
+// lane holds, and a wave's lanes at nodes and at leaves progress together.
+// A leaf of more than two triangles stays the lane's item, shortened by two,
+// for the next step.  Returns true when the ray is done.
 template <int W, bool ANY>
 __device__ inline bool trav_step(const WNode<W> *__restrict__ nodes, const float4 *__restrict__ tris,
                                  const TravRay &r, TravStack &st, int &node, TravHit &h) {
-    NodeRegs<W> nr;
-    load_node<W>(nodes, node, nr);
-    return trav_visit<W, ANY>(nr, tris, r, st, node, h);
+    constexpr int Q = W / 4;
+    const bool inner = node >= 0;
+    const int enc = ~node;
+    const int first = enc >> 4, cnt = (enc & 15) + 1;
+    const float4 *pn = reinterpret_cast<const float4 *>(nodes + (inner ? node : 0));
+    const float4 *p0 = inner ? pn : tris + 3 * first;
+    const float4 *p1 = inner ? pn + 3 : tris + 3 * (cnt >= 2 ? first + 1 : first);
+    // rows 0-5 from p0/p1; the rest of a node (box rows beyond the sixth of
+    // 8-wide nodes, the child codes) from pn, which a leaf lane points at the
+    // root so that its loads stay in bounds
+    float4 rows[6 * Q];
+    int4 crow[Q];
+    rows[0] = p0[0]; rows[1] = p0[1]; rows[2] = p0[2];
+    rows[3] = p1[0]; rows[4] = p1[1]; rows[5] = p1[2];
+#pragma unroll
+    for (int j = 6; j < 6 * Q; ++j) rows[j] = pn[j];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) crow[q] = reinterpret_cast<const int4 *>(pn)[6 * Q + q];
+    // every fetched word is used here, before the branch on the item's kind:
+    // the loads issue together as full 16-byte loads (the compiler would
+    // otherwise narrow rows to the words the leaf test reads and sink the
+    // child codes' load below the box tests, a second round trip per node)
+#pragma unroll
+    for (int j = 0; j < 6 * Q; ++j) asm volatile("" ::"v"(rows[j].x), "v"(rows[j].y), "v"(rows[j].z), "v"(rows[j].w));
+#pragma unroll
+    for (int q = 0; q < Q; ++q) asm volatile("" ::"v"(crow[q].x), "v"(crow[q].y), "v"(crow[q].z), "v"(crow[q].w));
+    if (inner) {
+        NodeRegs<W> nr;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const float4 a = rows[0 * Q + q], b = rows[1 * Q + q], c = rows[2 * Q + q];
+            const float4 d = rows[3 * Q + q], e = rows[4 * Q + q], f = rows[5 * Q + q];
+            const int4 g = crow[q];
+            nr.lox[4 * q + 0] = a.x; nr.lox[4 * q + 1] = a.y; nr.lox[4 * q + 2] = a.z; nr.lox[4 * q + 3] = a.w;
+            nr.hix[4 * q + 0] = b.x; nr.hix[4 * q + 1] = b.y; nr.hix[4 * q + 2] = b.z; nr.hix[4 * q + 3] = b.w;
+            nr.loy[4 * q + 0] = c.x; nr.loy[4 * q + 1] = c.y; nr.loy[4 * q + 2] = c.z; nr.loy[4 * q + 3] = c.w;
+            nr.hiy[4 * q + 0] = d.x; nr.hiy[4 * q + 1] = d.y; nr.hiy[4 * q + 2] = d.z; nr.hiy[4 * q + 3] = d.w;
+            nr.loz[4 * q + 0] = e.x; nr.loz[4 * q + 1] = e.y; nr.loz[4 * q + 2] = e.z; nr.loz[4 * q + 3] = e.w;
+            nr.hiz[4 * q + 0] = f.x; nr.hiz[4 * q + 1] = f.y; nr.hiz[4 * q + 2] = f.z; nr.hiz[4 * q + 3] = f.w;
+            nr.child[4 * q + 0] = g.x; nr.child[4 * q + 1] = g.y; nr.child[4 * q + 2] = g.z; nr.child[4 * q + 3] = g.w;
+        }
+        return visit_inner<W, ANY>(nr, r, st, node, h);
+    }
+    const float4 *row = rows;
+#ifdef AUR_TRAV_STATS
+    st.n_leaves += 1;
+    st.n_tris += cnt >= 2 ? 2u : 1u;
+#endif
+    if (take_tri<ANY>(row[0], row[1], row[2], r, h)) return true;
+    if (cnt >= 2 && take_tri<ANY>(row[3], row[4], row[5], r, h)) return true;
+    if (cnt > 2) {
+        node = ~(((first + 2) << 4) | (cnt - 3));
+        return false;
+    }
+    if (st.sp == 0) return true;
+    node = st.pop();
+    return false;
 }
 
 // One whole ray, from node `root` (a batched launch's forest of BVHs).
