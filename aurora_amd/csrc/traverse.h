@@ -28,7 +28,9 @@ namespace aur {
 
 // A lane whose ray is done waits for this many lanes of its wave before they
 // fetch new rays together (traverse_stream)
-#define AUR_REFILL_LANES 8
+#ifndef AUR_REFILL_LANES
+#define AUR_REFILL_LANES 32
+#endif
 
 struct TravHit { float t, u, v; int prim; };
 
