@@ -176,72 +176,50 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_shadow(SceneView sv, W
 #endif
 }
 
-// ---------------------------------------------------------------------------
-// shade: one path vertex per queue entry
-// ---------------------------------------------------------------------------
-// 96 VGPRs (5 waves per SIMD) without the shader code; the shader, normal-map
-// and environment-map code needs more registers and runs at 4.
-template <bool TEX, bool CAM = false>
-__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView wv, int q) {
-    __shared__ EmitScratch es;
-    const int n = wv.counts[q];
-    const int stride = gridDim.x * blockDim.x;
-    // whole workgroups iterate together: the emit step reserves per workgroup
-    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const int i = base + threadIdx.x;
-        ShadeOut o;
-        o.cont = o.finish = false;
-        o.sh[0].valid = o.sh[1].valid = false;
-        if (i < n) shade_compute<TEX, CAM>(sv, wv, q, i, [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); }, 0, o);
-        shade_emit(wv, q, i < n, o, es);
+// One finished path: its radiance La + Lb (clamped) goes to its output slot;
+// ext_dup / hit_dup count the camera rays a whole-pass render traced more
+// than once (later light samples of a pass share the pass's camera ray).
+__device__ inline void finish_path(const SceneView &sv, const WfView &wv, const float4 a, const float4 b,
+                                   const int4 m, unsigned &ext_dup, unsigned &hit_dup) {
+    const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
+    // whole passes land by (pass, light sample, pixel) for the resolve;
+    // a camera miss is marked by alpha + 2 (the resolve takes it once);
+    // injected paths land by id
+    int slot = m.y;
+    float w = a.w;
+    if (wv.src.mode == 0) {
+        int pixel, sample, sub, local;
+        path_decode(sv, wv, m.y, &pixel, &sample, &sub, &local);
+        slot = m.y - local + pixel;
+        const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
+        if (cam_miss) w += 2.f;
+        // the reference traces a pass's camera ray once for all its light
+        // samples (integrator.cpp:160-164): later samples do not count it
+        if (sub > 0) { ext_dup += 1; hit_dup += cam_miss ? 0 : 1; }
+    }
+    wv.out[slot] = make_float4(L.x, L.y, L.z, w);
+    const long long ri = wv.src.rec_base + m.y;
+    if (wv.src.adj_rec && ri >= 0 && ri < wv.src.rec_count) {
+        // the adjoint's first replay: radiance before the clamp and the
+        // number of 0.2 steps the clamp took (integrator.cpp:276-283)
+        f3 Lc = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
+        int k = 0;
+        if (!(m.x & F_NOCLAMP))
+            while (k < AUR_FIREFLY_MAX_STEPS && (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY)) {
+                Lc = mul3(Lc, 0.2f);
+                ++k;
+            }
+        const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
+        // bits 0-15 clamp steps, bit 16 camera miss, bits 17-24 rays traced
+        wv.src.adj_rec[ri] = make_float4(
+            a.x + b.x, a.y + b.y, a.z + b.z,
+            __int_as_float(k | (cam_miss ? 0x10000 : 0) | (min(path_ext_rays(m.x), 255) << 17)));
     }
 }
 
-// ---------------------------------------------------------------------------
-// finish: firefly clamp (integrator.cpp:277-281), per-path output
-// ---------------------------------------------------------------------------
-__global__ void k_finish(SceneView sv, WfView wv) {
-    const int n = wv.counts[C_FINISH];
-    const int stride = gridDim.x * blockDim.x;
-    unsigned ext_dup = 0, hit_dup = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float4 a = wv.finLa[i], b = wv.finLb[i];
-        const int4 m = wv.finMeta[i];
-        const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
-        // whole passes land by (pass, light sample, pixel) for the resolve;
-        // a camera miss is marked by alpha + 2 (the resolve takes it once);
-        // injected paths land by id
-        int slot = m.y;
-        float w = a.w;
-        if (wv.src.mode == 0) {
-            int pixel, sample, sub, local;
-            path_decode(sv, wv, m.y, &pixel, &sample, &sub, &local);
-            slot = m.y - local + pixel;
-            const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
-            if (cam_miss) w += 2.f;
-            // the reference traces a pass's camera ray once for all its light
-            // samples (integrator.cpp:160-164): later samples do not count it
-            if (sub > 0) { ext_dup += 1; hit_dup += cam_miss ? 0 : 1; }
-        }
-        wv.out[slot] = make_float4(L.x, L.y, L.z, w);
-        const long long ri = wv.src.rec_base + m.y;
-        if (wv.src.adj_rec && ri >= 0 && ri < wv.src.rec_count) {
-            // the adjoint's first replay: radiance before the clamp and the
-            // number of 0.2 steps the clamp took (integrator.cpp:276-283)
-            f3 Lc = mk3(a.x + b.x, a.y + b.y, a.z + b.z);
-            int k = 0;
-            if (!(m.x & F_NOCLAMP))
-                while (k < AUR_FIREFLY_MAX_STEPS && (Lc.x > AUR_FIREFLY || Lc.y > AUR_FIREFLY || Lc.z > AUR_FIREFLY)) {
-                    Lc = mul3(Lc, 0.2f);
-                    ++k;
-                }
-            const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
-            // bits 0-15 clamp steps, bit 16 camera miss, bits 17-24 rays traced
-            wv.src.adj_rec[ri] = make_float4(
-                a.x + b.x, a.y + b.y, a.z + b.z,
-                __int_as_float(k | (cam_miss ? 0x10000 : 0) | (min(path_ext_rays(m.x), 255) << 17)));
-        }
-    }
+// Takes the duplicate camera-ray counts out of the run's statistics (every
+// lane of the wave calls this).
+__device__ inline void finish_dups(const WfView &wv, unsigned ext_dup, unsigned hit_dup) {
     if (wv.src.mode == 0 && wv.src.ls > 1) {
         for (int o = 32; o > 0; o >>= 1) {
             ext_dup += __shfl_xor(ext_dup, o);
@@ -252,6 +230,44 @@ __global__ void k_finish(SceneView sv, WfView wv) {
             atomicAdd(&wv.stats[S_HIT], (unsigned long long)(0ull - hit_dup));
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// shade: one path vertex per queue entry
+// ---------------------------------------------------------------------------
+// 96 VGPRs (5 waves per SIMD) without the shader code; the shader, normal-map
+// and environment-map code needs more registers and runs at 4.
+template <bool TEX, bool CAM = false>
+__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView wv, int q) {
+    __shared__ EmitScratch es;
+    const int n = wv.counts[q];
+    const int stride = gridDim.x * blockDim.x;
+    unsigned ext_dup = 0, hit_dup = 0;
+    // whole workgroups iterate together: the emit step reserves per workgroup
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        ShadeOut o;
+        o.cont = o.finish = false;
+        o.sh[0].valid = o.sh[1].valid = false;
+        if (i < n) shade_compute<TEX, CAM>(sv, wv, q, i, [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); }, 0, o);
+        shade_emit<true>(wv, q, i < n, o, es,
+                         [&](const ShadeOut &f) { finish_path(sv, wv, f.La, f.Lb, f.meta, ext_dup, hit_dup); });
+    }
+    finish_dups(wv, ext_dup, hit_dup);
+}
+
+// ---------------------------------------------------------------------------
+// finish: firefly clamp (integrator.cpp:277-281), per-path output
+// ---------------------------------------------------------------------------
+// The paths that finished with shadow rays of their last vertex pending
+// (the others were written by k_shade).
+__global__ void k_finish(SceneView sv, WfView wv) {
+    const int n = wv.counts[C_FINISH];
+    const int stride = gridDim.x * blockDim.x;
+    unsigned ext_dup = 0, hit_dup = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        finish_path(sv, wv, wv.finLa[i], wv.finLb[i], wv.finMeta[i], ext_dup, hit_dup);
+    finish_dups(wv, ext_dup, hit_dup);
 }
 
 __global__ void k_clear_counts(int *counts, int q_done) {

@@ -383,12 +383,19 @@ struct EmitScratch {
 };
 
 // Every thread of the workgroup calls this once per loop round (valid = it
-// shaded an entry this round).
-__device__ inline void shade_emit(const WfView &wv, int q, bool valid, const ShadeOut &o, EmitScratch &es) {
+// shaded an entry this round).  With DIRECT, a path that finishes here with
+// no shadow ray of this vertex pending (a miss, or the vertex after the last
+// one) has its final radiance already: `direct(o)` writes its output in
+// place of a finish-list record (48 B written, read back by the finish
+// kernel).  Paths whose shadow rays still add to them go to the finish list.
+template <bool DIRECT, class Direct>
+__device__ inline void shade_emit(const WfView &wv, int q, bool valid, const ShadeOut &o, EmitScratch &es,
+                                  Direct direct) {
     const int nq = 1 - q;
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    const bool pc = valid && o.cont, pf = valid && o.finish;
+    const bool pd = DIRECT && valid && o.finish && !o.sh[0].valid && !o.sh[1].valid;
+    const bool pc = valid && o.cont, pf = valid && o.finish && !pd;
     const bool p0 = valid && o.sh[0].valid, p1 = valid && o.sh[1].valid;
     const unsigned long long mc = __ballot(pc), mf = __ballot(pf), m0 = __ballot(p0), m1 = __ballot(p1);
     if (lane == 0) {
@@ -426,6 +433,7 @@ __device__ inline void shade_emit(const WfView &wv, int q, bool valid, const Sha
         wv.finLb[fs] = o.Lb;
         wv.finMeta[fs] = o.meta;
     }
+    if (pd) direct(o);
     const int target = o.cont ? slot : (fs | SH_TO_FINISH);
     int s = ss;
 #pragma unroll
@@ -439,6 +447,11 @@ __device__ inline void shade_emit(const WfView &wv, int q, bool valid, const Sha
             ++s;
         }
     }
+}
+
+// Every finishing path through the finish list (the batched launch).
+__device__ inline void shade_emit(const WfView &wv, int q, bool valid, const ShadeOut &o, EmitScratch &es) {
+    shade_emit<false>(wv, q, valid, o, es, [](const ShadeOut &) {});
 }
 
 }  // namespace aur
