@@ -241,9 +241,38 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bextend(const JobDev *
 // traversal and shading code sees uniform scene pointers, as in the
 // single-scene kernels.
 
+// One finished path's ring entry: clamped radiance and its packed counts.
+__device__ inline void bfinish_path(const BatchView &bv, long long iter, const float4 a, const float4 b,
+                                    const int4 m) {
+    const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
+    const JobDev &jd = bv.jobs[m.w];
+    const long long w = (jd.start + (long long)m.z * (jd.sv.width * jd.sv.height) + m.y) & bv.ring_mask;
+    // a camera miss is marked by alpha + 2 (the resolve takes it once); a
+    // pass's later light samples do not count the shared camera ray
+    const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
+    int pass, sub;
+    job_block(jd, m.z, &pass, &sub);
+    const int dup = sub > 0 ? 1 : 0;
+    bv.ring[w] = make_float4(L.x, L.y, L.z, a.w + (cam_miss ? 2.f : 0.f));
+    bv.ring_cnt[w] = pack_counts(path_ext_rays(m.x) - dup, path_hit_rays(m.x) - (dup && !cam_miss ? 1 : 0),
+                                 (int)b.w, iter);
+}
+
+// The paths that finished with shadow rays of their last vertex pending
+// (the others were written by k_bshade).
+__global__ void __launch_bounds__(256) k_bfinish(BatchView bv, WfView wv) {
+    const int n = wv.counts[C_FINISH];
+    const long long iter = bv.ctl[CTL_ITER] - 1;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        bfinish_path(bv, iter, wv.finLa[i], wv.finLb[i], wv.finMeta[i]);
+}
+
 template <bool TEX>
-__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__restrict__ jobs, WfView wv, int q) {
+__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(BatchView bv, WfView wv, int q) {
     __shared__ EmitScratch es;
+    const JobDev *__restrict__ jobs = bv.jobs;
+    const long long iter = bv.ctl[CTL_ITER] - 1;
     const int n = wv.counts[q];
     const int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -268,7 +297,8 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(const JobDev *__res
                 }
             }
         }
-        shade_emit(wv, q, i < n, o, es);
+        shade_emit<true>(wv, q, i < n, o, es,
+                         [&](const ShadeOut &f) { bfinish_path(bv, iter, f.La, f.Lb, f.meta); });
     }
 }
 
@@ -293,28 +323,6 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_bshadow(const JobDev *
             const float4 c = wv.sray[3 * i + 2];
             shadow_add(wv, nq, __float_as_int(c.w), c);
         }
-    }
-}
-
-__global__ void __launch_bounds__(256) k_bfinish(BatchView bv, WfView wv) {
-    const int n = wv.counts[C_FINISH];
-    const long long iter = bv.ctl[CTL_ITER] - 1;
-    const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float4 a = wv.finLa[i], b = wv.finLb[i];
-        const int4 m = wv.finMeta[i];
-        const f3 L = firefly_clamp(mk3(a.x + b.x, a.y + b.y, a.z + b.z), m.x);
-        const JobDev &jd = bv.jobs[m.w];
-        const long long w = (jd.start + (long long)m.z * (jd.sv.width * jd.sv.height) + m.y) & bv.ring_mask;
-        // a camera miss is marked by alpha + 2 (the resolve takes it once); a
-        // pass's later light samples do not count the shared camera ray
-        const bool cam_miss = (m.x & F_MISS) && path_ext_rays(m.x) == 1;
-        int pass, sub;
-        job_block(jd, m.z, &pass, &sub);
-        const int dup = sub > 0 ? 1 : 0;
-        bv.ring[w] = make_float4(L.x, L.y, L.z, a.w + (cam_miss ? 2.f : 0.f));
-        bv.ring_cnt[w] = pack_counts(path_ext_rays(m.x) - dup, path_hit_rays(m.x) - (dup && !cam_miss ? 1 : 0),
-                                     (int)b.w, iter);
     }
 }
 
@@ -451,8 +459,8 @@ static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q
     hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs,                       \
                        static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, 1 - q)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
-    if (tex) hipLaunchKernelGGL(k_bshade<true>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
-    else hipLaunchKernelGGL(k_bshade<false>, dim3(sb), dim3(256), 0, c->stream, bv.jobs, wv, q);
+    if (tex) hipLaunchKernelGGL(k_bshade<true>, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
+    else hipLaunchKernelGGL(k_bshade<false>, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BSHD);
 #undef AUR_BEXT
 #undef AUR_BSHD

@@ -449,9 +449,4 @@ __device__ inline void shade_emit(const WfView &wv, int q, bool valid, const Sha
     }
 }
 
-// Every finishing path through the finish list (the batched launch).
-__device__ inline void shade_emit(const WfView &wv, int q, bool valid, const ShadeOut &o, EmitScratch &es) {
-    shade_emit<false>(wv, q, valid, o, es, [](const ShadeOut &) {});
-}
-
 }  // namespace aur
