@@ -135,11 +135,11 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_extend(SceneView sv, W
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
             float4 a, b;
             if constexpr (CAM) camera_ray_of(sv, path_rng<false>(sv, wv, i), &a, &b);
-            else { a = wv.ray[q][2 * i]; b = wv.ray[q][2 * i + 1]; }
+            else { a = ld_s(&wv.ray[q][2 * i]); b = ld_s(&wv.ray[q][2 * i + 1]); }
             o = mk3(a.x, a.y, a.z); d = mk3(b.x, b.y, b.z); tmin = a.w; tmax = b.w;
         },
         [&](int i, const TravHit &h) {
-            wv.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
+            st_s(&wv.hit[i], make_float4(h.t, h.u, h.v, __int_as_float(h.prim)));
             hits += h.prim >= 0 ? 1 : 0;
         });
     // one add per wave into this workgroup's partial (no global same-address atomics)
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(B, AUR_TRAV_WAVES(W)) k_shadow(SceneView sv, W
     st.init(s_stack, B, wv.sovf, wv.ovf_per_thread);
     const WNode<W> *nodes = static_cast<const WNode<W> *>(sv.nodes);
     for (int i = gtid; i < n; i += stride) {
-        float4 a = wv.sray[3 * i], b = wv.sray[3 * i + 1], c = wv.sray[3 * i + 2];
+        float4 a = ld_s(&wv.sray[3 * i]), b = ld_s(&wv.sray[3 * i + 1]), c = ld_s(&wv.sray[3 * i + 2]);
         TravHit h;
         traverse<W, true>(nodes, sv.tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), AUR_RAY_BIAS, b.w, st, h);
         if (h.prim < 0) shadow_add(wv, nq, __float_as_int(c.w), c);
@@ -197,7 +197,7 @@ __device__ inline void finish_path(const SceneView &sv, const WfView &wv, const 
         // samples (integrator.cpp:160-164): later samples do not count it
         if (sub > 0) { ext_dup += 1; hit_dup += cam_miss ? 0 : 1; }
     }
-    wv.out[slot] = make_float4(L.x, L.y, L.z, w);
+    st_s(&wv.out[slot], make_float4(L.x, L.y, L.z, w));
     const long long ri = wv.src.rec_base + m.y;
     if (wv.src.adj_rec && ri >= 0 && ri < wv.src.rec_count) {
         // the adjoint's first replay: radiance before the clamp and the
