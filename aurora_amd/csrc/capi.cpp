@@ -518,6 +518,18 @@ void aur_destroy(aur_ctx *ctx) {
             (void)hipEventDestroy(ctx->c.ev_shaded);
             (void)hipEventDestroy(ctx->c.ev_done);
         }
+        if (ctx->c.l1_main) {
+            (void)hipStreamSynchronize(ctx->c.l1_main);
+            (void)hipStreamSynchronize(ctx->c.l1_side);
+            (void)hipStreamDestroy(ctx->c.l1_main);
+            (void)hipStreamDestroy(ctx->c.l1_side);
+            (void)hipEventDestroy(ctx->c.l1_fork);
+            (void)hipEventDestroy(ctx->c.l1_shaded);
+            (void)hipEventDestroy(ctx->c.l1_done);
+            (void)hipEventDestroy(ctx->c.ev_resolved[0]);
+            (void)hipEventDestroy(ctx->c.ev_resolved[1]);
+            (void)hipEventDestroy(ctx->c.ev_start);
+        }
         delete ctx;
         if (own && s) (void)hipStreamDestroy(s);
     } catch (...) {

@@ -208,6 +208,16 @@ struct Ctx {
     hipEvent_t ev_fork = nullptr, ev_shaded = nullptr, ev_done = nullptr;
 
     Wavefront wf;
+    // run_paths traces its chunks on two path pipelines at the same time:
+    // lane 0 is `wf` on `stream` and `side`, lane 1 is `wf2` on two streams
+    // of its own (created on first use, default priority).  A chunk's resolve
+    // waits for the previous chunk's (ev_resolved), so passes are added to
+    // the framebuffer in order.
+    Wavefront wf2;
+    hipStream_t l1_main = nullptr, l1_side = nullptr;
+    hipEvent_t l1_fork = nullptr, l1_shaded = nullptr, l1_done = nullptr;
+    hipEvent_t ev_resolved[2] = {nullptr, nullptr};
+    hipEvent_t ev_start = nullptr;
     KernelTimer timer;
     LaunchCfg launch;
     BatchState batch;

@@ -451,8 +451,7 @@ int trace_blocks(const Ctx *c) {
 }
 size_t trace_threads(const Ctx *c) { return (size_t)trace_blocks(c) * c->launch.trace_block; }
 
-void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
-    Wavefront &w = c->wf;
+static void ensure_wf(Ctx *c, Wavefront &w, size_t cap, int width, int depth) {
     int need = (width - 1) * (depth + 1) + 2;
     size_t ovf = need > AUR_STACK_LDS ? (size_t)(need - AUR_STACK_LDS) : 0;
     if (cap <= w.cap && ovf <= w.stack_ovf_per_thread && w.counts.p &&
@@ -485,10 +484,11 @@ void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) {
     w.cap = cap;
 }
 
+void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) { ensure_wf(c, c->wf, cap, width, depth); }
+
 static void ensure_wavefront(Ctx *c, size_t cap) { ensure_wavefront_for(c, cap, c->bvh.width, c->bvh.depth); }
 
-WfView wf_view(Ctx *c, const PathSource &src) {
-    Wavefront &w = c->wf;
+static WfView wf_view_of(Ctx *c, Wavefront &w, const PathSource &src) {
     WfView v;
     for (int k = 0; k < 2; ++k) {
         v.ray[k] = w.ray[k].as<float4>();
@@ -515,129 +515,198 @@ WfView wf_view(Ctx *c, const PathSource &src) {
     return v;
 }
 
-// One batch of n paths: generate, then maxdepth+1 rounds of
-// extend/shade/shadow/finish.  Results land in wf.out (by path id).
-static void run_batch(Ctx *c, const SceneView &sv, WfView wv, int n) {
+WfView wf_view(Ctx *c, const PathSource &src) { return wf_view_of(c, c->wf, src); }
+
+// The streams and events of one path pipeline.
+struct Lane {
+    hipStream_t main, side;
+    hipEvent_t fork, shaded, done;
+};
+
+// One batch of n paths on lane L: generate, then maxdepth+1 rounds of
+// extend/shade/shadow/finish.  Results land in the lane's out (by path id).
+static void run_batch(Ctx *c, const Lane &L, const SceneView &sv, WfView wv, int n) {
     // hash-sampled paths recompute their camera rays where they are used
     const bool cam_inline = sv.qmc == nullptr;
     if (cam_inline) {
-        TIMED(c, KT_GENERATE, hipLaunchKernelGGL(k_start, dim3(1), dim3(1), 0, c->stream, wv, n));
+        TIMED_ON(c, KT_GENERATE, L.main, hipLaunchKernelGGL(k_start, dim3(1), dim3(1), 0, L.main, wv, n));
     } else {
         const int gen_blocks = std::min((n + 255) / 256, persistent_blocks(c, 256));
-        TIMED(c, KT_GENERATE, hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, c->stream, sv, wv, n));
+        TIMED_ON(c, KT_GENERATE, L.main,
+                 hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(256), 0, L.main, sv, wv, n));
     }
     const int tb = trace_blocks(c), sb = persistent_blocks(c, 256);
     const bool tex = shade_needs_tex(c);
     // A bounce's shadow rays and finished paths go to the side stream, so
     // they run beside the next bounce's extension rays, which read and write
     // none of their buffers (the shadow traversal has its own stack overflow
-    // region); the next shade waits for them, and the caller's stream for
-    // the last of them.  With kernel timing on, everything stays on the
+    // region); the next shade waits for them, and the lane's stream for the
+    // last of them.  With kernel timing on, everything stays on the
     // context's stream so that each kernel's time is its own.
     const bool overlap = !c->timer.on;
-    if (overlap && !c->side) {
-        HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_shaded, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
-    }
-    hipStream_t s2 = overlap ? c->side : c->stream;
+    hipStream_t s2 = overlap ? L.side : L.main;
     if (overlap) {
-        HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
-        HIP_CHECK(hipStreamWaitEvent(s2, c->ev_fork, 0));
+        HIP_CHECK(hipEventRecord(L.fork, L.main));
+        HIP_CHECK(hipStreamWaitEvent(s2, L.fork, 0));
     }
     int q = 0;
     for (int it = 0; it <= sv.maxdepth; ++it) {
         const bool cam = cam_inline && it == 0;
-#define AUR_EXT(w_, b_)                                                                                    \
-    do {                                                                                                   \
-        if (cam) hipLaunchKernelGGL((k_extend<w_, b_, true>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q); \
-        else hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, sv, wv, q);          \
+#define AUR_EXT(w_, b_)                                                                                  \
+    do {                                                                                                 \
+        if (cam) hipLaunchKernelGGL((k_extend<w_, b_, true>), dim3(tb), dim3(b_), 0, L.main, sv, wv, q); \
+        else hipLaunchKernelGGL((k_extend<w_, b_>), dim3(tb), dim3(b_), 0, L.main, sv, wv, q);          \
     } while (0)
 #define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, s2, sv, wv, 1 - q)
-        TIMED(c, KT_EXTEND, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
-        if (overlap && it > 0) HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_done, 0));
-        if (tex && cam) TIMED(c, KT_SHADE, hipLaunchKernelGGL((k_shade<true, true>), dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
-        else if (tex) TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<true>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
-        else if (cam) TIMED(c, KT_SHADE, hipLaunchKernelGGL((k_shade<false, true>), dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
-        else TIMED(c, KT_SHADE, hipLaunchKernelGGL(k_shade<false>, dim3(sb), dim3(256), 0, c->stream, sv, wv, q));
+        TIMED_ON(c, KT_EXTEND, L.main, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
+        if (overlap && it > 0) HIP_CHECK(hipStreamWaitEvent(L.main, L.done, 0));
+        if (tex && cam) TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL((k_shade<true, true>), dim3(sb), dim3(256), 0, L.main, sv, wv, q));
+        else if (tex) TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL(k_shade<true>, dim3(sb), dim3(256), 0, L.main, sv, wv, q));
+        else if (cam) TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL((k_shade<false, true>), dim3(sb), dim3(256), 0, L.main, sv, wv, q));
+        else TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL(k_shade<false>, dim3(sb), dim3(256), 0, L.main, sv, wv, q));
         if (overlap) {
-            HIP_CHECK(hipEventRecord(c->ev_shaded, c->stream));
-            HIP_CHECK(hipStreamWaitEvent(s2, c->ev_shaded, 0));
+            HIP_CHECK(hipEventRecord(L.shaded, L.main));
+            HIP_CHECK(hipStreamWaitEvent(s2, L.shaded, 0));
         }
         TIMED_ON(c, KT_SHADOW, s2, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_SHD));
 #undef AUR_EXT
 #undef AUR_SHD
         TIMED_ON(c, KT_FINISH, s2, hipLaunchKernelGGL(k_finish, dim3(sb), dim3(256), 0, s2, sv, wv));
         hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, s2, wv.counts, q);
-        if (overlap) HIP_CHECK(hipEventRecord(c->ev_done, s2));
+        if (overlap) HIP_CHECK(hipEventRecord(L.done, s2));
         q = 1 - q;
     }
-    if (overlap) HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_done, 0));
+    if (overlap) HIP_CHECK(hipStreamWaitEvent(L.main, L.done, 0));
     HIP_CHECK(hipGetLastError());
 }
+
+static void ensure_lane_streams(Ctx *c) {
+    if (!c->side) {
+        HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_shaded, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
+    }
+    if (!c->l1_main) {
+        HIP_CHECK(hipStreamCreateWithFlags(&c->l1_main, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&c->l1_side, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&c->l1_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->l1_shaded, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->l1_done, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_resolved[0], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_resolved[1], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming));
+    }
+}
+
+// One chunk of run_paths: whole passes (mode 0) or a range of paths.
+struct Chunk {
+    size_t done, n;
+    int first_pass, passes;
+};
 
 void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, aur_stats *stats) {
     if (!c->built) throw StateError("render before aur_build_accel");
     const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
     const int ls = std::max(1, src.ls);
-    // mode 0 runs whole passes per batch (a pass of ls paths per pixel fits)
+    // Chunks of up to pool_paths paths; with kernel timing off they alternate
+    // between two lanes of half the pool each, so one chunk's shading and
+    // latency-bound last bounces run beside the other's traversal (two
+    // contexts rendering halves of the 1080p frame side by side measured
+    // 2060-2160 Mpaths/s against 1870-1900 for one; three or four, 1880-1980).
+    // mode 0 traces whole passes per chunk (a pass of ls paths per pixel fits)
+    const bool two = !c->timer.on && n_paths > 1;
     const size_t pool = c->launch.pool_paths;
-    size_t cap = src.mode == 0 ? std::max<size_t>(wh * ls, std::min<size_t>(n_paths, pool))
-                               : std::min<size_t>(n_paths, pool);
+    size_t want = std::min<size_t>(n_paths, two ? std::max<size_t>(pool / 2, 1) : pool);
+    if (two) want = std::min(want, (n_paths + 1) / 2);
+    const size_t cap = src.mode == 0 ? std::max<size_t>(wh * ls, want) : std::max<size_t>(want, 1);
     if (cap > kMaxPoolPaths) throw CapacityError("one pass needs more path slots than the queues can index");
-    ensure_wavefront(c, std::max<size_t>(cap, 1));
+    std::vector<Chunk> chunks;
+    {
+        size_t done = 0;
+        int pass = src.first_pass;
+        while (done < n_paths) {
+            Chunk ch{done, std::min(cap, n_paths - done), pass, 0};
+            if (src.mode == 0) {
+                // as many whole passes as fit
+                size_t m = 0;
+                while (done + m < n_paths) {
+                    const size_t np = wh * (size_t)(pass + ch.passes == 0 ? 1 : ls);
+                    if (ch.passes > 0 && m + np > cap) break;
+                    m += np;
+                    ++ch.passes;
+                }
+                if (done + m > n_paths) throw std::invalid_argument("path count is not a whole number of passes");
+                ch.n = m;
+                pass += ch.passes;
+            }
+            chunks.push_back(ch);
+            done += ch.n;
+        }
+    }
+    const int n_lanes = two && chunks.size() > 1 ? 2 : 1;
+    Wavefront *wfs[2] = {&c->wf, &c->wf2};
+    for (int l = 0; l < n_lanes; ++l) ensure_wf(c, *wfs[l], cap, c->bvh.width, c->bvh.depth);
+    if (!c->timer.on) ensure_lane_streams(c);
+    const Lane lanes[2] = {
+        {c->stream, c->timer.on ? c->stream : c->side, c->ev_fork, c->ev_shaded, c->ev_done},
+        {c->l1_main, c->l1_side, c->l1_fork, c->l1_shaded, c->l1_done},
+    };
     SceneView sv = scene_view(c);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, c->stream));
-    hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, c->wf.counts.as<int>(),
-                       c->wf.stats.as<unsigned long long>(), 1);
-    fill_zero(c, c->wf.partials.p, sizeof(unsigned long long) * trace_blocks(c));
-    size_t done = 0;
-    int pass = src.first_pass;
-    while (done < n_paths) {
-        size_t n = std::min(cap, n_paths - done);
+    for (int l = 0; l < n_lanes; ++l) {
+        hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, wfs[l]->counts.as<int>(),
+                           wfs[l]->stats.as<unsigned long long>(), 1);
+        fill_zero(c, wfs[l]->partials.p, sizeof(unsigned long long) * trace_blocks(c));
+    }
+    if (n_lanes == 2) {
+        HIP_CHECK(hipEventRecord(c->ev_start, c->stream));
+        HIP_CHECK(hipStreamWaitEvent(c->l1_main, c->ev_start, 0));
+    }
+    for (size_t k = 0; k < chunks.size(); ++k) {
+        const Chunk &ch = chunks[k];
+        const int l = (int)(k % (size_t)n_lanes);
+        const Lane &L = lanes[l];
         PathSource s = src;
         s.ls = ls;
-        int passes = 0;
         if (src.mode == 0) {
-            // as many whole passes as fit
-            s.first_pass = pass;
-            size_t m = 0;
-            while (done + m < n_paths) {
-                const size_t np = wh * (size_t)(pass + passes == 0 ? 1 : ls);
-                if (passes > 0 && m + np > cap) break;
-                m += np;
-                ++passes;
-            }
-            if (done + m > n_paths) throw std::invalid_argument("path count is not a whole number of passes");
-            n = m;
-            pass += passes;
+            s.first_pass = ch.first_pass;
         } else {
-            s.pixel = src.pixel + done;
-            s.sample = src.sample + done;
-            s.slots = src.slots ? src.slots + done * (size_t)(2 + AUR_SLOTS_PER_VERTEX * c->scene.opt.maxdepth) : nullptr;
+            s.pixel = src.pixel + ch.done;
+            s.sample = src.sample + ch.done;
+            s.slots = src.slots ? src.slots + ch.done * (size_t)(2 + AUR_SLOTS_PER_VERTEX * c->scene.opt.maxdepth) : nullptr;
         }
-        if (src.adj_rec) s.rec_base = src.rec_base + (long long)done;
-        WfView wv = wf_view(c, s);
-        hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, c->stream, wv.counts, wv.stats, 0);
-        run_batch(c, sv, wv, (int)n);
+        if (src.adj_rec) s.rec_base = src.rec_base + (long long)ch.done;
+        WfView wv = wf_view_of(c, *wfs[l], s);
+        hipLaunchKernelGGL(k_reset_all, dim3(1), dim3(64), 0, L.main, wv.counts, wv.stats, 0);
+        run_batch(c, L, sv, wv, (int)ch.n);
         if (src.adj_rec && src.rec_only) {
             // records only
         } else if (src.mode == 0) {
+            // passes go into the framebuffer in order: after the previous
+            // chunk's resolve, which ran on the other lane
+            if (n_lanes == 2 && k > 0) HIP_CHECK(hipStreamWaitEvent(L.main, c->ev_resolved[1 - l], 0));
             int blocks = std::min<int>((int)((wh + 255) / 256), persistent_blocks(c, 256));
-            TIMED(c, KT_RESOLVE, hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, c->stream, wv.out, (int)wh,
-                                                    s.first_pass, passes, ls, out_rgba, src.display,
-                                                    (int)(src.display_base + (s.first_pass - src.first_pass))));
+            TIMED_ON(c, KT_RESOLVE, L.main,
+                     hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, L.main, wv.out, (int)wh,
+                                        s.first_pass, ch.passes, ls, out_rgba, src.display,
+                                        (int)(src.display_base + (s.first_pass - src.first_pass))));
+            if (n_lanes == 2) HIP_CHECK(hipEventRecord(c->ev_resolved[l], L.main));
         } else {
-            HIP_CHECK(hipMemcpyAsync(out_rgba + done, wv.out, sizeof(float4) * n, hipMemcpyDeviceToDevice, c->stream));
+            HIP_CHECK(hipMemcpyAsync(out_rgba + ch.done, wv.out, sizeof(float4) * ch.n, hipMemcpyDeviceToDevice, L.main));
         }
-        done += n;
     }
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->wf.partials.as<unsigned long long>(),
-                       trace_blocks(c), c->wf.stats.as<unsigned long long>());
+    if (n_lanes == 2) {
+        // the context's stream waits for lane 1
+        HIP_CHECK(hipEventRecord(c->ev_start, c->l1_main));
+        HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_start, 0));
+    }
+    for (int l = 0; l < n_lanes; ++l)
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, wfs[l]->partials.as<unsigned long long>(),
+                           trace_blocks(c), wfs[l]->stats.as<unsigned long long>());
     HIP_CHECK(hipEventRecord(e1, c->stream));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -646,12 +715,16 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     (void)hipEventDestroy(e1);
     timer_collect(c);
     if (stats) {
-        unsigned long long s[8];
-        HIP_CHECK(hipMemcpy(s, c->wf.stats.p, sizeof s, hipMemcpyDeviceToHost));
-        stats->paths = (int64_t)s[S_PATHS];
-        stats->ext_rays = (int64_t)s[S_EXT];
-        stats->hit_rays = (int64_t)s[S_HIT];
-        stats->shadow_rays = (int64_t)s[S_SHADOW];
+        unsigned long long tot[8] = {0};
+        for (int l = 0; l < n_lanes; ++l) {
+            unsigned long long s8[8];
+            HIP_CHECK(hipMemcpy(s8, wfs[l]->stats.p, sizeof s8, hipMemcpyDeviceToHost));
+            for (int j = 0; j < 8; ++j) tot[j] += s8[j];
+        }
+        stats->paths = (int64_t)tot[S_PATHS];
+        stats->ext_rays = (int64_t)tot[S_EXT];
+        stats->hit_rays = (int64_t)tot[S_HIT];
+        stats->shadow_rays = (int64_t)tot[S_SHADOW];
         stats->ms = ms;
     }
 }
