@@ -511,23 +511,18 @@ void aur_destroy(aur_ctx *ctx) {
         }
         hipStream_t s = ctx->c.stream;
         bool own = ctx->c.own_stream;
-        if (ctx->c.side) {
-            (void)hipStreamSynchronize(ctx->c.side);
-            (void)hipStreamDestroy(ctx->c.side);
-            (void)hipEventDestroy(ctx->c.ev_fork);
-            (void)hipEventDestroy(ctx->c.ev_shaded);
-            (void)hipEventDestroy(ctx->c.ev_done);
-        }
-        if (ctx->c.l1_main) {
-            (void)hipStreamSynchronize(ctx->c.l1_main);
-            (void)hipStreamSynchronize(ctx->c.l1_side);
-            (void)hipStreamDestroy(ctx->c.l1_main);
-            (void)hipStreamDestroy(ctx->c.l1_side);
-            (void)hipEventDestroy(ctx->c.l1_fork);
-            (void)hipEventDestroy(ctx->c.l1_shaded);
-            (void)hipEventDestroy(ctx->c.l1_done);
-            (void)hipEventDestroy(ctx->c.ev_resolved[0]);
-            (void)hipEventDestroy(ctx->c.ev_resolved[1]);
+        if (ctx->c.lane_stream[0]) {
+            for (hipStream_t ls : ctx->c.lane_stream) {
+                (void)hipStreamSynchronize(ls);
+                (void)hipStreamDestroy(ls);
+            }
+            for (int l = 0; l < 2; ++l) {
+                (void)hipEventDestroy(ctx->c.lane_fork[l]);
+                (void)hipEventDestroy(ctx->c.lane_shaded[l]);
+                (void)hipEventDestroy(ctx->c.lane_done[l]);
+                (void)hipEventDestroy(ctx->c.ev_resolved[l]);
+                (void)hipEventDestroy(ctx->c.ev_join[l]);
+            }
             (void)hipEventDestroy(ctx->c.ev_start);
         }
         delete ctx;

@@ -201,23 +201,22 @@ struct Ctx {
     size_t rec_wh = 0;
     std::vector<uint64_t> rec_version;
 
-    // Second stream of the wavefront: a bounce's shadow rays and finished
-    // paths are traced on it while the next bounce's extension rays are
-    // traced on `stream` (created on first use, default priority).
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_shaded = nullptr, ev_done = nullptr;
-
+    // run_paths traces its chunks on two path pipelines ("lanes") at the same
+    // time: lane 0 on `wf`, lane 1 on `wf2`.  Each lane has a main stream and
+    // a side stream (a bounce's shadow rays and finished paths are traced on
+    // the side stream while the next bounce's extension rays are traced on
+    // the main one): lane_stream[2 l] and lane_stream[2 l + 1], all four
+    // created together on first use (default priority), so that the runtime
+    // spreads them over its hardware queues even when many contexts exist.
+    // `stream` forks into them (ev_start) and joins them at the end
+    // (ev_join); a chunk's resolve waits for the previous chunk's
+    // (ev_resolved), so passes are added to the framebuffer in order.
     Wavefront wf;
-    // run_paths traces its chunks on two path pipelines at the same time:
-    // lane 0 is `wf` on `stream` and `side`, lane 1 is `wf2` on two streams
-    // of its own (created on first use, default priority).  A chunk's resolve
-    // waits for the previous chunk's (ev_resolved), so passes are added to
-    // the framebuffer in order.
     Wavefront wf2;
-    hipStream_t l1_main = nullptr, l1_side = nullptr;
-    hipEvent_t l1_fork = nullptr, l1_shaded = nullptr, l1_done = nullptr;
-    hipEvent_t ev_resolved[2] = {nullptr, nullptr};
-    hipEvent_t ev_start = nullptr;
+    hipStream_t lane_stream[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t lane_fork[2] = {nullptr, nullptr}, lane_shaded[2] = {nullptr, nullptr};
+    hipEvent_t lane_done[2] = {nullptr, nullptr}, ev_resolved[2] = {nullptr, nullptr};
+    hipEvent_t ev_start = nullptr, ev_join[2] = {nullptr, nullptr};
     KernelTimer timer;
     LaunchCfg launch;
     BatchState batch;

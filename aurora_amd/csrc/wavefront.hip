@@ -581,22 +581,16 @@ static void run_batch(Ctx *c, const Lane &L, const SceneView &sv, WfView wv, int
 }
 
 static void ensure_lane_streams(Ctx *c) {
-    if (!c->side) {
-        HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_shaded, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
+    if (c->lane_stream[0]) return;
+    for (hipStream_t &ls : c->lane_stream) HIP_CHECK(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+    for (int l = 0; l < 2; ++l) {
+        HIP_CHECK(hipEventCreateWithFlags(&c->lane_fork[l], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->lane_shaded[l], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->lane_done[l], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_resolved[l], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_join[l], hipEventDisableTiming));
     }
-    if (!c->l1_main) {
-        HIP_CHECK(hipStreamCreateWithFlags(&c->l1_main, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamCreateWithFlags(&c->l1_side, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&c->l1_fork, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->l1_shaded, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->l1_done, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_resolved[0], hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_resolved[1], hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming));
-    }
+    HIP_CHECK(hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming));
 }
 
 // One chunk of run_paths: whole passes (mode 0) or a range of paths.
@@ -609,13 +603,18 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     if (!c->built) throw StateError("render before aur_build_accel");
     const size_t wh = (size_t)c->scene.opt.width * c->scene.opt.height;
     const int ls = std::max(1, src.ls);
-    // Chunks of up to pool_paths paths; with kernel timing off they alternate
-    // between two lanes of half the pool each, so one chunk's shading and
-    // latency-bound last bounces run beside the other's traversal (two
-    // contexts rendering halves of the 1080p frame side by side measured
-    // 2060-2160 Mpaths/s against 1870-1900 for one; three or four, 1880-1980).
-    // mode 0 traces whole passes per chunk (a pass of ls paths per pixel fits)
-    const bool two = !c->timer.on && n_paths > 1;
+    // Chunks of up to pool_paths paths; with kernel timing off, renders of at
+    // least kTwoLaneMin paths alternate their chunks between two lanes of half
+    // the pool each, so one chunk's shading and latency-bound last bounces run
+    // beside the other's traversal (two contexts rendering halves of the 1080p
+    // frame side by side measured 2060-2160 Mpaths/s against 1870-1900 for
+    // one; three or four, 1880-1980).  Below about a million paths per lane
+    // the second lane costs more than it overlaps (box room 256x256: 4 passes
+    // 330 against 500 Mpaths/s, 16 passes 970 against 997, 64 passes 1583
+    // against 1312).  mode 0 traces whole passes per chunk (a pass of ls
+    // paths per pixel fits).
+    constexpr size_t kTwoLaneMin = size_t(1) << 21;
+    const bool two = !c->timer.on && n_paths >= kTwoLaneMin;
     const size_t pool = c->launch.pool_paths;
     size_t want = std::min<size_t>(n_paths, two ? std::max<size_t>(pool / 2, 1) : pool);
     if (two) want = std::min(want, (n_paths + 1) / 2);
@@ -647,10 +646,16 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     const int n_lanes = two && chunks.size() > 1 ? 2 : 1;
     Wavefront *wfs[2] = {&c->wf, &c->wf2};
     for (int l = 0; l < n_lanes; ++l) ensure_wf(c, *wfs[l], cap, c->bvh.width, c->bvh.depth);
+    // With kernel timing on, everything runs on the context's stream; one
+    // lane runs on it with lane 0's side stream (a fork into a stream of its
+    // own measured 3 % slower on 1 ms renders); two lanes run on the four
+    // lane streams, forked from the context's stream and joined back.
     if (!c->timer.on) ensure_lane_streams(c);
+    const bool lanes_on = n_lanes == 2;
     const Lane lanes[2] = {
-        {c->stream, c->timer.on ? c->stream : c->side, c->ev_fork, c->ev_shaded, c->ev_done},
-        {c->l1_main, c->l1_side, c->l1_fork, c->l1_shaded, c->l1_done},
+        {lanes_on ? c->lane_stream[0] : c->stream, c->timer.on ? c->stream : c->lane_stream[1], c->lane_fork[0],
+         c->lane_shaded[0], c->lane_done[0]},
+        {c->lane_stream[2], c->lane_stream[3], c->lane_fork[1], c->lane_shaded[1], c->lane_done[1]},
     };
     SceneView sv = scene_view(c);
     hipEvent_t e0, e1;
@@ -662,9 +667,9 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
                            wfs[l]->stats.as<unsigned long long>(), 1);
         fill_zero(c, wfs[l]->partials.p, sizeof(unsigned long long) * trace_blocks(c));
     }
-    if (n_lanes == 2) {
+    if (lanes_on) {
         HIP_CHECK(hipEventRecord(c->ev_start, c->stream));
-        HIP_CHECK(hipStreamWaitEvent(c->l1_main, c->ev_start, 0));
+        for (int l = 0; l < n_lanes; ++l) HIP_CHECK(hipStreamWaitEvent(lanes[l].main, c->ev_start, 0));
     }
     for (size_t k = 0; k < chunks.size(); ++k) {
         const Chunk &ch = chunks[k];
@@ -699,10 +704,12 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
             HIP_CHECK(hipMemcpyAsync(out_rgba + ch.done, wv.out, sizeof(float4) * ch.n, hipMemcpyDeviceToDevice, L.main));
         }
     }
-    if (n_lanes == 2) {
-        // the context's stream waits for lane 1
-        HIP_CHECK(hipEventRecord(c->ev_start, c->l1_main));
-        HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_start, 0));
+    if (lanes_on) {
+        // the context's stream waits for the lanes
+        for (int l = 0; l < n_lanes; ++l) {
+            HIP_CHECK(hipEventRecord(c->ev_join[l], lanes[l].main));
+            HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join[l], 0));
+        }
     }
     for (int l = 0; l < n_lanes; ++l)
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, wfs[l]->partials.as<unsigned long long>(),
