@@ -492,6 +492,7 @@ int aur_create(aur_ctx **out, int hip_device) {
         noalloc_init(hip_device);  // the device is current (set_device above)
         HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
         c->c.own_stream = true;
+        ensure_lane_streams(&c->c);
         *out = c.release();
     });
 }
@@ -521,9 +522,9 @@ void aur_destroy(aur_ctx *ctx) {
                 (void)hipEventDestroy(ctx->c.lane_shaded[l]);
                 (void)hipEventDestroy(ctx->c.lane_done[l]);
                 (void)hipEventDestroy(ctx->c.ev_resolved[l]);
-                (void)hipEventDestroy(ctx->c.ev_join[l]);
             }
             (void)hipEventDestroy(ctx->c.ev_start);
+            (void)hipEventDestroy(ctx->c.ev_join);
         }
         delete ctx;
         if (own && s) (void)hipStreamDestroy(s);

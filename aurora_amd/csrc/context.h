@@ -205,18 +205,19 @@ struct Ctx {
     // time: lane 0 on `wf`, lane 1 on `wf2`.  Each lane has a main stream and
     // a side stream (a bounce's shadow rays and finished paths are traced on
     // the side stream while the next bounce's extension rays are traced on
-    // the main one): lane_stream[2 l] and lane_stream[2 l + 1], all four
-    // created together on first use (default priority), so that the runtime
-    // spreads them over its hardware queues even when many contexts exist.
-    // `stream` forks into them (ev_start) and joins them at the end
-    // (ev_join); a chunk's resolve waits for the previous chunk's
-    // (ev_resolved), so passes are added to the framebuffer in order.
+    // the main one).  Lane 0 runs on `stream` and lane_stream[0], lane 1 on
+    // lane_stream[1] and [2]; the three are created right after `stream`, so
+    // that the runtime puts the four streams on different hardware queues
+    // even when many contexts exist (default priority).  `stream` forks into
+    // lane 1 (ev_start) and joins it at the end (ev_join); a chunk's resolve
+    // waits for the previous chunk's (ev_resolved), so passes are added to
+    // the framebuffer in order.
     Wavefront wf;
     Wavefront wf2;
-    hipStream_t lane_stream[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t lane_stream[3] = {nullptr, nullptr, nullptr};
     hipEvent_t lane_fork[2] = {nullptr, nullptr}, lane_shaded[2] = {nullptr, nullptr};
     hipEvent_t lane_done[2] = {nullptr, nullptr}, ev_resolved[2] = {nullptr, nullptr};
-    hipEvent_t ev_start = nullptr, ev_join[2] = {nullptr, nullptr};
+    hipEvent_t ev_start = nullptr, ev_join = nullptr;
     KernelTimer timer;
     LaunchCfg launch;
     BatchState batch;
@@ -273,6 +274,8 @@ void noalloc_init(int device);
 void trace_rays_noalloc(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, int mode);
 // sizes the path pool and the traversal stack overflow for BVHs of the given width and depth
 void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth);
+// creates the lane streams and events of run_paths (aur_create, after `stream`)
+void ensure_lane_streams(Ctx *c);
 // batch.hip: jobs[k] renders passes [first_pass[k], first_pass[k] + n_passes[k]) into its framebuffer
 void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_passes, int n_jobs, int order, int use_graph,
               int iters_per_poll, aur_job_result *results, aur_stats *total_stats);

@@ -580,7 +580,7 @@ static void run_batch(Ctx *c, const Lane &L, const SceneView &sv, WfView wv, int
     HIP_CHECK(hipGetLastError());
 }
 
-static void ensure_lane_streams(Ctx *c) {
+void ensure_lane_streams(Ctx *c) {
     if (c->lane_stream[0]) return;
     for (hipStream_t &ls : c->lane_stream) HIP_CHECK(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
     for (int l = 0; l < 2; ++l) {
@@ -588,9 +588,9 @@ static void ensure_lane_streams(Ctx *c) {
         HIP_CHECK(hipEventCreateWithFlags(&c->lane_shaded[l], hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->lane_done[l], hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_resolved[l], hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_join[l], hipEventDisableTiming));
     }
     HIP_CHECK(hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
 }
 
 // One chunk of run_paths: whole passes (mode 0) or a range of paths.
@@ -646,16 +646,12 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     const int n_lanes = two && chunks.size() > 1 ? 2 : 1;
     Wavefront *wfs[2] = {&c->wf, &c->wf2};
     for (int l = 0; l < n_lanes; ++l) ensure_wf(c, *wfs[l], cap, c->bvh.width, c->bvh.depth);
-    // With kernel timing on, everything runs on the context's stream; one
-    // lane runs on it with lane 0's side stream (a fork into a stream of its
-    // own measured 3 % slower on 1 ms renders); two lanes run on the four
-    // lane streams, forked from the context's stream and joined back.
-    if (!c->timer.on) ensure_lane_streams(c);
+    // with kernel timing on, everything runs on the context's stream
+    ensure_lane_streams(c);
     const bool lanes_on = n_lanes == 2;
     const Lane lanes[2] = {
-        {lanes_on ? c->lane_stream[0] : c->stream, c->timer.on ? c->stream : c->lane_stream[1], c->lane_fork[0],
-         c->lane_shaded[0], c->lane_done[0]},
-        {c->lane_stream[2], c->lane_stream[3], c->lane_fork[1], c->lane_shaded[1], c->lane_done[1]},
+        {c->stream, c->timer.on ? c->stream : c->lane_stream[0], c->lane_fork[0], c->lane_shaded[0], c->lane_done[0]},
+        {c->lane_stream[1], c->lane_stream[2], c->lane_fork[1], c->lane_shaded[1], c->lane_done[1]},
     };
     SceneView sv = scene_view(c);
     hipEvent_t e0, e1;
@@ -669,7 +665,7 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
     }
     if (lanes_on) {
         HIP_CHECK(hipEventRecord(c->ev_start, c->stream));
-        for (int l = 0; l < n_lanes; ++l) HIP_CHECK(hipStreamWaitEvent(lanes[l].main, c->ev_start, 0));
+        HIP_CHECK(hipStreamWaitEvent(lanes[1].main, c->ev_start, 0));
     }
     for (size_t k = 0; k < chunks.size(); ++k) {
         const Chunk &ch = chunks[k];
@@ -705,11 +701,9 @@ void run_paths(Ctx *c, const PathSource &src, size_t n_paths, float4 *out_rgba, 
         }
     }
     if (lanes_on) {
-        // the context's stream waits for the lanes
-        for (int l = 0; l < n_lanes; ++l) {
-            HIP_CHECK(hipEventRecord(c->ev_join[l], lanes[l].main));
-            HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join[l], 0));
-        }
+        // the context's stream (lane 0) waits for lane 1
+        HIP_CHECK(hipEventRecord(c->ev_join, lanes[1].main));
+        HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     for (int l = 0; l < n_lanes; ++l)
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, wfs[l]->partials.as<unsigned long long>(),
