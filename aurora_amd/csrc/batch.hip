@@ -48,7 +48,7 @@ struct JobDev {
     uint32_t seed;
     int first_pass;
     int ls;              // light samples: paths per pixel in passes after pass 0
-    long long start;     // first work index of the job in launch order
+    long long start;     // first work index of the job in its pipeline's work sequence
     long long n_paths;
     int root;            // the job's root node in the launch's node arena
 };
@@ -427,8 +427,8 @@ __global__ void k_bresolve(BatchView bv, const ResolveSeg *__restrict__ segs, in
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static BatchView batch_view(Ctx *c, int n_jobs, long long total, long long ring_mask) {
-    BatchState &b = c->batch;
+// The device view of one pipeline's state; the node arena is `arena`'s.
+static BatchView batch_view(BatchState &b, const BatchState &arena, int n_jobs, long long total, long long ring_mask) {
     BatchView v;
     v.jobs = b.jobs.as<JobDev>();
     v.starts = b.starts.as<long long>();
@@ -441,31 +441,31 @@ static BatchView batch_view(Ctx *c, int n_jobs, long long total, long long ring_
     v.t_end = b.t_end.as<unsigned long long>();
     v.t0 = b.t0.as<long long>();
     v.jstats = b.jstats.as<unsigned long long>();
-    v.nodes = b.arena_nodes.p;
-    v.tris = b.arena_tris.as<float4>();
+    v.nodes = arena.arena_nodes.p;
+    v.tris = arena.arena_tris.as<float4>();
     return v;
 }
 
-// One iteration over extension queue q.
-static void batch_iteration(Ctx *c, const BatchView &bv, const WfView &wv, int q, int width, bool tex) {
+// One iteration over extension queue q, on stream st.
+static void batch_iteration(Ctx *c, hipStream_t st, const BatchView &bv, const WfView &wv, int q, int width, bool tex) {
     const int tb = trace_blocks(c);
     const int sb = c->n_cu * 4;
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, c->stream, bv, wv, q);
-    hipLaunchKernelGGL(k_regen, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, bv, wv, q);
+    hipLaunchKernelGGL(k_regen, dim3(sb), dim3(256), 0, st, bv, wv, q);
 #define AUR_BEXT(w_, b_)                                                                                    \
-    hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs,                       \
+    hipLaunchKernelGGL((k_bextend<w_, b_>), dim3(tb), dim3(b_), 0, st, bv.jobs,                              \
                        static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, q)
 #define AUR_BSHD(w_, b_)                                                                                    \
-    hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, c->stream, bv.jobs,                       \
+    hipLaunchKernelGGL((k_bshadow<w_, b_>), dim3(tb), dim3(b_), 0, st, bv.jobs,                              \
                        static_cast<const WNode<w_> *>(bv.nodes), bv.tris, wv, 1 - q)
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BEXT);
-    if (tex) hipLaunchKernelGGL(k_bshade<true>, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
-    else hipLaunchKernelGGL(k_bshade<false>, dim3(sb), dim3(256), 0, c->stream, bv, wv, q);
+    if (tex) hipLaunchKernelGGL(k_bshade<true>, dim3(sb), dim3(256), 0, st, bv, wv, q);
+    else hipLaunchKernelGGL(k_bshade<false>, dim3(sb), dim3(256), 0, st, bv, wv, q);
     AUR_TRAV_DISPATCH(width, c->launch.trace_block, AUR_BSHD);
 #undef AUR_BEXT
 #undef AUR_BSHD
-    hipLaunchKernelGGL(k_bfinish, dim3(sb), dim3(256), 0, c->stream, bv, wv);
-    hipLaunchKernelGGL(k_bclear_queue, dim3(1), dim3(64), 0, c->stream, wv.counts, q);
+    hipLaunchKernelGGL(k_bfinish, dim3(sb), dim3(256), 0, st, bv, wv);
+    hipLaunchKernelGGL(k_bclear_queue, dim3(1), dim3(64), 0, st, wv.counts, q);
 }
 
 static double job_cost(const Ctx *j, long long paths) {
@@ -488,11 +488,10 @@ static long long pass_floor(long long x, long long wh, int first_pass, int ls) {
 
 // Resolves every job's whole passes finished below global work index `to`
 // (its frontier job_done moves pass by pass) and sums their statistics.
-static void resolve_range(Ctx *c, const BatchView &bv, const std::vector<long long> &starts,
+static void resolve_range(Ctx *c, BatchState &b, hipStream_t st, const BatchView &bv, const std::vector<long long> &starts,
                           const std::vector<long long> &npaths, const std::vector<int> &wh,
                           const std::vector<int> &jfirst, const std::vector<int> &jls, std::vector<long long> &job_done,
                           long long to) {
-    BatchState &b = c->batch;
     ResolveSeg *seg = static_cast<ResolveSeg *>(b.host_segs);
     StatChunk *chk = static_cast<StatChunk *>(b.host_chunks);
     int n = 0, nc = 0;
@@ -520,11 +519,11 @@ static void resolve_range(Ctx *c, const BatchView &bv, const std::vector<long lo
         }
     }
     if (n == 0) return;
-    HIP_CHECK(hipMemcpyAsync(b.segs.p, b.host_segs, sizeof(ResolveSeg) * n, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(b.chunks.p, b.host_chunks, sizeof(StatChunk) * nc, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(b.segs.p, b.host_segs, sizeof(ResolveSeg) * n, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(b.chunks.p, b.host_chunks, sizeof(StatChunk) * nc, hipMemcpyHostToDevice, st));
     const long long blocks = std::min<long long>((threads + 255) / 256, (long long)c->n_cu * 8);
-    hipLaunchKernelGGL(k_bresolve, dim3((unsigned)blocks), dim3(256), 0, c->stream, bv, b.segs.as<ResolveSeg>(), n, threads);
-    hipLaunchKernelGGL(k_bstats, dim3(nc), dim3(256), 0, c->stream, bv, b.chunks.as<StatChunk>());
+    hipLaunchKernelGGL(k_bresolve, dim3((unsigned)blocks), dim3(256), 0, st, bv, b.segs.as<ResolveSeg>(), n, threads);
+    hipLaunchKernelGGL(k_bstats, dim3(nc), dim3(256), 0, st, bv, b.chunks.as<StatChunk>());
     HIP_CHECK(hipGetLastError());
 }
 
@@ -532,7 +531,6 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
               int iters_per_poll, aur_job_result *results, aur_stats *total_stats) {
     if (n_jobs <= 0) throw std::invalid_argument("no jobs");
     int width = 0, depth = 0, max_path = 0;
-    long long max_pass = 0;  // work items of the largest pass
     bool tex = false;
     std::vector<long long> paths(n_jobs);
     for (int k = 0; k < n_jobs; ++k) {
@@ -553,101 +551,151 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
         tex = tex || shade_needs_tex(j);
         paths[k] = (long long)pass_paths((size_t)j->scene.opt.width * j->scene.opt.height, first_pass[k], n_passes[k],
                                          j->scene.opt.lightsamples);
-        max_pass = std::max(max_pass, (long long)j->scene.opt.width * j->scene.opt.height * j->scene.opt.lightsamples);
     }
     // launch order
     std::vector<int> ord(n_jobs);
     std::iota(ord.begin(), ord.end(), 0);
     if (order == 0)
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return job_cost(jobs[a], paths[a]) < job_cost(jobs[b], paths[b]); });
-    // job records and framebuffers
-    std::vector<JobDev> jd(n_jobs);
-    std::vector<long long> starts(n_jobs), npaths(n_jobs);
-    std::vector<int> wh(n_jobs), jfirst(n_jobs), jls(n_jobs);
-    std::vector<long long> job_done(n_jobs, 0);  // resolved work prefix of each job
-    long long total = 0;
-    for (int r = 0; r < n_jobs; ++r) {
-        Ctx *j = jobs[ord[r]];
-        HIP_CHECK(hipStreamSynchronize(j->stream));
-        const int w = j->scene.opt.width, h = j->scene.opt.height;
-        wh[r] = w * h;
-        bool resized = j->fb_w != w || j->fb_h != h;
-        j->accum.reserve(sizeof(float4) * (size_t)wh[r]);
-        j->display.reserve(sizeof(ushort4) * (size_t)wh[r]);
-        if (first_pass[ord[r]] == 0 || resized) {
-            fill_zero(c, j->accum.p, sizeof(float4) * (size_t)wh[r]);
-            fill_zero(c, j->display.p, sizeof(ushort4) * (size_t)wh[r]);
-            j->display_passes = 0;
+    long long total_all = 0;
+    for (int k = 0; k < n_jobs; ++k) total_all += paths[k];
+    // Two pipelines ("lanes") run side by side when the work is large enough
+    // (as single-scene renders do): each takes jobs in launch order, a job
+    // going to the lane with less estimated work so far, and has its own job
+    // table, control block, ring, path pool and stream; the node arena is
+    // shared.  The host polls the lanes in turn and queues a lane's next
+    // iterations right after handling its status.
+    const int n_lanes = (n_jobs >= 2 && total_all >= (1LL << 21)) ? 2 : 1;
+    struct BLane {
+        BatchState *b;
+        Wavefront *wf;
+        hipStream_t stream;
+        std::vector<int> rows;  // launch positions, in this lane's order
+        std::vector<JobDev> jd;
+        std::vector<long long> starts, npaths, job_done;
+        std::vector<int> wh, jfirst, jls;
+        long long total = 0, max_pass = 0, resolved = 0;
+        size_t pool = 0, ring = 0;
+        WfView wv;
+        BatchView bv;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        bool done = false;
+    };
+    BLane lanes[2];
+    lanes[0].b = &c->batch;
+    lanes[0].wf = &c->wf;
+    lanes[0].stream = c->stream;
+    lanes[1].b = &c->batch2;
+    lanes[1].wf = &c->wf2;
+    lanes[1].stream = c->lane_stream[1];
+    {
+        double cost[2] = {0.0, 0.0};
+        for (int r = 0; r < n_jobs; ++r) {
+            const int l = (n_lanes == 2 && cost[1] < cost[0]) ? 1 : 0;
+            cost[l] += job_cost(jobs[ord[r]], paths[ord[r]]);
+            lanes[l].rows.push_back(r);
         }
-        j->fb_w = w;
-        j->fb_h = h;
-        jd[r].sv = scene_view(j);
-        jd[r].accum = j->accum.as<float4>();
-        jd[r].display = j->display.as<ushort4>();
-        jd[r].display_base = j->display_passes;
-        jd[r].seed = j->seed;
-        jd[r].first_pass = first_pass[ord[r]];
-        jd[r].ls = j->scene.opt.lightsamples;
-        jfirst[r] = jd[r].first_pass;
-        jls[r] = jd[r].ls;
-        jd[r].start = total;
-        jd[r].n_paths = paths[ord[r]];
-        starts[r] = total;
-        npaths[r] = paths[ord[r]];
-        total += paths[ord[r]];
     }
-    // buffers
-    const size_t pool = std::max<size_t>(64, std::min<size_t>(c->launch.pool_paths, (size_t)std::max<long long>(total, 64)));
-    if (pool > kMaxPoolPaths) throw CapacityError("path pool too large");
+    // node arena offsets per launch position
+    std::vector<long long> node_off(n_jobs), tri_off(n_jobs);
+    long long node_total = 0, tri_total = 0;
+    for (int r = 0; r < n_jobs; ++r) {
+        const Ctx *j = jobs[ord[r]];
+        node_off[r] = node_total;
+        tri_off[r] = tri_total;
+        node_total += j->bvh.n_nodes;
+        tri_total += j->n_tris;
+    }
+    if (node_total > 0x7ffffffeLL || tri_total >= (1LL << 27))
+        throw CapacityError("the batch's BVHs exceed the node arena's index range");
+    // job records and framebuffers
+    for (int l = 0; l < n_lanes; ++l) {
+        BLane &L = lanes[l];
+        for (int r : L.rows) {
+            Ctx *j = jobs[ord[r]];
+            HIP_CHECK(hipStreamSynchronize(j->stream));
+            const int w = j->scene.opt.width, h = j->scene.opt.height;
+            const int whr = w * h;
+            bool resized = j->fb_w != w || j->fb_h != h;
+            j->accum.reserve(sizeof(float4) * (size_t)whr);
+            j->display.reserve(sizeof(ushort4) * (size_t)whr);
+            if (first_pass[ord[r]] == 0 || resized) {
+                fill_zero(c, j->accum.p, sizeof(float4) * (size_t)whr);
+                fill_zero(c, j->display.p, sizeof(ushort4) * (size_t)whr);
+                j->display_passes = 0;
+            }
+            j->fb_w = w;
+            j->fb_h = h;
+            JobDev d{};
+            d.sv = scene_view(j);
+            d.accum = j->accum.as<float4>();
+            d.display = j->display.as<ushort4>();
+            d.display_base = j->display_passes;
+            d.seed = j->seed;
+            d.first_pass = first_pass[ord[r]];
+            d.ls = j->scene.opt.lightsamples;
+            d.start = L.total;
+            d.n_paths = paths[ord[r]];
+            d.root = (int)node_off[r];
+            L.jd.push_back(d);
+            L.wh.push_back(whr);
+            L.jfirst.push_back(d.first_pass);
+            L.jls.push_back(d.ls);
+            L.starts.push_back(L.total);
+            L.npaths.push_back(paths[ord[r]]);
+            L.job_done.push_back(0);
+            L.total += paths[ord[r]];
+            L.max_pass = std::max(L.max_pass, (long long)whr * d.ls);
+        }
+    }
     // iterations between polls, and the lag after which issued work has finished
     const int lag = max_path + 1;
     const int pairs_per_poll = std::max(1, std::min((iters_per_poll > 0 ? iters_per_poll : 8), BATCH_HIST - lag - 4) / 2);
-    // the ring holds every item between the resolved prefix and the cursor;
-    // a job resolves whole passes, so its prefix may trail by one pass
-    size_t ring = 1;
-    while (ring < pool * (size_t)(2 * pairs_per_poll + lag + 2) + (size_t)max_pass) ring <<= 1;
-    ensure_wavefront_for(c, pool, width, depth);
-    BatchState &b = c->batch;
-    b.jobs.reserve(sizeof(JobDev) * n_jobs);
-    b.starts.reserve(sizeof(long long) * n_jobs);
-    b.t_end.reserve(sizeof(unsigned long long) * n_jobs);
-    b.jstats.reserve(sizeof(unsigned long long) * J_STRIDE * n_jobs);
-    b.ctl.reserve(sizeof(long long) * CTL_SIZE);
-    b.t0.reserve(sizeof(long long));
-    b.ring.reserve(sizeof(float4) * ring);
-    b.ring_cnt.reserve(sizeof(unsigned) * ring);
-    b.segs.reserve(sizeof(ResolveSeg) * n_jobs);
-    // a resolve covers less than the ring: at most ring / STAT_CHUNK chunks plus one per job
-    const size_t max_chunks = ring / STAT_CHUNK + (size_t)n_jobs + 1;
-    b.chunks.reserve(sizeof(StatChunk) * max_chunks);
-    if (b.host_chunks_cap < max_chunks) {
-        if (b.host_chunks) (void)hipHostFree(b.host_chunks);
-        b.host_chunks = nullptr;
-        b.host_chunks_cap = 0;
-        HIP_CHECK(hipHostMalloc(&b.host_chunks, sizeof(StatChunk) * max_chunks, hipHostMallocDefault));
-        b.host_chunks_cap = max_chunks;
-    }
-    if (!b.host_status) HIP_CHECK(hipHostMalloc(&b.host_status, sizeof(long long) * (CTL_SIZE + 8), hipHostMallocDefault));
-    if (b.host_segs_cap < (size_t)n_jobs) {
-        if (b.host_segs) (void)hipHostFree(b.host_segs);
-        b.host_segs = nullptr;
-        b.host_segs_cap = 0;
-        HIP_CHECK(hipHostMalloc(&b.host_segs, sizeof(ResolveSeg) * (size_t)n_jobs, hipHostMallocDefault));
-        b.host_segs_cap = (size_t)n_jobs;
-    }
-    // the node arena: job r's nodes from node_off, its triangles from tri_off
-    {
-        long long node_total = 0, tri_total = 0;
-        std::vector<long long> node_off(n_jobs), tri_off(n_jobs);
-        for (int r = 0; r < n_jobs; ++r) {
-            const Ctx *j = jobs[ord[r]];
-            node_off[r] = node_total;
-            tri_off[r] = tri_total;
-            node_total += j->bvh.n_nodes;
-            tri_total += j->n_tris;
+    for (int l = 0; l < n_lanes; ++l) {
+        BLane &L = lanes[l];
+        BatchState &b = *L.b;
+        const int nj = (int)L.rows.size();
+        L.pool = std::max<size_t>(64, std::min<size_t>(c->launch.pool_paths, (size_t)std::max<long long>(L.total, 64)));
+        if (L.pool > kMaxPoolPaths) throw CapacityError("path pool too large");
+        // the ring holds every item between the resolved prefix and the cursor;
+        // a job resolves whole passes, so its prefix may trail by one pass
+        L.ring = 1;
+        while (L.ring < L.pool * (size_t)(2 * pairs_per_poll + lag + 2) + (size_t)L.max_pass) L.ring <<= 1;
+        ensure_wavefront_on(c, *L.wf, L.pool, width, depth);
+        b.jobs.reserve(sizeof(JobDev) * nj);
+        b.starts.reserve(sizeof(long long) * nj);
+        b.t_end.reserve(sizeof(unsigned long long) * nj);
+        b.jstats.reserve(sizeof(unsigned long long) * J_STRIDE * nj);
+        b.ctl.reserve(sizeof(long long) * CTL_SIZE);
+        b.t0.reserve(sizeof(long long));
+        b.ring.reserve(sizeof(float4) * L.ring);
+        b.ring_cnt.reserve(sizeof(unsigned) * L.ring);
+        b.segs.reserve(sizeof(ResolveSeg) * nj);
+        // a resolve covers less than the ring: at most ring / STAT_CHUNK chunks plus one per job
+        const size_t max_chunks = L.ring / STAT_CHUNK + (size_t)nj + 1;
+        b.chunks.reserve(sizeof(StatChunk) * max_chunks);
+        if (b.host_chunks_cap < max_chunks) {
+            if (b.host_chunks) (void)hipHostFree(b.host_chunks);
+            b.host_chunks = nullptr;
+            b.host_chunks_cap = 0;
+            HIP_CHECK(hipHostMalloc(&b.host_chunks, sizeof(StatChunk) * max_chunks, hipHostMallocDefault));
+            b.host_chunks_cap = max_chunks;
         }
-        if (node_total > 0x7ffffffeLL || tri_total >= (1LL << 27))
-            throw CapacityError("the batch's BVHs exceed the node arena's index range");
+        if (!b.host_status)
+            HIP_CHECK(hipHostMalloc(&b.host_status, sizeof(long long) * (CTL_SIZE + 8), hipHostMallocDefault));
+        if (b.host_segs_cap < (size_t)nj) {
+            if (b.host_segs) (void)hipHostFree(b.host_segs);
+            b.host_segs = nullptr;
+            b.host_segs_cap = 0;
+            HIP_CHECK(hipHostMalloc(&b.host_segs, sizeof(ResolveSeg) * (size_t)nj, hipHostMallocDefault));
+            b.host_segs_cap = (size_t)nj;
+        }
+    }
+    // the node arena (in lane 0's state): job r's nodes from node_off, its
+    // triangles from tri_off
+    {
+        BatchState &b = c->batch;
         const size_t nsz = width == 4 ? sizeof(WNode<4>) : sizeof(WNode<8>);
         b.arena_nodes.reserve(nsz * (size_t)std::max<long long>(node_total, 1));
         b.arena_tris.reserve(sizeof(float4) * 3 * (size_t)std::max<long long>(tri_total, 1));
@@ -668,118 +716,151 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
             if (j->n_tris > 0)
                 HIP_CHECK(hipMemcpyAsync(b.arena_tris.as<float4>() + 3 * (size_t)tri_off[r], j->bvh.tri_sorted.p,
                                          sizeof(float4) * 3 * (size_t)j->n_tris, hipMemcpyDeviceToDevice, c->stream));
-            jd[r].root = (int)node_off[r];
         }
     }
-    // the copies read host vectors that outlive the stream synchronisation below
-    HIP_CHECK(hipMemcpyAsync(b.jobs.p, jd.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(b.starts.p, starts.data(), sizeof(long long) * n_jobs, hipMemcpyHostToDevice, c->stream));
-
     PathSource src;
     src.mode = 0;
     src.first_pass = 0;
     src.pixel = src.sample = nullptr;
     src.slots = nullptr;
-    WfView wv = wf_view(c, src);
-    BatchView bv = batch_view(c, n_jobs, total, (long long)ring - 1);
-    bv.pool = (int)pool;
+    for (int l = 0; l < n_lanes; ++l) {
+        BLane &L = lanes[l];
+        const int nj = (int)L.rows.size();
+        // the copies read host vectors that outlive the stream synchronisation below
+        HIP_CHECK(hipMemcpyAsync(L.b->jobs.p, L.jd.data(), sizeof(JobDev) * nj, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(L.b->starts.p, L.starts.data(), sizeof(long long) * nj, hipMemcpyHostToDevice, c->stream));
+        L.wv = wf_view_on(c, *L.wf, src);
+        L.bv = batch_view(*L.b, c->batch, nj, L.total, (long long)L.ring - 1);
+        L.bv.pool = (int)L.pool;
+    }
 
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, c->stream));
-    hipLaunchKernelGGL(k_binit, dim3(std::max(1, std::min((n_jobs + 255) / 256, c->n_cu * 4))), dim3(256), 0, c->stream, bv, wv);
+    for (int l = 0; l < n_lanes; ++l)
+        hipLaunchKernelGGL(k_binit, dim3(std::max(1, std::min(((int)lanes[l].rows.size() + 255) / 256, c->n_cu * 4))),
+                           dim3(256), 0, c->stream, lanes[l].bv, lanes[l].wv);
     HIP_CHECK(hipGetLastError());
-
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    if (n_lanes == 2) {
+        // lane 1 starts after the set-up above
+        HIP_CHECK(hipEventRecord(c->ev_start, c->stream));
+        HIP_CHECK(hipStreamWaitEvent(lanes[1].stream, c->ev_start, 0));
+    }
     auto release = [&]() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
+        for (BLane &L : lanes) {
+            if (L.gexec) (void)hipGraphExecDestroy(L.gexec);
+            if (L.graph) (void)hipGraphDestroy(L.graph);
+            L.gexec = nullptr;
+            L.graph = nullptr;
+        }
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         e0 = e1 = nullptr;
     };
-    long long *hs = static_cast<long long *>(b.host_status);
-    const int *hcounts = reinterpret_cast<const int *>(hs + CTL_SIZE);
     // A path holds its slot for at most maxdepth + 1 iterations, so every slot
-    // completes a path at least that often and the launch drains within
+    // completes a path at least that often and a lane drains within
     // (total / pool + 1) * (maxdepth + 1) iterations; twice that guards the loop.
-    const long long max_iters = 2 * (total / (long long)pool + 1) * (long long)max_path + 64;
-    long long resolved = 0;
     float ms = 0.f;
-    try {
-        if (use_graph) {
-            HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            batch_iteration(c, bv, wv, 0, width, tex);
-            batch_iteration(c, bv, wv, 1, width, tex);
-            HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
-            HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    auto issue = [&](BLane &L) {
+        for (int k = 0; k < pairs_per_poll; ++k) {
+            if (L.gexec) HIP_CHECK(hipGraphLaunch(L.gexec, L.stream));
+            else {
+                batch_iteration(c, L.stream, L.bv, L.wv, 0, width, tex);
+                batch_iteration(c, L.stream, L.bv, L.wv, 1, width, tex);
+            }
         }
-        for (;;) {
-            for (int k = 0; k < pairs_per_poll; ++k) {
-                if (gexec) HIP_CHECK(hipGraphLaunch(gexec, c->stream));
-                else {
-                    batch_iteration(c, bv, wv, 0, width, tex);
-                    batch_iteration(c, bv, wv, 1, width, tex);
+        HIP_CHECK(hipGetLastError());
+        long long *hs = static_cast<long long *>(L.b->host_status);
+        HIP_CHECK(hipMemcpyAsync(hs, L.b->ctl.p, sizeof(long long) * CTL_TIME, hipMemcpyDeviceToHost, L.stream));
+        HIP_CHECK(hipMemcpyAsync(hs + CTL_SIZE, L.wv.counts, sizeof(int) * 4, hipMemcpyDeviceToHost, L.stream));
+    };
+    try {
+        for (int l = 0; l < n_lanes; ++l) {
+            BLane &L = lanes[l];
+            if (use_graph) {
+                HIP_CHECK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+                batch_iteration(c, L.stream, L.bv, L.wv, 0, width, tex);
+                batch_iteration(c, L.stream, L.bv, L.wv, 1, width, tex);
+                HIP_CHECK(hipStreamEndCapture(L.stream, &L.graph));
+                HIP_CHECK(hipGraphInstantiate(&L.gexec, L.graph, nullptr, nullptr, 0));
+            }
+        }
+        for (int l = 0; l < n_lanes; ++l) issue(lanes[l]);
+        for (int remaining = n_lanes; remaining > 0;) {
+            for (int l = 0; l < n_lanes; ++l) {
+                BLane &L = lanes[l];
+                if (L.done) continue;
+                HIP_CHECK(hipStreamSynchronize(L.stream));
+                const long long *hs = static_cast<const long long *>(L.b->host_status);
+                const int *hcounts = reinterpret_cast<const int *>(hs + CTL_SIZE);
+                const long long cur = hs[CTL_CURSOR], iters = hs[CTL_ITER];
+                // after a pair of iterations the live paths sit in queue 0
+                const bool done = cur >= L.total && hcounts[C_Q0] == 0;
+                long long safe = 0;
+                if (done) safe = L.total;
+                else if (iters - 1 - lag >= 0) safe = hs[CTL_HIST + (int)((iters - 1 - lag) % BATCH_HIST)];
+                if (done) hipLaunchKernelGGL(k_bstamp, dim3(1), dim3(64), 0, L.stream, L.bv);
+                if (safe > L.resolved) {
+                    resolve_range(c, *L.b, L.stream, L.bv, L.starts, L.npaths, L.wh, L.jfirst, L.jls, L.job_done, safe);
+                    L.resolved = safe;
                 }
+                if (done) {
+                    L.done = true;
+                    --remaining;
+                    continue;
+                }
+                const long long max_iters = 2 * (L.total / (long long)L.pool + 1) * (long long)max_path + 64;
+                if (iters > max_iters) throw StateError("batched launch did not drain");
+                issue(L);
             }
-            HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipMemcpyAsync(hs, b.ctl.p, sizeof(long long) * CTL_TIME, hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(hipMemcpyAsync(hs + CTL_SIZE, wv.counts, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(hipStreamSynchronize(c->stream));
-            const long long cur = hs[CTL_CURSOR], iters = hs[CTL_ITER];
-            // after a pair of iterations the live paths sit in queue 0
-            const bool done = cur >= total && hcounts[C_Q0] == 0;
-            long long safe = 0;
-            if (done) safe = total;
-            else if (iters - 1 - lag >= 0) safe = hs[CTL_HIST + (int)((iters - 1 - lag) % BATCH_HIST)];
-            if (done) hipLaunchKernelGGL(k_bstamp, dim3(1), dim3(64), 0, c->stream, bv);
-            if (safe > resolved) {
-                resolve_range(c, bv, starts, npaths, wh, jfirst, jls, job_done, safe);
-                resolved = safe;
-            }
-            if (done) break;
-            if (iters > max_iters) throw StateError("batched launch did not drain");
+        }
+        if (n_lanes == 2) {
+            HIP_CHECK(hipEventRecord(c->ev_join, lanes[1].stream));
+            HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
         }
         HIP_CHECK(hipEventRecord(e1, c->stream));
         HIP_CHECK(hipEventSynchronize(e1));
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     } catch (...) {
-        (void)hipStreamSynchronize(c->stream);
+        for (int l = 0; l < n_lanes; ++l) (void)hipStreamSynchronize(lanes[l].stream);
         release();
         throw;
     }
     release();
 
-    std::vector<unsigned long long> t_end(n_jobs);
-    std::vector<unsigned long long> js((size_t)n_jobs * J_STRIDE);
-    long long t0 = 0;
-    HIP_CHECK(hipMemcpy(t_end.data(), b.t_end.p, sizeof(unsigned long long) * n_jobs, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(js.data(), b.jstats.p, sizeof(unsigned long long) * js.size(), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(&t0, b.t0.p, sizeof t0, hipMemcpyDeviceToHost));
     int rate_khz = 0;
     HIP_CHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device));
     aur_stats tot;
     std::memset(&tot, 0, sizeof tot);
-    for (int r = 0; r < n_jobs; ++r) {
-        if ((long long)js[(size_t)r * J_STRIDE + J_PATHS] != npaths[r])
-            throw StateError("batched launch: job " + std::to_string(ord[r]) + " finished " +
-                             std::to_string(js[(size_t)r * J_STRIDE + J_PATHS]) + " of " + std::to_string(npaths[r]) + " paths");
-        aur_job_result &o = results[ord[r]];
-        o.paths = paths[ord[r]];
-        o.ext_rays = (int64_t)js[(size_t)r * J_STRIDE + J_EXT];
-        o.hit_rays = (int64_t)js[(size_t)r * J_STRIDE + J_HIT];
-        o.shadow_rays = (int64_t)js[(size_t)r * J_STRIDE + J_SHADOW];
-        o.latency_ms = (paths[ord[r]] > 0 && rate_khz > 0) ? (double)((long long)t_end[r] - t0) / (double)rate_khz : 0.0;
-        o.order = r;
-        jobs[ord[r]]->display_passes += n_passes[ord[r]];
-        tot.paths += o.paths;
-        tot.ext_rays += o.ext_rays;
-        tot.hit_rays += o.hit_rays;
-        tot.shadow_rays += o.shadow_rays;
+    for (int l = 0; l < n_lanes; ++l) {
+        BLane &L = lanes[l];
+        const int nj = (int)L.rows.size();
+        std::vector<unsigned long long> t_end(nj);
+        std::vector<unsigned long long> js((size_t)nj * J_STRIDE);
+        long long t0 = 0;
+        HIP_CHECK(hipMemcpy(t_end.data(), L.b->t_end.p, sizeof(unsigned long long) * nj, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(js.data(), L.b->jstats.p, sizeof(unsigned long long) * js.size(), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(&t0, L.b->t0.p, sizeof t0, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nj; ++i) {
+            const int r = L.rows[i];
+            if ((long long)js[(size_t)i * J_STRIDE + J_PATHS] != L.npaths[i])
+                throw StateError("batched launch: job " + std::to_string(ord[r]) + " finished " +
+                                 std::to_string(js[(size_t)i * J_STRIDE + J_PATHS]) + " of " +
+                                 std::to_string(L.npaths[i]) + " paths");
+            aur_job_result &o = results[ord[r]];
+            o.paths = paths[ord[r]];
+            o.ext_rays = (int64_t)js[(size_t)i * J_STRIDE + J_EXT];
+            o.hit_rays = (int64_t)js[(size_t)i * J_STRIDE + J_HIT];
+            o.shadow_rays = (int64_t)js[(size_t)i * J_STRIDE + J_SHADOW];
+            o.latency_ms = (paths[ord[r]] > 0 && rate_khz > 0) ? (double)((long long)t_end[i] - t0) / (double)rate_khz : 0.0;
+            o.order = r;
+            jobs[ord[r]]->display_passes += n_passes[ord[r]];
+            tot.paths += o.paths;
+            tot.ext_rays += o.ext_rays;
+            tot.hit_rays += o.hit_rays;
+            tot.shadow_rays += o.shadow_rays;
+        }
     }
     tot.ms = ms;
     if (total_stats) *total_stats = tot;

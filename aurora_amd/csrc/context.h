@@ -218,6 +218,9 @@ struct Ctx {
     hipEvent_t lane_fork[2] = {nullptr, nullptr}, lane_shaded[2] = {nullptr, nullptr};
     hipEvent_t lane_done[2] = {nullptr, nullptr}, ev_resolved[2] = {nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_join = nullptr;
+    // the second pipeline of a batched launch (run_jobs) keeps its state here
+    // and runs on lane_stream[1] with wf2
+    BatchState batch2;
     KernelTimer timer;
     LaunchCfg launch;
     BatchState batch;
@@ -276,6 +279,8 @@ void trace_rays_noalloc(Ctx *c, const aur_ray *d_rays, int n, aur_hit *d_hits, i
 void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth);
 // creates the lane streams and events of run_paths (aur_create, after `stream`)
 void ensure_lane_streams(Ctx *c);
+// the same for a given wavefront (lane 1 of a batched launch uses wf2)
+void ensure_wavefront_on(Ctx *c, Wavefront &w, size_t cap, int width, int depth);
 // batch.hip: jobs[k] renders passes [first_pass[k], first_pass[k] + n_passes[k]) into its framebuffer
 void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_passes, int n_jobs, int order, int use_graph,
               int iters_per_poll, aur_job_result *results, aur_stats *total_stats);

@@ -485,6 +485,7 @@ static void ensure_wf(Ctx *c, Wavefront &w, size_t cap, int width, int depth) {
 }
 
 void ensure_wavefront_for(Ctx *c, size_t cap, int width, int depth) { ensure_wf(c, c->wf, cap, width, depth); }
+void ensure_wavefront_on(Ctx *c, Wavefront &w, size_t cap, int width, int depth) { ensure_wf(c, w, cap, width, depth); }
 
 static void ensure_wavefront(Ctx *c, size_t cap) { ensure_wavefront_for(c, cap, c->bvh.width, c->bvh.depth); }
 
@@ -516,6 +517,7 @@ static WfView wf_view_of(Ctx *c, Wavefront &w, const PathSource &src) {
 }
 
 WfView wf_view(Ctx *c, const PathSource &src) { return wf_view_of(c, c->wf, src); }
+WfView wf_view_on(Ctx *c, Wavefront &w, const PathSource &src) { return wf_view_of(c, w, src); }
 
 // The streams and events of one path pipeline.
 struct Lane {

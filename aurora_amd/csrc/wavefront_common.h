@@ -114,6 +114,7 @@ __device__ inline void shadow_add(const WfView &wv, int nq, int tag, float4 c) {
 }
 
 WfView wf_view(Ctx *c, const PathSource &src);
+WfView wf_view_on(Ctx *c, Wavefront &w, const PathSource &src);
 
 // ---------------------------------------------------------------------------
 // wave-level helpers (wave64)
