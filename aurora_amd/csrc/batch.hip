@@ -565,6 +565,9 @@ void run_jobs(Ctx *c, Ctx *const *jobs, const int *first_pass, const int *n_pass
     // table, control block, ring, path pool and stream; the node arena is
     // shared.  The host polls the lanes in turn and queues a lane's next
     // iterations right after handling its status.
+    // (subsets of the 64-job workload: 8 jobs, 2 M paths 460 against 425
+    // Mpaths/s; 21 jobs, 16 M paths 1100 against 855; all 64, 1.58 G paths
+    // 1590 against 1260)
     const int n_lanes = (n_jobs >= 2 && total_all >= (1LL << 21)) ? 2 : 1;
     struct BLane {
         BatchState *b;

@@ -510,21 +510,6 @@ def large_leg(fwd_passes: int = 256, adj_passes: int = 64, fd_passes: int = 1, n
             "fd_checked": int(len(pick)), "fd_h": h, "fd_passes": fd_passes, "fd_max_rel_err": float("%.3g" % worst)}
 
 
-def _warm_jobs(rs, info, spp, pool):
-    """A job list of leading passes with at least `pool` paths in total (or all
-    of the work, if that is less): a batched launch of it sizes the executor's
-    path pool as the full run will."""
-    jobs, total = [], 0
-    for r, i, n in zip(rs, info, spp):
-        wh = i["width"] * i["height"]
-        k = min(n, max(1, -(-(pool - total) // wh)))
-        jobs.append((r, 0, k))
-        total += k * wh
-        if total >= pool:
-            break
-    return jobs
-
-
 def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
     """Batched uneven workload: the same jobs rendered once through the batched
     persistent launch and once as per-scene launches (one aur_render per job,
@@ -551,9 +536,10 @@ def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
     tuning.apply_launch(ex, cfg)
     pool = int(cfg["pool_paths"])
     # warm both launch paths with their full working sets, so that no buffer
-    # grows inside a timed region: a batched launch of at least `pool` paths,
-    # and per job the passes of one whole chunk (up to `pool` paths)
-    batch_submit(ex, _warm_jobs(rs, info, spp, pool), use_graph=use_graph)
+    # grows inside a timed region: the same batched launch (each pipeline's
+    # path pool and ring are sized by the jobs it is given), and per job the
+    # passes of one whole chunk (up to `pool` paths)
+    batch_submit(ex, [(r, 0, n) for r, n in zip(rs, spp)], use_graph=use_graph)
     for r, i, n in zip(rs, info, spp):
         r.render_passes(0, min(n, max(1, pool // (i["width"] * i["height"]))), fetch=False)
     _hip_sync()
@@ -621,8 +607,7 @@ def batch_leg(n_jobs: int = 64, seed: int = 3, use_graph: bool = True):
     # the same jobs through one batched launch
     ex2 = Renderer()
     tuning.apply_launch(ex2, cfg)
-    batch_submit(ex2, _warm_jobs([r for r, _ in rs_sub], [info[k] for k in sub], [n for _, n in rs_sub], pool),
-                 use_graph=use_graph)
+    batch_submit(ex2, [(r, 0, n) for r, n in rs_sub], use_graph=use_graph)
     _hip_sync()
     t0 = time.perf_counter()
     _, tot_sub = batch_submit(ex2, [(r, 0, n) for r, n in rs_sub], use_graph=use_graph)
