@@ -225,7 +225,11 @@ typedef struct {
 int aur_create(aur_ctx **out, int hip_device);
 void aur_destroy(aur_ctx *ctx);
 const char *aur_last_error(void);
-/* hip_stream: a hipStream_t, or NULL for the library's own stream. */
+/* hip_stream: a hipStream_t, or NULL for the library's own stream.  Renders
+ * of at least 2 M paths also run on three streams the context creates for
+ * itself (a second path pipeline and the side streams that trace shadow rays
+ * beside extension rays); they start after work already on the context's
+ * stream and the context's stream waits for them before the call returns. */
 int aur_set_stream(aur_ctx *ctx, void *hip_stream);
 int aur_set_seed(aur_ctx *ctx, uint32_t seed);
 /* Sample source of the paths' random numbers.  AUR_SAMPLER_HASH (default):
@@ -349,7 +353,8 @@ typedef struct {
     int iters_per_poll; /* bounces between completion checks by the host (0: 8) */
 } aur_batch_opts;
 /* Renders every job through one pool of path slots on exec's stream (exec
- * may be one of the jobs' contexts).  All jobs need the same BVH width.
+ * may be one of the jobs' contexts); launches of at least 2 M paths split the
+ * jobs over two such pools, the second on one of exec's own streams.  All jobs need the same BVH width.
  * opts may be NULL (shortest first, graph replay).  results has n_jobs
  * entries, in the order of jobs; total (may be NULL) gets the sums and the
  * device time of the whole launch. */
@@ -424,7 +429,8 @@ typedef struct {
 /* enable != 0 turns timing on and resets the totals.  While timing is on,
  * the tracer's kernels run one after another on the context's stream
  * (normally a bounce's shadow and finish kernels overlap the next bounce's
- * extension kernel on a second stream), so each kernel's time is its own. */
+ * extension kernel on a second stream, and large renders run two path
+ * pipelines at once), so each kernel's time is its own. */
 int aur_set_profiling(aur_ctx *ctx, int enable);
 int aur_kernel_stats(aur_ctx *ctx, aur_kernel_stat *out, int max, int *n_out);
 
