@@ -157,13 +157,15 @@ __device__ inline bool visit_inner(const NodeRegs<W> &nr, const TravRay &r, Trav
                                    const TravHit &h) {
     // exact-t ties must still be visited, so the far bound is widened slightly
     const float tfar_lim = h.t * 1.0000004f;
-    // Closest hit orders the entered children by 32-bit keys: the entry
-    // distance's bits (a non-negative float orders as an unsigned integer)
-    // with the slot in the low bits, so the sort moves one register per
-    // child.  Missed children get ~0u.
+    // The entered children are visited in the order of 32-bit keys: the
+    // entry distance's bits (a non-negative float orders as an unsigned
+    // integer) with the slot in the low bits, so the sort moves one register
+    // per child.  Closest hit orders inner children and leaves alike front to
+    // back; an any-hit query takes the entered leaves first (the query ends
+    // at its first accepted triangle), then the inner children front to
+    // back.  Missed children get ~0u.
     constexpr unsigned SLOT = W == 4 ? 3u : 7u;
     unsigned key[W];
-    unsigned inner_m = 0, leaf_m = 0;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         float x0 = fmaf(nr.lox[k], r.ix, -r.ox), x1 = fmaf(nr.hix[k], r.ix, -r.ox);
@@ -171,49 +173,21 @@ __device__ inline bool visit_inner(const NodeRegs<W> &nr, const TravRay &r, Trav
         float z0 = fmaf(nr.loz[k], r.iz, -r.oz), z1 = fmaf(nr.hiz[k], r.iz, -r.oz);
         float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
         float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
-        const bool hit = a <= b;
-        if (ANY) {
-            if (hit) {
-                if (nr.child[k] < 0) leaf_m |= 1u << k;
-                else inner_m |= 1u << k;
-            }
-        } else {
-            // a >= tmin >= 0: clearing the sign bit only turns -0 into +0
-            key[k] = hit ? ((__float_as_uint(a) & 0x7fffffffu & ~SLOT) | (unsigned)k) : ~0u;
-        }
+        // a >= tmin >= 0: clearing the sign bit only turns -0 into +0; an
+        // any-hit query takes its entered leaves first (key = slot)
+        const unsigned dist = (ANY && nr.child[k] < 0) ? 0u : (__float_as_uint(a) & 0x7fffffffu & ~SLOT);
+        key[k] = a <= b ? (dist | (unsigned)k) : ~0u;
     }
 #ifdef AUR_TRAV_STATS
     st.n_visits += 1;
 #endif
-    if (ANY) {
-        // any hit: the leaves first (the query ends at its first accepted
-        // triangle), then the inner children, each in slot order
-        int next = 0;
-        bool have = false;
+    sort_keys<W>(key);
+    if (key[0] != ~0u) {
 #pragma unroll
-        for (int k = W - 1; k >= 0; --k)
-            if ((inner_m >> k) & 1u) {
-                if (have) st.push(next);
-                next = nr.child[k];
-                have = true;
-            }
-#pragma unroll
-        for (int k = W - 1; k >= 0; --k)
-            if ((leaf_m >> k) & 1u) {
-                if (have) st.push(next);
-                next = nr.child[k];
-                have = true;
-            }
-        if (have) { node = next; return false; }
-    } else {
-        sort_keys<W>(key);
-        if (key[0] != ~0u) {
-#pragma unroll
-            for (int k = W - 1; k >= 1; --k)
-                if (key[k] != ~0u) st.push(child_of<W>(nr.child, key[k] & SLOT));
-            node = child_of<W>(nr.child, key[0] & SLOT);
-            return false;
-        }
+        for (int k = W - 1; k >= 1; --k)
+            if (key[k] != ~0u) st.push(child_of<W>(nr.child, key[k] & SLOT));
+        node = child_of<W>(nr.child, key[0] & SLOT);
+        return false;
     }
     if (st.sp == 0) return true;
     node = st.pop();
