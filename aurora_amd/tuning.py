@@ -99,8 +99,19 @@ def _time_render(renderer, passes: int, repeats: int) -> float:
     return paths / best / 1e6
 
 
+# Short renders of the structure and launch stages trace at least this many
+# paths: on a small image two passes measure launch overhead, not the
+# candidates (box room 256x256: the 2-pass ranking picked a launch shape 3 %
+# slower than the default at its 16-pass frame, and the previous entry was
+# 12 % slower)
+MIN_TIMED_PATHS = 1 << 20
+
+
 def autotune(renderer, passes: int = 2, repeats: int = 3, table: str = TABLE) -> dict:
     """Times the structure candidates, then the launch shapes; returns the table entry."""
+    info0 = renderer.info()
+    wh0 = info0["width"] * info0["height"]
+    passes = max(passes, min(info0["pixelsamples"], -(-MIN_TIMED_PATHS // wh0)))
     structure = []
     apply_launch(renderer, DEFAULT)
     for w in WIDTHS:
