@@ -158,7 +158,8 @@ class BatchOpts(C.Structure):
 
 
 class LaunchOpts(C.Structure):
-    _fields_ = [("trace_block", C.c_int), ("trace_waves_per_cu", C.c_int), ("pool_paths", C.c_int)]
+    _fields_ = [("trace_block", C.c_int), ("trace_waves_per_cu", C.c_int), ("pool_paths", C.c_int),
+                ("shade_list_tiles", C.c_int)]
 
 
 _P = C.c_void_p

@@ -489,6 +489,7 @@ int aur_create(aur_ctx **out, int hip_device) {
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, hip_device));
         c->c.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        c->c.launch.shade_list_tiles = 16 * c->c.n_cu;  // four tiles per shade workgroup
         noalloc_init(hip_device);  // the device is current (set_device above)
         HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
         c->c.own_stream = true;
@@ -1166,6 +1167,7 @@ int aur_set_launch(aur_ctx *ctx, const aur_launch_opts *opts) {
         if (opts->trace_block > 0) l.trace_block = opts->trace_block;
         if (opts->trace_waves_per_cu > 0) l.trace_waves_per_cu = opts->trace_waves_per_cu;
         if (opts->pool_paths > 0) l.pool_paths = (size_t)opts->pool_paths;
+        if (opts->shade_list_tiles > 0) l.shade_list_tiles = opts->shade_list_tiles;
     });
 }
 
@@ -1177,6 +1179,7 @@ int aur_get_launch(aur_ctx *ctx, aur_launch_opts *out) {
         out->trace_block = l.trace_block;
         out->trace_waves_per_cu = l.trace_waves_per_cu;
         out->pool_paths = (int)std::min<size_t>(l.pool_paths, 0x7fffffff);
+        out->shade_list_tiles = l.shade_list_tiles;
     });
 }
 

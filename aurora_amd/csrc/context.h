@@ -127,6 +127,10 @@ struct LaunchCfg {
     // Every chunk ends in a latency-bound drain (the last bounces of its
     // longest paths), so larger chunks amortise it over more paths
     size_t pool_paths = 128u << 20;
+    // a bounce's queue is shaded from per-workgroup hit lists when it holds
+    // at least this many 1024-entry tiles (in place below); aur_create sets
+    // four per shade workgroup
+    int shade_list_tiles = 4096;
 };
 
 // Device state of a batched launch (batch.hip), kept between calls.

@@ -232,26 +232,147 @@ __device__ inline void finish_dups(const WfView &wv, unsigned ext_dup, unsigned 
     }
 }
 
+// An entry of extension queue q with no vertex to shade -- its ray missed,
+// or the vertex is past maxdepth (every entry of the last bounce: all
+// entries of a bounce are the same vertex of their paths) -- finishes as
+// shading would finish it (integrator.cpp:287-297, 262-271, then the clamp
+// and output of integrator.cpp:276-283).
+template <bool TEX, bool CAM>
+__device__ inline void finish_unshaded(const SceneView &sv, const WfView &wv, int q, int i, int prim,
+                                       unsigned &ext_dup, unsigned &hit_dup) {
+    int4 meta;
+    float4 La, Lb;
+    if constexpr (CAM) {  // the camera vertex of a hash-sampled render: nothing stored yet
+        meta = make_int4(RT_CAMERA, i, 0, 0);
+        La = Lb = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        meta = ld_s(&wv.meta[q][i]);
+        La = ld_s(&wv.La[q][i]);
+        Lb = ld_s(&wv.Lb[q][i]);
+    }
+    int flags = meta.x;
+    if (prim < 0) {
+        const auto make_rng = [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); };
+        const auto get_rb = [&] {
+            if constexpr (CAM) {
+                float4 a, b;
+                camera_ray_of(sv, make_rng(meta), &a, &b);
+                return b;
+            } else {
+                return ld_s(&wv.ray[q][2 * i + 1]);
+            }
+        };
+        const auto get_T = [&] {
+            if constexpr (CAM) return make_float4(1.f, 1.f, 1.f, 1.f);
+            else return ld_s(&wv.T[q][i]);
+        };
+        miss_radiance<TEX>(sv, meta, make_rng, get_rb, get_T, La, flags);
+        flags |= F_MISS;
+    }
+    finish_path(sv, wv, La, Lb, make_int4(flags, meta.y, meta.z, meta.w), ext_dup, hit_dup);
+}
+
 // ---------------------------------------------------------------------------
-// shade: one path vertex per queue entry
+// shade: one path vertex per queue entry with a vertex to shade.  Each
+// workgroup takes tiles of SHADE_TILE entries: it reads their hit records,
+// finishes the entries with no vertex to shade (finish_unshaded), lists the
+// others in LDS in queue order, and shades the list in rounds of a full
+// workgroup, carrying fewer than a round's worth into its next tile.  Most
+// bounce rays of an open scene miss, and a wave of queue entries almost
+// always holds a few hits: shading the queue in place ran the whole vertex
+// code on nearly every wave with a quarter of its lanes.
 // ---------------------------------------------------------------------------
 // 96 VGPRs (5 waves per SIMD) without the shader code; the shader, normal-map
 // and environment-map code needs more registers and runs at 4.
+#define SHADE_PER_THREAD 4
+#define SHADE_TILE (SHADE_PER_THREAD * 256)
+#ifndef AUR_SHADE_WAVES
+#define AUR_SHADE_WAVES 5
+#endif
 template <bool TEX, bool CAM = false>
-__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView wv, int q) {
+__global__ void __launch_bounds__(256, TEX ? 4 : AUR_SHADE_WAVES) k_shade(SceneView sv, WfView wv, int q, int last,
+                                                                          int list_tiles) {
     __shared__ EmitScratch es;
+    __shared__ int list[SHADE_TILE + 256];
+    __shared__ int cnt[SHADE_PER_THREAD][4];
     const int n = wv.counts[q];
-    const int stride = gridDim.x * blockDim.x;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    // Listing pays off when each workgroup has tiles enough for the hit
+    // records' read to hide behind other workgroups' shading (list_tiles:
+    // four per workgroup by default); a shorter queue is shaded in place, a
+    // workgroup's worth of entries per round (shade_compute finishes its
+    // misses itself).
+    const bool listed = (n + SHADE_TILE - 1) / SHADE_TILE >= list_tiles;
+    const int tile_size = listed ? SHADE_TILE : 256;
+    const int n_tiles = (n + tile_size - 1) / tile_size;
     unsigned ext_dup = 0, hit_dup = 0;
-    // whole workgroups iterate together: the emit step reserves per workgroup
-    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const int i = base + threadIdx.x;
-        ShadeOut o;
-        o.cont = o.finish = false;
-        o.sh[0].valid = o.sh[1].valid = false;
-        if (i < n) shade_compute<TEX, CAM>(sv, wv, q, i, [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); }, 0, o);
-        shade_emit<true>(wv, q, i < n, o, es,
-                         [&](const ShadeOut &f) { finish_path(sv, wv, f.La, f.Lb, f.meta, ext_dup, hit_dup); });
+    int have = 0;  // entries waiting to be shaded (the same in every thread)
+    for (int tile = blockIdx.x;; tile += gridDim.x) {
+        const bool more = tile < n_tiles;
+        const int base = tile * tile_size;
+        if (more && !listed) {
+            have = min(256, n - base);
+        } else if (more) {
+            int prim[SHADE_PER_THREAD];
+#pragma unroll
+            for (int k = 0; k < SHADE_PER_THREAD; ++k) {
+                const int i = base + k * 256 + (int)threadIdx.x;
+                prim[k] = i < n ? __float_as_int(ld_s(&wv.hit[i]).w) : 0;
+            }
+            unsigned long long m[SHADE_PER_THREAD];
+#pragma unroll
+            for (int k = 0; k < SHADE_PER_THREAD; ++k) {
+                const int i = base + k * 256 + (int)threadIdx.x;
+                const bool valid = i < n, fin = valid && (prim[k] < 0 || last);
+                m[k] = __ballot(valid && !fin);
+                if (fin) finish_unshaded<TEX, CAM>(sv, wv, q, i, prim[k], ext_dup, hit_dup);
+            }
+            if (lane == 0)
+#pragma unroll
+                for (int k = 0; k < SHADE_PER_THREAD; ++k) cnt[k][wave] = __popcll(m[k]);
+            __syncthreads();
+            // list positions in queue order: entry group k, then wave
+            int pos = have, tot = 0;
+#pragma unroll
+            for (int k = 0; k < SHADE_PER_THREAD; ++k)
+                for (int w = 0; w < 4; ++w) {
+                    const int c = cnt[k][w];
+                    if (w == wave && ((m[k] >> lane) & 1ull))
+                        list[pos + tot + __popcll(m[k] & lt)] = base + k * 256 + (int)threadIdx.x;
+                    tot += c;
+                }
+            have += __builtin_amdgcn_readfirstlane(tot);
+            __syncthreads();
+        }
+        int head = 0;
+        while (have - head >= 256 || ((!more || !listed) && have > head)) {
+            const int j = head + (int)threadIdx.x;
+            const bool valid = j < have;
+            ShadeOut o;
+            o.cont = o.finish = false;
+            o.sh[0].valid = o.sh[1].valid = false;
+            if (valid)
+                shade_compute<TEX, CAM>(sv, wv, q, listed ? list[j] : base + j,
+                                        [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); }, 0, o);
+            // (its barriers also end this round's reads of the list)
+            shade_emit<true>(wv, q, valid, o, es,
+                             [&](const ShadeOut &f) { finish_path(sv, wv, f.La, f.Lb, f.meta, ext_dup, hit_dup); });
+            head += 256;
+        }
+        if (!more) break;
+        if (!listed) {
+            have = 0;
+            continue;
+        }
+        // the rest (fewer than 256) moves to the front; head >= 256 when any
+        // round ran, so the ranges do not overlap
+        if (head > 0) {
+            const int rest = have - head;
+            if ((int)threadIdx.x < rest) list[threadIdx.x] = list[head + threadIdx.x];
+            have = rest;
+            __syncthreads();
+        }
     }
     finish_dups(wv, ext_dup, hit_dup);
 }
@@ -562,10 +683,14 @@ static void run_batch(Ctx *c, const Lane &L, const SceneView &sv, WfView wv, int
 #define AUR_SHD(w_, b_) hipLaunchKernelGGL((k_shadow<w_, b_>), dim3(tb), dim3(b_), 0, s2, sv, wv, 1 - q)
         TIMED_ON(c, KT_EXTEND, L.main, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
         if (overlap && it > 0) HIP_CHECK(hipStreamWaitEvent(L.main, L.done, 0));
-        if (tex && cam) TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL((k_shade<true, true>), dim3(sb), dim3(256), 0, L.main, sv, wv, q));
-        else if (tex) TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL(k_shade<true>, dim3(sb), dim3(256), 0, L.main, sv, wv, q));
-        else if (cam) TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL((k_shade<false, true>), dim3(sb), dim3(256), 0, L.main, sv, wv, q));
-        else TIMED_ON(c, KT_SHADE, L.main, hipLaunchKernelGGL(k_shade<false>, dim3(sb), dim3(256), 0, L.main, sv, wv, q));
+        const int last = it == sv.maxdepth;
+#define AUR_SHADE(t_, c_) \
+    hipLaunchKernelGGL((k_shade<t_, c_>), dim3(sb), dim3(256), 0, L.main, sv, wv, q, last, c->launch.shade_list_tiles)
+        if (tex && cam) TIMED_ON(c, KT_SHADE, L.main, AUR_SHADE(true, true));
+        else if (tex) TIMED_ON(c, KT_SHADE, L.main, AUR_SHADE(true, false));
+        else if (cam) TIMED_ON(c, KT_SHADE, L.main, AUR_SHADE(false, true));
+        else TIMED_ON(c, KT_SHADE, L.main, AUR_SHADE(false, false));
+#undef AUR_SHADE
         if (overlap) {
             HIP_CHECK(hipEventRecord(L.shaded, L.main));
             HIP_CHECK(hipStreamWaitEvent(s2, L.shaded, 0));

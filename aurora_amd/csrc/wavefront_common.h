@@ -199,6 +199,45 @@ struct ShadeOut {
     int job;
 };
 
+// What a path whose extension ray missed sees (flags: its flags, updated).
+// get_rb() gives the ray's direction record and get_T() its throughput; each
+// is called only on the branch that needs it (a camera miss under an
+// environment map, a mirror miss under an environment light).
+template <bool TEX, class MakeRng, class GetRb, class GetT>
+__device__ inline void miss_radiance(const SceneView &sv, const int4 &meta, MakeRng make_rng, GetRb get_rb, GetT get_T,
+                                     float4 &La, int &flags) {
+    const int rtype = flags & 3;
+    if (rtype == RT_CAMERA) {  // integrator.cpp:287-297: camera miss sees the environment
+        f3 e = mk3(sv.env[0], sv.env[1], sv.env[2]);
+        if constexpr (TEX) {
+            if (sv.has_env) {
+                const float4 rb = get_rb();
+                e = env_miss(sv, mk3(rb.x, rb.y, rb.z));
+            } else {
+                e = mk3(0.f, 0.f, 0.f);
+            }
+        }
+        La = make_float4(e.x, e.y, e.z, sv.has_env ? 1.f : 0.f);
+        flags |= F_NOCLAMP;
+    } else if (rtype == RT_MIRROR && sv.has_env) {
+        // integrator.cpp:262-271: a mirror ray that misses adds every env
+        // light's eval() in the direction of the path's camera ray
+        const float4 T4 = get_T();
+        f3 cd = mk3(0.f, 0.f, 0.f);
+        if constexpr (TEX) {
+            const auto rng = make_rng(meta);
+            cd = camera_dir(sv, rng.pixel, rng.u(0), rng.u(1));
+        }
+        for (int li = 0; li < sv.n_lights; ++li) {
+            const DLight &l = sv.lights[li];
+            if (l.type != LIGHT_ENV) continue;
+            f3 e = mk3(l.Le[0], l.Le[1], l.Le[2]);
+            if constexpr (TEX) e = env_eval(sv, l, cd);
+            La.x += e.x * T4.x; La.y += e.y * T4.y; La.z += e.z * T4.z;
+        }
+    }
+}
+
 // TEX selects the code for shaders, normal maps, environment maps and the
 // Halton sample source; the host uses it only for scenes that need any.
 // CAM: the camera vertex of a hash-sampled render, whose ray and state are
@@ -234,27 +273,7 @@ __device__ inline void shade_compute(const SceneView &sv, const WfView &wv, int 
     float ntmax = 0.f;
     if (prim < 0) {
         finish = true;
-        if (rtype == RT_CAMERA) {  // integrator.cpp:287-297: camera miss sees the environment
-            f3 e = mk3(sv.env[0], sv.env[1], sv.env[2]);
-            if constexpr (TEX) e = sv.has_env ? env_miss(sv, mk3(rb.x, rb.y, rb.z)) : mk3(0.f, 0.f, 0.f);
-            La = make_float4(e.x, e.y, e.z, sv.has_env ? 1.f : 0.f);
-            flags |= F_NOCLAMP;
-        } else if (rtype == RT_MIRROR && sv.has_env) {
-            // integrator.cpp:262-271: a mirror ray that misses adds every env
-            // light's eval() in the direction of the path's camera ray
-            f3 cd = mk3(0.f, 0.f, 0.f);
-            if constexpr (TEX) {
-                const auto rng = make_rng(meta);
-                cd = camera_dir(sv, rng.pixel, rng.u(0), rng.u(1));
-            }
-            for (int li = 0; li < sv.n_lights; ++li) {
-                const DLight &l = sv.lights[li];
-                if (l.type != LIGHT_ENV) continue;
-                f3 e = mk3(l.Le[0], l.Le[1], l.Le[2]);
-                if constexpr (TEX) e = env_eval(sv, l, cd);
-                La.x += e.x * T4.x; La.y += e.y * T4.y; La.z += e.z * T4.z;
-            }
-        }
+        miss_radiance<TEX>(sv, meta, make_rng, [&] { return rb; }, [&] { return T4; }, La, flags);
     } else if (tb >= sv.maxdepth) {
         finish = true;  // the extension ray after the last vertex is traced but not shaded
     } else {

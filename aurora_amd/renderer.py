@@ -230,15 +230,17 @@ class Renderer:
         self.last_stats = st.as_dict()
         return dict(self.last_stats, draws=draws.value)
 
-    def set_launch(self, trace_block: int = 0, trace_waves_per_cu: int = 0, pool_paths: int = 0) -> None:
+    def set_launch(self, trace_block: int = 0, trace_waves_per_cu: int = 0, pool_paths: int = 0,
+                   shade_list_tiles: int = 0) -> None:
         """Launch shape of the tracer kernels; 0 keeps a field."""
-        o = L.LaunchOpts(int(trace_block), int(trace_waves_per_cu), int(pool_paths))
+        o = L.LaunchOpts(int(trace_block), int(trace_waves_per_cu), int(pool_paths), int(shade_list_tiles))
         L.check(self.lib.aur_set_launch(self._ctx, C.byref(o)))
 
     def get_launch(self) -> dict:
         o = L.LaunchOpts()
         L.check(self.lib.aur_get_launch(self._ctx, C.byref(o)))
-        return {"trace_block": o.trace_block, "trace_waves_per_cu": o.trace_waves_per_cu, "pool_paths": o.pool_paths}
+        return {"trace_block": o.trace_block, "trace_waves_per_cu": o.trace_waves_per_cu, "pool_paths": o.pool_paths,
+                "shade_list_tiles": o.shade_list_tiles}
 
     def render_paths(self, pixel, sample, slots=None):
         """Per-path RGBA for explicit (pixel, sample) pairs; optional injected uniforms."""

@@ -174,9 +174,12 @@ def test_json_comments_and_sorted_members(tmp_path):
 def test_launch_options_validation():
     r = host_renderer()
     d = r.get_launch()
-    assert d == {"trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 128 << 20}
+    assert d == {"trace_block": 64, "trace_waves_per_cu": 32, "pool_paths": 128 << 20, "shade_list_tiles": 4096}
     r.set_launch(trace_block=256, pool_paths=1 << 16)
-    assert r.get_launch() == {"trace_block": 256, "trace_waves_per_cu": 32, "pool_paths": 1 << 16}
+    assert r.get_launch() == {"trace_block": 256, "trace_waves_per_cu": 32, "pool_paths": 1 << 16,
+                              "shade_list_tiles": 4096}
+    r.set_launch(shade_list_tiles=1)
+    assert r.get_launch()["shade_list_tiles"] == 1
     _expect_error(lambda: r.set_launch(trace_block=100), -1)
     _expect_error(lambda: r.set_launch(trace_waves_per_cu=65), -1)
     _expect_error(lambda: r.set_launch(pool_paths=10), -1)
