@@ -277,8 +277,8 @@ __device__ inline void adj_count(unsigned long long *stats, unsigned paths, unsi
 
 __device__ inline void adj_path(const SceneView &sv, const AdjParams &ap, int i, int *pixel, PathRngT<false> *rng) {
     const int wh = sv.width * sv.height;
-    const int pass = i / wh;
-    *pixel = tiled_pixel(i - pass * wh, sv.width, sv.height);
+    const int pass = udiv(i, sv.div_wh);
+    *pixel = tiled_pixel(sv, i - pass * wh);
     *rng = make_path_rng<false>(sv, nullptr, ap.seed, *pixel, (uint32_t)(ap.first_pass + pass));
 }
 

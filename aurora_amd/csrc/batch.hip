@@ -163,7 +163,7 @@ __global__ void k_regen(BatchView bv, WfView wv, int q) {
         const int wh = jd.sv.width * jd.sv.height;
         const int blk = (int)(local / wh);
         const int l = (int)(local - (long long)blk * wh);
-        const int pixel = tiled_pixel(l, jd.sv.width, jd.sv.height);
+        const int pixel = tiled_pixel(jd.sv, l);
         int pass, sub;
         job_block(jd, blk, &pass, &sub);
         const PathRng rng = make_path_rng(jd.sv, nullptr, jd.seed, pixel, (uint32_t)(jd.first_pass + pass), sub, jd.ls);
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256, TEX ? 4 : 5) k_bshade(BatchView bv, WfVie
                         int pass, sub;
                         job_block(jd, m.z, &pass, &sub);
                         return make_path_rng<TEX>(jd.sv, nullptr, jd.seed,
-                                                  tiled_pixel(m.y, jd.sv.width, jd.sv.height),
+                                                  tiled_pixel(jd.sv, m.y),
                                                   (uint32_t)(jd.first_pass + pass), sub, jd.ls);
                     }, j, o);
                     break;
@@ -392,7 +392,7 @@ __global__ void k_bresolve(BatchView bv, const ResolveSeg *__restrict__ segs, in
         const JobDev &jd = bv.jobs[sg.job];
         const int wh = jd.sv.width * jd.sv.height;
         const long long l = t - sg.tbase;
-        const int pix = tiled_pixel((int)l, jd.sv.width, jd.sv.height);
+        const int pix = tiled_pixel(jd.sv, (int)l);
         float4 *px = jd.accum + pix;
         float4 s = *px;
         ushort4 d = jd.display[pix];

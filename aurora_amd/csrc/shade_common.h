@@ -8,6 +8,26 @@
 
 namespace aur {
 
+// Division by a divisor fixed for a launch, without the ~25-instruction
+// integer divide: Granlund & Montgomery, "Division by invariant integers using
+// multiplication" (PLDI 1994), unsigned case.  With l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1 (which fits 32 bits for d < 2^31),
+// n / d = (mulhi(m, n) + n) >> l for every 0 <= n < 2^31 (mulhi(m, n) <= n,
+// so the sum does not wrap).
+struct UDiv {
+    uint32_t m;
+    int l;
+};
+inline UDiv udiv_make(uint32_t d) {
+    int l = 0;
+    while ((1ull << l) < d) ++l;
+    return {(uint32_t)((((1ull << l) - d) << 32) / d + 1), l};
+}
+__device__ __host__ inline int udiv(int n, const UDiv &d) {
+    const uint32_t t = (uint32_t)(((uint64_t)(uint32_t)n * d.m) >> 32);
+    return (int)((t + (uint32_t)n) >> d.l);
+}
+
 struct SceneView {
     const void *nodes;
     const float4 *tris;
@@ -29,6 +49,7 @@ struct SceneView {
     float env[3];
     float fov;
     int width, height, maxdepth, mindepth;
+    UDiv div_w, div_8w, div_wh;  // width, 8 * width, width * height
 };
 
 SceneView scene_view(Ctx *c);
@@ -44,6 +65,21 @@ __device__ __host__ inline int tiled_pixel(int local, int w, int h) {
     const int y0 = band * 8;
     const int hb = h - y0 < 8 ? h - y0 : 8;
     const int l = local - y0 * w;
+    const int cx = l / (hb * 8);
+    const int cw = w - cx * 8 < 8 ? w - cx * 8 : 8;
+    const int idx = l - cx * hb * 8;
+    return (y0 + idx / cw) * w + cx * 8 + idx % cw;
+}
+
+// The same order on the device, with the scene's divisors and whole 8x8
+// tiles (all but the image's right and bottom edges) decoded by shifts.
+__device__ inline int tiled_pixel(const SceneView &sv, int local) {
+    const int w = sv.width;
+    const int y0 = udiv(local, sv.div_8w) * 8;
+    const int hb = sv.height - y0 < 8 ? sv.height - y0 : 8;
+    const int l = local - y0 * w;
+    if (hb == 8 && l < (w & ~7) * 8)
+        return (y0 + ((l >> 3) & 7)) * w + ((l >> 6) << 3) + (l & 7);
     const int cx = l / (hb * 8);
     const int cw = w - cx * 8 < 8 ? w - cx * 8 : 8;
     const int idx = l - cx * hb * 8;

@@ -42,13 +42,13 @@ __device__ inline void path_decode(const SceneView &sv, const WfView &wv, int pa
         } else {
             if (p == 0) { id -= wh; p = 1; }
             const int per = wv.src.ls * wh;
-            const int k = id / per;
+            const int k = udiv(id, wv.div_per);
             const int r = id - k * per;
             *sample = p + k;
-            *sub = r / wh;
+            *sub = udiv(r, sv.div_wh);
             *local = r - *sub * wh;
         }
-        *pixel = tiled_pixel(*local, sv.width, sv.height);
+        *pixel = tiled_pixel(sv, *local);
     } else {
         *pixel = wv.src.pixel[path];
         *sample = wv.src.sample[path];
@@ -555,6 +555,9 @@ SceneView scene_view(Ctx *c) {
     sv.fov = c->scene.opt.fov;
     sv.width = c->scene.opt.width;
     sv.height = c->scene.opt.height;
+    sv.div_w = udiv_make((uint32_t)std::max(sv.width, 1));
+    sv.div_8w = udiv_make(8u * (uint32_t)std::max(sv.width, 1));
+    sv.div_wh = udiv_make((uint32_t)std::max(sv.width * sv.height, 1));
     sv.maxdepth = c->scene.opt.maxdepth;
     sv.mindepth = c->scene.opt.mindepth;
     return sv;
@@ -632,6 +635,7 @@ static WfView wf_view_of(Ctx *c, Wavefront &w, const PathSource &src) {
     v.ovf_per_thread = (int)w.stack_ovf_per_thread;
     v.sovf = v.ovf + trace_threads(c) * std::max<size_t>(w.stack_ovf_per_thread, 1);
     v.src = src;
+    v.div_per = udiv_make((uint32_t)std::max(src.ls, 1) * (uint32_t)std::max(c->scene.opt.width * c->scene.opt.height, 1));
     v.seed = c->seed;
     v.nslots = 2 + AUR_SLOTS_PER_VERTEX * c->scene.opt.maxdepth;
     return v;

@@ -93,6 +93,7 @@ struct WfView {
     int *sovf;                     // and shadow rays, which may run beside them
     int ovf_per_thread;
     PathSource src;
+    UDiv div_per;       // src.ls * width * height (passes after the first)
     uint32_t seed;
     int nslots;
 };
@@ -153,7 +154,7 @@ __device__ inline void wave_count_job(unsigned long long *cnt, int stride, int j
 // camera.cpp:34-62: pixel (x, y) jittered by two uniforms, pinhole at the
 // origin looking down -z; the ray spans [0.1, 1e7].
 __device__ inline f3 camera_dir(const SceneView &sv, int pixel, float r0, float r1) {
-    int x = pixel % sv.width, y = pixel / sv.width;
+    const int y = udiv(pixel, sv.div_w), x = pixel - y * sv.width;
     float fovx = sv.fov;
     float fovy = ((float)sv.height / (float)sv.width) * fovx;
     float u = x + r0, v = y + r1;
