@@ -237,36 +237,17 @@ __device__ inline void finish_dups(const WfView &wv, unsigned ext_dup, unsigned 
 // entries of a bounce are the same vertex of their paths) -- finishes as
 // shading would finish it (integrator.cpp:287-297, 262-271, then the clamp
 // and output of integrator.cpp:276-283).
-template <bool TEX, bool CAM>
+template <bool TEX>
 __device__ inline void finish_unshaded(const SceneView &sv, const WfView &wv, int q, int i, int prim,
                                        unsigned &ext_dup, unsigned &hit_dup) {
-    int4 meta;
-    float4 La, Lb;
-    if constexpr (CAM) {  // the camera vertex of a hash-sampled render: nothing stored yet
-        meta = make_int4(RT_CAMERA, i, 0, 0);
-        La = Lb = make_float4(0.f, 0.f, 0.f, 0.f);
-    } else {
-        meta = ld_s(&wv.meta[q][i]);
-        La = ld_s(&wv.La[q][i]);
-        Lb = ld_s(&wv.Lb[q][i]);
-    }
+    const int4 meta = ld_s(&wv.meta[q][i]);
+    float4 La = ld_s(&wv.La[q][i]);
+    const float4 Lb = ld_s(&wv.Lb[q][i]);
     int flags = meta.x;
     if (prim < 0) {
-        const auto make_rng = [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); };
-        const auto get_rb = [&] {
-            if constexpr (CAM) {
-                float4 a, b;
-                camera_ray_of(sv, make_rng(meta), &a, &b);
-                return b;
-            } else {
-                return ld_s(&wv.ray[q][2 * i + 1]);
-            }
-        };
-        const auto get_T = [&] {
-            if constexpr (CAM) return make_float4(1.f, 1.f, 1.f, 1.f);
-            else return ld_s(&wv.T[q][i]);
-        };
-        miss_radiance<TEX>(sv, meta, make_rng, get_rb, get_T, La, flags);
+        miss_radiance<TEX>(
+            sv, meta, [&](int4 m) { return path_rng<TEX>(sv, wv, m.y); },
+            [&] { return ld_s(&wv.ray[q][2 * i + 1]); }, [&] { return ld_s(&wv.T[q][i]); }, La, flags);
         flags |= F_MISS;
     }
     finish_path(sv, wv, La, Lb, make_int4(flags, meta.y, meta.z, meta.w), ext_dup, hit_dup);
@@ -300,10 +281,10 @@ __global__ void __launch_bounds__(256, TEX ? 4 : AUR_SHADE_WAVES) k_shade(SceneV
     const unsigned long long lt = (1ull << lane) - 1ull;
     // Listing pays off when each workgroup has tiles enough for the hit
     // records' read to hide behind other workgroups' shading (list_tiles:
-    // four per workgroup by default); a shorter queue is shaded in place, a
-    // workgroup's worth of entries per round (shade_compute finishes its
-    // misses itself).
-    const bool listed = (n + SHADE_TILE - 1) / SHADE_TILE >= list_tiles;
+    // four per workgroup by default); a shorter queue, and the camera
+    // bounce, are shaded in place, a workgroup's worth of entries per round
+    // (shade_compute finishes its misses itself).
+    const bool listed = !CAM && (n + SHADE_TILE - 1) / SHADE_TILE >= list_tiles;
     const int tile_size = listed ? SHADE_TILE : 256;
     const int n_tiles = (n + tile_size - 1) / tile_size;
     unsigned ext_dup = 0, hit_dup = 0;
@@ -326,7 +307,7 @@ __global__ void __launch_bounds__(256, TEX ? 4 : AUR_SHADE_WAVES) k_shade(SceneV
                 const int i = base + k * 256 + (int)threadIdx.x;
                 const bool valid = i < n, fin = valid && (prim[k] < 0 || last);
                 m[k] = __ballot(valid && !fin);
-                if (fin) finish_unshaded<TEX, CAM>(sv, wv, q, i, prim[k], ext_dup, hit_dup);
+                if (fin) finish_unshaded<TEX>(sv, wv, q, i, prim[k], ext_dup, hit_dup);
             }
             if (lane == 0)
 #pragma unroll
@@ -688,8 +669,11 @@ static void run_batch(Ctx *c, const Lane &L, const SceneView &sv, WfView wv, int
         TIMED_ON(c, KT_EXTEND, L.main, AUR_TRAV_DISPATCH(c->bvh.width, c->launch.trace_block, AUR_EXT));
         if (overlap && it > 0) HIP_CHECK(hipStreamWaitEvent(L.main, L.done, 0));
         const int last = it == sv.maxdepth;
+        // the camera bounce is shaded in place: a frame's camera rays mostly
+        // hit (at 1080p, listing them cost 1 ms of shading per frame)
+        const int list_tiles = it == 0 ? 0x7fffffff : c->launch.shade_list_tiles;
 #define AUR_SHADE(t_, c_) \
-    hipLaunchKernelGGL((k_shade<t_, c_>), dim3(sb), dim3(256), 0, L.main, sv, wv, q, last, c->launch.shade_list_tiles)
+    hipLaunchKernelGGL((k_shade<t_, c_>), dim3(sb), dim3(256), 0, L.main, sv, wv, q, last, list_tiles)
         if (tex && cam) TIMED_ON(c, KT_SHADE, L.main, AUR_SHADE(true, true));
         else if (tex) TIMED_ON(c, KT_SHADE, L.main, AUR_SHADE(true, false));
         else if (cam) TIMED_ON(c, KT_SHADE, L.main, AUR_SHADE(false, true));

@@ -367,10 +367,11 @@ typedef struct {
     int trace_waves_per_cu; /* persistent traversal grid, in waves per CU (1..64) */
     int pool_paths;         /* path slots in flight (64 .. 357913941): the aur_batch_submit pool, and
                                the largest chunk of whole passes aur_render traces at once */
-    int shade_list_tiles;   /* aur_render's shading: a bounce whose queue holds at least this many
-                               1024-entry tiles first finishes its misses and shades full waves of
-                               the hits; shorter queues are shaded in place (default: four tiles per
-                               shade workgroup, 16 per CU; the image is the same either way) */
+    int shade_list_tiles;   /* aur_render's shading: a bounce after the camera bounce whose queue
+                               holds at least this many 1024-entry tiles first finishes its misses
+                               and shades full waves of the hits; shorter queues, and the camera
+                               bounce, are shaded in place (default: four tiles per shade
+                               workgroup, 16 per CU; the image is the same either way) */
 } aur_launch_opts;
 /* Fields <= 0 keep their current value. */
 int aur_set_launch(aur_ctx *ctx, const aur_launch_opts *opts);

@@ -1,15 +1,15 @@
 """GPU: the two ways a bounce is shaded give the same render bit for bit.
 
-A long bounce queue is shaded from per-workgroup hit lists: each workgroup
-reads a tile of hit records, finishes the entries with no vertex to shade
-(misses, and every entry after the last vertex) and shades the listed hits
-in full waves; a shorter queue is shaded in place.  The listing threshold
+A long bounce queue after the camera bounce is shaded from per-workgroup hit
+lists: each workgroup reads a tile of hit records, finishes the entries with
+no vertex to shade (misses, and every entry after the last vertex) and
+shades the listed hits in full waves; a shorter queue, and the camera
+bounce, are shaded in place.  The listing threshold
 (aur_launch_opts.shade_list_tiles, in 1024-entry tiles) is forced to one
-tile (every bounce listed) and to never, and framebuffer, FP16 display and
+tile (every bounce after the first listed) and to never, and framebuffer, FP16 display and
 ray counts must agree on: an open scene under a constant environment
-(displaced spheres, hash sampling: the camera bounce rebuilds its rays),
-the textured stage (environment map, mirror rays that miss, the shader
-code), the box room with the Halton sampler (stored camera rays) and with
+(displaced spheres), the textured stage (environment map, mirror rays that
+miss, the shader code), the box room with the Halton sampler and with
 light samples (later samples of a pass share its camera ray), and the
 adjoint's first replay (per-path records written when paths finish; its
 gradients agree to the float-atomic summation order of the second replay).
