@@ -267,11 +267,8 @@ __device__ inline void finish_unshaded(const SceneView &sv, const WfView &wv, in
 // and environment-map code needs more registers and runs at 4.
 #define SHADE_PER_THREAD 4
 #define SHADE_TILE (SHADE_PER_THREAD * 256)
-#ifndef AUR_SHADE_WAVES
-#define AUR_SHADE_WAVES 5
-#endif
 template <bool TEX, bool CAM = false>
-__global__ void __launch_bounds__(256, TEX ? 4 : AUR_SHADE_WAVES) k_shade(SceneView sv, WfView wv, int q, int last,
+__global__ void __launch_bounds__(256, TEX ? 4 : 5) k_shade(SceneView sv, WfView wv, int q, int last,
                                                                           int list_tiles) {
     __shared__ EmitScratch es;
     __shared__ int list[SHADE_TILE + 256];
