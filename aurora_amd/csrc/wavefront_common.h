@@ -464,7 +464,11 @@ __device__ inline void shade_emit(const WfView &wv, int q, bool valid, const Sha
     __syncthreads();
     const int slot = es.base[wave][0] + __popcll(mc & lt);
     const int fs = es.base[wave][1] + __popcll(mf & lt);
-    const int ss = es.base[wave][2] + __popcll(m0 & lt) + __popcll(m1 & lt);
+    // a wave's light-sample rays first, then its BSDF-sampled rays: each
+    // group's records are written contiguously (they add into different
+    // accumulators, La and Lb, so their order changes nothing)
+    const int ss0 = es.base[wave][2] + __popcll(m0 & lt);
+    const int ss1 = es.base[wave][2] + __popcll(m0) + __popcll(m1 & lt);
     if (pc) {
         st_s(&wv.ray[nq][2 * slot], make_float4(o.P.x, o.P.y, o.P.z, AUR_RAY_BIAS));
         st_s(&wv.ray[nq][2 * slot + 1], make_float4(o.ndir.x, o.ndir.y, o.ndir.z, o.ntmax));
@@ -480,16 +484,15 @@ __device__ inline void shade_emit(const WfView &wv, int q, bool valid, const Sha
     }
     if (pd) direct(o);
     const int target = o.cont ? slot : (fs | SH_TO_FINISH);
-    int s = ss;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const bool pk = k ? p1 : p0;
         if (pk) {
+            const int s = k ? ss1 : ss0;
             const ShadowOut &h = o.sh[k];
             st_s(&wv.sray[3 * s], make_float4(h.o.x, h.o.y, h.o.z, __int_as_float(o.job)));
             st_s(&wv.sray[3 * s + 1], make_float4(h.d.x, h.d.y, h.d.z, h.tmax));
             st_s(&wv.sray[3 * s + 2], make_float4(h.c.x, h.c.y, h.c.z, __int_as_float(target | (k ? SH_TO_LB : 0))));
-            ++s;
         }
     }
 }
